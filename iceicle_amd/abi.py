@@ -1,0 +1,146 @@
+"""ctypes mirror of the plain-C descriptors in include/iceicle_amd.h.
+
+Both the product library (libiceicle_amd.so) and the CPU oracle take these
+exact structs, so a test builds one descriptor and hands it to both.
+"""
+import ctypes as C
+
+ICE_MAX_BCFLAG = 4
+
+# status codes
+ICE_OK = 0
+ICE_ERR_ARG = -1
+ICE_ERR_UNSUPPORTED = -2
+ICE_ERR_HIP = -3
+ICE_ERR_ALLOC = -4
+ICE_ERR_STATE = -5
+ICE_ERR_NUMERIC = -6
+
+# boundary conditions (reference enum BOUNDARY_CONDITIONS, geometry/face.hpp:23-103)
+BC = dict(PERIODIC=0, PARALLEL_COM=1, NEUMANN=2, DIRICHLET=3, EXTRAPOLATION=4, RIEMANN=5,
+          NO_SLIP_ISOTHERMAL=6, SLIP_WALL=7, WALL_GENERAL=8, INLET=9, OUTLET=10,
+          SPACETIME_PAST=11, SPACETIME_FUTURE=12, INTERIOR=13)
+
+PERIODIC_REFERENCE = 0
+PERIODIC_COUPLED = 1
+
+FLUX_BURGERS = 0
+FLUX_NAVIER_STOKES = 1
+FLUX_EULER = 2
+
+VISC_CONSTANT = 0
+VISC_DIMENSIONLESS_SUTHERLAND = 1
+
+FIELD = dict(NONE=0, CONSTANT=1, SINE=2, SINUSOIDS=3, NS_MMS_STATE=4, NS_MMS_SOURCE=5,
+             ISENTROPIC_VORTEX=6, EXP_DECAY_SINE=7)
+
+FACE_REC_FIXED = 6
+
+
+class Field(C.Structure):
+    _fields_ = [("kind", C.c_int), ("p", C.c_double * 32)]
+
+    @staticmethod
+    def make(kind, params=()):
+        f = Field()
+        f.kind = FIELD[kind] if isinstance(kind, str) else int(kind)
+        for i, v in enumerate(params):
+            f.p[i] = float(v)
+        return f
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [
+        ("ndim", C.c_int),
+        ("nelem", C.c_int * 3),
+        ("xmin", C.c_double * 3),
+        ("xmax", C.c_double * 3),
+        ("bctype", C.c_int * 6),
+        ("bcflag", C.c_int * 6),
+        ("coord", C.POINTER(C.c_double)),
+        ("periodic_mode", C.c_int),
+    ]
+
+
+class SpaceDesc(C.Structure):
+    _fields_ = [("order", C.c_int), ("geometry_order", C.c_int)]
+
+
+class MeshCounts(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("ndim", "nelem", "nnode", "nvert_el", "nfacenode", "nface",
+                                       "nface_interior")]
+
+
+class SpaceInfo(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("ndim", "order", "nbasis", "nqp", "nfqp", "nelem", "nface",
+                                       "nface_interior", "nface_boundary")] + [("ndof", C.c_int64)]
+
+
+class DiscParams(C.Structure):
+    _fields_ = [
+        ("flux", C.c_int),
+        ("mu", C.c_double),
+        ("a", C.c_double * 3),
+        ("b", C.c_double * 3),
+        ("gamma", C.c_double),
+        ("Rgas", C.c_double),
+        ("Pr", C.c_double),
+        ("ref", C.c_double * 8),
+        ("viscosity_kind", C.c_int),
+        ("visc_mu", C.c_double),
+        ("isothermal_T", C.c_double * ICE_MAX_BCFLAG),
+        ("freestream", C.c_double * 6),
+        ("ns_face_energy_textbook", C.c_int),
+        ("sigma_ic", C.c_double),
+        ("interior_penalty", C.c_int),
+        ("dirichlet", Field * ICE_MAX_BCFLAG),
+        ("neumann", Field * ICE_MAX_BCFLAG),
+        ("source", Field),
+    ]
+
+
+def default_params(flux):
+    """Reference defaults (burgers.hpp:14-40, navier_stokes.hpp:129-213, :321-340)."""
+    p = DiscParams()
+    p.flux = flux
+    p.mu = 0.001
+    p.gamma = 1.4
+    p.Rgas = 287.052874
+    p.Pr = 0.72
+    for i in range(8):
+        p.ref[i] = 1.0
+    p.viscosity_kind = VISC_CONSTANT
+    p.visc_mu = 1.0
+    for i in range(ICE_MAX_BCFLAG):
+        p.isothermal_T[i] = 1.0
+    p.freestream[0] = 1.0
+    p.freestream[2] = 1.0
+    p.freestream[5] = 1.0
+    return p
+
+
+def mesh_desc(ndim, nelem, xmin, xmax, bctype, bcflag=None, coord=None, periodic_mode=PERIODIC_REFERENCE):
+    """Build a MeshDesc.  `bctype` entries may be names from BC or ints.
+    `coord` (optional) is a C-contiguous float64 numpy array [nnode][ndim];
+    the caller keeps it alive while the descriptor is used."""
+    m = MeshDesc()
+    m.ndim = ndim
+    for d in range(ndim):
+        m.nelem[d] = int(nelem[d])
+        m.xmin[d] = float(xmin[d])
+        m.xmax[d] = float(xmax[d])
+    for k in range(2 * ndim):
+        b = bctype[k]
+        m.bctype[k] = BC[b] if isinstance(b, str) else int(b)
+        m.bcflag[k] = 0 if bcflag is None else int(bcflag[k])
+    if coord is not None:
+        m.coord = coord.ctypes.data_as(C.POINTER(C.c_double))
+    m.periodic_mode = periodic_mode
+    return m
+
+
+def space_desc(order):
+    s = SpaceDesc()
+    s.order = order
+    s.geometry_order = 1
+    return s

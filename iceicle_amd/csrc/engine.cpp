@@ -1,0 +1,474 @@
+// Host engine behind the C-ABI: FE space construction (reference tables,
+// trace maps, device upload) and discretization handles.  Replaces the
+// reference's FESpace constructor (fespace.hpp:200-416) and the
+// ConservationLawDDG object (conservation_law.hpp:92-230) for the device path.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/iceicle_amd.h"
+#include "engine.h"
+#include "kernels.h"
+#include "mesh.hpp"
+#include "physics_dev.h"
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define ICE_HIP(call)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) throw ice::Error{ICE_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)}; \
+    } while (0)
+
+template <class T>
+T* upload(const std::vector<T>& h) {
+    T* d = nullptr;
+    if (h.empty()) return nullptr;
+    ICE_HIP(hipMalloc(&d, h.size() * sizeof(T)));
+    ICE_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+}
+
+// Gauss-Legendre points in the reference's order (+x1, -x1, +x2, -x2, ..., 0)
+// with Newton from cos(pi (i - 1/4) / (n + 1/2)) (quadrules_1d.hpp:36-76).
+void gauss_legendre(int n, double* x, double* w) {
+    const int m = n / 2;
+    double wc = 2.0;
+    for (int i = 1; i <= m; ++i) {
+        double z = std::cos(M_PI * (i - 0.25) / (n + 0.5));
+        double dp = 0.0;
+        for (int it = 0; it < 50; ++it) {
+            double p0 = 1.0, p1 = z;
+            for (int k = 1; k < n; ++k) {
+                const double p2 = ((2 * k + 1) * z * p1 - k * p0) / (k + 1);
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = n * (z * p1 - p0) / (z * z - 1.0);
+            const double dz = p1 / dp;
+            z -= dz;
+            if (std::abs(dz) < 1e-16) break;
+        }
+        const double wi = 2.0 / ((1 - z * z) * dp * dp);
+        if (n % 2) wc -= 2 * wi;
+        x[2 * (i - 1)] = z;
+        w[2 * (i - 1)] = wi;
+        x[2 * (i - 1) + 1] = -z;
+        w[2 * (i - 1) + 1] = wi;
+    }
+    if (n % 2) { x[n - 1] = 0.0; w[n - 1] = wc; }
+}
+
+// Equispaced Lagrange basis on [-1,1] (lagrange_1d.hpp:19-212): value,
+// first and second derivative of all p+1 polynomials at xi.
+void lagrange(int p, double xi, double* N, double* dN, double* d2N) {
+    double xn[8], wb[8];
+    xn[0] = -1.0;
+    for (int j = 1; j <= p; ++j) xn[j] = xn[j - 1] + 2.0 / p;
+    for (int j = 0; j <= p; ++j) {
+        wb[j] = 1.0;
+        for (int k = 0; k <= p; ++k)
+            if (k != j) wb[j] *= (xn[j] - xn[k]);
+        wb[j] = 1.0 / wb[j];
+    }
+    if (p == 1) {
+        N[0] = 0.5 * (1 - xi);
+        N[1] = 1.0 - N[0];
+        dN[0] = -0.5;
+        dN[1] = 0.5;
+        d2N[0] = d2N[1] = 0.0;
+        return;
+    }
+    for (int j = 0; j <= p; ++j) {
+        // products over subsets excluding j: value, first and second derivative
+        double v = 1.0, d1 = 0.0, d2 = 0.0;
+        for (int k = 0; k <= p; ++k) {
+            if (k == j) continue;
+            const double f = xi - xn[k];
+            d2 = d2 * f + 2.0 * d1;
+            d1 = d1 * f + v;
+            v = v * f;
+        }
+        N[j] = v * wb[j];
+        dN[j] = d1 * wb[j];
+        d2N[j] = d2 * wb[j];
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- handles
+struct ice_space {
+    ice::HostMesh mesh;
+    int ndim = 0, p = 0, n = 0, nb = 0, nq = 0, nfq = 0;
+    bool coupled = false;
+    std::vector<double> tab;          // host copy of the S tables
+    std::vector<int4> fint, fbdy;
+    std::vector<int> efaces;
+    std::vector<signed char> fpt, fvo;
+    std::vector<double> X;
+    ice::DevSpace dev{};
+    void* dmem[8] = {};
+    ~ice_space() {
+        for (void* p : dmem)
+            if (p) (void)hipFree(p);
+    }
+};
+
+struct ice_disc {
+    ice_space* space = nullptr;
+    ice_disc_params prm{};
+    ice::PhysParams ph{};
+    int neq = 0;
+    bool ddgic = false;
+    int has_source = 0;
+    ice_field* d_fields = nullptr;
+    double* d_Q = nullptr;
+    double* d_Pic = nullptr;
+    ~ice_disc() {
+        if (d_fields) (void)hipFree(d_fields);
+        if (d_Q) (void)hipFree(d_Q);
+        if (d_Pic) (void)hipFree(d_Pic);
+    }
+};
+
+namespace ice {
+
+const DevSpace& space_dev(const ice_space* s) { return s->dev; }
+const ice_space* disc_space(const ice_disc* d) { return d->space; }
+int disc_neq(const ice_disc* d) { return d->neq; }
+void set_error(const std::string& m) { g_err = m; }
+
+}  // namespace ice
+
+extern "C" {
+
+int ice_version(void) { return ICE_API_VERSION; }
+const char* ice_last_error(void) { return g_err.c_str(); }
+
+int ice_disc_params_default(int flux, ice_disc_params* out) {
+    if (!out) return fail(ICE_ERR_ARG, "null output");
+    std::memset(out, 0, sizeof(*out));
+    out->flux = flux;
+    out->mu = 0.001;
+    out->gamma = 1.4;
+    out->Rgas = 287.052874;
+    out->Pr = 0.72;
+    for (double& r : out->ref) r = 1.0;
+    out->viscosity_kind = ICE_VISC_CONSTANT;
+    out->visc_mu = 1.0;
+    for (double& t : out->isothermal_T) t = 1.0;
+    out->freestream[0] = 1.0;
+    out->freestream[2] = 1.0;
+    out->freestream[5] = 1.0;
+    return ICE_OK;
+}
+
+int ice_mesh_export(const ice_mesh_desc* d, ice_mesh_counts* counts, double* coord, int* conn, int* faces) {
+    if (!d || !counts) return fail(ICE_ERR_ARG, "null argument");
+    try {
+        const ice::HostMesh m = ice::build_mesh(*d, false);
+        const int D = m.ndim;
+        const int nfv = (D > 1) ? (1 << (D - 1)) : 1;
+        counts->ndim = D;
+        counts->nelem = m.nelem;
+        counts->nnode = m.nnode;
+        counts->nvert_el = 1 << D;
+        counts->nfacenode = nfv;
+        counts->nface = (int)m.faces.size();
+        counts->nface_interior = m.n_interior;
+        if (coord) std::memcpy(coord, m.coord.data(), m.coord.size() * sizeof(double));
+        if (conn) std::memcpy(conn, m.conn.data(), m.conn.size() * sizeof(int));
+        if (faces) {
+            int k = 0;
+            for (const auto& f : m.faces) {
+                faces[k++] = f.elemL;
+                faces[k++] = f.elemR;
+                faces[k++] = f.fnl * 512;
+                faces[k++] = f.fnr * 512 + f.orient;
+                faces[k++] = f.bctype;
+                faces[k++] = f.bcflag;
+                for (int j = 0; j < nfv; ++j) faces[k++] = f.nodes[j];
+            }
+        }
+        return ICE_OK;
+    } catch (const ice::Error& e) {
+        return fail(e.code, e.msg);
+    }
+}
+
+int ice_space_create(const ice_mesh_desc* md, const ice_space_desc* sd, ice_space** out) {
+    if (!md || !sd || !out) return fail(ICE_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (sd->order < 1 || sd->order > ICE_MAX_ORDER) return fail(ICE_ERR_UNSUPPORTED, "order must be 1..ICE_MAX_ORDER");
+    if (sd->geometry_order != 1) return fail(ICE_ERR_UNSUPPORTED, "only geometry order 1 is supported");
+    if (md->periodic_mode != ICE_PERIODIC_REFERENCE && md->periodic_mode != ICE_PERIODIC_COUPLED)
+        return fail(ICE_ERR_ARG, "bad periodic_mode");
+    try {
+        auto s = std::make_unique<ice_space>();
+        s->coupled = md->periodic_mode == ICE_PERIODIC_COUPLED;
+        s->mesh = ice::build_mesh(*md, s->coupled);
+        const ice::HostMesh& m = s->mesh;
+        const int D = m.ndim, p = sd->order, n = p + 1;
+        s->ndim = D;
+        s->p = p;
+        s->n = n;
+        s->nb = 1;
+        for (int d = 0; d < D; ++d) s->nb *= n;
+        s->nq = s->nb;
+        s->nfq = s->nb / n;
+        const int NV = 1 << D;
+        // element vertex coordinates, element-local tensor order
+        s->X.resize((size_t)m.nelem * NV * D);
+        for (int e = 0; e < m.nelem; ++e)
+            for (int v = 0; v < NV; ++v)
+                for (int d = 0; d < D; ++d)
+                    s->X[((size_t)e * NV + v) * D + d] = m.coord[(size_t)m.conn[(size_t)e * NV + v] * D + d];
+        // S tables: GL points, then -1, +1
+        double gx[8], gw[8];
+        gauss_legendre(n, gx, gw);
+        const int ns = n + 2;
+        s->tab.assign(ns + 3 * ns * n + n, 0.0);
+        double* xs = s->tab.data();
+        double* L0 = xs + ns;
+        double* L1 = L0 + ns * n;
+        double* L2 = L1 + ns * n;
+        double* wg = L2 + ns * n;
+        for (int k = 0; k < n; ++k) { xs[k] = gx[k]; wg[k] = gw[k]; }
+        xs[n] = -1.0;
+        xs[n + 1] = 1.0;
+        for (int k = 0; k < ns; ++k) lagrange(p, xs[k], L0 + k * n, L1 + k * n, L2 + k * n);
+        // trace point and face vertex tables
+        const int no = ice::n_orientations(D);
+        s->fpt.resize((size_t)2 * D * no * s->nfq * D);
+        for (int j = 0; j < 2 * D; ++j)
+            for (int o = 0; o < no; ++o)
+                for (int q = 0; q < s->nfq; ++q) {
+                    int pt[3];
+                    ice::trace_point_indices(D, n, j, o, q, pt);
+                    for (int d = 0; d < D; ++d) s->fpt[(((size_t)j * no + o) * s->nfq + q) * D + d] = (signed char)pt[d];
+                }
+        s->fvo.assign(2 * D * 4, 0);
+        for (int j = 0; j < 2 * D; ++j) {
+            int v[4];
+            ice::face_vertex_order(D, j, v);
+            for (int k = 0; k < ((D > 1) ? (1 << (D - 1)) : 1); ++k) s->fvo[j * 4 + k] = (signed char)v[k];
+        }
+        // device face lists
+        s->efaces.assign((size_t)m.nelem * 2 * D, -1);
+        for (int f = 0; f < m.n_interior; ++f) {
+            const auto& fc = m.faces[f];
+            s->fint.push_back(int4{fc.elemL, fc.elemR, ice::pack_face_info(fc.fnl, fc.fnr, fc.orient, ICE_BC_INTERIOR, 0), 0});
+        }
+        if (s->coupled) {
+            for (int f = m.n_interior; f < (int)m.faces.size(); ++f) {
+                const auto& fa = m.faces[f];
+                if (fa.bctype != ICE_BC_PERIODIC || fa.fnl >= D) continue;
+                const auto& fb = m.faces[fa.bcflag];
+                const int idim = fa.fnl;
+                const int nfv = (D > 1) ? (1 << (D - 1)) : 1;
+                int va[4];
+                for (int k = 0; k < nfv; ++k) {
+                    // partner of A's node on B's side: same ordinates, last index along idim
+                    int r = fa.nodes[k], ijk[3] = {0, 0, 0}, stride = 1, idx = 0;
+                    for (int d = 0; d < D; ++d) { ijk[d] = r % m.nnd[d]; r /= m.nnd[d]; }
+                    ijk[idim] = m.nnd[idim] - 1;
+                    for (int d = 0; d < D; ++d) { idx += ijk[d] * stride; stride *= m.nnd[d]; }
+                    va[k] = idx;
+                }
+                const int orient = ice::face_orientation(D, fb.nodes, va);
+                if (orient < 0) throw ice::Error{ICE_ERR_ARG, "periodic faces do not match"};
+                s->fint.push_back(int4{fb.elemL, fa.elemL, ice::pack_face_info(fb.fnl, fa.fnl, orient, ICE_BC_INTERIOR, idim + 1), 0});
+            }
+        }
+        for (int f = m.n_interior; f < (int)m.faces.size(); ++f) {
+            const auto& fc = m.faces[f];
+            if (s->coupled && fc.bctype == ICE_BC_PERIODIC) continue;
+            s->fbdy.push_back(int4{fc.elemL, ice::pack_face_info(fc.fnl, 0, 0, fc.bctype, 0), fc.bcflag, 0});
+        }
+        const int nfi = (int)s->fint.size(), nfb = (int)s->fbdy.size();
+        for (int f = 0; f < nfi; ++f) {
+            const int4 r = s->fint[f];
+            const int fnl = r.z & 7, fnr = (r.z >> 3) & 7, orient = (r.z >> 6) & 7;
+            s->efaces[(size_t)r.x * 2 * D + fnl] = (f << 4);
+            s->efaces[(size_t)r.y * 2 * D + fnr] = (f << 4) | (1 << 3) | orient;
+        }
+        for (int b = 0; b < nfb; ++b) {
+            const int4 r = s->fbdy[b];
+            s->efaces[(size_t)r.x * 2 * D + (r.y & 7)] = ((nfi + b) << 4);
+        }
+        // upload
+        s->dmem[0] = upload(s->X);
+        s->dmem[1] = upload(s->fint);
+        s->dmem[2] = upload(s->fbdy);
+        s->dmem[3] = upload(s->efaces);
+        s->dmem[4] = upload(s->tab);
+        s->dmem[5] = upload(s->fpt);
+        s->dmem[6] = upload(s->fvo);
+        ice::DevSpace& dv = s->dev;
+        dv.ndim = D;
+        dv.p = p;
+        dv.n = n;
+        dv.nelem = m.nelem;
+        dv.nfi = nfi;
+        dv.nfb = nfb;
+        dv.norient = no;
+        for (int d = 0; d < 3; ++d) dv.period[d] = m.period[d];
+        dv.X = (const double*)s->dmem[0];
+        dv.fint = (const int4*)s->dmem[1];
+        dv.fbdy = (const int4*)s->dmem[2];
+        dv.efaces = (const int*)s->dmem[3];
+        dv.tab = (const double*)s->dmem[4];
+        dv.fpt = (const signed char*)s->dmem[5];
+        dv.fvo = (const signed char*)s->dmem[6];
+        *out = s.release();
+        return ICE_OK;
+    } catch (const ice::Error& e) {
+        return fail(e.code, e.msg);
+    } catch (const std::exception& e) {
+        return fail(ICE_ERR_ALLOC, e.what());
+    }
+}
+
+int ice_space_destroy(ice_space* s) {
+    delete s;
+    return ICE_OK;
+}
+
+int ice_space_query(const ice_space* s, ice_space_info* info) {
+    if (!s || !info) return fail(ICE_ERR_ARG, "null argument");
+    info->ndim = s->ndim;
+    info->order = s->p;
+    info->nbasis = s->nb;
+    info->nqp = s->nq;
+    info->nfqp = s->nfq;
+    info->nelem = s->mesh.nelem;
+    info->nface = s->dev.nfi + s->dev.nfb;
+    info->nface_interior = s->dev.nfi;
+    info->nface_boundary = s->dev.nfb;
+    info->ndof = (int64_t)s->mesh.nelem * s->nb;
+    return ICE_OK;
+}
+
+int ice_space_qp_coords(const ice_space* s, double* xq) {
+    if (!s || !xq) return fail(ICE_ERR_ARG, "null argument");
+    const int D = s->ndim, n = s->n, NV = 1 << D;
+    for (int e = 0; e < s->mesh.nelem; ++e)
+        for (int q = 0; q < s->nq; ++q) {
+            int r = q, qd[3];
+            for (int d = D - 1; d >= 0; --d) { qd[d] = r % n; r /= n; }
+            double x[3] = {0, 0, 0};
+            for (int v = 0; v < NV; ++v) {
+                double phi = 1.0;
+                for (int d = 0; d < D; ++d) {
+                    const double n0 = 0.5 * (1 - s->tab[qd[d]]);
+                    phi *= ((v >> (D - 1 - d)) & 1) ? (1.0 - n0) : n0;
+                }
+                for (int d = 0; d < D; ++d) x[d] += phi * s->X[((size_t)e * NV + v) * D + d];
+            }
+            for (int d = 0; d < D; ++d) xq[((size_t)e * s->nq + q) * D + d] = x[d];
+        }
+    return ICE_OK;
+}
+
+int ice_disc_create(ice_space* s, const ice_disc_params* p, ice_disc** out) {
+    if (!s || !p || !out) return fail(ICE_ERR_ARG, "null argument");
+    *out = nullptr;
+    const int D = s->ndim;
+    if (p->flux != ICE_FLUX_BURGERS && p->flux != ICE_FLUX_NAVIER_STOKES && p->flux != ICE_FLUX_EULER)
+        return fail(ICE_ERR_ARG, "unknown flux family");
+    const bool ns = p->flux != ICE_FLUX_BURGERS;
+    // boundary types the reference defines for this flux family
+    for (const auto& f : s->mesh.faces) {
+        if (f.bctype == ICE_BC_INTERIOR) continue;
+        const int bt = f.bctype;
+        if ((bt == ICE_BC_DIRICHLET || bt == ICE_BC_NEUMANN || bt == ICE_BC_NO_SLIP_ISOTHERMAL) &&
+            (f.bcflag < 0 || f.bcflag >= ICE_MAX_BCFLAG))
+            return fail(ICE_ERR_ARG, "bcflag out of range for a Dirichlet/Neumann/isothermal boundary");
+        if (ns) {
+            const bool ok = bt == ICE_BC_DIRICHLET || bt == ICE_BC_NEUMANN || bt == ICE_BC_EXTRAPOLATION ||
+                            bt == ICE_BC_SPACETIME_FUTURE || bt == ICE_BC_SLIP_WALL || bt == ICE_BC_NO_SLIP_ISOTHERMAL ||
+                            bt == ICE_BC_WALL_GENERAL || bt == ICE_BC_RIEMANN || (bt == ICE_BC_PERIODIC && s->coupled);
+            if (!ok)
+                return fail(ICE_ERR_UNSUPPORTED,
+                            "boundary condition has no Navier-Stokes implementation in the reference (apply_bc logs an "
+                            "anomaly and returns an uninitialised state)");
+        }
+    }
+    try {
+        auto d = std::make_unique<ice_disc>();
+        d->space = s;
+        d->prm = *p;
+        d->neq = ns ? D + 2 : 1;
+        ice::PhysParams& ph = d->ph;
+        std::memset(&ph, 0, sizeof(ph));
+        ph.mu = p->mu;
+        for (int k = 0; k < 3; ++k) { ph.a[k] = p->a[k]; ph.b[k] = p->b[k]; }
+        ph.gamma = p->gamma;
+        ph.Pr = p->Pr;
+        const double* r = p->ref;  // t, l, rho, u, e, p, T, mu
+        ph.Re = r[2] * r[3] * r[1] / r[7];
+        ph.Eu = r[5] / (r[2] * r[3] * r[3]);
+        ph.e_coeff = r[3] * r[3] / r[4];
+        const double cp = p->gamma / (p->gamma - 1) * p->Rgas;
+        ph.T_coeff = cp * r[6] * r[2] / r[5];
+        ph.visc_kind = p->viscosity_kind;
+        ph.visc_mu = p->visc_mu;
+        ph.mu_ratio = 1.716e-5 / r[7];
+        ph.T0r = 273.1 / r[6];
+        ph.Tsr = 110.5 / r[6];
+        for (int k = 0; k < ICE_MAX_BCFLAG; ++k) ph.isoT[k] = p->isothermal_T[k];
+        ph.fs_rho = p->freestream[0];
+        ph.fs_u = p->freestream[1];
+        for (int k = 0; k < 3; ++k) ph.fs_dir[k] = p->freestream[2 + k];
+        ph.fs_T = p->freestream[5];
+        ph.textbook_energy = p->ns_face_energy_textbook;
+        ph.sigma_ic = p->sigma_ic;
+        const int order = s->p;
+        ph.beta0 = (double)(order + 1) * (order + 1);
+        ph.beta1 = p->interior_penalty ? 0.0 : 1.0 / std::max((double)(2 * order * (order + 1)), 1.0);
+        ph.beta0_bdy = ph.beta0;
+        d->ddgic = (!ns) && p->sigma_ic != 0.0;
+        d->has_source = p->source.kind != ICE_FIELD_NONE;
+        std::vector<ice_field> fields(2 * ICE_MAX_BCFLAG + 1);
+        for (int k = 0; k < ICE_MAX_BCFLAG; ++k) {
+            fields[k] = p->dirichlet[k];
+            fields[ICE_MAX_BCFLAG + k] = p->neumann[k];
+        }
+        fields[2 * ICE_MAX_BCFLAG] = p->source;
+        d->d_fields = upload(fields);
+        const size_t rows = (size_t)(s->dev.nfi + s->dev.nfb) * s->nfq * d->neq;
+        ICE_HIP(hipMalloc(&d->d_Q, std::max<size_t>(rows, 1) * sizeof(double)));
+        if (d->ddgic) ICE_HIP(hipMalloc(&d->d_Pic, std::max<size_t>(rows * D, 1) * sizeof(double)));
+        *out = d.release();
+        return ICE_OK;
+    } catch (const ice::Error& e) {
+        return fail(e.code, e.msg);
+    }
+}
+
+int ice_disc_destroy(ice_disc* d) {
+    delete d;
+    return ICE_OK;
+}
+
+int ice_residual(ice_disc* d, const double* u, double* r, void* stream) {
+    if (!d || !u || !r) return fail(ICE_ERR_ARG, "null argument");
+    const ice_space* s = d->space;
+    const int rc = ice::residual_fp64(s->ndim, s->p, d->prm.flux, s->dev, d->ph, d->d_fields, d->has_source, d->ddgic, u,
+                                      d->d_Q, d->d_Pic, r, (hipStream_t)stream);
+    if (rc != 0) return fail(ICE_ERR_HIP, std::string("residual launch failed: ") + hipGetErrorString((hipError_t)rc));
+    return ICE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,37 @@
+// Device-side views shared by the kernels and the host engine.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/iceicle_amd.h"
+
+namespace ice {
+
+struct PhysParams;
+
+// Everything a residual launch needs about the FE space, as raw device
+// pointers (all owned by the ice_space handle).
+struct DevSpace {
+    int ndim, p, n;
+    int nelem;
+    int nfi;                 // interior faces (+ coupled periodic pairs), Q rows [0, nfi)
+    int nfb;                 // boundary faces, Q rows [nfi, nfi + nfb)
+    int norient;             // orientation codes per face number in fpt
+    double period[3];
+    const double* X;         // [nelem][2^ndim][ndim] element vertex coordinates
+    const int4* fint;        // {elemL, elemR, info, 0}
+    const int4* fbdy;        // {elemL, info, bcflag, 0}
+    const int* efaces;       // [nelem][2*ndim]: (Q row << 4) | (side << 3) | orientation, -1 none
+    const double* tab;       // xs[n+2] | L0[(n+2)n] | L1 | L2 | wgl[n]
+    const signed char* fpt;  // [2*ndim][norient][nfq][ndim] S-indices of trace points
+    const signed char* fvo;  // [2*ndim][4] face vertex order (element-local vertex ids)
+};
+
+// face info word: fnl | fnr << 3 | orient << 6 | bctype << 9 | pdim << 13
+inline int pack_face_info(int fnl, int fnr, int orient, int bctype, int pdim) {
+    return fnl | (fnr << 3) | (orient << 6) | (bctype << 9) | (pdim << 13);
+}
+
+int residual_fp64(int ndim, int p, int flux, const DevSpace& S, const PhysParams& ph, const ice_field* fields,
+                  int has_source, bool ddgic, const double* U, double* Q, double* Pic, double* R, hipStream_t st);
+
+}  // namespace ice

@@ -1,0 +1,835 @@
+// DG residual kernels for CDNA4 (gfx950), FP64.
+//
+// One residual = two launches on the caller's stream:
+//  1. face kernels (interior + boundary): one thread per face quadrature
+//     point; evaluate both traces (value, physical gradient, normal Hessian
+//     term), the numerical fluxes and the DDG single-valued gradient, and
+//     write the weighted normal flux Q[face][fq][v] (and the DDGIC lifting
+//     vector when sigma_ic != 0).  This is ConservationLawDDG::trace_integral /
+//     boundaryIntegral (conservation_law.hpp:434-1176) up to the final
+//     multiplication by the trace basis.
+//  2. element kernel: a workgroup owns a tile of elements whose coefficients
+//     are staged in LDS with one coalesced copy; phase 1 has one thread per
+//     (element, volume quadrature point) and evaluates the physical flux
+//     contracted into the reference frame; phase 2 has one thread per
+//     (element, basis function) and sums the volume term
+//     (domain_integral, :244-314), the source term and the lifted face fluxes
+//     of the element's 2*ndim faces, then writes R with one coalesced store.
+// The face-flux buffer replaces the reference's scatter of per-face residual
+// blocks (form_residual.hpp:137-242): every element gathers its own faces, so
+// the result is deterministic and needs no atomics.
+#include <hip/hip_runtime.h>
+
+#include "fields.h"
+#include "kernels.h"
+#include "physics_dev.h"
+
+namespace ice {
+
+template <int B, int E>
+struct Pow { static constexpr int value = B * Pow<B, E - 1>::value; };
+template <int B>
+struct Pow<B, 0> { static constexpr int value = 1; };
+
+// ---- geometry helpers (geometry order 1 hypercubes) ----------------------
+
+// Q1 map at xi: point, Jacobian J[i][a] = dx_i/dxi_a and Hessian H[i][a][b]
+// (hypercube::transform/jacobian/hessian, HypercubeTransformations.hpp:52-161).
+template <int D>
+__device__ __forceinline__ void q1_map(const double (&X)[1 << D][D], const double* xi, double* x, double (&J)[D][D],
+                                       double (&H)[D][D][D]) {
+    double N[D][2], dN[D][2];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        N[d][0] = 0.5 * (1 - xi[d]);
+        N[d][1] = 1.0 - N[d][0];
+        dN[d][0] = -0.5;
+        dN[d][1] = 0.5;
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        if (x) x[i] = 0;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            J[i][a] = 0;
+#pragma unroll
+            for (int b = 0; b < D; ++b) H[i][a][b] = 0;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < (1 << D); ++k) {
+        int bit[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) bit[d] = (k >> (D - 1 - d)) & 1;
+        double phi = 1.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) phi *= N[d][bit[d]];
+        double g[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            double t = 1.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) t *= (d == a) ? dN[d][bit[d]] : N[d][bit[d]];
+            g[a] = t;
+        }
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            if (x) x[i] += phi * X[k][i];
+#pragma unroll
+            for (int a = 0; a < D; ++a) J[i][a] += g[a] * X[k][i];
+        }
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int b = 0; b < D; ++b) {
+                if (a == b) continue;
+                double t = 1.0;
+#pragma unroll
+                for (int d = 0; d < D; ++d) t *= (d == a || d == b) ? dN[d][bit[d]] : N[d][bit[d]];
+#pragma unroll
+                for (int i = 0; i < D; ++i) H[i][a][b] += t * X[k][i];
+            }
+    }
+}
+
+template <int D>
+__device__ __forceinline__ double det_adj(const double (&J)[D][D], double (&A)[D][D]) {
+    if constexpr (D == 1) {
+        A[0][0] = 1.0;
+        return J[0][0];
+    } else if constexpr (D == 2) {
+        A[0][0] = J[1][1]; A[0][1] = -J[0][1];
+        A[1][0] = -J[1][0]; A[1][1] = J[0][0];
+        return J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    } else {
+        A[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+        A[0][1] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+        A[0][2] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+        A[1][0] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+        A[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+        A[1][2] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+        A[2][0] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        A[2][1] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+        A[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        return J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+               J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void load_vertices(const double* __restrict__ Xall, int e, double (&X)[1 << D][D]) {
+    const double* p = Xall + (size_t)e * (1 << D) * D;
+#pragma unroll
+    for (int k = 0; k < (1 << D); ++k)
+#pragma unroll
+        for (int d = 0; d < D; ++d) X[k][d] = p[k * D + d];
+}
+
+// Face map from the left element's face vertices at face quadrature point
+// fq: physical point x, unit normal and sqrt(det(J^T J)).
+// (HypercubeFace::transform/Jacobian + calc_ortho/normalize/rootRiemannMetric;
+//  1D: PointFace conventions, HypercubeTransformations.hpp:1358-1456.)
+template <int D, int N>
+__device__ __forceinline__ double face_geometry(const DevSpace& S, const double (&X)[1 << D][D], int fnl, int fq,
+                                                double* x, double* nrm) {
+    if constexpr (D == 1) {
+        const int v = S.fvo[fnl * 4 + 0];
+        x[0] = X[v][0];
+        nrm[0] = (fnl == 0) ? -1.0 : 1.0;
+        return 1.0;
+    } else {
+        constexpr int TD = D - 1;
+        constexpr int NFV = 1 << TD;
+        const double* xs = S.tab;
+        double s[TD];
+        {
+            int r = fq;
+#pragma unroll
+            for (int k = TD - 1; k >= 0; --k) { s[k] = xs[r % N]; r /= N; }
+        }
+        double Jf[D][TD];
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            x[i] = 0;
+#pragma unroll
+            for (int t = 0; t < TD; ++t) Jf[i][t] = 0;
+        }
+#pragma unroll
+        for (int k = 0; k < NFV; ++k) {
+            const int v = S.fvo[fnl * 4 + k];
+            double psi = 1.0, dpsi[TD];
+#pragma unroll
+            for (int t = 0; t < TD; ++t) {
+                const int bit = (k >> (TD - 1 - t)) & 1;
+                const double n0 = 0.5 * (1 - s[t]);
+                psi *= bit ? (1.0 - n0) : n0;
+            }
+#pragma unroll
+            for (int t = 0; t < TD; ++t) {
+                double g = 1.0;
+#pragma unroll
+                for (int u = 0; u < TD; ++u) {
+                    const int bit = (k >> (TD - 1 - u)) & 1;
+                    const double n0 = 0.5 * (1 - s[u]);
+                    g *= (u == t) ? (bit ? 0.5 : -0.5) : (bit ? (1.0 - n0) : n0);
+                }
+                dpsi[t] = g;
+            }
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                x[i] += psi * X[v][i];
+#pragma unroll
+                for (int t = 0; t < TD; ++t) Jf[i][t] += dpsi[t] * X[v][i];
+            }
+        }
+        double sg;
+        if constexpr (D == 2) {
+            nrm[0] = Jf[1][0];
+            nrm[1] = -Jf[0][0];
+            sg = sqrt(Jf[0][0] * Jf[0][0] + Jf[1][0] * Jf[1][0]);
+        } else {
+            nrm[0] = Jf[1][0] * Jf[2][1] - Jf[2][0] * Jf[1][1];
+            nrm[1] = Jf[2][0] * Jf[0][1] - Jf[0][0] * Jf[2][1];
+            nrm[2] = Jf[0][0] * Jf[1][1] - Jf[1][0] * Jf[0][1];
+            double g00 = 0, g01 = 0, g11 = 0;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                g00 += Jf[i][0] * Jf[i][0];
+                g01 += Jf[i][0] * Jf[i][1];
+                g11 += Jf[i][1] * Jf[i][1];
+            }
+            sg = sqrt(g00 * g11 - g01 * g01);
+        }
+        double nn = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) nn += nrm[d] * nrm[d];
+        nn = sqrt(nn);
+#pragma unroll
+        for (int d = 0; d < D; ++d) nrm[d] /= nn;
+        return sg;
+    }
+}
+
+// Trace of one element at a tensor point of S (indices pt[D]): value u[M],
+// physical gradient gx[M][D] and, if HESS, the normal Hessian term
+// hn[M][d] = sum_j (d2u/dx_j dx_d) n_j of the physical Hessian
+// (PhysDomainEval + eval_phys_hess_basis, finite_element.hpp:328-398,497-564).
+template <int D, int N, int M, bool HESS>
+__device__ __forceinline__ void trace_eval(const DevSpace& S, const double* __restrict__ Ue, const int* pt,
+                                           const double (&X)[1 << D][D], const double* nrm, double* u, double* gx,
+                                           double* hn) {
+    constexpr int NH = D * (D + 1) / 2;
+    const double* xs = S.tab;
+    const double* L0 = S.tab + (N + 2);
+    const double* L1 = L0 + (N + 2) * N;
+    const double* L2 = L1 + (N + 2) * N;
+    double a0[D][N], a1[D][N], a2[D][N];
+    double xi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        xi[d] = xs[pt[d]];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            a0[d][i] = L0[pt[d] * N + i];
+            a1[d][i] = L1[pt[d] * N + i];
+            a2[d][i] = HESS ? L2[pt[d] * N + i] : 0.0;
+        }
+    }
+    double gr[M][D], hr[M][NH];
+#pragma unroll
+    for (int v = 0; v < M; ++v) {
+        u[v] = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) gr[v][d] = 0;
+#pragma unroll
+        for (int h = 0; h < NH; ++h) hr[v][h] = 0;
+    }
+    constexpr int NB = Pow<N, D>::value;
+#pragma unroll 1
+    for (int i = 0; i < NB; ++i) {
+        int id[D];
+        {
+            int r = i;
+#pragma unroll
+            for (int d = D - 1; d >= 0; --d) { id[d] = r % N; r /= N; }
+        }
+        double c0 = 1.0, cg[D], ch[NH];
+#pragma unroll
+        for (int d = 0; d < D; ++d) c0 *= a0[d][id[d]];
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            double t = 1.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) t *= (d == k) ? a1[d][id[d]] : a0[d][id[d]];
+            cg[k] = t;
+        }
+        if constexpr (HESS) {
+            int h = 0;
+#pragma unroll
+            for (int a = 0; a < D; ++a)
+#pragma unroll
+                for (int b = a; b < D; ++b) {
+                    double t = 1.0;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        if (a == b) t *= (d == a) ? a2[d][id[d]] : a0[d][id[d]];
+                        else t *= (d == a || d == b) ? a1[d][id[d]] : a0[d][id[d]];
+                    }
+                    ch[h++] = t;
+                }
+        }
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            const double c = Ue[i * M + v];
+            u[v] += c * c0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) gr[v][k] += c * cg[k];
+            if constexpr (HESS) {
+#pragma unroll
+                for (int h = 0; h < NH; ++h) hr[v][h] += c * ch[h];
+            }
+        }
+    }
+    double J[D][D], Hm[D][D][D], A[D][D];
+    q1_map<D>(X, xi, nullptr, J, Hm);
+    double det = det_adj<D>(J, A);
+    det = (det == 0.0) ? 1.0 : det;
+    const double idet = 1.0 / det;
+#pragma unroll
+    for (int v = 0; v < M; ++v)
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            double s = 0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) s += gr[v][k] * A[k][j];
+            gx[v * D + j] = s * idet;
+        }
+    if constexpr (HESS) {
+        const double idet2 = idet * idet;
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            double rhs[D][D];
+            {
+                int h = 0;
+#pragma unroll
+                for (int a = 0; a < D; ++a)
+#pragma unroll
+                    for (int b = a; b < D; ++b) {
+                        rhs[a][b] = hr[v][h];
+                        rhs[b][a] = hr[v][h];
+                        ++h;
+                    }
+            }
+#pragma unroll
+            for (int a = 0; a < D; ++a)
+#pragma unroll
+                for (int b = 0; b < D; ++b)
+#pragma unroll
+                    for (int k = 0; k < D; ++k) rhs[a][b] -= Hm[k][a][b] * gx[v * D + k];
+            // Hx[a][b] = sum_{k,l} A[k][b] A[l][a] rhs[l][k] / det^2 ; hn[d] = sum_j Hx[j][d] n_j
+            double T[D][D];  // T[l][b] = sum_k rhs[l][k] A[k][b]
+#pragma unroll
+            for (int l = 0; l < D; ++l)
+#pragma unroll
+                for (int b = 0; b < D; ++b) {
+                    double s = 0;
+#pragma unroll
+                    for (int k = 0; k < D; ++k) s += rhs[l][k] * A[k][b];
+                    T[l][b] = s;
+                }
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                double s = 0;
+#pragma unroll
+                for (int j = 0; j < D; ++j) {
+                    double hx = 0;
+#pragma unroll
+                    for (int l = 0; l < D; ++l) hx += A[l][j] * T[l][d];
+                    s += hx * nrm[j];
+                }
+                hn[v * D + d] = s * idet2;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ int info_fnl(int info) { return info & 7; }
+__device__ __forceinline__ int info_fnr(int info) { return (info >> 3) & 7; }
+__device__ __forceinline__ int info_orient(int info) { return (info >> 6) & 7; }
+__device__ __forceinline__ int info_bctype(int info) { return (info >> 9) & 15; }
+__device__ __forceinline__ int info_pdim(int info) { return (info >> 13) & 3; }
+
+template <int D>
+__device__ __forceinline__ void centroid(const double (&X)[1 << D][D], double* c) {
+    const double w = 1.0 / (1 << D);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        double s = 0;
+#pragma unroll
+        for (int k = 0; k < (1 << D); ++k) s += w * X[k][d];
+        c[d] = s;
+    }
+}
+
+template <int N, int D>
+__device__ __forceinline__ double face_weight(const DevSpace& S, int fq) {
+    const double* wgl = S.tab + (N + 2) + 3 * (N + 2) * N;
+    if constexpr (D == 1) {
+        return 1.0;
+    } else {
+        double w = 1.0;
+        int idx[D - 1];
+        int r = fq;
+#pragma unroll
+        for (int k = D - 2; k >= 0; --k) { idx[k] = r % N; r /= N; }
+#pragma unroll
+        for (int k = 0; k < D - 1; ++k) w = (k == 0) ? wgl[idx[k]] : w * wgl[idx[k]];
+        return w;
+    }
+}
+
+// ---- interior faces ------------------------------------------------------
+template <int D, int P, class PH>
+__global__ __launch_bounds__(256) void face_interior_kernel(DevSpace S, PhysParams ph, const double* __restrict__ U,
+                                                            double* __restrict__ Q, double* __restrict__ Pic) {
+    constexpr int N = P + 1;
+    constexpr int NB = Pow<N, D>::value;
+    constexpr int NFQ = Pow<N, D - 1>::value;
+    constexpr int M = PH::M;
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = tid / NFQ;
+    const int q = tid - f * NFQ;
+    if (f >= S.nfi) return;
+    const int4 rec = S.fint[f];
+    const int eL = rec.x, eR = rec.y, info = rec.z;
+    const int fnl = info_fnl(info), fnr = info_fnr(info), orient = info_orient(info), pdim = info_pdim(info);
+    double XL[1 << D][D], XR[1 << D][D];
+    load_vertices<D>(S.X, eL, XL);
+    load_vertices<D>(S.X, eR, XR);
+    double x[D], nrm[D];
+    const double sqrtg = face_geometry<D, N>(S, XL, fnl, q, x, nrm);
+    const double w = face_weight<N, D>(S, q);
+    double cL[D], cR[D];
+    centroid<D>(XL, cL);
+    centroid<D>(XR, cR);
+    if (pdim) cR[pdim - 1] += S.period[pdim - 1];
+    int ptL[D], ptR[D];
+    const signed char* fpL = S.fpt + ((fnl * S.norient + 0) * NFQ + q) * D;
+    const signed char* fpR = S.fpt + ((fnr * S.norient + orient) * NFQ + q) * D;
+#pragma unroll
+    for (int d = 0; d < D; ++d) { ptL[d] = fpL[d]; ptR[d] = fpR[d]; }
+    double uL[M], uR[M], gL[M * D], gR[M * D], hL[M * D], hR[M * D];
+    trace_eval<D, N, M, true>(S, U + (size_t)eL * NB * M, ptL, XL, nrm, uL, gL, hL);
+    trace_eval<D, N, M, true>(S, U + (size_t)eR * NB * M, ptR, XR, nrm, uR, gR, hR);
+    double fadv[M], fvis[M], gddg[M * D], uavg[M];
+    PH::conv(ph, uL, uR, nrm, fadv);
+    double h = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) h += nrm[d] * ((x[d] - cL[d]) + (cR[d] - x[d]));
+    h = copysign(fmax(fabs(h), 2.220446049250313e-16), h);
+#pragma unroll
+    for (int v = 0; v < M; ++v) {
+        const double jump = uR[v] - uL[v];
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            gddg[v * D + d] = ph.beta0 * jump / h * nrm[d] + 0.5 * (gL[v * D + d] + gR[v * D + d]) +
+                              ph.beta1 * h * (hR[v * D + d] - hL[v * D + d]);
+        uavg[v] = 0.5 * (uL[v] + uR[v]);
+    }
+    PH::diff(ph, uavg, gddg, nrm, fvis);
+    const double ws = w * sqrtg;
+    double* Qf = Q + ((size_t)f * NFQ + q) * M;
+#pragma unroll
+    for (int v = 0; v < M; ++v) Qf[v] = fvis[v] * ws - fadv[v] * ws;
+    if constexpr (PH::HAS_HOMOGENEITY) {
+        if (Pic) {
+            double G[M * D * M * D];
+            PH::homogeneity(ph, uavg, G);
+            double* Pf = Pic + ((size_t)f * NFQ + q) * M * D;
+#pragma unroll
+            for (int v = 0; v < M; ++v)
+#pragma unroll
+                for (int s = 0; s < D; ++s) {
+                    double acc = 0;
+#pragma unroll
+                    for (int k = 0; k < D; ++k)
+#pragma unroll
+                        for (int r = 0; r < M; ++r) acc += G[((v * D + k) * M + r) * D + s] * nrm[k] * (uR[r] - uL[r]);
+                    Pf[v * D + s] = 0.5 * ph.sigma_ic * acc * ws;
+                }
+        }
+    }
+}
+
+// ---- boundary faces ------------------------------------------------------
+template <int D, int P, class PH>
+__global__ __launch_bounds__(256) void face_boundary_kernel(DevSpace S, PhysParams ph, const ice_field* __restrict__ fields,
+                                                            const double* __restrict__ U, double* __restrict__ Q,
+                                                            double* __restrict__ Pic) {
+    constexpr int N = P + 1;
+    constexpr int NB = Pow<N, D>::value;
+    constexpr int NFQ = Pow<N, D - 1>::value;
+    constexpr int M = PH::M;
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = tid / NFQ;
+    const int q = tid - b * NFQ;
+    if (b >= S.nfb) return;
+    const int4 rec = S.fbdy[b];
+    const int eL = rec.x, info = rec.y, bcflag = rec.z;
+    const int fnl = info_fnl(info), bctype = info_bctype(info);
+    const size_t slot = (size_t)S.nfi + b;
+    double* Qf = Q + (slot * NFQ + q) * M;
+    double* Pf = nullptr;
+    if constexpr (PH::HAS_HOMOGENEITY) {
+        if (Pic) {
+            Pf = Pic + (slot * NFQ + q) * M * D;
+#pragma unroll
+            for (int k = 0; k < M * D; ++k) Pf[k] = 0.0;
+        }
+    }
+    const bool general = !(bctype == ICE_BC_DIRICHLET || bctype == ICE_BC_NEUMANN || bctype == ICE_BC_EXTRAPOLATION ||
+                           bctype == ICE_BC_SPACETIME_FUTURE);
+    if (general && !PH::IMPLEMENTS_BCS) {  // boundaryIntegral default branch without apply_bc
+#pragma unroll
+        for (int v = 0; v < M; ++v) Qf[v] = 0.0;
+        return;
+    }
+    double XL[1 << D][D];
+    load_vertices<D>(S.X, eL, XL);
+    double x[D], nrm[D];
+    const double sqrtg = face_geometry<D, N>(S, XL, fnl, q, x, nrm);
+    const double ws = face_weight<N, D>(S, q) * sqrtg;
+    if (bctype == ICE_BC_NEUMANN) {
+        double g[M], fv[M];
+        ice_field_eval(&fields[ICE_MAX_BCFLAG + (bcflag & (ICE_MAX_BCFLAG - 1))], D, M, x, g);
+        PH::neumann(ph, g, fv);
+#pragma unroll
+        for (int v = 0; v < M; ++v) Qf[v] = fv[v] * ws;
+        return;
+    }
+    int pt[D];
+    const signed char* fp = S.fpt + ((fnl * S.norient + 0) * NFQ + q) * D;
+#pragma unroll
+    for (int d = 0; d < D; ++d) pt[d] = fp[d];
+    double uL[M], gL[M * D];
+    trace_eval<D, N, M, false>(S, U + (size_t)eL * NB * M, pt, XL, nrm, uL, gL, nullptr);
+    double fadv[M], fvis[M], gddg[M * D], uavg[M], uB[M];
+    if (bctype == ICE_BC_EXTRAPOLATION || bctype == ICE_BC_SPACETIME_FUTURE) {
+        PH::conv(ph, uL, uL, nrm, fadv);
+        PH::diff(ph, uL, gL, nrm, fvis);
+    } else {
+        if (bctype == ICE_BC_DIRICHLET) ice_field_eval(&fields[bcflag & (ICE_MAX_BCFLAG - 1)], D, M, x, uB);
+        else PH::apply_bc(ph, uL, gL, nrm, bctype, bcflag, uB);
+        double cL[D];
+        centroid<D>(XL, cL);
+        double h = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) h += fabs(nrm[d] * (x[d] - cL[d]));
+        h = copysign(fmax(fabs(h), 2.220446049250313e-16), h);
+        PH::conv(ph, uL, uB, nrm, fadv);
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            const double jump = uB[v] - uL[v];
+#pragma unroll
+            for (int d = 0; d < D; ++d) gddg[v * D + d] = ph.beta0_bdy * jump / h * nrm[d] + gL[v * D + d];
+            uavg[v] = 0.5 * (uL[v] + uB[v]);
+        }
+        PH::diff(ph, uavg, gddg, nrm, fvis);
+        if constexpr (PH::HAS_HOMOGENEITY) {
+            if (Pf && bctype == ICE_BC_DIRICHLET) {
+                double G[M * D * M * D];
+                PH::homogeneity(ph, uavg, G);
+#pragma unroll
+                for (int v = 0; v < M; ++v)
+#pragma unroll
+                    for (int s = 0; s < D; ++s) {
+                        double acc = 0;
+#pragma unroll
+                        for (int k = 0; k < D; ++k)
+#pragma unroll
+                            for (int r = 0; r < M; ++r) acc += G[((v * D + k) * M + r) * D + s] * nrm[k] * (uB[r] - uL[r]);
+                        Pf[v * D + s] = ph.sigma_ic * acc * ws;
+                    }
+            }
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < M; ++v) Qf[v] = fvis[v] * ws - fadv[v] * ws;
+}
+
+// ---- elements: volume integral + face lifting -----------------------------
+template <int D, int P>
+struct ElemTile {
+    static constexpr int N = P + 1;
+    static constexpr int NB = Pow<N, D>::value;
+    static constexpr int EPB = (256 / NB) > 0 ? (256 / NB) : 1;
+    static constexpr int NT = EPB * NB;
+};
+
+template <int D, int P, class PH>
+__global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph, const ice_field* __restrict__ fields,
+                                                      int has_source, const double* __restrict__ U,
+                                                      const double* __restrict__ Q, const double* __restrict__ Pic,
+                                                      double* __restrict__ R) {
+    using T = ElemTile<D, P>;
+    constexpr int N = T::N, NB = T::NB, EPB = T::EPB;
+    constexpr int NFQ = Pow<N, D - 1>::value;
+    constexpr int M = PH::M;
+    __shared__ double sU[EPB * NB * M];
+    __shared__ double sG[EPB * NB * M * D];
+    __shared__ double sS[EPB * NB * M];
+    const int e0 = blockIdx.x * EPB;
+    const int le = threadIdx.x / NB;
+    const int k = threadIdx.x - le * NB;
+    const int e = e0 + le;
+    const bool active = (threadIdx.x < T::NT) && (e < S.nelem);
+    {
+        const size_t base = (size_t)e0 * NB * M;
+        const size_t total = (size_t)S.nelem * NB * M;
+        for (int idx = threadIdx.x; idx < EPB * NB * M; idx += blockDim.x)
+            sU[idx] = (base + idx < total) ? U[base + idx] : 0.0;
+    }
+    __syncthreads();
+    const double* xs = S.tab;
+    const double* L0 = S.tab + (N + 2);
+    const double* L1 = L0 + (N + 2) * N;
+    const double* wgl = L1 + 2 * (N + 2) * N;
+    double X[1 << D][D];
+    int kd[D];
+    {
+        int r = k;
+#pragma unroll
+        for (int d = D - 1; d >= 0; --d) { kd[d] = r % N; r /= N; }
+    }
+    if (active) {
+        load_vertices<D>(S.X, e, X);
+        // phase 1: (element, quadrature point kd)
+        double xi[D], wq = 1.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            xi[d] = xs[kd[d]];
+            wq = (d == 0) ? wgl[kd[d]] : wq * wgl[kd[d]];
+        }
+        double x[D], J[D][D], Hm[D][D][D], A[D][D];
+        q1_map<D>(X, xi, x, J, Hm);
+        const double det = det_adj<D>(J, A);
+        const double detc = fmax(0.0, det);
+        const double detg = (det == 0.0) ? 1.0 : det;
+        const double* Ue = sU + le * NB * M;
+        double u[M], gr[M][D];
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            u[v] = 0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) gr[v][d] = 0;
+        }
+#pragma unroll 1
+        for (int i = 0; i < NB; ++i) {
+            int id[D];
+            {
+                int r = i;
+#pragma unroll
+                for (int d = D - 1; d >= 0; --d) { id[d] = r % N; r /= N; }
+            }
+            double c0 = 1.0, cg[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) c0 *= L0[kd[d] * N + id[d]];
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                double t = 1.0;
+#pragma unroll
+                for (int d = 0; d < D; ++d) t *= (d == a) ? L1[kd[d] * N + id[d]] : L0[kd[d] * N + id[d]];
+                cg[a] = t;
+            }
+#pragma unroll
+            for (int v = 0; v < M; ++v) {
+                const double c = Ue[i * M + v];
+                u[v] += c * c0;
+#pragma unroll
+                for (int a = 0; a < D; ++a) gr[v][a] += c * cg[a];
+            }
+        }
+        double gx[M * D], F[M * D];
+#pragma unroll
+        for (int v = 0; v < M; ++v)
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                double s = 0;
+#pragma unroll
+                for (int a = 0; a < D; ++a) s += gr[v][a] * A[a][j];
+                gx[v * D + j] = s / detg;
+            }
+        PH::flux(ph, u, gx, F);
+        const double fac = detc * wq / detg;
+        double* Gq = sG + (le * NB + k) * M * D;
+#pragma unroll
+        for (int v = 0; v < M; ++v)
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                double s = 0;
+#pragma unroll
+                for (int j = 0; j < D; ++j) s += F[v * D + j] * A[a][j];
+                Gq[v * D + a] = s * fac;
+            }
+        double* Sq = sS + (le * NB + k) * M;
+        if (has_source) {
+            double src[M];
+            ice_field_eval(&fields[2 * ICE_MAX_BCFLAG], D, M, x, src);
+#pragma unroll
+            for (int v = 0; v < M; ++v) Sq[v] = src[v] * detc * wq;
+        } else {
+#pragma unroll
+            for (int v = 0; v < M; ++v) Sq[v] = 0.0;
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    // phase 2: (element, basis function kd)
+    double r[M];
+#pragma unroll
+    for (int v = 0; v < M; ++v) r[v] = 0;
+#pragma unroll 1
+    for (int q = 0; q < NB; ++q) {
+        int qd[D];
+        {
+            int rr = q;
+#pragma unroll
+            for (int d = D - 1; d >= 0; --d) { qd[d] = rr % N; rr /= N; }
+        }
+        double c0 = 1.0, cg[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) c0 *= L0[qd[d] * N + kd[d]];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            double t = 1.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) t *= (d == a) ? L1[qd[d] * N + kd[d]] : L0[qd[d] * N + kd[d]];
+            cg[a] = t;
+        }
+        const double* Gq = sG + (le * NB + q) * M * D;
+        const double* Sq = sS + (le * NB + q) * M;
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            double s = 0;
+#pragma unroll
+            for (int a = 0; a < D; ++a) s += Gq[v * D + a] * cg[a];
+            r[v] += s - Sq[v] * c0;
+        }
+    }
+    // lift the face fluxes of the 2*D faces of this element
+#pragma unroll 1
+    for (int j = 0; j < 2 * D; ++j) {
+        const int ef = S.efaces[(size_t)e * 2 * D + j];
+        if (ef < 0) continue;
+        const size_t slot = (size_t)(ef >> 4);
+        const int side = (ef >> 3) & 1;
+        const int orient = ef & 7;
+        const double sgn = side ? -1.0 : 1.0;
+        const signed char* fp = S.fpt + ((size_t)(j * S.norient + orient) * NFQ) * D;
+        const double* Qs = Q + slot * NFQ * M;
+#pragma unroll 1
+        for (int fq = 0; fq < NFQ; ++fq) {
+            double phi = 1.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) phi *= L0[fp[fq * D + d] * N + kd[d]];
+#pragma unroll
+            for (int v = 0; v < M; ++v) r[v] += sgn * Qs[fq * M + v] * phi;
+        }
+        if constexpr (PH::HAS_HOMOGENEITY) {
+            if (Pic) {
+                const double* Ps = Pic + slot * NFQ * M * D;
+#pragma unroll 1
+                for (int fq = 0; fq < NFQ; ++fq) {
+                    double xi[D], cg[D];
+#pragma unroll
+                    for (int d = 0; d < D; ++d) xi[d] = xs[fp[fq * D + d]];
+#pragma unroll
+                    for (int a = 0; a < D; ++a) {
+                        double t = 1.0;
+#pragma unroll
+                        for (int d = 0; d < D; ++d) {
+                            const int s = fp[fq * D + d];
+                            t *= (d == a) ? L1[s * N + kd[d]] : L0[s * N + kd[d]];
+                        }
+                        cg[a] = t;
+                    }
+                    double J[D][D], Hm[D][D][D], A[D][D];
+                    q1_map<D>(X, xi, nullptr, J, Hm);
+                    double det = det_adj<D>(J, A);
+                    det = (det == 0.0) ? 1.0 : det;
+                    double gphi[D];
+#pragma unroll
+                    for (int jj = 0; jj < D; ++jj) {
+                        double s = 0;
+#pragma unroll
+                        for (int a = 0; a < D; ++a) s += cg[a] * A[a][jj];
+                        gphi[jj] = s / det;
+                    }
+#pragma unroll
+                    for (int v = 0; v < M; ++v) {
+                        double s = 0;
+#pragma unroll
+                        for (int d = 0; d < D; ++d) s += Ps[(fq * M + v) * D + d] * gphi[d];
+                        r[v] -= s;
+                    }
+                }
+            }
+        }
+    }
+    double* Re = R + ((size_t)e * NB + k) * M;
+#pragma unroll
+    for (int v = 0; v < M; ++v) Re[v] = r[v];
+}
+
+// ---- launcher ------------------------------------------------------------
+template <int D, int P, class PH>
+int launch_residual(const DevSpace& S, const PhysParams& ph, const ice_field* fields, int has_source, bool ddgic,
+                    const double* U, double* Q, double* Pic, double* R, hipStream_t st) {
+    constexpr int N = P + 1;
+    constexpr int NFQ = Pow<N, D - 1>::value;
+    double* pic = (PH::HAS_HOMOGENEITY && ddgic) ? Pic : nullptr;
+    if (S.nfi > 0) {
+        const long nt = (long)S.nfi * NFQ;
+        face_interior_kernel<D, P, PH><<<(unsigned)((nt + 255) / 256), 256, 0, st>>>(S, ph, U, Q, pic);
+    }
+    if (S.nfb > 0) {
+        const long nt = (long)S.nfb * NFQ;
+        face_boundary_kernel<D, P, PH><<<(unsigned)((nt + 255) / 256), 256, 0, st>>>(S, ph, fields, U, Q, pic);
+    }
+    using T = ElemTile<D, P>;
+    const int nblk = (S.nelem + T::EPB - 1) / T::EPB;
+    element_kernel<D, P, PH><<<nblk, 256, 0, st>>>(S, ph, fields, has_source, U, Q, pic, R);
+    return (int)hipGetLastError();
+}
+
+template <class PH, int D>
+int dispatch_order(int p, const DevSpace& S, const PhysParams& ph, const ice_field* fields, int has_source, bool ddgic,
+                   const double* U, double* Q, double* Pic, double* R, hipStream_t st) {
+    switch (p) {
+    case 1: return launch_residual<D, 1, PH>(S, ph, fields, has_source, ddgic, U, Q, Pic, R, st);
+    case 2: return launch_residual<D, 2, PH>(S, ph, fields, has_source, ddgic, U, Q, Pic, R, st);
+    case 3: return launch_residual<D, 3, PH>(S, ph, fields, has_source, ddgic, U, Q, Pic, R, st);
+    case 4: return launch_residual<D, 4, PH>(S, ph, fields, has_source, ddgic, U, Q, Pic, R, st);
+    default: return -1;
+    }
+}
+
+int residual_fp64(int ndim, int p, int flux, const DevSpace& S, const PhysParams& ph, const ice_field* fields,
+                  int has_source, bool ddgic, const double* U, double* Q, double* Pic, double* R, hipStream_t st) {
+    if (flux == ICE_FLUX_BURGERS) {
+        if (ndim == 1) return dispatch_order<Burgers<1>, 1>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st);
+        if (ndim == 2) return dispatch_order<Burgers<2>, 2>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st);
+        if (ndim == 3) return dispatch_order<Burgers<3>, 3>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st);
+    } else if (flux == ICE_FLUX_NAVIER_STOKES) {
+        if (ndim == 1) return dispatch_order<NavierStokes<1, true>, 1>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st);
+        if (ndim == 2) return dispatch_order<NavierStokes<2, true>, 2>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st);
+        if (ndim == 3) return dispatch_order<NavierStokes<3, true>, 3>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st);
+    } else if (flux == ICE_FLUX_EULER) {
+        if (ndim == 1) return dispatch_order<NavierStokes<1, false>, 1>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st);
+        if (ndim == 2) return dispatch_order<NavierStokes<2, false>, 2>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st);
+        if (ndim == 3) return dispatch_order<NavierStokes<3, false>, 3>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st);
+    }
+    return -1;
+}
+
+}  // namespace ice

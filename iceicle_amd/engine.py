@@ -1,0 +1,142 @@
+"""Python front end of libiceicle_amd.so (ctypes over the C-ABI in
+include/iceicle_amd.h).
+
+Mirrors the reference's C++ entry points for the residual path:
+  FESpace(mesh, order)                 ~ FESpace ctor (fespace.hpp:200-416)
+  ConservationLawDDG(space, params)    ~ ConservationLawDDG (conservation_law.hpp:92-230)
+  form_residual(space, disc, u, res)   ~ solvers::form_residual (form_residual.hpp:56-268)
+Device memory and streams come from torch (ROCm); `u` and `res` are CUDA
+tensors in fe_layout_right order.  There is no CPU fallback: without the
+built library or a GPU these raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libiceicle_amd.so")
+_LIB = None
+
+# every symbol declared in include/iceicle_amd.h, with its ctypes signature
+P = C.c_void_p
+_SIGS = {
+    "ice_version": ([], C.c_int),
+    "ice_last_error": ([], C.c_char_p),
+    "ice_disc_params_default": ([C.c_int, C.POINTER(abi.DiscParams)], C.c_int),
+    "ice_mesh_export": ([C.POINTER(abi.MeshDesc), C.POINTER(abi.MeshCounts), C.POINTER(C.c_double),
+                         C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+    "ice_space_create": ([C.POINTER(abi.MeshDesc), C.POINTER(abi.SpaceDesc), C.POINTER(P)], C.c_int),
+    "ice_space_destroy": ([P], C.c_int),
+    "ice_space_query": ([P, C.POINTER(abi.SpaceInfo)], C.c_int),
+    "ice_space_qp_coords": ([P, C.POINTER(C.c_double)], C.c_int),
+    "ice_disc_create": ([P, C.POINTER(abi.DiscParams), C.POINTER(P)], C.c_int),
+    "ice_disc_destroy": ([P], C.c_int),
+    "ice_residual": ([P, P, P, P], C.c_int),
+}
+
+
+class IceError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load the engine library; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise IceError(f"{LIB_PATH} is missing: build it with `python -m iceicle_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _LIB = L
+    return _LIB
+
+
+def check(rc):
+    if rc != abi.ICE_OK:
+        raise IceError(f"iceicle_amd error {rc}: {lib().ice_last_error().decode()}")
+
+
+def mesh_export(mesh):
+    """Host-only mesh build: (coord, conn, faces) in reference order."""
+    L = lib()
+    cnt = abi.MeshCounts()
+    check(L.ice_mesh_export(C.byref(mesh), C.byref(cnt), None, None, None))
+    coord = np.empty((cnt.nnode, cnt.ndim))
+    conn = np.empty((cnt.nelem, cnt.nvert_el), dtype=np.int32)
+    faces = np.empty((cnt.nface, abi.FACE_REC_FIXED + cnt.nfacenode), dtype=np.int32)
+    check(L.ice_mesh_export(C.byref(mesh), C.byref(cnt), coord.ctypes.data_as(C.POINTER(C.c_double)),
+                            conn.ctypes.data_as(C.POINTER(C.c_int)), faces.ctypes.data_as(C.POINTER(C.c_int))))
+    return coord, conn, faces, cnt
+
+
+def _require_cuda(t, name):
+    import torch
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+        raise IceError(f"{name} must be a contiguous float64 CUDA tensor")
+
+
+class FESpace:
+    """Mesh + reference tables + trace maps resident on the current GPU."""
+
+    def __init__(self, mesh, order):
+        L = lib()
+        self._mesh = mesh
+        self._space_desc = abi.space_desc(order)
+        h = P()
+        check(L.ice_space_create(C.byref(mesh), C.byref(self._space_desc), C.byref(h)))
+        self.h = h
+        info = abi.SpaceInfo()
+        check(L.ice_space_query(self.h, C.byref(info)))
+        self.info = info
+        self.ndim, self.order, self.nbasis = info.ndim, info.order, info.nbasis
+        self.nelem, self.nqp, self.nfqp = info.nelem, info.nqp, info.nfqp
+
+    def qp_coords(self):
+        xq = np.empty((self.nelem, self.nqp, self.ndim))
+        check(lib().ice_space_qp_coords(self.h, xq.ctypes.data_as(C.POINTER(C.c_double))))
+        return xq
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ice_space_destroy(self.h)
+            self.h = None
+
+
+class ConservationLawDDG:
+    """Discretization bound to an FESpace (flux family + DDG parameters)."""
+
+    def __init__(self, space, params):
+        L = lib()
+        self.space = space
+        self.params = params
+        h = P()
+        check(L.ice_disc_create(space.h, C.byref(params), C.byref(h)))
+        self.h = h
+        self.neq = 1 if params.flux == abi.FLUX_BURGERS else space.ndim + 2
+        self.size = space.nelem * space.nbasis * self.neq
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ice_disc_destroy(self.h)
+            self.h = None
+
+
+def current_stream():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def form_residual(space, disc, u, res, stream=None):
+    """res <- R(u) on the GPU (FP64)."""
+    _require_cuda(u, "u")
+    _require_cuda(res, "res")
+    if u.numel() != disc.size or res.numel() != disc.size:
+        raise IceError(f"u/res must have {disc.size} entries")
+    st = current_stream() if stream is None else stream
+    check(lib().ice_residual(disc.h, C.c_void_p(u.data_ptr()), C.c_void_p(res.data_ptr()), st))

@@ -1,0 +1,220 @@
+/*
+ * iceicle_amd.h -- C-ABI of the MI355X DG residual/Jacobian engine.
+ *
+ * The reference (ICEicle) has no FFI: its boundary is the duck-typed C++
+ * template `form_residual(FESpace&, disc&, fespan u, fespan res)`
+ * (src/solvers/include/iceicle/form_residual.hpp:56-268) together with the
+ * ConservationLawDDG discretization (src/FiniteElement/include/iceicle/disc/
+ * conservation_law.hpp:92-1289).  Every entry point below replaces one piece
+ * of that C++ interface; each declaration cites the piece it replaces.
+ *
+ * Conventions
+ *  - Plain C types only: pointers, sizes, ints, doubles.  Device pointers are
+ *    caller-owned HIP device allocations; `stream` is a hipStream_t passed as
+ *    void* (NULL = the legacy default stream).
+ *  - Status: 0 (ICE_OK) on success, a negative ICE_ERR_* class otherwise.
+ *    ice_last_error() returns the message of the last failure on the calling
+ *    thread.  Handles are not thread-safe; distinct handles are independent.
+ *  - Solution layout: the reference `fe_layout_right` order, global index
+ *    (e * nbasis + i) * neq + v (layout_right.hpp:200-214, dglayout.hpp:113).
+ *  - Basis/quadrature/mesh ordering follow the reference exactly: tensor
+ *    indices with dimension 0 slowest (tensor_product.hpp:321-380,
+ *    HypercubeGaussLegendre.hpp:17-67), nodes and elements x-fastest and the
+ *    face order of the uniform AbstractMesh constructor (mesh.hpp:553-913).
+ */
+#ifndef ICEICLE_AMD_H
+#define ICEICLE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ICE_API_VERSION 1
+#define ICE_MAX_DIM 3
+#define ICE_MAX_ORDER 4
+#define ICE_MAX_BCFLAG 4
+
+/* ---- status codes ---------------------------------------------------- */
+#define ICE_OK 0
+#define ICE_ERR_ARG (-1)
+#define ICE_ERR_UNSUPPORTED (-2)
+#define ICE_ERR_HIP (-3)
+#define ICE_ERR_ALLOC (-4)
+#define ICE_ERR_STATE (-5)
+#define ICE_ERR_NUMERIC (-6)
+
+/* ---- boundary conditions: same codes as the reference enum
+ *      BOUNDARY_CONDITIONS (geometry/face.hpp:23-103) ------------------- */
+#define ICE_BC_PERIODIC 0
+#define ICE_BC_PARALLEL_COM 1
+#define ICE_BC_NEUMANN 2
+#define ICE_BC_DIRICHLET 3
+#define ICE_BC_EXTRAPOLATION 4
+#define ICE_BC_RIEMANN 5
+#define ICE_BC_NO_SLIP_ISOTHERMAL 6
+#define ICE_BC_SLIP_WALL 7
+#define ICE_BC_WALL_GENERAL 8
+#define ICE_BC_INLET 9
+#define ICE_BC_OUTLET 10
+#define ICE_BC_SPACETIME_PAST 11
+#define ICE_BC_SPACETIME_FUTURE 12
+#define ICE_BC_INTERIOR 13
+
+/* ---- periodic handling ----------------------------------------------
+ * REFERENCE: periodic faces stay boundary faces whose bcflag is the partner
+ *   face index (mesh.hpp:857-866); boundaryIntegral then takes its `default`
+ *   branch (conservation_law.hpp:1043-1051), which adds no flux for Burgers.
+ * COUPLED: each partner pair is replaced by one interior face (textbook
+ *   periodic coupling); h_ddg uses each side's own centroid-to-face distance. */
+#define ICE_PERIODIC_REFERENCE 0
+#define ICE_PERIODIC_COUPLED 1
+
+/* ---- flux families (disc/burgers.hpp, disc/navier_stokes.hpp) ---------- */
+#define ICE_FLUX_BURGERS 0        /* BurgersFlux + BurgersUpwind + BurgersDiffusionFlux */
+#define ICE_FLUX_NAVIER_STOKES 1  /* Flux<full_ns> + VanLeer + DiffusionFlux<full_ns> */
+#define ICE_FLUX_EULER 2          /* Flux<euler> + VanLeer + DiffusionFlux<euler> (zero) */
+
+#define ICE_VISC_CONSTANT 0               /* constant_viscosity (navier_stokes.hpp:231-242) */
+#define ICE_VISC_DIMENSIONLESS_SUTHERLAND 1 /* DimensionlessSutherlands (:271-301) */
+
+/* ---- analytic fields: Dirichlet/Neumann values, sources, initial states.
+ * The reference takes std::function callbacks (conservation_law.hpp:152-170);
+ * across a C-ABI they are named analytic fields with parameters. */
+#define ICE_FIELD_NONE 0
+#define ICE_FIELD_CONSTANT 1      /* out[v] = p[v] */
+#define ICE_FIELD_SINE 2          /* out[0] = p[0] * sin(p[1] * x + p[2]) (1D heat pin) */
+#define ICE_FIELD_SINUSOIDS 3     /* out[0] = sum_k A_k sin(2 pi (m_k x + n_k y + l_k z));
+                                     p[0] = K, then (A, m, n, l) per k, K <= 7 */
+#define ICE_FIELD_NS_MMS_STATE 4  /* manufactured 2D NS state (conservative vars) */
+#define ICE_FIELD_NS_MMS_SOURCE 5 /* its source term, sign convention of user_source */
+#define ICE_FIELD_ISENTROPIC_VORTEX 6 /* isentropic vortex state, p = (x0, y0, strength, rho_inf, u_inf, v_inf) */
+#define ICE_FIELD_EXP_DECAY_SINE 7 /* out[0] = exp(-p[0] * p[1]) * sin(x): exact 1D heat solution at t = p[1] */
+
+typedef struct ice_field {
+    int kind;
+    double p[32];
+} ice_field;
+
+/* ---- mesh: uniform hypercube box (AbstractMesh uniform constructor,
+ *      mesh.hpp:553-913, geometry order 1), optional moved node coordinates. */
+typedef struct ice_mesh_desc {
+    int ndim;                 /* 1, 2 or 3 */
+    int nelem[3];             /* elements per direction */
+    double xmin[3];
+    double xmax[3];
+    int bctype[6];            /* per domain face, hypercube numbering: face k is
+                                 normal to axis k % ndim, negative side if k < ndim */
+    int bcflag[6];
+    const double* coord;      /* optional host array [n_nodes][ndim] of node
+                                 coordinates in the generator's node order
+                                 (x fastest); NULL keeps the uniform nodes */
+    int periodic_mode;        /* ICE_PERIODIC_* */
+} ice_mesh_desc;
+
+typedef struct ice_space_desc {
+    int order;                /* basis order p, 1 .. ICE_MAX_ORDER */
+    int geometry_order;       /* must be 1 (see mesh.hpp:259-264 vs :584) */
+} ice_space_desc;
+
+typedef struct ice_mesh_counts {
+    int ndim;
+    int nelem;
+    int nnode;
+    int nvert_el;             /* 2^ndim */
+    int nfacenode;            /* 2^(ndim-1) */
+    int nface;                /* total faces (interior first, then boundary) */
+    int nface_interior;
+} ice_mesh_counts;
+
+/* Face record exported by ice_mesh_export, one per face in reference order:
+ * elemL, elemR, face_info_l, face_info_r, bctype, bcflag, nodes[nfacenode]
+ * with face_info = face_nr * 512 + orientation (face.hpp:207-209). */
+#define ICE_FACE_REC_FIXED 6
+
+typedef struct ice_space_info {
+    int ndim;
+    int order;
+    int nbasis;               /* (p+1)^ndim */
+    int nqp;                  /* volume quadrature points (p+1)^ndim */
+    int nfqp;                 /* face quadrature points (p+1)^(ndim-1) */
+    int nelem;
+    int nface;
+    int nface_interior;
+    int nface_boundary;
+    int64_t ndof;             /* nelem * nbasis (per component) */
+} ice_space_info;
+
+/* Discretization parameters: ConservationLawDDG public members
+ * (conservation_law.hpp:132-170) plus the flux functor coefficients. */
+typedef struct ice_disc_params {
+    int flux;                 /* ICE_FLUX_* */
+    /* BurgersCoefficients (burgers.hpp:14-40) */
+    double mu;
+    double a[3];
+    double b[3];
+    /* Navier-Stokes physics (navier_stokes.hpp:321-770) */
+    double gamma;             /* default 1.4 */
+    double Rgas;              /* default 287.052874 */
+    double Pr;                /* default 0.72 */
+    double ref[8];            /* ReferenceParameters t,l,rho,u,e,p,T,mu (default all 1) */
+    int viscosity_kind;       /* ICE_VISC_* */
+    double visc_mu;           /* constant viscosity value */
+    double isothermal_T[ICE_MAX_BCFLAG];
+    double freestream[6];     /* rho_inf, u_inf, dir x,y,z, T_inf (RIEMANN bc) */
+    int ns_face_energy_textbook; /* 0: face energy flux as navier_stokes.hpp:1277-1279
+                                    (mu*gamma/Pr * grad E . n); 1: consistent with the
+                                    volume heat flux (grad e) */
+    /* ConservationLawDDG */
+    double sigma_ic;          /* DDGIC multiplier */
+    int interior_penalty;     /* beta1 = 0 */
+    ice_field dirichlet[ICE_MAX_BCFLAG];
+    ice_field neumann[ICE_MAX_BCFLAG];
+    ice_field source;         /* user_source; kind NONE = no source */
+} ice_disc_params;
+
+/* Fill an ice_disc_params with the reference defaults for a flux family. */
+int ice_disc_params_default(int flux, ice_disc_params* out);
+
+/* ---- opaque handles -------------------------------------------------- */
+typedef struct ice_space ice_space;
+typedef struct ice_disc ice_disc;
+
+/* Library version and last error message of the calling thread. */
+int ice_version(void);
+const char* ice_last_error(void);
+
+/* Host-only mesh build (no device use).  Replaces the uniform AbstractMesh
+ * constructor (mesh.hpp:553-913) as an inspection path: with coord/conn/faces
+ * NULL it only fills `counts`; otherwise it writes node coordinates
+ * [nnode][ndim], element connectivity [nelem][2^ndim] (element-local tensor
+ * order, mesh.hpp:637-649) and face records [nface][6 + nfacenode]. */
+int ice_mesh_export(const ice_mesh_desc* mesh, ice_mesh_counts* counts,
+                    double* coord, int* conn, int* faces);
+
+/* FESpace construction (fespace.hpp:200-416): mesh, reference tables and the
+ * trace maps are built on the host and uploaded to the current HIP device. */
+int ice_space_create(const ice_mesh_desc* mesh, const ice_space_desc* space,
+                     ice_space** out);
+int ice_space_destroy(ice_space* space);
+int ice_space_query(const ice_space* space, ice_space_info* info);
+
+/* Physical coordinates of the volume quadrature points, host array
+ * [nelem][nqp][ndim] (FiniteElement::transform at getQP, finite_element.hpp:406-408). */
+int ice_space_qp_coords(const ice_space* space, double* xq);
+
+/* Discretization (ConservationLawDDG constructor + public members). */
+int ice_disc_create(ice_space* space, const ice_disc_params* params, ice_disc** out);
+int ice_disc_destroy(ice_disc* disc);
+
+/* form_residual (form_residual.hpp:56-268): res <- R(u), FP64, device pointers
+ * in fe_layout_right order, both nelem*nbasis*neq doubles. */
+int ice_residual(ice_disc* disc, const double* u_dev, double* res_dev, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ICEICLE_AMD_H */

@@ -1,0 +1,206 @@
+"""ctypes front end of the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use
+this module, and only as the checker.  The oracle is a serial FP64
+restatement of the reference's form_residual path (see oracle/*.hpp for the
+file:line of every function it follows).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from iceicle_amd import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle)")
+        L = C.CDLL(path)
+        P = C.c_void_p
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int)
+        L.ice_oracle_create.restype = P
+        L.ice_oracle_create.argtypes = [C.POINTER(abi.MeshDesc), C.POINTER(abi.SpaceDesc), C.POINTER(abi.DiscParams), C.c_int]
+        L.ice_oracle_destroy.argtypes = [P]
+        L.ice_oracle_last_error.restype = C.c_char_p
+        for name, args in [
+            ("ice_oracle_info", [P, ip]),
+            ("ice_oracle_residual", [P, dp, dp]),
+            ("ice_oracle_faces", [P, ip]),
+            ("ice_oracle_mesh", [P, dp, ip]),
+            ("ice_oracle_project", [P, C.POINTER(abi.Field), dp]),
+            ("ice_oracle_rk3_ssp", [P, dp, C.c_double, C.c_int, C.c_int]),
+            ("ice_oracle_rk3_tvd_tfinal", [P, dp, C.c_double, C.c_double]),
+            ("ice_oracle_trace_points", [P, dp, dp]),
+            ("ice_oracle_face_geometry", [P, dp, dp, dp]),
+            ("ice_oracle_qp_coords", [P, dp]),
+            ("ice_oracle_mass_apply", [P, dp, dp]),
+            ("ice_oracle_lagrange_1d", [C.c_int, C.c_double, dp, dp, dp]),
+            ("ice_oracle_gauss_legendre", [C.c_int, dp, dp]),
+            ("ice_oracle_tensor_basis", [C.c_int, C.c_int, dp, dp, dp, dp]),
+            ("ice_oracle_element_jacobian", [C.c_int, dp, dp, dp]),
+            ("ice_oracle_ns_flux", [C.POINTER(abi.DiscParams), C.c_int, dp, dp, dp]),
+            ("ice_oracle_vanleer", [C.POINTER(abi.DiscParams), C.c_int, dp, dp, dp, dp]),
+        ]:
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        L.ice_oracle_l2_error.argtypes = [P, C.POINTER(abi.Field), dp]
+        L.ice_oracle_l2_error.restype = C.c_double
+        L.ice_oracle_cfl_dt.argtypes = [P, C.c_double]
+        L.ice_oracle_cfl_dt.restype = C.c_double
+        _LIB = L
+    return _LIB
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError("oracle: " + lib().ice_oracle_last_error().decode())
+
+
+class Oracle:
+    """One mesh + FE space + ConservationLawDDG on the CPU."""
+
+    def __init__(self, mesh, space, params, fix3d_boundary_order=True):
+        L = lib()
+        self._mesh, self._space, self._params = mesh, space, params
+        self.h = L.ice_oracle_create(C.byref(mesh), C.byref(space), C.byref(params), int(fix3d_boundary_order))
+        if not self.h:
+            raise RuntimeError("oracle: " + L.ice_oracle_last_error().decode())
+        info = np.zeros(10, dtype=np.int32)
+        _check(L.ice_oracle_info(self.h, _ip(info)))
+        (self.ndim, self.nelem, self.nbasis, self.neq, self.nface, self.nface_interior,
+         self.nfqp, self.nqp, self.nnode, self.nperiodic) = (int(v) for v in info)
+        self.size = self.nelem * self.nbasis * self.neq
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ice_oracle_destroy(self.h)
+            self.h = None
+
+    def residual(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        r = np.empty(self.size)
+        _check(lib().ice_oracle_residual(self.h, _dp(u), _dp(r)))
+        return r
+
+    def faces(self):
+        nfv = 1 if self.ndim == 1 else 2 ** (self.ndim - 1)
+        rec = np.empty((self.nface, abi.FACE_REC_FIXED + nfv), dtype=np.int32)
+        _check(lib().ice_oracle_faces(self.h, _ip(rec)))
+        return rec
+
+    def mesh(self):
+        coord = np.empty((self.nnode, self.ndim))
+        conn = np.empty((self.nelem, 2 ** self.ndim), dtype=np.int32)
+        _check(lib().ice_oracle_mesh(self.h, _dp(coord), _ip(conn)))
+        return coord, conn
+
+    def project(self, field):
+        u = np.empty(self.size)
+        _check(lib().ice_oracle_project(self.h, C.byref(field), _dp(u)))
+        return u
+
+    def l2_error(self, field, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        return lib().ice_oracle_l2_error(self.h, C.byref(field), _dp(u))
+
+    def rk3_ssp(self, u, dt, nsteps, textbook=True):
+        u = np.array(u, dtype=np.float64)
+        _check(lib().ice_oracle_rk3_ssp(self.h, _dp(u), dt, nsteps, 1 if textbook else 0))
+        return u
+
+    def rk3_tvd_tfinal(self, u, dt, tfinal):
+        u = np.array(u, dtype=np.float64)
+        n = lib().ice_oracle_rk3_tvd_tfinal(self.h, _dp(u), dt, tfinal)
+        if n < 0:
+            _check(-1)
+        return u, n
+
+    def cfl_dt(self, cfl):
+        return lib().ice_oracle_cfl_dt(self.h, cfl)
+
+    def trace_points(self):
+        n = self.nface * self.nfqp
+        xl = np.empty((n, self.ndim))
+        xr = np.empty((n, self.ndim))
+        _check(lib().ice_oracle_trace_points(self.h, _dp(xl), _dp(xr)))
+        return xl.reshape(self.nface, self.nfqp, self.ndim), xr.reshape(self.nface, self.nfqp, self.ndim)
+
+    def face_geometry(self):
+        n = self.nface * self.nfqp
+        x = np.empty((n, self.ndim))
+        nrm = np.empty((n, self.ndim))
+        sg = np.empty(n)
+        _check(lib().ice_oracle_face_geometry(self.h, _dp(x), _dp(nrm), _dp(sg)))
+        sh = (self.nface, self.nfqp)
+        return x.reshape(*sh, self.ndim), nrm.reshape(*sh, self.ndim), sg.reshape(sh)
+
+    def qp_coords(self):
+        xq = np.empty((self.nelem, self.nqp, self.ndim))
+        _check(lib().ice_oracle_qp_coords(self.h, _dp(xq)))
+        return xq
+
+    def mass_apply(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        out = np.zeros(self.size)
+        _check(lib().ice_oracle_mass_apply(self.h, _dp(r), _dp(out)))
+        return out
+
+
+def lagrange_1d(p, xi):
+    N, dN, d2N = (np.zeros(p + 1) for _ in range(3))
+    _check(lib().ice_oracle_lagrange_1d(p, xi, _dp(N), _dp(dN), _dp(d2N)))
+    return N, dN, d2N
+
+
+def gauss_legendre(n):
+    x, w = np.zeros(n), np.zeros(n)
+    _check(lib().ice_oracle_gauss_legendre(n, _dp(x), _dp(w)))
+    return x, w
+
+
+def tensor_basis(ndim, p, xi):
+    nb = (p + 1) ** ndim
+    xi = np.ascontiguousarray(xi, dtype=np.float64)
+    B, G, H = np.zeros(nb), np.zeros(nb * ndim), np.zeros(nb * ndim * ndim)
+    _check(lib().ice_oracle_tensor_basis(ndim, p, _dp(xi), _dp(B), _dp(G), _dp(H)))
+    return B, G.reshape(nb, ndim), H.reshape(nb, ndim, ndim)
+
+
+def element_jacobian(ndim, X, xi):
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    xi = np.ascontiguousarray(xi, dtype=np.float64)
+    J = np.zeros(ndim * ndim)
+    _check(lib().ice_oracle_element_jacobian(ndim, _dp(X), _dp(xi), _dp(J)))
+    return J.reshape(ndim, ndim)
+
+
+def ns_flux(params, ndim, u, gradu):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    gradu = np.ascontiguousarray(gradu, dtype=np.float64)
+    F = np.zeros((ndim + 2) * ndim)
+    _check(lib().ice_oracle_ns_flux(C.byref(params), ndim, _dp(u), _dp(gradu), _dp(F)))
+    return F.reshape(ndim + 2, ndim)
+
+
+def vanleer(params, ndim, uL, uR, n):
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (uL, uR, n)]
+    f = np.zeros(ndim + 2)
+    _check(lib().ice_oracle_vanleer(C.byref(params), ndim, _dp(arrs[0]), _dp(arrs[1]), _dp(arrs[2]), _dp(f)))
+    return f

@@ -1,0 +1,176 @@
+// CPU ORACLE (test infrastructure only; never linked into the product).
+//
+// Callers of form_residual restated from the reference:
+//  L2 projection + element mass solve (disc/projection.hpp:83-110,
+//    solvers/element_linear_solve.hpp:37-90), l2_error (disc/l2_error.hpp:37-94),
+//  calculate_mass_matrix (element/finite_element.hpp:586-611),
+//  CFLTimestep / FixedTimestep / termination (solvers/explicit_utils.hpp:23-166),
+//  RK3SSP::step (solvers/ssp_rk3.hpp:106-195), RK3TVD::step (tvd_rk3.hpp:107-231),
+//  ExplicitEuler::step (explicit_euler.hpp:87-138),
+//  form_petsc_jacobian_fd DG part (form_petsc_jacobian.hpp:55-317, fd_utils.hpp:10-14),
+//  domain_integral_jacobian (conservation_law.hpp:316-432).
+#pragma once
+#include "disc.hpp"
+
+namespace oracle {
+
+template <int D>
+std::vector<double> mass_matrix(const FESpace<D>& fs, int e) {
+    const int nb = fs.nb;
+    std::vector<double> M(nb * nb, 0.0);
+    const auto g = fs.geo(e);
+    for (int q = 0; q < fs.nq; ++q) {
+        double J[D][D], H[D][D][D];
+        g.eval(fs.q1, fs.quad.pts[q].data(), nullptr, J, H);
+        const double detJ = determinant<D>(J);
+        const double* bi = &fs.B[q * nb];
+        for (int i = 0; i < nb; ++i)
+            for (int j = 0; j < nb; ++j) M[i * nb + j] += bi[i] * bi[j] * fs.quad.w[q] * detJ;
+    }
+    return M;
+}
+
+// Projection (projection.hpp:83-110) + ElementLinearSolver solve per element
+template <int D>
+void project(const FESpace<D>& fs, int neq, const Callback& f, double* u) {
+    const int nb = fs.nb;
+    std::vector<double> b(nb * neq), bb(nb), xx(nb);
+    for (int e = 0; e < fs.mesh.nelem; ++e) {
+        std::fill(b.begin(), b.end(), 0.0);
+        const auto g = fs.geo(e);
+        for (int q = 0; q < fs.nq; ++q) {
+            double x[D], J[D][D], H[D][D][D];
+            g.eval(fs.q1, fs.quad.pts[q].data(), x, J, H);
+            const double detJ = determinant<D>(J);
+            double fe[8];
+            f(x, fe);
+            for (int i = 0; i < nb; ++i)
+                for (int v = 0; v < neq; ++v) b[i * neq + v] += fe[v] * fs.quad.w[q] * fs.B[q * nb + i] * detJ;
+        }
+        DenseLU lu;
+        lu.factor(mass_matrix(fs, e), nb);
+        for (int v = 0; v < neq; ++v) {
+            for (int i = 0; i < nb; ++i) bb[i] = b[i * neq + v];
+            lu.solve(bb.data(), xx.data());
+            for (int i = 0; i < nb; ++i) u[((size_t)e * nb + i) * neq + v] = xx[i];
+        }
+    }
+}
+
+// l2_error (l2_error.hpp:37-94), |detJ|
+template <int D>
+double l2_error(const FESpace<D>& fs, int neq, const Callback& exact, const double* u) {
+    const int nb = fs.nb;
+    std::vector<double> l2(neq, 0.0);
+    for (int e = 0; e < fs.mesh.nelem; ++e) {
+        const auto g = fs.geo(e);
+        for (int q = 0; q < fs.nq; ++q) {
+            double x[D], J[D][D], H[D][D][D];
+            g.eval(fs.q1, fs.quad.pts[q].data(), x, J, H);
+            const double detJ = determinant<D>(J);
+            double fe[8], uu[8] = {};
+            exact(x, fe);
+            for (int i = 0; i < nb; ++i)
+                for (int v = 0; v < neq; ++v) uu[v] += fs.B[q * nb + i] * u[((size_t)e * nb + i) * neq + v];
+            for (int v = 0; v < neq; ++v) l2[v] += std::pow(uu[v] - fe[v], 2) * fs.quad.w[q] * std::abs(detJ);
+        }
+    }
+    double s = 0;
+    for (int v = 0; v < neq; ++v) s += l2[v];
+    return std::sqrt(s);
+}
+
+// Per-element LU factors of the mass matrices.  The reference rebuilds and
+// refactors them at every stage (ssp_rk3.hpp:142-163); the arithmetic is the
+// same so they are cached here.
+template <int D>
+struct MassInverse {
+    std::vector<DenseLU> lu;
+    explicit MassInverse(const FESpace<D>& fs) {
+        lu.resize(fs.mesh.nelem);
+        for (int e = 0; e < fs.mesh.nelem; ++e) lu[e].factor(mass_matrix(fs, e), fs.nb);
+    }
+    // out += M^-1 r per element and component (singular: 1e-8 * r, tvd_rk3.hpp:190-204)
+    void apply_add(const FESpace<D>& fs, int neq, const double* r, double* out) const {
+        const int nb = fs.nb;
+        std::vector<double> b(nb), x(nb);
+        for (int e = 0; e < fs.mesh.nelem; ++e)
+            for (int v = 0; v < neq; ++v) {
+                for (int i = 0; i < nb; ++i) b[i] = r[((size_t)e * nb + i) * neq + v];
+                if (lu[e].singular) {
+                    for (int i = 0; i < nb; ++i) out[((size_t)e * nb + i) * neq + v] += 1e-8 * b[i];
+                    continue;
+                }
+                lu[e].solve(b.data(), x.data());
+                for (int i = 0; i < nb; ++i) out[((size_t)e * nb + i) * neq + v] += x[i];
+            }
+    }
+};
+
+// CFLTimestep::operator() (explicit_utils.hpp:61-90)
+template <int D, class Disc>
+double cfl_timestep(const FESpace<D>& fs, Disc& disc, double cfl) {
+    double reflen = 1e8;
+    for (int e = 0; e < fs.mesh.nelem; ++e) {
+        double xi[D] = {}, J[D][D], H[D][D][D];
+        fs.geo(e).eval(fs.q1, xi, nullptr, J, H);
+        reflen = std::min(reflen, std::pow(determinant<D>(J), 1.0 / D));
+    }
+    return disc.dt_from_cfl(cfl, reflen) / (2 * std::max(1, fs.p) + 1);
+}
+
+enum class RKMode { REFERENCE_COMPAT = 0, TEXTBOOK = 1 };
+
+// RK3SSP::step.  REFERENCE_COMPAT evaluates every stage residual at u^n
+// (ssp_rk3.hpp:132-137, form_residual is called with `u`); TEXTBOOK uses the
+// stage states.
+template <int D, class Disc>
+void rk3_ssp_step(const FESpace<D>& fs, const Disc& disc, const MassInverse<D>& minv, double dt, RKMode mode,
+                  double* u) {
+    constexpr int neq = Disc::neq;
+    const size_t N = (size_t)fs.mesh.nelem * fs.nb * neq;
+    std::vector<double> res(N), r1(N, 0.0), r2(N, 0.0), r3(N, 0.0), us(N);
+    auto stage = [&](const double* ust, std::vector<double>& out) {
+        std::fill(out.begin(), out.end(), 0.0);
+        form_residual(fs, disc, mode == RKMode::REFERENCE_COMPAT ? u : ust, res.data());
+        minv.apply_add(fs, neq, res.data(), out.data());
+    };
+    stage(u, r1);
+    for (size_t k = 0; k < N; ++k) us[k] = u[k] + dt * r1[k];
+    stage(us.data(), r2);
+    for (size_t k = 0; k < N; ++k) us[k] = u[k] + 0.25 * dt * r1[k] + 0.25 * dt * r2[k];
+    stage(us.data(), r3);
+    for (size_t k = 0; k < N; ++k) u[k] += 1.0 / 6.0 * dt * r1[k];
+    for (size_t k = 0; k < N; ++k) u[k] += 1.0 / 6.0 * dt * r2[k];
+    for (size_t k = 0; k < N; ++k) u[k] += 2.0 / 3.0 * dt * r3[k];
+}
+
+// RK3TVD::step (tvd_rk3.hpp:107-231): in-place Shu-Osher stages
+template <int D, class Disc>
+void rk3_tvd_step(const FESpace<D>& fs, const Disc& disc, const MassInverse<D>& minv, double dt, double* u) {
+    constexpr int neq = Disc::neq;
+    const size_t N = (size_t)fs.mesh.nelem * fs.nb * neq;
+    static const double alfa[3] = {0.0, 0.75, 1.0 / 3.0};
+    static const double beta[3] = {1.0, 0.25, 2.0 / 3.0};
+    std::vector<double> res(N), r1(N), uold(u, u + N);
+    for (int s = 0; s < 3; ++s) {
+        std::fill(r1.begin(), r1.end(), 0.0);
+        form_residual(fs, disc, u, res.data());
+        minv.apply_add(fs, neq, res.data(), r1.data());
+        for (size_t k = 0; k < N; ++k) u[k] = alfa[s] * uold[k] + beta[s] * u[k];
+        for (size_t k = 0; k < N; ++k) u[k] += beta[s] * dt * r1[k];
+    }
+}
+
+// ExplicitEuler::step (explicit_euler.hpp:87-138)
+template <int D, class Disc>
+void explicit_euler_step(const FESpace<D>& fs, const Disc& disc, const MassInverse<D>& minv, double dt, double* u) {
+    constexpr int neq = Disc::neq;
+    const size_t N = (size_t)fs.mesh.nelem * fs.nb * neq;
+    std::vector<double> res(N), r1(N, 0.0);
+    form_residual(fs, disc, u, res.data());
+    minv.apply_add(fs, neq, res.data(), r1.data());
+    for (size_t k = 0; k < N; ++k) u[k] += dt * r1[k];
+}
+
+}  // namespace oracle
