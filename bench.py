@@ -1,0 +1,169 @@
+"""Benchmark: DG residual throughput on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): 2D compressible
+Navier-Stokes manufactured steady problem, 128 x 128 quads with seeded 5%
+interior node perturbation, P3 Lagrange, 4 x 4 Gauss-Legendre, 4 conserved
+variables, VanLeer + DDG viscous fluxes, Dirichlet boundaries, manufactured
+source.  One step = one FP64 residual evaluation R(u) (form_residual) over
+the whole mesh with u and R resident in HBM.
+
+Prints one JSON line (see the repository's measurement notes for the fields).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=128, help="elements per direction")
+    ap.add_argument("--order", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
+                    help="only run 20 device copies of an MB-sized buffer (HBM counter calibration)")
+    return ap.parse_args()
+
+
+def cpu_baseline(mesh, order, params, u, seconds):
+    """The oracle (serial FP64 restatement of the reference path) on the same
+    workload, timed on this host: repeated full residuals until `seconds`."""
+    from iceicle_amd import abi
+    from oracle.oracle import Oracle
+    o = Oracle(mesh, abi.space_desc(order), params)
+    o.residual(u)  # warm
+    n, t0 = 0, time.perf_counter()
+    while True:
+        o.residual(u)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or n >= 100:
+            break
+    return n, el, o.size
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
+        sys.exit(2)
+    import torch
+    from iceicle_amd import abi, engine, workloads
+
+    if args.calibrate_copy:
+        n = args.calibrate_copy * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(20):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes_read": 8 * n, "calibrate_copy_bytes_written": 8 * n, "copies": 20}))
+        return
+
+    mesh, order, params, coords = workloads.ns_mms_2d(args.n, args.order)
+    space = engine.FESpace(mesh, order)
+    disc = engine.ConservationLawDDG(space, params)
+    neq = disc.neq
+    u_host = workloads.mms_state_perturbed(space, neq, workloads.SEED0 + 2)
+    u = torch.from_numpy(u_host).cuda()
+    r = torch.empty_like(u)
+
+    for _ in range(args.warmup):
+        engine.form_residual(space, disc, u, r)
+    torch.cuda.synchronize()
+
+    disc.profile_begin(args.steps)
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(args.steps):
+        engine.form_residual(space, disc, u, r)
+    stop.record()
+    torch.cuda.synchronize()
+    total_ms = start.elapsed_time(stop)
+    stage_ms, ncalls = disc.profile_end()
+    ms_per_step = total_ms / args.steps
+    ndof = space.nelem * space.nbasis * neq
+    gdofs = ndof / (ms_per_step * 1e-3) / 1e9
+    assert bool(torch.isfinite(r).all())
+
+    # per-kernel compulsory (unique) bytes of one launch
+    info = space.info
+    D, nb, nfq = space.ndim, space.nbasis, space.nfqp
+    E, nfi, nfb = space.nelem, info.nface_interior, info.nface_boundary
+    nvert = 2 ** D
+    kernels = {
+        "face_interior": (stage_ms[0] / ncalls,
+                          8 * E * nb * neq + 8 * E * nvert * D + 16 * nfi + 8 * nfi * nfq * neq),
+        "face_boundary": (stage_ms[1] / ncalls,
+                          8 * nfb * nb * neq + 8 * nfb * nvert * D + 16 * nfb + 8 * nfb * nfq * neq),
+        "element": (stage_ms[2] / ncalls,
+                    8 * E * nb * neq + 8 * E * nvert * D + 4 * 2 * D * E + 8 * (nfi + nfb) * nfq * neq
+                    + 8 * E * nb * neq),
+    }
+    dom = max(kernels, key=lambda k: kernels[k][0])
+    kms, kbytes = kernels[dom]
+    achieved = kbytes / (kms * 1e-3) / 1e9
+    nnode = (args.n + 1) ** 2
+    b_alg = workloads.residual_bytes(D, order, neq, E, nnode, nfi + nfb)
+    res_gbs = b_alg / (ms_per_step * 1e-3) / 1e9
+
+    out = {
+        "metric": "residual GDOF/s (FP64 vs mixed, 1 GPU); Newton time to ||R||<1e-10",
+        "value": round(gdofs, 4),
+        "unit": "GDOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"2D NS MMS {args.n}x{args.n} quads, 5% perturbed, P{order}, "
+                        f"{order + 1}x{order + 1} GL, FP64 residual",
+            "elements": E, "unknowns": ndof, "neq": neq, "order": order,
+        },
+        "kernels_ms": {k: round(v[0], 6) for k, v in kernels.items()},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": None,
+            "bytes_per_launch": int(kbytes),
+        },
+        "residual_roofline": {"bytes": int(b_alg), "achieved_GBs": round(res_gbs, 2),
+                              "frac": round(res_gbs / HBM_PEAK_GBS, 5)},
+    }
+    if not args.no_cpu_baseline:
+        n, el, size = cpu_baseline(mesh, order, params, u_host, args.cpu_seconds)
+        out["cpu_baseline"] = {
+            "value": round(size * n / el / 1e9, 6),
+            "unit": "GDOF/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"{n} full residuals of the same workload on the host CPU ({el:.1f} s)",
+            "gpu_speedup": round(gdofs / (size * n / el / 1e9), 1),
+        }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

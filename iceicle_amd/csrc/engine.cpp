@@ -15,6 +15,7 @@
 #include "kernels.h"
 #include "mesh.hpp"
 #include "physics_dev.h"
+#include "fields.h"
 
 namespace {
 thread_local std::string g_err;
@@ -126,6 +127,9 @@ struct ice_space {
 };
 
 struct ice_disc {
+    std::vector<hipEvent_t> prof_ev;  // 4 per profiled residual
+    int prof_max = 0, prof_n = 0;
+    bool profiling = false;
     ice_space* space = nullptr;
     ice_disc_params prm{};
     ice::PhysParams ph{};
@@ -136,6 +140,7 @@ struct ice_disc {
     double* d_Q = nullptr;
     double* d_Pic = nullptr;
     ~ice_disc() {
+        for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
         if (d_fields) (void)hipFree(d_fields);
         if (d_Q) (void)hipFree(d_Q);
         if (d_Pic) (void)hipFree(d_Pic);
@@ -465,9 +470,96 @@ int ice_disc_destroy(ice_disc* d) {
 int ice_residual(ice_disc* d, const double* u, double* r, void* stream) {
     if (!d || !u || !r) return fail(ICE_ERR_ARG, "null argument");
     const ice_space* s = d->space;
+    hipEvent_t* ev = nullptr;
+    if (d->profiling && d->prof_n < d->prof_max) ev = &d->prof_ev[(size_t)4 * d->prof_n++];
     const int rc = ice::residual_fp64(s->ndim, s->p, d->prm.flux, s->dev, d->ph, d->d_fields, d->has_source, d->ddgic, u,
-                                      d->d_Q, d->d_Pic, r, (hipStream_t)stream);
+                                      d->d_Q, d->d_Pic, r, (hipStream_t)stream, ev);
     if (rc != 0) return fail(ICE_ERR_HIP, std::string("residual launch failed: ") + hipGetErrorString((hipError_t)rc));
+    return ICE_OK;
+}
+
+int ice_profile_begin(ice_disc* d, int max_calls) {
+    if (!d || max_calls < 1) return fail(ICE_ERR_ARG, "bad argument");
+    try {
+        while ((int)d->prof_ev.size() < 4 * max_calls) {
+            hipEvent_t e;
+            ICE_HIP(hipEventCreate(&e));
+            d->prof_ev.push_back(e);
+        }
+    } catch (const ice::Error& e) {
+        return fail(e.code, e.msg);
+    }
+    d->prof_max = max_calls;
+    d->prof_n = 0;
+    d->profiling = true;
+    return ICE_OK;
+}
+
+int ice_profile_end(ice_disc* d, double* stage_ms, int* ncalls) {
+    if (!d || !stage_ms || !ncalls) return fail(ICE_ERR_ARG, "null argument");
+    if (!d->profiling) return fail(ICE_ERR_STATE, "profiling not active");
+    d->profiling = false;
+    for (int k = 0; k < 3; ++k) stage_ms[k] = 0.0;
+    for (int c = 0; c < d->prof_n; ++c) {
+        hipEvent_t* ev = &d->prof_ev[(size_t)4 * c];
+        if (hipEventSynchronize(ev[3]) != hipSuccess) return fail(ICE_ERR_HIP, "event synchronize failed");
+        for (int k = 0; k < 3; ++k) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, ev[k], ev[k + 1]) != hipSuccess) return fail(ICE_ERR_HIP, "event timing failed");
+            stage_ms[k] += ms;
+        }
+    }
+    *ncalls = d->prof_n;
+    return ICE_OK;
+}
+
+int ice_field_eval_points(const ice_field* f, int ndim, int neq, int npts, const double* x, double* out) {
+    if (!f || !x || !out || ndim < 1 || ndim > 3 || neq < 1 || neq > 5) return fail(ICE_ERR_ARG, "bad argument");
+    for (int i = 0; i < npts; ++i) ice_field_eval(f, ndim, neq, x + (size_t)i * ndim, out + (size_t)i * neq);
+    return ICE_OK;
+}
+
+int ice_space_node_coords(const ice_space* s, double* xn) {
+    if (!s || !xn) return fail(ICE_ERR_ARG, "null argument");
+    const int D = s->ndim, n = s->n, p = s->p, NV = 1 << D;
+    for (int e = 0; e < s->mesh.nelem; ++e)
+        for (int i = 0; i < s->nb; ++i) {
+            int r = i, id[3];
+            for (int d = D - 1; d >= 0; --d) { id[d] = r % n; r /= n; }
+            double xi[3];
+            for (int d = 0; d < D; ++d) {
+                double x = -1.0;  // equispaced nodes by cumulative steps (lagrange_1d.hpp:40-45)
+                for (int j = 0; j < id[d]; ++j) x += 2.0 / p;
+                xi[d] = x;
+            }
+            double x[3] = {0, 0, 0};
+            for (int v = 0; v < NV; ++v) {
+                double phi = 1.0;
+                for (int d = 0; d < D; ++d) {
+                    const double n0 = 0.5 * (1 - xi[d]);
+                    phi *= ((v >> (D - 1 - d)) & 1) ? (1.0 - n0) : n0;
+                }
+                for (int d = 0; d < D; ++d) x[d] += phi * s->X[((size_t)e * NV + v) * D + d];
+            }
+            for (int d = 0; d < D; ++d) xn[((size_t)e * s->nb + i) * D + d] = x[d];
+        }
+    return ICE_OK;
+}
+
+int ice_trace_points(int ndim, int order, int face_nr, int orientation, double* xi) {
+    if (ndim < 1 || ndim > 3 || order < 1 || order > ICE_MAX_ORDER || face_nr < 0 || face_nr >= 2 * ndim || !xi ||
+        orientation < 0 || orientation >= ice::n_orientations(ndim))
+        return fail(ICE_ERR_ARG, "bad argument");
+    const int n = order + 1;
+    double gx[8], gw[8];
+    gauss_legendre(n, gx, gw);
+    int nfq = 1;
+    for (int d = 0; d < ndim - 1; ++d) nfq *= n;
+    for (int q = 0; q < nfq; ++q) {
+        int pt[3];
+        ice::trace_point_indices(ndim, n, face_nr, orientation, q, pt);
+        for (int d = 0; d < ndim; ++d) xi[q * ndim + d] = (pt[d] < n) ? gx[pt[d]] : (pt[d] == n ? -1.0 : 1.0);
+    }
     return ICE_OK;
 }
 

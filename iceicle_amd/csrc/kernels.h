@@ -32,6 +32,7 @@ inline int pack_face_info(int fnl, int fnr, int orient, int bctype, int pdim) {
 }
 
 int residual_fp64(int ndim, int p, int flux, const DevSpace& S, const PhysParams& ph, const ice_field* fields,
-                  int has_source, bool ddgic, const double* U, double* Q, double* Pic, double* R, hipStream_t st);
+                  int has_source, bool ddgic, const double* U, double* Q, double* Pic, double* R, hipStream_t st,
+                  hipEvent_t* ev = nullptr);
 
 }  // namespace ice

@@ -784,50 +784,55 @@ __global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph,
 // ---- launcher ------------------------------------------------------------
 template <int D, int P, class PH>
 int launch_residual(const DevSpace& S, const PhysParams& ph, const ice_field* fields, int has_source, bool ddgic,
-                    const double* U, double* Q, double* Pic, double* R, hipStream_t st) {
+                    const double* U, double* Q, double* Pic, double* R, hipStream_t st, hipEvent_t* ev) {
     constexpr int N = P + 1;
     constexpr int NFQ = Pow<N, D - 1>::value;
     double* pic = (PH::HAS_HOMOGENEITY && ddgic) ? Pic : nullptr;
+    if (ev) (void)hipEventRecord(ev[0], st);
     if (S.nfi > 0) {
         const long nt = (long)S.nfi * NFQ;
         face_interior_kernel<D, P, PH><<<(unsigned)((nt + 255) / 256), 256, 0, st>>>(S, ph, U, Q, pic);
     }
+    if (ev) (void)hipEventRecord(ev[1], st);
     if (S.nfb > 0) {
         const long nt = (long)S.nfb * NFQ;
         face_boundary_kernel<D, P, PH><<<(unsigned)((nt + 255) / 256), 256, 0, st>>>(S, ph, fields, U, Q, pic);
     }
+    if (ev) (void)hipEventRecord(ev[2], st);
     using T = ElemTile<D, P>;
     const int nblk = (S.nelem + T::EPB - 1) / T::EPB;
     element_kernel<D, P, PH><<<nblk, 256, 0, st>>>(S, ph, fields, has_source, U, Q, pic, R);
+    if (ev) (void)hipEventRecord(ev[3], st);
     return (int)hipGetLastError();
 }
 
 template <class PH, int D>
 int dispatch_order(int p, const DevSpace& S, const PhysParams& ph, const ice_field* fields, int has_source, bool ddgic,
-                   const double* U, double* Q, double* Pic, double* R, hipStream_t st) {
+                   const double* U, double* Q, double* Pic, double* R, hipStream_t st, hipEvent_t* ev) {
     switch (p) {
-    case 1: return launch_residual<D, 1, PH>(S, ph, fields, has_source, ddgic, U, Q, Pic, R, st);
-    case 2: return launch_residual<D, 2, PH>(S, ph, fields, has_source, ddgic, U, Q, Pic, R, st);
-    case 3: return launch_residual<D, 3, PH>(S, ph, fields, has_source, ddgic, U, Q, Pic, R, st);
-    case 4: return launch_residual<D, 4, PH>(S, ph, fields, has_source, ddgic, U, Q, Pic, R, st);
+    case 1: return launch_residual<D, 1, PH>(S, ph, fields, has_source, ddgic, U, Q, Pic, R, st, ev);
+    case 2: return launch_residual<D, 2, PH>(S, ph, fields, has_source, ddgic, U, Q, Pic, R, st, ev);
+    case 3: return launch_residual<D, 3, PH>(S, ph, fields, has_source, ddgic, U, Q, Pic, R, st, ev);
+    case 4: return launch_residual<D, 4, PH>(S, ph, fields, has_source, ddgic, U, Q, Pic, R, st, ev);
     default: return -1;
     }
 }
 
 int residual_fp64(int ndim, int p, int flux, const DevSpace& S, const PhysParams& ph, const ice_field* fields,
-                  int has_source, bool ddgic, const double* U, double* Q, double* Pic, double* R, hipStream_t st) {
+                  int has_source, bool ddgic, const double* U, double* Q, double* Pic, double* R, hipStream_t st,
+                  hipEvent_t* ev) {
     if (flux == ICE_FLUX_BURGERS) {
-        if (ndim == 1) return dispatch_order<Burgers<1>, 1>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st);
-        if (ndim == 2) return dispatch_order<Burgers<2>, 2>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st);
-        if (ndim == 3) return dispatch_order<Burgers<3>, 3>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st);
+        if (ndim == 1) return dispatch_order<Burgers<1>, 1>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st, ev);
+        if (ndim == 2) return dispatch_order<Burgers<2>, 2>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st, ev);
+        if (ndim == 3) return dispatch_order<Burgers<3>, 3>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st, ev);
     } else if (flux == ICE_FLUX_NAVIER_STOKES) {
-        if (ndim == 1) return dispatch_order<NavierStokes<1, true>, 1>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st);
-        if (ndim == 2) return dispatch_order<NavierStokes<2, true>, 2>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st);
-        if (ndim == 3) return dispatch_order<NavierStokes<3, true>, 3>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st);
+        if (ndim == 1) return dispatch_order<NavierStokes<1, true>, 1>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st, ev);
+        if (ndim == 2) return dispatch_order<NavierStokes<2, true>, 2>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st, ev);
+        if (ndim == 3) return dispatch_order<NavierStokes<3, true>, 3>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st, ev);
     } else if (flux == ICE_FLUX_EULER) {
-        if (ndim == 1) return dispatch_order<NavierStokes<1, false>, 1>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st);
-        if (ndim == 2) return dispatch_order<NavierStokes<2, false>, 2>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st);
-        if (ndim == 3) return dispatch_order<NavierStokes<3, false>, 3>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st);
+        if (ndim == 1) return dispatch_order<NavierStokes<1, false>, 1>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st, ev);
+        if (ndim == 2) return dispatch_order<NavierStokes<2, false>, 2>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st, ev);
+        if (ndim == 3) return dispatch_order<NavierStokes<3, false>, 3>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st, ev);
     }
     return -1;
 }

@@ -35,6 +35,12 @@ _SIGS = {
     "ice_disc_create": ([P, C.POINTER(abi.DiscParams), C.POINTER(P)], C.c_int),
     "ice_disc_destroy": ([P], C.c_int),
     "ice_residual": ([P, P, P, P], C.c_int),
+    "ice_profile_begin": ([P, C.c_int], C.c_int),
+    "ice_profile_end": ([P, C.POINTER(C.c_double), C.POINTER(C.c_int)], C.c_int),
+    "ice_field_eval_points": ([C.POINTER(abi.Field), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
+                               C.POINTER(C.c_double)], C.c_int),
+    "ice_space_node_coords": ([P, C.POINTER(C.c_double)], C.c_int),
+    "ice_trace_points": ([C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)], C.c_int),
 }
 
 
@@ -75,6 +81,25 @@ def mesh_export(mesh):
     return coord, conn, faces, cnt
 
 
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def field_eval(field, ndim, neq, x):
+    """Evaluate an analytic field at points x[npts][ndim] on the host."""
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, ndim)
+    out = np.empty((x.shape[0], neq))
+    check(lib().ice_field_eval_points(C.byref(field), ndim, neq, x.shape[0], _dp(x), _dp(out)))
+    return out
+
+
+def trace_points(ndim, order, face_nr, orientation):
+    nfq = (order + 1) ** (ndim - 1)
+    xi = np.empty((nfq, ndim))
+    check(lib().ice_trace_points(ndim, order, face_nr, orientation, _dp(xi)))
+    return xi
+
+
 def _require_cuda(t, name):
     import torch
     if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
@@ -102,6 +127,16 @@ class FESpace:
         check(lib().ice_space_qp_coords(self.h, xq.ctypes.data_as(C.POINTER(C.c_double))))
         return xq
 
+    def node_coords(self):
+        xn = np.empty((self.nelem, self.nbasis, self.ndim))
+        check(lib().ice_space_node_coords(self.h, _dp(xn)))
+        return xn
+
+    def interpolate(self, field, neq):
+        """Nodal interpolant of an analytic field (host array, fe_layout_right)."""
+        xn = self.node_coords().reshape(-1, self.ndim)
+        return field_eval(field, self.ndim, neq, xn).ravel()
+
     def __del__(self):
         if getattr(self, "h", None):
             lib().ice_space_destroy(self.h)
@@ -120,6 +155,17 @@ class ConservationLawDDG:
         self.h = h
         self.neq = 1 if params.flux == abi.FLUX_BURGERS else space.ndim + 2
         self.size = space.nelem * space.nbasis * self.neq
+
+    def profile_begin(self, max_calls):
+        check(lib().ice_profile_begin(self.h, max_calls))
+
+    def profile_end(self):
+        """Summed milliseconds of (interior-face, boundary-face, element) kernels
+        and the number of residuals profiled since profile_begin."""
+        ms = (C.c_double * 3)()
+        n = C.c_int()
+        check(lib().ice_profile_end(self.h, ms, C.byref(n)))
+        return [ms[0], ms[1], ms[2]], n.value
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -213,6 +213,29 @@ int ice_disc_destroy(ice_disc* disc);
  * in fe_layout_right order, both nelem*nbasis*neq doubles. */
 int ice_residual(ice_disc* disc, const double* u_dev, double* res_dev, void* stream);
 
+/* Kernel timing (no reference counterpart; the reference has no
+ * instrumentation).  Between begin and end, every residual on this handle
+ * records HIP events around its interior-face, boundary-face and element
+ * launches on the caller's stream; end waits for them and writes the summed
+ * milliseconds per stage to stage_ms[3] and the number of residuals to *ncalls. */
+int ice_profile_begin(ice_disc* disc, int max_calls);
+int ice_profile_end(ice_disc* disc, double* stage_ms, int* ncalls);
+
+/* Host-side evaluation of an analytic field at npts points x[npts][ndim]
+ * (the callbacks of conservation_law.hpp:146-170 and of the initial
+ * conditions, initialization.hpp:37-104). */
+int ice_field_eval_points(const ice_field* f, int ndim, int neq, int npts, const double* x, double* out);
+
+/* Physical coordinates of the Lagrange nodes of every element, host array
+ * [nelem][nbasis][ndim] (the nodal points of HypercubeLagrangeBasis mapped by
+ * the element transformation). */
+int ice_space_node_coords(const ice_space* space, double* xn);
+
+/* Reference-element point of every face quadrature point for face number
+ * `face_nr` seen with orientation `orientation` (TraceSpace transform_xiL /
+ * transform_xiR, hypercube_face.hpp:230-252): xi[nfqp][ndim], host-only. */
+int ice_trace_points(int ndim, int order, int face_nr, int orientation, double* xi);
+
 #ifdef __cplusplus
 }
 #endif
