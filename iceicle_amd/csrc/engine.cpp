@@ -139,11 +139,13 @@ struct ice_disc {
     ice_field* d_fields = nullptr;
     double* d_Q = nullptr;
     double* d_Pic = nullptr;
+    double* d_bval = nullptr;
+    double* d_srcload = nullptr;
+    ice::DiscDev dd{};
     ~ice_disc() {
         for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
-        if (d_fields) (void)hipFree(d_fields);
-        if (d_Q) (void)hipFree(d_Q);
-        if (d_Pic) (void)hipFree(d_Pic);
+        for (void* p : {(void*)d_fields, (void*)d_Q, (void*)d_Pic, (void*)d_bval, (void*)d_srcload})
+            if (p) (void)hipFree(p);
     }
 };
 
@@ -455,6 +457,16 @@ int ice_disc_create(ice_space* s, const ice_disc_params* p, ice_disc** out) {
         const size_t rows = (size_t)(s->dev.nfi + s->dev.nfb) * s->nfq * d->neq;
         ICE_HIP(hipMalloc(&d->d_Q, std::max<size_t>(rows, 1) * sizeof(double)));
         if (d->ddgic) ICE_HIP(hipMalloc(&d->d_Pic, std::max<size_t>(rows * D, 1) * sizeof(double)));
+        if (s->dev.nfb > 0) ICE_HIP(hipMalloc(&d->d_bval, (size_t)s->dev.nfb * s->nfq * d->neq * sizeof(double)));
+        if (d->has_source) ICE_HIP(hipMalloc(&d->d_srcload, (size_t)s->mesh.nelem * s->nb * d->neq * sizeof(double)));
+        if (ice::setup_disc_tables(D, s->p, d->neq, s->dev, d->d_fields, d->d_bval, d->d_srcload, nullptr) != 0)
+            throw ice::Error{ICE_ERR_HIP, "table setup launch failed"};
+        ICE_HIP(hipDeviceSynchronize());
+        d->dd.fields = d->d_fields;
+        d->dd.bval = d->d_bval;
+        d->dd.srcload = d->d_srcload;
+        d->dd.Q = d->d_Q;
+        d->dd.Pic = d->d_Pic;
         *out = d.release();
         return ICE_OK;
     } catch (const ice::Error& e) {
@@ -472,8 +484,7 @@ int ice_residual(ice_disc* d, const double* u, double* r, void* stream) {
     const ice_space* s = d->space;
     hipEvent_t* ev = nullptr;
     if (d->profiling && d->prof_n < d->prof_max) ev = &d->prof_ev[(size_t)4 * d->prof_n++];
-    const int rc = ice::residual_fp64(s->ndim, s->p, d->prm.flux, s->dev, d->ph, d->d_fields, d->has_source, d->ddgic, u,
-                                      d->d_Q, d->d_Pic, r, (hipStream_t)stream, ev);
+    const int rc = ice::residual_fp64(s->ndim, s->p, d->prm.flux, s->dev, d->ph, d->dd, u, r, (hipStream_t)stream, ev);
     if (rc != 0) return fail(ICE_ERR_HIP, std::string("residual launch failed: ") + hipGetErrorString((hipError_t)rc));
     return ICE_OK;
 }

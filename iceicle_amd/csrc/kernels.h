@@ -31,8 +31,21 @@ inline int pack_face_info(int fnl, int fnr, int orient, int bctype, int pdim) {
     return fnl | (fnr << 3) | (orient << 6) | (bctype << 9) | (pdim << 13);
 }
 
-int residual_fp64(int ndim, int p, int flux, const DevSpace& S, const PhysParams& ph, const ice_field* fields,
-                  int has_source, bool ddgic, const double* U, double* Q, double* Pic, double* R, hipStream_t st,
-                  hipEvent_t* ev = nullptr);
+// Per-discretization device data.
+struct DiscDev {
+    const ice_field* fields;  // [dirichlet x4 | neumann x4 | source]
+    const double* bval;       // [nfb][nfq][neq] Dirichlet / Neumann values at boundary points, or null
+    const double* srcload;    // [nelem][nbasis][neq] source load vector, or null
+    double* Q;                // [nfi + nfb][nfq][neq] weighted normal face fluxes
+    double* Pic;              // [nfi + nfb][nfq][neq][ndim] DDGIC lifting vectors, or null
+};
+
+int residual_fp64(int ndim, int p, int flux, const DevSpace& S, const PhysParams& ph, const DiscDev& dd,
+                  const double* U, double* R, hipStream_t st, hipEvent_t* ev = nullptr);
+
+// One-time tables of u-independent terms: boundary values at the boundary
+// face quadrature points and the source load vector sum_q s(x_q) phi_i w detJ.
+int setup_disc_tables(int ndim, int p, int neq, const DevSpace& S, const ice_field* fields, double* bval,
+                      double* srcload, hipStream_t st);
 
 }  // namespace ice

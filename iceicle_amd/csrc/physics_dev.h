@@ -96,7 +96,7 @@ struct NavierStokes {
     static constexpr bool HAS_HOMOGENEITY = false;
 
     struct State {
-        double rho, rhoE, E, e, p, T, c, vv;
+        double rho, irho, rhoE, E, e, p, T, c, vv;
         double m[D], v[D];
     };
 
@@ -104,21 +104,20 @@ struct NavierStokes {
     __device__ __forceinline__ static State state(const PhysParams& p, const double* u) {
         State s;
         s.rho = fmax(1e-14, u[0]);
-        const double irho = 1.0 / s.rho;
+        s.irho = 1.0 / s.rho;
         s.vv = 0;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             s.m[d] = u[1 + d];
-            s.v[d] = u[1 + d] / s.rho;
+            s.v[d] = u[1 + d] * s.irho;
             s.vv += s.v[d] * s.v[d];
         }
-        (void)irho;
         s.rhoE = u[D + 1];
-        s.E = s.rhoE / s.rho;
+        s.E = s.rhoE * s.irho;
         s.e = s.E - 0.5 * p.e_coeff * s.vv;
         s.p = fmax(1e-12, (p.gamma - 1) / (p.e_coeff * p.Eu) * s.rho * s.e);
-        s.T = s.p / s.rho * p.gamma / (p.gamma - 1) / p.T_coeff;
-        s.c = sqrt((p.gamma * p.Eu * s.p) / s.rho);
+        s.T = s.p * s.irho * (p.gamma / ((p.gamma - 1) * p.T_coeff));
+        s.c = sqrt((p.gamma * p.Eu) * s.p * s.irho);
         return s;
     }
     __device__ __forceinline__ static double viscosity(const PhysParams& p, double T) {
@@ -138,9 +137,9 @@ struct NavierStokes {
 #pragma unroll
         for (int i = 0; i < D; ++i)
 #pragma unroll
-            for (int j = 0; j < D; ++j) gv[i][j] = g[(1 + i) * D + j] / s.rho - s.v[i] / s.rho * g[j];
+            for (int j = 0; j < D; ++j) gv[i][j] = (g[(1 + i) * D + j] - s.v[i] * g[j]) * s.irho;
 #pragma unroll
-        for (int j = 0; j < D; ++j) gE[j] = (g[(D + 1) * D + j] - s.E * g[j]) / s.rho;
+        for (int j = 0; j < D; ++j) gE[j] = (g[(D + 1) * D + j] - s.E * g[j]) * s.irho;
         double div = 0;
 #pragma unroll
         for (int k = 0; k < D; ++k) div += gv[k][k];
@@ -150,12 +149,13 @@ struct NavierStokes {
             for (int j = 0; j < D; ++j) tau[i][j] = mu * (gv[i][j] + gv[j][i]);
 #pragma unroll
         for (int i = 0; i < D; ++i) tau[i][i] -= 2.0 / 3.0 * mu * div;
+        const double iec = 1.0 / p.e_coeff, kq = mu * p.gamma / p.Pr;
 #pragma unroll
         for (int j = 0; j < D; ++j) {
-            double qj = gE[j] / p.e_coeff;
+            double qj = gE[j] * iec;
 #pragma unroll
             for (int k = 0; k < D; ++k) qj -= s.v[k] * gv[k][j];
-            q[j] = qj * (mu * p.gamma / p.Pr);
+            q[j] = qj * kq;
         }
     }
 
@@ -173,16 +173,16 @@ struct NavierStokes {
         if constexpr (FULL) {
             double tau[D][D], gE[D], q[D];
             viscous(p, s, g, viscosity(p, s.T), tau, gE, q);
-            const double ec = p.e_coeff, Re = p.Re;
+            const double iRe = 1.0 / p.Re, ecRe = p.e_coeff * iRe;
 #pragma unroll
             for (int j = 0; j < D; ++j) {
                 double ef = q[j];
 #pragma unroll
                 for (int i = 0; i < D; ++i) {
-                    F[(1 + i) * D + j] -= tau[i][j] / Re;
+                    F[(1 + i) * D + j] -= tau[i][j] * iRe;
                     ef += s.v[i] * tau[i][j];
                 }
-                F[(D + 1) * D + j] -= ef * ec / Re;
+                F[(D + 1) * D + j] -= ef * ecRe;
             }
         }
     }
@@ -238,16 +238,16 @@ struct NavierStokes {
             const double mu = viscosity(p, s.T);
             double tau[D][D], gE[D], q[D];
             viscous(p, s, g, mu, tau, gE, q);
-            const double ec = p.e_coeff, Re = p.Re;
+            const double iRe = 1.0 / p.Re, ecRe = p.e_coeff * iRe, kE = mu * p.gamma / p.Pr;
 #pragma unroll
             for (int j = 0; j < D; ++j) {
-                double ef = p.textbook_energy ? q[j] * n[j] : mu * p.gamma / p.Pr * gE[j] * n[j];
+                double ef = p.textbook_energy ? q[j] * n[j] : kE * gE[j] * n[j];
 #pragma unroll
                 for (int i = 0; i < D; ++i) {
-                    f[1 + i] += tau[i][j] / Re * n[j];
+                    f[1 + i] += tau[i][j] * iRe * n[j];
                     ef += s.v[i] * tau[i][j] * n[j];
                 }
-                f[D + 1] += ef * ec / Re;
+                f[D + 1] += ef * ecRe;
             }
         }
     }

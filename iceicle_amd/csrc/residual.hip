@@ -245,7 +245,7 @@ __device__ __forceinline__ void trace_eval(const DevSpace& S, const double* __re
         for (int h = 0; h < NH; ++h) hr[v][h] = 0;
     }
     constexpr int NB = Pow<N, D>::value;
-#pragma unroll 1
+#pragma unroll
     for (int i = 0; i < NB; ++i) {
         int id[D];
         {
@@ -390,13 +390,12 @@ __device__ __forceinline__ double face_weight(const DevSpace& S, int fq) {
 
 // ---- interior faces ------------------------------------------------------
 template <int D, int P, class PH>
-__global__ __launch_bounds__(256) void face_interior_kernel(DevSpace S, PhysParams ph, const double* __restrict__ U,
-                                                            double* __restrict__ Q, double* __restrict__ Pic) {
+__device__ __forceinline__ void face_interior(const DevSpace& S, const PhysParams& ph, const double* __restrict__ U,
+                                              double* __restrict__ Q, double* __restrict__ Pic, int tid) {
     constexpr int N = P + 1;
     constexpr int NB = Pow<N, D>::value;
     constexpr int NFQ = Pow<N, D - 1>::value;
     constexpr int M = PH::M;
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int f = tid / NFQ;
     const int q = tid - f * NFQ;
     if (f >= S.nfi) return;
@@ -463,14 +462,13 @@ __global__ __launch_bounds__(256) void face_interior_kernel(DevSpace S, PhysPara
 
 // ---- boundary faces ------------------------------------------------------
 template <int D, int P, class PH>
-__global__ __launch_bounds__(256) void face_boundary_kernel(DevSpace S, PhysParams ph, const ice_field* __restrict__ fields,
-                                                            const double* __restrict__ U, double* __restrict__ Q,
-                                                            double* __restrict__ Pic) {
+__device__ __forceinline__ void face_boundary(const DevSpace& S, const PhysParams& ph, const double* __restrict__ bval,
+                                              const double* __restrict__ U, double* __restrict__ Q,
+                                              double* __restrict__ Pic, int tid) {
     constexpr int N = P + 1;
     constexpr int NB = Pow<N, D>::value;
     constexpr int NFQ = Pow<N, D - 1>::value;
     constexpr int M = PH::M;
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int b = tid / NFQ;
     const int q = tid - b * NFQ;
     if (b >= S.nfb) return;
@@ -499,9 +497,11 @@ __global__ __launch_bounds__(256) void face_boundary_kernel(DevSpace S, PhysPara
     double x[D], nrm[D];
     const double sqrtg = face_geometry<D, N>(S, XL, fnl, q, x, nrm);
     const double ws = face_weight<N, D>(S, q) * sqrtg;
+    const double* bv = bval + ((size_t)b * NFQ + q) * M;
     if (bctype == ICE_BC_NEUMANN) {
         double g[M], fv[M];
-        ice_field_eval(&fields[ICE_MAX_BCFLAG + (bcflag & (ICE_MAX_BCFLAG - 1))], D, M, x, g);
+#pragma unroll
+        for (int v = 0; v < M; ++v) g[v] = bv[v];
         PH::neumann(ph, g, fv);
 #pragma unroll
         for (int v = 0; v < M; ++v) Qf[v] = fv[v] * ws;
@@ -518,8 +518,12 @@ __global__ __launch_bounds__(256) void face_boundary_kernel(DevSpace S, PhysPara
         PH::conv(ph, uL, uL, nrm, fadv);
         PH::diff(ph, uL, gL, nrm, fvis);
     } else {
-        if (bctype == ICE_BC_DIRICHLET) ice_field_eval(&fields[bcflag & (ICE_MAX_BCFLAG - 1)], D, M, x, uB);
-        else PH::apply_bc(ph, uL, gL, nrm, bctype, bcflag, uB);
+        if (bctype == ICE_BC_DIRICHLET) {
+#pragma unroll
+            for (int v = 0; v < M; ++v) uB[v] = bv[v];
+        } else {
+            PH::apply_bc(ph, uL, gL, nrm, bctype, bcflag, uB);
+        }
         double cL[D];
         centroid<D>(XL, cL);
         double h = 0;
@@ -557,6 +561,18 @@ __global__ __launch_bounds__(256) void face_boundary_kernel(DevSpace S, PhysPara
     for (int v = 0; v < M; ++v) Qf[v] = fvis[v] * ws - fadv[v] * ws;
 }
 
+// Interior and boundary faces in one launch: blocks [0, nbi) take interior
+// face points, the rest boundary face points.
+template <int D, int P, class PH>
+__global__ __launch_bounds__(256) void face_kernel(DevSpace S, PhysParams ph, DiscDev dd, const double* __restrict__ U,
+                                                   int nbi) {
+    if ((int)blockIdx.x < nbi) {
+        face_interior<D, P, PH>(S, ph, U, dd.Q, dd.Pic, blockIdx.x * blockDim.x + threadIdx.x);
+    } else {
+        face_boundary<D, P, PH>(S, ph, dd.bval, U, dd.Q, dd.Pic, (blockIdx.x - nbi) * blockDim.x + threadIdx.x);
+    }
+}
+
 // ---- elements: volume integral + face lifting -----------------------------
 template <int D, int P>
 struct ElemTile {
@@ -567,17 +583,15 @@ struct ElemTile {
 };
 
 template <int D, int P, class PH>
-__global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph, const ice_field* __restrict__ fields,
-                                                      int has_source, const double* __restrict__ U,
-                                                      const double* __restrict__ Q, const double* __restrict__ Pic,
-                                                      double* __restrict__ R) {
+__global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph, const double* __restrict__ srcload,
+                                                      const double* __restrict__ U, const double* __restrict__ Q,
+                                                      const double* __restrict__ Pic, double* __restrict__ R) {
     using T = ElemTile<D, P>;
     constexpr int N = T::N, NB = T::NB, EPB = T::EPB;
     constexpr int NFQ = Pow<N, D - 1>::value;
     constexpr int M = PH::M;
     __shared__ double sU[EPB * NB * M];
     __shared__ double sG[EPB * NB * M * D];
-    __shared__ double sS[EPB * NB * M];
     const int e0 = blockIdx.x * EPB;
     const int le = threadIdx.x / NB;
     const int k = threadIdx.x - le * NB;
@@ -623,7 +637,16 @@ __global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph,
 #pragma unroll
             for (int d = 0; d < D; ++d) gr[v][d] = 0;
         }
-#pragma unroll 1
+        // 1D basis values/derivatives at this point, one row per axis
+        double b0[D][N], b1[D][N];
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                b0[d][i] = L0[kd[d] * N + i];
+                b1[d][i] = L1[kd[d] * N + i];
+            }
+#pragma unroll
         for (int i = 0; i < NB; ++i) {
             int id[D];
             {
@@ -633,12 +656,12 @@ __global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph,
             }
             double c0 = 1.0, cg[D];
 #pragma unroll
-            for (int d = 0; d < D; ++d) c0 *= L0[kd[d] * N + id[d]];
+            for (int d = 0; d < D; ++d) c0 *= b0[d][id[d]];
 #pragma unroll
             for (int a = 0; a < D; ++a) {
                 double t = 1.0;
 #pragma unroll
-                for (int d = 0; d < D; ++d) t *= (d == a) ? L1[kd[d] * N + id[d]] : L0[kd[d] * N + id[d]];
+                for (int d = 0; d < D; ++d) t *= (d == a) ? b1[d][id[d]] : b0[d][id[d]];
                 cg[a] = t;
             }
 #pragma unroll
@@ -650,6 +673,7 @@ __global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph,
             }
         }
         double gx[M * D], F[M * D];
+        const double idetg = 1.0 / detg;
 #pragma unroll
         for (int v = 0; v < M; ++v)
 #pragma unroll
@@ -657,10 +681,10 @@ __global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph,
                 double s = 0;
 #pragma unroll
                 for (int a = 0; a < D; ++a) s += gr[v][a] * A[a][j];
-                gx[v * D + j] = s / detg;
+                gx[v * D + j] = s * idetg;
             }
         PH::flux(ph, u, gx, F);
-        const double fac = detc * wq / detg;
+        const double fac = detc * wq * idetg;
         double* Gq = sG + (le * NB + k) * M * D;
 #pragma unroll
         for (int v = 0; v < M; ++v)
@@ -671,16 +695,6 @@ __global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph,
                 for (int j = 0; j < D; ++j) s += F[v * D + j] * A[a][j];
                 Gq[v * D + a] = s * fac;
             }
-        double* Sq = sS + (le * NB + k) * M;
-        if (has_source) {
-            double src[M];
-            ice_field_eval(&fields[2 * ICE_MAX_BCFLAG], D, M, x, src);
-#pragma unroll
-            for (int v = 0; v < M; ++v) Sq[v] = src[v] * detc * wq;
-        } else {
-#pragma unroll
-            for (int v = 0; v < M; ++v) Sq[v] = 0.0;
-        }
     }
     __syncthreads();
     if (!active) return;
@@ -688,7 +702,16 @@ __global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph,
     double r[M];
 #pragma unroll
     for (int v = 0; v < M; ++v) r[v] = 0;
-#pragma unroll 1
+    // test function kd at the 1D points: one column per axis
+    double t0[D][N], t1[D][N];
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+            t0[d][q] = L0[q * N + kd[d]];
+            t1[d][q] = L1[q * N + kd[d]];
+        }
+#pragma unroll
     for (int q = 0; q < NB; ++q) {
         int qd[D];
         {
@@ -696,25 +719,27 @@ __global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph,
 #pragma unroll
             for (int d = D - 1; d >= 0; --d) { qd[d] = rr % N; rr /= N; }
         }
-        double c0 = 1.0, cg[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) c0 *= L0[qd[d] * N + kd[d]];
+        double cg[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) {
             double t = 1.0;
 #pragma unroll
-            for (int d = 0; d < D; ++d) t *= (d == a) ? L1[qd[d] * N + kd[d]] : L0[qd[d] * N + kd[d]];
+            for (int d = 0; d < D; ++d) t *= (d == a) ? t1[d][qd[d]] : t0[d][qd[d]];
             cg[a] = t;
         }
         const double* Gq = sG + (le * NB + q) * M * D;
-        const double* Sq = sS + (le * NB + q) * M;
 #pragma unroll
         for (int v = 0; v < M; ++v) {
             double s = 0;
 #pragma unroll
             for (int a = 0; a < D; ++a) s += Gq[v * D + a] * cg[a];
-            r[v] += s - Sq[v] * c0;
+            r[v] += s;
         }
+    }
+    if (srcload) {
+        const double* sl = srcload + ((size_t)e * NB + k) * M;
+#pragma unroll
+        for (int v = 0; v < M; ++v) r[v] -= sl[v];
     }
     // lift the face fluxes of the 2*D faces of this element
 #pragma unroll 1
@@ -781,58 +806,143 @@ __global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph,
     for (int v = 0; v < M; ++v) Re[v] = r[v];
 }
 
-// ---- launcher ------------------------------------------------------------
-template <int D, int P, class PH>
-int launch_residual(const DevSpace& S, const PhysParams& ph, const ice_field* fields, int has_source, bool ddgic,
-                    const double* U, double* Q, double* Pic, double* R, hipStream_t st, hipEvent_t* ev) {
+// ---- one-time tables ----------------------------------------------------
+// Boundary values at boundary face quadrature points: Dirichlet values
+// (dirichlet_callbacks[bcflag](x), conservation_law.hpp:683) or Neumann data
+// (neumann_callbacks[bcflag](x), :810); zero for other boundary types.
+template <int D, int P>
+__global__ __launch_bounds__(256) void bval_kernel(DevSpace S, const ice_field* __restrict__ fields, int M,
+                                                   double* __restrict__ bval) {
     constexpr int N = P + 1;
     constexpr int NFQ = Pow<N, D - 1>::value;
-    double* pic = (PH::HAS_HOMOGENEITY && ddgic) ? Pic : nullptr;
-    if (ev) (void)hipEventRecord(ev[0], st);
-    if (S.nfi > 0) {
-        const long nt = (long)S.nfi * NFQ;
-        face_interior_kernel<D, P, PH><<<(unsigned)((nt + 255) / 256), 256, 0, st>>>(S, ph, U, Q, pic);
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = tid / NFQ, q = tid - b * NFQ;
+    if (b >= S.nfb) return;
+    const int4 rec = S.fbdy[b];
+    const int bctype = info_bctype(rec.y), bcflag = rec.z & (ICE_MAX_BCFLAG - 1);
+    double* out = bval + ((size_t)b * NFQ + q) * M;
+    double X[1 << D][D], x[D], nrm[D];
+    load_vertices<D>(S.X, rec.x, X);
+    (void)face_geometry<D, N>(S, X, info_fnl(rec.y), q, x, nrm);
+    double v[8];
+    for (int k = 0; k < M; ++k) v[k] = 0.0;
+    if (bctype == ICE_BC_DIRICHLET) ice_field_eval(&fields[bcflag], D, M, x, v);
+    else if (bctype == ICE_BC_NEUMANN) ice_field_eval(&fields[ICE_MAX_BCFLAG + bcflag], D, M, x, v);
+    for (int k = 0; k < M; ++k) out[k] = v[k];
+}
+
+// Source load vector b[e][i][v] = sum_q s_v(x_q) phi_i(xi_q) max(0, detJ) w_q
+// (the user_source term of domain_integral, conservation_law.hpp:300-312).
+template <int D, int P>
+__global__ __launch_bounds__(256) void srcload_kernel(DevSpace S, const ice_field* __restrict__ fields, int M,
+                                                      double* __restrict__ out) {
+    constexpr int N = P + 1;
+    constexpr int NB = Pow<N, D>::value;
+    const int e = blockIdx.x;
+    const int i = threadIdx.x;
+    if (i >= NB) return;
+    const double* xs = S.tab;
+    const double* L0 = S.tab + (N + 2);
+    const double* wgl = L0 + 3 * (N + 2) * N;
+    double X[1 << D][D];
+    load_vertices<D>(S.X, e, X);
+    int id[D];
+    {
+        int r = i;
+        for (int d = D - 1; d >= 0; --d) { id[d] = r % N; r /= N; }
     }
-    if (ev) (void)hipEventRecord(ev[1], st);
-    if (S.nfb > 0) {
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int q = 0; q < NB; ++q) {
+        int qd[D];
+        int r = q;
+        for (int d = D - 1; d >= 0; --d) { qd[d] = r % N; r /= N; }
+        double xi[D], wq = 1.0, phi = 1.0;
+        for (int d = 0; d < D; ++d) {
+            xi[d] = xs[qd[d]];
+            wq = (d == 0) ? wgl[qd[d]] : wq * wgl[qd[d]];
+            phi *= L0[qd[d] * N + id[d]];
+        }
+        double x[D], J[D][D], Hm[D][D][D], A[D][D];
+        q1_map<D>(X, xi, x, J, Hm);
+        const double detc = fmax(0.0, det_adj<D>(J, A));
+        double src[8];
+        ice_field_eval(&fields[2 * ICE_MAX_BCFLAG], D, M, x, src);
+        for (int v = 0; v < M; ++v) acc[v] += src[v] * phi * detc * wq;
+    }
+    for (int v = 0; v < M; ++v) out[((size_t)e * NB + i) * M + v] = acc[v];
+}
+
+template <int D, int P>
+int launch_setup(const DevSpace& S, const ice_field* fields, int M, double* bval, double* srcload, hipStream_t st) {
+    constexpr int N = P + 1;
+    constexpr int NB = Pow<N, D>::value;
+    constexpr int NFQ = Pow<N, D - 1>::value;
+    if (bval && S.nfb > 0) {
         const long nt = (long)S.nfb * NFQ;
-        face_boundary_kernel<D, P, PH><<<(unsigned)((nt + 255) / 256), 256, 0, st>>>(S, ph, fields, U, Q, pic);
+        bval_kernel<D, P><<<(unsigned)((nt + 255) / 256), 256, 0, st>>>(S, fields, M, bval);
     }
+    if (srcload) srcload_kernel<D, P><<<S.nelem, ((NB + 63) / 64) * 64, 0, st>>>(S, fields, M, srcload);
+    return (int)hipGetLastError();
+}
+
+int setup_disc_tables(int ndim, int p, int neq, const DevSpace& S, const ice_field* fields, double* bval,
+                      double* srcload, hipStream_t st) {
+#define ICE_SETUP_CASE(DD, PP) \
+    if (ndim == DD && p == PP) return launch_setup<DD, PP>(S, fields, neq, bval, srcload, st);
+    ICE_SETUP_CASE(1, 1) ICE_SETUP_CASE(1, 2) ICE_SETUP_CASE(1, 3) ICE_SETUP_CASE(1, 4)
+    ICE_SETUP_CASE(2, 1) ICE_SETUP_CASE(2, 2) ICE_SETUP_CASE(2, 3) ICE_SETUP_CASE(2, 4)
+    ICE_SETUP_CASE(3, 1) ICE_SETUP_CASE(3, 2) ICE_SETUP_CASE(3, 3) ICE_SETUP_CASE(3, 4)
+#undef ICE_SETUP_CASE
+    return -1;
+}
+
+// ---- launcher ------------------------------------------------------------
+template <int D, int P, class PH>
+int launch_residual(const DevSpace& S, const PhysParams& ph, const DiscDev& dd, const double* U, double* R,
+                    hipStream_t st, hipEvent_t* ev) {
+    constexpr int N = P + 1;
+    constexpr int NFQ = Pow<N, D - 1>::value;
+    DiscDev d = dd;
+    if (!PH::HAS_HOMOGENEITY) d.Pic = nullptr;
+    if (ev) (void)hipEventRecord(ev[0], st);
+    const int nbi = (int)(((long)S.nfi * NFQ + 255) / 256);
+    const int nbb = (int)(((long)S.nfb * NFQ + 255) / 256);
+    if (nbi + nbb > 0) face_kernel<D, P, PH><<<nbi + nbb, 256, 0, st>>>(S, ph, d, U, nbi);
+    if (ev) (void)hipEventRecord(ev[1], st);
     if (ev) (void)hipEventRecord(ev[2], st);
     using T = ElemTile<D, P>;
     const int nblk = (S.nelem + T::EPB - 1) / T::EPB;
-    element_kernel<D, P, PH><<<nblk, 256, 0, st>>>(S, ph, fields, has_source, U, Q, pic, R);
+    element_kernel<D, P, PH><<<nblk, 256, 0, st>>>(S, ph, d.srcload, U, d.Q, d.Pic, R);
     if (ev) (void)hipEventRecord(ev[3], st);
     return (int)hipGetLastError();
 }
 
 template <class PH, int D>
-int dispatch_order(int p, const DevSpace& S, const PhysParams& ph, const ice_field* fields, int has_source, bool ddgic,
-                   const double* U, double* Q, double* Pic, double* R, hipStream_t st, hipEvent_t* ev) {
+int dispatch_order(int p, const DevSpace& S, const PhysParams& ph, const DiscDev& dd, const double* U, double* R,
+                   hipStream_t st, hipEvent_t* ev) {
     switch (p) {
-    case 1: return launch_residual<D, 1, PH>(S, ph, fields, has_source, ddgic, U, Q, Pic, R, st, ev);
-    case 2: return launch_residual<D, 2, PH>(S, ph, fields, has_source, ddgic, U, Q, Pic, R, st, ev);
-    case 3: return launch_residual<D, 3, PH>(S, ph, fields, has_source, ddgic, U, Q, Pic, R, st, ev);
-    case 4: return launch_residual<D, 4, PH>(S, ph, fields, has_source, ddgic, U, Q, Pic, R, st, ev);
+    case 1: return launch_residual<D, 1, PH>(S, ph, dd, U, R, st, ev);
+    case 2: return launch_residual<D, 2, PH>(S, ph, dd, U, R, st, ev);
+    case 3: return launch_residual<D, 3, PH>(S, ph, dd, U, R, st, ev);
+    case 4: return launch_residual<D, 4, PH>(S, ph, dd, U, R, st, ev);
     default: return -1;
     }
 }
 
-int residual_fp64(int ndim, int p, int flux, const DevSpace& S, const PhysParams& ph, const ice_field* fields,
-                  int has_source, bool ddgic, const double* U, double* Q, double* Pic, double* R, hipStream_t st,
-                  hipEvent_t* ev) {
+int residual_fp64(int ndim, int p, int flux, const DevSpace& S, const PhysParams& ph, const DiscDev& dd,
+                  const double* U, double* R, hipStream_t st, hipEvent_t* ev) {
     if (flux == ICE_FLUX_BURGERS) {
-        if (ndim == 1) return dispatch_order<Burgers<1>, 1>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st, ev);
-        if (ndim == 2) return dispatch_order<Burgers<2>, 2>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st, ev);
-        if (ndim == 3) return dispatch_order<Burgers<3>, 3>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st, ev);
+        if (ndim == 1) return dispatch_order<Burgers<1>, 1>(p, S, ph, dd, U, R, st, ev);
+        if (ndim == 2) return dispatch_order<Burgers<2>, 2>(p, S, ph, dd, U, R, st, ev);
+        if (ndim == 3) return dispatch_order<Burgers<3>, 3>(p, S, ph, dd, U, R, st, ev);
     } else if (flux == ICE_FLUX_NAVIER_STOKES) {
-        if (ndim == 1) return dispatch_order<NavierStokes<1, true>, 1>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st, ev);
-        if (ndim == 2) return dispatch_order<NavierStokes<2, true>, 2>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st, ev);
-        if (ndim == 3) return dispatch_order<NavierStokes<3, true>, 3>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st, ev);
+        if (ndim == 1) return dispatch_order<NavierStokes<1, true>, 1>(p, S, ph, dd, U, R, st, ev);
+        if (ndim == 2) return dispatch_order<NavierStokes<2, true>, 2>(p, S, ph, dd, U, R, st, ev);
+        if (ndim == 3) return dispatch_order<NavierStokes<3, true>, 3>(p, S, ph, dd, U, R, st, ev);
     } else if (flux == ICE_FLUX_EULER) {
-        if (ndim == 1) return dispatch_order<NavierStokes<1, false>, 1>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st, ev);
-        if (ndim == 2) return dispatch_order<NavierStokes<2, false>, 2>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st, ev);
-        if (ndim == 3) return dispatch_order<NavierStokes<3, false>, 3>(p, S, ph, fields, has_source, ddgic, U, Q, Pic, R, st, ev);
+        if (ndim == 1) return dispatch_order<NavierStokes<1, false>, 1>(p, S, ph, dd, U, R, st, ev);
+        if (ndim == 2) return dispatch_order<NavierStokes<2, false>, 2>(p, S, ph, dd, U, R, st, ev);
+        if (ndim == 3) return dispatch_order<NavierStokes<3, false>, 3>(p, S, ph, dd, U, R, st, ev);
     }
     return -1;
 }

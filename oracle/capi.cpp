@@ -279,6 +279,15 @@ int ice_oracle_element_jacobian(int ndim, const double* X, const double* xi, dou
     else return -1;
     return 0;
 }
+// trace coordinates -> element reference point for face `face_nr` seen with
+// `orient` (orient_transform then trace_to_ref), for arbitrary s
+int ice_oracle_trace_ref(int ndim, int face_nr, int orient, const double* s, double* xi) {
+    double sr[3] = {0, 0, 0};
+    if (ndim == 1) { trace_to_ref<1>(face_nr, s, xi); return 0; }
+    if (ndim == 2) { orient_transform<2>(orient, s, sr); trace_to_ref<2>(face_nr, sr, xi); return 0; }
+    if (ndim == 3) { orient_transform<3>(orient, s, sr); trace_to_ref<3>(face_nr, sr, xi); return 0; }
+    return -1;
+}
 // NS physical flux (zero gradient allowed) and VanLeer, 1..3D, for the
 // known answers of test_ns.cpp:66-107
 int ice_oracle_ns_flux(const ice_disc_params* p, int ndim, const double* u, const double* gradu, double* F) {

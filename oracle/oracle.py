@@ -48,6 +48,7 @@ def lib():
             ("ice_oracle_element_jacobian", [C.c_int, dp, dp, dp]),
             ("ice_oracle_ns_flux", [C.POINTER(abi.DiscParams), C.c_int, dp, dp, dp]),
             ("ice_oracle_vanleer", [C.POINTER(abi.DiscParams), C.c_int, dp, dp, dp, dp]),
+            ("ice_oracle_trace_ref", [C.c_int, C.c_int, C.c_int, dp, dp]),
         ]:
             fn = getattr(L, name)
             fn.argtypes = args
@@ -189,6 +190,15 @@ def element_jacobian(ndim, X, xi):
     J = np.zeros(ndim * ndim)
     _check(lib().ice_oracle_element_jacobian(ndim, _dp(X), _dp(xi), _dp(J)))
     return J.reshape(ndim, ndim)
+
+
+def trace_ref(ndim, face_nr, orient, s):
+    s = np.ascontiguousarray(np.atleast_1d(s), dtype=np.float64)
+    if s.size == 0:
+        s = np.zeros(1)
+    xi = np.zeros(ndim)
+    _check(lib().ice_oracle_trace_ref(ndim, face_nr, orient, _dp(s), _dp(xi)))
+    return xi
 
 
 def ns_flux(params, ndim, u, gradu):
