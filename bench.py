@@ -105,13 +105,15 @@ def main():
     E, nfi, nfb = space.nelem, info.nface_interior, info.nface_boundary
     nvert = 2 ** D
     kernels = {
-        "face_interior": (stage_ms[0] / ncalls,
-                          8 * E * nb * neq + 8 * E * nvert * D + 16 * nfi + 8 * nfi * nfq * neq),
-        "face_boundary": (stage_ms[1] / ncalls,
-                          8 * nfb * nb * neq + 8 * nfb * nvert * D + 16 * nfb + 8 * nfb * nfq * neq),
+        # interior + boundary faces: states and vertices of every element once,
+        # face records, boundary values, weighted normal fluxes written
+        "faces": (stage_ms[0] / ncalls,
+                  8 * E * nb * neq + 8 * E * nvert * D + 16 * (nfi + nfb) + 8 * nfb * nfq * neq
+                  + 8 * (nfi + nfb) * nfq * neq),
+        # elements: state, vertices, face links, face fluxes and source load read, residual written
         "element": (stage_ms[2] / ncalls,
                     8 * E * nb * neq + 8 * E * nvert * D + 4 * 2 * D * E + 8 * (nfi + nfb) * nfq * neq
-                    + 8 * E * nb * neq),
+                    + 8 * E * nb * neq + 8 * E * nb * neq),
     }
     dom = max(kernels, key=lambda k: kernels[k][0])
     kms, kbytes = kernels[dom]

@@ -99,6 +99,29 @@ class DiscParams(C.Structure):
     ]
 
 
+STEP_RK3_SSP = 0
+STEP_RK3_TVD = 1
+STEP_EXPLICIT_EULER = 2
+
+
+class StepperDesc(C.Structure):
+    _fields_ = [("kind", C.c_int), ("reference_compat", C.c_int), ("use_cfl", C.c_int), ("dt", C.c_double),
+                ("cfl", C.c_double), ("use_tfinal", C.c_int), ("tfinal", C.c_double), ("ntime", C.c_longlong)]
+
+
+def stepper_desc(kind=STEP_RK3_SSP, reference_compat=False, dt=None, cfl=None, tfinal=None, ntime=0):
+    s = StepperDesc()
+    s.kind = kind
+    s.reference_compat = int(reference_compat)
+    s.use_cfl = int(cfl is not None)
+    s.dt = dt or 0.0
+    s.cfl = cfl or 0.0
+    s.use_tfinal = int(tfinal is not None)
+    s.tfinal = tfinal or 0.0
+    s.ntime = ntime
+    return s
+
+
 def default_params(flux):
     """Reference defaults (burgers.hpp:14-40, navier_stokes.hpp:129-213, :321-340)."""
     p = DiscParams()

@@ -141,10 +141,11 @@ struct ice_disc {
     double* d_Pic = nullptr;
     double* d_bval = nullptr;
     double* d_srcload = nullptr;
+    unsigned long long* d_wave = nullptr;
     ice::DiscDev dd{};
     ~ice_disc() {
         for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
-        for (void* p : {(void*)d_fields, (void*)d_Q, (void*)d_Pic, (void*)d_bval, (void*)d_srcload})
+        for (void* p : {(void*)d_fields, (void*)d_Q, (void*)d_Pic, (void*)d_bval, (void*)d_srcload, (void*)d_wave})
             if (p) (void)hipFree(p);
     }
 };
@@ -245,7 +246,7 @@ int ice_space_create(const ice_mesh_desc* md, const ice_space_desc* sd, ice_spac
         double gx[8], gw[8];
         gauss_legendre(n, gx, gw);
         const int ns = n + 2;
-        s->tab.assign(ns + 3 * ns * n + n, 0.0);
+        s->tab.assign(ns + 3 * ns * n + n + n * n, 0.0);
         double* xs = s->tab.data();
         double* L0 = xs + ns;
         double* L1 = L0 + ns * n;
@@ -255,6 +256,30 @@ int ice_space_create(const ice_mesh_desc* md, const ice_space_desc* sd, ice_spac
         xs[n] = -1.0;
         xs[n + 1] = 1.0;
         for (int k = 0; k < ns; ++k) lagrange(p, xs[k], L0 + k * n, L1 + k * n, L2 + k * n);
+        // Binv = B1^-1 with B1[q][i] = l_i(x_q): the mass matrix of any element is
+        // B^T W B with B = B1 x ... x B1 square, so M^-1 = B^-1 W^-1 B^-T
+        {
+            double* Binv = wg + n;
+            std::vector<double> a(L0, L0 + n * n), inv(n * n, 0.0);
+            for (int i = 0; i < n; ++i) inv[i * n + i] = 1.0;
+            for (int c = 0; c < n; ++c) {  // Gauss-Jordan with partial pivoting
+                int piv = c;
+                for (int r = c + 1; r < n; ++r)
+                    if (std::abs(a[r * n + c]) > std::abs(a[piv * n + c])) piv = r;
+                for (int j = 0; j < n; ++j) {
+                    std::swap(a[c * n + j], a[piv * n + j]);
+                    std::swap(inv[c * n + j], inv[piv * n + j]);
+                }
+                const double d = a[c * n + c];
+                for (int j = 0; j < n; ++j) { a[c * n + j] /= d; inv[c * n + j] /= d; }
+                for (int r = 0; r < n; ++r) {
+                    if (r == c) continue;
+                    const double f = a[r * n + c];
+                    for (int j = 0; j < n; ++j) { a[r * n + j] -= f * a[c * n + j]; inv[r * n + j] -= f * inv[c * n + j]; }
+                }
+            }
+            for (int k = 0; k < n * n; ++k) Binv[k] = inv[k];  // Binv[i][q]
+        }
         // trace point and face vertex tables
         const int no = ice::n_orientations(D);
         s->fpt.resize((size_t)2 * D * no * s->nfq * D);
@@ -461,7 +486,10 @@ int ice_disc_create(ice_space* s, const ice_disc_params* p, ice_disc** out) {
         if (d->has_source) ICE_HIP(hipMalloc(&d->d_srcload, (size_t)s->mesh.nelem * s->nb * d->neq * sizeof(double)));
         if (ice::setup_disc_tables(D, s->p, d->neq, s->dev, d->d_fields, d->d_bval, d->d_srcload, nullptr) != 0)
             throw ice::Error{ICE_ERR_HIP, "table setup launch failed"};
+        ICE_HIP(hipMalloc(&d->d_wave, 4 * sizeof(unsigned long long)));
+        ICE_HIP(hipMemset(d->d_wave, 0, 4 * sizeof(unsigned long long)));
         ICE_HIP(hipDeviceSynchronize());
+        d->dd.wave = d->d_wave;
         d->dd.fields = d->d_fields;
         d->dd.bval = d->d_bval;
         d->dd.srcload = d->d_srcload;
@@ -486,6 +514,61 @@ int ice_residual(ice_disc* d, const double* u, double* r, void* stream) {
     if (d->profiling && d->prof_n < d->prof_max) ev = &d->prof_ev[(size_t)4 * d->prof_n++];
     const int rc = ice::residual_fp64(s->ndim, s->p, d->prm.flux, s->dev, d->ph, d->dd, u, r, (hipStream_t)stream, ev);
     if (rc != 0) return fail(ICE_ERR_HIP, std::string("residual launch failed: ") + hipGetErrorString((hipError_t)rc));
+    return ICE_OK;
+}
+
+int ice_mass_inverse_apply(ice_disc* d, const double* r, double* out, void* stream) {
+    if (!d || !r || !out) return fail(ICE_ERR_ARG, "null argument");
+    const ice_space* s = d->space;
+    const int rc = ice::launch_minv(s->ndim, s->p, d->neq, s->dev, r, out, (hipStream_t)stream);
+    if (rc != 0) return fail(ICE_ERR_HIP, "mass inverse launch failed");
+    return ICE_OK;
+}
+
+int ice_project(ice_disc* d, const ice_field* f, double* u, void* stream) {
+    if (!d || !f || !u) return fail(ICE_ERR_ARG, "null argument");
+    const ice_space* s = d->space;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t n = (size_t)s->mesh.nelem * s->nb * d->neq;
+    double* b = nullptr;
+    ice_field* df = nullptr;
+    int rc = 0;
+    if (hipMalloc(&b, n * sizeof(double)) != hipSuccess || hipMalloc(&df, sizeof(ice_field)) != hipSuccess) {
+        if (b) (void)hipFree(b);
+        return fail(ICE_ERR_ALLOC, "projection scratch allocation failed");
+    }
+    rc |= (int)hipMemcpyAsync(df, f, sizeof(ice_field), hipMemcpyHostToDevice, st);
+    rc |= ice::projection_rhs(s->ndim, s->p, d->neq, s->dev, df, b, st);
+    rc |= ice::launch_minv(s->ndim, s->p, d->neq, s->dev, b, u, st);
+    rc |= (int)hipStreamSynchronize(st);
+    (void)hipFree(b);
+    (void)hipFree(df);
+    if (rc != 0) return fail(ICE_ERR_HIP, "projection failed");
+    return ICE_OK;
+}
+
+int ice_l2_error(ice_disc* d, const ice_field* f, const double* u, double* err, void* stream) {
+    if (!d || !f || !u || !err) return fail(ICE_ERR_ARG, "null argument");
+    const ice_space* s = d->space;
+    hipStream_t st = (hipStream_t)stream;
+    double* terms = nullptr;
+    ice_field* df = nullptr;
+    if (hipMalloc(&terms, (size_t)s->mesh.nelem * sizeof(double)) != hipSuccess ||
+        hipMalloc(&df, sizeof(ice_field)) != hipSuccess) {
+        if (terms) (void)hipFree(terms);
+        return fail(ICE_ERR_ALLOC, "l2 scratch allocation failed");
+    }
+    int rc = (int)hipMemcpyAsync(df, f, sizeof(ice_field), hipMemcpyHostToDevice, st);
+    rc |= ice::l2_error_terms(s->ndim, s->p, d->neq, s->dev, df, u, terms, st);
+    std::vector<double> h(s->mesh.nelem);
+    rc |= (int)hipMemcpyAsync(h.data(), terms, h.size() * sizeof(double), hipMemcpyDeviceToHost, st);
+    rc |= (int)hipStreamSynchronize(st);
+    (void)hipFree(terms);
+    (void)hipFree(df);
+    if (rc != 0) return fail(ICE_ERR_HIP, "l2 error failed");
+    double acc = 0.0;
+    for (double v : h) acc += v;
+    *err = std::sqrt(acc);
     return ICE_OK;
 }
 
@@ -514,11 +597,11 @@ int ice_profile_end(ice_disc* d, double* stage_ms, int* ncalls) {
     for (int c = 0; c < d->prof_n; ++c) {
         hipEvent_t* ev = &d->prof_ev[(size_t)4 * c];
         if (hipEventSynchronize(ev[3]) != hipSuccess) return fail(ICE_ERR_HIP, "event synchronize failed");
-        for (int k = 0; k < 3; ++k) {
-            float ms = 0.f;
-            if (hipEventElapsedTime(&ms, ev[k], ev[k + 1]) != hipSuccess) return fail(ICE_ERR_HIP, "event timing failed");
-            stage_ms[k] += ms;
-        }
+        float ms0 = 0.f, ms2 = 0.f;
+        if (hipEventElapsedTime(&ms0, ev[0], ev[1]) != hipSuccess || hipEventElapsedTime(&ms2, ev[1], ev[3]) != hipSuccess)
+            return fail(ICE_ERR_HIP, "event timing failed");
+        stage_ms[0] += ms0;
+        stage_ms[2] += ms2;
     }
     *ncalls = d->prof_n;
     return ICE_OK;
@@ -572,6 +655,198 @@ int ice_trace_points(int ndim, int order, int face_nr, int orientation, double* 
         for (int d = 0; d < ndim; ++d) xi[q * ndim + d] = (pt[d] < n) ? gx[pt[d]] : (pt[d] == n ? -1.0 : 1.0);
     }
     return ICE_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ explicit steppers
+struct ice_stepper {
+    ice_disc* disc = nullptr;
+    ice_stepper_desc desc{};
+    ice::StepControl ctl{};
+    size_t n = 0;
+    double* buf[5] = {};      // res, k1, k2, k3, ustage / uold
+    ice::StepState* st = nullptr;
+    ~ice_stepper() {
+        for (double* b : buf)
+            if (b) (void)hipFree(b);
+        if (st) (void)hipFree(st);
+    }
+};
+
+namespace {
+
+int step_once(ice_stepper* S, double* u, hipStream_t st) {
+    ice_disc* d = S->disc;
+    const ice_space* sp = d->space;
+    const size_t n = S->n;
+    double *res = S->buf[0], *k1 = S->buf[1], *k2 = S->buf[2], *k3 = S->buf[3], *us = S->buf[4];
+    const double* dt = &S->st->dt;
+    int rc = 0;
+    auto R = [&](const double* x) {
+        rc |= ice::residual_fp64(sp->ndim, sp->p, d->prm.flux, sp->dev, d->ph, d->dd, x, res, st, nullptr);
+    };
+    auto Minv = [&](double* k) { rc |= ice::launch_minv(sp->ndim, sp->p, d->neq, sp->dev, res, k, st); };
+    auto copy = [&](const double* x, double* y) {
+        rc |= (int)hipMemcpyAsync(y, x, n * sizeof(double), hipMemcpyDeviceToDevice, st);
+    };
+    rc |= ice::launch_step_begin(S->st, d->d_wave, S->ctl, st);
+    switch (S->desc.kind) {
+    case ICE_STEP_RK3_SSP:
+        if (S->desc.reference_compat) {
+            // ssp_rk3.hpp:132-190 with every stage residual evaluated at u^n
+            R(u);
+            Minv(k1);
+            rc |= ice::launch_axpy_dt(n, 1.0 / 6.0, dt, k1, u, st);
+            rc |= ice::launch_axpy_dt(n, 1.0 / 6.0, dt, k1, u, st);
+            rc |= ice::launch_axpy_dt(n, 2.0 / 3.0, dt, k1, u, st);
+        } else {
+            R(u);
+            Minv(k1);
+            copy(u, us);
+            rc |= ice::launch_axpy_dt(n, 1.0, dt, k1, us, st);
+            R(us);
+            Minv(k2);
+            copy(u, us);
+            rc |= ice::launch_axpy_dt(n, 0.25, dt, k1, us, st);
+            rc |= ice::launch_axpy_dt(n, 0.25, dt, k2, us, st);
+            R(us);
+            Minv(k3);
+            rc |= ice::launch_axpy_dt(n, 1.0 / 6.0, dt, k1, u, st);
+            rc |= ice::launch_axpy_dt(n, 1.0 / 6.0, dt, k2, u, st);
+            rc |= ice::launch_axpy_dt(n, 2.0 / 3.0, dt, k3, u, st);
+        }
+        break;
+    case ICE_STEP_RK3_TVD: {
+        // tvd_rk3.hpp:117-226: Shu-Osher stages in place
+        static const double alfa[3] = {0.0, 0.75, 1.0 / 3.0};
+        static const double beta[3] = {1.0, 0.25, 2.0 / 3.0};
+        copy(u, us);
+        for (int s = 0; s < 3; ++s) {
+            R(u);
+            Minv(k1);
+            rc |= ice::launch_axpby(n, alfa[s], us, beta[s], u, st);
+            rc |= ice::launch_axpy_dt(n, beta[s], dt, k1, u, st);
+        }
+        break;
+    }
+    default:  // ExplicitEuler (explicit_euler.hpp:87-138)
+        R(u);
+        Minv(k1);
+        rc |= ice::launch_axpy_dt(n, 1.0, dt, k1, u, st);
+        break;
+    }
+    rc |= ice::launch_step_end(S->st, st);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ice_stepper_create(ice_disc* d, const ice_stepper_desc* desc, ice_stepper** out) {
+    if (!d || !desc || !out) return fail(ICE_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (desc->kind < ICE_STEP_RK3_SSP || desc->kind > ICE_STEP_EXPLICIT_EULER) return fail(ICE_ERR_ARG, "bad stepper kind");
+    if (!desc->use_cfl && !(desc->dt > 0)) return fail(ICE_ERR_ARG, "fixed time step must be positive");
+    try {
+        auto S = std::make_unique<ice_stepper>();
+        S->disc = d;
+        S->desc = *desc;
+        const ice_space* sp = d->space;
+        S->n = (size_t)sp->mesh.nelem * sp->nb * d->neq;
+        for (double*& b : S->buf) ICE_HIP(hipMalloc(&b, S->n * sizeof(double)));
+        ICE_HIP(hipMalloc(&S->st, sizeof(ice::StepState)));
+        ice::StepState zero{0.0, 0.0, 0, 0};
+        ICE_HIP(hipMemcpy(S->st, &zero, sizeof(zero), hipMemcpyHostToDevice));
+        // reference length: min over elements of det(J(centroid))^(1/ndim) (explicit_utils.hpp:70-83)
+        const int D = sp->ndim, NV = 1 << D;
+        double reflen = 1e8;
+        for (int e = 0; e < sp->mesh.nelem; ++e) {
+            double J[3][3] = {};
+            for (int k = 0; k < NV; ++k)
+                for (int a = 0; a < D; ++a) {
+                    double t = 1.0;
+                    for (int dd = 0; dd < D; ++dd) {
+                        const int bit = (k >> (D - 1 - dd)) & 1;
+                        t *= (dd == a) ? (bit ? 0.5 : -0.5) : 0.5;
+                    }
+                    for (int i = 0; i < D; ++i) J[i][a] += t * sp->X[((size_t)e * NV + k) * D + i];
+                }
+            double det = (D == 1) ? J[0][0]
+                         : (D == 2) ? J[0][0] * J[1][1] - J[0][1] * J[1][0]
+                                    : J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                                          J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                                          J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+            reflen = std::min(reflen, std::pow(det, 1.0 / D));
+        }
+        ice::StepControl& c = S->ctl;
+        c.use_cfl = desc->use_cfl;
+        c.dt_fixed = desc->dt;
+        c.cfl = desc->cfl;
+        c.reflen = reflen;
+        c.mu = d->prm.mu;
+        c.pmax = std::max(1, sp->p);
+        c.flux = d->prm.flux;
+        c.use_tfinal = desc->use_tfinal;
+        c.tfinal = desc->tfinal;
+        c.ntime = desc->ntime;
+        *out = S.release();
+        return ICE_OK;
+    } catch (const ice::Error& e) {
+        return fail(e.code, e.msg);
+    }
+}
+
+int ice_stepper_destroy(ice_stepper* s) {
+    delete s;
+    return ICE_OK;
+}
+
+int ice_stepper_step(ice_stepper* s, double* u, void* stream) {
+    if (!s || !u) return fail(ICE_ERR_ARG, "null argument");
+    if (step_once(s, u, (hipStream_t)stream) != 0) return fail(ICE_ERR_HIP, "step launch failed");
+    return ICE_OK;
+}
+
+int ice_stepper_status(ice_stepper* s, double* time, double* dt, long long* itime, int* done) {
+    if (!s) return fail(ICE_ERR_ARG, "null argument");
+    ice::StepState h{};
+    if (hipMemcpy(&h, s->st, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess) return fail(ICE_ERR_HIP, "status copy failed");
+    if (time) *time = h.time;
+    if (dt) *dt = h.dt;
+    if (itime) *itime = h.itime;
+    if (done) {
+        const ice::StepControl& c = s->ctl;
+        *done = c.use_tfinal ? (h.time >= c.tfinal) : (h.itime >= c.ntime);
+    }
+    return ICE_OK;
+}
+
+int ice_stepper_solve(ice_stepper* s, double* u, void* stream, long long* nsteps, double* time) {
+    if (!s || !u) return fail(ICE_ERR_ARG, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    ice_disc* d = s->disc;
+    const ice_space* sp = d->space;
+    // RK3SSP::solve / RK3TVD::solve: one residual first for the wave speeds
+    if (ice::residual_fp64(sp->ndim, sp->p, d->prm.flux, sp->dev, d->ph, d->dd, u, s->buf[0], st, nullptr) != 0)
+        return fail(ICE_ERR_HIP, "residual launch failed");
+    const long long chunk = s->ctl.use_tfinal ? 32 : std::max<long long>(1, s->ctl.ntime);
+    for (;;) {
+        for (long long k = 0; k < chunk; ++k)
+            if (step_once(s, u, st) != 0) return fail(ICE_ERR_HIP, "step launch failed");
+        if (hipStreamSynchronize(st) != hipSuccess) return fail(ICE_ERR_HIP, "stream synchronize failed");
+        int done = 0;
+        long long it = 0;
+        double t = 0;
+        int rc = ice_stepper_status(s, &t, nullptr, &it, &done);
+        if (rc != ICE_OK) return rc;
+        if (done || (!s->ctl.use_tfinal)) {
+            if (nsteps) *nsteps = it;
+            if (time) *time = t;
+            return ICE_OK;
+        }
+    }
 }
 
 }  // extern "C"

@@ -38,7 +38,42 @@ struct DiscDev {
     const double* srcload;    // [nelem][nbasis][neq] source load vector, or null
     double* Q;                // [nfi + nfb][nfq][neq] weighted normal face fluxes
     double* Pic;              // [nfi + nfb][nfq][neq][ndim] DDGIC lifting vectors, or null
+    unsigned long long* wave; // wave-speed bookkeeping (bits of non-negative doubles), or null:
+                              // [0] running max |dF/du| (Burgers lambda_max, burgers.hpp:89)
+                              // [1] value at the last quadrature point (NS lambda_max, navier_stokes.hpp:930)
+                              // [2] running max viscosity (NS visc_max, :964-966)
 };
+
+// Device-side state of an explicit time integration.
+struct StepState {
+    double dt;
+    double time;
+    long long itime;
+    int done;
+};
+
+// Time-step and termination settings (FixedTimestep / CFLTimestep,
+// TimestepTermination / TfinalTermination).
+struct StepControl {
+    int use_cfl;
+    double dt_fixed, cfl, reflen, mu;
+    int pmax, flux;
+    int use_tfinal;
+    double tfinal;
+    long long ntime;
+};
+
+// Projection right-hand side b = sum_q f(x_q) phi_i w detJ (projection.hpp:83-110)
+int projection_rhs(int ndim, int p, int neq, const DevSpace& S, const ice_field* f, double* b, hipStream_t st);
+// Per-element squared L2 error terms (l2_error.hpp:37-94)
+int l2_error_terms(int ndim, int p, int neq, const DevSpace& S, const ice_field* f, const double* U, double* out,
+                   hipStream_t st);
+
+int launch_minv(int ndim, int p, int neq, const DevSpace& S, const double* r, double* k, hipStream_t st);
+int launch_axpy_dt(size_t n, double coef, const double* dtp, const double* x, double* y, hipStream_t st);
+int launch_axpby(size_t n, double a, const double* x, double b, double* y, hipStream_t st);
+int launch_step_begin(StepState* s, unsigned long long* wave, const StepControl& c, hipStream_t st);
+int launch_step_end(StepState* s, hipStream_t st);
 
 int residual_fp64(int ndim, int p, int flux, const DevSpace& S, const PhysParams& ph, const DiscDev& dd,
                   const double* U, double* R, hipStream_t st, hipEvent_t* ev = nullptr);

@@ -57,6 +57,7 @@ struct Burgers {
         }
         return sqrt(s);
     }
+    __device__ __forceinline__ static double visc_at(const PhysParams& p, const double*) { return p.mu; }
     __device__ __forceinline__ static void conv(const PhysParams& p, const double* uL, const double* uR,
                                                 const double* n, double* out) {
         const double ua = 0.5 * (uL[0] + uR[0]);
@@ -127,6 +128,9 @@ struct NavierStokes {
     __device__ __forceinline__ static double wave_speed(const PhysParams& p, const double* u) {
         const State s = state(p, u);
         return s.c + sqrt(s.vv);
+    }
+    __device__ __forceinline__ static double visc_at(const PhysParams& p, const double* u) {
+        return FULL ? viscosity(p, state(p, u).T) : 0.0;
     }
 
     // velocity and total-energy gradients (calc_state_gradients<CONSERVATIVE>, :541-575),

@@ -585,7 +585,8 @@ struct ElemTile {
 template <int D, int P, class PH>
 __global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph, const double* __restrict__ srcload,
                                                       const double* __restrict__ U, const double* __restrict__ Q,
-                                                      const double* __restrict__ Pic, double* __restrict__ R) {
+                                                      const double* __restrict__ Pic, double* __restrict__ R,
+                                                      unsigned long long* __restrict__ wave) {
     using T = ElemTile<D, P>;
     constexpr int N = T::N, NB = T::NB, EPB = T::EPB;
     constexpr int NFQ = Pow<N, D - 1>::value;
@@ -615,6 +616,7 @@ __global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph,
 #pragma unroll
         for (int d = D - 1; d >= 0; --d) { kd[d] = r % N; r /= N; }
     }
+    double lam_w = 0.0, vis_w = 0.0;
     if (active) {
         load_vertices<D>(S.X, e, X);
         // phase 1: (element, quadrature point kd)
@@ -684,6 +686,12 @@ __global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph,
                 gx[v * D + j] = s * idetg;
             }
         PH::flux(ph, u, gx, F);
+        if (wave) {
+            // lambda_max / visc_max bookkeeping of the physical flux calls
+            lam_w = PH::wave_speed(ph, u);
+            vis_w = PH::visc_at(ph, u);
+            if (e == S.nelem - 1 && k == NB - 1) wave[1] = (unsigned long long)__double_as_longlong(lam_w);
+        }
         const double fac = detc * wq * idetg;
         double* Gq = sG + (le * NB + k) * M * D;
 #pragma unroll
@@ -695,6 +703,16 @@ __global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph,
                 for (int j = 0; j < D; ++j) s += F[v * D + j] * A[a][j];
                 Gq[v * D + a] = s * fac;
             }
+    }
+    if (wave) {  // wavefront max, inactive lanes contribute 0
+        for (int off = 32; off > 0; off >>= 1) {
+            lam_w = fmax(lam_w, __shfl_xor(lam_w, off));
+            vis_w = fmax(vis_w, __shfl_xor(vis_w, off));
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicMax(&wave[0], (unsigned long long)__double_as_longlong(lam_w));
+            atomicMax(&wave[2], (unsigned long long)__double_as_longlong(vis_w));
+        }
     }
     __syncthreads();
     if (!active) return;
@@ -834,7 +852,7 @@ __global__ __launch_bounds__(256) void bval_kernel(DevSpace S, const ice_field* 
 // Source load vector b[e][i][v] = sum_q s_v(x_q) phi_i(xi_q) max(0, detJ) w_q
 // (the user_source term of domain_integral, conservation_law.hpp:300-312).
 template <int D, int P>
-__global__ __launch_bounds__(256) void srcload_kernel(DevSpace S, const ice_field* __restrict__ fields, int M,
+__global__ __launch_bounds__(256) void srcload_kernel(DevSpace S, const ice_field* __restrict__ field, int M, int clamp,
                                                       double* __restrict__ out) {
     constexpr int N = P + 1;
     constexpr int NB = Pow<N, D>::value;
@@ -864,9 +882,10 @@ __global__ __launch_bounds__(256) void srcload_kernel(DevSpace S, const ice_fiel
         }
         double x[D], J[D][D], Hm[D][D][D], A[D][D];
         q1_map<D>(X, xi, x, J, Hm);
-        const double detc = fmax(0.0, det_adj<D>(J, A));
+        const double det = det_adj<D>(J, A);
+        const double detc = clamp ? fmax(0.0, det) : det;
         double src[8];
-        ice_field_eval(&fields[2 * ICE_MAX_BCFLAG], D, M, x, src);
+        ice_field_eval(field, D, M, x, src);
         for (int v = 0; v < M; ++v) acc[v] += src[v] * phi * detc * wq;
     }
     for (int v = 0; v < M; ++v) out[((size_t)e * NB + i) * M + v] = acc[v];
@@ -881,8 +900,83 @@ int launch_setup(const DevSpace& S, const ice_field* fields, int M, double* bval
         const long nt = (long)S.nfb * NFQ;
         bval_kernel<D, P><<<(unsigned)((nt + 255) / 256), 256, 0, st>>>(S, fields, M, bval);
     }
-    if (srcload) srcload_kernel<D, P><<<S.nelem, ((NB + 63) / 64) * 64, 0, st>>>(S, fields, M, srcload);
+    if (srcload)
+        srcload_kernel<D, P><<<S.nelem, ((NB + 63) / 64) * 64, 0, st>>>(S, fields + 2 * ICE_MAX_BCFLAG, M, 1, srcload);
     return (int)hipGetLastError();
+}
+
+// L2 error integrand per element: sum_q |u_h(x_q) - f(x_q)|^2 w_q |detJ_q|
+// summed over components (l2_error.hpp:37-94); one thread per element.
+template <int D, int P>
+__global__ __launch_bounds__(256) void l2err_kernel(DevSpace S, const ice_field* __restrict__ f, int M,
+                                                    const double* __restrict__ U, double* __restrict__ out) {
+    constexpr int N = P + 1;
+    constexpr int NB = Pow<N, D>::value;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= S.nelem) return;
+    const double* xs = S.tab;
+    const double* L0 = S.tab + (N + 2);
+    const double* wgl = L0 + 3 * (N + 2) * N;
+    double X[1 << D][D];
+    load_vertices<D>(S.X, e, X);
+    double acc = 0.0;
+    for (int q = 0; q < NB; ++q) {
+        int qd[D];
+        int r = q;
+        for (int d = D - 1; d >= 0; --d) { qd[d] = r % N; r /= N; }
+        double xi[D], wq = 1.0;
+        for (int d = 0; d < D; ++d) {
+            xi[d] = xs[qd[d]];
+            wq = (d == 0) ? wgl[qd[d]] : wq * wgl[qd[d]];
+        }
+        double x[D], J[D][D], Hm[D][D][D], A[D][D];
+        q1_map<D>(X, xi, x, J, Hm);
+        const double adet = fabs(det_adj<D>(J, A));
+        double fe[8], uh[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        ice_field_eval(f, D, M, x, fe);
+        for (int i = 0; i < NB; ++i) {
+            int id[D];
+            int t = i;
+            for (int d = D - 1; d >= 0; --d) { id[d] = t % N; t /= N; }
+            double phi = 1.0;
+            for (int d = 0; d < D; ++d) phi *= L0[qd[d] * N + id[d]];
+            for (int v = 0; v < M; ++v) uh[v] += phi * U[((size_t)e * NB + i) * M + v];
+        }
+        for (int v = 0; v < M; ++v) acc += (uh[v] - fe[v]) * (uh[v] - fe[v]) * wq * adet;
+    }
+    out[e] = acc;
+}
+
+template <int D, int P>
+int launch_proj(const DevSpace& S, const ice_field* f, int M, double* b, hipStream_t st) {
+    constexpr int NB = Pow<P + 1, D>::value;
+    srcload_kernel<D, P><<<S.nelem, ((NB + 63) / 64) * 64, 0, st>>>(S, f, M, 0, b);
+    return (int)hipGetLastError();
+}
+
+template <int D, int P>
+int launch_l2(const DevSpace& S, const ice_field* f, int M, const double* U, double* out, hipStream_t st) {
+    l2err_kernel<D, P><<<(S.nelem + 255) / 256, 256, 0, st>>>(S, f, M, U, out);
+    return (int)hipGetLastError();
+}
+
+int projection_rhs(int ndim, int p, int neq, const DevSpace& S, const ice_field* f, double* b, hipStream_t st) {
+#define ICE_PCASE(DD, PP) if (ndim == DD && p == PP) return launch_proj<DD, PP>(S, f, neq, b, st);
+    ICE_PCASE(1, 1) ICE_PCASE(1, 2) ICE_PCASE(1, 3) ICE_PCASE(1, 4)
+    ICE_PCASE(2, 1) ICE_PCASE(2, 2) ICE_PCASE(2, 3) ICE_PCASE(2, 4)
+    ICE_PCASE(3, 1) ICE_PCASE(3, 2) ICE_PCASE(3, 3) ICE_PCASE(3, 4)
+#undef ICE_PCASE
+    return -1;
+}
+
+int l2_error_terms(int ndim, int p, int neq, const DevSpace& S, const ice_field* f, const double* U, double* out,
+                   hipStream_t st) {
+#define ICE_LCASE(DD, PP) if (ndim == DD && p == PP) return launch_l2<DD, PP>(S, f, neq, U, out, st);
+    ICE_LCASE(1, 1) ICE_LCASE(1, 2) ICE_LCASE(1, 3) ICE_LCASE(1, 4)
+    ICE_LCASE(2, 1) ICE_LCASE(2, 2) ICE_LCASE(2, 3) ICE_LCASE(2, 4)
+    ICE_LCASE(3, 1) ICE_LCASE(3, 2) ICE_LCASE(3, 3) ICE_LCASE(3, 4)
+#undef ICE_LCASE
+    return -1;
 }
 
 int setup_disc_tables(int ndim, int p, int neq, const DevSpace& S, const ice_field* fields, double* bval,
@@ -909,10 +1003,9 @@ int launch_residual(const DevSpace& S, const PhysParams& ph, const DiscDev& dd, 
     const int nbb = (int)(((long)S.nfb * NFQ + 255) / 256);
     if (nbi + nbb > 0) face_kernel<D, P, PH><<<nbi + nbb, 256, 0, st>>>(S, ph, d, U, nbi);
     if (ev) (void)hipEventRecord(ev[1], st);
-    if (ev) (void)hipEventRecord(ev[2], st);
     using T = ElemTile<D, P>;
     const int nblk = (S.nelem + T::EPB - 1) / T::EPB;
-    element_kernel<D, P, PH><<<nblk, 256, 0, st>>>(S, ph, d.srcload, U, d.Q, d.Pic, R);
+    element_kernel<D, P, PH><<<nblk, 256, 0, st>>>(S, ph, d.srcload, U, d.Q, d.Pic, R, d.wave);
     if (ev) (void)hipEventRecord(ev[3], st);
     return (int)hipGetLastError();
 }

@@ -1,0 +1,266 @@
+// Explicit time stepping on the device: inverse mass matrix, CFL time step
+// and the vector updates of RK3SSP / RK3TVD / ExplicitEuler
+// (solvers/ssp_rk3.hpp:106-195, tvd_rk3.hpp:107-231, explicit_euler.hpp:87-138,
+// explicit_utils.hpp:61-160).
+//
+// Inverse mass matrix: the reference builds M_e = sum_q phi_i phi_j w_q detJ_q
+// and LU-solves it at every stage (ssp_rk3.hpp:142-163).  With as many
+// quadrature points as basis functions, M_e = B^T W_e B where B = B1 x .. x B1
+// is the square tensor-product Vandermonde of the Lagrange basis at the
+// Gauss points and W_e = diag(w_q detJ_q), so M_e^-1 = B^-1 W_e^-1 B^-T: two
+// sum-factorised passes with the N x N table B1^-1 and one scaling per
+// quadrature point, with no per-element storage.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "physics_dev.h"
+
+namespace ice {
+
+template <int B, int E>
+struct PowS { static constexpr int value = B * PowS<B, E - 1>::value; };
+template <int B>
+struct PowS<B, 0> { static constexpr int value = 1; };
+
+template <int D>
+__device__ __forceinline__ double q1_det(const double* __restrict__ Xe, const double* xi) {
+    double J[D][D];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int a = 0; a < D; ++a) J[i][a] = 0;
+#pragma unroll
+    for (int k = 0; k < (1 << D); ++k) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            double t = 1.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const int bit = (k >> (D - 1 - d)) & 1;
+                const double n0 = 0.5 * (1 - xi[d]);
+                t *= (d == a) ? (bit ? 0.5 : -0.5) : (bit ? 1.0 - n0 : n0);
+            }
+#pragma unroll
+            for (int i = 0; i < D; ++i) J[i][a] += t * Xe[k * D + i];
+        }
+    }
+    if constexpr (D == 1) return J[0][0];
+    else if constexpr (D == 2) return J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    else
+        return J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+               J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+}
+
+// k = M^-1 r for every element and component.  Elements whose mass matrix is
+// singular (a zero detJ at a quadrature point) get 1e-8 r, the fallback of
+// tvd_rk3.hpp:190-204.
+template <int D, int P, int M>
+__global__ __launch_bounds__(256) void minv_kernel(DevSpace S, const double* __restrict__ r, double* __restrict__ k) {
+    constexpr int N = P + 1;
+    constexpr int NB = PowS<N, D>::value;
+    constexpr int EPB = (256 / NB) > 0 ? 256 / NB : 1;
+    __shared__ double bufA[EPB * NB * M];
+    __shared__ double bufB[EPB * NB * M];
+    __shared__ int sing[EPB];
+    const int e0 = blockIdx.x * EPB;
+    const int le = threadIdx.x / NB;
+    const int kk = threadIdx.x - le * NB;
+    const int e = e0 + le;
+    const bool active = threadIdx.x < EPB * NB && e < S.nelem;
+    const double* xs = S.tab;
+    const double* wgl = S.tab + (N + 2) + 3 * (N + 2) * N;
+    const double* Binv = wgl + N;  // Binv[i][q]
+    if (threadIdx.x < EPB) sing[threadIdx.x] = 0;
+    {
+        const size_t base = (size_t)e0 * NB * M, total = (size_t)S.nelem * NB * M;
+        for (int idx = threadIdx.x; idx < EPB * NB * M; idx += blockDim.x)
+            bufA[idx] = (base + idx < total) ? r[base + idx] : 0.0;
+    }
+    int kd[D];
+    {
+        int t = kk;
+#pragma unroll
+        for (int d = D - 1; d >= 0; --d) { kd[d] = t % N; t /= N; }
+    }
+    __syncthreads();
+    double* cur = bufA;
+    double* nxt = bufB;
+    // y = B^-T r: contract axis d with Binv[i_d][q_d]
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        if (active) {
+            int stride = 1;
+#pragma unroll
+            for (int dd = D - 1; dd > d; --dd) stride *= N;
+            const int base = kk - kd[d] * stride;
+            double acc[M];
+#pragma unroll
+            for (int v = 0; v < M; ++v) acc[v] = 0;
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                const double c = Binv[j * N + kd[d]];
+                const double* src = cur + (le * NB + base + j * stride) * M;
+#pragma unroll
+                for (int v = 0; v < M; ++v) acc[v] += c * src[v];
+            }
+#pragma unroll
+            for (int v = 0; v < M; ++v) nxt[(le * NB + kk) * M + v] = acc[v];
+        }
+        __syncthreads();
+        double* t = cur; cur = nxt; nxt = t;
+    }
+    // z = y / (w_q detJ_q)
+    if (active) {
+        double xi[D], w = 1.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            xi[d] = xs[kd[d]];
+            w = (d == 0) ? wgl[kd[d]] : w * wgl[kd[d]];
+        }
+        const double det = q1_det<D>(S.X + (size_t)e * (1 << D) * D, xi);
+        if (det == 0.0) sing[le] = 1;
+        const double s = 1.0 / (w * det);
+#pragma unroll
+        for (int v = 0; v < M; ++v) cur[(le * NB + kk) * M + v] *= s;
+    }
+    __syncthreads();
+    // out = B^-1 z: contract axis d with Binv[i_d][q_d]
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        if (active) {
+            int stride = 1;
+#pragma unroll
+            for (int dd = D - 1; dd > d; --dd) stride *= N;
+            const int base = kk - kd[d] * stride;
+            double acc[M];
+#pragma unroll
+            for (int v = 0; v < M; ++v) acc[v] = 0;
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                const double c = Binv[kd[d] * N + j];
+                const double* src = cur + (le * NB + base + j * stride) * M;
+#pragma unroll
+                for (int v = 0; v < M; ++v) acc[v] += c * src[v];
+            }
+#pragma unroll
+            for (int v = 0; v < M; ++v) nxt[(le * NB + kk) * M + v] = acc[v];
+        }
+        __syncthreads();
+        double* t = cur; cur = nxt; nxt = t;
+    }
+    if (active) {
+        const size_t o = ((size_t)e * NB + kk) * M;
+        if (sing[le]) {
+#pragma unroll
+            for (int v = 0; v < M; ++v) k[o + v] = 1e-8 * r[o + v];
+        } else {
+#pragma unroll
+            for (int v = 0; v < M; ++v) k[o + v] = cur[(le * NB + kk) * M + v];
+        }
+    }
+}
+
+// y += coef * (*dt) * x   (axpy, fespan.hpp:283-300, with the step size read on the device)
+__global__ void axpy_dt_kernel(size_t n, double coef, const double* __restrict__ dtp, const double* __restrict__ x,
+                               double* __restrict__ y) {
+    const double a = coef * (dtp ? *dtp : 1.0);
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        y[i] += a * x[i];
+}
+
+// y = a x + b y  (axpby, fespan.hpp:312-330)
+__global__ void axpby_kernel(size_t n, double a, const double* __restrict__ x, double b, double* __restrict__ y) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        y[i] = a * x[i] + b * y[i];
+}
+
+// Time-step control on the device, so steps run without host round trips.
+// CFLTimestep (explicit_utils.hpp:61-90) with the flux family's dt_from_cfl
+// (burgers.hpp:103-108 / navier_stokes.hpp:1191-1203), then the termination
+// condition's limit_dt (explicit_utils.hpp:123-160).
+__global__ void step_begin_kernel(StepState* st, unsigned long long* wave, StepControl c) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const bool done = c.use_tfinal ? (st->time >= c.tfinal) : (st->itime >= c.ntime);
+    st->done = done ? 1 : 0;
+    double dt = c.dt_fixed;
+    if (c.use_cfl) {
+        const double lam_run = __longlong_as_double((long long)wave[0]);
+        const double lam_last = __longlong_as_double((long long)wave[1]);
+        const double visc_max = __longlong_as_double((long long)wave[2]);
+        if (c.flux == ICE_FLUX_BURGERS) {
+            dt = (c.reflen * c.cfl) / (c.mu / c.reflen + lam_run);
+        } else {
+            dt = (c.reflen * c.cfl) / lam_last;
+            if (c.flux == ICE_FLUX_NAVIER_STOKES) dt = fmin(dt, c.reflen * c.reflen * c.cfl / visc_max);
+            wave[1] = 0ull;  // dt_from_cfl resets lambda_max and visc_max
+            wave[2] = 0ull;
+        }
+        dt /= (2 * c.pmax + 1);
+    }
+    if (c.use_tfinal) dt = fmin(dt, c.tfinal - st->time);
+    st->dt = done ? 0.0 : dt;
+}
+
+__global__ void step_end_kernel(StepState* st) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (!st->done) {
+        st->time += st->dt;
+        st->itime += 1;
+    }
+}
+
+template <int D, int P, int M>
+int launch_minv_t(const DevSpace& S, const double* r, double* k, hipStream_t st) {
+    constexpr int NB = PowS<P + 1, D>::value;
+    constexpr int EPB = (256 / NB) > 0 ? 256 / NB : 1;
+    minv_kernel<D, P, M><<<(S.nelem + EPB - 1) / EPB, 256, 0, st>>>(S, r, k);
+    return (int)hipGetLastError();
+}
+
+template <int D, int M>
+int minv_order(int p, const DevSpace& S, const double* r, double* k, hipStream_t st) {
+    switch (p) {
+    case 1: return launch_minv_t<D, 1, M>(S, r, k, st);
+    case 2: return launch_minv_t<D, 2, M>(S, r, k, st);
+    case 3: return launch_minv_t<D, 3, M>(S, r, k, st);
+    case 4: return launch_minv_t<D, 4, M>(S, r, k, st);
+    default: return -1;
+    }
+}
+
+int launch_minv(int ndim, int p, int neq, const DevSpace& S, const double* r, double* k, hipStream_t st) {
+    if (ndim == 1 && neq == 1) return minv_order<1, 1>(p, S, r, k, st);
+    if (ndim == 1 && neq == 3) return minv_order<1, 3>(p, S, r, k, st);
+    if (ndim == 2 && neq == 1) return minv_order<2, 1>(p, S, r, k, st);
+    if (ndim == 2 && neq == 4) return minv_order<2, 4>(p, S, r, k, st);
+    if (ndim == 3 && neq == 1) return minv_order<3, 1>(p, S, r, k, st);
+    if (ndim == 3 && neq == 5) return minv_order<3, 5>(p, S, r, k, st);
+    return -1;
+}
+
+static unsigned grid_for(size_t n) {
+    const size_t b = (n + 255) / 256;
+    return (unsigned)(b > 8192 ? 8192 : (b ? b : 1));
+}
+
+int launch_axpy_dt(size_t n, double coef, const double* dtp, const double* x, double* y, hipStream_t st) {
+    axpy_dt_kernel<<<grid_for(n), 256, 0, st>>>(n, coef, dtp, x, y);
+    return (int)hipGetLastError();
+}
+
+int launch_axpby(size_t n, double a, const double* x, double b, double* y, hipStream_t st) {
+    axpby_kernel<<<grid_for(n), 256, 0, st>>>(n, a, x, b, y);
+    return (int)hipGetLastError();
+}
+
+int launch_step_begin(StepState* s, unsigned long long* wave, const StepControl& c, hipStream_t st) {
+    step_begin_kernel<<<1, 64, 0, st>>>(s, wave, c);
+    return (int)hipGetLastError();
+}
+
+int launch_step_end(StepState* s, hipStream_t st) {
+    step_end_kernel<<<1, 64, 0, st>>>(s);
+    return (int)hipGetLastError();
+}
+
+}  // namespace ice

@@ -41,6 +41,15 @@ _SIGS = {
                                C.POINTER(C.c_double)], C.c_int),
     "ice_space_node_coords": ([P, C.POINTER(C.c_double)], C.c_int),
     "ice_trace_points": ([C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)], C.c_int),
+    "ice_mass_inverse_apply": ([P, P, P, P], C.c_int),
+    "ice_project": ([P, C.POINTER(abi.Field), P, P], C.c_int),
+    "ice_l2_error": ([P, C.POINTER(abi.Field), P, C.POINTER(C.c_double), P], C.c_int),
+    "ice_stepper_create": ([P, C.POINTER(abi.StepperDesc), C.POINTER(P)], C.c_int),
+    "ice_stepper_destroy": ([P], C.c_int),
+    "ice_stepper_step": ([P, P, P], C.c_int),
+    "ice_stepper_solve": ([P, P, P, C.POINTER(C.c_longlong), C.POINTER(C.c_double)], C.c_int),
+    "ice_stepper_status": ([P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_longlong),
+                            C.POINTER(C.c_int)], C.c_int),
 }
 
 
@@ -186,3 +195,59 @@ def form_residual(space, disc, u, res, stream=None):
         raise IceError(f"u/res must have {disc.size} entries")
     st = current_stream() if stream is None else stream
     check(lib().ice_residual(disc.h, C.c_void_p(u.data_ptr()), C.c_void_p(res.data_ptr()), st))
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def mass_inverse_apply(disc, r, out, stream=None):
+    """out <- M^-1 r per element (inverse mass matrix of the explicit steppers)."""
+    _require_cuda(r, "r")
+    _require_cuda(out, "out")
+    check(lib().ice_mass_inverse_apply(disc.h, _ptr(r), _ptr(out), stream or current_stream()))
+
+
+def project(disc, field, u, stream=None):
+    """u <- L2 projection of an analytic field (synchronous)."""
+    _require_cuda(u, "u")
+    check(lib().ice_project(disc.h, C.byref(field), _ptr(u), stream or current_stream()))
+
+
+def l2_error(disc, field, u, stream=None):
+    _require_cuda(u, "u")
+    err = C.c_double()
+    check(lib().ice_l2_error(disc.h, C.byref(field), _ptr(u), C.byref(err), stream or current_stream()))
+    return err.value
+
+
+class ExplicitStepper:
+    """RK3SSP / RK3TVD / ExplicitEuler on the device (abi.stepper_desc)."""
+
+    def __init__(self, disc, desc):
+        self.disc = disc
+        self.desc = desc
+        h = P()
+        check(lib().ice_stepper_create(disc.h, C.byref(desc), C.byref(h)))
+        self.h = h
+
+    def step(self, u, stream=None):
+        _require_cuda(u, "u")
+        check(lib().ice_stepper_step(self.h, _ptr(u), stream or current_stream()))
+
+    def solve(self, u, stream=None):
+        _require_cuda(u, "u")
+        n = C.c_longlong()
+        t = C.c_double()
+        check(lib().ice_stepper_solve(self.h, _ptr(u), stream or current_stream(), C.byref(n), C.byref(t)))
+        return n.value, t.value
+
+    def status(self):
+        t, dt, it, done = C.c_double(), C.c_double(), C.c_longlong(), C.c_int()
+        check(lib().ice_stepper_status(self.h, C.byref(t), C.byref(dt), C.byref(it), C.byref(done)))
+        return dict(time=t.value, dt=dt.value, itime=it.value, done=bool(done.value))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ice_stepper_destroy(self.h)
+            self.h = None

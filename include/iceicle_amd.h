@@ -213,11 +213,56 @@ int ice_disc_destroy(ice_disc* disc);
  * in fe_layout_right order, both nelem*nbasis*neq doubles. */
 int ice_residual(ice_disc* disc, const double* u_dev, double* res_dev, void* stream);
 
+/* Inverse mass matrix per element and component: out = M_e^-1 r.  Replaces
+ * calculate_mass_matrix + decompose_lu + sub_lu of the explicit steppers
+ * (finite_element.hpp:586-611, ssp_rk3.hpp:142-163); singular element mass
+ * matrices fall back to 1e-8 r as tvd_rk3.hpp:190-204. */
+int ice_mass_inverse_apply(ice_disc* disc, const double* r_dev, double* out_dev, void* stream);
+
+/* L2 projection of an analytic field onto the space (Projection::domain_integral
+ * + ElementLinearSolver, projection.hpp:83-110, element_linear_solve.hpp:37-90,
+ * as used by projection_initialization, initialization.hpp:20-30).  Synchronous. */
+int ice_project(ice_disc* disc, const ice_field* f, double* u_dev, void* stream);
+
+/* L2 error of u against an analytic field (l2_error.hpp:37-94).  Synchronous. */
+int ice_l2_error(ice_disc* disc, const ice_field* f, const double* u_dev, double* err, void* stream);
+
+/* ---- explicit time integration (RK3SSP ssp_rk3.hpp:106-233, RK3TVD
+ *      tvd_rk3.hpp:107-270, ExplicitEuler explicit_euler.hpp:87-160) ----- */
+#define ICE_STEP_RK3_SSP 0
+#define ICE_STEP_RK3_TVD 1
+#define ICE_STEP_EXPLICIT_EULER 2
+
+typedef struct ice_stepper_desc {
+    int kind;                 /* ICE_STEP_* */
+    int reference_compat;     /* RK3_SSP only: 1 evaluates every stage residual at u^n
+                                 as ssp_rk3.hpp:137 does; 0 = textbook SSP-RK3 */
+    int use_cfl;              /* 1: CFLTimestep (explicit_utils.hpp:61-90), 0: FixedTimestep */
+    double dt;                /* fixed step */
+    double cfl;               /* CFL number */
+    int use_tfinal;           /* 1: TfinalTermination, 0: TimestepTermination */
+    double tfinal;
+    long long ntime;
+} ice_stepper_desc;
+
+typedef struct ice_stepper ice_stepper;
+
+int ice_stepper_create(ice_disc* disc, const ice_stepper_desc* desc, ice_stepper** out);
+int ice_stepper_destroy(ice_stepper* stepper);
+/* One step of u (device, fe_layout_right) on `stream`; time step, time and
+ * step count live on the device, so consecutive steps need no host sync. */
+int ice_stepper_step(ice_stepper* stepper, double* u_dev, void* stream);
+/* solve(): one residual for the wave speeds, then steps until termination. */
+int ice_stepper_solve(ice_stepper* stepper, double* u_dev, void* stream, long long* nsteps, double* time);
+/* Synchronous read of the device-side time, last step size and step count. */
+int ice_stepper_status(ice_stepper* stepper, double* time, double* dt, long long* itime, int* done);
+
 /* Kernel timing (no reference counterpart; the reference has no
  * instrumentation).  Between begin and end, every residual on this handle
- * records HIP events around its interior-face, boundary-face and element
- * launches on the caller's stream; end waits for them and writes the summed
- * milliseconds per stage to stage_ms[3] and the number of residuals to *ncalls. */
+ * records HIP events around its face launch (interior + boundary faces) and
+ * its element launch on the caller's stream; end waits for them and writes
+ * the summed milliseconds to stage_ms[0] (faces) and stage_ms[2] (elements),
+ * stage_ms[1] = 0, and the number of residuals to *ncalls. */
 int ice_profile_begin(ice_disc* disc, int max_calls);
 int ice_profile_end(ice_disc* disc, double* stage_ms, int* ncalls);
 

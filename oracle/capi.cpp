@@ -32,6 +32,8 @@ struct ProblemBase {
     virtual void face_geometry(double* x, double* nrm, double* sqrtg) const = 0;
     virtual void qp_coords(double* xq) const = 0;
     virtual void mass_apply(const double* r, double* out) const = 0;
+    virtual void rk_solve(double* u, int kind, int compat, int use_cfl, double dt, double cfl, int use_tfinal,
+                          double tfinal, long long ntime, long long* nsteps, double* time) = 0;
 };
 
 template <int D, class PF, class CF, class DF>
@@ -144,6 +146,28 @@ struct Problem final : ProblemBase {
     void mass_apply(const double* r, double* out) const override {
         const_cast<Problem*>(this)->mi().apply_add(fs, neq, r, out);
     }
+    // RK3SSP/RK3TVD/ExplicitEuler ::solve (ssp_rk3.hpp:208-233, tvd_rk3.hpp:240-270,
+    // explicit_euler.hpp:150-170): an initial residual, then steps until the
+    // termination condition holds; dt from FixedTimestep or CFLTimestep,
+    // limited by TfinalTermination::limit_dt
+    void rk_solve(double* u, int kind, int compat, int use_cfl, double dt, double cfl, int use_tfinal, double tfinal,
+                  long long ntime, long long* nsteps, double* time) override {
+        std::vector<double> r((size_t)fs.mesh.nelem * fs.nb * neq);
+        form_residual(fs, disc, u, r.data());
+        double t = 0;
+        long long it = 0;
+        while (!(use_tfinal ? t >= tfinal : it >= ntime)) {
+            double dtk = use_cfl ? cfl_timestep(fs, disc, cfl) : dt;
+            if (use_tfinal) dtk = std::min(dtk, tfinal - t);
+            if (kind == 0) rk3_ssp_step(fs, disc, mi(), dtk, compat ? RKMode::REFERENCE_COMPAT : RKMode::TEXTBOOK, u);
+            else if (kind == 1) rk3_tvd_step(fs, disc, mi(), dtk, u);
+            else explicit_euler_step(fs, disc, mi(), dtk, u);
+            t += dtk;
+            ++it;
+        }
+        if (nsteps) *nsteps = it;
+        if (time) *time = t;
+    }
 };
 
 template <int D>
@@ -241,6 +265,10 @@ int ice_oracle_trace_points(void* h, double* xiL, double* xiR) { ORACLE_TRY(stat
 int ice_oracle_face_geometry(void* h, double* x, double* n, double* sg) { ORACLE_TRY(static_cast<Handle*>(h)->prob->face_geometry(x, n, sg)) }
 int ice_oracle_qp_coords(void* h, double* xq) { ORACLE_TRY(static_cast<Handle*>(h)->prob->qp_coords(xq)) }
 int ice_oracle_mass_apply(void* h, const double* r, double* out) { ORACLE_TRY(static_cast<Handle*>(h)->prob->mass_apply(r, out)) }
+int ice_oracle_rk_solve(void* h, double* u, int kind, int compat, int use_cfl, double dt, double cfl, int use_tfinal,
+                        double tfinal, long long ntime, long long* nsteps, double* time) {
+    ORACLE_TRY(static_cast<Handle*>(h)->prob->rk_solve(u, kind, compat, use_cfl, dt, cfl, use_tfinal, tfinal, ntime, nsteps, time))
+}
 
 // reference-element primitives for closed-form checks
 int ice_oracle_lagrange_1d(int p, double xi, double* N, double* dN, double* d2N) {

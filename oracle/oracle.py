@@ -49,6 +49,8 @@ def lib():
             ("ice_oracle_ns_flux", [C.POINTER(abi.DiscParams), C.c_int, dp, dp, dp]),
             ("ice_oracle_vanleer", [C.POINTER(abi.DiscParams), C.c_int, dp, dp, dp, dp]),
             ("ice_oracle_trace_ref", [C.c_int, C.c_int, C.c_int, dp, dp]),
+            ("ice_oracle_rk_solve", [P, dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                                     C.c_double, C.c_longlong, C.POINTER(C.c_longlong), C.POINTER(C.c_double)]),
         ]:
             fn = getattr(L, name)
             fn.argtypes = args
@@ -132,6 +134,17 @@ class Oracle:
         if n < 0:
             _check(-1)
         return u, n
+
+    def rk_solve(self, u, kind=0, compat=False, dt=0.0, cfl=None, tfinal=None, ntime=0):
+        """Explicit solve as the reference's solve(): kind 0 RK3-SSP, 1 RK3-TVD,
+        2 explicit Euler; CFL time step if cfl is given, else fixed dt."""
+        u = np.array(u, dtype=np.float64)
+        n = C.c_longlong()
+        t = C.c_double()
+        _check(lib().ice_oracle_rk_solve(self.h, _dp(u), kind, int(compat), int(cfl is not None), dt,
+                                         cfl or 0.0, int(tfinal is not None), tfinal or 0.0, ntime,
+                                         C.byref(n), C.byref(t)))
+        return u, n.value, t.value
 
     def cfl_dt(self, cfl):
         return lib().ice_oracle_cfl_dt(self.h, cfl)
