@@ -118,6 +118,7 @@ struct ice_space {
     std::vector<int> efaces;
     std::vector<signed char> fpt, fvo;
     std::vector<double> X;
+    std::vector<int> jcols;           // [nelem][2*ndim+1] block columns of the element-block Jacobian
     ice::DevSpace dev{};
     void* dmem[8] = {};
     ~ice_space() {
@@ -142,10 +143,13 @@ struct ice_disc {
     double* d_bval = nullptr;
     double* d_srcload = nullptr;
     unsigned long long* d_wave = nullptr;
+    double* d_wave_part = nullptr;
+    int wave_users = 0;       // CFL steppers bound to this discretization
     ice::DiscDev dd{};
     ~ice_disc() {
         for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
-        for (void* p : {(void*)d_fields, (void*)d_Q, (void*)d_Pic, (void*)d_bval, (void*)d_srcload, (void*)d_wave})
+        for (void* p : {(void*)d_fields, (void*)d_Q, (void*)d_Pic, (void*)d_bval, (void*)d_srcload, (void*)d_wave,
+                        (void*)d_wave_part})
             if (p) (void)hipFree(p);
     }
 };
@@ -339,7 +343,22 @@ int ice_space_create(const ice_mesh_desc* md, const ice_space_desc* sd, ice_spac
             const int4 r = s->fbdy[b];
             s->efaces[(size_t)r.x * 2 * D + (r.y & 7)] = ((nfi + b) << 4);
         }
+        // Jacobian block columns: the element itself, then the element across each local face
+        {
+            const int ns = 2 * D + 1;
+            s->jcols.assign((size_t)m.nelem * ns, -1);
+            for (int e = 0; e < m.nelem; ++e) {
+                s->jcols[(size_t)e * ns] = e;
+                for (int j = 0; j < 2 * D; ++j) {
+                    const int v = s->efaces[(size_t)e * 2 * D + j];
+                    if (v < 0 || (v >> 4) >= nfi) continue;
+                    const int4 r = s->fint[v >> 4];
+                    s->jcols[(size_t)e * ns + 1 + j] = ((v >> 3) & 1) ? r.x : r.y;
+                }
+            }
+        }
         // upload
+        s->dmem[7] = upload(s->jcols);
         s->dmem[0] = upload(s->X);
         s->dmem[1] = upload(s->fint);
         s->dmem[2] = upload(s->fbdy);
@@ -363,6 +382,7 @@ int ice_space_create(const ice_mesh_desc* md, const ice_space_desc* sd, ice_spac
         dv.tab = (const double*)s->dmem[4];
         dv.fpt = (const signed char*)s->dmem[5];
         dv.fvo = (const signed char*)s->dmem[6];
+        dv.jcols = (const int*)s->dmem[7];
         *out = s.release();
         return ICE_OK;
     } catch (const ice::Error& e) {
@@ -488,8 +508,14 @@ int ice_disc_create(ice_space* s, const ice_disc_params* p, ice_disc** out) {
             throw ice::Error{ICE_ERR_HIP, "table setup launch failed"};
         ICE_HIP(hipMalloc(&d->d_wave, 4 * sizeof(unsigned long long)));
         ICE_HIP(hipMemset(d->d_wave, 0, 4 * sizeof(unsigned long long)));
+        {
+            const int nb = s->nb, epb = std::max(1, 256 / nb);
+            const size_t nblk = (size_t)(s->mesh.nelem + epb - 1) / epb;
+            ICE_HIP(hipMalloc(&d->d_wave_part, 2 * nblk * sizeof(double)));
+        }
         ICE_HIP(hipDeviceSynchronize());
-        d->dd.wave = d->d_wave;
+        d->dd.wave = nullptr;  // wave-speed bookkeeping only while a CFL stepper uses it
+        d->dd.wave_part = d->d_wave_part;
         d->dd.fields = d->d_fields;
         d->dd.bval = d->d_bval;
         d->dd.srcload = d->d_srcload;
@@ -569,6 +595,41 @@ int ice_l2_error(ice_disc* d, const ice_field* f, const double* u, double* err, 
     double acc = 0.0;
     for (double v : h) acc += v;
     *err = std::sqrt(acc);
+    return ICE_OK;
+}
+
+int ice_jacobian_layout(const ice_disc* d, int* nslots, int* block_size, int* cols) {
+    if (!d || !nslots || !block_size) return fail(ICE_ERR_ARG, "null argument");
+    const ice_space* s = d->space;
+    *nslots = 2 * s->ndim + 1;
+    *block_size = s->nb * d->neq;
+    if (cols) std::memcpy(cols, s->jcols.data(), s->jcols.size() * sizeof(int));
+    return ICE_OK;
+}
+
+int ice_jacobian(ice_disc* d, const double* u, double* r, double* blocks, void* stream) {
+    if (!d || !u || !blocks) return fail(ICE_ERR_ARG, "null argument");
+    const ice_space* s = d->space;
+    if (s->ndim > 2) return fail(ICE_ERR_UNSUPPORTED, "the device Jacobian covers ndim <= 2");
+    if (d->ddgic) return fail(ICE_ERR_UNSUPPORTED, "the device Jacobian does not cover the DDGIC term (sigma_ic != 0)");
+    hipStream_t st = (hipStream_t)stream;
+    if (r) {
+        const int rc = ice::residual_fp64(s->ndim, s->p, d->prm.flux, s->dev, d->ph, d->dd, u, r, st, nullptr);
+        if (rc != 0) return fail(ICE_ERR_HIP, "residual launch failed");
+    }
+    const size_t bs = (size_t)s->nb * d->neq;
+    const size_t nvals = (size_t)s->mesh.nelem * (2 * s->ndim + 1) * bs * bs;
+    const int rc = ice::jacobian_fp64(s->ndim, s->p, d->prm.flux, s->dev, d->ph, d->dd, u, blocks, nvals, st);
+    if (rc != 0) return fail(ICE_ERR_HIP, std::string("jacobian launch failed: ") + hipGetErrorString((hipError_t)rc));
+    return ICE_OK;
+}
+
+int ice_jacobian_apply(const ice_disc* d, const double* blocks, const double* x, double* y, void* stream) {
+    if (!d || !blocks || !x || !y) return fail(ICE_ERR_ARG, "null argument");
+    const ice_space* s = d->space;
+    const int rc = ice::bsr_apply<double, double>(s->mesh.nelem, 2 * s->ndim + 1, s->nb * d->neq, s->dev.jcols, blocks,
+                                                  x, nullptr, y, (hipStream_t)stream);
+    if (rc != 0) return fail(ICE_ERR_HIP, "block product launch failed");
     return ICE_OK;
 }
 
@@ -661,6 +722,7 @@ int ice_trace_points(int ndim, int order, int face_nr, int orientation, double* 
 
 // ------------------------------------------------------------ explicit steppers
 struct ice_stepper {
+    bool tracks_wave = false;
     ice_disc* disc = nullptr;
     ice_stepper_desc desc{};
     ice::StepControl ctl{};
@@ -668,6 +730,7 @@ struct ice_stepper {
     double* buf[5] = {};      // res, k1, k2, k3, ustage / uold
     ice::StepState* st = nullptr;
     ~ice_stepper() {
+        if (tracks_wave && disc && --disc->wave_users == 0) disc->dd.wave = nullptr;
         for (double* b : buf)
             if (b) (void)hipFree(b);
         if (st) (void)hipFree(st);
@@ -791,6 +854,11 @@ int ice_stepper_create(ice_disc* d, const ice_stepper_desc* desc, ice_stepper** 
         c.use_tfinal = desc->use_tfinal;
         c.tfinal = desc->tfinal;
         c.ntime = desc->ntime;
+        if (desc->use_cfl) {
+            S->tracks_wave = true;
+            d->wave_users++;
+            d->dd.wave = d->d_wave;
+        }
         *out = S.release();
         return ICE_OK;
     } catch (const ice::Error& e) {

@@ -1,0 +1,573 @@
+// Finite-difference element-block Jacobian on the device
+// (form_petsc_jacobian_fd DG part, form_petsc_jacobian.hpp:55-317, and
+// ConservationLawDDG::domain_integral_jacobian, conservation_law.hpp:316-432).
+//
+// Storage: block-sparse rows, one block row per element with 2*ndim+1 slots:
+// slot 0 = diagonal block, slot 1+j = the neighbour across local face j.
+// Blocks are (nb*neq)^2 row-major [i*neq+v][j*neq+w] in FP64.
+//
+// Face blocks: one workgroup per face.  The reference perturbs one
+// coefficient at a time and re-integrates the whole face; a coefficient
+// perturbation changes the traces at each face point by eps*phi_i,
+// eps*grad(phi_i) and eps*Hess(phi_i).n only, so each (column, face point)
+// pair re-evaluates just the pointwise numerical flux from the stored base
+// traces.  The flux differences are lifted to block entries in LDS; blocks
+// coupling two different elements belong to exactly one face and are
+// stored directly, diagonal blocks are accumulated with atomics.
+#include <hip/hip_runtime.h>
+
+#include "dg_common.h"
+#include "fields.h"
+#include "kernels.h"
+#include "physics_dev.h"
+
+namespace ice {
+
+namespace {
+
+constexpr double SQRT_EPS = 1.4901161193847656e-08;  // sqrt(DBL_EPSILON)
+
+// phi_i, reference gradient and reference Hessian at S-point pt (tensor products)
+template <int D, int N>
+__device__ __forceinline__ void basis_at(const double* tab, const int* pt, int i, double& phi, double* gref,
+                                         double* href) {
+    const double* L0 = tab + (N + 2);
+    const double* L1 = L0 + (N + 2) * N;
+    const double* L2 = L1 + (N + 2) * N;
+    int id[D];
+    int r = i;
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) { id[d] = r % N; r /= N; }
+    phi = 1.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) phi *= L0[pt[d] * N + id[d]];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        double t = 1.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) t *= (d == a) ? L1[pt[d] * N + id[d]] : L0[pt[d] * N + id[d]];
+        gref[a] = t;
+    }
+    if (href) {
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int b = 0; b < D; ++b) {
+                double t = 1.0;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    if (a == b) t *= (d == a) ? L2[pt[d] * N + id[d]] : L0[pt[d] * N + id[d]];
+                    else t *= (d == a || d == b) ? L1[pt[d] * N + id[d]] : L0[pt[d] * N + id[d]];
+                }
+                href[a * D + b] = t;
+            }
+    }
+}
+
+// per (side, face point) geometry kept in LDS: adj(J)/det, the map Hessian
+template <int D>
+struct PointGeo {
+    double Ai[D][D];    // adj(J) / det
+    double Hm[D][D][D];
+};
+
+template <int D>
+__device__ __forceinline__ void make_geo(const double (&X)[1 << D][D], const double* xi, PointGeo<D>& g) {
+    double J[D][D], A[D][D];
+    q1_map<D>(X, xi, nullptr, J, g.Hm);
+    double det = det_adj<D>(J, A);
+    det = (det == 0.0) ? 1.0 : det;
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) g.Ai[a][b] = A[a][b] / det;
+}
+
+// physical gradient and normal Hessian term of one basis function
+template <int D>
+__device__ __forceinline__ void phys_basis(const PointGeo<D>& g, const double* gref, const double* href,
+                                           const double* nrm, double* gx, double* hn) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        double s = 0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) s += gref[k] * g.Ai[k][j];
+        gx[j] = s;
+    }
+    if (hn) {
+        double rhs[D][D];
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int b = 0; b < D; ++b) {
+                double s = href[a * D + b];
+#pragma unroll
+                for (int k = 0; k < D; ++k) s -= g.Hm[k][a][b] * gx[k];
+                rhs[a][b] = s;
+            }
+        // Hx = Ai^T rhs Ai ; hn[d] = sum_j Hx[j][d] n_j
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            double s = 0;
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                double hx = 0;
+#pragma unroll
+                for (int l = 0; l < D; ++l)
+#pragma unroll
+                    for (int k = 0; k < D; ++k) hx += g.Ai[l][j] * rhs[l][k] * g.Ai[k][d];
+                s += hx * nrm[j];
+            }
+            hn[d] = s;
+        }
+    }
+}
+
+__device__ __forceinline__ void atomic_add_f64(double* a, double v) { atomicAdd(a, v); }
+
+}  // namespace
+
+// ---- interior faces --------------------------------------------------------
+template <int D, int P, class PH>
+__global__ __launch_bounds__(256) void jac_face_kernel(DevSpace S, PhysParams ph, const double* __restrict__ U,
+                                                       double* __restrict__ J, int bslots) {
+    constexpr int N = P + 1;
+    constexpr int NB = Pow<N, D>::value;
+    constexpr int NFQ = Pow<N, D - 1>::value;
+    constexpr int M = PH::M;
+    constexpr int NBM = NB * M;
+    constexpr int NC = 2 * NBM;  // columns: L dofs then R dofs
+    __shared__ PointGeo<D> sgeo[2][NFQ];
+    __shared__ double su[2][NFQ][M], sg[2][NFQ][M * D], shn[2][NFQ][M * D];
+    __shared__ double snrm[NFQ][D], sh[NFQ], sws[NFQ];
+    __shared__ int spt[2][NFQ][D];
+    __shared__ double sQ0[NFQ][M];
+    __shared__ double sdQ[NC][NFQ][M];
+    __shared__ double sred[256];
+    __shared__ double seps;
+    const int f = blockIdx.x;
+    const int4 rec = S.fint[f];
+    const int eL = rec.x, eR = rec.y, info = rec.z;
+    const int fnl = info_fnl(info), fnr = info_fnr(info), orient = info_orient(info), pdim = info_pdim(info);
+    const int tid = threadIdx.x;
+    if (tid < 2 * NFQ) {
+        const int side = tid / NFQ, q = tid % NFQ;
+        double XL[1 << D][D], XS[1 << D][D];
+        load_vertices<D>(S.X, eL, XL);
+        load_vertices<D>(S.X, side ? eR : eL, XS);
+        double x[D], nrm[D];
+        const double sqrtg = face_geometry<D, N>(S, XL, fnl, q, x, nrm);
+        int pt[D];
+        const signed char* fp =
+            S.fpt + (((side ? fnr : fnl) * S.norient + (side ? orient : 0)) * NFQ + q) * D;
+        double xi[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            pt[d] = fp[d];
+            spt[side][q][d] = pt[d];
+            xi[d] = S.tab[pt[d]];
+        }
+        make_geo<D>(XS, xi, sgeo[side][q]);
+        double u[M], g[M * D], hn[M * D];
+        trace_eval<D, N, M, true>(S, U + (size_t)(side ? eR : eL) * NB * M, pt, XS, nrm, u, g, hn);
+#pragma unroll
+        for (int v = 0; v < M; ++v) su[side][q][v] = u[v];
+#pragma unroll
+        for (int k = 0; k < M * D; ++k) { sg[side][q][k] = g[k]; shn[side][q][k] = hn[k]; }
+        if (side == 0) {
+            double XR[1 << D][D], cL[D], cR[D];
+            load_vertices<D>(S.X, eR, XR);
+            centroid<D>(XL, cL);
+            centroid<D>(XR, cR);
+            if (pdim) cR[pdim - 1] += S.period[pdim - 1];
+#pragma unroll
+            for (int d = 0; d < D; ++d) snrm[q][d] = nrm[d];
+            sh[q] = ddg_h_interior<D>(nrm, x, cL, cR);
+            sws[q] = face_weight<N, D>(S, q) * sqrtg;
+        }
+    }
+    __syncthreads();
+    if (tid < NFQ) {
+        double Q[M];
+        interior_point_flux<D, PH>(ph, su[0][tid], sg[0][tid], shn[0][tid], su[1][tid], sg[1][tid], shn[1][tid],
+                                   snrm[tid], sh[tid], sws[tid], Q);
+#pragma unroll
+        for (int v = 0; v < M; ++v) sQ0[tid][v] = Q[v];
+    }
+    __syncthreads();
+    // eps = max(sqrt eps, sqrt eps * max(|resL|, |resR|)) (form_petsc_jacobian.hpp:191)
+    double sL = 0, sR = 0;
+    for (int r = tid; r < NC; r += blockDim.x) {
+        const int side = r / NBM, iv = r % NBM, i = iv / M, v = iv % M;
+        double acc = 0;
+        for (int q = 0; q < NFQ; ++q) {
+            double phi, gref[D];
+            basis_at<D, N>(S.tab, spt[side][q], i, phi, gref, nullptr);
+            acc += sQ0[q][v] * phi;
+        }
+        if (side) sR += acc * acc;
+        else sL += acc * acc;
+    }
+    sred[tid] = sL;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) sred[tid] += sred[tid + s];
+        __syncthreads();
+    }
+    const double nL = sqrt(sred[0]);
+    __syncthreads();
+    sred[tid] = sR;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) sred[tid] += sred[tid + s];
+        __syncthreads();
+    }
+    if (tid == 0) seps = fmax(SQRT_EPS, SQRT_EPS * fmax(nL, sqrt(sred[0])));
+    __syncthreads();
+    const double eps = seps;
+    // flux differences for every (column, face point)
+    for (int it = tid; it < NC * NFQ; it += blockDim.x) {
+        const int c = it / NFQ, q = it % NFQ;
+        const int side = c / NBM, iw = c % NBM, i = iw / M, w = iw % M;
+        double phi, gref[D], href[D * D], gx[D], hn[D];
+        basis_at<D, N>(S.tab, spt[side][q], i, phi, gref, href);
+        phys_basis<D>(sgeo[side][q], gref, href, snrm[q], gx, hn);
+        double u[2][M], g[2][M * D], h[2][M * D];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+#pragma unroll
+            for (int v = 0; v < M; ++v) u[s][v] = su[s][q][v];
+#pragma unroll
+            for (int k = 0; k < M * D; ++k) { g[s][k] = sg[s][q][k]; h[s][k] = shn[s][q][k]; }
+        }
+        u[side][w] += eps * phi;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            g[side][w * D + d] += eps * gx[d];
+            h[side][w * D + d] += eps * hn[d];
+        }
+        double Q[M];
+        interior_point_flux<D, PH>(ph, u[0], g[0], h[0], u[1], g[1], h[1], snrm[q], sh[q], sws[q], Q);
+#pragma unroll
+        for (int v = 0; v < M; ++v) sdQ[c][q][v] = Q[v] - sQ0[q][v];
+    }
+    __syncthreads();
+    // lift: row (side', i', v'), column c
+    const size_t bs = NBM;
+    const double ieps = 1.0 / eps;
+    for (int ent = tid; ent < NC * NC; ent += blockDim.x) {
+        const int r = ent / NC, c = ent % NC;
+        const int rs = r / NBM, iv = r % NBM, i = iv / M, v = iv % M;
+        const int cs = c / NBM, cw = c % NBM;
+        double acc = 0;
+#pragma unroll
+        for (int q = 0; q < NFQ; ++q) {
+            double phi, gref[D];
+            basis_at<D, N>(S.tab, spt[rs][q], i, phi, gref, nullptr);
+            acc += sdQ[c][q][v] * phi;
+        }
+        const double val = (rs ? -acc : acc) * ieps;
+        const int erow = rs ? eR : eL;
+        if (rs == cs) {
+            atomic_add_f64(J + ((size_t)erow * bslots) * bs * bs + (size_t)iv * bs + cw, val);
+        } else {
+            const int slot = 1 + (rs ? fnr : fnl);
+            J[((size_t)erow * bslots + slot) * bs * bs + (size_t)iv * bs + cw] = val;
+        }
+    }
+}
+
+// ---- boundary faces --------------------------------------------------------
+template <int D, int P, class PH>
+__global__ __launch_bounds__(256) void jac_bface_kernel(DevSpace S, PhysParams ph, const double* __restrict__ bval,
+                                                        const double* __restrict__ U, double* __restrict__ J,
+                                                        int bslots) {
+    constexpr int N = P + 1;
+    constexpr int NB = Pow<N, D>::value;
+    constexpr int NFQ = Pow<N, D - 1>::value;
+    constexpr int M = PH::M;
+    constexpr int NBM = NB * M;
+    __shared__ PointGeo<D> sgeo[NFQ];
+    __shared__ double su[NFQ][M], sg[NFQ][M * D];
+    __shared__ double snrm[NFQ][D], sh[NFQ], sws[NFQ];
+    __shared__ int spt[NFQ][D];
+    __shared__ double sQ0[NFQ][M];
+    __shared__ double sdQ[NBM][NFQ][M];
+    __shared__ double sred[256];
+    __shared__ double seps;
+    const int b = blockIdx.x;
+    const int4 rec = S.fbdy[b];
+    const int eL = rec.x, info = rec.y, bcflag = rec.z;
+    const int fnl = info_fnl(info), bctype = info_bctype(info);
+    const bool general = !(bctype == ICE_BC_DIRICHLET || bctype == ICE_BC_NEUMANN || bctype == ICE_BC_EXTRAPOLATION ||
+                           bctype == ICE_BC_SPACETIME_FUTURE);
+    if (bctype == ICE_BC_NEUMANN || (general && !PH::IMPLEMENTS_BCS)) return;  // no dependence on u
+    const int tid = threadIdx.x;
+    if (tid < NFQ) {
+        const int q = tid;
+        double XL[1 << D][D];
+        load_vertices<D>(S.X, eL, XL);
+        double x[D], nrm[D];
+        const double sqrtg = face_geometry<D, N>(S, XL, fnl, q, x, nrm);
+        int pt[D];
+        const signed char* fp = S.fpt + ((fnl * S.norient + 0) * NFQ + q) * D;
+        double xi[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            pt[d] = fp[d];
+            spt[q][d] = pt[d];
+            xi[d] = S.tab[pt[d]];
+        }
+        make_geo<D>(XL, xi, sgeo[q]);
+        double u[M], g[M * D];
+        trace_eval<D, N, M, false>(S, U + (size_t)eL * NB * M, pt, XL, nrm, u, g, nullptr);
+#pragma unroll
+        for (int v = 0; v < M; ++v) su[q][v] = u[v];
+#pragma unroll
+        for (int k = 0; k < M * D; ++k) sg[q][k] = g[k];
+        double cL[D];
+        centroid<D>(XL, cL);
+#pragma unroll
+        for (int d = 0; d < D; ++d) snrm[q][d] = nrm[d];
+        sh[q] = ddg_h_boundary<D>(nrm, x, cL);
+        sws[q] = face_weight<N, D>(S, q) * sqrtg;
+        double Q[M];
+        boundary_point_flux<D, PH>(ph, bctype, bcflag, u, g, bval + ((size_t)b * NFQ + q) * M, nrm, sh[q], sws[q], Q,
+                                   nullptr);
+#pragma unroll
+        for (int v = 0; v < M; ++v) sQ0[q][v] = Q[v];
+    }
+    __syncthreads();
+    double sL = 0;
+    for (int r = tid; r < NBM; r += blockDim.x) {
+        const int i = r / M, v = r % M;
+        double acc = 0;
+        for (int q = 0; q < NFQ; ++q) {
+            double phi, gref[D];
+            basis_at<D, N>(S.tab, spt[q], i, phi, gref, nullptr);
+            acc += sQ0[q][v] * phi;
+        }
+        sL += acc * acc;
+    }
+    sred[tid] = sL;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) sred[tid] += sred[tid + s];
+        __syncthreads();
+    }
+    if (tid == 0) seps = fmax(SQRT_EPS, SQRT_EPS * sqrt(sred[0]));  // form_petsc_jacobian.hpp:122
+    __syncthreads();
+    const double eps = seps;
+    for (int it = tid; it < NBM * NFQ; it += blockDim.x) {
+        const int c = it / NFQ, q = it % NFQ;
+        const int i = c / M, w = c % M;
+        double phi, gref[D], gx[D];
+        basis_at<D, N>(S.tab, spt[q], i, phi, gref, nullptr);
+        phys_basis<D>(sgeo[q], gref, nullptr, snrm[q], gx, nullptr);
+        double u[M], g[M * D];
+#pragma unroll
+        for (int v = 0; v < M; ++v) u[v] = su[q][v];
+#pragma unroll
+        for (int k = 0; k < M * D; ++k) g[k] = sg[q][k];
+        u[w] += eps * phi;
+#pragma unroll
+        for (int d = 0; d < D; ++d) g[w * D + d] += eps * gx[d];
+        double Q[M];
+        boundary_point_flux<D, PH>(ph, bctype, bcflag, u, g, bval + ((size_t)b * NFQ + q) * M, snrm[q], sh[q], sws[q],
+                                   Q, nullptr);
+#pragma unroll
+        for (int v = 0; v < M; ++v) sdQ[c][q][v] = Q[v] - sQ0[q][v];
+    }
+    __syncthreads();
+    const size_t bs = NBM;
+    const double ieps = 1.0 / eps;
+    for (int ent = tid; ent < NBM * NBM; ent += blockDim.x) {
+        const int r = ent / NBM, c = ent % NBM;
+        const int i = r / M, v = r % M;
+        double acc = 0;
+#pragma unroll
+        for (int q = 0; q < NFQ; ++q) {
+            double phi, gref[D];
+            basis_at<D, N>(S.tab, spt[q], i, phi, gref, nullptr);
+            acc += sdQ[c][q][v] * phi;
+        }
+        atomic_add_f64(J + ((size_t)eL * bslots) * bs * bs + (size_t)r * bs + c, acc * ieps);
+    }
+}
+
+// ---- element volume blocks ---------------------------------------------------
+template <int D, int P, class PH>
+__global__ __launch_bounds__(256) void jac_elem_kernel(DevSpace S, PhysParams ph, const double* __restrict__ U,
+                                                       double* __restrict__ J, int bslots) {
+    constexpr int N = P + 1;
+    constexpr int NB = Pow<N, D>::value;
+    constexpr int M = PH::M;
+    constexpr int NBM = NB * M;
+    __shared__ double sdFdu[NB][M][D][M];
+    __shared__ double sdFdg[NB][M][D][M][D];
+    __shared__ double sgx[NB][NB][D];  // [q][i][d] physical basis gradients
+    __shared__ double sdw[NB];
+    const int e = blockIdx.x;
+    const int tid = threadIdx.x;
+    const double* xs = S.tab;
+    const double* L0 = S.tab + (N + 2);
+    const double* wgl = L0 + 3 * (N + 2) * N;
+    const double* Ue = U + (size_t)e * NBM;
+    if (tid < NB) {
+        const int q = tid;
+        int qd[D], pt[D];
+        {
+            int r = q;
+#pragma unroll
+            for (int d = D - 1; d >= 0; --d) { qd[d] = r % N; r /= N; }
+        }
+        double xi[D], wq = 1.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            pt[d] = qd[d];
+            xi[d] = xs[qd[d]];
+            wq = (d == 0) ? wgl[qd[d]] : wq * wgl[qd[d]];
+        }
+        double X[1 << D][D];
+        load_vertices<D>(S.X, e, X);
+        double Jm[D][D], Hm[D][D][D], A[D][D];
+        q1_map<D>(X, xi, nullptr, Jm, Hm);
+        const double det = det_adj<D>(Jm, A);
+        const double detg = (det == 0.0) ? 1.0 : det;
+        sdw[q] = fmax(0.0, det) * wq;
+        double u[M] = {}, g[M * D] = {};
+        for (int i = 0; i < NB; ++i) {
+            double phi, gref[D];
+            basis_at<D, N>(S.tab, pt, i, phi, gref, nullptr);
+            double gxi[D];
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                double s = 0;
+#pragma unroll
+                for (int k = 0; k < D; ++k) s += gref[k] * A[k][j];
+                gxi[j] = s / detg;
+                sgx[q][i][j] = gxi[j];
+            }
+#pragma unroll
+            for (int v = 0; v < M; ++v) {
+                const double c = Ue[i * M + v];
+                u[v] += c * phi;
+#pragma unroll
+                for (int j = 0; j < D; ++j) g[v * D + j] += c * gxi[j];
+            }
+        }
+        double F[M * D], Fp[M * D];
+        PH::flux(ph, u, g, F);
+        double fro = 0;
+#pragma unroll
+        for (int k = 0; k < M * D; ++k) fro += F[k] * F[k];
+        const double eps = fmax(SQRT_EPS, SQRT_EPS * sqrt(fro));  // conservation_law.hpp:366-369
+        const double ieps = 1.0 / eps;
+#pragma unroll
+        for (int w = 0; w < M; ++w) {
+            const double uo = u[w];
+            u[w] += eps;
+            PH::flux(ph, u, g, Fp);
+#pragma unroll
+            for (int v = 0; v < M; ++v)
+#pragma unroll
+                for (int d = 0; d < D; ++d) sdFdu[q][v][d][w] = (Fp[v * D + d] - F[v * D + d]) * ieps;
+            u[w] = uo;
+#pragma unroll
+            for (int jd = 0; jd < D; ++jd) {
+                const double go = g[w * D + jd];
+                g[w * D + jd] += eps;
+                PH::flux(ph, u, g, Fp);
+#pragma unroll
+                for (int v = 0; v < M; ++v)
+#pragma unroll
+                    for (int d = 0; d < D; ++d) sdFdg[q][v][d][w][jd] = (Fp[v * D + d] - F[v * D + d]) * ieps;
+                g[w * D + jd] = go;
+            }
+        }
+    }
+    __syncthreads();
+    const size_t bs = NBM;
+    for (int ent = tid; ent < NBM * NBM; ent += blockDim.x) {
+        const int r = ent / NBM, c = ent % NBM;
+        const int i = r / M, v = r % M, j = c / M, w = c % M;
+        double acc = 0;
+        for (int q = 0; q < NB; ++q) {
+            int qd[D];
+            int t = q;
+#pragma unroll
+            for (int d = D - 1; d >= 0; --d) { qd[d] = t % N; t /= N; }
+            int jd[D];
+            t = j;
+#pragma unroll
+            for (int d = D - 1; d >= 0; --d) { jd[d] = t % N; t /= N; }
+            double phij = 1.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) phij *= L0[qd[d] * N + jd[d]];
+            double s = 0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                double t2 = sdFdu[q][v][d][w] * phij;
+#pragma unroll
+                for (int ee = 0; ee < D; ++ee) t2 += sdFdg[q][v][d][w][ee] * sgx[q][j][ee];
+                s += sgx[q][i][d] * t2;
+            }
+            acc += s * sdw[q];
+        }
+        atomic_add_f64(J + ((size_t)e * bslots) * bs * bs + (size_t)r * bs + c, acc);
+    }
+}
+
+__global__ void fill_zero_kernel(size_t n, double* __restrict__ a) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = 0.0;
+}
+
+__global__ void to_fp32_kernel(size_t n, const double* __restrict__ a, float* __restrict__ b) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
+
+template <int D, int P, class PH>
+int launch_jac(const DevSpace& S, const PhysParams& ph, const DiscDev& dd, const double* U, double* J,
+               hipStream_t st) {
+    const int bslots = 2 * D + 1;
+    if (S.nfi > 0) jac_face_kernel<D, P, PH><<<S.nfi, 256, 0, st>>>(S, ph, U, J, bslots);
+    if (S.nfb > 0) jac_bface_kernel<D, P, PH><<<S.nfb, 256, 0, st>>>(S, ph, dd.bval, U, J, bslots);
+    jac_elem_kernel<D, P, PH><<<S.nelem, 256, 0, st>>>(S, ph, U, J, bslots);
+    return (int)hipGetLastError();
+}
+
+template <class PH, int D>
+int jac_order(int p, const DevSpace& S, const PhysParams& ph, const DiscDev& dd, const double* U, double* J,
+              hipStream_t st) {
+    switch (p) {
+    case 1: return launch_jac<D, 1, PH>(S, ph, dd, U, J, st);
+    case 2: return launch_jac<D, 2, PH>(S, ph, dd, U, J, st);
+    case 3: return launch_jac<D, 3, PH>(S, ph, dd, U, J, st);
+    case 4: return launch_jac<D, 4, PH>(S, ph, dd, U, J, st);
+    default: return -1;
+    }
+}
+
+int jacobian_fp64(int ndim, int p, int flux, const DevSpace& S, const PhysParams& ph, const DiscDev& dd,
+                  const double* U, double* J, size_t nvals, hipStream_t st) {
+    fill_zero_kernel<<<4096, 256, 0, st>>>(nvals, J);
+    if (flux == ICE_FLUX_BURGERS) {
+        if (ndim == 1) return jac_order<Burgers<1>, 1>(p, S, ph, dd, U, J, st);
+        if (ndim == 2) return jac_order<Burgers<2>, 2>(p, S, ph, dd, U, J, st);
+    } else if (flux == ICE_FLUX_NAVIER_STOKES) {
+        if (ndim == 1) return jac_order<NavierStokes<1, true>, 1>(p, S, ph, dd, U, J, st);
+        if (ndim == 2) return jac_order<NavierStokes<2, true>, 2>(p, S, ph, dd, U, J, st);
+    } else if (flux == ICE_FLUX_EULER) {
+        if (ndim == 1) return jac_order<NavierStokes<1, false>, 1>(p, S, ph, dd, U, J, st);
+        if (ndim == 2) return jac_order<NavierStokes<2, false>, 2>(p, S, ph, dd, U, J, st);
+    }
+    return -1;
+}
+
+int convert_fp32(size_t n, const double* a, float* b, hipStream_t st) {
+    to_fp32_kernel<<<4096, 256, 0, st>>>(n, a, b);
+    return (int)hipGetLastError();
+}
+
+}  // namespace ice

@@ -24,6 +24,7 @@ struct DevSpace {
     const double* tab;       // xs[n+2] | L0[(n+2)n] | L1 | L2 | wgl[n]
     const signed char* fpt;  // [2*ndim][norient][nfq][ndim] S-indices of trace points
     const signed char* fvo;  // [2*ndim][4] face vertex order (element-local vertex ids)
+    const int* jcols;        // [nelem][2*ndim+1] Jacobian block columns (-1: boundary face)
 };
 
 // face info word: fnl | fnr << 3 | orient << 6 | bctype << 9 | pdim << 13
@@ -38,6 +39,7 @@ struct DiscDev {
     const double* srcload;    // [nelem][nbasis][neq] source load vector, or null
     double* Q;                // [nfi + nfb][nfq][neq] weighted normal face fluxes
     double* Pic;              // [nfi + nfb][nfq][neq][ndim] DDGIC lifting vectors, or null
+    double* wave_part;        // per-block [max wave speed, max viscosity] when tracking, else null
     unsigned long long* wave; // wave-speed bookkeeping (bits of non-negative doubles), or null:
                               // [0] running max |dF/du| (Burgers lambda_max, burgers.hpp:89)
                               // [1] value at the last quadrature point (NS lambda_max, navier_stokes.hpp:930)
@@ -82,5 +84,16 @@ int residual_fp64(int ndim, int p, int flux, const DevSpace& S, const PhysParams
 // face quadrature points and the source load vector sum_q s(x_q) phi_i w detJ.
 int setup_disc_tables(int ndim, int p, int neq, const DevSpace& S, const ice_field* fields, double* bval,
                       double* srcload, hipStream_t st);
+
+// Finite-difference element-block Jacobian into J[nelem][2*ndim+1][bs][bs]
+// (zeroed first); ndim <= 2, no DDGIC.
+int jacobian_fp64(int ndim, int p, int flux, const DevSpace& S, const PhysParams& ph, const DiscDev& dd,
+                  const double* U, double* J, size_t nvals, hipStream_t st);
+
+// Block-sparse products y = J x (or y = b - J x when b is given), blocks
+// [nelem][nslots][bs][bs] in T, vectors in V.
+template <class T, class V>
+int bsr_apply(int nelem, int nslots, int bs, const int* cols, const T* J, const V* x, const V* b, V* y,
+              hipStream_t st);
 
 }  // namespace ice

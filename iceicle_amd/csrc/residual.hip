@@ -20,373 +20,12 @@
 // the result is deterministic and needs no atomics.
 #include <hip/hip_runtime.h>
 
+#include "dg_common.h"
 #include "fields.h"
 #include "kernels.h"
 #include "physics_dev.h"
 
 namespace ice {
-
-template <int B, int E>
-struct Pow { static constexpr int value = B * Pow<B, E - 1>::value; };
-template <int B>
-struct Pow<B, 0> { static constexpr int value = 1; };
-
-// ---- geometry helpers (geometry order 1 hypercubes) ----------------------
-
-// Q1 map at xi: point, Jacobian J[i][a] = dx_i/dxi_a and Hessian H[i][a][b]
-// (hypercube::transform/jacobian/hessian, HypercubeTransformations.hpp:52-161).
-template <int D>
-__device__ __forceinline__ void q1_map(const double (&X)[1 << D][D], const double* xi, double* x, double (&J)[D][D],
-                                       double (&H)[D][D][D]) {
-    double N[D][2], dN[D][2];
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        N[d][0] = 0.5 * (1 - xi[d]);
-        N[d][1] = 1.0 - N[d][0];
-        dN[d][0] = -0.5;
-        dN[d][1] = 0.5;
-    }
-#pragma unroll
-    for (int i = 0; i < D; ++i) {
-        if (x) x[i] = 0;
-#pragma unroll
-        for (int a = 0; a < D; ++a) {
-            J[i][a] = 0;
-#pragma unroll
-            for (int b = 0; b < D; ++b) H[i][a][b] = 0;
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < (1 << D); ++k) {
-        int bit[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) bit[d] = (k >> (D - 1 - d)) & 1;
-        double phi = 1.0;
-#pragma unroll
-        for (int d = 0; d < D; ++d) phi *= N[d][bit[d]];
-        double g[D];
-#pragma unroll
-        for (int a = 0; a < D; ++a) {
-            double t = 1.0;
-#pragma unroll
-            for (int d = 0; d < D; ++d) t *= (d == a) ? dN[d][bit[d]] : N[d][bit[d]];
-            g[a] = t;
-        }
-#pragma unroll
-        for (int i = 0; i < D; ++i) {
-            if (x) x[i] += phi * X[k][i];
-#pragma unroll
-            for (int a = 0; a < D; ++a) J[i][a] += g[a] * X[k][i];
-        }
-#pragma unroll
-        for (int a = 0; a < D; ++a)
-#pragma unroll
-            for (int b = 0; b < D; ++b) {
-                if (a == b) continue;
-                double t = 1.0;
-#pragma unroll
-                for (int d = 0; d < D; ++d) t *= (d == a || d == b) ? dN[d][bit[d]] : N[d][bit[d]];
-#pragma unroll
-                for (int i = 0; i < D; ++i) H[i][a][b] += t * X[k][i];
-            }
-    }
-}
-
-template <int D>
-__device__ __forceinline__ double det_adj(const double (&J)[D][D], double (&A)[D][D]) {
-    if constexpr (D == 1) {
-        A[0][0] = 1.0;
-        return J[0][0];
-    } else if constexpr (D == 2) {
-        A[0][0] = J[1][1]; A[0][1] = -J[0][1];
-        A[1][0] = -J[1][0]; A[1][1] = J[0][0];
-        return J[0][0] * J[1][1] - J[0][1] * J[1][0];
-    } else {
-        A[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-        A[0][1] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
-        A[0][2] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
-        A[1][0] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-        A[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
-        A[1][2] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
-        A[2][0] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-        A[2][1] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
-        A[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-        return J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-               J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-    }
-}
-
-template <int D>
-__device__ __forceinline__ void load_vertices(const double* __restrict__ Xall, int e, double (&X)[1 << D][D]) {
-    const double* p = Xall + (size_t)e * (1 << D) * D;
-#pragma unroll
-    for (int k = 0; k < (1 << D); ++k)
-#pragma unroll
-        for (int d = 0; d < D; ++d) X[k][d] = p[k * D + d];
-}
-
-// Face map from the left element's face vertices at face quadrature point
-// fq: physical point x, unit normal and sqrt(det(J^T J)).
-// (HypercubeFace::transform/Jacobian + calc_ortho/normalize/rootRiemannMetric;
-//  1D: PointFace conventions, HypercubeTransformations.hpp:1358-1456.)
-template <int D, int N>
-__device__ __forceinline__ double face_geometry(const DevSpace& S, const double (&X)[1 << D][D], int fnl, int fq,
-                                                double* x, double* nrm) {
-    if constexpr (D == 1) {
-        const int v = S.fvo[fnl * 4 + 0];
-        x[0] = X[v][0];
-        nrm[0] = (fnl == 0) ? -1.0 : 1.0;
-        return 1.0;
-    } else {
-        constexpr int TD = D - 1;
-        constexpr int NFV = 1 << TD;
-        const double* xs = S.tab;
-        double s[TD];
-        {
-            int r = fq;
-#pragma unroll
-            for (int k = TD - 1; k >= 0; --k) { s[k] = xs[r % N]; r /= N; }
-        }
-        double Jf[D][TD];
-#pragma unroll
-        for (int i = 0; i < D; ++i) {
-            x[i] = 0;
-#pragma unroll
-            for (int t = 0; t < TD; ++t) Jf[i][t] = 0;
-        }
-#pragma unroll
-        for (int k = 0; k < NFV; ++k) {
-            const int v = S.fvo[fnl * 4 + k];
-            double psi = 1.0, dpsi[TD];
-#pragma unroll
-            for (int t = 0; t < TD; ++t) {
-                const int bit = (k >> (TD - 1 - t)) & 1;
-                const double n0 = 0.5 * (1 - s[t]);
-                psi *= bit ? (1.0 - n0) : n0;
-            }
-#pragma unroll
-            for (int t = 0; t < TD; ++t) {
-                double g = 1.0;
-#pragma unroll
-                for (int u = 0; u < TD; ++u) {
-                    const int bit = (k >> (TD - 1 - u)) & 1;
-                    const double n0 = 0.5 * (1 - s[u]);
-                    g *= (u == t) ? (bit ? 0.5 : -0.5) : (bit ? (1.0 - n0) : n0);
-                }
-                dpsi[t] = g;
-            }
-#pragma unroll
-            for (int i = 0; i < D; ++i) {
-                x[i] += psi * X[v][i];
-#pragma unroll
-                for (int t = 0; t < TD; ++t) Jf[i][t] += dpsi[t] * X[v][i];
-            }
-        }
-        double sg;
-        if constexpr (D == 2) {
-            nrm[0] = Jf[1][0];
-            nrm[1] = -Jf[0][0];
-            sg = sqrt(Jf[0][0] * Jf[0][0] + Jf[1][0] * Jf[1][0]);
-        } else {
-            nrm[0] = Jf[1][0] * Jf[2][1] - Jf[2][0] * Jf[1][1];
-            nrm[1] = Jf[2][0] * Jf[0][1] - Jf[0][0] * Jf[2][1];
-            nrm[2] = Jf[0][0] * Jf[1][1] - Jf[1][0] * Jf[0][1];
-            double g00 = 0, g01 = 0, g11 = 0;
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                g00 += Jf[i][0] * Jf[i][0];
-                g01 += Jf[i][0] * Jf[i][1];
-                g11 += Jf[i][1] * Jf[i][1];
-            }
-            sg = sqrt(g00 * g11 - g01 * g01);
-        }
-        double nn = 0;
-#pragma unroll
-        for (int d = 0; d < D; ++d) nn += nrm[d] * nrm[d];
-        nn = sqrt(nn);
-#pragma unroll
-        for (int d = 0; d < D; ++d) nrm[d] /= nn;
-        return sg;
-    }
-}
-
-// Trace of one element at a tensor point of S (indices pt[D]): value u[M],
-// physical gradient gx[M][D] and, if HESS, the normal Hessian term
-// hn[M][d] = sum_j (d2u/dx_j dx_d) n_j of the physical Hessian
-// (PhysDomainEval + eval_phys_hess_basis, finite_element.hpp:328-398,497-564).
-template <int D, int N, int M, bool HESS>
-__device__ __forceinline__ void trace_eval(const DevSpace& S, const double* __restrict__ Ue, const int* pt,
-                                           const double (&X)[1 << D][D], const double* nrm, double* u, double* gx,
-                                           double* hn) {
-    constexpr int NH = D * (D + 1) / 2;
-    const double* xs = S.tab;
-    const double* L0 = S.tab + (N + 2);
-    const double* L1 = L0 + (N + 2) * N;
-    const double* L2 = L1 + (N + 2) * N;
-    double a0[D][N], a1[D][N], a2[D][N];
-    double xi[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        xi[d] = xs[pt[d]];
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            a0[d][i] = L0[pt[d] * N + i];
-            a1[d][i] = L1[pt[d] * N + i];
-            a2[d][i] = HESS ? L2[pt[d] * N + i] : 0.0;
-        }
-    }
-    double gr[M][D], hr[M][NH];
-#pragma unroll
-    for (int v = 0; v < M; ++v) {
-        u[v] = 0;
-#pragma unroll
-        for (int d = 0; d < D; ++d) gr[v][d] = 0;
-#pragma unroll
-        for (int h = 0; h < NH; ++h) hr[v][h] = 0;
-    }
-    constexpr int NB = Pow<N, D>::value;
-#pragma unroll
-    for (int i = 0; i < NB; ++i) {
-        int id[D];
-        {
-            int r = i;
-#pragma unroll
-            for (int d = D - 1; d >= 0; --d) { id[d] = r % N; r /= N; }
-        }
-        double c0 = 1.0, cg[D], ch[NH];
-#pragma unroll
-        for (int d = 0; d < D; ++d) c0 *= a0[d][id[d]];
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
-            double t = 1.0;
-#pragma unroll
-            for (int d = 0; d < D; ++d) t *= (d == k) ? a1[d][id[d]] : a0[d][id[d]];
-            cg[k] = t;
-        }
-        if constexpr (HESS) {
-            int h = 0;
-#pragma unroll
-            for (int a = 0; a < D; ++a)
-#pragma unroll
-                for (int b = a; b < D; ++b) {
-                    double t = 1.0;
-#pragma unroll
-                    for (int d = 0; d < D; ++d) {
-                        if (a == b) t *= (d == a) ? a2[d][id[d]] : a0[d][id[d]];
-                        else t *= (d == a || d == b) ? a1[d][id[d]] : a0[d][id[d]];
-                    }
-                    ch[h++] = t;
-                }
-        }
-#pragma unroll
-        for (int v = 0; v < M; ++v) {
-            const double c = Ue[i * M + v];
-            u[v] += c * c0;
-#pragma unroll
-            for (int k = 0; k < D; ++k) gr[v][k] += c * cg[k];
-            if constexpr (HESS) {
-#pragma unroll
-                for (int h = 0; h < NH; ++h) hr[v][h] += c * ch[h];
-            }
-        }
-    }
-    double J[D][D], Hm[D][D][D], A[D][D];
-    q1_map<D>(X, xi, nullptr, J, Hm);
-    double det = det_adj<D>(J, A);
-    det = (det == 0.0) ? 1.0 : det;
-    const double idet = 1.0 / det;
-#pragma unroll
-    for (int v = 0; v < M; ++v)
-#pragma unroll
-        for (int j = 0; j < D; ++j) {
-            double s = 0;
-#pragma unroll
-            for (int k = 0; k < D; ++k) s += gr[v][k] * A[k][j];
-            gx[v * D + j] = s * idet;
-        }
-    if constexpr (HESS) {
-        const double idet2 = idet * idet;
-#pragma unroll
-        for (int v = 0; v < M; ++v) {
-            double rhs[D][D];
-            {
-                int h = 0;
-#pragma unroll
-                for (int a = 0; a < D; ++a)
-#pragma unroll
-                    for (int b = a; b < D; ++b) {
-                        rhs[a][b] = hr[v][h];
-                        rhs[b][a] = hr[v][h];
-                        ++h;
-                    }
-            }
-#pragma unroll
-            for (int a = 0; a < D; ++a)
-#pragma unroll
-                for (int b = 0; b < D; ++b)
-#pragma unroll
-                    for (int k = 0; k < D; ++k) rhs[a][b] -= Hm[k][a][b] * gx[v * D + k];
-            // Hx[a][b] = sum_{k,l} A[k][b] A[l][a] rhs[l][k] / det^2 ; hn[d] = sum_j Hx[j][d] n_j
-            double T[D][D];  // T[l][b] = sum_k rhs[l][k] A[k][b]
-#pragma unroll
-            for (int l = 0; l < D; ++l)
-#pragma unroll
-                for (int b = 0; b < D; ++b) {
-                    double s = 0;
-#pragma unroll
-                    for (int k = 0; k < D; ++k) s += rhs[l][k] * A[k][b];
-                    T[l][b] = s;
-                }
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                double s = 0;
-#pragma unroll
-                for (int j = 0; j < D; ++j) {
-                    double hx = 0;
-#pragma unroll
-                    for (int l = 0; l < D; ++l) hx += A[l][j] * T[l][d];
-                    s += hx * nrm[j];
-                }
-                hn[v * D + d] = s * idet2;
-            }
-        }
-    }
-}
-
-__device__ __forceinline__ int info_fnl(int info) { return info & 7; }
-__device__ __forceinline__ int info_fnr(int info) { return (info >> 3) & 7; }
-__device__ __forceinline__ int info_orient(int info) { return (info >> 6) & 7; }
-__device__ __forceinline__ int info_bctype(int info) { return (info >> 9) & 15; }
-__device__ __forceinline__ int info_pdim(int info) { return (info >> 13) & 3; }
-
-template <int D>
-__device__ __forceinline__ void centroid(const double (&X)[1 << D][D], double* c) {
-    const double w = 1.0 / (1 << D);
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        double s = 0;
-#pragma unroll
-        for (int k = 0; k < (1 << D); ++k) s += w * X[k][d];
-        c[d] = s;
-    }
-}
-
-template <int N, int D>
-__device__ __forceinline__ double face_weight(const DevSpace& S, int fq) {
-    const double* wgl = S.tab + (N + 2) + 3 * (N + 2) * N;
-    if constexpr (D == 1) {
-        return 1.0;
-    } else {
-        double w = 1.0;
-        int idx[D - 1];
-        int r = fq;
-#pragma unroll
-        for (int k = D - 2; k >= 0; --k) { idx[k] = r % N; r /= N; }
-#pragma unroll
-        for (int k = 0; k < D - 1; ++k) w = (k == 0) ? wgl[idx[k]] : w * wgl[idx[k]];
-        return w;
-    }
-}
 
 // ---- interior faces ------------------------------------------------------
 template <int D, int P, class PH>
@@ -420,26 +59,18 @@ __device__ __forceinline__ void face_interior(const DevSpace& S, const PhysParam
     double uL[M], uR[M], gL[M * D], gR[M * D], hL[M * D], hR[M * D];
     trace_eval<D, N, M, true>(S, U + (size_t)eL * NB * M, ptL, XL, nrm, uL, gL, hL);
     trace_eval<D, N, M, true>(S, U + (size_t)eR * NB * M, ptR, XR, nrm, uR, gR, hR);
-    double fadv[M], fvis[M], gddg[M * D], uavg[M];
-    PH::conv(ph, uL, uR, nrm, fadv);
-    double h = 0;
-#pragma unroll
-    for (int d = 0; d < D; ++d) h += nrm[d] * ((x[d] - cL[d]) + (cR[d] - x[d]));
-    h = copysign(fmax(fabs(h), 2.220446049250313e-16), h);
-#pragma unroll
-    for (int v = 0; v < M; ++v) {
-        const double jump = uR[v] - uL[v];
-#pragma unroll
-        for (int d = 0; d < D; ++d)
-            gddg[v * D + d] = ph.beta0 * jump / h * nrm[d] + 0.5 * (gL[v * D + d] + gR[v * D + d]) +
-                              ph.beta1 * h * (hR[v * D + d] - hL[v * D + d]);
-        uavg[v] = 0.5 * (uL[v] + uR[v]);
-    }
-    PH::diff(ph, uavg, gddg, nrm, fvis);
+    const double h = ddg_h_interior<D>(nrm, x, cL, cR);
     const double ws = w * sqrtg;
     double* Qf = Q + ((size_t)f * NFQ + q) * M;
+    {
+        double Qv[M];
+        interior_point_flux<D, PH>(ph, uL, gL, hL, uR, gR, hR, nrm, h, ws, Qv);
 #pragma unroll
-    for (int v = 0; v < M; ++v) Qf[v] = fvis[v] * ws - fadv[v] * ws;
+        for (int v = 0; v < M; ++v) Qf[v] = Qv[v];
+    }
+    double uavg[M];
+#pragma unroll
+    for (int v = 0; v < M; ++v) uavg[v] = 0.5 * (uL[v] + uR[v]);
     if constexpr (PH::HAS_HOMOGENEITY) {
         if (Pic) {
             double G[M * D * M * D];
@@ -498,67 +129,40 @@ __device__ __forceinline__ void face_boundary(const DevSpace& S, const PhysParam
     const double sqrtg = face_geometry<D, N>(S, XL, fnl, q, x, nrm);
     const double ws = face_weight<N, D>(S, q) * sqrtg;
     const double* bv = bval + ((size_t)b * NFQ + q) * M;
-    if (bctype == ICE_BC_NEUMANN) {
-        double g[M], fv[M];
+    double uL[M] = {}, gL[M * D] = {};
+    if (bctype != ICE_BC_NEUMANN) {
+        int pt[D];
+        const signed char* fp = S.fpt + ((fnl * S.norient + 0) * NFQ + q) * D;
 #pragma unroll
-        for (int v = 0; v < M; ++v) g[v] = bv[v];
-        PH::neumann(ph, g, fv);
-#pragma unroll
-        for (int v = 0; v < M; ++v) Qf[v] = fv[v] * ws;
-        return;
+        for (int d = 0; d < D; ++d) pt[d] = fp[d];
+        trace_eval<D, N, M, false>(S, U + (size_t)eL * NB * M, pt, XL, nrm, uL, gL, nullptr);
     }
-    int pt[D];
-    const signed char* fp = S.fpt + ((fnl * S.norient + 0) * NFQ + q) * D;
+    double cL[D];
+    centroid<D>(XL, cL);
+    const double hb = ddg_h_boundary<D>(nrm, x, cL);
+    double Qv[M], uB[M];
+    boundary_point_flux<D, PH>(ph, bctype, bcflag, uL, gL, bv, nrm, hb, ws, Qv, uB);
 #pragma unroll
-    for (int d = 0; d < D; ++d) pt[d] = fp[d];
-    double uL[M], gL[M * D];
-    trace_eval<D, N, M, false>(S, U + (size_t)eL * NB * M, pt, XL, nrm, uL, gL, nullptr);
-    double fadv[M], fvis[M], gddg[M * D], uavg[M], uB[M];
-    if (bctype == ICE_BC_EXTRAPOLATION || bctype == ICE_BC_SPACETIME_FUTURE) {
-        PH::conv(ph, uL, uL, nrm, fadv);
-        PH::diff(ph, uL, gL, nrm, fvis);
-    } else {
-        if (bctype == ICE_BC_DIRICHLET) {
+    for (int v = 0; v < M; ++v) Qf[v] = Qv[v];
+    if constexpr (PH::HAS_HOMOGENEITY) {
+        if (Pf && bctype == ICE_BC_DIRICHLET) {
+            double uavg[M], G[M * D * M * D];
 #pragma unroll
-            for (int v = 0; v < M; ++v) uB[v] = bv[v];
-        } else {
-            PH::apply_bc(ph, uL, gL, nrm, bctype, bcflag, uB);
-        }
-        double cL[D];
-        centroid<D>(XL, cL);
-        double h = 0;
+            for (int v = 0; v < M; ++v) uavg[v] = 0.5 * (uL[v] + uB[v]);
+            PH::homogeneity(ph, uavg, G);
 #pragma unroll
-        for (int d = 0; d < D; ++d) h += fabs(nrm[d] * (x[d] - cL[d]));
-        h = copysign(fmax(fabs(h), 2.220446049250313e-16), h);
-        PH::conv(ph, uL, uB, nrm, fadv);
+            for (int v = 0; v < M; ++v)
 #pragma unroll
-        for (int v = 0; v < M; ++v) {
-            const double jump = uB[v] - uL[v];
+                for (int s = 0; s < D; ++s) {
+                    double acc = 0;
 #pragma unroll
-            for (int d = 0; d < D; ++d) gddg[v * D + d] = ph.beta0_bdy * jump / h * nrm[d] + gL[v * D + d];
-            uavg[v] = 0.5 * (uL[v] + uB[v]);
-        }
-        PH::diff(ph, uavg, gddg, nrm, fvis);
-        if constexpr (PH::HAS_HOMOGENEITY) {
-            if (Pf && bctype == ICE_BC_DIRICHLET) {
-                double G[M * D * M * D];
-                PH::homogeneity(ph, uavg, G);
+                    for (int k = 0; k < D; ++k)
 #pragma unroll
-                for (int v = 0; v < M; ++v)
-#pragma unroll
-                    for (int s = 0; s < D; ++s) {
-                        double acc = 0;
-#pragma unroll
-                        for (int k = 0; k < D; ++k)
-#pragma unroll
-                            for (int r = 0; r < M; ++r) acc += G[((v * D + k) * M + r) * D + s] * nrm[k] * (uB[r] - uL[r]);
-                        Pf[v * D + s] = ph.sigma_ic * acc * ws;
-                    }
-            }
+                        for (int r = 0; r < M; ++r) acc += G[((v * D + k) * M + r) * D + s] * nrm[k] * (uB[r] - uL[r]);
+                    Pf[v * D + s] = ph.sigma_ic * acc * ws;
+                }
         }
     }
-#pragma unroll
-    for (int v = 0; v < M; ++v) Qf[v] = fvis[v] * ws - fadv[v] * ws;
 }
 
 // Interior and boundary faces in one launch: blocks [0, nbi) take interior
@@ -586,7 +190,8 @@ template <int D, int P, class PH>
 __global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph, const double* __restrict__ srcload,
                                                       const double* __restrict__ U, const double* __restrict__ Q,
                                                       const double* __restrict__ Pic, double* __restrict__ R,
-                                                      unsigned long long* __restrict__ wave) {
+                                                      unsigned long long* __restrict__ wave,
+                                                      double* __restrict__ wave_part) {
     using T = ElemTile<D, P>;
     constexpr int N = T::N, NB = T::NB, EPB = T::EPB;
     constexpr int NFQ = Pow<N, D - 1>::value;
@@ -704,17 +309,24 @@ __global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph,
                 Gq[v * D + a] = s * fac;
             }
     }
-    if (wave) {  // wavefront max, inactive lanes contribute 0
+    __shared__ double red[2 * 4];
+    if (wave) {  // block max into wave_part (inactive lanes contribute 0); folded by wave_reduce_kernel
         for (int off = 32; off > 0; off >>= 1) {
             lam_w = fmax(lam_w, __shfl_xor(lam_w, off));
             vis_w = fmax(vis_w, __shfl_xor(vis_w, off));
         }
         if ((threadIdx.x & 63) == 0) {
-            atomicMax(&wave[0], (unsigned long long)__double_as_longlong(lam_w));
-            atomicMax(&wave[2], (unsigned long long)__double_as_longlong(vis_w));
+            red[2 * (threadIdx.x >> 6)] = lam_w;
+            red[2 * (threadIdx.x >> 6) + 1] = vis_w;
         }
     }
     __syncthreads();
+    if (wave && threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { a = fmax(a, red[2 * w]); b = fmax(b, red[2 * w + 1]); }
+        wave_part[2 * blockIdx.x] = a;
+        wave_part[2 * blockIdx.x + 1] = b;
+    }
     if (!active) return;
     // phase 2: (element, basis function kd)
     double r[M];
@@ -822,6 +434,33 @@ __global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph,
     double* Re = R + ((size_t)e * NB + k) * M;
 #pragma unroll
     for (int v = 0; v < M; ++v) Re[v] = r[v];
+}
+
+// Fold the per-block maxima of one residual into the running maxima
+// (lambda_max of BurgersFlux never resets, burgers.hpp:89; visc_max, navier_stokes.hpp:964-966).
+__global__ void wave_reduce_kernel(const double* __restrict__ part, int nblk, unsigned long long* __restrict__ wave) {
+    __shared__ double sa[256], sb[256];
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
+        a = fmax(a, part[2 * i]);
+        b = fmax(b, part[2 * i + 1]);
+    }
+    sa[threadIdx.x] = a;
+    sb[threadIdx.x] = b;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sa[threadIdx.x] = fmax(sa[threadIdx.x], sa[threadIdx.x + s]);
+            sb[threadIdx.x] = fmax(sb[threadIdx.x], sb[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double ra = fmax(__longlong_as_double((long long)wave[0]), sa[0]);
+        const double rb = fmax(__longlong_as_double((long long)wave[2]), sb[0]);
+        wave[0] = (unsigned long long)__double_as_longlong(ra);
+        wave[2] = (unsigned long long)__double_as_longlong(rb);
+    }
 }
 
 // ---- one-time tables ----------------------------------------------------
@@ -1005,7 +644,8 @@ int launch_residual(const DevSpace& S, const PhysParams& ph, const DiscDev& dd, 
     if (ev) (void)hipEventRecord(ev[1], st);
     using T = ElemTile<D, P>;
     const int nblk = (S.nelem + T::EPB - 1) / T::EPB;
-    element_kernel<D, P, PH><<<nblk, 256, 0, st>>>(S, ph, d.srcload, U, d.Q, d.Pic, R, d.wave);
+    element_kernel<D, P, PH><<<nblk, 256, 0, st>>>(S, ph, d.srcload, U, d.Q, d.Pic, R, d.wave, d.wave_part);
+    if (d.wave) wave_reduce_kernel<<<1, 256, 0, st>>>(d.wave_part, nblk, d.wave);
     if (ev) (void)hipEventRecord(ev[3], st);
     return (int)hipGetLastError();
 }

@@ -50,6 +50,9 @@ _SIGS = {
     "ice_stepper_solve": ([P, P, P, C.POINTER(C.c_longlong), C.POINTER(C.c_double)], C.c_int),
     "ice_stepper_status": ([P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                             C.POINTER(C.c_int)], C.c_int),
+    "ice_jacobian_layout": ([P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+    "ice_jacobian": ([P, P, P, P, P], C.c_int),
+    "ice_jacobian_apply": ([P, P, P, P, P], C.c_int),
 }
 
 
@@ -251,3 +254,38 @@ class ExplicitStepper:
         if getattr(self, "h", None):
             lib().ice_stepper_destroy(self.h)
             self.h = None
+
+
+class BlockJacobian:
+    """Element-block Jacobian of a discretization on the device
+    (form_petsc_jacobian_fd, form_petsc_jacobian.hpp:55-317).
+
+    `blocks` is a float64 CUDA tensor [nelem][nslots][bs][bs]; slot 0 is the
+    diagonal block, slot 1+j couples an element with the element across its
+    local face j; `cols[e][s]` names that element (-1 on a boundary face)."""
+
+    def __init__(self, disc):
+        import torch
+        self.disc = disc
+        ns, bs = C.c_int(), C.c_int()
+        check(lib().ice_jacobian_layout(disc.h, C.byref(ns), C.byref(bs), None))
+        self.nslots, self.bs = ns.value, bs.value
+        nelem = disc.space.nelem
+        self.cols = np.empty((nelem, self.nslots), dtype=np.int32)
+        check(lib().ice_jacobian_layout(disc.h, C.byref(ns), C.byref(bs),
+                                        self.cols.ctypes.data_as(C.POINTER(C.c_int))))
+        self.blocks = torch.empty((nelem, self.nslots, self.bs, self.bs), dtype=torch.float64, device="cuda")
+
+    def assemble(self, u, res=None, stream=None):
+        """blocks <- dR/du at u; res <- R(u) when given."""
+        _require_cuda(u, "u")
+        if res is not None:
+            _require_cuda(res, "res")
+        check(lib().ice_jacobian(self.disc.h, _ptr(u), _ptr(res) if res is not None else None,
+                                 _ptr(self.blocks), stream or current_stream()))
+
+    def apply(self, x, y, stream=None):
+        """y <- J x."""
+        _require_cuda(x, "x")
+        _require_cuda(y, "y")
+        check(lib().ice_jacobian_apply(self.disc.h, _ptr(self.blocks), _ptr(x), _ptr(y), stream or current_stream()))

@@ -227,6 +227,29 @@ int ice_project(ice_disc* disc, const ice_field* f, double* u_dev, void* stream)
 /* L2 error of u against an analytic field (l2_error.hpp:37-94).  Synchronous. */
 int ice_l2_error(ice_disc* disc, const ice_field* f, const double* u_dev, double* err, void* stream);
 
+/* ---- element-block Jacobian (form_petsc_jacobian_fd DG part,
+ *      form_petsc_jacobian.hpp:55-317, with the element block of
+ *      ConservationLawDDG::domain_integral_jacobian, conservation_law.hpp:316-432)
+ *
+ * Block-sparse rows, one per element, nslots = 2*ndim + 1 blocks per row:
+ * slot 0 is the diagonal block, slot 1+j couples the element with the element
+ * across its local face j.  Blocks are bs x bs, bs = nbasis*neq, row-major
+ * [i*neq+v][j*neq+w] like the reference's dense element blocks; rows and
+ * columns are the fe_layout_right offsets of the two elements.  A slot whose
+ * local face is a boundary face has column -1 and an all-zero block.  On a
+ * periodic direction with fewer than three elements two slots of one row can
+ * name the same column; their blocks then add. */
+int ice_jacobian_layout(const ice_disc* disc, int* nslots, int* block_size, int* cols /* [nelem][nslots], or NULL */);
+/* Finite-difference blocks at u (device) into blocks_dev[nelem][nslots][bs][bs]
+ * (FP64), and the residual into r_dev unless it is NULL.  The face blocks
+ * use the reference's eps = max(sqrt(eps_mach), sqrt(eps_mach) * max over the
+ * face's element residual norms), the element blocks the per-quadrature-point
+ * eps of domain_integral_jacobian.  ndim <= 2 and sigma_ic = 0 only
+ * (ICE_ERR_UNSUPPORTED otherwise). */
+int ice_jacobian(ice_disc* disc, const double* u_dev, double* r_dev, double* blocks_dev, void* stream);
+/* y = J x for blocks in the layout above (FP64 blocks, FP64 vectors). */
+int ice_jacobian_apply(const ice_disc* disc, const double* blocks_dev, const double* x_dev, double* y_dev, void* stream);
+
 /* ---- explicit time integration (RK3SSP ssp_rk3.hpp:106-233, RK3TVD
  *      tvd_rk3.hpp:107-270, ExplicitEuler explicit_euler.hpp:87-160) ----- */
 #define ICE_STEP_RK3_SSP 0

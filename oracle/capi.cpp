@@ -34,6 +34,8 @@ struct ProblemBase {
     virtual void mass_apply(const double* r, double* out) const = 0;
     virtual void rk_solve(double* u, int kind, int compat, int use_cfl, double dt, double cfl, int use_tfinal,
                           double tfinal, long long ntime, long long* nsteps, double* time) = 0;
+    // blocks in BSR slot order: per row e, slot 0 = (e,e), slot 1+j = neighbour across local face j
+    virtual void jacobian(const double* u, double* res, int* cols, double* blocks) = 0;
 };
 
 template <int D, class PF, class CF, class DF>
@@ -145,6 +147,26 @@ struct Problem final : ProblemBase {
     }
     void mass_apply(const double* r, double* out) const override {
         const_cast<Problem*>(this)->mi().apply_add(fs, neq, r, out);
+    }
+    void jacobian(const double* u, double* res, int* cols, double* blocks) override {
+        std::map<std::pair<int, int>, std::vector<double>> B;
+        fd_jacobian(fs, disc, u, res, B);
+        const int E = fs.mesh.nelem, S = 2 * D + 1, bs = fs.nb * neq;
+        std::vector<int> nbr((size_t)E * S, -1);
+        for (int e = 0; e < E; ++e) nbr[(size_t)e * S] = e;
+        for (const auto* list : {&fs.interior, &fs.periodic})
+            for (const auto& t : *list) {
+                nbr[(size_t)t.elemL * S + 1 + t.fnl] = t.elemR;
+                nbr[(size_t)t.elemR * S + 1 + t.fnr] = t.elemL;
+            }
+        for (int e = 0; e < E; ++e)
+            for (int sl = 0; sl < S; ++sl) {
+                const int c = nbr[(size_t)e * S + sl];
+                cols[(size_t)e * S + sl] = c;
+                double* dst = blocks + ((size_t)e * S + sl) * bs * bs;
+                auto it = (c >= 0) ? B.find({e, c}) : B.end();
+                for (int k = 0; k < bs * bs; ++k) dst[k] = (it != B.end() && (sl == 0 || c != e)) ? it->second[k] : 0.0;
+            }
     }
     // RK3SSP/RK3TVD/ExplicitEuler ::solve (ssp_rk3.hpp:208-233, tvd_rk3.hpp:240-270,
     // explicit_euler.hpp:150-170): an initial residual, then steps until the
@@ -265,6 +287,9 @@ int ice_oracle_trace_points(void* h, double* xiL, double* xiR) { ORACLE_TRY(stat
 int ice_oracle_face_geometry(void* h, double* x, double* n, double* sg) { ORACLE_TRY(static_cast<Handle*>(h)->prob->face_geometry(x, n, sg)) }
 int ice_oracle_qp_coords(void* h, double* xq) { ORACLE_TRY(static_cast<Handle*>(h)->prob->qp_coords(xq)) }
 int ice_oracle_mass_apply(void* h, const double* r, double* out) { ORACLE_TRY(static_cast<Handle*>(h)->prob->mass_apply(r, out)) }
+int ice_oracle_jacobian(void* h, const double* u, double* res, int* cols, double* blocks) {
+    ORACLE_TRY(static_cast<Handle*>(h)->prob->jacobian(u, res, cols, blocks))
+}
 int ice_oracle_rk_solve(void* h, double* u, int kind, int compat, int use_cfl, double dt, double cfl, int use_tfinal,
                         double tfinal, long long ntime, long long* nsteps, double* time) {
     ORACLE_TRY(static_cast<Handle*>(h)->prob->rk_solve(u, kind, compat, use_cfl, dt, cfl, use_tfinal, tfinal, ntime, nsteps, time))

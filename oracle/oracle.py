@@ -49,6 +49,7 @@ def lib():
             ("ice_oracle_ns_flux", [C.POINTER(abi.DiscParams), C.c_int, dp, dp, dp]),
             ("ice_oracle_vanleer", [C.POINTER(abi.DiscParams), C.c_int, dp, dp, dp, dp]),
             ("ice_oracle_trace_ref", [C.c_int, C.c_int, C.c_int, dp, dp]),
+            ("ice_oracle_jacobian", [P, dp, dp, ip, dp]),
             ("ice_oracle_rk_solve", [P, dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                                      C.c_double, C.c_longlong, C.POINTER(C.c_longlong), C.POINTER(C.c_double)]),
         ]:
@@ -145,6 +146,18 @@ class Oracle:
                                          cfl or 0.0, int(tfinal is not None), tfinal or 0.0, ntime,
                                          C.byref(n), C.byref(t)))
         return u, n.value, t.value
+
+    def jacobian(self, u):
+        """(res, cols[nelem][2 ndim + 1], blocks[nelem][2 ndim + 1][bs][bs]) by the
+        reference's finite-difference block Jacobian."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        S = 2 * self.ndim + 1
+        bs = self.nbasis * self.neq
+        res = np.empty(self.size)
+        cols = np.empty((self.nelem, S), dtype=np.int32)
+        blocks = np.empty((self.nelem, S, bs, bs))
+        _check(lib().ice_oracle_jacobian(self.h, _dp(u), _dp(res), _ip(cols), _dp(blocks)))
+        return res, cols, blocks
 
     def cfl_dt(self, cfl):
         return lib().ice_oracle_cfl_dt(self.h, cfl)

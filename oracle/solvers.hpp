@@ -10,6 +10,8 @@
 //  form_petsc_jacobian_fd DG part (form_petsc_jacobian.hpp:55-317, fd_utils.hpp:10-14),
 //  domain_integral_jacobian (conservation_law.hpp:316-432).
 #pragma once
+#include <map>
+
 #include "disc.hpp"
 
 namespace oracle {
@@ -171,6 +173,143 @@ void explicit_euler_step(const FESpace<D>& fs, const Disc& disc, const MassInver
     form_residual(fs, disc, u, res.data());
     minv.apply_add(fs, neq, res.data(), r1.data());
     for (size_t k = 0; k < N; ++k) u[k] += dt * r1[k];
+}
+
+// ---------------------------------------------------------------------------
+// Block Jacobian by finite differences (form_petsc_jacobian_fd DG part,
+// form_petsc_jacobian.hpp:55-317, with scale_fd_epsilon, fd_utils.hpp:10-14)
+// and the element block from domain_integral_jacobian (conservation_law.hpp:316-432).
+// Blocks are (nb*neq)^2 row-major [i*neq+v][j*neq+w], keyed by (row elem, col elem).
+
+template <int D, class Disc>
+void domain_integral_jacobian(const FESpace<D>& fs, const Disc& disc, int e, const double* U, double* jac) {
+    constexpr int m = Disc::neq;
+    const int nb = fs.nb, bs = nb * m;
+    const auto g = fs.geo(e);
+    std::vector<double> gx(nb * D);
+    const double sqeps = std::sqrt(std::numeric_limits<double>::epsilon());
+    for (int q = 0; q < fs.nq; ++q) {
+        const double w = fs.quad.w[q];
+        double x[D], J[D][D], H[D][D][D];
+        g.eval(fs.q1, fs.quad.pts[q].data(), x, J, H);
+        double detJ = std::max(0.0, determinant<D>(J));
+        const double* bi = &fs.B[q * nb];
+        phys_grad<D>(nb, J, &fs.G[q * nb * D], gx.data());
+        double u[m] = {}, gu[m * D];
+        for (int i = 0; i < nb; ++i)
+            for (int v = 0; v < m; ++v) u[v] += U[i * m + v] * bi[i];
+        Disc::contract(nb, U, gx.data(), D, gu);
+        double F[m * D], Fp[m * D];
+        disc.phys_flux(u, gu, F);
+        double fro = 0;
+        for (int k = 0; k < m * D; ++k) fro += F[k] * F[k];
+        const double eps = std::max(sqeps, sqeps * std::sqrt(fro));
+        double dFdu[m][D][m], dFdg[m][D][m][D];
+        for (int jw = 0; jw < m; ++jw) {
+            const double uo = u[jw];
+            u[jw] += eps;
+            disc.phys_flux(u, gu, Fp);
+            for (int ie = 0; ie < m; ++ie)
+                for (int id = 0; id < D; ++id) dFdu[ie][id][jw] = (Fp[ie * D + id] - F[ie * D + id]) / eps;
+            u[jw] = uo;
+            for (int jd = 0; jd < D; ++jd) {
+                const double go = gu[jw * D + jd];
+                gu[jw * D + jd] += eps;
+                disc.phys_flux(u, gu, Fp);
+                for (int ie = 0; ie < m; ++ie)
+                    for (int id = 0; id < D; ++id) dFdg[ie][id][jw][jd] = (Fp[ie * D + id] - F[ie * D + id]) / eps;
+                gu[jw * D + jd] = go;
+            }
+        }
+        for (int it = 0; it < nb; ++it)
+            for (int ie = 0; ie < m; ++ie)
+                for (int id = 0; id < D; ++id)
+                    for (int jd = 0; jd < nb; ++jd)
+                        for (int je = 0; je < m; ++je) {
+                            double& a = jac[(it * m + ie) * bs + jd * m + je];
+                            a += dFdu[ie][id][je] * bi[jd] * gx[it * D + id] * detJ * w;
+                            for (int jj = 0; jj < D; ++jj)
+                                a += dFdg[ie][id][je][jj] * gx[jd * D + jj] * gx[it * D + id] * detJ * w;
+                        }
+    }
+}
+
+template <int D, class Disc>
+void fd_jacobian(const FESpace<D>& fs, const Disc& disc, const double* u, double* res,
+                 std::map<std::pair<int, int>, std::vector<double>>& blocks) {
+    constexpr int m = Disc::neq;
+    const int nb = fs.nb, bs = nb * m;
+    const double sqeps = std::sqrt(std::numeric_limits<double>::epsilon());
+    auto blk = [&](int r, int c) -> std::vector<double>& {
+        auto& b = blocks[{r, c}];
+        if (b.empty()) b.assign((size_t)bs * bs, 0.0);
+        return b;
+    };
+    auto nrm2 = [](const std::vector<double>& v) {
+        double s = 0;
+        for (double x : v) s += x * x;
+        return std::sqrt(s);
+    };
+    std::fill(res, res + (size_t)fs.mesh.nelem * bs, 0.0);
+    std::vector<double> uL(bs), uR(bs), rL(bs), rR(bs), rLp(bs), rRp(bs);
+    for (const auto& t : fs.boundary) {
+        if (fs.coupled_periodic && t.bctype == BC_PERIODIC) continue;
+        std::copy(u + (size_t)t.elemL * bs, u + (size_t)(t.elemL + 1) * bs, uL.begin());
+        std::copy(u + (size_t)t.elemR * bs, u + (size_t)(t.elemR + 1) * bs, uR.begin());
+        std::fill(rL.begin(), rL.end(), 0.0);
+        disc.boundary_integral(fs, t, uL.data(), uR.data(), rL.data());
+        for (int k = 0; k < bs; ++k) res[(size_t)t.elemL * bs + k] += rL[k];
+        const double eps = std::max(sqeps, sqeps * nrm2(rL));
+        auto& J = blk(t.elemL, t.elemL);
+        for (int c = 0; c < bs; ++c) {
+            const double old = uL[c];
+            uL[c] += eps;
+            std::fill(rLp.begin(), rLp.end(), 0.0);
+            disc.boundary_integral(fs, t, uL.data(), uR.data(), rLp.data());
+            for (int r = 0; r < bs; ++r) J[(size_t)r * bs + c] += (rLp[r] - rL[r]) / eps;
+            uL[c] = old;
+        }
+    }
+    auto interior = [&](const typename FESpace<D>::Trace& t) {
+        std::copy(u + (size_t)t.elemL * bs, u + (size_t)(t.elemL + 1) * bs, uL.begin());
+        std::copy(u + (size_t)t.elemR * bs, u + (size_t)(t.elemR + 1) * bs, uR.begin());
+        std::fill(rL.begin(), rL.end(), 0.0);
+        std::fill(rR.begin(), rR.end(), 0.0);
+        disc.trace_integral(fs, t, uL.data(), uR.data(), rL.data(), rR.data());
+        for (int k = 0; k < bs; ++k) res[(size_t)t.elemL * bs + k] += rL[k];
+        for (int k = 0; k < bs; ++k) res[(size_t)t.elemR * bs + k] += rR[k];
+        const double eps = std::max(sqeps, sqeps * std::max(nrm2(rL), nrm2(rR)));
+        for (int side = 0; side < 2; ++side) {
+            std::vector<double>& us = side ? uR : uL;
+            const int ec = side ? t.elemR : t.elemL;
+            std::vector<double> JL((size_t)bs * bs, 0.0), JR((size_t)bs * bs, 0.0);
+            for (int c = 0; c < bs; ++c) {
+                const double old = us[c];
+                us[c] += eps;
+                std::fill(rLp.begin(), rLp.end(), 0.0);
+                std::fill(rRp.begin(), rRp.end(), 0.0);
+                disc.trace_integral(fs, t, uL.data(), uR.data(), rLp.data(), rRp.data());
+                for (int r = 0; r < bs; ++r) {
+                    JL[(size_t)r * bs + c] += (rLp[r] - rL[r]) / eps;
+                    JR[(size_t)r * bs + c] += (rRp[r] - rR[r]) / eps;
+                }
+                us[c] = old;
+            }
+            auto& BL = blk(t.elemL, ec);
+            auto& BR = blk(t.elemR, ec);
+            for (size_t k = 0; k < JL.size(); ++k) { BL[k] += JL[k]; BR[k] += JR[k]; }
+        }
+    };
+    for (const auto& t : fs.interior) interior(t);
+    for (const auto& t : fs.periodic) interior(t);
+    for (int e = 0; e < fs.mesh.nelem; ++e) {
+        std::copy(u + (size_t)e * bs, u + (size_t)(e + 1) * bs, uL.begin());
+        std::fill(rL.begin(), rL.end(), 0.0);
+        disc.domain_integral(fs, e, uL.data(), rL.data());
+        auto& J = blk(e, e);
+        domain_integral_jacobian(fs, disc, e, uL.data(), J.data());
+        for (int k = 0; k < bs; ++k) res[(size_t)e * bs + k] += rL[k];
+    }
 }
 
 }  // namespace oracle
