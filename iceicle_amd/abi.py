@@ -167,3 +167,37 @@ def space_desc(order):
     s.order = order
     s.geometry_order = 1
     return s
+
+
+LS_NONE = 0
+LS_WOLFE = 1
+LS_CORRIGAN = 2
+PREC_FP64 = 0
+PREC_FP32 = 1
+NEWTON_MAX_HIST = 64
+
+
+class NewtonOpts(C.Structure):
+    _fields_ = [("tau_abs", C.c_double), ("tau_rel", C.c_double), ("kmax", C.c_int), ("linesearch", C.c_int),
+                ("ls_kmax", C.c_int), ("ls_alpha_initial", C.c_double), ("ls_alpha_max", C.c_double),
+                ("ls_c1", C.c_double), ("ls_c2", C.c_double), ("ls_alpha_min", C.c_double),
+                ("jacobian_precision", C.c_int), ("gmres_restart", C.c_int), ("gmres_maxit", C.c_int),
+                ("linear_rtol", C.c_double), ("refine_max", C.c_int), ("verbose", C.c_int)]
+
+
+class NewtonReport(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("r0", C.c_double), ("rnorm", C.c_double),
+                ("linear_iterations", C.c_longlong), ("refinements", C.c_int), ("singular_blocks", C.c_int),
+                ("seconds", C.c_double), ("jacobian_seconds", C.c_double), ("linear_seconds", C.c_double),
+                ("linesearch_seconds", C.c_double), ("nhist", C.c_int),
+                ("rhist", C.c_double * NEWTON_MAX_HIST), ("alpha", C.c_double * NEWTON_MAX_HIST),
+                ("linear_its", C.c_int * NEWTON_MAX_HIST)]
+
+    def as_dict(self):
+        k = self.nhist
+        return dict(iterations=self.iterations, converged=bool(self.converged), r0=self.r0, rnorm=self.rnorm,
+                    linear_iterations=self.linear_iterations, refinements=self.refinements,
+                    singular_blocks=self.singular_blocks, seconds=self.seconds,
+                    jacobian_seconds=self.jacobian_seconds, linear_seconds=self.linear_seconds,
+                    linesearch_seconds=self.linesearch_seconds, rhist=list(self.rhist[:k]),
+                    alpha=list(self.alpha[:k]), linear_its=list(self.linear_its[:k]))

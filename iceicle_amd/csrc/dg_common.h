@@ -397,6 +397,27 @@ __device__ __forceinline__ void interior_point_flux(const PhysParams& ph, const 
     for (int v = 0; v < M; ++v) Q[v] = fvis[v] * ws - fadv[v] * ws;
 }
 
+// The same flux from the sum gs = gL + gR of the two elements' shares of the
+// DDG gradient (each (grad u)/2 -/+ beta1 h (Hess u) n, see trace_kernel).
+template <int D, class PH>
+__device__ __forceinline__ void interior_point_flux_split(const PhysParams& ph, const double* uL, const double* uR,
+                                                          const double* gs, const double* nrm, double h, double ws,
+                                                          double* Q) {
+    constexpr int M = PH::M;
+    double fadv[M], fvis[M], gddg[M * D], uavg[M];
+    PH::conv(ph, uL, uR, nrm, fadv);
+#pragma unroll
+    for (int v = 0; v < M; ++v) {
+        const double jump = uR[v] - uL[v];
+#pragma unroll
+        for (int d = 0; d < D; ++d) gddg[v * D + d] = ph.beta0 * jump / h * nrm[d] + gs[v * D + d];
+        uavg[v] = 0.5 * (uL[v] + uR[v]);
+    }
+    PH::diff(ph, uavg, gddg, nrm, fvis);
+#pragma unroll
+    for (int v = 0; v < M; ++v) Q[v] = fvis[v] * ws - fadv[v] * ws;
+}
+
 // Weighted normal flux at one boundary face point (boundaryIntegral,
 // conservation_law.hpp:631-1176).  bv: Dirichlet values or Neumann data at
 // the point; hb: the guarded boundary DDG distance.  Returns false when the

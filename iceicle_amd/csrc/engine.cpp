@@ -116,11 +116,11 @@ struct ice_space {
     std::vector<double> tab;          // host copy of the S tables
     std::vector<int4> fint, fbdy;
     std::vector<int> efaces;
-    std::vector<signed char> fpt, fvo;
+    std::vector<signed char> fpt, fvo, fperm, finv;
     std::vector<double> X;
     std::vector<int> jcols;           // [nelem][2*ndim+1] block columns of the element-block Jacobian
     ice::DevSpace dev{};
-    void* dmem[8] = {};
+    void* dmem[12] = {};
     ~ice_space() {
         for (void* p : dmem)
             if (p) (void)hipFree(p);
@@ -140,6 +140,7 @@ struct ice_disc {
     ice_field* d_fields = nullptr;
     double* d_Q = nullptr;
     double* d_Pic = nullptr;
+    double* d_T = nullptr;
     double* d_bval = nullptr;
     double* d_srcload = nullptr;
     unsigned long long* d_wave = nullptr;
@@ -149,7 +150,7 @@ struct ice_disc {
     ~ice_disc() {
         for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
         for (void* p : {(void*)d_fields, (void*)d_Q, (void*)d_Pic, (void*)d_bval, (void*)d_srcload, (void*)d_wave,
-                        (void*)d_wave_part})
+                        (void*)d_wave_part, (void*)d_T})
             if (p) (void)hipFree(p);
     }
 };
@@ -157,6 +158,31 @@ struct ice_disc {
 namespace ice {
 
 const DevSpace& space_dev(const ice_space* s) { return s->dev; }
+
+int disc_residual(ice_disc* d, const double* u, double* r, hipStream_t st) {
+    const ice_space* s = d->space;
+    return residual_fp64(s->ndim, s->p, d->prm.flux, s->dev, d->ph, d->dd, u, r, st, nullptr);
+}
+
+// residual (if r) and FD block Jacobian at u
+int disc_jacobian(ice_disc* d, const double* u, double* r, double* J, hipStream_t st) {
+    const ice_space* s = d->space;
+    if (r && disc_residual(d, u, r, st) != 0) return -1;
+    const size_t bs = (size_t)s->nb * d->neq;
+    const size_t nvals = (size_t)s->mesh.nelem * (2 * s->ndim + 1) * bs * bs;
+    return jacobian_fp64(s->ndim, s->p, d->prm.flux, s->dev, d->ph, d->dd, u, J, nvals, st);
+}
+
+// Jacobian layout; nonzero when the device Jacobian does not cover the discretization
+int disc_dims(const ice_disc* d, int* nelem, int* nslots, int* bs) {
+    const ice_space* s = d->space;
+    *nelem = s->mesh.nelem;
+    *nslots = 2 * s->ndim + 1;
+    *bs = s->nb * d->neq;
+    return (s->ndim > 2 || d->ddgic) ? -1 : 0;
+}
+
+const int* disc_jcols(const ice_disc* d) { return d->space->dev.jcols; }
 const ice_space* disc_space(const ice_disc* d) { return d->space; }
 int disc_neq(const ice_disc* d) { return d->neq; }
 void set_error(const std::string& m) { g_err = m; }
@@ -294,6 +320,22 @@ int ice_space_create(const ice_mesh_desc* md, const ice_space_desc* sd, ice_spac
                     ice::trace_point_indices(D, n, j, o, q, pt);
                     for (int d = 0; d < D; ++d) s->fpt[(((size_t)j * no + o) * s->nfq + q) * D + d] = (signed char)pt[d];
                 }
+        // face point q of orientation o -> the same point in the element's canonical (orientation 0) order
+        s->fperm.assign((size_t)2 * D * no * s->nfq, -1);
+        s->finv.assign((size_t)2 * D * no * s->nfq, -1);
+        for (int j = 0; j < 2 * D; ++j)
+            for (int o = 0; o < no; ++o)
+                for (int q = 0; q < s->nfq; ++q) {
+                    const signed char* a = &s->fpt[(((size_t)j * no + o) * s->nfq + q) * D];
+                    int hit = -1;
+                    for (int q2 = 0; q2 < s->nfq && hit < 0; ++q2) {
+                        const signed char* b = &s->fpt[(((size_t)j * no + 0) * s->nfq + q2) * D];
+                        if (std::memcmp(a, b, D) == 0) hit = q2;
+                    }
+                    if (hit < 0) throw ice::Error{ICE_ERR_ARG, "orientation does not permute the face points"};
+                    s->fperm[((size_t)j * no + o) * s->nfq + q] = (signed char)hit;
+                    s->finv[((size_t)j * no + o) * s->nfq + hit] = (signed char)q;
+                }
         s->fvo.assign(2 * D * 4, 0);
         for (int j = 0; j < 2 * D; ++j) {
             int v[4];
@@ -383,6 +425,18 @@ int ice_space_create(const ice_mesh_desc* md, const ice_space_desc* sd, ice_spac
         dv.fpt = (const signed char*)s->dmem[5];
         dv.fvo = (const signed char*)s->dmem[6];
         dv.jcols = (const int*)s->dmem[7];
+        s->dmem[8] = upload(s->fperm);
+        s->dmem[9] = upload(s->finv);
+        dv.fperm = (const signed char*)s->dmem[8];
+        dv.finv = (const signed char*)s->dmem[9];
+        {
+            const size_t ng = (size_t)(nfi + nfb) * s->nfq * (D + 2);
+            ICE_HIP(hipMalloc(&s->dmem[10], std::max<size_t>(ng, 1) * sizeof(double)));
+            dv.fgeo = (const double*)s->dmem[10];
+            if (ice::face_geometry_table(D, p, dv, (double*)s->dmem[10], nullptr) != 0)
+                throw ice::Error{ICE_ERR_HIP, "face geometry launch failed"};
+            ICE_HIP(hipDeviceSynchronize());
+        }
         *out = s.release();
         return ICE_OK;
     } catch (const ice::Error& e) {
@@ -502,6 +556,8 @@ int ice_disc_create(ice_space* s, const ice_disc_params* p, ice_disc** out) {
         const size_t rows = (size_t)(s->dev.nfi + s->dev.nfb) * s->nfq * d->neq;
         ICE_HIP(hipMalloc(&d->d_Q, std::max<size_t>(rows, 1) * sizeof(double)));
         if (d->ddgic) ICE_HIP(hipMalloc(&d->d_Pic, std::max<size_t>(rows * D, 1) * sizeof(double)));
+        ICE_HIP(hipMalloc(&d->d_T, std::max<size_t>(ice::trace_buffer_size(D, s->p, p->flux, s->mesh.nelem), 1) *
+                                     sizeof(double)));
         if (s->dev.nfb > 0) ICE_HIP(hipMalloc(&d->d_bval, (size_t)s->dev.nfb * s->nfq * d->neq * sizeof(double)));
         if (d->has_source) ICE_HIP(hipMalloc(&d->d_srcload, (size_t)s->mesh.nelem * s->nb * d->neq * sizeof(double)));
         if (ice::setup_disc_tables(D, s->p, d->neq, s->dev, d->d_fields, d->d_bval, d->d_srcload, nullptr) != 0)
@@ -521,6 +577,7 @@ int ice_disc_create(ice_space* s, const ice_disc_params* p, ice_disc** out) {
         d->dd.srcload = d->d_srcload;
         d->dd.Q = d->d_Q;
         d->dd.Pic = d->d_Pic;
+        d->dd.T = d->d_T;
         *out = d.release();
         return ICE_OK;
     } catch (const ice::Error& e) {

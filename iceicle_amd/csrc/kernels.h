@@ -25,6 +25,9 @@ struct DevSpace {
     const signed char* fpt;  // [2*ndim][norient][nfq][ndim] S-indices of trace points
     const signed char* fvo;  // [2*ndim][4] face vertex order (element-local vertex ids)
     const int* jcols;        // [nelem][2*ndim+1] Jacobian block columns (-1: boundary face)
+    const double* fgeo;      // [nfi + nfb][nfq][ndim + 2] unit normal, w sqrt(g), DDG h per face point
+    const signed char* fperm;  // [2*ndim][norient][nfq] face point q -> the element's canonical point
+    const signed char* finv;   // inverse of fperm
 };
 
 // face info word: fnl | fnr << 3 | orient << 6 | bctype << 9 | pdim << 13
@@ -39,6 +42,7 @@ struct DiscDev {
     const double* srcload;    // [nelem][nbasis][neq] source load vector, or null
     double* Q;                // [nfi + nfb][nfq][neq] weighted normal face fluxes
     double* Pic;              // [nfi + nfb][nfq][neq][ndim] DDGIC lifting vectors, or null
+    double* T;                // [nelem][2*ndim][nfq][nv][neq] element face traces
     double* wave_part;        // per-block [max wave speed, max viscosity] when tracking, else null
     unsigned long long* wave; // wave-speed bookkeeping (bits of non-negative doubles), or null:
                               // [0] running max |dF/du| (Burgers lambda_max, burgers.hpp:89)
@@ -84,6 +88,11 @@ int residual_fp64(int ndim, int p, int flux, const DevSpace& S, const PhysParams
 // face quadrature points and the source load vector sum_q s(x_q) phi_i w detJ.
 int setup_disc_tables(int ndim, int p, int neq, const DevSpace& S, const ice_field* fields, double* bval,
                       double* srcload, hipStream_t st);
+
+// Face geometry table of the space (see DevSpace::fgeo) and the size in
+// doubles of a discretization's face trace buffer.
+int face_geometry_table(int ndim, int p, const DevSpace& S, double* out, hipStream_t st);
+size_t trace_buffer_size(int ndim, int p, int flux, int nelem);
 
 // Finite-difference element-block Jacobian into J[nelem][2*ndim+1][bs][bs]
 // (zeroed first); ndim <= 2, no DDGIC.

@@ -95,4 +95,396 @@ template int bsr_apply<float, float>(int, int, int, const int*, const float*, co
 template int bsr_apply<float, double>(int, int, int, const int*, const float*, const double*, const double*,
                                       double*, hipStream_t);
 
+
+// ---------------------------------------------------------------------------
+// Block-Jacobi factors: in-place Gauss-Jordan inversion with partial pivoting
+// of every diagonal block (slot 0 of each block row), one workgroup per
+// block, the block held in LDS.  A block whose pivot vanishes is replaced by
+// the identity and counted in *nsingular.
+
+namespace {
+
+template <class T>
+__global__ __launch_bounds__(256) void block_inverse_kernel(int nelem, int nslots, int bs, const double* __restrict__ J,
+                                                            T* __restrict__ Dinv, int* __restrict__ nsingular) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* A = reinterpret_cast<double*>(smem_raw);  // bs * bs
+    int* perm = reinterpret_cast<int*>(A + (size_t)bs * bs);
+    __shared__ int s_piv;
+    __shared__ int s_bad;
+    const int e = blockIdx.x;
+    const int tid = threadIdx.x;
+    const double* src = J + (size_t)e * nslots * bs * bs;
+    for (int k = tid; k < bs * bs; k += blockDim.x) A[k] = src[k];
+    if (tid == 0) s_bad = 0;
+    __syncthreads();
+    for (int k = 0; k < bs; ++k) {
+        // pivot search by the first wave: largest |A[i][k]|, i >= k, lowest index on ties
+        if (tid < 64) {
+            double best = -1.0;
+            int bi = k;
+            for (int i = k + tid; i < bs; i += 64) {
+                const double a = fabs(A[(size_t)i * bs + k]);
+                if (a > best) { best = a; bi = i; }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ob = __shfl_xor(best, off);
+                const int oi = __shfl_xor(bi, off);
+                if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+            }
+            if (tid == 0) {
+                s_piv = bi;
+                perm[k] = bi;
+                if (!(best > 0.0)) s_bad = 1;
+            }
+        }
+        __syncthreads();
+        if (s_bad) break;
+        const int pr = s_piv;
+        if (pr != k)
+            for (int c = tid; c < bs; c += blockDim.x) {
+                const double t = A[(size_t)k * bs + c];
+                A[(size_t)k * bs + c] = A[(size_t)pr * bs + c];
+                A[(size_t)pr * bs + c] = t;
+            }
+        __syncthreads();
+        const double ipiv = 1.0 / A[(size_t)k * bs + k];
+        __syncthreads();
+        for (int c = tid; c < bs; c += blockDim.x) {
+            const double v = (c == k) ? 1.0 : A[(size_t)k * bs + c];
+            A[(size_t)k * bs + c] = v * ipiv;
+        }
+        __syncthreads();
+        // eliminate column k from every other row
+        for (int idx = tid; idx < bs * bs; idx += blockDim.x) {
+            const int i = idx / bs, c = idx % bs;
+            if (i == k) continue;
+            const double f = A[(size_t)i * bs + k];
+            const double pk = A[(size_t)k * bs + c];
+            if (c == k) continue;
+            A[(size_t)i * bs + c] -= f * pk;
+        }
+        __syncthreads();
+        for (int i = tid; i < bs; i += blockDim.x)
+            if (i != k) A[(size_t)i * bs + k] = -A[(size_t)i * bs + k] * A[(size_t)k * bs + k];
+        __syncthreads();
+    }
+    T* dst = Dinv + (size_t)e * bs * bs;
+    if (s_bad) {
+        for (int k = tid; k < bs * bs; k += blockDim.x) dst[k] = (T)((k / bs == k % bs) ? 1.0 : 0.0);
+        if (tid == 0) atomicAdd(nsingular, 1);
+        return;
+    }
+    // undo the row interchanges as column interchanges, last first
+    for (int k = bs - 1; k >= 0; --k) {
+        const int pr = perm[k];
+        if (pr != k)
+            for (int i = tid; i < bs; i += blockDim.x) {
+                const double t = A[(size_t)i * bs + k];
+                A[(size_t)i * bs + k] = A[(size_t)i * bs + pr];
+                A[(size_t)i * bs + pr] = t;
+            }
+        __syncthreads();
+    }
+    for (int k = tid; k < bs * bs; k += blockDim.x) dst[k] = (T)A[k];
+}
+
+// y[e] = D[e] x[e] for dense bs x bs blocks (block-Jacobi application), one
+// block row per lane group as in bsr_kernel
+template <class T, class V>
+__global__ __launch_bounds__(256) void block_diag_kernel(int nelem, int bs, int G, int epb, const T* __restrict__ D,
+                                                         const V* __restrict__ x, V* __restrict__ y) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    V* sx = reinterpret_cast<V*>(smem_raw);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wpe = 4 / epb;
+    const int le = wave / wpe, wi = wave % wpe;
+    const int e0 = blockIdx.x * epb;
+    for (int k = threadIdx.x; k < epb * bs; k += blockDim.x) {
+        const int ee = e0 + k / bs;
+        sx[k] = (ee < nelem) ? x[(size_t)ee * bs + k % bs] : (V)0;
+    }
+    __syncthreads();
+    const int e = e0 + le;
+    if (e >= nelem) return;
+    const int rpw = 64 / G;
+    const int sub = lane / G, gl = lane % G;
+    const V* xs = sx + (size_t)le * bs;
+    for (int r0 = wi * rpw; r0 < bs; r0 += wpe * rpw) {
+        const int r = r0 + sub;
+        V acc = 0;
+        if (r < bs) {
+            const T* Dr = D + ((size_t)e * bs + r) * bs;
+            for (int c = gl; c < bs; c += G) acc += (V)Dr[c] * xs[c];
+        }
+        for (int off = G >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+        if (gl == 0 && r < bs) y[(size_t)e * bs + r] = acc;
+    }
+}
+
+__global__ void convert_slots_kernel(size_t n, const double* __restrict__ a, float* __restrict__ b) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
+
+}  // namespace
+
+template <class T>
+int block_inverse(int nelem, int nslots, int bs, const double* J, T* Dinv, int* nsingular, hipStream_t st) {
+    const size_t sh = (size_t)bs * bs * sizeof(double) + (size_t)bs * sizeof(int);
+    if (sh > 160 * 1024) return -1;
+    block_inverse_kernel<T><<<nelem, 256, sh, st>>>(nelem, nslots, bs, J, Dinv, nsingular);
+    return (int)hipGetLastError();
+}
+template int block_inverse<float>(int, int, int, const double*, float*, int*, hipStream_t);
+template int block_inverse<double>(int, int, int, const double*, double*, int*, hipStream_t);
+
+template <class T, class V>
+int block_diag_apply(int nelem, int bs, const T* D, const V* x, V* y, hipStream_t st) {
+    const int G = group_width(bs);
+    const int epb = (bs >= 32) ? 1 : 4;
+    const int grid = (nelem + epb - 1) / epb;
+    block_diag_kernel<T, V><<<grid, 256, (size_t)epb * bs * sizeof(V), st>>>(nelem, bs, G, epb, D, x, y);
+    return (int)hipGetLastError();
+}
+template int block_diag_apply<float, double>(int, int, const float*, const double*, double*, hipStream_t);
+template int block_diag_apply<double, double>(int, int, const double*, const double*, double*, hipStream_t);
+
+int convert_to_fp32(size_t n, const double* a, float* b, hipStream_t st) {
+    convert_slots_kernel<<<4096, 256, 0, st>>>(n, a, b);
+    return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Vector kernels of GMRES.  Reductions are two-stage and deterministic:
+// per-workgroup partial sums in a fixed order, then one workgroup per
+// result summing the partials in a fixed order.
+
+namespace {
+
+constexpr int VEC_EPT = 8;  // entries per thread in the reduction kernels
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+    const int nw = blockDim.x >> 6;
+    for (int w = 0; w < nw; ++w) s += red[w];
+    return s;
+}
+
+// partial[blk][j] = sum_i V_j[i] w[i] for j < nv, partial[blk][nv] = w.w
+__global__ __launch_bounds__(256) void mdot_partial_kernel(size_t n, int nv, const double* __restrict__ V, size_t ldv,
+                                                           const double* __restrict__ w, double* __restrict__ part,
+                                                           int npart) {
+    __shared__ double red[4];
+    const size_t base = (size_t)blockIdx.x * blockDim.x * VEC_EPT + threadIdx.x;
+    double wr[VEC_EPT];
+#pragma unroll
+    for (int e = 0; e < VEC_EPT; ++e) {
+        const size_t i = base + (size_t)e * blockDim.x;
+        wr[e] = (i < n) ? w[i] : 0.0;
+    }
+    for (int j = 0; j <= nv; ++j) {
+        double s = 0.0;
+        if (j < nv) {
+            const double* vj = V + (size_t)j * ldv;
+#pragma unroll
+            for (int e = 0; e < VEC_EPT; ++e) {
+                const size_t i = base + (size_t)e * blockDim.x;
+                if (i < n) s += vj[i] * wr[e];
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < VEC_EPT; ++e) s += wr[e] * wr[e];
+        }
+        s = block_sum(s, red);
+        if (threadIdx.x == 0) part[(size_t)blockIdx.x * npart + j] = s;
+    }
+}
+
+// out[j] = sum_blk part[blk][j], one workgroup per j
+__global__ __launch_bounds__(256) void reduce_partials_kernel(int nblk, const double* __restrict__ part, int npart,
+                                                              double* __restrict__ out) {
+    __shared__ double red[4];
+    const int j = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += part[(size_t)b * npart + j];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) out[j] = s;
+}
+
+// w -= sum_j h[j] V_j ; partial[blk] = w.w afterwards
+__global__ __launch_bounds__(256) void maxpy_norm_kernel(size_t n, int nv, const double* __restrict__ V, size_t ldv,
+                                                         const double* __restrict__ h, double* __restrict__ w,
+                                                         double* __restrict__ part) {
+    __shared__ double red[4];
+    const size_t base = (size_t)blockIdx.x * blockDim.x * VEC_EPT + threadIdx.x;
+    double acc[VEC_EPT];
+#pragma unroll
+    for (int e = 0; e < VEC_EPT; ++e) {
+        const size_t i = base + (size_t)e * blockDim.x;
+        acc[e] = (i < n) ? w[i] : 0.0;
+    }
+    for (int j = 0; j < nv; ++j) {
+        const double hj = h[j];
+        const double* vj = V + (size_t)j * ldv;
+#pragma unroll
+        for (int e = 0; e < VEC_EPT; ++e) {
+            const size_t i = base + (size_t)e * blockDim.x;
+            if (i < n) acc[e] -= hj * vj[i];
+        }
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int e = 0; e < VEC_EPT; ++e) {
+        const size_t i = base + (size_t)e * blockDim.x;
+        if (i < n) {
+            w[i] = acc[e];
+            s += acc[e] * acc[e];
+        }
+    }
+    s = block_sum(s, red);
+    if (threadIdx.x == 0 && part) part[blockIdx.x] = s;
+}
+
+// x += sum_j y[j] V_j
+__global__ __launch_bounds__(256) void combine_kernel(size_t n, int nv, const double* __restrict__ V, size_t ldv,
+                                                      const double* __restrict__ y, double* __restrict__ x) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double s = x[i];
+        for (int j = 0; j < nv; ++j) s += y[j] * V[(size_t)j * ldv + i];
+        x[i] = s;
+    }
+}
+
+// x *= *s (or 1 / *s when inv)
+__global__ void scale_kernel(size_t n, const double* __restrict__ s, int inv, double* __restrict__ x) {
+    const double f = inv ? ((*s != 0.0) ? 1.0 / *s : 0.0) : *s;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        x[i] *= f;
+}
+
+// y = a x + b y
+__global__ void axpby_kernel2(size_t n, double a, const double* __restrict__ x, double b, double* __restrict__ y) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        y[i] = a * x[i] + (b == 0.0 ? 0.0 : b * y[i]);
+}
+
+// Arnoldi bookkeeping of step k (one thread): column k of the Hessenberg
+// matrix from the two Gram-Schmidt passes, the norm of the new vector,
+// Givens rotations, residual estimate.  sc layout: see GmresScalars.
+__global__ void arnoldi_kernel(int k, int m, const double* __restrict__ h1, const double* __restrict__ h2,
+                               double* __restrict__ sc) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double* H = sc;                       // (m+1) x m, column-major: H[j + (m+1) k]
+    double* cs = H + (size_t)(m + 1) * m;
+    double* sn = cs + m;
+    double* g = sn + m;                   // m+1
+    double* hn = g + m + 1;               // [0] = ||v_{k+1}|| before normalisation, [1] = |g_{k+1}|
+    double* col = H + (size_t)(m + 1) * k;
+    for (int j = 0; j <= k; ++j) col[j] = h1[j] + h2[j];
+    const double hk1 = sqrt(fmax(h2[k + 1], 0.0));  // h2[k+1] = squared norm after the second pass
+    col[k + 1] = hk1;
+    hn[0] = hk1;
+    for (int j = 0; j < k; ++j) {
+        const double a = col[j], b = col[j + 1];
+        col[j] = cs[j] * a + sn[j] * b;
+        col[j + 1] = -sn[j] * a + cs[j] * b;
+    }
+    const double a = col[k], b = col[k + 1];
+    const double r = hypot(a, b);
+    const double c = (r > 0.0) ? a / r : 1.0, s = (r > 0.0) ? b / r : 0.0;
+    cs[k] = c;
+    sn[k] = s;
+    col[k] = r;
+    col[k + 1] = 0.0;
+    g[k + 1] = -s * g[k];
+    g[k] = c * g[k];
+    hn[1] = fabs(g[k + 1]);
+}
+
+// back substitution H(0:k,0:k) y = g(0:k), result into y
+__global__ void hessenberg_solve_kernel(int k, int m, const double* __restrict__ sc, double* __restrict__ y) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double* H = sc;
+    const double* g = sc + (size_t)(m + 1) * m + 2 * m;
+    for (int i = k - 1; i >= 0; --i) {
+        double s = g[i];
+        for (int j = i + 1; j < k; ++j) s -= H[i + (size_t)(m + 1) * j] * y[j];
+        const double d = H[i + (size_t)(m + 1) * i];
+        y[i] = (d != 0.0) ? s / d : 0.0;
+    }
+}
+
+// g = (beta, 0, ...), beta = sqrt(*nrm2)
+__global__ void gmres_init_kernel(int m, const double* __restrict__ nrm2, double* __restrict__ sc) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double* g = sc + (size_t)(m + 1) * m + 2 * m;
+    g[0] = sqrt(fmax(*nrm2, 0.0));
+    for (int j = 1; j <= m; ++j) g[j] = 0.0;
+    double* hn = g + m + 1;
+    hn[0] = g[0];
+    hn[1] = g[0];
+}
+
+int vec_grid(size_t n) { return (int)((n + 256 * VEC_EPT - 1) / (256 * VEC_EPT)); }
+
+}  // namespace
+
+int gmres_partials(size_t n) { return vec_grid(n); }
+
+int launch_mdot(size_t n, int nv, const double* V, size_t ldv, const double* w, double* part, int npart, double* out,
+                hipStream_t st) {
+    const int nb = vec_grid(n);
+    mdot_partial_kernel<<<nb, 256, 0, st>>>(n, nv, V, ldv, w, part, npart);
+    reduce_partials_kernel<<<nv + 1, 256, 0, st>>>(nb, part, npart, out);
+    return (int)hipGetLastError();
+}
+
+int launch_maxpy_norm(size_t n, int nv, const double* V, size_t ldv, const double* h, double* w, double* part,
+                      double* out, hipStream_t st) {
+    const int nb = vec_grid(n);
+    maxpy_norm_kernel<<<nb, 256, 0, st>>>(n, nv, V, ldv, h, w, part);
+    if (out) reduce_partials_kernel<<<1, 256, 0, st>>>(nb, part, 1, out);
+    return (int)hipGetLastError();
+}
+
+int launch_norm2(size_t n, const double* w, double* part, double* out, hipStream_t st) {
+    return launch_mdot(n, 0, nullptr, 0, w, part, 1, out, st);
+}
+
+int launch_combine(size_t n, int nv, const double* V, size_t ldv, const double* y, double* x, hipStream_t st) {
+    combine_kernel<<<2048, 256, 0, st>>>(n, nv, V, ldv, y, x);
+    return (int)hipGetLastError();
+}
+
+int launch_scale(size_t n, const double* s, int inv, double* x, hipStream_t st) {
+    scale_kernel<<<2048, 256, 0, st>>>(n, s, inv, x);
+    return (int)hipGetLastError();
+}
+
+int launch_axpby2(size_t n, double a, const double* x, double b, double* y, hipStream_t st) {
+    axpby_kernel2<<<2048, 256, 0, st>>>(n, a, x, b, y);
+    return (int)hipGetLastError();
+}
+
+int launch_arnoldi(int k, int m, const double* h1, const double* h2, double* sc, hipStream_t st) {
+    arnoldi_kernel<<<1, 64, 0, st>>>(k, m, h1, h2, sc);
+    return (int)hipGetLastError();
+}
+
+int launch_hessenberg_solve(int k, int m, const double* sc, double* y, hipStream_t st) {
+    hessenberg_solve_kernel<<<1, 64, 0, st>>>(k, m, sc, y);
+    return (int)hipGetLastError();
+}
+
+int launch_gmres_init(int m, const double* nrm2, double* sc, hipStream_t st) {
+    gmres_init_kernel<<<1, 64, 0, st>>>(m, nrm2, sc);
+    return (int)hipGetLastError();
+}
+
 }  // namespace ice

@@ -1,0 +1,521 @@
+// Newton-linesearch steady solve on the device (PetscNewton::solve,
+// petsc_newton.hpp:205-329) with the device block Jacobian and a device
+// GMRES in place of PETSc KSP.  The control flow (steps, linesearch,
+// convergence test) runs on the host; every vector operation and reduction
+// runs on the caller's stream, and the host reads back only scalars.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <limits>
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/iceicle_amd.h"
+#include "engine.h"
+#include "kernels.h"
+
+namespace ice {
+// krylov.hip
+template <class T>
+int block_inverse(int nelem, int nslots, int bs, const double* J, T* Dinv, int* nsingular, hipStream_t st);
+template <class T, class V>
+int block_diag_apply(int nelem, int bs, const T* D, const V* x, V* y, hipStream_t st);
+int convert_to_fp32(size_t n, const double* a, float* b, hipStream_t st);
+int gmres_partials(size_t n);
+int launch_mdot(size_t n, int nv, const double* V, size_t ldv, const double* w, double* part, int npart, double* out,
+                hipStream_t st);
+int launch_maxpy_norm(size_t n, int nv, const double* V, size_t ldv, const double* h, double* w, double* part,
+                      double* out, hipStream_t st);
+int launch_norm2(size_t n, const double* w, double* part, double* out, hipStream_t st);
+int launch_combine(size_t n, int nv, const double* V, size_t ldv, const double* y, double* x, hipStream_t st);
+int launch_scale(size_t n, const double* s, int inv, double* x, hipStream_t st);
+int launch_axpby2(size_t n, double a, const double* x, double b, double* y, hipStream_t st);
+int launch_arnoldi(int k, int m, const double* h1, const double* h2, double* sc, hipStream_t st);
+int launch_hessenberg_solve(int k, int m, const double* sc, double* y, hipStream_t st);
+int launch_gmres_init(int m, const double* nrm2, double* sc, hipStream_t st);
+}  // namespace ice
+
+namespace {
+
+struct Fail {
+    int code;
+    std::string msg;
+};
+
+#define NT_HIP(call)                                                                               \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) throw Fail{ICE_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)}; \
+    } while (0)
+#define NT_LAUNCH(call)                                                        \
+    do {                                                                       \
+        if ((call) != 0) throw Fail{ICE_ERR_HIP, std::string("launch: ") + #call}; \
+    } while (0)
+
+double now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+namespace ice {
+
+using Phi = std::function<double(double)>;
+
+// finite_difference, nonlinear_solver_utils.hpp:74-79
+double fd_slope(const Phi& f, double eps, double x) {
+    const double p1 = f(x);
+    const double p2 = f(x + eps);
+    return (p2 - p1) / eps;
+}
+
+// zoom, nonlinear_solver_utils.hpp:82-124 (cubic interpolation between
+// alpha_lo and alpha_hi; the midpoint if it does not finish)
+double ls_zoom(const Phi& f, double lo, double hi, int kmax, double c1, double c2) {
+    const double eps = std::sqrt(std::numeric_limits<double>::epsilon());
+    for (int k = 0; k < kmax; ++k) {
+        const double phi_lo = f(lo), dphi_lo = fd_slope(f, eps, lo);
+        const double phi_hi = f(hi), dphi_hi = fd_slope(f, eps, hi);
+        const double d1 = dphi_lo + dphi_hi - 3 * (phi_lo - phi_hi) / (lo - hi);
+        double d2 = std::sqrt(std::max(d1 * d1 - dphi_lo * dphi_hi, 1e-8));
+        d2 = std::copysign(d2, hi - lo);
+        const double aj = hi - (hi - lo) * ((dphi_hi + d2 - d1) / (dphi_hi - dphi_lo + 2 * d2));
+        const double phi_aj = f(aj);
+        const double dphi_0 = fd_slope(f, eps, 0.0);
+        if (phi_aj > f(0.0) + c1 * aj * dphi_0 || phi_aj >= phi_lo) {
+            hi = aj;
+        } else {
+            const double dphi_aj = fd_slope(f, eps, aj);
+            if (std::abs(dphi_aj) <= -c2 * dphi_0) return aj;
+            if (dphi_aj * (hi - lo) >= 0) hi = lo;
+            lo = aj;
+        }
+    }
+    return 0.5 * (lo + hi);
+}
+
+// wolfe_ls, nonlinear_solver_utils.hpp:139-170
+double wolfe_linesearch(const Phi& f, const ice_newton_opts& o) {
+    const double eps = std::sqrt(std::numeric_limits<double>::epsilon());
+    double a_im1 = 0.0, a_i = o.ls_alpha_initial;
+    const double phi_0 = f(0.0);
+    const double dphi_0 = fd_slope(f, eps, 0.0);
+    for (int k = 0; k < o.ls_kmax; ++k) {
+        const double phi_i = f(a_i);
+        const double phi_im1 = f(a_im1);
+        if (phi_i > phi_0 + o.ls_c1 * a_i * dphi_0 || (phi_i >= phi_im1 && k > 0))
+            return ls_zoom(f, a_im1, a_i, o.ls_kmax, o.ls_c1, o.ls_c2);
+        const double dphi_i = fd_slope(f, eps, a_i);
+        if (std::abs(dphi_i) <= -o.ls_c2 * dphi_0) return a_i;
+        if (dphi_i >= 0) return ls_zoom(f, a_i, a_im1, o.ls_kmax, o.ls_c1, o.ls_c2);
+        const double next = 0.5 * (a_i + o.ls_alpha_max);
+        a_im1 = a_i;
+        a_i = next;
+    }
+    return a_i;
+}
+
+// corrigan_ls, nonlinear_solver_utils.hpp:172-191: bisection between
+// alpha_min and alpha_max; returns the last alpha tried
+double corrigan_linesearch(const Phi& f, const ice_newton_opts& o) {
+    double a = o.ls_alpha_min, b = o.ls_alpha_max;
+    double alpha = o.ls_alpha_initial;
+    for (int k = 0; k < o.ls_kmax; ++k) {
+        const double phi_0 = f(0.0);
+        if (k > 0) alpha = 0.5 * (a + b);
+        const double phi_star = f(alpha);
+        if (std::abs(phi_star) < std::abs(phi_0)) {
+            if (alpha == o.ls_alpha_initial) break;
+            a = alpha;
+        } else {
+            b = alpha;
+        }
+    }
+    return alpha;
+}
+
+}  // namespace ice
+
+struct ice_newton {
+    ice_disc* disc = nullptr;
+    ice_newton_opts opt{};
+    int nelem = 0, nslots = 0, bs = 0;
+    size_t n = 0, nblkvals = 0;
+    int m = 30, npart = 0, nred = 0;
+    // device memory
+    double* J64 = nullptr;
+    float* J32 = nullptr;
+    void* Dinv = nullptr;       // float or double [nelem][bs][bs]
+    double* V = nullptr;        // (m+1) x n Krylov basis
+    double* z = nullptr;        // n
+    double* t = nullptr;        // n
+    double* res = nullptr;      // n, R(u)
+    double* du = nullptr;       // n
+    double* ustep = nullptr;    // n
+    double* rwork = nullptr;    // n
+    double* xr = nullptr;       // n, refinement residual
+    double* part = nullptr;     // nred x (m+2)
+    double* h1 = nullptr;       // m+2
+    double* h2 = nullptr;       // m+2
+    double* sc = nullptr;       // Hessenberg, rotations, g, norms
+    double* y = nullptr;        // m
+    int* nsing = nullptr;
+    double* host = nullptr;     // pinned scalars
+    bool factored = false;
+    std::vector<void*> owned;
+
+    ~ice_newton() {
+        for (void* p : owned)
+            if (p) (void)hipFree(p);
+        if (host) (void)hipHostFree(host);
+    }
+    template <class T>
+    T* alloc(size_t count) {
+        void* p = nullptr;
+        NT_HIP(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+        owned.push_back(p);
+        return static_cast<T*>(p);
+    }
+
+    bool fp32() const { return opt.jacobian_precision == ICE_PREC_FP32; }
+    double* Vj(int j) const { return V + (size_t)j * n; }
+
+    double read_scalar(const double* d, hipStream_t st) {
+        NT_HIP(hipMemcpyAsync(host, d, sizeof(double), hipMemcpyDeviceToHost, st));
+        NT_HIP(hipStreamSynchronize(st));
+        return host[0];
+    }
+    double norm(const double* v, hipStream_t st) {
+        NT_LAUNCH(ice::launch_norm2(n, v, part, h1, st));
+        return std::sqrt(std::max(read_scalar(h1, st), 0.0));
+    }
+
+    // Krylov operator w = A z and preconditioner z = M^-1 v
+    void apply_A(const double* x, double* w, hipStream_t st) {
+        const int* cols = ice::disc_jcols(disc);
+        if (fp32()) NT_LAUNCH((ice::bsr_apply<float, double>(nelem, nslots, bs, cols, J32, x, nullptr, w, st)));
+        else NT_LAUNCH((ice::bsr_apply<double, double>(nelem, nslots, bs, cols, J64, x, nullptr, w, st)));
+    }
+    void apply_M(const double* v, double* out, hipStream_t st) {
+        if (fp32()) NT_LAUNCH((ice::block_diag_apply<float, double>(nelem, bs, (const float*)Dinv, v, out, st)));
+        else NT_LAUNCH((ice::block_diag_apply<double, double>(nelem, bs, (const double*)Dinv, v, out, st)));
+    }
+
+    void factor(hipStream_t st) {
+        NT_HIP(hipMemsetAsync(nsing, 0, sizeof(int), st));
+        if (fp32()) {
+            NT_LAUNCH(ice::convert_to_fp32(nblkvals, J64, J32, st));
+            NT_LAUNCH(ice::block_inverse<float>(nelem, nslots, bs, J64, (float*)Dinv, nsing, st));
+        } else {
+            NT_LAUNCH(ice::block_inverse<double>(nelem, nslots, bs, J64, (double*)Dinv, nsing, st));
+        }
+        factored = true;
+    }
+
+    // Restarted GMRES (right preconditioning, classical Gram-Schmidt with one
+    // reorthogonalisation pass): x += correction solving A x = b to
+    // ||b - A x|| <= rtol ||b|| or maxit iterations.  x must hold 0 on entry.
+    int gmres(const double* b, double* x, double rtol, int maxit, hipStream_t st, double* est) {
+        int total = 0;
+        NT_HIP(hipMemcpyAsync(Vj(0), b, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        NT_LAUNCH(ice::launch_norm2(n, Vj(0), part, h1, st));
+        NT_LAUNCH(ice::launch_gmres_init(m, h1, sc, st));
+        const double* beta_d = sc + (size_t)(m + 1) * m + 2 * m;  // g[0]
+        const double beta0 = read_scalar(beta_d, st);
+        *est = 0.0;
+        if (!(beta0 > 0.0)) return 0;
+        const double target = rtol * beta0;
+        double* hn = sc + (size_t)(m + 1) * m + 2 * m + m + 1;
+        bool done = false;
+        while (!done) {
+            NT_LAUNCH(ice::launch_scale(n, beta_d, 1, Vj(0), st));
+            int k = 0;
+            double r = 0.0;
+            for (; k < m;) {
+                apply_M(Vj(k), z, st);
+                apply_A(z, Vj(k + 1), st);
+                double* w = Vj(k + 1);
+                NT_LAUNCH(ice::launch_mdot(n, k + 1, V, n, w, part, npart, h1, st));
+                NT_LAUNCH(ice::launch_maxpy_norm(n, k + 1, V, n, h1, w, part, nullptr, st));
+                NT_LAUNCH(ice::launch_mdot(n, k + 1, V, n, w, part, npart, h2, st));
+                NT_LAUNCH(ice::launch_maxpy_norm(n, k + 1, V, n, h2, w, part, h2 + k + 1, st));
+                NT_LAUNCH(ice::launch_arnoldi(k, m, h1, h2, sc, st));
+                NT_LAUNCH(ice::launch_scale(n, hn, 1, w, st));
+                ++k;
+                ++total;
+                r = read_scalar(hn + 1, st);
+                if (!(r > target) || total >= maxit || !std::isfinite(r)) break;
+            }
+            // x += M^-1 V_k y_k
+            NT_LAUNCH(ice::launch_hessenberg_solve(k, m, sc, y, st));
+            NT_LAUNCH(ice::launch_axpby2(n, 0.0, t, 0.0, t, st));
+            NT_LAUNCH(ice::launch_combine(n, k, V, n, y, t, st));
+            apply_M(t, z, st);
+            NT_LAUNCH(ice::launch_axpby2(n, 1.0, z, 1.0, x, st));
+            *est = r / beta0;
+            if (!(r > target) || total >= maxit || !std::isfinite(r)) break;
+            // restart from the true residual of the Krylov operator
+            apply_A(x, z, st);
+            NT_HIP(hipMemcpyAsync(Vj(0), b, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+            NT_LAUNCH(ice::launch_axpby2(n, -1.0, z, 1.0, Vj(0), st));
+            NT_LAUNCH(ice::launch_norm2(n, Vj(0), part, h1, st));
+            NT_LAUNCH(ice::launch_gmres_init(m, h1, sc, st));
+            const double beta = read_scalar(beta_d, st);
+            if (!(beta > target)) {
+                *est = beta / beta0;
+                break;
+            }
+        }
+        return total;
+    }
+
+    // J x = b: GMRES on the Krylov operator, then FP64 refinement passes
+    // xr = b - J64 x, x += GMRES(xr) until ||xr|| <= linear_rtol ||b||
+    int linear_solve(const double* b, double* x, hipStream_t st, int* refinements, double* relres) {
+        const double bn = norm(b, st);
+        NT_LAUNCH(ice::launch_axpby2(n, 0.0, x, 0.0, x, st));
+        *refinements = 0;
+        if (!(bn > 0.0)) {
+            *relres = 0.0;
+            return 0;
+        }
+        const int* cols = ice::disc_jcols(disc);
+        double est = 0.0;
+        int its = gmres(b, x, opt.linear_rtol, opt.gmres_maxit, st, &est);
+        double rel = 1.0;
+        for (int pass = 0;; ++pass) {
+            NT_LAUNCH((ice::bsr_apply<double, double>(nelem, nslots, bs, cols, J64, x, b, xr, st)));
+            rel = norm(xr, st) / bn;
+            if (!fp32() || rel <= opt.linear_rtol || pass >= opt.refine_max || its >= opt.gmres_maxit ||
+                !std::isfinite(rel))
+                break;
+            NT_LAUNCH(ice::launch_axpby2(n, 0.0, rwork, 0.0, rwork, st));
+            its += gmres(xr, rwork, std::min(0.1, opt.linear_rtol / rel), opt.gmres_maxit - its, st, &est);
+            NT_LAUNCH(ice::launch_axpby2(n, 1.0, rwork, 1.0, x, st));
+            ++*refinements;
+        }
+        *relres = rel;
+        return its;
+    }
+};
+
+namespace ice {
+int newton_residual_norm(ice_newton* s, const double* u, double* r, hipStream_t st, double* nrm) {
+    if (disc_residual(s->disc, u, r, st) != 0) return -1;
+    *nrm = s->norm(r, st);
+    return 0;
+}
+}  // namespace ice
+
+extern "C" {
+
+int ice_newton_opts_default(ice_newton_opts* o) {
+    if (!o) return ICE_ERR_ARG;
+    std::memset(o, 0, sizeof(*o));
+    o->tau_abs = 1e-8;
+    o->tau_rel = 1e-8;
+    o->kmax = 100;
+    o->linesearch = ICE_LS_NONE;
+    o->ls_kmax = 20;
+    o->ls_alpha_initial = 1.0;
+    o->ls_alpha_max = 10.0;
+    o->ls_c1 = 1e-4;
+    o->ls_c2 = 0.9;
+    o->ls_alpha_min = 0.0;
+    o->jacobian_precision = ICE_PREC_FP32;
+    o->gmres_restart = 30;
+    o->gmres_maxit = 10000;
+    o->linear_rtol = 1e-5;
+    o->refine_max = 3;
+    o->verbose = 0;
+    return ICE_OK;
+}
+
+int ice_newton_create(ice_disc* d, const ice_newton_opts* opts, ice_newton** out) {
+    if (!d || !out) {
+        ice::set_error("null argument");
+        return ICE_ERR_ARG;
+    }
+    *out = nullptr;
+    ice_newton_opts o;
+    if (opts) o = *opts;
+    else ice_newton_opts_default(&o);
+    if (o.gmres_restart < 1 || o.gmres_restart > 200 || o.kmax < 0 || o.gmres_maxit < 1 ||
+        (o.jacobian_precision != ICE_PREC_FP32 && o.jacobian_precision != ICE_PREC_FP64) ||
+        (o.linesearch < ICE_LS_NONE || o.linesearch > ICE_LS_CORRIGAN)) {
+        ice::set_error("invalid Newton options");
+        return ICE_ERR_ARG;
+    }
+    try {
+        auto s = std::make_unique<ice_newton>();
+        s->disc = d;
+        s->opt = o;
+        s->m = o.gmres_restart;
+        if (ice::disc_dims(d, &s->nelem, &s->nslots, &s->bs) != 0) throw Fail{ICE_ERR_UNSUPPORTED, "no Jacobian for this discretization"};
+        s->n = (size_t)s->nelem * s->bs;
+        s->nblkvals = (size_t)s->nelem * s->nslots * s->bs * s->bs;
+        s->nred = ice::gmres_partials(s->n);
+        s->npart = s->m + 2;
+        s->J64 = s->alloc<double>(s->nblkvals);
+        if (s->fp32()) {
+            s->J32 = s->alloc<float>(s->nblkvals);
+            s->Dinv = s->alloc<float>((size_t)s->nelem * s->bs * s->bs);
+        } else {
+            s->Dinv = s->alloc<double>((size_t)s->nelem * s->bs * s->bs);
+        }
+        s->V = s->alloc<double>((size_t)(s->m + 1) * s->n);
+        for (double** p : {&s->z, &s->t, &s->res, &s->du, &s->ustep, &s->rwork, &s->xr}) *p = s->alloc<double>(s->n);
+        s->part = s->alloc<double>((size_t)s->nred * s->npart);
+        s->h1 = s->alloc<double>(s->m + 2);
+        s->h2 = s->alloc<double>(s->m + 2);
+        s->sc = s->alloc<double>((size_t)(s->m + 1) * s->m + 2 * s->m + (s->m + 1) + 2);
+        s->y = s->alloc<double>(s->m);
+        s->nsing = s->alloc<int>(1);
+        NT_HIP(hipHostMalloc((void**)&s->host, 8 * sizeof(double)));
+        *out = s.release();
+        return ICE_OK;
+    } catch (const Fail& f) {
+        ice::set_error(f.msg);
+        return f.code;
+    }
+}
+
+int ice_newton_destroy(ice_newton* s) {
+    delete s;
+    return ICE_OK;
+}
+
+int ice_newton_linear_solve(ice_newton* s, const double* u, const double* b, double* x, int* iterations,
+                            double* rel_residual, void* stream) {
+    if (!s || !b || !x) {
+        ice::set_error("null argument");
+        return ICE_ERR_ARG;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    try {
+        if (u) {
+            if (ice::disc_jacobian(s->disc, u, nullptr, s->J64, st) != 0) throw Fail{ICE_ERR_HIP, "jacobian failed"};
+            s->factor(st);
+        }
+        if (!s->factored) throw Fail{ICE_ERR_STATE, "no Jacobian assembled yet"};
+        int ref = 0;
+        double rel = 0.0;
+        const int its = s->linear_solve(b, x, st, &ref, &rel);
+        if (iterations) *iterations = its;
+        if (rel_residual) *rel_residual = rel;
+        NT_HIP(hipStreamSynchronize(st));
+        return ICE_OK;
+    } catch (const Fail& f) {
+        ice::set_error(f.msg);
+        return f.code;
+    }
+}
+
+int ice_newton_solve(ice_newton* s, double* u, ice_newton_report* rep, void* stream) {
+    if (!s || !u || !rep) {
+        ice::set_error("null argument");
+        return ICE_ERR_ARG;
+    }
+    std::memset(rep, 0, sizeof(*rep));
+    hipStream_t st = (hipStream_t)stream;
+    const ice_newton_opts& o = s->opt;
+    const size_t n = s->n;
+    try {
+        const double t0 = now();
+        double tj = 0, tl = 0, tls = 0;
+        // residual and Jacobian at u0
+        double ta = now();
+        if (ice::disc_jacobian(s->disc, u, s->res, s->J64, st) != 0) throw Fail{ICE_ERR_HIP, "jacobian failed"};
+        const double r0 = s->norm(s->res, st);
+        tj += now() - ta;
+        rep->r0 = r0;
+        double rk = r0;
+        bool have_jac = true;
+        int k = 0;
+        bool converged = false;
+        for (k = 0; k < o.kmax; ++k) {
+            ta = now();
+            if (!have_jac) {
+                if (ice::disc_jacobian(s->disc, u, nullptr, s->J64, st) != 0) throw Fail{ICE_ERR_HIP, "jacobian failed"};
+            }
+            s->factor(st);
+            NT_HIP(hipStreamSynchronize(st));
+            tj += now() - ta;
+            ta = now();
+            int ref = 0;
+            double rel = 0.0;
+            const int its = s->linear_solve(s->res, s->du, st, &ref, &rel);
+            tl += now() - ta;
+            rep->linear_iterations += its;
+            rep->refinements += ref;
+            NT_HIP(hipMemcpyAsync(s->host + 1, s->nsing, sizeof(int), hipMemcpyDeviceToHost, st));
+            NT_HIP(hipStreamSynchronize(st));
+            rep->singular_blocks = std::max(rep->singular_blocks, *reinterpret_cast<int*>(s->host + 1));
+            // linesearch on phi(alpha) = ||R(u - alpha du)||_2
+            ta = now();
+            double alpha = 1.0;
+            if (o.linesearch != ICE_LS_NONE) {
+                std::map<double, double> memo;
+                memo[0.0] = rk;
+                auto phi = [&](double a) -> double {
+                    auto it = memo.find(a);
+                    if (it != memo.end()) return it->second;
+                    NT_HIP(hipMemcpyAsync(s->ustep, u, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+                    NT_LAUNCH(ice::launch_axpby2(n, -a, s->du, 1.0, s->ustep, st));
+                    double v = 0.0;
+                    if (ice::newton_residual_norm(s, s->ustep, s->rwork, st, &v) != 0)
+                        throw Fail{ICE_ERR_HIP, "residual failed"};
+                    if (!std::isfinite(v)) v = 1e9;  // BIG_RESIDUAL, petsc_newton.hpp:258-282
+                    if (o.verbose) std::printf("linesearch: alpha = %.6e | linesearch residual = %.6e\n", a, v);
+                    memo[a] = v;
+                    return v;
+                };
+                if (o.linesearch == ICE_LS_WOLFE) alpha = ice::wolfe_linesearch(phi, o);
+                else alpha = ice::corrigan_linesearch(phi, o);
+            }
+            NT_LAUNCH(ice::launch_axpby2(n, -alpha, s->du, 1.0, u, st));
+            tls += now() - ta;
+            // residual at the new u; the Jacobian there is assembled at the
+            // start of the next step only if the solve goes on
+            ta = now();
+            if (ice::newton_residual_norm(s, u, s->res, st, &rk) != 0) throw Fail{ICE_ERR_HIP, "residual failed"};
+            tj += now() - ta;
+            have_jac = false;
+            if (k < ICE_NEWTON_MAX_HIST) {
+                rep->rhist[k] = rk;
+                rep->alpha[k] = alpha;
+                rep->linear_its[k] = its;
+                rep->nhist = k + 1;
+            }
+            if (o.verbose)
+                std::printf("newton %d: ||R|| = %.6e alpha = %.4f gmres its = %d (rel %.2e, refinements %d)\n", k, rk,
+                            alpha, its, rel, ref);
+            if (rk <= o.tau_abs + o.tau_rel * r0) {  // ConservationCriteria::done_callback
+                converged = true;
+                ++k;
+                break;
+            }
+        }
+        rep->iterations = k;
+        rep->converged = converged ? 1 : 0;
+        rep->rnorm = rk;
+        NT_HIP(hipStreamSynchronize(st));
+        rep->seconds = now() - t0;
+        rep->jacobian_seconds = tj;
+        rep->linear_seconds = tl;
+        rep->linesearch_seconds = tls;
+        return ICE_OK;
+    } catch (const Fail& f) {
+        ice::set_error(f.msg);
+        return f.code;
+    }
+}
+
+}  // extern "C"

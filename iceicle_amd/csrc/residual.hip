@@ -20,6 +20,8 @@
 // the result is deterministic and needs no atomics.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "dg_common.h"
 #include "fields.h"
 #include "kernels.h"
@@ -175,6 +177,510 @@ __global__ __launch_bounds__(256) void face_kernel(DevSpace S, PhysParams ph, Di
     } else {
         face_boundary<D, P, PH>(S, ph, dd.bval, U, dd.Q, dd.Pic, (blockIdx.x - nbi) * blockDim.x + threadIdx.x);
     }
+}
+
+// ---- split pipeline: traces per element face, then fluxes per face point ----
+//
+// trace_kernel: for every element and each of its 2*D faces, the trace at the
+// face's quadrature points (in the element's own canonical order) is formed
+// once from the LDS-staged coefficients and stored as
+//   T[e][j][q'][0][v]      = u_v
+//   T[e][j][q'][1 + d][v]  = g_d  (the element's share of the DDG gradient)
+// where on interior faces g = grad(u)/2 + s beta1 h (Hess(u) n), s = -1 on
+// the left and +1 on the right element, so that the face flux needs
+// gL + gR = (grad uL + grad uR)/2 + beta1 h (Hess uR - Hess uL) n
+// (conservation_law.hpp:529-552); on boundary faces g = grad(u).
+// flux_kernel then evaluates the numerical fluxes at each face point from the
+// two stored traces and the precomputed face geometry (normal, w sqrt(g), h).
+
+template <class PH, int D>
+struct TraceRec {
+    // Euler needs only values: its diffusive flux is identically zero
+    static constexpr bool NEED_GRAD = !(std::is_same<PH, NavierStokes<D, false>>::value);
+    static constexpr int NV = NEED_GRAD ? 1 + D : 1;
+};
+
+// Trace of one element at point pt of its face j by sum factorisation:
+// the coefficients are first contracted along the face normal direction k
+// with the 1D basis values, first and second derivatives at the face's
+// endpoint, C_r[t] = sum_a U[a along k, t] L_r(+-1)[a], then evaluated along
+// the tangential directions at the point.  Gives u, the physical gradient
+// and the normal Hessian term hn (as trace_eval, one component at a time to
+// keep the register footprint small), and stores the trace record of
+// trace_kernel: out[v] = u, out[(1 + d) M + v] = gx/2 + bh hn on interior
+// faces, gx on boundary faces.
+template <int D, int N, int M>
+__device__ __forceinline__ void face_trace_sf(const DevSpace& S, const double* Ue, int j, const int* pt,
+                                              const double (&X)[1 << D][D], const double* nrm, bool interior,
+                                              double bh, double* __restrict__ out) {
+    constexpr int NB = Pow<N, D>::value;
+    constexpr int NT = NB / N;  // tangential tensor points
+    constexpr int NH = D * (D + 1) / 2;
+    const double* xs = S.tab;
+    const double* L0 = S.tab + (N + 2);
+    const double* L1 = L0 + (N + 2) * N;
+    const double* L2 = L1 + (N + 2) * N;
+    const int k = j % D;
+    const int ep = pt[k];
+    double xi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) xi[d] = xs[pt[d]];
+    double Jm[D][D], Hm[D][D][D], A[D][D];
+    q1_map<D>(X, xi, nullptr, Jm, Hm);
+    double det = det_adj<D>(Jm, A);
+    det = (det == 0.0) ? 1.0 : det;
+    const double idet = 1.0 / det;
+    double ln[3][N];  // normal-direction 1D rows at the endpoint
+#pragma unroll
+    for (int a = 0; a < N; ++a) {
+        ln[0][a] = L0[ep * N + a];
+        ln[1][a] = L1[ep * N + a];
+        ln[2][a] = L2[ep * N + a];
+    }
+    // tangential dims in ascending order and their 1D rows at the point
+    int td[D > 1 ? D - 1 : 1];
+    {
+        int c = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            if (d != k) td[c++] = d;
+    }
+    // strides of the element's tensor index: dimension 0 slowest
+    int stride[D];
+    {
+        int st = 1;
+#pragma unroll
+        for (int d = D - 1; d >= 0; --d) { stride[d] = st; st *= N; }
+    }
+#pragma unroll 1
+    for (int v = 0; v < M; ++v) {
+        double C[3][NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            // tangential multi-index of t (first tangential dim slowest)
+            int base = 0;
+            {
+                int r = t;
+#pragma unroll
+                for (int c = D - 2; c >= 0; --c) {
+                    base += (r % N) * stride[td[c]];
+                    r /= N;
+                }
+            }
+            double c0 = 0, c1 = 0, c2 = 0;
+#pragma unroll
+            for (int a = 0; a < N; ++a) {
+                const double uv = Ue[(base + a * stride[k]) * M + v];
+                c0 += uv * ln[0][a];
+                c1 += uv * ln[1][a];
+                c2 += uv * ln[2][a];
+            }
+            C[0][t] = c0;
+            C[1][t] = c1;
+            C[2][t] = c2;
+        }
+        // reference derivatives: gr[d], hr[(a,b) a <= b]
+        double uv, gr[D], hr[NH];
+        if constexpr (D == 1) {
+            uv = C[0][0];
+            gr[0] = C[1][0];
+            hr[0] = C[2][0];
+        } else if constexpr (D == 2) {
+            const int p1 = pt[td[0]];
+            double s0 = 0, sk = 0, s1 = 0, skk = 0, sk1 = 0, s11 = 0;
+#pragma unroll
+            for (int b = 0; b < N; ++b) {
+                const double l0 = L0[p1 * N + b], l1 = L1[p1 * N + b], l2 = L2[p1 * N + b];
+                s0 += C[0][b] * l0;
+                sk += C[1][b] * l0;
+                s1 += C[0][b] * l1;
+                skk += C[2][b] * l0;
+                sk1 += C[1][b] * l1;
+                s11 += C[0][b] * l2;
+            }
+            uv = s0;
+            // hr order (0,0), (0,1), (1,1)
+            const bool k0 = (k == 0);
+            gr[0] = k0 ? sk : s1;
+            gr[1] = k0 ? s1 : sk;
+            hr[0] = k0 ? skk : s11;
+            hr[1] = sk1;
+            hr[2] = k0 ? s11 : skk;
+        } else {
+            const int p1 = pt[td[0]], p2 = pt[td[1]];
+            double s[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // 0, k, 1, 2, kk, k1, k2, 11, 12, 22
+#pragma unroll
+            for (int b1 = 0; b1 < N; ++b1) {
+                const double a0 = L0[p1 * N + b1], a1 = L1[p1 * N + b1], a2 = L2[p1 * N + b1];
+                double r0[3] = {0, 0, 0}, r1[3] = {0, 0, 0}, r2[3] = {0, 0, 0};  // [C index][deriv along dim2]
+#pragma unroll
+                for (int b2 = 0; b2 < N; ++b2) {
+                    const double c0 = L0[p2 * N + b2], c1 = L1[p2 * N + b2], c2 = L2[p2 * N + b2];
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) {
+                        const double cv = C[r][b1 * N + b2];
+                        r0[r] += cv * c0;
+                        r1[r] += cv * c1;
+                        r2[r] += cv * c2;
+                    }
+                }
+                s[0] += r0[0] * a0;
+                s[1] += r0[1] * a0;
+                s[2] += r0[0] * a1;
+                s[3] += r1[0] * a0;
+                s[4] += r0[2] * a0;
+                s[5] += r0[1] * a1;
+                s[6] += r1[1] * a0;
+                s[7] += r0[0] * a2;
+                s[8] += r1[0] * a1;
+                s[9] += r2[0] * a0;
+            }
+            uv = s[0];
+            // (k, o1, o2) = (0,1,2), (1,0,2) or (2,0,1); hr order 00 01 02 11 12 22
+            if (k == 0) {
+                gr[0] = s[1]; gr[1] = s[2]; gr[2] = s[3];
+                hr[0] = s[4]; hr[1] = s[5]; hr[2] = s[6]; hr[3] = s[7]; hr[4] = s[8]; hr[5] = s[9];
+            } else if (k == 1) {
+                gr[1] = s[1]; gr[0] = s[2]; gr[2] = s[3];
+                hr[3] = s[4]; hr[1] = s[5]; hr[4] = s[6]; hr[0] = s[7]; hr[2] = s[8]; hr[5] = s[9];
+            } else {
+                gr[2] = s[1]; gr[0] = s[2]; gr[1] = s[3];
+                hr[5] = s[4]; hr[2] = s[5]; hr[4] = s[6]; hr[0] = s[7]; hr[1] = s[8]; hr[3] = s[9];
+            }
+        }
+        // physical gradient and normal Hessian term (finite_element.hpp:220-258, 328-398)
+        double gx[D];
+#pragma unroll
+        for (int jj = 0; jj < D; ++jj) {
+            double sacc = 0;
+#pragma unroll
+            for (int kk = 0; kk < D; ++kk) sacc += gr[kk] * A[kk][jj];
+            gx[jj] = sacc * idet;
+        }
+        double rhs[D][D];
+        {
+            int h = 0;
+#pragma unroll
+            for (int a = 0; a < D; ++a)
+#pragma unroll
+                for (int b = a; b < D; ++b) {
+                    rhs[a][b] = hr[h];
+                    rhs[b][a] = hr[h];
+                    ++h;
+                }
+        }
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int b = 0; b < D; ++b)
+#pragma unroll
+                for (int kk = 0; kk < D; ++kk) rhs[a][b] -= Hm[kk][a][b] * gx[kk];
+        double Tm[D][D];
+#pragma unroll
+        for (int l = 0; l < D; ++l)
+#pragma unroll
+            for (int b = 0; b < D; ++b) {
+                double sacc = 0;
+#pragma unroll
+                for (int kk = 0; kk < D; ++kk) sacc += rhs[l][kk] * A[kk][b];
+                Tm[l][b] = sacc;
+            }
+        const double idet2 = idet * idet;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            double sacc = 0;
+#pragma unroll
+            for (int jj = 0; jj < D; ++jj) {
+                double hx = 0;
+#pragma unroll
+                for (int l = 0; l < D; ++l) hx += A[l][jj] * Tm[l][d];
+                sacc += hx * nrm[jj];
+            }
+            const double hn = sacc * idet2;
+            out[(1 + d) * M + v] = interior ? 0.5 * gx[d] + bh * hn : gx[d];
+        }
+        out[v] = uv;
+    }
+}
+
+template <int D, int P, class PH>
+__global__ __launch_bounds__(256) void trace_kernel(DevSpace S, PhysParams ph, const double* __restrict__ U,
+                                                    double* __restrict__ T) {
+    constexpr int N = P + 1;
+    constexpr int NB = Pow<N, D>::value;
+    constexpr int NFQ = Pow<N, D - 1>::value;
+    constexpr int M = PH::M;
+    constexpr int TPE = 2 * D * NFQ;                       // trace points per element
+    constexpr int EPB = (256 / TPE) > 0 ? (256 / TPE) : 1;  // elements per block
+    constexpr bool GRAD = TraceRec<PH, D>::NEED_GRAD;
+    constexpr int NV = TraceRec<PH, D>::NV;
+    __shared__ double sU[EPB * NB * M];
+    const int e0 = blockIdx.x * EPB;
+    {
+        const size_t base = (size_t)e0 * NB * M;
+        const size_t total = (size_t)S.nelem * NB * M;
+        for (int idx = threadIdx.x; idx < EPB * NB * M; idx += blockDim.x)
+            sU[idx] = (base + idx < total) ? U[base + idx] : 0.0;
+    }
+    __syncthreads();
+    const int le = threadIdx.x / TPE;
+    const int t = threadIdx.x - le * TPE;
+    const int e = e0 + le;
+    if (threadIdx.x >= EPB * TPE || e >= S.nelem) return;
+    const int j = t / NFQ, qp = t - j * NFQ;
+    int pt[D];
+    {
+        const signed char* fp = S.fpt + ((j * S.norient + 0) * NFQ + qp) * D;
+#pragma unroll
+        for (int d = 0; d < D; ++d) pt[d] = fp[d];
+    }
+    const double* L0 = S.tab + (N + 2);
+    const double* L1 = L0 + (N + 2) * N;
+    const double* L2 = L1 + (N + 2) * N;
+    double* out = T + (((size_t)e * 2 * D + j) * NFQ + qp) * NV * M;
+    const double* Ue = sU + le * NB * M;
+    if constexpr (!GRAD) {
+        double a0[D][N];
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+#pragma unroll
+            for (int i = 0; i < N; ++i) a0[d][i] = L0[pt[d] * N + i];
+        double u[M];
+#pragma unroll
+        for (int v = 0; v < M; ++v) u[v] = 0;
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            int id[D];
+            int r = i;
+#pragma unroll
+            for (int d = D - 1; d >= 0; --d) { id[d] = r % N; r /= N; }
+            double c0 = 1.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) c0 *= a0[d][id[d]];
+#pragma unroll
+            for (int v = 0; v < M; ++v) u[v] += Ue[i * M + v] * c0;
+        }
+#pragma unroll
+        for (int v = 0; v < M; ++v) out[v] = u[v];
+    } else {
+        // face row, side and orientation of this element face; the face point
+        // q of this element point qp (inverse of the orientation permutation)
+        const int ef = S.efaces[(size_t)e * 2 * D + j];
+        const int row = ef >> 4;
+        const bool interior = ef >= 0 && row < S.nfi;
+        const int side = (ef >> 3) & 1;
+        const int orient = ef & 7;
+        const int q = interior ? S.finv[(j * S.norient + orient) * NFQ + qp] : qp;
+        double X[1 << D][D];
+        load_vertices<D>(S.X, e, X);
+        double nrm[D], h = 1.0;
+        if (ef >= 0) {
+            const double* g = S.fgeo + ((size_t)row * NFQ + q) * (D + 2);
+#pragma unroll
+            for (int d = 0; d < D; ++d) nrm[d] = g[d];
+            h = g[D + 1];
+        } else {
+#pragma unroll
+            for (int d = 0; d < D; ++d) nrm[d] = 0.0;
+        }
+        const double bh = (side ? 1.0 : -1.0) * ph.beta1 * h;
+        face_trace_sf<D, N, M>(S, Ue, j, pt, X, nrm, interior, bh, out);
+    }
+}
+
+// Numerical fluxes at face points from stored traces: interior points first
+// (blocks [0, nbi)), then boundary points.
+template <int D, int P, class PH>
+__global__ __launch_bounds__(256) void flux_kernel(DevSpace S, PhysParams ph, DiscDev dd, const double* __restrict__ T,
+                                                   int nbi) {
+    constexpr int N = P + 1;
+    constexpr int NFQ = Pow<N, D - 1>::value;
+    constexpr int M = PH::M;
+    constexpr int NV = TraceRec<PH, D>::NV;
+    if ((int)blockIdx.x < nbi) {
+        const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+        const int f = tid / NFQ, q = tid - f * NFQ;
+        if (f >= S.nfi) return;
+        const int4 rec = S.fint[f];
+        const int eL = rec.x, eR = rec.y, info = rec.z;
+        const int fnl = info_fnl(info), fnr = info_fnr(info), orient = info_orient(info);
+        const int qr = S.fperm[(fnr * S.norient + orient) * NFQ + q];
+        const double* tL = T + (((size_t)eL * 2 * D + fnl) * NFQ + q) * NV * M;
+        const double* tR = T + (((size_t)eR * 2 * D + fnr) * NFQ + qr) * NV * M;
+        const double* g = S.fgeo + ((size_t)f * NFQ + q) * (D + 2);
+        double nrm[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) nrm[d] = g[d];
+        const double ws = g[D], h = g[D + 1];
+        double uL[M], uR[M], gs[M * D];
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            uL[v] = tL[v];
+            uR[v] = tR[v];
+        }
+        if constexpr (NV > 1) {
+#pragma unroll
+            for (int v = 0; v < M; ++v)
+#pragma unroll
+                for (int d = 0; d < D; ++d) gs[v * D + d] = tL[(1 + d) * M + v] + tR[(1 + d) * M + v];
+        } else {
+#pragma unroll
+            for (int k = 0; k < M * D; ++k) gs[k] = 0.0;
+        }
+        double Qv[M];
+        interior_point_flux_split<D, PH>(ph, uL, uR, gs, nrm, h, ws, Qv);
+        double* Qf = dd.Q + ((size_t)f * NFQ + q) * M;
+#pragma unroll
+        for (int v = 0; v < M; ++v) Qf[v] = Qv[v];
+        if constexpr (PH::HAS_HOMOGENEITY) {
+            if (dd.Pic) {
+                double uavg[M], G[M * D * M * D];
+#pragma unroll
+                for (int v = 0; v < M; ++v) uavg[v] = 0.5 * (uL[v] + uR[v]);
+                PH::homogeneity(ph, uavg, G);
+                double* Pf = dd.Pic + ((size_t)f * NFQ + q) * M * D;
+#pragma unroll
+                for (int v = 0; v < M; ++v)
+#pragma unroll
+                    for (int s = 0; s < D; ++s) {
+                        double acc = 0;
+#pragma unroll
+                        for (int k = 0; k < D; ++k)
+#pragma unroll
+                            for (int r = 0; r < M; ++r)
+                                acc += G[((v * D + k) * M + r) * D + s] * nrm[k] * (uR[r] - uL[r]);
+                        Pf[v * D + s] = 0.5 * ph.sigma_ic * acc * ws;
+                    }
+            }
+        }
+    } else {
+        const int tid = (blockIdx.x - nbi) * blockDim.x + threadIdx.x;
+        const int b = tid / NFQ, q = tid - b * NFQ;
+        if (b >= S.nfb) return;
+        const int4 rec = S.fbdy[b];
+        const int eL = rec.x, info = rec.y, bcflag = rec.z;
+        const int fnl = info_fnl(info), bctype = info_bctype(info);
+        const size_t slot = (size_t)S.nfi + b;
+        double* Qf = dd.Q + (slot * NFQ + q) * M;
+        double* Pf = nullptr;
+        if constexpr (PH::HAS_HOMOGENEITY) {
+            if (dd.Pic) {
+                Pf = dd.Pic + (slot * NFQ + q) * M * D;
+#pragma unroll
+                for (int k = 0; k < M * D; ++k) Pf[k] = 0.0;
+            }
+        }
+        const bool general = !(bctype == ICE_BC_DIRICHLET || bctype == ICE_BC_NEUMANN ||
+                               bctype == ICE_BC_EXTRAPOLATION || bctype == ICE_BC_SPACETIME_FUTURE);
+        if (general && !PH::IMPLEMENTS_BCS) {
+#pragma unroll
+            for (int v = 0; v < M; ++v) Qf[v] = 0.0;
+            return;
+        }
+        const double* g = S.fgeo + (slot * NFQ + q) * (D + 2);
+        double nrm[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) nrm[d] = g[d];
+        const double ws = g[D], hb = g[D + 1];
+        double uL[M] = {}, gL[M * D] = {};
+        if (bctype != ICE_BC_NEUMANN) {
+            const double* tL = T + (((size_t)eL * 2 * D + fnl) * NFQ + q) * NV * M;
+#pragma unroll
+            for (int v = 0; v < M; ++v) uL[v] = tL[v];
+            if constexpr (NV > 1) {
+#pragma unroll
+                for (int v = 0; v < M; ++v)
+#pragma unroll
+                    for (int d = 0; d < D; ++d) gL[v * D + d] = tL[(1 + d) * M + v];
+            }
+        }
+        const double* bv = dd.bval + ((size_t)b * NFQ + q) * M;
+        double Qv[M], uB[M];
+        boundary_point_flux<D, PH>(ph, bctype, bcflag, uL, gL, bv, nrm, hb, ws, Qv, uB);
+#pragma unroll
+        for (int v = 0; v < M; ++v) Qf[v] = Qv[v];
+        if constexpr (PH::HAS_HOMOGENEITY) {
+            if (Pf && bctype == ICE_BC_DIRICHLET) {
+                double uavg[M], G[M * D * M * D];
+#pragma unroll
+                for (int v = 0; v < M; ++v) uavg[v] = 0.5 * (uL[v] + uB[v]);
+                PH::homogeneity(ph, uavg, G);
+#pragma unroll
+                for (int v = 0; v < M; ++v)
+#pragma unroll
+                    for (int s = 0; s < D; ++s) {
+                        double acc = 0;
+#pragma unroll
+                        for (int k = 0; k < D; ++k)
+#pragma unroll
+                            for (int r = 0; r < M; ++r)
+                                acc += G[((v * D + k) * M + r) * D + s] * nrm[k] * (uB[r] - uL[r]);
+                        Pf[v * D + s] = ph.sigma_ic * acc * ws;
+                    }
+            }
+        }
+    }
+}
+
+// Face geometry table (u-independent): per face row and face point the unit
+// normal, w sqrt(g) and the guarded DDG distance h (conservation_law.hpp:
+// 484-527 interior, 700-713 boundary).
+template <int D, int P>
+__global__ __launch_bounds__(256) void fgeo_kernel(DevSpace S, double* __restrict__ out) {
+    constexpr int N = P + 1;
+    constexpr int NFQ = Pow<N, D - 1>::value;
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = tid / NFQ, q = tid - row * NFQ;
+    if (row >= S.nfi + S.nfb) return;
+    double XL[1 << D][D], x[D], nrm[D], cL[D], h, sqrtg;
+    if (row < S.nfi) {
+        const int4 rec = S.fint[row];
+        double XR[1 << D][D], cR[D];
+        load_vertices<D>(S.X, rec.x, XL);
+        load_vertices<D>(S.X, rec.y, XR);
+        sqrtg = face_geometry<D, N>(S, XL, info_fnl(rec.z), q, x, nrm);
+        centroid<D>(XL, cL);
+        centroid<D>(XR, cR);
+        const int pdim = info_pdim(rec.z);
+        if (pdim) cR[pdim - 1] += S.period[pdim - 1];
+        h = ddg_h_interior<D>(nrm, x, cL, cR);
+    } else {
+        const int4 rec = S.fbdy[row - S.nfi];
+        load_vertices<D>(S.X, rec.x, XL);
+        sqrtg = face_geometry<D, N>(S, XL, info_fnl(rec.y), q, x, nrm);
+        centroid<D>(XL, cL);
+        h = ddg_h_boundary<D>(nrm, x, cL);
+    }
+    double* o = out + ((size_t)row * NFQ + q) * (D + 2);
+#pragma unroll
+    for (int d = 0; d < D; ++d) o[d] = nrm[d];
+    o[D] = face_weight<N, D>(S, q) * sqrtg;
+    o[D + 1] = h;
+}
+
+template <int D, int P>
+int launch_fgeo(const DevSpace& S, double* out, hipStream_t st) {
+    constexpr int NFQ = Pow<P + 1, D - 1>::value;
+    const long nt = (long)(S.nfi + S.nfb) * NFQ;
+    if (nt > 0) fgeo_kernel<D, P><<<(unsigned)((nt + 255) / 256), 256, 0, st>>>(S, out);
+    return (int)hipGetLastError();
+}
+
+int face_geometry_table(int ndim, int p, const DevSpace& S, double* out, hipStream_t st) {
+#define ICE_GCASE(DD, PP) if (ndim == DD && p == PP) return launch_fgeo<DD, PP>(S, out, st);
+    ICE_GCASE(1, 1) ICE_GCASE(1, 2) ICE_GCASE(1, 3) ICE_GCASE(1, 4)
+    ICE_GCASE(2, 1) ICE_GCASE(2, 2) ICE_GCASE(2, 3) ICE_GCASE(2, 4)
+    ICE_GCASE(3, 1) ICE_GCASE(3, 2) ICE_GCASE(3, 3) ICE_GCASE(3, 4)
+#undef ICE_GCASE
+    return -1;
+}
+
+size_t trace_buffer_size(int ndim, int p, int flux, int nelem) {
+    int nfq = 1;
+    for (int d = 0; d < ndim - 1; ++d) nfq *= p + 1;
+    const int m = flux == ICE_FLUX_BURGERS ? 1 : ndim + 2;
+    const int nv = flux == ICE_FLUX_EULER ? 1 : 1 + ndim;
+    return (size_t)nelem * 2 * ndim * nfq * nv * m;
 }
 
 // ---- elements: volume integral + face lifting -----------------------------
@@ -640,7 +1146,12 @@ int launch_residual(const DevSpace& S, const PhysParams& ph, const DiscDev& dd, 
     if (ev) (void)hipEventRecord(ev[0], st);
     const int nbi = (int)(((long)S.nfi * NFQ + 255) / 256);
     const int nbb = (int)(((long)S.nfb * NFQ + 255) / 256);
-    if (nbi + nbb > 0) face_kernel<D, P, PH><<<nbi + nbb, 256, 0, st>>>(S, ph, d, U, nbi);
+    {
+        constexpr int TPE = 2 * D * NFQ;
+        constexpr int EPB = (256 / TPE) > 0 ? (256 / TPE) : 1;
+        trace_kernel<D, P, PH><<<(S.nelem + EPB - 1) / EPB, 256, 0, st>>>(S, ph, U, d.T);
+    }
+    if (nbi + nbb > 0) flux_kernel<D, P, PH><<<nbi + nbb, 256, 0, st>>>(S, ph, d, d.T, nbi);
     if (ev) (void)hipEventRecord(ev[1], st);
     using T = ElemTile<D, P>;
     const int nblk = (S.nelem + T::EPB - 1) / T::EPB;

@@ -53,6 +53,11 @@ _SIGS = {
     "ice_jacobian_layout": ([P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
     "ice_jacobian": ([P, P, P, P, P], C.c_int),
     "ice_jacobian_apply": ([P, P, P, P, P], C.c_int),
+    "ice_newton_opts_default": ([C.POINTER(abi.NewtonOpts)], C.c_int),
+    "ice_newton_create": ([P, C.POINTER(abi.NewtonOpts), C.POINTER(P)], C.c_int),
+    "ice_newton_destroy": ([P], C.c_int),
+    "ice_newton_solve": ([P, P, C.POINTER(abi.NewtonReport), P], C.c_int),
+    "ice_newton_linear_solve": ([P, P, P, P, C.POINTER(C.c_int), C.POINTER(C.c_double), P], C.c_int),
 }
 
 
@@ -289,3 +294,48 @@ class BlockJacobian:
         _require_cuda(x, "x")
         _require_cuda(y, "y")
         check(lib().ice_jacobian_apply(self.disc.h, _ptr(self.blocks), _ptr(x), _ptr(y), stream or current_stream()))
+
+
+def newton_opts(**kw):
+    """Newton options with the reference's defaults, overridden by keyword."""
+    o = abi.NewtonOpts()
+    check(lib().ice_newton_opts_default(C.byref(o)))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise IceError(f"unknown Newton option {k}")
+        setattr(o, k, v)
+    return o
+
+
+class PetscNewton:
+    """Newton-linesearch steady solver on the device, named after the
+    reference's PetscNewton (petsc_newton.hpp:124-329) whose role it takes;
+    the linear algebra is the library's own GMRES + block Jacobi."""
+
+    def __init__(self, disc, opts=None, **kw):
+        self.disc = disc
+        self.opts = opts if opts is not None else newton_opts(**kw)
+        h = P()
+        check(lib().ice_newton_create(disc.h, C.byref(self.opts), C.byref(h)))
+        self.h = h
+
+    def solve(self, u, stream=None):
+        """Solve R(u) = 0 in place from the initial guess u; returns the report as a dict."""
+        _require_cuda(u, "u")
+        rep = abi.NewtonReport()
+        check(lib().ice_newton_solve(self.h, _ptr(u), C.byref(rep), stream or current_stream()))
+        return rep.as_dict()
+
+    def linear_solve(self, b, x, u=None, stream=None):
+        """x <- J^-1 b for the Jacobian at u (or the last one assembled)."""
+        _require_cuda(b, "b")
+        _require_cuda(x, "x")
+        its, rel = C.c_int(), C.c_double()
+        check(lib().ice_newton_linear_solve(self.h, _ptr(u) if u is not None else None, _ptr(b), _ptr(x),
+                                            C.byref(its), C.byref(rel), stream or current_stream()))
+        return its.value, rel.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ice_newton_destroy(self.h)
+            self.h = None

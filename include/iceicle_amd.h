@@ -250,6 +250,74 @@ int ice_jacobian(ice_disc* disc, const double* u_dev, double* r_dev, double* blo
 /* y = J x for blocks in the layout above (FP64 blocks, FP64 vectors). */
 int ice_jacobian_apply(const ice_disc* disc, const double* blocks_dev, const double* x_dev, double* y_dev, void* stream);
 
+/* ---- Newton-linesearch steady solve (PetscNewton::solve,
+ *      petsc_newton.hpp:205-329; ConvergenceCriteria and linesearches,
+ *      nonlinear_solver_utils.hpp:23-268)
+ *
+ * Each step assembles the finite-difference block Jacobian (ice_jacobian),
+ * solves J du = R with restarted GMRES right-preconditioned by block Jacobi
+ * (the reference hands this to PETSc KSP, default GMRES(30) with SOR,
+ * petsc_newton.hpp:174-183), picks alpha by the linesearch on
+ * phi(alpha) = ||R(u - alpha du)||_2 (non-finite -> 1e9), sets u -= alpha du,
+ * and stops when ||R||_2 <= tau_abs + tau_rel * ||R(u_0)||_2 or after kmax
+ * steps.  With ICE_PREC_FP32 the Krylov operator and the block-Jacobi factors
+ * are FP32 copies of the FP64 blocks (FP64 vectors and accumulation) and the
+ * correction is refined against the FP64 blocks: r = R - J du in FP64 until
+ * ||r|| <= linear_rtol ||R|| or refine_max passes. */
+#define ICE_LS_NONE 0
+#define ICE_LS_WOLFE 1
+#define ICE_LS_CORRIGAN 2
+#define ICE_PREC_FP64 0
+#define ICE_PREC_FP32 1
+#define ICE_NEWTON_MAX_HIST 64
+
+typedef struct {
+    double tau_abs, tau_rel;  /* ConvergenceCriteria (defaults 1e-8, 1e-8) */
+    int kmax;                 /* Newton steps (default 100) */
+    int linesearch;           /* ICE_LS_* (default none) */
+    int ls_kmax;              /* linesearch max_it (20) */
+    double ls_alpha_initial;  /* 1 */
+    double ls_alpha_max;      /* 10 */
+    double ls_c1, ls_c2;      /* Wolfe constants (1e-4, 0.9) */
+    double ls_alpha_min;      /* Corrigan alpha_min (0) */
+    int jacobian_precision;   /* ICE_PREC_* (default FP32) */
+    int gmres_restart;        /* 30 */
+    int gmres_maxit;          /* 10000 */
+    double linear_rtol;       /* 1e-5 */
+    int refine_max;           /* FP64 refinement passes (3) */
+    int verbose;              /* 1: one line per Newton step on stdout */
+} ice_newton_opts;
+
+typedef struct {
+    int iterations;           /* Newton steps taken */
+    int converged;
+    double r0, rnorm;         /* ||R|| at u_0 and at the returned u */
+    long long linear_iterations;
+    int refinements;
+    int singular_blocks;      /* diagonal blocks replaced by the identity in block Jacobi */
+    double seconds, jacobian_seconds, linear_seconds, linesearch_seconds;
+    int nhist;
+    double rhist[ICE_NEWTON_MAX_HIST];   /* ||R|| after step k */
+    double alpha[ICE_NEWTON_MAX_HIST];   /* linesearch multiplier of step k */
+    int linear_its[ICE_NEWTON_MAX_HIST]; /* GMRES iterations of step k */
+} ice_newton_report;
+
+typedef struct ice_newton ice_newton;
+
+/* Reference defaults (see the field comments). */
+int ice_newton_opts_default(ice_newton_opts* opts);
+/* Allocates the Jacobian (FP64 blocks, FP32 copies when asked), block-Jacobi
+ * factors and the Krylov basis for this discretization. */
+int ice_newton_create(ice_disc* disc, const ice_newton_opts* opts, ice_newton** out);
+int ice_newton_destroy(ice_newton* solver);
+/* Solve R(u) = 0 from the initial guess in u (device, fe_layout_right).
+ * Synchronous; returns ICE_OK whether or not it converged (see report). */
+int ice_newton_solve(ice_newton* solver, double* u_dev, ice_newton_report* report, void* stream);
+/* The linear solve alone, J x = b for the Jacobian last assembled by the
+ * solver at u (assembled here first when u_dev is not NULL); x and b device. */
+int ice_newton_linear_solve(ice_newton* solver, const double* u_dev, const double* b_dev, double* x_dev,
+                            int* iterations, double* rel_residual, void* stream);
+
 /* ---- explicit time integration (RK3SSP ssp_rk3.hpp:106-233, RK3TVD
  *      tvd_rk3.hpp:107-270, ExplicitEuler explicit_euler.hpp:87-160) ----- */
 #define ICE_STEP_RK3_SSP 0

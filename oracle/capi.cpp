@@ -36,6 +36,7 @@ struct ProblemBase {
                           double tfinal, long long ntime, long long* nsteps, double* time) = 0;
     // blocks in BSR slot order: per row e, slot 0 = (e,e), slot 1+j = neighbour across local face j
     virtual void jacobian(const double* u, double* res, int* cols, double* blocks) = 0;
+    virtual int newton(double* u, double tau_abs, double tau_rel, int kmax, double* rhist, int nhist) = 0;
 };
 
 template <int D, class PF, class CF, class DF>
@@ -147,6 +148,9 @@ struct Problem final : ProblemBase {
     }
     void mass_apply(const double* r, double* out) const override {
         const_cast<Problem*>(this)->mi().apply_add(fs, neq, r, out);
+    }
+    int newton(double* u, double tau_abs, double tau_rel, int kmax, double* rhist, int nhist) override {
+        return newton_dense(fs, disc, u, tau_abs, tau_rel, kmax, rhist, nhist);
     }
     void jacobian(const double* u, double* res, int* cols, double* blocks) override {
         std::map<std::pair<int, int>, std::vector<double>> B;
@@ -289,6 +293,14 @@ int ice_oracle_qp_coords(void* h, double* xq) { ORACLE_TRY(static_cast<Handle*>(
 int ice_oracle_mass_apply(void* h, const double* r, double* out) { ORACLE_TRY(static_cast<Handle*>(h)->prob->mass_apply(r, out)) }
 int ice_oracle_jacobian(void* h, const double* u, double* res, int* cols, double* blocks) {
     ORACLE_TRY(static_cast<Handle*>(h)->prob->jacobian(u, res, cols, blocks))
+}
+int ice_oracle_newton(void* h, double* u, double tau_abs, double tau_rel, int kmax, double* rhist, int nhist) {
+    try {
+        return static_cast<Handle*>(h)->prob->newton(u, tau_abs, tau_rel, kmax, rhist, nhist);
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -2;
+    }
 }
 int ice_oracle_rk_solve(void* h, double* u, int kind, int compat, int use_cfl, double dt, double cfl, int use_tfinal,
                         double tfinal, long long ntime, long long* nsteps, double* time) {

@@ -58,6 +58,8 @@ def lib():
             fn.restype = C.c_int
         L.ice_oracle_l2_error.argtypes = [P, C.POINTER(abi.Field), dp]
         L.ice_oracle_l2_error.restype = C.c_double
+        L.ice_oracle_newton.argtypes = [P, dp, C.c_double, C.c_double, C.c_int, dp, C.c_int]
+        L.ice_oracle_newton.restype = C.c_int
         L.ice_oracle_cfl_dt.argtypes = [P, C.c_double]
         L.ice_oracle_cfl_dt.restype = C.c_double
         _LIB = L
@@ -158,6 +160,15 @@ class Oracle:
         blocks = np.empty((self.nelem, S, bs, bs))
         _check(lib().ice_oracle_jacobian(self.h, _dp(u), _dp(res), _ip(cols), _dp(blocks)))
         return res, cols, blocks
+
+    def newton(self, u, tau_abs=1e-10, tau_rel=0.0, kmax=20):
+        """Newton with a dense LU linear solve; returns (u, steps, ||R|| history)."""
+        u = np.array(u, dtype=np.float64)
+        hist = np.zeros(64)
+        k = lib().ice_oracle_newton(self.h, _dp(u), tau_abs, tau_rel, kmax, _dp(hist), 64)
+        if k < 0:
+            _check(-1)
+        return u, k, hist[:min(k, 64)]
 
     def cfl_dt(self, cfl):
         return lib().ice_oracle_cfl_dt(self.h, cfl)

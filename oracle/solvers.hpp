@@ -312,4 +312,52 @@ void fd_jacobian(const FESpace<D>& fs, const Disc& disc, const double* u, double
     }
 }
 
+// Newton without linesearch (PetscNewton::solve with no_linesearch,
+// petsc_newton.hpp:205-329) where the linear solve J du = R is a dense LU of
+// the assembled block Jacobian: a direct solve the device GMRES is checked
+// against.  Stops when ||R||_2 <= tau_abs + tau_rel ||R(u0)||_2
+// (ConvergenceCriteria, nonlinear_solver_utils.hpp:23-60); returns the steps
+// taken, or -1 when the dense system is too large.
+template <int D, class Disc>
+int newton_dense(const FESpace<D>& fs, const Disc& disc, double* u, double tau_abs, double tau_rel, int kmax,
+                 double* rhist, int nhist) {
+    constexpr int m = Disc::neq;
+    const int nb = fs.nb, bs = nb * m, E = fs.mesh.nelem;
+    const size_t N = (size_t)E * bs;
+    if (N > 6000) return -1;
+    std::vector<double> res(N), du(N), A;
+    auto nrm2 = [&](const std::vector<double>& v) {
+        double s2 = 0;
+        for (double x : v) s2 += x * x;
+        return std::sqrt(s2);
+    };
+    std::map<std::pair<int, int>, std::vector<double>> B;
+    fd_jacobian(fs, disc, u, res.data(), B);
+    const double r0 = nrm2(res);
+    int k = 0;
+    for (k = 0; k < kmax; ++k) {
+        A.assign(N * N, 0.0);
+        for (const auto& kv : B) {
+            const int re = kv.first.first, ce = kv.first.second;
+            for (int i = 0; i < bs; ++i)
+                for (int j = 0; j < bs; ++j)
+                    A[((size_t)re * bs + i) * N + (size_t)ce * bs + j] += kv.second[(size_t)i * bs + j];
+        }
+        DenseLU lu;
+        lu.factor(A, (int)N);
+        if (lu.singular) throw std::runtime_error("singular Jacobian");
+        lu.solve(res.data(), du.data());
+        for (size_t i = 0; i < N; ++i) u[i] -= du[i];
+        B.clear();
+        fd_jacobian(fs, disc, u, res.data(), B);
+        const double rk = nrm2(res);
+        if (k < nhist) rhist[k] = rk;
+        if (rk <= tau_abs + tau_rel * r0) {
+            ++k;
+            break;
+        }
+    }
+    return k;
+}
+
 }  // namespace oracle
