@@ -5,9 +5,13 @@ Navier-Stokes manufactured steady problem, 128 x 128 quads with seeded 5%
 interior node perturbation, P3 Lagrange, 4 x 4 Gauss-Legendre, 4 conserved
 variables, VanLeer + DDG viscous fluxes, Dirichlet boundaries, manufactured
 source.  One step = one FP64 residual evaluation R(u) (form_residual) over
-the whole mesh with u and R resident in HBM.
+the whole mesh with u and R resident in HBM.  After the timed steps, one
+Newton-linesearch solve of the same problem from a uniform free-stream state
+to ||R||_2 < 1e-10 is timed (the metric's second part, under "newton").
 
-Prints one JSON line (see the repository's measurement notes for the fields).
+Prints one JSON line: value = residual GDOF/s; roofline = the slowest
+kernel's algorithmic bytes over its event-timed duration against 8 TB/s;
+cpu_baseline = the CPU oracle on full residuals of the same workload.
 """
 import argparse
 import json
@@ -32,6 +36,7 @@ def parse():
     ap.add_argument("--order", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-newton", action="store_true", help="skip the Newton time-to-solution leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="only run 20 device copies of an MB-sized buffer (HBM counter calibration)")
     return ap.parse_args()
@@ -103,24 +108,57 @@ def main():
     info = space.info
     D, nb, nfq = space.ndim, space.nbasis, space.nfqp
     E, nfi, nfb = space.nelem, info.nface_interior, info.nface_boundary
+    F = nfi + nfb
     nvert = 2 ** D
+    nv = 1 + D  # trace record: value + the element's share of the DDG gradient
+    geo_rec = 8 * (D + 2)  # normal, w sqrt(g), h per face point
+    trace_bytes = 8 * E * 2 * D * nfq * nv * neq
     kernels = {
-        # interior + boundary faces: states and vertices of every element once,
-        # face records, boundary values, weighted normal fluxes written
-        "faces": (stage_ms[0] / ncalls,
-                  8 * E * nb * neq + 8 * E * nvert * D + 16 * (nfi + nfb) + 8 * nfb * nfq * neq
-                  + 8 * (nfi + nfb) * nfq * neq),
-        # elements: state, vertices, face links, face fluxes and source load read, residual written
+        # element face traces: state, vertices, face links and face-point
+        # geometry read, traces written
+        "trace": (stage_ms[0] / ncalls,
+                  8 * E * nb * neq + 8 * E * nvert * D + 4 * 2 * D * E + geo_rec * E * 2 * D * nfq + trace_bytes),
+        # face fluxes: traces of both sides, face records and geometry,
+        # boundary values read, weighted normal fluxes written
+        "flux": (stage_ms[1] / ncalls,
+                 trace_bytes + 16 * F + geo_rec * F * nfq + 8 * nfb * nfq * neq + 8 * F * nfq * neq),
+        # elements: state, vertices, face links, face fluxes and source load
+        # read, residual written
         "element": (stage_ms[2] / ncalls,
-                    8 * E * nb * neq + 8 * E * nvert * D + 4 * 2 * D * E + 8 * (nfi + nfb) * nfq * neq
+                    8 * E * nb * neq + 8 * E * nvert * D + 4 * 2 * D * E + 8 * F * nfq * neq
                     + 8 * E * nb * neq + 8 * E * nb * neq),
     }
     dom = max(kernels, key=lambda k: kernels[k][0])
     kms, kbytes = kernels[dom]
     achieved = kbytes / (kms * 1e-3) / 1e9
     nnode = (args.n + 1) ** 2
-    b_alg = workloads.residual_bytes(D, order, neq, E, nnode, nfi + nfb)
+    b_alg = workloads.residual_bytes(D, order, neq, E, nnode, F)
     res_gbs = b_alg / (ms_per_step * 1e-3) / 1e9
+
+    newton = None
+    if not args.no_newton:
+        # Newton-linesearch from a uniform free-stream state (rho 1, u 0.3,
+        # v 0, p 1/gamma) to ||R||_2 < 1e-10, FP32 Jacobian blocks and
+        # preconditioner with FP64 residual and refinement
+        fs = np.tile([1.0, 0.3, 0.0, 1.0 / 1.4 / 0.4 + 0.045], disc.size // 4)
+        un = torch.from_numpy(fs).cuda()
+        solver = engine.PetscNewton(disc, tau_abs=1e-10, tau_rel=0.0, kmax=30)
+        print("newton solve ...", file=sys.stderr, flush=True)
+        rep = solver.solve(un)
+        print(f"newton done: {rep['iterations']} steps, {rep['seconds']:.2f} s", file=sys.stderr, flush=True)
+        newton = {
+            "seconds": round(rep["seconds"], 4),
+            "converged": rep["converged"],
+            "steps": rep["iterations"],
+            "residual_norms": [float(f"{v:.3e}") for v in rep["rhist"]],
+            "r0": float(f"{rep['r0']:.4e}"),
+            "linear_iterations": rep["linear_iterations"],
+            "jacobian_seconds": round(rep["jacobian_seconds"], 4),
+            "linear_seconds": round(rep["linear_seconds"], 4),
+            "initial_guess": "uniform free stream",
+            "linear_solver": "GMRES(30), multicolour block SGS, FP32 blocks + FP64 refinement, rtol 1e-5",
+        }
+        del solver
 
     out = {
         "metric": "residual GDOF/s (FP64 vs mixed, 1 GPU); Newton time to ||R||<1e-10",
@@ -141,6 +179,7 @@ def main():
             "elements": E, "unknowns": ndof, "neq": neq, "order": order,
         },
         "kernels_ms": {k: round(v[0], 6) for k, v in kernels.items()},
+        "kernel_bytes": {k: int(v[1]) for k, v in kernels.items()},
         "roofline": {
             "bound": "hbm",
             "kernel": dom,
@@ -154,6 +193,8 @@ def main():
         "residual_roofline": {"bytes": int(b_alg), "achieved_GBs": round(res_gbs, 2),
                               "frac": round(res_gbs / HBM_PEAK_GBS, 5)},
     }
+    if newton is not None:
+        out["newton"] = newton
     if not args.no_cpu_baseline:
         n, el, size = cpu_baseline(mesh, order, params, u_host, args.cpu_seconds)
         out["cpu_baseline"] = {

@@ -174,6 +174,8 @@ LS_WOLFE = 1
 LS_CORRIGAN = 2
 PREC_FP64 = 0
 PREC_FP32 = 1
+PC_BLOCK_JACOBI = 0
+PC_BLOCK_SGS = 1
 NEWTON_MAX_HIST = 64
 
 
@@ -182,7 +184,8 @@ class NewtonOpts(C.Structure):
                 ("ls_kmax", C.c_int), ("ls_alpha_initial", C.c_double), ("ls_alpha_max", C.c_double),
                 ("ls_c1", C.c_double), ("ls_c2", C.c_double), ("ls_alpha_min", C.c_double),
                 ("jacobian_precision", C.c_int), ("gmres_restart", C.c_int), ("gmres_maxit", C.c_int),
-                ("linear_rtol", C.c_double), ("refine_max", C.c_int), ("verbose", C.c_int)]
+                ("linear_rtol", C.c_double), ("refine_max", C.c_int), ("verbose", C.c_int),
+                ("preconditioner", C.c_int)]
 
 
 class NewtonReport(C.Structure):

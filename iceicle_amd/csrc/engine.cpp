@@ -715,10 +715,12 @@ int ice_profile_end(ice_disc* d, double* stage_ms, int* ncalls) {
     for (int c = 0; c < d->prof_n; ++c) {
         hipEvent_t* ev = &d->prof_ev[(size_t)4 * c];
         if (hipEventSynchronize(ev[3]) != hipSuccess) return fail(ICE_ERR_HIP, "event synchronize failed");
-        float ms0 = 0.f, ms2 = 0.f;
-        if (hipEventElapsedTime(&ms0, ev[0], ev[1]) != hipSuccess || hipEventElapsedTime(&ms2, ev[1], ev[3]) != hipSuccess)
+        float ms0 = 0.f, ms1 = 0.f, ms2 = 0.f;
+        if (hipEventElapsedTime(&ms0, ev[0], ev[1]) != hipSuccess || hipEventElapsedTime(&ms1, ev[1], ev[2]) != hipSuccess ||
+            hipEventElapsedTime(&ms2, ev[2], ev[3]) != hipSuccess)
             return fail(ICE_ERR_HIP, "event timing failed");
         stage_ms[0] += ms0;
+        stage_ms[1] += ms1;
         stage_ms[2] += ms2;
     }
     *ncalls = d->prof_n;

@@ -4,6 +4,8 @@
 // reference's PetscNewton drives (petsc_newton.hpp:154-244).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.h"
 
 namespace ice {
@@ -106,7 +108,8 @@ namespace {
 
 template <class T>
 __global__ __launch_bounds__(256) void block_inverse_kernel(int nelem, int nslots, int bs, const double* __restrict__ J,
-                                                            T* __restrict__ Dinv, int* __restrict__ nsingular) {
+                                                            T* __restrict__ Dinv, int* __restrict__ nsingular,
+                                                            int transposed) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double* A = reinterpret_cast<double*>(smem_raw);  // bs * bs
     int* perm = reinterpret_cast<int*>(A + (size_t)bs * bs);
@@ -186,7 +189,11 @@ __global__ __launch_bounds__(256) void block_inverse_kernel(int nelem, int nslot
             }
         __syncthreads();
     }
-    for (int k = tid; k < bs * bs; k += blockDim.x) dst[k] = (T)A[k];
+    if (transposed) {
+        for (int k = tid; k < bs * bs; k += blockDim.x) dst[k] = (T)A[(size_t)(k % bs) * bs + k / bs];
+    } else {
+        for (int k = tid; k < bs * bs; k += blockDim.x) dst[k] = (T)A[k];
+    }
 }
 
 // y[e] = D[e] x[e] for dense bs x bs blocks (block-Jacobi application), one
@@ -230,14 +237,15 @@ __global__ void convert_slots_kernel(size_t n, const double* __restrict__ a, flo
 }  // namespace
 
 template <class T>
-int block_inverse(int nelem, int nslots, int bs, const double* J, T* Dinv, int* nsingular, hipStream_t st) {
+int block_inverse(int nelem, int nslots, int bs, const double* J, T* Dinv, int* nsingular, bool transposed,
+                  hipStream_t st) {
     const size_t sh = (size_t)bs * bs * sizeof(double) + (size_t)bs * sizeof(int);
     if (sh > 160 * 1024) return -1;
-    block_inverse_kernel<T><<<nelem, 256, sh, st>>>(nelem, nslots, bs, J, Dinv, nsingular);
+    block_inverse_kernel<T><<<nelem, 256, sh, st>>>(nelem, nslots, bs, J, Dinv, nsingular, transposed ? 1 : 0);
     return (int)hipGetLastError();
 }
-template int block_inverse<float>(int, int, int, const double*, float*, int*, hipStream_t);
-template int block_inverse<double>(int, int, int, const double*, double*, int*, hipStream_t);
+template int block_inverse<float>(int, int, int, const double*, float*, int*, bool, hipStream_t);
+template int block_inverse<double>(int, int, int, const double*, double*, int*, bool, hipStream_t);
 
 template <class T, class V>
 int block_diag_apply(int nelem, int bs, const T* D, const V* x, V* y, hipStream_t st) {
@@ -484,6 +492,357 @@ int launch_hessenberg_solve(int k, int m, const double* sc, double* y, hipStream
 
 int launch_gmres_init(int m, const double* nrm2, double* sc, hipStream_t st) {
     gmres_init_kernel<<<1, 64, 0, st>>>(m, nrm2, sc);
+    return (int)hipGetLastError();
+}
+
+}  // namespace ice
+
+// ---------------------------------------------------------------------------
+// Multicolour block symmetric Gauss-Seidel, M = (D + L) D^-1 (D + U) with the
+// elements ordered colour by colour (no two face neighbours share a colour,
+// so the rows of one colour are independent).  One launch per colour and
+// sweep direction:
+//   forward  (colour c):  y_e = D_e^-1 (r_e - sum_{col colour < c} J_e,col y_col)
+//   backward (colour c):  y_e = y_e - D_e^-1 sum_{col colour > c} J_e,col y_col
+// The row's partial sum is kept in LDS and multiplied by the factored
+// diagonal block in the same workgroup.
+
+namespace ice {
+namespace {
+
+template <class T, class V, int NS, bool FWD>
+__global__ __launch_bounds__(256) void sgs_sweep_kernel(int nrows, const int* __restrict__ elems,
+                                                        const int* __restrict__ ecolor, int c, int bs, int G, int epb,
+                                                        const int* __restrict__ cols, const T* __restrict__ J,
+                                                        const T* __restrict__ Dinv, const V* __restrict__ r,
+                                                        V* __restrict__ y) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    V* sx = reinterpret_cast<V*>(smem_raw);        // epb * NS * bs neighbour values
+    V* st = sx + (size_t)epb * NS * bs;             // epb * bs partial sums
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wpe = 4 / epb;
+    const int le = wave / wpe, wi = wave % wpe;
+    const int k0 = blockIdx.x * epb;
+    for (int k = threadIdx.x; k < epb * NS * bs; k += blockDim.x) {
+        const int lel = k / (NS * bs), rem = k % (NS * bs), s = rem / bs, cc = rem % bs;
+        V v = 0;
+        if (k0 + lel < nrows && s > 0) {
+            const int e = elems[k0 + lel];
+            const int col = cols[(size_t)e * NS + s];
+            if (col >= 0) {
+                const int cc2 = ecolor[col];
+                if (FWD ? (cc2 < c) : (cc2 > c)) v = y[(size_t)col * bs + cc];
+            }
+        }
+        sx[k] = v;
+    }
+    __syncthreads();
+    const bool active = (k0 + le) < nrows;
+    const int e = active ? elems[k0 + le] : 0;
+    const int rpw = 64 / G;
+    const int sub = lane / G, gl = lane % G;
+    const V* xs = sx + (size_t)le * NS * bs;
+    if (active) {
+        int ecol[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int col = cols[(size_t)e * NS + s];
+            ecol[s] = (s > 0 && col >= 0 && (FWD ? (ecolor[col] < c) : (ecolor[col] > c))) ? col : -1;
+        }
+        for (int r0 = wi * rpw; r0 < bs; r0 += wpe * rpw) {
+            const int row = r0 + sub;
+            V acc = 0;
+            if (row < bs) {
+#pragma unroll
+                for (int s = 1; s < NS; ++s) {
+                    if (ecol[s] < 0) continue;
+                    const T* Jr = J + (((size_t)e * NS + s) * bs + row) * bs;
+                    for (int cc = gl; cc < bs; cc += G) acc += (V)Jr[cc] * xs[s * bs + cc];
+                }
+            }
+            for (int off = G >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+            if (gl == 0 && row < bs) st[le * bs + row] = FWD ? r[(size_t)e * bs + row] - acc : acc;
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    const V* ts = st + le * bs;
+    for (int r0 = wi * rpw; r0 < bs; r0 += wpe * rpw) {
+        const int row = r0 + sub;
+        V acc = 0;
+        if (row < bs) {
+            const T* Dr = Dinv + ((size_t)e * bs + row) * bs;
+            for (int cc = gl; cc < bs; cc += G) acc += (V)Dr[cc] * ts[cc];
+        }
+        for (int off = G >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+        if (gl == 0 && row < bs) {
+            const size_t i = (size_t)e * bs + row;
+            y[i] = FWD ? acc : y[i] - acc;
+        }
+    }
+}
+
+}  // namespace
+
+template <class T, class V>
+int sgs_sweep(bool forward, int nrows, const int* elems, const int* ecolor, int c, int nslots, int bs,
+              const int* cols, const T* J, const T* Dinv, const V* r, V* y, hipStream_t st) {
+    if (nrows <= 0) return 0;
+    const int G = group_width(bs);
+    const int epb = (bs >= 32) ? 1 : 4;
+    const int grid = (nrows + epb - 1) / epb;
+    const size_t sh = ((size_t)epb * nslots * bs + (size_t)epb * bs) * sizeof(V);
+#define ICE_SGS(NSV)                                                                                               \
+    case NSV:                                                                                                      \
+        if (forward)                                                                                               \
+            sgs_sweep_kernel<T, V, NSV, true><<<grid, 256, sh, st>>>(nrows, elems, ecolor, c, bs, G, epb, cols, J, \
+                                                                     Dinv, r, y);                                  \
+        else                                                                                                       \
+            sgs_sweep_kernel<T, V, NSV, false><<<grid, 256, sh, st>>>(nrows, elems, ecolor, c, bs, G, epb, cols, \
+                                                                      J, Dinv, r, y);                              \
+        break;
+    switch (nslots) {
+        ICE_SGS(3)
+        ICE_SGS(5)
+        ICE_SGS(7)
+    default: return -1;
+    }
+#undef ICE_SGS
+    return (int)hipGetLastError();
+}
+template int sgs_sweep<float, double>(bool, int, const int*, const int*, int, int, int, const int*, const float*,
+                                      const float*, const double*, double*, hipStream_t);
+template int sgs_sweep<double, double>(bool, int, const int*, const int*, int, int, int, const int*, const double*,
+                                       const double*, const double*, double*, hipStream_t);
+
+}  // namespace ice
+
+// ---------------------------------------------------------------------------
+// Column-major ("transposed") block kernels for the Krylov operator and the
+// preconditioner.  A block's entry (r, c) sits at [c][r], so the threads of
+// one block row set own one row each and read a column with one coalesced
+// load per column step, accumulating without any cross-lane reduction.
+// Threads map to (element, row) pairs; the x segments the workgroup's rows
+// need are staged in LDS.
+
+namespace ice {
+namespace {
+
+constexpr int RT_THREADS = 256;
+
+// elements [e_first, e_last] touched by the threads of this workgroup
+__device__ __forceinline__ void rows_span(int nrows_total, int bs, int& r0, int& e_first, int& e_last) {
+    r0 = blockIdx.x * RT_THREADS;
+    const int r1 = min(r0 + RT_THREADS, nrows_total) - 1;
+    e_first = r0 / bs;
+    e_last = r1 / bs;
+}
+
+template <class T, class V, int NS>
+__global__ __launch_bounds__(RT_THREADS) void bsrT_kernel(int nelem, int bs, const int* __restrict__ cols,
+                                                          const T* __restrict__ JT, const V* __restrict__ x,
+                                                          const V* __restrict__ b, V* __restrict__ y) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    V* sx = reinterpret_cast<V*>(smem_raw);  // [ne][NS][bs]
+    const int ntot = nelem * bs;
+    int r0, ef, el;
+    rows_span(ntot, bs, r0, ef, el);
+    const int ne = el - ef + 1;
+    for (int k = threadIdx.x; k < ne * NS * bs; k += blockDim.x) {
+        const int le = k / (NS * bs), rem = k % (NS * bs), s = rem / bs, c = rem % bs;
+        const int col = cols[(size_t)(ef + le) * NS + s];
+        sx[k] = (col >= 0) ? x[(size_t)col * bs + c] : (V)0;
+    }
+    __syncthreads();
+    const int g = r0 + threadIdx.x;
+    if (g >= ntot) return;
+    const int e = g / bs, r = g - e * bs;
+    const V* xs = sx + (size_t)(e - ef) * NS * bs;
+    V acc0 = 0, acc1 = 0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        if (cols[(size_t)e * NS + s] < 0) continue;
+        const T* Jc = JT + ((size_t)e * NS + s) * bs * bs + r;
+        const V* xv = xs + s * bs;
+        int c = 0;
+        for (; c + 1 < bs; c += 2) {
+            acc0 += (V)Jc[(size_t)c * bs] * xv[c];
+            acc1 += (V)Jc[(size_t)(c + 1) * bs] * xv[c + 1];
+        }
+        if (c < bs) acc0 += (V)Jc[(size_t)c * bs] * xv[c];
+    }
+    const V acc = acc0 + acc1;
+    y[g] = b ? b[g] - acc : acc;
+}
+
+// y = D x with column-major dense blocks D[e][c][r]
+template <class T, class V>
+__global__ __launch_bounds__(RT_THREADS) void diagT_kernel(int nelem, int bs, const T* __restrict__ DT,
+                                                           const V* __restrict__ x, V* __restrict__ y) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    V* sx = reinterpret_cast<V*>(smem_raw);
+    const int ntot = nelem * bs;
+    int r0, ef, el;
+    rows_span(ntot, bs, r0, ef, el);
+    const int ne = el - ef + 1;
+    for (int k = threadIdx.x; k < ne * bs; k += blockDim.x) sx[k] = x[(size_t)ef * bs + k];
+    __syncthreads();
+    const int g = r0 + threadIdx.x;
+    if (g >= ntot) return;
+    const int e = g / bs, r = g - e * bs;
+    const V* xv = sx + (size_t)(e - ef) * bs;
+    const T* Dc = DT + (size_t)e * bs * bs + r;
+    V acc0 = 0, acc1 = 0;
+    int c = 0;
+    for (; c + 1 < bs; c += 2) {
+        acc0 += (V)Dc[(size_t)c * bs] * xv[c];
+        acc1 += (V)Dc[(size_t)(c + 1) * bs] * xv[c + 1];
+    }
+    if (c < bs) acc0 += (V)Dc[(size_t)c * bs] * xv[c];
+    y[g] = acc0 + acc1;
+}
+
+// one colour of a multicolour block Gauss-Seidel sweep over the rows of the
+// elements elems[0..nel), column-major blocks (see sgs_sweep_kernel).  A
+// workgroup owns whole elements (epb of them; its threads loop over their
+// rows), because the multiplication by the factored diagonal block needs the
+// element's complete partial-sum vector.
+template <class T, class V, int NS, bool FWD>
+__global__ __launch_bounds__(RT_THREADS) void sgsT_kernel(int nel, int epb, const int* __restrict__ elems,
+                                                          const int* __restrict__ ecolor, int colour, int bs,
+                                                          const int* __restrict__ cols, const T* __restrict__ JT,
+                                                          const T* __restrict__ DT, const V* __restrict__ rhs,
+                                                          V* __restrict__ y) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    V* sx = reinterpret_cast<V*>(smem_raw);  // [epb][NS][bs] neighbour values
+    V* st = sx + (size_t)epb * NS * bs;      // [epb][bs] partial sums
+    const int l0 = blockIdx.x * epb;
+    const int ne = min(epb, nel - l0);
+    for (int k = threadIdx.x; k < ne * NS * bs; k += blockDim.x) {
+        const int le = k / (NS * bs), rem = k % (NS * bs), s = rem / bs, c = rem % bs;
+        V v = 0;
+        if (s > 0) {
+            const int e = elems[l0 + le];
+            const int col = cols[(size_t)e * NS + s];
+            if (col >= 0 && (FWD ? ecolor[col] < colour : ecolor[col] > colour)) v = y[(size_t)col * bs + c];
+        }
+        sx[k] = v;
+    }
+    __syncthreads();
+    for (int g = threadIdx.x; g < ne * bs; g += blockDim.x) {
+        const int le = g / bs, r = g - le * bs;
+        const int e = elems[l0 + le];
+        const V* xs = sx + (size_t)le * NS * bs;
+        V acc0 = 0, acc1 = 0;
+#pragma unroll
+        for (int s = 1; s < NS; ++s) {
+            const int col = cols[(size_t)e * NS + s];
+            if (col < 0 || !(FWD ? ecolor[col] < colour : ecolor[col] > colour)) continue;
+            const T* Jc = JT + ((size_t)e * NS + s) * bs * bs + r;
+            const V* xv = xs + s * bs;
+            int c = 0;
+            for (; c + 1 < bs; c += 2) {
+                acc0 += (V)Jc[(size_t)c * bs] * xv[c];
+                acc1 += (V)Jc[(size_t)(c + 1) * bs] * xv[c + 1];
+            }
+            if (c < bs) acc0 += (V)Jc[(size_t)c * bs] * xv[c];
+        }
+        const V acc = acc0 + acc1;
+        st[g] = FWD ? rhs[(size_t)e * bs + r] - acc : acc;
+    }
+    __syncthreads();
+    for (int g = threadIdx.x; g < ne * bs; g += blockDim.x) {
+        const int le = g / bs, r = g - le * bs;
+        const int e = elems[l0 + le];
+        const V* tv = st + (size_t)le * bs;
+        const T* Dc = DT + (size_t)e * bs * bs + r;
+        V acc0 = 0, acc1 = 0;
+        int c = 0;
+        for (; c + 1 < bs; c += 2) {
+            acc0 += (V)Dc[(size_t)c * bs] * tv[c];
+            acc1 += (V)Dc[(size_t)(c + 1) * bs] * tv[c + 1];
+        }
+        if (c < bs) acc0 += (V)Dc[(size_t)c * bs] * tv[c];
+        const size_t i = (size_t)e * bs + r;
+        y[i] = FWD ? acc0 + acc1 : y[i] - (acc0 + acc1);
+    }
+}
+
+__global__ void convert_transpose_kernel(size_t nblocks, int bs, const double* __restrict__ a, float* __restrict__ b) {
+    const size_t bb = (size_t)bs * bs;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nblocks * bb;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const size_t blk = i / bb, k = i - blk * bb;
+        const int c = (int)(k / bs), r = (int)(k - (size_t)c * bs);  // destination [c][r]
+        b[i] = (float)a[blk * bb + (size_t)r * bs + c];
+    }
+}
+
+// elements spanned by at most RT_THREADS consecutive rows
+int span_elems(int bs) { return (RT_THREADS + bs - 2) / bs + 1; }
+
+}  // namespace
+
+template <class T, class V>
+int bsrT_apply(int nelem, int nslots, int bs, const int* cols, const T* JT, const V* x, const V* b, V* y,
+               hipStream_t st) {
+    const long ntot = (long)nelem * bs;
+    const int grid = (int)((ntot + RT_THREADS - 1) / RT_THREADS);
+    const size_t sh = (size_t)span_elems(bs) * nslots * bs * sizeof(V);
+    switch (nslots) {
+    case 3: bsrT_kernel<T, V, 3><<<grid, RT_THREADS, sh, st>>>(nelem, bs, cols, JT, x, b, y); break;
+    case 5: bsrT_kernel<T, V, 5><<<grid, RT_THREADS, sh, st>>>(nelem, bs, cols, JT, x, b, y); break;
+    case 7: bsrT_kernel<T, V, 7><<<grid, RT_THREADS, sh, st>>>(nelem, bs, cols, JT, x, b, y); break;
+    default: return -1;
+    }
+    return (int)hipGetLastError();
+}
+template int bsrT_apply<float, double>(int, int, int, const int*, const float*, const double*, const double*, double*,
+                                       hipStream_t);
+
+template <class T, class V>
+int diagT_apply(int nelem, int bs, const T* DT, const V* x, V* y, hipStream_t st) {
+    const long ntot = (long)nelem * bs;
+    const int grid = (int)((ntot + RT_THREADS - 1) / RT_THREADS);
+    diagT_kernel<T, V><<<grid, RT_THREADS, (size_t)span_elems(bs) * bs * sizeof(V), st>>>(nelem, bs, DT, x, y);
+    return (int)hipGetLastError();
+}
+template int diagT_apply<float, double>(int, int, const float*, const double*, double*, hipStream_t);
+template int diagT_apply<double, double>(int, int, const double*, const double*, double*, hipStream_t);
+
+template <class T, class V>
+int sgsT_sweep(bool forward, int nel, const int* elems, const int* ecolor, int colour, int nslots, int bs,
+               const int* cols, const T* JT, const T* DT, const V* rhs, V* y, hipStream_t st) {
+    if (nel <= 0) return 0;
+    const int epb = std::max(1, RT_THREADS / bs);
+    const int grid = (nel + epb - 1) / epb;
+    const size_t sh = (size_t)epb * (nslots + 1) * bs * sizeof(V);
+    if (sh > 160 * 1024) return -1;
+#define ICE_SGST(NSV)                                                                                               \
+    case NSV:                                                                                                       \
+        if (forward)                                                                                                \
+            sgsT_kernel<T, V, NSV, true><<<grid, RT_THREADS, sh, st>>>(nel, epb, elems, ecolor, colour, bs, cols,  \
+                                                                       JT, DT, rhs, y);                             \
+        else                                                                                                        \
+            sgsT_kernel<T, V, NSV, false><<<grid, RT_THREADS, sh, st>>>(nel, epb, elems, ecolor, colour, bs, cols, \
+                                                                        JT, DT, rhs, y);                            \
+        break;
+    switch (nslots) {
+        ICE_SGST(3)
+        ICE_SGST(5)
+        ICE_SGST(7)
+    default: return -1;
+    }
+#undef ICE_SGST
+    return (int)hipGetLastError();
+}
+template int sgsT_sweep<float, double>(bool, int, const int*, const int*, int, int, int, const int*, const float*,
+                                       const float*, const double*, double*, hipStream_t);
+template int sgsT_sweep<double, double>(bool, int, const int*, const int*, int, int, int, const int*, const double*,
+                                        const double*, const double*, double*, hipStream_t);
+
+int convert_transpose_fp32(size_t nblocks, int bs, const double* a, float* b, hipStream_t st) {
+    convert_transpose_kernel<<<8192, 256, 0, st>>>(nblocks, bs, a, b);
     return (int)hipGetLastError();
 }
 

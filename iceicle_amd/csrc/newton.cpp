@@ -24,7 +24,17 @@
 namespace ice {
 // krylov.hip
 template <class T>
-int block_inverse(int nelem, int nslots, int bs, const double* J, T* Dinv, int* nsingular, hipStream_t st);
+int block_inverse(int nelem, int nslots, int bs, const double* J, T* Dinv, int* nsingular, bool transposed,
+                  hipStream_t st);
+template <class T, class V>
+int bsrT_apply(int nelem, int nslots, int bs, const int* cols, const T* JT, const V* x, const V* b, V* y,
+               hipStream_t st);
+template <class T, class V>
+int diagT_apply(int nelem, int bs, const T* DT, const V* x, V* y, hipStream_t st);
+template <class T, class V>
+int sgsT_sweep(bool forward, int nel, const int* elems, const int* ecolor, int colour, int nslots, int bs,
+               const int* cols, const T* JT, const T* DT, const V* rhs, V* y, hipStream_t st);
+int convert_transpose_fp32(size_t nblocks, int bs, const double* a, float* b, hipStream_t st);
 template <class T, class V>
 int block_diag_apply(int nelem, int bs, const T* D, const V* x, V* y, hipStream_t st);
 int convert_to_fp32(size_t n, const double* a, float* b, hipStream_t st);
@@ -40,6 +50,9 @@ int launch_axpby2(size_t n, double a, const double* x, double b, double* y, hipS
 int launch_arnoldi(int k, int m, const double* h1, const double* h2, double* sc, hipStream_t st);
 int launch_hessenberg_solve(int k, int m, const double* sc, double* y, hipStream_t st);
 int launch_gmres_init(int m, const double* nrm2, double* sc, hipStream_t st);
+template <class T, class V>
+int sgs_sweep(bool forward, int nrows, const int* elems, const int* ecolor, int c, int nslots, int bs,
+              const int* cols, const T* J, const T* Dinv, const V* r, V* y, hipStream_t st);
 }  // namespace ice
 
 namespace {
@@ -151,8 +164,8 @@ struct ice_newton {
     int m = 30, npart = 0, nred = 0;
     // device memory
     double* J64 = nullptr;
-    float* J32 = nullptr;
-    void* Dinv = nullptr;       // float or double [nelem][bs][bs]
+    float* J32 = nullptr;       // FP32 copy, each block column-major ([c][r])
+    void* Dinv = nullptr;       // [nelem][bs][bs] factored diagonal blocks: FP32 column-major, or FP64 row-major
     double* V = nullptr;        // (m+1) x n Krylov basis
     double* z = nullptr;        // n
     double* t = nullptr;        // n
@@ -167,6 +180,9 @@ struct ice_newton {
     double* sc = nullptr;       // Hessenberg, rotations, g, norms
     double* y = nullptr;        // m
     int* nsing = nullptr;
+    int* ecolor = nullptr;      // element colour (multicolour Gauss-Seidel)
+    int* celems = nullptr;      // elements grouped by colour
+    std::vector<int> coff;      // colour offsets into celems
     double* host = nullptr;     // pinned scalars
     bool factored = false;
     std::vector<void*> owned;
@@ -200,21 +216,69 @@ struct ice_newton {
     // Krylov operator w = A z and preconditioner z = M^-1 v
     void apply_A(const double* x, double* w, hipStream_t st) {
         const int* cols = ice::disc_jcols(disc);
-        if (fp32()) NT_LAUNCH((ice::bsr_apply<float, double>(nelem, nslots, bs, cols, J32, x, nullptr, w, st)));
+        if (fp32()) NT_LAUNCH((ice::bsrT_apply<float, double>(nelem, nslots, bs, cols, J32, x, nullptr, w, st)));
         else NT_LAUNCH((ice::bsr_apply<double, double>(nelem, nslots, bs, cols, J64, x, nullptr, w, st)));
     }
     void apply_M(const double* v, double* out, hipStream_t st) {
-        if (fp32()) NT_LAUNCH((ice::block_diag_apply<float, double>(nelem, bs, (const float*)Dinv, v, out, st)));
+        if (opt.preconditioner == ICE_PC_BLOCK_SGS) {
+            const int nc = (int)coff.size() - 1;
+            const int* cols = ice::disc_jcols(disc);
+            for (int pass = 0; pass < 2; ++pass)
+                for (int k = 0; k < nc; ++k) {
+                    const bool fwd = pass == 0;
+                    const int c = fwd ? k : nc - 1 - k;
+                    if (!fwd && c == nc - 1) continue;  // last colour: backward sweep leaves it unchanged
+                    const int* el = celems + coff[c];
+                    const int nr = coff[c + 1] - coff[c];
+                    if (fp32())
+                        NT_LAUNCH((ice::sgsT_sweep<float, double>(fwd, nr, el, ecolor, c, nslots, bs, cols, J32,
+                                                                  (const float*)Dinv, v, out, st)));
+                    else
+                        NT_LAUNCH((ice::sgs_sweep<double, double>(fwd, nr, el, ecolor, c, nslots, bs, cols, J64,
+                                                                  (const double*)Dinv, v, out, st)));
+                }
+            return;
+        }
+        if (fp32()) NT_LAUNCH((ice::diagT_apply<float, double>(nelem, bs, (const float*)Dinv, v, out, st)));
         else NT_LAUNCH((ice::block_diag_apply<double, double>(nelem, bs, (const double*)Dinv, v, out, st)));
+    }
+
+    // greedy colouring of the element face graph (neighbours differ)
+    void colour() {
+        std::vector<int> cols((size_t)nelem * nslots);
+        int ns = 0, b = 0;
+        if (ice_jacobian_layout(disc, &ns, &b, cols.data()) != ICE_OK) throw Fail{ICE_ERR_STATE, "no Jacobian layout"};
+        std::vector<int> col(nelem, -1);
+        int nc = 0;
+        for (int e = 0; e < nelem; ++e) {
+            unsigned used = 0;
+            for (int s = 1; s < nslots; ++s) {
+                const int nb = cols[(size_t)e * nslots + s];
+                if (nb >= 0 && nb != e && col[nb] >= 0) used |= 1u << col[nb];
+            }
+            int c = 0;
+            while (used & (1u << c)) ++c;
+            col[e] = c;
+            nc = std::max(nc, c + 1);
+        }
+        coff.assign(nc + 1, 0);
+        for (int e = 0; e < nelem; ++e) ++coff[col[e] + 1];
+        for (int c = 0; c < nc; ++c) coff[c + 1] += coff[c];
+        std::vector<int> order(nelem), fill(coff.begin(), coff.end() - 1);
+        for (int e = 0; e < nelem; ++e) order[fill[col[e]]++] = e;
+        ecolor = alloc<int>(nelem);
+        celems = alloc<int>(nelem);
+        NT_HIP(hipMemcpy(ecolor, col.data(), nelem * sizeof(int), hipMemcpyHostToDevice));
+        NT_HIP(hipMemcpy(celems, order.data(), nelem * sizeof(int), hipMemcpyHostToDevice));
     }
 
     void factor(hipStream_t st) {
         NT_HIP(hipMemsetAsync(nsing, 0, sizeof(int), st));
         if (fp32()) {
-            NT_LAUNCH(ice::convert_to_fp32(nblkvals, J64, J32, st));
-            NT_LAUNCH(ice::block_inverse<float>(nelem, nslots, bs, J64, (float*)Dinv, nsing, st));
+            NT_LAUNCH(ice::convert_transpose_fp32((size_t)nelem * nslots, bs, J64, J32, st));
+            NT_LAUNCH(ice::block_inverse<float>(nelem, nslots, bs, J64, (float*)Dinv, nsing, true, st));
         } else {
-            NT_LAUNCH(ice::block_inverse<double>(nelem, nslots, bs, J64, (double*)Dinv, nsing, st));
+            NT_LAUNCH(ice::block_inverse<double>(nelem, nslots, bs, J64, (double*)Dinv, nsing, false, st));
         }
         factored = true;
     }
@@ -335,6 +399,7 @@ int ice_newton_opts_default(ice_newton_opts* o) {
     o->linear_rtol = 1e-5;
     o->refine_max = 3;
     o->verbose = 0;
+    o->preconditioner = ICE_PC_BLOCK_SGS;
     return ICE_OK;
 }
 
@@ -349,7 +414,8 @@ int ice_newton_create(ice_disc* d, const ice_newton_opts* opts, ice_newton** out
     else ice_newton_opts_default(&o);
     if (o.gmres_restart < 1 || o.gmres_restart > 200 || o.kmax < 0 || o.gmres_maxit < 1 ||
         (o.jacobian_precision != ICE_PREC_FP32 && o.jacobian_precision != ICE_PREC_FP64) ||
-        (o.linesearch < ICE_LS_NONE || o.linesearch > ICE_LS_CORRIGAN)) {
+        (o.linesearch < ICE_LS_NONE || o.linesearch > ICE_LS_CORRIGAN) ||
+        (o.preconditioner != ICE_PC_BLOCK_JACOBI && o.preconditioner != ICE_PC_BLOCK_SGS)) {
         ice::set_error("invalid Newton options");
         return ICE_ERR_ARG;
     }
@@ -378,6 +444,7 @@ int ice_newton_create(ice_disc* d, const ice_newton_opts* opts, ice_newton** out
         s->sc = s->alloc<double>((size_t)(s->m + 1) * s->m + 2 * s->m + (s->m + 1) + 2);
         s->y = s->alloc<double>(s->m);
         s->nsing = s->alloc<int>(1);
+        if (o.preconditioner == ICE_PC_BLOCK_SGS) s->colour();
         NT_HIP(hipHostMalloc((void**)&s->host, 8 * sizeof(double)));
         *out = s.release();
         return ICE_OK;

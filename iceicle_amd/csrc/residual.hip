@@ -1,14 +1,17 @@
 // DG residual kernels for CDNA4 (gfx950), FP64.
 //
-// One residual = two launches on the caller's stream:
-//  1. face kernels (interior + boundary): one thread per face quadrature
-//     point; evaluate both traces (value, physical gradient, normal Hessian
-//     term), the numerical fluxes and the DDG single-valued gradient, and
-//     write the weighted normal flux Q[face][fq][v] (and the DDGIC lifting
-//     vector when sigma_ic != 0).  This is ConservationLawDDG::trace_integral /
-//     boundaryIntegral (conservation_law.hpp:434-1176) up to the final
-//     multiplication by the trace basis.
-//  2. element kernel: a workgroup owns a tile of elements whose coefficients
+// One residual = three launches on the caller's stream:
+//  1. trace kernel: per element and element face, the trace (value and the
+//     element's share of the DDG gradient) at the face quadrature points,
+//     from the LDS-staged coefficients by sum factorisation;
+//  2. flux kernel (interior + boundary faces): one thread per face
+//     quadrature point; the numerical fluxes and the DDG single-valued
+//     gradient from the two stored traces, written as the weighted normal
+//     flux Q[face][fq][v] (and the DDGIC lifting vector when sigma_ic != 0).
+//     Together this is ConservationLawDDG::trace_integral / boundaryIntegral
+//     (conservation_law.hpp:434-1176) up to the final multiplication by the
+//     trace basis.
+//  3. element kernel: a workgroup owns a tile of elements whose coefficients
 //     are staged in LDS with one coalesced copy; phase 1 has one thread per
 //     (element, volume quadrature point) and evaluates the physical flux
 //     contracted into the reference frame; phase 2 has one thread per
@@ -26,6 +29,18 @@
 #include "fields.h"
 #include "kernels.h"
 #include "physics_dev.h"
+
+// Minimum waves per SIMD requested from the register allocator for the
+// three residual kernels (build-time tuning knobs; 1 = no constraint).
+#ifndef ICE_TRACE_MINW
+#define ICE_TRACE_MINW 1
+#endif
+#ifndef ICE_FLUX_MINW
+#define ICE_FLUX_MINW 1
+#endif
+#ifndef ICE_ELEM_MINW
+#define ICE_ELEM_MINW 1
+#endif
 
 namespace ice {
 
@@ -404,7 +419,7 @@ __device__ __forceinline__ void face_trace_sf(const DevSpace& S, const double* U
 }
 
 template <int D, int P, class PH>
-__global__ __launch_bounds__(256) void trace_kernel(DevSpace S, PhysParams ph, const double* __restrict__ U,
+__global__ __launch_bounds__(256, ICE_TRACE_MINW) void trace_kernel(DevSpace S, PhysParams ph, const double* __restrict__ U,
                                                     double* __restrict__ T) {
     constexpr int N = P + 1;
     constexpr int NB = Pow<N, D>::value;
@@ -491,7 +506,7 @@ __global__ __launch_bounds__(256) void trace_kernel(DevSpace S, PhysParams ph, c
 // Numerical fluxes at face points from stored traces: interior points first
 // (blocks [0, nbi)), then boundary points.
 template <int D, int P, class PH>
-__global__ __launch_bounds__(256) void flux_kernel(DevSpace S, PhysParams ph, DiscDev dd, const double* __restrict__ T,
+__global__ __launch_bounds__(256, ICE_FLUX_MINW) void flux_kernel(DevSpace S, PhysParams ph, DiscDev dd, const double* __restrict__ T,
                                                    int nbi) {
     constexpr int N = P + 1;
     constexpr int NFQ = Pow<N, D - 1>::value;
@@ -942,6 +957,262 @@ __global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph,
     for (int v = 0; v < M; ++v) Re[v] = r[v];
 }
 
+// 2D element kernel by sum factorisation.  Thread k of an element is the
+// tensor pair (k0, k1) (dimension 0 slowest), taken in four phases:
+//  1a (a, q1):  T0[a][q1] = sum_b U[a][b] l_b(x_q1),  T1 = sum_b U[a][b] l_b'(x_q1)
+//  1b (q0, q1): u, du/dxi from T, geometry, physical flux -> reference-frame flux G
+//  2a (q0, i1): S0 = sum_q1 G0[q0][q1] l_i1(x_q1),  S1 = sum_q1 G1[q0][q1] l_i1'(x_q1)
+//  2b (i0, i1): sum_q0 S0[q0][i1] l_i0'(x_q0) + S1[q0][i1] l_i0(x_q0), minus the
+//               source load, plus the lifted face fluxes; R written once.
+// Same terms as element_kernel (domain_integral, conservation_law.hpp:244-314,
+// and the trace_integral lifting), 2N instead of N^2 multiply-adds per
+// point and basis function.  The lifting skips faces on which the basis
+// function's node does not lie (its trace there is zero).
+template <int P, class PH>
+__global__ __launch_bounds__(256, ICE_ELEM_MINW) void element_kernel_2d(DevSpace S, PhysParams ph, const double* __restrict__ srcload,
+                                                         const double* __restrict__ U, const double* __restrict__ Q,
+                                                         const double* __restrict__ Pic, double* __restrict__ R,
+                                                         unsigned long long* __restrict__ wave,
+                                                         double* __restrict__ wave_part) {
+    constexpr int D = 2;
+    using TL = ElemTile<D, P>;
+    constexpr int N = TL::N, NB = TL::NB, EPB = TL::EPB;
+    constexpr int NFQ = N;
+    constexpr int M = PH::M;
+    // the staged coefficients are dead after phase 1a, so the reference-frame
+    // fluxes of phase 1b reuse their storage
+    __shared__ double sUG[EPB * NB * M * D];
+    __shared__ double sT[EPB * 2 * NB * M];
+    double* sU = sUG;
+    double* sG = sUG;
+    const int e0 = blockIdx.x * EPB;
+    const int le = threadIdx.x / NB;
+    const int k = threadIdx.x - le * NB;
+    const int k0 = k / N, k1 = k - (k / N) * N;
+    const int e = e0 + le;
+    const bool active = (threadIdx.x < TL::NT) && (e < S.nelem);
+    {
+        const size_t base = (size_t)e0 * NB * M;
+        const size_t total = (size_t)S.nelem * NB * M;
+        for (int idx = threadIdx.x; idx < EPB * NB * M; idx += blockDim.x)
+            sU[idx] = (base + idx < total) ? U[base + idx] : 0.0;
+    }
+    __syncthreads();
+    const double* xs = S.tab;
+    const double* L0 = S.tab + (N + 2);
+    const double* L1 = L0 + (N + 2) * N;
+    const double* wgl = L1 + 2 * (N + 2) * N;
+    double* T0 = sT + (size_t)le * 2 * NB * M;  // [a][q1][v]
+    double* T1 = T0 + NB * M;
+    // phase 1a: (a = k0, q1 = k1)
+    if (active) {
+        const double* Ua = sU + ((size_t)le * NB + k0 * N) * M;
+        double l0[N], l1[N];
+#pragma unroll
+        for (int b = 0; b < N; ++b) {
+            l0[b] = L0[k1 * N + b];
+            l1[b] = L1[k1 * N + b];
+        }
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            double t0 = 0, t1 = 0;
+#pragma unroll
+            for (int b = 0; b < N; ++b) {
+                const double c = Ua[b * M + v];
+                t0 += c * l0[b];
+                t1 += c * l1[b];
+            }
+            T0[k * M + v] = t0;
+            T1[k * M + v] = t1;
+        }
+    }
+    __syncthreads();
+    double X[1 << D][D];
+    double lam_w = 0.0, vis_w = 0.0;
+    // phase 1b: (q0 = k0, q1 = k1)
+    if (active) {
+        load_vertices<D>(S.X, e, X);
+        double xi[D] = {xs[k0], xs[k1]};
+        const double wq = wgl[k0] * wgl[k1];
+        double x[D], J[D][D], Hm[D][D][D], A[D][D];
+        q1_map<D>(X, xi, x, J, Hm);
+        const double det = det_adj<D>(J, A);
+        const double detc = fmax(0.0, det);
+        const double detg = (det == 0.0) ? 1.0 : det;
+        double l0[N], l1[N];
+#pragma unroll
+        for (int a = 0; a < N; ++a) {
+            l0[a] = L0[k0 * N + a];
+            l1[a] = L1[k0 * N + a];
+        }
+        double u[M], gx[M * D], F[M * D];
+        const double idetg = 1.0 / detg;
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            double s0 = 0, s1 = 0, sg0 = 0;
+#pragma unroll
+            for (int a = 0; a < N; ++a) {
+                const double t0 = T0[(a * N + k1) * M + v];
+                s0 += t0 * l0[a];
+                sg0 += t0 * l1[a];
+                s1 += T1[(a * N + k1) * M + v] * l0[a];
+            }
+            u[v] = s0;
+            // grad_x = grad_xi A / det (finite_element.hpp:220-258)
+            gx[v * D + 0] = (sg0 * A[0][0] + s1 * A[1][0]) * idetg;
+            gx[v * D + 1] = (sg0 * A[0][1] + s1 * A[1][1]) * idetg;
+        }
+        PH::flux(ph, u, gx, F);
+        if (wave) {
+            lam_w = PH::wave_speed(ph, u);
+            vis_w = PH::visc_at(ph, u);
+            if (e == S.nelem - 1 && k == NB - 1) wave[1] = (unsigned long long)__double_as_longlong(lam_w);
+        }
+        const double fac = detc * wq * idetg;
+        double* Gq = sG + ((size_t)le * NB + k) * M * D;
+#pragma unroll
+        for (int v = 0; v < M; ++v)
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                double sacc = 0;
+#pragma unroll
+                for (int j = 0; j < D; ++j) sacc += F[v * D + j] * A[a][j];
+                Gq[v * D + a] = sacc * fac;
+            }
+    }
+    __shared__ double red[2 * 4];
+    if (wave) {
+        for (int off = 32; off > 0; off >>= 1) {
+            lam_w = fmax(lam_w, __shfl_xor(lam_w, off));
+            vis_w = fmax(vis_w, __shfl_xor(vis_w, off));
+        }
+        if ((threadIdx.x & 63) == 0) {
+            red[2 * (threadIdx.x >> 6)] = lam_w;
+            red[2 * (threadIdx.x >> 6) + 1] = vis_w;
+        }
+    }
+    __syncthreads();
+    if (wave && threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { a = fmax(a, red[2 * w]); b = fmax(b, red[2 * w + 1]); }
+        wave_part[2 * blockIdx.x] = a;
+        wave_part[2 * blockIdx.x + 1] = b;
+    }
+    // phase 2a: (q0 = k0, i1 = k1); T0/T1 reused as S0/S1 [q0][i1][v]
+    if (active) {
+        double l0[N], l1[N];
+#pragma unroll
+        for (int q1 = 0; q1 < N; ++q1) {
+            l0[q1] = L0[q1 * N + k1];
+            l1[q1] = L1[q1 * N + k1];
+        }
+        const double* Gr = sG + ((size_t)le * NB + k0 * N) * M * D;
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            double s0 = 0, s1 = 0;
+#pragma unroll
+            for (int q1 = 0; q1 < N; ++q1) {
+                s0 += Gr[(q1 * M + v) * D + 0] * l0[q1];
+                s1 += Gr[(q1 * M + v) * D + 1] * l1[q1];
+            }
+            T0[k * M + v] = s0;
+            T1[k * M + v] = s1;
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    // phase 2b: (i0 = k0, i1 = k1)
+    double r[M];
+    {
+        double l0[N], l1[N];
+#pragma unroll
+        for (int q0 = 0; q0 < N; ++q0) {
+            l0[q0] = L0[q0 * N + k0];
+            l1[q0] = L1[q0 * N + k0];
+        }
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            double sacc = 0;
+#pragma unroll
+            for (int q0 = 0; q0 < N; ++q0)
+                sacc += T0[(q0 * N + k1) * M + v] * l1[q0] + T1[(q0 * N + k1) * M + v] * l0[q0];
+            r[v] = sacc;
+        }
+    }
+    if (srcload) {
+        const double* sl = srcload + ((size_t)e * NB + k) * M;
+#pragma unroll
+        for (int v = 0; v < M; ++v) r[v] -= sl[v];
+    }
+    const int kd[D] = {k0, k1};
+#pragma unroll 1
+    for (int j = 0; j < 2 * D; ++j) {
+        const int ef = S.efaces[(size_t)e * 2 * D + j];
+        if (ef < 0) continue;
+        const size_t slot = (size_t)(ef >> 4);
+        const int side = (ef >> 3) & 1;
+        const int orient = ef & 7;
+        const double sgn = side ? -1.0 : 1.0;
+        const signed char* fp = S.fpt + ((size_t)(j * S.norient + orient) * NFQ) * D;
+        const int kn = j % D;
+        const bool on_face = kd[kn] == ((j < D) ? 0 : N - 1);
+        if (on_face) {
+            const double* Qs = Q + slot * NFQ * M;
+#pragma unroll
+            for (int fq = 0; fq < NFQ; ++fq) {
+                double phi = 1.0;
+#pragma unroll
+                for (int d = 0; d < D; ++d) phi *= L0[fp[fq * D + d] * N + kd[d]];
+#pragma unroll
+                for (int v = 0; v < M; ++v) r[v] += sgn * Qs[fq * M + v] * phi;
+            }
+        }
+        if constexpr (PH::HAS_HOMOGENEITY) {
+            if (Pic) {
+                const double* Ps = Pic + slot * NFQ * M * D;
+#pragma unroll 1
+                for (int fq = 0; fq < NFQ; ++fq) {
+                    double xi[D], cg[D];
+#pragma unroll
+                    for (int d = 0; d < D; ++d) xi[d] = xs[fp[fq * D + d]];
+#pragma unroll
+                    for (int a = 0; a < D; ++a) {
+                        double t = 1.0;
+#pragma unroll
+                        for (int d = 0; d < D; ++d) {
+                            const int sidx = fp[fq * D + d];
+                            t *= (d == a) ? L1[sidx * N + kd[d]] : L0[sidx * N + kd[d]];
+                        }
+                        cg[a] = t;
+                    }
+                    double J[D][D], Hm[D][D][D], A[D][D];
+                    q1_map<D>(X, xi, nullptr, J, Hm);
+                    double det = det_adj<D>(J, A);
+                    det = (det == 0.0) ? 1.0 : det;
+                    double gphi[D];
+#pragma unroll
+                    for (int jj = 0; jj < D; ++jj) {
+                        double sacc = 0;
+#pragma unroll
+                        for (int a = 0; a < D; ++a) sacc += cg[a] * A[a][jj];
+                        gphi[jj] = sacc / det;
+                    }
+#pragma unroll
+                    for (int v = 0; v < M; ++v) {
+                        double sacc = 0;
+#pragma unroll
+                        for (int d = 0; d < D; ++d) sacc += Ps[(fq * M + v) * D + d] * gphi[d];
+                        r[v] -= sacc;
+                    }
+                }
+            }
+        }
+    }
+    double* Re = R + ((size_t)e * NB + k) * M;
+#pragma unroll
+    for (int v = 0; v < M; ++v) Re[v] = r[v];
+}
+
 // Fold the per-block maxima of one residual into the running maxima
 // (lambda_max of BurgersFlux never resets, burgers.hpp:89; visc_max, navier_stokes.hpp:964-966).
 __global__ void wave_reduce_kernel(const double* __restrict__ part, int nblk, unsigned long long* __restrict__ wave) {
@@ -1151,11 +1422,15 @@ int launch_residual(const DevSpace& S, const PhysParams& ph, const DiscDev& dd, 
         constexpr int EPB = (256 / TPE) > 0 ? (256 / TPE) : 1;
         trace_kernel<D, P, PH><<<(S.nelem + EPB - 1) / EPB, 256, 0, st>>>(S, ph, U, d.T);
     }
-    if (nbi + nbb > 0) flux_kernel<D, P, PH><<<nbi + nbb, 256, 0, st>>>(S, ph, d, d.T, nbi);
     if (ev) (void)hipEventRecord(ev[1], st);
+    if (nbi + nbb > 0) flux_kernel<D, P, PH><<<nbi + nbb, 256, 0, st>>>(S, ph, d, d.T, nbi);
+    if (ev) (void)hipEventRecord(ev[2], st);
     using T = ElemTile<D, P>;
     const int nblk = (S.nelem + T::EPB - 1) / T::EPB;
-    element_kernel<D, P, PH><<<nblk, 256, 0, st>>>(S, ph, d.srcload, U, d.Q, d.Pic, R, d.wave, d.wave_part);
+    if constexpr (D == 2)
+        element_kernel_2d<P, PH><<<nblk, 256, 0, st>>>(S, ph, d.srcload, U, d.Q, d.Pic, R, d.wave, d.wave_part);
+    else
+        element_kernel<D, P, PH><<<nblk, 256, 0, st>>>(S, ph, d.srcload, U, d.Q, d.Pic, R, d.wave, d.wave_part);
     if (d.wave) wave_reduce_kernel<<<1, 256, 0, st>>>(d.wave_part, nblk, d.wave);
     if (ev) (void)hipEventRecord(ev[3], st);
     return (int)hipGetLastError();

@@ -177,7 +177,7 @@ class ConservationLawDDG:
         check(lib().ice_profile_begin(self.h, max_calls))
 
     def profile_end(self):
-        """Summed milliseconds of (interior-face, boundary-face, element) kernels
+        """Summed milliseconds of the (face trace, face flux, element) kernels
         and the number of residuals profiled since profile_begin."""
         ms = (C.c_double * 3)()
         n = C.c_int()

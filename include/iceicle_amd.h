@@ -260,7 +260,10 @@ int ice_jacobian_apply(const ice_disc* disc, const double* blocks_dev, const dou
  * petsc_newton.hpp:174-183), picks alpha by the linesearch on
  * phi(alpha) = ||R(u - alpha du)||_2 (non-finite -> 1e9), sets u -= alpha du,
  * and stops when ||R||_2 <= tau_abs + tau_rel * ||R(u_0)||_2 or after kmax
- * steps.  With ICE_PREC_FP32 the Krylov operator and the block-Jacobi factors
+ * steps.  The preconditioner is block Jacobi or one multicolour block
+ * symmetric Gauss-Seidel sweep (elements coloured so that face neighbours
+ * differ; the reference's default is PETSc's point SOR).  With ICE_PREC_FP32
+ * the Krylov operator and the block-Jacobi factors
  * are FP32 copies of the FP64 blocks (FP64 vectors and accumulation) and the
  * correction is refined against the FP64 blocks: r = R - J du in FP64 until
  * ||r|| <= linear_rtol ||R|| or refine_max passes. */
@@ -269,6 +272,8 @@ int ice_jacobian_apply(const ice_disc* disc, const double* blocks_dev, const dou
 #define ICE_LS_CORRIGAN 2
 #define ICE_PREC_FP64 0
 #define ICE_PREC_FP32 1
+#define ICE_PC_BLOCK_JACOBI 0
+#define ICE_PC_BLOCK_SGS 1
 #define ICE_NEWTON_MAX_HIST 64
 
 typedef struct {
@@ -286,6 +291,8 @@ typedef struct {
     double linear_rtol;       /* 1e-5 */
     int refine_max;           /* FP64 refinement passes (3) */
     int verbose;              /* 1: one line per Newton step on stdout */
+    int preconditioner;       /* ICE_PC_*: block Jacobi, or multicolour block
+                                 symmetric Gauss-Seidel (default) */
 } ice_newton_opts;
 
 typedef struct {
@@ -350,10 +357,10 @@ int ice_stepper_status(ice_stepper* stepper, double* time, double* dt, long long
 
 /* Kernel timing (no reference counterpart; the reference has no
  * instrumentation).  Between begin and end, every residual on this handle
- * records HIP events around its face launch (interior + boundary faces) and
- * its element launch on the caller's stream; end waits for them and writes
- * the summed milliseconds to stage_ms[0] (faces) and stage_ms[2] (elements),
- * stage_ms[1] = 0, and the number of residuals to *ncalls. */
+ * records HIP events around its three launches on the caller's stream; end
+ * waits for them and writes the summed milliseconds to stage_ms[0] (element
+ * face traces), stage_ms[1] (face fluxes) and stage_ms[2] (element volume
+ * terms and lifting), and the number of residuals to *ncalls. */
 int ice_profile_begin(ice_disc* disc, int max_calls);
 int ice_profile_end(ice_disc* disc, double* stage_ms, int* ncalls);
 
