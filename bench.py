@@ -135,6 +135,39 @@ def main():
     b_alg = workloads.residual_bytes(D, order, neq, E, nnode, F)
     res_gbs = b_alg / (ms_per_step * 1e-3) / 1e9
 
+    # mixed-precision residual on the same state: throughput and its error
+    # against the FP64 residual (which matches the CPU oracle to 1e-15)
+    for _ in range(args.warmup):
+        engine.form_residual(space, disc, u, r, precision="mixed")
+    r64 = r.clone()
+    engine.form_residual(space, disc, u, r64)
+    torch.cuda.synchronize()
+    disc.profile_begin(args.steps)
+    start.record()
+    for _ in range(args.steps):
+        engine.form_residual(space, disc, u, r, precision="mixed")
+    stop.record()
+    torch.cuda.synchronize()
+    mixed_ms = start.elapsed_time(stop) / args.steps
+    mstage, mcalls = disc.profile_end()
+    mixed_err = float((r - r64).abs().max() / r64.abs().max())
+    # the same comparison from the uniform free-stream state, where R is far from zero
+    uf = torch.from_numpy(np.tile([1.0, 0.3, 0.0, 1.0 / 1.4 / 0.4 + 0.045], disc.size // 4)).cuda()
+    rf64, rf32 = torch.empty_like(uf), torch.empty_like(uf)
+    engine.form_residual(space, disc, uf, rf64)
+    engine.form_residual(space, disc, uf, rf32, precision="mixed")
+    torch.cuda.synchronize()
+    mixed_err_fs = float((rf32 - rf64).abs().max() / rf64.abs().max())
+    mixed = {
+        "value": round(ndof / (mixed_ms * 1e-3) / 1e9, 4),
+        "unit": "GDOF/s",
+        "ms_per_step": round(mixed_ms, 6),
+        "rel_err_vs_fp64": float(f"{mixed_err:.3e}"),
+        "rel_err_vs_fp64_free_stream": float(f"{mixed_err_fs:.3e}"),
+        "kernels_ms": {k: round(v / mcalls, 6) for k, v in zip(("trace", "flux", "element"), mstage)},
+        "arithmetic": "FP32 interpolation, fluxes and face data; FP64 geometry, source load and output",
+    }
+
     newton = None
     if not args.no_newton:
         # Newton-linesearch from a uniform free-stream state (rho 1, u 0.3,
@@ -193,6 +226,7 @@ def main():
         "residual_roofline": {"bytes": int(b_alg), "achieved_GBs": round(res_gbs, 2),
                               "frac": round(res_gbs / HBM_PEAK_GBS, 5)},
     }
+    out["mixed"] = mixed
     if newton is not None:
         out["newton"] = newton
     if not args.no_cpu_baseline:

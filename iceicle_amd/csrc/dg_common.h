@@ -376,21 +376,22 @@ __device__ __forceinline__ double face_weight(const DevSpace& S, int fq) {
 // DDG single-valued gradient and weighted normal flux at one interior face
 // point (conservation_law.hpp:502-575): Q = (F_visc - F_conv) w sqrt(g).
 template <int D, class PH>
-__device__ __forceinline__ void interior_point_flux(const PhysParams& ph, const double* uL, const double* gL,
-                                                    const double* hL, const double* uR, const double* gR,
-                                                    const double* hR, const double* nrm, double h, double ws,
-                                                    double* Q) {
+__device__ __forceinline__ void interior_point_flux(const typename PH::Params& ph, const typename PH::real* uL, const typename PH::real* gL,
+                                                    const typename PH::real* hL, const typename PH::real* uR, const typename PH::real* gR,
+                                                    const typename PH::real* hR, const typename PH::real* nrm, typename PH::real h, typename PH::real ws,
+                                                    typename PH::real* Q) {
     constexpr int M = PH::M;
-    double fadv[M], fvis[M], gddg[M * D], uavg[M];
+    using R = typename PH::real;
+    R fadv[M], fvis[M], gddg[M * D], uavg[M];
     PH::conv(ph, uL, uR, nrm, fadv);
 #pragma unroll
     for (int v = 0; v < M; ++v) {
-        const double jump = uR[v] - uL[v];
+        const R jump = uR[v] - uL[v];
 #pragma unroll
         for (int d = 0; d < D; ++d)
-            gddg[v * D + d] = ph.beta0 * jump / h * nrm[d] + 0.5 * (gL[v * D + d] + gR[v * D + d]) +
+            gddg[v * D + d] = ph.beta0 * jump / h * nrm[d] + R(0.5) * (gL[v * D + d] + gR[v * D + d]) +
                               ph.beta1 * h * (hR[v * D + d] - hL[v * D + d]);
-        uavg[v] = 0.5 * (uL[v] + uR[v]);
+        uavg[v] = R(0.5) * (uL[v] + uR[v]);
     }
     PH::diff(ph, uavg, gddg, nrm, fvis);
 #pragma unroll
@@ -400,18 +401,19 @@ __device__ __forceinline__ void interior_point_flux(const PhysParams& ph, const 
 // The same flux from the sum gs = gL + gR of the two elements' shares of the
 // DDG gradient (each (grad u)/2 -/+ beta1 h (Hess u) n, see trace_kernel).
 template <int D, class PH>
-__device__ __forceinline__ void interior_point_flux_split(const PhysParams& ph, const double* uL, const double* uR,
-                                                          const double* gs, const double* nrm, double h, double ws,
-                                                          double* Q) {
+__device__ __forceinline__ void interior_point_flux_split(const typename PH::Params& ph, const typename PH::real* uL, const typename PH::real* uR,
+                                                          const typename PH::real* gs, const typename PH::real* nrm, typename PH::real h, typename PH::real ws,
+                                                          typename PH::real* Q) {
     constexpr int M = PH::M;
-    double fadv[M], fvis[M], gddg[M * D], uavg[M];
+    using R = typename PH::real;
+    R fadv[M], fvis[M], gddg[M * D], uavg[M];
     PH::conv(ph, uL, uR, nrm, fadv);
 #pragma unroll
     for (int v = 0; v < M; ++v) {
-        const double jump = uR[v] - uL[v];
+        const R jump = uR[v] - uL[v];
 #pragma unroll
         for (int d = 0; d < D; ++d) gddg[v * D + d] = ph.beta0 * jump / h * nrm[d] + gs[v * D + d];
-        uavg[v] = 0.5 * (uL[v] + uR[v]);
+        uavg[v] = R(0.5) * (uL[v] + uR[v]);
     }
     PH::diff(ph, uavg, gddg, nrm, fvis);
 #pragma unroll
@@ -423,13 +425,14 @@ __device__ __forceinline__ void interior_point_flux_split(const PhysParams& ph, 
 // the point; hb: the guarded boundary DDG distance.  Returns false when the
 // boundary adds nothing (default branch of a flux without apply_bc).
 template <int D, class PH>
-__device__ __forceinline__ bool boundary_point_flux(const PhysParams& ph, int bctype, int bcflag, const double* uL,
-                                                    const double* gL, const double* bv, const double* nrm, double hb,
-                                                    double ws, double* Q, double* uB_out) {
+__device__ __forceinline__ bool boundary_point_flux(const typename PH::Params& ph, int bctype, int bcflag, const typename PH::real* uL,
+                                                    const typename PH::real* gL, const typename PH::real* bv, const typename PH::real* nrm, typename PH::real hb,
+                                                    typename PH::real ws, typename PH::real* Q, typename PH::real* uB_out) {
     constexpr int M = PH::M;
-    double fadv[M], fvis[M], gddg[M * D], uavg[M], uB[M];
+    using R = typename PH::real;
+    R fadv[M], fvis[M], gddg[M * D], uavg[M], uB[M];
     if (bctype == ICE_BC_NEUMANN) {
-        double g[M];
+        R g[M];
 #pragma unroll
         for (int v = 0; v < M; ++v) g[v] = bv[v];
         PH::neumann(ph, g, fvis);
@@ -441,7 +444,7 @@ __device__ __forceinline__ bool boundary_point_flux(const PhysParams& ph, int bc
                            bctype == ICE_BC_SPACETIME_FUTURE);
     if (general && !PH::IMPLEMENTS_BCS) {
 #pragma unroll
-        for (int v = 0; v < M; ++v) Q[v] = 0.0;
+        for (int v = 0; v < M; ++v) Q[v] = R(0);
         return false;
     }
     if (bctype == ICE_BC_EXTRAPOLATION || bctype == ICE_BC_SPACETIME_FUTURE) {
@@ -459,10 +462,10 @@ __device__ __forceinline__ bool boundary_point_flux(const PhysParams& ph, int bc
         PH::conv(ph, uL, uB, nrm, fadv);
 #pragma unroll
         for (int v = 0; v < M; ++v) {
-            const double jump = uB[v] - uL[v];
+            const R jump = uB[v] - uL[v];
 #pragma unroll
             for (int d = 0; d < D; ++d) gddg[v * D + d] = ph.beta0_bdy * jump / hb * nrm[d] + gL[v * D + d];
-            uavg[v] = 0.5 * (uL[v] + uB[v]);
+            uavg[v] = R(0.5) * (uL[v] + uB[v]);
         }
         PH::diff(ph, uavg, gddg, nrm, fvis);
     }

@@ -427,6 +427,11 @@ int ice_space_create(const ice_mesh_desc* md, const ice_space_desc* sd, ice_spac
         dv.jcols = (const int*)s->dmem[7];
         s->dmem[8] = upload(s->fperm);
         s->dmem[9] = upload(s->finv);
+        {
+            std::vector<float> tf(s->tab.begin(), s->tab.end());
+            s->dmem[11] = upload(tf);
+            dv.tabf = (const float*)s->dmem[11];
+        }
         dv.fperm = (const signed char*)s->dmem[8];
         dv.finv = (const signed char*)s->dmem[9];
         {
@@ -596,6 +601,17 @@ int ice_residual(ice_disc* d, const double* u, double* r, void* stream) {
     hipEvent_t* ev = nullptr;
     if (d->profiling && d->prof_n < d->prof_max) ev = &d->prof_ev[(size_t)4 * d->prof_n++];
     const int rc = ice::residual_fp64(s->ndim, s->p, d->prm.flux, s->dev, d->ph, d->dd, u, r, (hipStream_t)stream, ev);
+    if (rc != 0) return fail(ICE_ERR_HIP, std::string("residual launch failed: ") + hipGetErrorString((hipError_t)rc));
+    return ICE_OK;
+}
+
+int ice_residual_mixed(ice_disc* d, const double* u, double* r, void* stream) {
+    if (!d || !u || !r) return fail(ICE_ERR_ARG, "null argument");
+    const ice_space* s = d->space;
+    if (s->ndim != 2) return fail(ICE_ERR_UNSUPPORTED, "the mixed-precision residual covers ndim = 2");
+    hipEvent_t* ev = nullptr;
+    if (d->profiling && d->prof_n < d->prof_max) ev = &d->prof_ev[(size_t)4 * d->prof_n++];
+    const int rc = ice::residual_mixed(s->ndim, s->p, d->prm.flux, s->dev, d->ph, d->dd, u, r, (hipStream_t)stream, ev);
     if (rc != 0) return fail(ICE_ERR_HIP, std::string("residual launch failed: ") + hipGetErrorString((hipError_t)rc));
     return ICE_OK;
 }

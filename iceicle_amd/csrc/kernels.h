@@ -6,7 +6,9 @@
 
 namespace ice {
 
-struct PhysParams;
+template <class R>
+struct PhysT;
+using PhysParams = PhysT<double>;
 
 // Everything a residual launch needs about the FE space, as raw device
 // pointers (all owned by the ice_space handle).
@@ -22,6 +24,7 @@ struct DevSpace {
     const int4* fbdy;        // {elemL, info, bcflag, 0}
     const int* efaces;       // [nelem][2*ndim]: (Q row << 4) | (side << 3) | orientation, -1 none
     const double* tab;       // xs[n+2] | L0[(n+2)n] | L1 | L2 | wgl[n]
+    const float* tabf;       // FP32 copy of tab
     const signed char* fpt;  // [2*ndim][norient][nfq][ndim] S-indices of trace points
     const signed char* fvo;  // [2*ndim][4] face vertex order (element-local vertex ids)
     const int* jcols;        // [nelem][2*ndim+1] Jacobian block columns (-1: boundary face)
@@ -83,6 +86,10 @@ int launch_step_end(StepState* s, hipStream_t st);
 
 int residual_fp64(int ndim, int p, int flux, const DevSpace& S, const PhysParams& ph, const DiscDev& dd,
                   const double* U, double* R, hipStream_t st, hipEvent_t* ev = nullptr);
+// FP32 interpolation, fluxes and face data with FP64 geometry and output;
+// ndim 2 only (-2 otherwise)
+int residual_mixed(int ndim, int p, int flux, const DevSpace& S, const PhysParams& ph, const DiscDev& dd,
+                   const double* U, double* R, hipStream_t st, hipEvent_t* ev = nullptr);
 
 // One-time tables of u-independent terms: boundary values at the boundary
 // face quadrature points and the source load vector sum_q s(x_q) phi_i w detJ.

@@ -342,7 +342,7 @@ struct ice_newton {
 
     // J x = b: GMRES on the Krylov operator, then FP64 refinement passes
     // xr = b - J64 x, x += GMRES(xr) until ||xr|| <= linear_rtol ||b||
-    int linear_solve(const double* b, double* x, hipStream_t st, int* refinements, double* relres) {
+    int linear_solve(const double* b, double* x, double rtol, hipStream_t st, int* refinements, double* relres) {
         const double bn = norm(b, st);
         NT_LAUNCH(ice::launch_axpby2(n, 0.0, x, 0.0, x, st));
         *refinements = 0;
@@ -352,16 +352,16 @@ struct ice_newton {
         }
         const int* cols = ice::disc_jcols(disc);
         double est = 0.0;
-        int its = gmres(b, x, opt.linear_rtol, opt.gmres_maxit, st, &est);
+        int its = gmres(b, x, rtol, opt.gmres_maxit, st, &est);
         double rel = 1.0;
         for (int pass = 0;; ++pass) {
             NT_LAUNCH((ice::bsr_apply<double, double>(nelem, nslots, bs, cols, J64, x, b, xr, st)));
             rel = norm(xr, st) / bn;
-            if (!fp32() || rel <= opt.linear_rtol || pass >= opt.refine_max || its >= opt.gmres_maxit ||
+            if (!fp32() || rel <= rtol || pass >= opt.refine_max || its >= opt.gmres_maxit ||
                 !std::isfinite(rel))
                 break;
             NT_LAUNCH(ice::launch_axpby2(n, 0.0, rwork, 0.0, rwork, st));
-            its += gmres(xr, rwork, std::min(0.1, opt.linear_rtol / rel), opt.gmres_maxit - its, st, &est);
+            its += gmres(xr, rwork, std::min(0.1, rtol / rel), opt.gmres_maxit - its, st, &est);
             NT_LAUNCH(ice::launch_axpby2(n, 1.0, rwork, 1.0, x, st));
             ++*refinements;
         }
@@ -400,6 +400,7 @@ int ice_newton_opts_default(ice_newton_opts* o) {
     o->refine_max = 3;
     o->verbose = 0;
     o->preconditioner = ICE_PC_BLOCK_SGS;
+    o->forcing = ICE_FORCING_FIXED;
     return ICE_OK;
 }
 
@@ -415,7 +416,8 @@ int ice_newton_create(ice_disc* d, const ice_newton_opts* opts, ice_newton** out
     if (o.gmres_restart < 1 || o.gmres_restart > 200 || o.kmax < 0 || o.gmres_maxit < 1 ||
         (o.jacobian_precision != ICE_PREC_FP32 && o.jacobian_precision != ICE_PREC_FP64) ||
         (o.linesearch < ICE_LS_NONE || o.linesearch > ICE_LS_CORRIGAN) ||
-        (o.preconditioner != ICE_PC_BLOCK_JACOBI && o.preconditioner != ICE_PC_BLOCK_SGS)) {
+        (o.preconditioner != ICE_PC_BLOCK_JACOBI && o.preconditioner != ICE_PC_BLOCK_SGS) ||
+        (o.forcing != ICE_FORCING_FIXED && o.forcing != ICE_FORCING_EW)) {
         ice::set_error("invalid Newton options");
         return ICE_ERR_ARG;
     }
@@ -474,7 +476,7 @@ int ice_newton_linear_solve(ice_newton* s, const double* u, const double* b, dou
         if (!s->factored) throw Fail{ICE_ERR_STATE, "no Jacobian assembled yet"};
         int ref = 0;
         double rel = 0.0;
-        const int its = s->linear_solve(b, x, st, &ref, &rel);
+        const int its = s->linear_solve(b, x, s->opt.linear_rtol, st, &ref, &rel);
         if (iterations) *iterations = its;
         if (rel_residual) *rel_residual = rel;
         NT_HIP(hipStreamSynchronize(st));
@@ -505,6 +507,11 @@ int ice_newton_solve(ice_newton* s, double* u, ice_newton_report* rep, void* str
         rep->r0 = r0;
         double rk = r0;
         bool have_jac = true;
+        // linear tolerance of each step: fixed, or Eisenstat-Walker choice 2
+        // (gamma 0.9, alpha 2, safeguarded, capped at 0.1; never tighter than
+        // needed to reach the absolute target)
+        double eta = (o.forcing == ICE_FORCING_EW) ? 0.1 : o.linear_rtol;
+        double r_prev = r0;
         int k = 0;
         bool converged = false;
         for (k = 0; k < o.kmax; ++k) {
@@ -518,7 +525,7 @@ int ice_newton_solve(ice_newton* s, double* u, ice_newton_report* rep, void* str
             ta = now();
             int ref = 0;
             double rel = 0.0;
-            const int its = s->linear_solve(s->res, s->du, st, &ref, &rel);
+            const int its = s->linear_solve(s->res, s->du, eta, st, &ref, &rel);
             tl += now() - ta;
             rep->linear_iterations += its;
             rep->refinements += ref;
@@ -555,6 +562,17 @@ int ice_newton_solve(ice_newton* s, double* u, ice_newton_report* rep, void* str
             if (ice::newton_residual_norm(s, u, s->res, st, &rk) != 0) throw Fail{ICE_ERR_HIP, "residual failed"};
             tj += now() - ta;
             have_jac = false;
+            if (o.forcing == ICE_FORCING_EW) {
+                const double ratio = rk / r_prev;
+                double e_new = 0.9 * ratio * ratio;
+                const double safe = 0.9 * eta * eta;
+                if (safe > 0.1) e_new = std::max(e_new, safe);
+                e_new = std::min(e_new, 0.1);
+                const double target = o.tau_abs + o.tau_rel * r0;
+                e_new = std::max(e_new, 0.5 * target / std::max(rk, 1e-300));
+                eta = std::max(e_new, o.linear_rtol * 1e-3);
+            }
+            r_prev = rk;
             if (k < ICE_NEWTON_MAX_HIST) {
                 rep->rhist[k] = rk;
                 rep->alpha[k] = alpha;

@@ -44,156 +44,6 @@
 
 namespace ice {
 
-// ---- interior faces ------------------------------------------------------
-template <int D, int P, class PH>
-__device__ __forceinline__ void face_interior(const DevSpace& S, const PhysParams& ph, const double* __restrict__ U,
-                                              double* __restrict__ Q, double* __restrict__ Pic, int tid) {
-    constexpr int N = P + 1;
-    constexpr int NB = Pow<N, D>::value;
-    constexpr int NFQ = Pow<N, D - 1>::value;
-    constexpr int M = PH::M;
-    const int f = tid / NFQ;
-    const int q = tid - f * NFQ;
-    if (f >= S.nfi) return;
-    const int4 rec = S.fint[f];
-    const int eL = rec.x, eR = rec.y, info = rec.z;
-    const int fnl = info_fnl(info), fnr = info_fnr(info), orient = info_orient(info), pdim = info_pdim(info);
-    double XL[1 << D][D], XR[1 << D][D];
-    load_vertices<D>(S.X, eL, XL);
-    load_vertices<D>(S.X, eR, XR);
-    double x[D], nrm[D];
-    const double sqrtg = face_geometry<D, N>(S, XL, fnl, q, x, nrm);
-    const double w = face_weight<N, D>(S, q);
-    double cL[D], cR[D];
-    centroid<D>(XL, cL);
-    centroid<D>(XR, cR);
-    if (pdim) cR[pdim - 1] += S.period[pdim - 1];
-    int ptL[D], ptR[D];
-    const signed char* fpL = S.fpt + ((fnl * S.norient + 0) * NFQ + q) * D;
-    const signed char* fpR = S.fpt + ((fnr * S.norient + orient) * NFQ + q) * D;
-#pragma unroll
-    for (int d = 0; d < D; ++d) { ptL[d] = fpL[d]; ptR[d] = fpR[d]; }
-    double uL[M], uR[M], gL[M * D], gR[M * D], hL[M * D], hR[M * D];
-    trace_eval<D, N, M, true>(S, U + (size_t)eL * NB * M, ptL, XL, nrm, uL, gL, hL);
-    trace_eval<D, N, M, true>(S, U + (size_t)eR * NB * M, ptR, XR, nrm, uR, gR, hR);
-    const double h = ddg_h_interior<D>(nrm, x, cL, cR);
-    const double ws = w * sqrtg;
-    double* Qf = Q + ((size_t)f * NFQ + q) * M;
-    {
-        double Qv[M];
-        interior_point_flux<D, PH>(ph, uL, gL, hL, uR, gR, hR, nrm, h, ws, Qv);
-#pragma unroll
-        for (int v = 0; v < M; ++v) Qf[v] = Qv[v];
-    }
-    double uavg[M];
-#pragma unroll
-    for (int v = 0; v < M; ++v) uavg[v] = 0.5 * (uL[v] + uR[v]);
-    if constexpr (PH::HAS_HOMOGENEITY) {
-        if (Pic) {
-            double G[M * D * M * D];
-            PH::homogeneity(ph, uavg, G);
-            double* Pf = Pic + ((size_t)f * NFQ + q) * M * D;
-#pragma unroll
-            for (int v = 0; v < M; ++v)
-#pragma unroll
-                for (int s = 0; s < D; ++s) {
-                    double acc = 0;
-#pragma unroll
-                    for (int k = 0; k < D; ++k)
-#pragma unroll
-                        for (int r = 0; r < M; ++r) acc += G[((v * D + k) * M + r) * D + s] * nrm[k] * (uR[r] - uL[r]);
-                    Pf[v * D + s] = 0.5 * ph.sigma_ic * acc * ws;
-                }
-        }
-    }
-}
-
-// ---- boundary faces ------------------------------------------------------
-template <int D, int P, class PH>
-__device__ __forceinline__ void face_boundary(const DevSpace& S, const PhysParams& ph, const double* __restrict__ bval,
-                                              const double* __restrict__ U, double* __restrict__ Q,
-                                              double* __restrict__ Pic, int tid) {
-    constexpr int N = P + 1;
-    constexpr int NB = Pow<N, D>::value;
-    constexpr int NFQ = Pow<N, D - 1>::value;
-    constexpr int M = PH::M;
-    const int b = tid / NFQ;
-    const int q = tid - b * NFQ;
-    if (b >= S.nfb) return;
-    const int4 rec = S.fbdy[b];
-    const int eL = rec.x, info = rec.y, bcflag = rec.z;
-    const int fnl = info_fnl(info), bctype = info_bctype(info);
-    const size_t slot = (size_t)S.nfi + b;
-    double* Qf = Q + (slot * NFQ + q) * M;
-    double* Pf = nullptr;
-    if constexpr (PH::HAS_HOMOGENEITY) {
-        if (Pic) {
-            Pf = Pic + (slot * NFQ + q) * M * D;
-#pragma unroll
-            for (int k = 0; k < M * D; ++k) Pf[k] = 0.0;
-        }
-    }
-    const bool general = !(bctype == ICE_BC_DIRICHLET || bctype == ICE_BC_NEUMANN || bctype == ICE_BC_EXTRAPOLATION ||
-                           bctype == ICE_BC_SPACETIME_FUTURE);
-    if (general && !PH::IMPLEMENTS_BCS) {  // boundaryIntegral default branch without apply_bc
-#pragma unroll
-        for (int v = 0; v < M; ++v) Qf[v] = 0.0;
-        return;
-    }
-    double XL[1 << D][D];
-    load_vertices<D>(S.X, eL, XL);
-    double x[D], nrm[D];
-    const double sqrtg = face_geometry<D, N>(S, XL, fnl, q, x, nrm);
-    const double ws = face_weight<N, D>(S, q) * sqrtg;
-    const double* bv = bval + ((size_t)b * NFQ + q) * M;
-    double uL[M] = {}, gL[M * D] = {};
-    if (bctype != ICE_BC_NEUMANN) {
-        int pt[D];
-        const signed char* fp = S.fpt + ((fnl * S.norient + 0) * NFQ + q) * D;
-#pragma unroll
-        for (int d = 0; d < D; ++d) pt[d] = fp[d];
-        trace_eval<D, N, M, false>(S, U + (size_t)eL * NB * M, pt, XL, nrm, uL, gL, nullptr);
-    }
-    double cL[D];
-    centroid<D>(XL, cL);
-    const double hb = ddg_h_boundary<D>(nrm, x, cL);
-    double Qv[M], uB[M];
-    boundary_point_flux<D, PH>(ph, bctype, bcflag, uL, gL, bv, nrm, hb, ws, Qv, uB);
-#pragma unroll
-    for (int v = 0; v < M; ++v) Qf[v] = Qv[v];
-    if constexpr (PH::HAS_HOMOGENEITY) {
-        if (Pf && bctype == ICE_BC_DIRICHLET) {
-            double uavg[M], G[M * D * M * D];
-#pragma unroll
-            for (int v = 0; v < M; ++v) uavg[v] = 0.5 * (uL[v] + uB[v]);
-            PH::homogeneity(ph, uavg, G);
-#pragma unroll
-            for (int v = 0; v < M; ++v)
-#pragma unroll
-                for (int s = 0; s < D; ++s) {
-                    double acc = 0;
-#pragma unroll
-                    for (int k = 0; k < D; ++k)
-#pragma unroll
-                        for (int r = 0; r < M; ++r) acc += G[((v * D + k) * M + r) * D + s] * nrm[k] * (uB[r] - uL[r]);
-                    Pf[v * D + s] = ph.sigma_ic * acc * ws;
-                }
-        }
-    }
-}
-
-// Interior and boundary faces in one launch: blocks [0, nbi) take interior
-// face points, the rest boundary face points.
-template <int D, int P, class PH>
-__global__ __launch_bounds__(256) void face_kernel(DevSpace S, PhysParams ph, DiscDev dd, const double* __restrict__ U,
-                                                   int nbi) {
-    if ((int)blockIdx.x < nbi) {
-        face_interior<D, P, PH>(S, ph, U, dd.Q, dd.Pic, blockIdx.x * blockDim.x + threadIdx.x);
-    } else {
-        face_boundary<D, P, PH>(S, ph, dd.bval, U, dd.Q, dd.Pic, (blockIdx.x - nbi) * blockDim.x + threadIdx.x);
-    }
-}
-
 // ---- split pipeline: traces per element face, then fluxes per face point ----
 //
 // trace_kernel: for every element and each of its 2*D faces, the trace at the
@@ -208,10 +58,19 @@ __global__ __launch_bounds__(256) void face_kernel(DevSpace S, PhysParams ph, Di
 // flux_kernel then evaluates the numerical fluxes at each face point from the
 // two stored traces and the precomputed face geometry (normal, w sqrt(g), h).
 
+// 1D tables in the kernels' arithmetic type
+template <class VT>
+__device__ __forceinline__ const VT* tab_of(const DevSpace& S);
+template <>
+__device__ __forceinline__ const double* tab_of<double>(const DevSpace& S) { return S.tab; }
+template <>
+__device__ __forceinline__ const float* tab_of<float>(const DevSpace& S) { return S.tabf; }
+
 template <class PH, int D>
 struct TraceRec {
     // Euler needs only values: its diffusive flux is identically zero
-    static constexpr bool NEED_GRAD = !(std::is_same<PH, NavierStokes<D, false>>::value);
+    static constexpr bool NEED_GRAD =
+        !(std::is_same<PH, NavierStokes<D, false, double>>::value || std::is_same<PH, NavierStokes<D, false, float>>::value);
     static constexpr int NV = NEED_GRAD ? 1 + D : 1;
 };
 
@@ -224,17 +83,17 @@ struct TraceRec {
 // keep the register footprint small), and stores the trace record of
 // trace_kernel: out[v] = u, out[(1 + d) M + v] = gx/2 + bh hn on interior
 // faces, gx on boundary faces.
-template <int D, int N, int M>
-__device__ __forceinline__ void face_trace_sf(const DevSpace& S, const double* Ue, int j, const int* pt,
+template <int D, int N, int M, class VT>
+__device__ __forceinline__ void face_trace_sf(const DevSpace& S, const VT* Ue, int j, const int* pt,
                                               const double (&X)[1 << D][D], const double* nrm, bool interior,
-                                              double bh, double* __restrict__ out) {
+                                              double bh, VT* __restrict__ out) {
     constexpr int NB = Pow<N, D>::value;
     constexpr int NT = NB / N;  // tangential tensor points
     constexpr int NH = D * (D + 1) / 2;
     const double* xs = S.tab;
-    const double* L0 = S.tab + (N + 2);
-    const double* L1 = L0 + (N + 2) * N;
-    const double* L2 = L1 + (N + 2) * N;
+    const VT* L0 = tab_of<VT>(S) + (N + 2);
+    const VT* L1 = L0 + (N + 2) * N;
+    const VT* L2 = L1 + (N + 2) * N;
     const int k = j % D;
     const int ep = pt[k];
     double xi[D];
@@ -245,7 +104,7 @@ __device__ __forceinline__ void face_trace_sf(const DevSpace& S, const double* U
     double det = det_adj<D>(Jm, A);
     det = (det == 0.0) ? 1.0 : det;
     const double idet = 1.0 / det;
-    double ln[3][N];  // normal-direction 1D rows at the endpoint
+    VT ln[3][N];  // normal-direction 1D rows at the endpoint
 #pragma unroll
     for (int a = 0; a < N; ++a) {
         ln[0][a] = L0[ep * N + a];
@@ -269,7 +128,7 @@ __device__ __forceinline__ void face_trace_sf(const DevSpace& S, const double* U
     }
 #pragma unroll 1
     for (int v = 0; v < M; ++v) {
-        double C[3][NT];
+        VT C[3][NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             // tangential multi-index of t (first tangential dim slowest)
@@ -282,10 +141,10 @@ __device__ __forceinline__ void face_trace_sf(const DevSpace& S, const double* U
                     r /= N;
                 }
             }
-            double c0 = 0, c1 = 0, c2 = 0;
+            VT c0 = 0, c1 = 0, c2 = 0;
 #pragma unroll
             for (int a = 0; a < N; ++a) {
-                const double uv = Ue[(base + a * stride[k]) * M + v];
+                const VT uv = Ue[(base + a * stride[k]) * M + v];
                 c0 += uv * ln[0][a];
                 c1 += uv * ln[1][a];
                 c2 += uv * ln[2][a];
@@ -302,10 +161,10 @@ __device__ __forceinline__ void face_trace_sf(const DevSpace& S, const double* U
             hr[0] = C[2][0];
         } else if constexpr (D == 2) {
             const int p1 = pt[td[0]];
-            double s0 = 0, sk = 0, s1 = 0, skk = 0, sk1 = 0, s11 = 0;
+            VT s0 = 0, sk = 0, s1 = 0, skk = 0, sk1 = 0, s11 = 0;
 #pragma unroll
             for (int b = 0; b < N; ++b) {
-                const double l0 = L0[p1 * N + b], l1 = L1[p1 * N + b], l2 = L2[p1 * N + b];
+                const VT l0 = L0[p1 * N + b], l1 = L1[p1 * N + b], l2 = L2[p1 * N + b];
                 s0 += C[0][b] * l0;
                 sk += C[1][b] * l0;
                 s1 += C[0][b] * l1;
@@ -323,17 +182,17 @@ __device__ __forceinline__ void face_trace_sf(const DevSpace& S, const double* U
             hr[2] = k0 ? s11 : skk;
         } else {
             const int p1 = pt[td[0]], p2 = pt[td[1]];
-            double s[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // 0, k, 1, 2, kk, k1, k2, 11, 12, 22
+            VT s[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // 0, k, 1, 2, kk, k1, k2, 11, 12, 22
 #pragma unroll
             for (int b1 = 0; b1 < N; ++b1) {
-                const double a0 = L0[p1 * N + b1], a1 = L1[p1 * N + b1], a2 = L2[p1 * N + b1];
-                double r0[3] = {0, 0, 0}, r1[3] = {0, 0, 0}, r2[3] = {0, 0, 0};  // [C index][deriv along dim2]
+                const VT a0 = L0[p1 * N + b1], a1 = L1[p1 * N + b1], a2 = L2[p1 * N + b1];
+                VT r0[3] = {0, 0, 0}, r1[3] = {0, 0, 0}, r2[3] = {0, 0, 0};  // [C index][deriv along dim2]
 #pragma unroll
                 for (int b2 = 0; b2 < N; ++b2) {
-                    const double c0 = L0[p2 * N + b2], c1 = L1[p2 * N + b2], c2 = L2[p2 * N + b2];
+                    const VT c0 = L0[p2 * N + b2], c1 = L1[p2 * N + b2], c2 = L2[p2 * N + b2];
 #pragma unroll
                     for (int r = 0; r < 3; ++r) {
-                        const double cv = C[r][b1 * N + b2];
+                        const VT cv = C[r][b1 * N + b2];
                         r0[r] += cv * c0;
                         r1[r] += cv * c1;
                         r2[r] += cv * c2;
@@ -412,15 +271,17 @@ __device__ __forceinline__ void face_trace_sf(const DevSpace& S, const double* U
                 sacc += hx * nrm[jj];
             }
             const double hn = sacc * idet2;
-            out[(1 + d) * M + v] = interior ? 0.5 * gx[d] + bh * hn : gx[d];
+            out[(1 + d) * M + v] = (VT)(interior ? 0.5 * gx[d] + bh * hn : gx[d]);
         }
-        out[v] = uv;
+        out[v] = (VT)uv;
     }
 }
 
 template <int D, int P, class PH>
-__global__ __launch_bounds__(256, ICE_TRACE_MINW) void trace_kernel(DevSpace S, PhysParams ph, const double* __restrict__ U,
-                                                    double* __restrict__ T) {
+__global__ __launch_bounds__(256, ICE_TRACE_MINW) void trace_kernel(DevSpace S, typename PH::Params ph,
+                                                                     const double* __restrict__ U,
+                                                                     typename PH::real* __restrict__ T) {
+    using VT = typename PH::real;
     constexpr int N = P + 1;
     constexpr int NB = Pow<N, D>::value;
     constexpr int NFQ = Pow<N, D - 1>::value;
@@ -429,13 +290,13 @@ __global__ __launch_bounds__(256, ICE_TRACE_MINW) void trace_kernel(DevSpace S, 
     constexpr int EPB = (256 / TPE) > 0 ? (256 / TPE) : 1;  // elements per block
     constexpr bool GRAD = TraceRec<PH, D>::NEED_GRAD;
     constexpr int NV = TraceRec<PH, D>::NV;
-    __shared__ double sU[EPB * NB * M];
+    __shared__ VT sU[EPB * NB * M];
     const int e0 = blockIdx.x * EPB;
     {
         const size_t base = (size_t)e0 * NB * M;
         const size_t total = (size_t)S.nelem * NB * M;
         for (int idx = threadIdx.x; idx < EPB * NB * M; idx += blockDim.x)
-            sU[idx] = (base + idx < total) ? U[base + idx] : 0.0;
+            sU[idx] = (VT)((base + idx < total) ? U[base + idx] : 0.0);
     }
     __syncthreads();
     const int le = threadIdx.x / TPE;
@@ -449,18 +310,16 @@ __global__ __launch_bounds__(256, ICE_TRACE_MINW) void trace_kernel(DevSpace S, 
 #pragma unroll
         for (int d = 0; d < D; ++d) pt[d] = fp[d];
     }
-    const double* L0 = S.tab + (N + 2);
-    const double* L1 = L0 + (N + 2) * N;
-    const double* L2 = L1 + (N + 2) * N;
-    double* out = T + (((size_t)e * 2 * D + j) * NFQ + qp) * NV * M;
-    const double* Ue = sU + le * NB * M;
+    const VT* L0 = tab_of<VT>(S) + (N + 2);
+    VT* out = T + (((size_t)e * 2 * D + j) * NFQ + qp) * NV * M;
+    const VT* Ue = sU + le * NB * M;
     if constexpr (!GRAD) {
-        double a0[D][N];
+        VT a0[D][N];
 #pragma unroll
         for (int d = 0; d < D; ++d)
 #pragma unroll
             for (int i = 0; i < N; ++i) a0[d][i] = L0[pt[d] * N + i];
-        double u[M];
+        VT u[M];
 #pragma unroll
         for (int v = 0; v < M; ++v) u[v] = 0;
 #pragma unroll
@@ -469,7 +328,7 @@ __global__ __launch_bounds__(256, ICE_TRACE_MINW) void trace_kernel(DevSpace S, 
             int r = i;
 #pragma unroll
             for (int d = D - 1; d >= 0; --d) { id[d] = r % N; r /= N; }
-            double c0 = 1.0;
+            VT c0 = 1;
 #pragma unroll
             for (int d = 0; d < D; ++d) c0 *= a0[d][id[d]];
 #pragma unroll
@@ -498,16 +357,19 @@ __global__ __launch_bounds__(256, ICE_TRACE_MINW) void trace_kernel(DevSpace S, 
 #pragma unroll
             for (int d = 0; d < D; ++d) nrm[d] = 0.0;
         }
-        const double bh = (side ? 1.0 : -1.0) * ph.beta1 * h;
-        face_trace_sf<D, N, M>(S, Ue, j, pt, X, nrm, interior, bh, out);
+        const double bh = (side ? 1.0 : -1.0) * (double)ph.beta1 * h;
+        face_trace_sf<D, N, M, VT>(S, Ue, j, pt, X, nrm, interior, bh, out);
     }
 }
 
 // Numerical fluxes at face points from stored traces: interior points first
 // (blocks [0, nbi)), then boundary points.
 template <int D, int P, class PH>
-__global__ __launch_bounds__(256, ICE_FLUX_MINW) void flux_kernel(DevSpace S, PhysParams ph, DiscDev dd, const double* __restrict__ T,
-                                                   int nbi) {
+__global__ __launch_bounds__(256, ICE_FLUX_MINW) void flux_kernel(DevSpace S, typename PH::Params ph, DiscDev dd,
+                                                                   const typename PH::real* __restrict__ T, int nbi) {
+    using VT = typename PH::real;
+    VT* const Qb = reinterpret_cast<VT*>(dd.Q);
+    VT* const Pb = reinterpret_cast<VT*>(dd.Pic);
     constexpr int N = P + 1;
     constexpr int NFQ = Pow<N, D - 1>::value;
     constexpr int M = PH::M;
@@ -520,14 +382,14 @@ __global__ __launch_bounds__(256, ICE_FLUX_MINW) void flux_kernel(DevSpace S, Ph
         const int eL = rec.x, eR = rec.y, info = rec.z;
         const int fnl = info_fnl(info), fnr = info_fnr(info), orient = info_orient(info);
         const int qr = S.fperm[(fnr * S.norient + orient) * NFQ + q];
-        const double* tL = T + (((size_t)eL * 2 * D + fnl) * NFQ + q) * NV * M;
-        const double* tR = T + (((size_t)eR * 2 * D + fnr) * NFQ + qr) * NV * M;
+        const VT* tL = T + (((size_t)eL * 2 * D + fnl) * NFQ + q) * NV * M;
+        const VT* tR = T + (((size_t)eR * 2 * D + fnr) * NFQ + qr) * NV * M;
         const double* g = S.fgeo + ((size_t)f * NFQ + q) * (D + 2);
-        double nrm[D];
+        VT nrm[D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) nrm[d] = g[d];
-        const double ws = g[D], h = g[D + 1];
-        double uL[M], uR[M], gs[M * D];
+        for (int d = 0; d < D; ++d) nrm[d] = (VT)g[d];
+        const VT ws = (VT)g[D], h = (VT)g[D + 1];
+        VT uL[M], uR[M], gs[M * D];
 #pragma unroll
         for (int v = 0; v < M; ++v) {
             uL[v] = tL[v];
@@ -540,31 +402,31 @@ __global__ __launch_bounds__(256, ICE_FLUX_MINW) void flux_kernel(DevSpace S, Ph
                 for (int d = 0; d < D; ++d) gs[v * D + d] = tL[(1 + d) * M + v] + tR[(1 + d) * M + v];
         } else {
 #pragma unroll
-            for (int k = 0; k < M * D; ++k) gs[k] = 0.0;
+            for (int k = 0; k < M * D; ++k) gs[k] = VT(0);
         }
-        double Qv[M];
+        VT Qv[M];
         interior_point_flux_split<D, PH>(ph, uL, uR, gs, nrm, h, ws, Qv);
-        double* Qf = dd.Q + ((size_t)f * NFQ + q) * M;
+        VT* Qf = Qb + ((size_t)f * NFQ + q) * M;
 #pragma unroll
         for (int v = 0; v < M; ++v) Qf[v] = Qv[v];
         if constexpr (PH::HAS_HOMOGENEITY) {
             if (dd.Pic) {
-                double uavg[M], G[M * D * M * D];
+                VT uavg[M], G[M * D * M * D];
 #pragma unroll
-                for (int v = 0; v < M; ++v) uavg[v] = 0.5 * (uL[v] + uR[v]);
+                for (int v = 0; v < M; ++v) uavg[v] = VT(0.5) * (uL[v] + uR[v]);
                 PH::homogeneity(ph, uavg, G);
-                double* Pf = dd.Pic + ((size_t)f * NFQ + q) * M * D;
+                VT* Pf = Pb + ((size_t)f * NFQ + q) * M * D;
 #pragma unroll
                 for (int v = 0; v < M; ++v)
 #pragma unroll
                     for (int s = 0; s < D; ++s) {
-                        double acc = 0;
+                        VT acc = 0;
 #pragma unroll
                         for (int k = 0; k < D; ++k)
 #pragma unroll
                             for (int r = 0; r < M; ++r)
                                 acc += G[((v * D + k) * M + r) * D + s] * nrm[k] * (uR[r] - uL[r]);
-                        Pf[v * D + s] = 0.5 * ph.sigma_ic * acc * ws;
+                        Pf[v * D + s] = VT(0.5) * ph.sigma_ic * acc * ws;
                     }
             }
         }
@@ -576,30 +438,30 @@ __global__ __launch_bounds__(256, ICE_FLUX_MINW) void flux_kernel(DevSpace S, Ph
         const int eL = rec.x, info = rec.y, bcflag = rec.z;
         const int fnl = info_fnl(info), bctype = info_bctype(info);
         const size_t slot = (size_t)S.nfi + b;
-        double* Qf = dd.Q + (slot * NFQ + q) * M;
-        double* Pf = nullptr;
+        VT* Qf = Qb + (slot * NFQ + q) * M;
+        VT* Pf = nullptr;
         if constexpr (PH::HAS_HOMOGENEITY) {
             if (dd.Pic) {
-                Pf = dd.Pic + (slot * NFQ + q) * M * D;
+                Pf = Pb + (slot * NFQ + q) * M * D;
 #pragma unroll
-                for (int k = 0; k < M * D; ++k) Pf[k] = 0.0;
+                for (int k = 0; k < M * D; ++k) Pf[k] = VT(0);
             }
         }
         const bool general = !(bctype == ICE_BC_DIRICHLET || bctype == ICE_BC_NEUMANN ||
                                bctype == ICE_BC_EXTRAPOLATION || bctype == ICE_BC_SPACETIME_FUTURE);
         if (general && !PH::IMPLEMENTS_BCS) {
 #pragma unroll
-            for (int v = 0; v < M; ++v) Qf[v] = 0.0;
+            for (int v = 0; v < M; ++v) Qf[v] = VT(0);
             return;
         }
         const double* g = S.fgeo + (slot * NFQ + q) * (D + 2);
-        double nrm[D];
+        VT nrm[D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) nrm[d] = g[d];
-        const double ws = g[D], hb = g[D + 1];
-        double uL[M] = {}, gL[M * D] = {};
+        for (int d = 0; d < D; ++d) nrm[d] = (VT)g[d];
+        const VT ws = (VT)g[D], hb = (VT)g[D + 1];
+        VT uL[M] = {}, gL[M * D] = {};
         if (bctype != ICE_BC_NEUMANN) {
-            const double* tL = T + (((size_t)eL * 2 * D + fnl) * NFQ + q) * NV * M;
+            const VT* tL = T + (((size_t)eL * 2 * D + fnl) * NFQ + q) * NV * M;
 #pragma unroll
             for (int v = 0; v < M; ++v) uL[v] = tL[v];
             if constexpr (NV > 1) {
@@ -609,22 +471,24 @@ __global__ __launch_bounds__(256, ICE_FLUX_MINW) void flux_kernel(DevSpace S, Ph
                     for (int d = 0; d < D; ++d) gL[v * D + d] = tL[(1 + d) * M + v];
             }
         }
-        const double* bv = dd.bval + ((size_t)b * NFQ + q) * M;
-        double Qv[M], uB[M];
+        VT bv[M];
+#pragma unroll
+        for (int v = 0; v < M; ++v) bv[v] = (VT)dd.bval[((size_t)b * NFQ + q) * M + v];
+        VT Qv[M], uB[M];
         boundary_point_flux<D, PH>(ph, bctype, bcflag, uL, gL, bv, nrm, hb, ws, Qv, uB);
 #pragma unroll
         for (int v = 0; v < M; ++v) Qf[v] = Qv[v];
         if constexpr (PH::HAS_HOMOGENEITY) {
             if (Pf && bctype == ICE_BC_DIRICHLET) {
-                double uavg[M], G[M * D * M * D];
+                VT uavg[M], G[M * D * M * D];
 #pragma unroll
-                for (int v = 0; v < M; ++v) uavg[v] = 0.5 * (uL[v] + uB[v]);
+                for (int v = 0; v < M; ++v) uavg[v] = VT(0.5) * (uL[v] + uB[v]);
                 PH::homogeneity(ph, uavg, G);
 #pragma unroll
                 for (int v = 0; v < M; ++v)
 #pragma unroll
                     for (int s = 0; s < D; ++s) {
-                        double acc = 0;
+                        VT acc = 0;
 #pragma unroll
                         for (int k = 0; k < D; ++k)
 #pragma unroll
@@ -969,9 +833,12 @@ __global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph,
 // point and basis function.  The lifting skips faces on which the basis
 // function's node does not lie (its trace there is zero).
 template <int P, class PH>
-__global__ __launch_bounds__(256, ICE_ELEM_MINW) void element_kernel_2d(DevSpace S, PhysParams ph, const double* __restrict__ srcload,
-                                                         const double* __restrict__ U, const double* __restrict__ Q,
-                                                         const double* __restrict__ Pic, double* __restrict__ R,
+__global__ __launch_bounds__(256, ICE_ELEM_MINW) void element_kernel_2d(DevSpace S, typename PH::Params ph,
+                                                                        const double* __restrict__ srcload,
+                                                                        const double* __restrict__ U,
+                                                                        const typename PH::real* __restrict__ Q,
+                                                                        const typename PH::real* __restrict__ Pic,
+                                                                        double* __restrict__ R,
                                                          unsigned long long* __restrict__ wave,
                                                          double* __restrict__ wave_part) {
     constexpr int D = 2;
@@ -979,12 +846,13 @@ __global__ __launch_bounds__(256, ICE_ELEM_MINW) void element_kernel_2d(DevSpace
     constexpr int N = TL::N, NB = TL::NB, EPB = TL::EPB;
     constexpr int NFQ = N;
     constexpr int M = PH::M;
+    using VT = typename PH::real;
     // the staged coefficients are dead after phase 1a, so the reference-frame
     // fluxes of phase 1b reuse their storage
-    __shared__ double sUG[EPB * NB * M * D];
-    __shared__ double sT[EPB * 2 * NB * M];
-    double* sU = sUG;
-    double* sG = sUG;
+    __shared__ VT sUG[EPB * NB * M * D];
+    __shared__ VT sT[EPB * 2 * NB * M];
+    VT* sU = sUG;
+    VT* sG = sUG;
     const int e0 = blockIdx.x * EPB;
     const int le = threadIdx.x / NB;
     const int k = threadIdx.x - le * NB;
@@ -995,19 +863,19 @@ __global__ __launch_bounds__(256, ICE_ELEM_MINW) void element_kernel_2d(DevSpace
         const size_t base = (size_t)e0 * NB * M;
         const size_t total = (size_t)S.nelem * NB * M;
         for (int idx = threadIdx.x; idx < EPB * NB * M; idx += blockDim.x)
-            sU[idx] = (base + idx < total) ? U[base + idx] : 0.0;
+            sU[idx] = (VT)((base + idx < total) ? U[base + idx] : 0.0);
     }
     __syncthreads();
     const double* xs = S.tab;
-    const double* L0 = S.tab + (N + 2);
-    const double* L1 = L0 + (N + 2) * N;
-    const double* wgl = L1 + 2 * (N + 2) * N;
-    double* T0 = sT + (size_t)le * 2 * NB * M;  // [a][q1][v]
-    double* T1 = T0 + NB * M;
+    const double* wgl = S.tab + (N + 2) + 3 * (N + 2) * N;
+    const VT* L0 = tab_of<VT>(S) + (N + 2);
+    const VT* L1 = L0 + (N + 2) * N;
+    VT* T0 = sT + (size_t)le * 2 * NB * M;  // [a][q1][v]
+    VT* T1 = T0 + NB * M;
     // phase 1a: (a = k0, q1 = k1)
     if (active) {
-        const double* Ua = sU + ((size_t)le * NB + k0 * N) * M;
-        double l0[N], l1[N];
+        const VT* Ua = sU + ((size_t)le * NB + k0 * N) * M;
+        VT l0[N], l1[N];
 #pragma unroll
         for (int b = 0; b < N; ++b) {
             l0[b] = L0[k1 * N + b];
@@ -1015,10 +883,10 @@ __global__ __launch_bounds__(256, ICE_ELEM_MINW) void element_kernel_2d(DevSpace
         }
 #pragma unroll
         for (int v = 0; v < M; ++v) {
-            double t0 = 0, t1 = 0;
+            VT t0 = 0, t1 = 0;
 #pragma unroll
             for (int b = 0; b < N; ++b) {
-                const double c = Ua[b * M + v];
+                const VT c = Ua[b * M + v];
                 t0 += c * l0[b];
                 t1 += c * l1[b];
             }
@@ -1039,28 +907,30 @@ __global__ __launch_bounds__(256, ICE_ELEM_MINW) void element_kernel_2d(DevSpace
         const double det = det_adj<D>(J, A);
         const double detc = fmax(0.0, det);
         const double detg = (det == 0.0) ? 1.0 : det;
-        double l0[N], l1[N];
+        VT l0[N], l1[N];
 #pragma unroll
         for (int a = 0; a < N; ++a) {
             l0[a] = L0[k0 * N + a];
             l1[a] = L1[k0 * N + a];
         }
-        double u[M], gx[M * D], F[M * D];
+        VT u[M], gx[M * D], F[M * D];
         const double idetg = 1.0 / detg;
+        const VT a00 = (VT)(A[0][0] * idetg), a01 = (VT)(A[0][1] * idetg), a10 = (VT)(A[1][0] * idetg),
+                 a11 = (VT)(A[1][1] * idetg);
 #pragma unroll
         for (int v = 0; v < M; ++v) {
-            double s0 = 0, s1 = 0, sg0 = 0;
+            VT s0 = 0, s1 = 0, sg0 = 0;
 #pragma unroll
             for (int a = 0; a < N; ++a) {
-                const double t0 = T0[(a * N + k1) * M + v];
+                const VT t0 = T0[(a * N + k1) * M + v];
                 s0 += t0 * l0[a];
                 sg0 += t0 * l1[a];
                 s1 += T1[(a * N + k1) * M + v] * l0[a];
             }
             u[v] = s0;
             // grad_x = grad_xi A / det (finite_element.hpp:220-258)
-            gx[v * D + 0] = (sg0 * A[0][0] + s1 * A[1][0]) * idetg;
-            gx[v * D + 1] = (sg0 * A[0][1] + s1 * A[1][1]) * idetg;
+            gx[v * D + 0] = sg0 * a00 + s1 * a10;
+            gx[v * D + 1] = sg0 * a01 + s1 * a11;
         }
         PH::flux(ph, u, gx, F);
         if (wave) {
@@ -1069,15 +939,16 @@ __global__ __launch_bounds__(256, ICE_ELEM_MINW) void element_kernel_2d(DevSpace
             if (e == S.nelem - 1 && k == NB - 1) wave[1] = (unsigned long long)__double_as_longlong(lam_w);
         }
         const double fac = detc * wq * idetg;
-        double* Gq = sG + ((size_t)le * NB + k) * M * D;
+        const VT ga[D][D] = {{(VT)(A[0][0] * fac), (VT)(A[0][1] * fac)}, {(VT)(A[1][0] * fac), (VT)(A[1][1] * fac)}};
+        VT* Gq = sG + ((size_t)le * NB + k) * M * D;
 #pragma unroll
         for (int v = 0; v < M; ++v)
 #pragma unroll
             for (int a = 0; a < D; ++a) {
-                double sacc = 0;
+                VT sacc = 0;
 #pragma unroll
-                for (int j = 0; j < D; ++j) sacc += F[v * D + j] * A[a][j];
-                Gq[v * D + a] = sacc * fac;
+                for (int j = 0; j < D; ++j) sacc += F[v * D + j] * ga[a][j];
+                Gq[v * D + a] = sacc;
             }
     }
     __shared__ double red[2 * 4];
@@ -1100,16 +971,16 @@ __global__ __launch_bounds__(256, ICE_ELEM_MINW) void element_kernel_2d(DevSpace
     }
     // phase 2a: (q0 = k0, i1 = k1); T0/T1 reused as S0/S1 [q0][i1][v]
     if (active) {
-        double l0[N], l1[N];
+        VT l0[N], l1[N];
 #pragma unroll
         for (int q1 = 0; q1 < N; ++q1) {
             l0[q1] = L0[q1 * N + k1];
             l1[q1] = L1[q1 * N + k1];
         }
-        const double* Gr = sG + ((size_t)le * NB + k0 * N) * M * D;
+        const VT* Gr = sG + ((size_t)le * NB + k0 * N) * M * D;
 #pragma unroll
         for (int v = 0; v < M; ++v) {
-            double s0 = 0, s1 = 0;
+            VT s0 = 0, s1 = 0;
 #pragma unroll
             for (int q1 = 0; q1 < N; ++q1) {
                 s0 += Gr[(q1 * M + v) * D + 0] * l0[q1];
@@ -1124,7 +995,7 @@ __global__ __launch_bounds__(256, ICE_ELEM_MINW) void element_kernel_2d(DevSpace
     // phase 2b: (i0 = k0, i1 = k1)
     double r[M];
     {
-        double l0[N], l1[N];
+        VT l0[N], l1[N];
 #pragma unroll
         for (int q0 = 0; q0 < N; ++q0) {
             l0[q0] = L0[q0 * N + k0];
@@ -1132,7 +1003,7 @@ __global__ __launch_bounds__(256, ICE_ELEM_MINW) void element_kernel_2d(DevSpace
         }
 #pragma unroll
         for (int v = 0; v < M; ++v) {
-            double sacc = 0;
+            VT sacc = 0;
 #pragma unroll
             for (int q0 = 0; q0 < N; ++q0)
                 sacc += T0[(q0 * N + k1) * M + v] * l1[q0] + T1[(q0 * N + k1) * M + v] * l0[q0];
@@ -1157,19 +1028,22 @@ __global__ __launch_bounds__(256, ICE_ELEM_MINW) void element_kernel_2d(DevSpace
         const int kn = j % D;
         const bool on_face = kd[kn] == ((j < D) ? 0 : N - 1);
         if (on_face) {
-            const double* Qs = Q + slot * NFQ * M;
+            const VT* Qs = Q + slot * NFQ * M;
+            const double* Ld = S.tab + (N + 2);
 #pragma unroll
             for (int fq = 0; fq < NFQ; ++fq) {
                 double phi = 1.0;
 #pragma unroll
-                for (int d = 0; d < D; ++d) phi *= L0[fp[fq * D + d] * N + kd[d]];
+                for (int d = 0; d < D; ++d) phi *= Ld[fp[fq * D + d] * N + kd[d]];
 #pragma unroll
-                for (int v = 0; v < M; ++v) r[v] += sgn * Qs[fq * M + v] * phi;
+                for (int v = 0; v < M; ++v) r[v] += sgn * (double)Qs[fq * M + v] * phi;
             }
         }
         if constexpr (PH::HAS_HOMOGENEITY) {
             if (Pic) {
-                const double* Ps = Pic + slot * NFQ * M * D;
+                const VT* Ps = Pic + slot * NFQ * M * D;
+                const double* Ld0 = S.tab + (N + 2);
+                const double* Ld1 = Ld0 + (N + 2) * N;
 #pragma unroll 1
                 for (int fq = 0; fq < NFQ; ++fq) {
                     double xi[D], cg[D];
@@ -1181,7 +1055,7 @@ __global__ __launch_bounds__(256, ICE_ELEM_MINW) void element_kernel_2d(DevSpace
 #pragma unroll
                         for (int d = 0; d < D; ++d) {
                             const int sidx = fp[fq * D + d];
-                            t *= (d == a) ? L1[sidx * N + kd[d]] : L0[sidx * N + kd[d]];
+                            t *= (d == a) ? Ld1[sidx * N + kd[d]] : Ld0[sidx * N + kd[d]];
                         }
                         cg[a] = t;
                     }
@@ -1201,7 +1075,7 @@ __global__ __launch_bounds__(256, ICE_ELEM_MINW) void element_kernel_2d(DevSpace
                     for (int v = 0; v < M; ++v) {
                         double sacc = 0;
 #pragma unroll
-                        for (int d = 0; d < D; ++d) sacc += Ps[(fq * M + v) * D + d] * gphi[d];
+                        for (int d = 0; d < D; ++d) sacc += (double)Ps[(fq * M + v) * D + d] * gphi[d];
                         r[v] -= sacc;
                     }
                 }
@@ -1407,38 +1281,47 @@ int setup_disc_tables(int ndim, int p, int neq, const DevSpace& S, const ice_fie
 }
 
 // ---- launcher ------------------------------------------------------------
+// PH fixes the arithmetic of the trace, flux and 2D element kernels (double,
+// or float for the mixed-precision residual); geometry, the source load and
+// the residual itself stay FP64.
 template <int D, int P, class PH>
-int launch_residual(const DevSpace& S, const PhysParams& ph, const DiscDev& dd, const double* U, double* R,
+int launch_residual(const DevSpace& S, const typename PH::Params& ph, const DiscDev& dd, const double* U, double* R,
                     hipStream_t st, hipEvent_t* ev) {
+    using VT = typename PH::real;
     constexpr int N = P + 1;
     constexpr int NFQ = Pow<N, D - 1>::value;
     DiscDev d = dd;
     if (!PH::HAS_HOMOGENEITY) d.Pic = nullptr;
+    VT* T = reinterpret_cast<VT*>(d.T);
     if (ev) (void)hipEventRecord(ev[0], st);
     const int nbi = (int)(((long)S.nfi * NFQ + 255) / 256);
     const int nbb = (int)(((long)S.nfb * NFQ + 255) / 256);
     {
         constexpr int TPE = 2 * D * NFQ;
         constexpr int EPB = (256 / TPE) > 0 ? (256 / TPE) : 1;
-        trace_kernel<D, P, PH><<<(S.nelem + EPB - 1) / EPB, 256, 0, st>>>(S, ph, U, d.T);
+        trace_kernel<D, P, PH><<<(S.nelem + EPB - 1) / EPB, 256, 0, st>>>(S, ph, U, T);
     }
     if (ev) (void)hipEventRecord(ev[1], st);
-    if (nbi + nbb > 0) flux_kernel<D, P, PH><<<nbi + nbb, 256, 0, st>>>(S, ph, d, d.T, nbi);
+    if (nbi + nbb > 0) flux_kernel<D, P, PH><<<nbi + nbb, 256, 0, st>>>(S, ph, d, T, nbi);
     if (ev) (void)hipEventRecord(ev[2], st);
-    using T = ElemTile<D, P>;
-    const int nblk = (S.nelem + T::EPB - 1) / T::EPB;
-    if constexpr (D == 2)
-        element_kernel_2d<P, PH><<<nblk, 256, 0, st>>>(S, ph, d.srcload, U, d.Q, d.Pic, R, d.wave, d.wave_part);
-    else
-        element_kernel<D, P, PH><<<nblk, 256, 0, st>>>(S, ph, d.srcload, U, d.Q, d.Pic, R, d.wave, d.wave_part);
+    using TL = ElemTile<D, P>;
+    const int nblk = (S.nelem + TL::EPB - 1) / TL::EPB;
+    if constexpr (D == 2) {
+        element_kernel_2d<P, PH><<<nblk, 256, 0, st>>>(S, ph, d.srcload, U, reinterpret_cast<const VT*>(d.Q),
+                                                         reinterpret_cast<const VT*>(d.Pic), R, d.wave, d.wave_part);
+    } else {
+        static_assert(std::is_same<VT, double>::value || D == 2, "the mixed-precision residual is 2D only");
+        if constexpr (std::is_same<VT, double>::value)
+            element_kernel<D, P, PH><<<nblk, 256, 0, st>>>(S, ph, d.srcload, U, d.Q, d.Pic, R, d.wave, d.wave_part);
+    }
     if (d.wave) wave_reduce_kernel<<<1, 256, 0, st>>>(d.wave_part, nblk, d.wave);
     if (ev) (void)hipEventRecord(ev[3], st);
     return (int)hipGetLastError();
 }
 
 template <class PH, int D>
-int dispatch_order(int p, const DevSpace& S, const PhysParams& ph, const DiscDev& dd, const double* U, double* R,
-                   hipStream_t st, hipEvent_t* ev) {
+int dispatch_order(int p, const DevSpace& S, const typename PH::Params& ph, const DiscDev& dd, const double* U,
+                   double* R, hipStream_t st, hipEvent_t* ev) {
     switch (p) {
     case 1: return launch_residual<D, 1, PH>(S, ph, dd, U, R, st, ev);
     case 2: return launch_residual<D, 2, PH>(S, ph, dd, U, R, st, ev);
@@ -1463,6 +1346,18 @@ int residual_fp64(int ndim, int p, int flux, const DevSpace& S, const PhysParams
         if (ndim == 2) return dispatch_order<NavierStokes<2, false>, 2>(p, S, ph, dd, U, R, st, ev);
         if (ndim == 3) return dispatch_order<NavierStokes<3, false>, 3>(p, S, ph, dd, U, R, st, ev);
     }
+    return -1;
+}
+
+// Mixed precision: FP32 interpolation, fluxes and face data, FP64 geometry,
+// source load and residual (2D).
+int residual_mixed(int ndim, int p, int flux, const DevSpace& S, const PhysParams& ph, const DiscDev& dd,
+                   const double* U, double* R, hipStream_t st, hipEvent_t* ev) {
+    if (ndim != 2) return -2;
+    const PhysT<float> pf = phys_cast<float>(ph);
+    if (flux == ICE_FLUX_BURGERS) return dispatch_order<Burgers<2, float>, 2>(p, S, pf, dd, U, R, st, ev);
+    if (flux == ICE_FLUX_NAVIER_STOKES) return dispatch_order<NavierStokes<2, true, float>, 2>(p, S, pf, dd, U, R, st, ev);
+    if (flux == ICE_FLUX_EULER) return dispatch_order<NavierStokes<2, false, float>, 2>(p, S, pf, dd, U, R, st, ev);
     return -1;
 }
 

@@ -35,6 +35,7 @@ _SIGS = {
     "ice_disc_create": ([P, C.POINTER(abi.DiscParams), C.POINTER(P)], C.c_int),
     "ice_disc_destroy": ([P], C.c_int),
     "ice_residual": ([P, P, P, P], C.c_int),
+    "ice_residual_mixed": ([P, P, P, P], C.c_int),
     "ice_profile_begin": ([P, C.c_int], C.c_int),
     "ice_profile_end": ([P, C.POINTER(C.c_double), C.POINTER(C.c_int)], C.c_int),
     "ice_field_eval_points": ([C.POINTER(abi.Field), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
@@ -195,14 +196,18 @@ def current_stream():
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
-def form_residual(space, disc, u, res, stream=None):
-    """res <- R(u) on the GPU (FP64)."""
+def form_residual(space, disc, u, res, stream=None, precision="fp64"):
+    """res <- R(u) on the GPU; precision "fp64" (the reference's arithmetic)
+    or "mixed" (FP32 interpolation and fluxes, FP64 geometry and output, 2D)."""
     _require_cuda(u, "u")
     _require_cuda(res, "res")
     if u.numel() != disc.size or res.numel() != disc.size:
         raise IceError(f"u/res must have {disc.size} entries")
     st = current_stream() if stream is None else stream
-    check(lib().ice_residual(disc.h, C.c_void_p(u.data_ptr()), C.c_void_p(res.data_ptr()), st))
+    fn = {"fp64": lib().ice_residual, "mixed": lib().ice_residual_mixed}.get(precision)
+    if fn is None:
+        raise IceError(f"unknown precision {precision!r}")
+    check(fn(disc.h, C.c_void_p(u.data_ptr()), C.c_void_p(res.data_ptr()), st))
 
 
 def _ptr(t):

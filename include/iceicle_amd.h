@@ -213,6 +213,14 @@ int ice_disc_destroy(ice_disc* disc);
  * in fe_layout_right order, both nelem*nbasis*neq doubles. */
 int ice_residual(ice_disc* disc, const double* u_dev, double* res_dev, void* stream);
 
+/* Mixed-precision residual (no reference counterpart; the reference is FP64
+ * throughout): the same R(u) with the trace interpolation, the physical and
+ * numerical fluxes, the volume contractions and the face data in FP32, and
+ * the geometric Jacobians, source load, face lifting sums and R in FP64.
+ * ndim = 2 only (ICE_ERR_UNSUPPORTED otherwise).  Expect ~1e-7 of the
+ * largest flux terms in absolute error. */
+int ice_residual_mixed(ice_disc* disc, const double* u_dev, double* r_dev, void* stream);
+
 /* Inverse mass matrix per element and component: out = M_e^-1 r.  Replaces
  * calculate_mass_matrix + decompose_lu + sub_lu of the explicit steppers
  * (finite_element.hpp:586-611, ssp_rk3.hpp:142-163); singular element mass
@@ -274,6 +282,8 @@ int ice_jacobian_apply(const ice_disc* disc, const double* blocks_dev, const dou
 #define ICE_PREC_FP32 1
 #define ICE_PC_BLOCK_JACOBI 0
 #define ICE_PC_BLOCK_SGS 1
+#define ICE_FORCING_FIXED 0
+#define ICE_FORCING_EW 1
 #define ICE_NEWTON_MAX_HIST 64
 
 typedef struct {
@@ -293,6 +303,9 @@ typedef struct {
     int verbose;              /* 1: one line per Newton step on stdout */
     int preconditioner;       /* ICE_PC_*: block Jacobi, or multicolour block
                                  symmetric Gauss-Seidel (default) */
+    int forcing;              /* ICE_FORCING_*: linear tolerance linear_rtol at
+                                 every step (default, as the reference's KSP),
+                                 or Eisenstat-Walker choice 2 */
 } ice_newton_opts;
 
 typedef struct {

@@ -96,3 +96,80 @@ def test_ns_3d(order):
     p = cases.ns_params(3)
     bct = ["DIRICHLET", "SLIP_WALL", "RIEMANN", "EXTRAPOLATION", "NO_SLIP_ISOTHERMAL", "DIRICHLET"]
     assert _run(3, [2, 3, 2], order, p, bct, perturb=0.1) <= TOL
+
+
+# ---- mixed precision (FP32 interpolation, fluxes and face data; FP64
+# geometry, source load and output) against the FP64 oracle.  FP32 rounding
+# of O(1) flux terms leaves ~1e-7 absolute error; the bound below is relative
+# to the largest residual entry of a far-from-equilibrium state.
+MIXED_TOL = 2e-5
+
+
+def _run_mixed(nel, order, params, bct, seed=0, state=None):
+    from iceicle_amd import engine
+    from oracle.oracle import Oracle
+    m = abi.mesh_desc(2, nel, [0.0, 0.0], [1.0, 1.0], bct)
+    coords = cases.perturbed_coords(m, 0.1, seed + 11)
+    m = abi.mesh_desc(2, nel, [0.0, 0.0], [1.0, 1.0], bct, coord=coords)
+    o = Oracle(m, abi.space_desc(order), params)
+    u = cases.random_state(params.flux, 2, o.nelem, o.nbasis, seed) if state is None else state
+    r_ref = o.residual(u)
+    space = engine.FESpace(m, order)
+    disc = engine.ConservationLawDDG(space, params)
+    ud = torch.from_numpy(u).cuda()
+    rd = torch.full_like(ud, float("nan"))
+    engine.form_residual(space, disc, ud, rd, precision="mixed")
+    torch.cuda.synchronize()
+    err = cases.rel_inf(rd.cpu().numpy(), r_ref)
+    print(f"mixed p={order} flux={params.flux} err={err:.3e}", flush=True)
+    return err
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_mixed_ns_mms(order):
+    from iceicle_amd import workloads
+    assert _run_mixed([4, 3], order, workloads.ns_mms_params(), ["DIRICHLET"] * 4) <= MIXED_TOL
+
+
+def test_mixed_euler_walls():
+    p = cases.ns_params(2, flux=abi.FLUX_EULER)
+    assert _run_mixed([3, 4], 3, p, ["SLIP_WALL", "RIEMANN", "DIRICHLET", "EXTRAPOLATION"]) <= MIXED_TOL
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_mixed_burgers(order):
+    p = cases.burgers_params(2, mu=0.02, a=(0.3, -0.2, 0))
+    assert _run_mixed([4, 4], order, p, ["DIRICHLET", "NEUMANN", "EXTRAPOLATION", "DIRICHLET"]) <= MIXED_TOL
+
+
+def test_mixed_rejects_3d():
+    from iceicle_amd import engine
+    m = abi.mesh_desc(3, [2, 2, 2], [0] * 3, [1] * 3, ["EXTRAPOLATION"] * 6)
+    disc = engine.ConservationLawDDG(engine.FESpace(m, 1), abi.default_params(abi.FLUX_EULER))
+    u = torch.ones(disc.size, dtype=torch.float64, device="cuda")
+    with pytest.raises(engine.IceError, match="ndim = 2"):
+        engine.form_residual(disc.space, disc, u, torch.empty_like(u), precision="mixed")
+
+
+def test_mixed_c2_against_fp64():
+    """Full-size 128 x 128 P3 workload: mixed vs FP64 device residual (the
+    FP64 one matches the oracle to 1e-15).  From the uniform free-stream
+    state (the Newton initial guess) R is far from zero and the bound is the
+    small-case one; at the manufactured state perturbed by 1e-3, R is ~1e-3 of
+    the flux terms it is summed from, so the same absolute FP32 rounding is
+    ~1e-4 of max |R|."""
+    from iceicle_amd import engine, workloads
+    m, order, p, keep = workloads.ns_mms_2d()
+    space = engine.FESpace(m, order)
+    disc = engine.ConservationLawDDG(space, p)
+    fs = np.tile([1.0, 0.3, 0.0, 1.0 / 1.4 / 0.4 + 0.045], disc.size // 4)
+    for name, state, tol in [("free stream", fs, MIXED_TOL),
+                             ("perturbed MMS", workloads.mms_state_perturbed(space, 4, workloads.SEED0 + 2), 5e-4)]:
+        u = torch.from_numpy(state).cuda()
+        r64, r32 = torch.empty_like(u), torch.empty_like(u)
+        engine.form_residual(space, disc, u, r64)
+        engine.form_residual(space, disc, u, r32, precision="mixed")
+        torch.cuda.synchronize()
+        err = float((r32 - r64).abs().max() / r64.abs().max())
+        print(f"C2 {name}: mixed vs fp64 rel err {err:.3e}", flush=True)
+        assert err <= tol
