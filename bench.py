@@ -175,7 +175,8 @@ def main():
         # preconditioner with FP64 residual and refinement
         fs = np.tile([1.0, 0.3, 0.0, 1.0 / 1.4 / 0.4 + 0.045], disc.size // 4)
         un = torch.from_numpy(fs).cuda()
-        solver = engine.PetscNewton(disc, tau_abs=1e-10, tau_rel=0.0, kmax=30)
+        solver = engine.PetscNewton(disc, tau_abs=1e-10, tau_rel=0.0, kmax=30, forcing=abi.FORCING_EW,
+                                    preconditioner=abi.PC_MULTIGRID)
         print("newton solve ...", file=sys.stderr, flush=True)
         rep = solver.solve(un)
         print(f"newton done: {rep['iterations']} steps, {rep['seconds']:.2f} s", file=sys.stderr, flush=True)
@@ -189,7 +190,8 @@ def main():
             "jacobian_seconds": round(rep["jacobian_seconds"], 4),
             "linear_seconds": round(rep["linear_seconds"], 4),
             "initial_guess": "uniform free stream",
-            "linear_solver": "GMRES(30), multicolour block SGS, FP32 blocks + FP64 refinement, rtol 1e-5",
+            "linear_solver": "GMRES(30) with an agglomeration multigrid V-cycle (P0 Galerkin coarse levels, "
+                             "block SGS smoothing), FP32 blocks + FP64 refinement, Eisenstat-Walker forcing",
         }
         del solver
 

@@ -176,6 +176,7 @@ PREC_FP64 = 0
 PREC_FP32 = 1
 PC_BLOCK_JACOBI = 0
 PC_BLOCK_SGS = 1
+PC_MULTIGRID = 2
 FORCING_FIXED = 0
 FORCING_EW = 1
 NEWTON_MAX_HIST = 64

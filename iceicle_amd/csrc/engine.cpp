@@ -183,6 +183,14 @@ int disc_dims(const ice_disc* d, int* nelem, int* nslots, int* bs) {
 }
 
 const int* disc_jcols(const ice_disc* d) { return d->space->dev.jcols; }
+
+// elements per direction of the structured mesh (x fastest), neq and nbasis
+void disc_grid(const ice_disc* d, int* nel, int* neq, int* nbasis) {
+    const ice_space* s = d->space;
+    for (int k = 0; k < 3; ++k) nel[k] = s->mesh.nel[k];
+    *neq = d->neq;
+    *nbasis = s->nb;
+}
 const ice_space* disc_space(const ice_disc* d) { return d->space; }
 int disc_neq(const ice_disc* d) { return d->neq; }
 void set_error(const std::string& m) { g_err = m; }

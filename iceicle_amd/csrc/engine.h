@@ -13,4 +13,5 @@ int disc_residual(ice_disc* d, const double* u, double* r, hipStream_t st);
 int disc_jacobian(ice_disc* d, const double* u, double* r, double* J, hipStream_t st);
 int disc_dims(const ice_disc* d, int* nelem, int* nslots, int* bs);
 const int* disc_jcols(const ice_disc* d);
+void disc_grid(const ice_disc* d, int* nel, int* neq, int* nbasis);
 }  // namespace ice

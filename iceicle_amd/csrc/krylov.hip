@@ -847,3 +847,136 @@ int convert_transpose_fp32(size_t nblocks, int bs, const double* a, float* b, hi
 }
 
 }  // namespace ice
+
+// ---------------------------------------------------------------------------
+// Geometric multigrid on element agglomerates (preconditioner of the Newton
+// solve): level 0 holds one constant per element and component (P0), the
+// Galerkin projection of the DG Jacobian onto the piecewise constants; each
+// further level merges 2 x 2 (x 2) neighbouring cells.  All levels keep the
+// block-row layout of the Jacobian (slot 0 the cell itself, slot 1 + j the
+// neighbour in the direction of local face j), with m x m blocks.
+
+namespace ice {
+namespace {
+
+// A0[e][s][v][w] = sum_{i,j} J[e][s][i m + v][j m + w]  (R J P with P the
+// injection of element constants, R = P^T); row-major FP64
+__global__ void galerkin_p0_kernel(int nelem, int nslots, int nb, int m, const double* __restrict__ J,
+                                   double* __restrict__ A0) {
+    const long tid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const long total = (long)nelem * nslots * m * m;
+    if (tid >= total) return;
+    const int w = (int)(tid % m), v = (int)((tid / m) % m);
+    const long es = tid / (m * m);
+    const int bs = nb * m;
+    const double* B = J + es * (long)bs * bs;
+    double acc = 0.0;
+    for (int i = 0; i < nb; ++i) {
+        const double* row = B + (long)(i * m + v) * bs + w;
+        for (int j = 0; j < nb; ++j) acc += row[j * m];
+    }
+    A0[tid] = acc;
+}
+
+// Ac[C][s][v][w] = sum over children c of C and slots t of c whose column
+// cell maps to slot s of C (the diagonal when it is a sibling)
+__global__ void galerkin_agg_kernel(int ncoarse, int nslots, int m, const int* __restrict__ children,
+                                    const int* __restrict__ fcols, const int* __restrict__ parent,
+                                    const double* __restrict__ Af, double* __restrict__ Ac) {
+    const long tid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const long total = (long)ncoarse * nslots * m * m;
+    if (tid >= total) return;
+    const int vw = (int)(tid % (m * m));
+    const int s = (int)((tid / (m * m)) % nslots);
+    const int C = (int)(tid / ((long)m * m * nslots));
+    double acc = 0.0;
+    for (int k = 0; k < 8; ++k) {
+        const int c = children[C * 8 + k];
+        if (c < 0) continue;
+        for (int t = 0; t < nslots; ++t) {
+            const int col = fcols[(long)c * nslots + t];
+            if (col < 0) continue;
+            const int target = (parent[col] == C) ? 0 : t;
+            if (target == s) acc += Af[(((long)c * nslots + t) * m * m) + vw];
+        }
+    }
+    Ac[tid] = acc;
+}
+
+// b0[e][v] = sum_i r[e][i][v]
+__global__ void restrict_nodes_kernel(int nelem, int nb, int m, const double* __restrict__ r, double* __restrict__ b0) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= nelem * m) return;
+    const int e = tid / m, v = tid - e * m;
+    const double* re = r + (size_t)e * nb * m + v;
+    double acc = 0.0;
+    for (int i = 0; i < nb; ++i) acc += re[i * m];
+    b0[tid] = acc;
+}
+
+// x[e][i][v] += x0[e][v]
+__global__ void prolong_nodes_kernel(int nelem, int nb, int m, const double* __restrict__ x0, double* __restrict__ x) {
+    const long tid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const long total = (long)nelem * nb * m;
+    if (tid >= total) return;
+    const int v = (int)(tid % m);
+    const long e = tid / ((long)nb * m);
+    x[tid] += x0[e * m + v];
+}
+
+// bc[C][v] = sum over children of r[c][v]
+__global__ void restrict_agg_kernel(int ncoarse, int m, const int* __restrict__ children, const double* __restrict__ r,
+                                    double* __restrict__ bc) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= ncoarse * m) return;
+    const int C = tid / m, v = tid - C * m;
+    double acc = 0.0;
+    for (int k = 0; k < 8; ++k) {
+        const int c = children[C * 8 + k];
+        if (c >= 0) acc += r[(size_t)c * m + v];
+    }
+    bc[tid] = acc;
+}
+
+// x[c][v] += xc[parent(c)][v]
+__global__ void prolong_agg_kernel(int nfine, int m, const int* __restrict__ parent, const double* __restrict__ xc,
+                                   double* __restrict__ x) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= nfine * m) return;
+    const int c = tid / m, v = tid - c * m;
+    x[tid] += xc[(size_t)parent[c] * m + v];
+}
+
+inline int grid_of(long n) { return (int)((n + 255) / 256); }
+
+}  // namespace
+
+int mg_galerkin_p0(int nelem, int nslots, int nb, int m, const double* J, double* A0, hipStream_t st) {
+    const long total = (long)nelem * nslots * m * m;
+    galerkin_p0_kernel<<<grid_of(total), 256, 0, st>>>(nelem, nslots, nb, m, J, A0);
+    return (int)hipGetLastError();
+}
+int mg_galerkin_agg(int ncoarse, int nslots, int m, const int* children, const int* fcols, const int* parent,
+                    const double* Af, double* Ac, hipStream_t st) {
+    const long total = (long)ncoarse * nslots * m * m;
+    galerkin_agg_kernel<<<grid_of(total), 256, 0, st>>>(ncoarse, nslots, m, children, fcols, parent, Af, Ac);
+    return (int)hipGetLastError();
+}
+int mg_restrict_nodes(int nelem, int nb, int m, const double* r, double* b0, hipStream_t st) {
+    restrict_nodes_kernel<<<grid_of((long)nelem * m), 256, 0, st>>>(nelem, nb, m, r, b0);
+    return (int)hipGetLastError();
+}
+int mg_prolong_nodes(int nelem, int nb, int m, const double* x0, double* x, hipStream_t st) {
+    prolong_nodes_kernel<<<grid_of((long)nelem * nb * m), 256, 0, st>>>(nelem, nb, m, x0, x);
+    return (int)hipGetLastError();
+}
+int mg_restrict_agg(int ncoarse, int m, const int* children, const double* r, double* bc, hipStream_t st) {
+    restrict_agg_kernel<<<grid_of((long)ncoarse * m), 256, 0, st>>>(ncoarse, m, children, r, bc);
+    return (int)hipGetLastError();
+}
+int mg_prolong_agg(int nfine, int m, const int* parent, const double* xc, double* x, hipStream_t st) {
+    prolong_agg_kernel<<<grid_of((long)nfine * m), 256, 0, st>>>(nfine, m, parent, xc, x);
+    return (int)hipGetLastError();
+}
+
+}  // namespace ice

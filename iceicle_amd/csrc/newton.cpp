@@ -35,6 +35,13 @@ template <class T, class V>
 int sgsT_sweep(bool forward, int nel, const int* elems, const int* ecolor, int colour, int nslots, int bs,
                const int* cols, const T* JT, const T* DT, const V* rhs, V* y, hipStream_t st);
 int convert_transpose_fp32(size_t nblocks, int bs, const double* a, float* b, hipStream_t st);
+int mg_galerkin_p0(int nelem, int nslots, int nb, int m, const double* J, double* A0, hipStream_t st);
+int mg_galerkin_agg(int ncoarse, int nslots, int m, const int* children, const int* fcols, const int* parent,
+                    const double* Af, double* Ac, hipStream_t st);
+int mg_restrict_nodes(int nelem, int nb, int m, const double* r, double* b0, hipStream_t st);
+int mg_prolong_nodes(int nelem, int nb, int m, const double* x0, double* x, hipStream_t st);
+int mg_restrict_agg(int ncoarse, int m, const int* children, const double* r, double* bc, hipStream_t st);
+int mg_prolong_agg(int nfine, int m, const int* parent, const double* xc, double* x, hipStream_t st);
 template <class T, class V>
 int block_diag_apply(int nelem, int bs, const T* D, const V* x, V* y, hipStream_t st);
 int convert_to_fp32(size_t n, const double* a, float* b, hipStream_t st);
@@ -156,6 +163,24 @@ double corrigan_linesearch(const Phi& f, const ice_newton_opts& o) {
 
 }  // namespace ice
 
+// One level of the agglomeration multigrid: n cells of an nx x ny x nz grid
+// (x fastest), m x m blocks in the Jacobian's slot layout.
+struct MGLevel {
+    int n = 0;
+    int dims[3] = {1, 1, 1};
+    std::vector<int> hcols;        // [n][nslots]
+    const int* cols = nullptr;     // device copy of hcols
+    int* parent = nullptr;         // [n] cell of the next coarser level (null on the coarsest)
+    int* children = nullptr;       // [n][8] cells of the next finer level (level 0: none)
+    double* A64 = nullptr;         // [n][nslots][m][m] row-major
+    float* A32 = nullptr;          // column-major FP32 copy
+    float* Dinv = nullptr;         // column-major factored diagonal blocks
+    int* ecolor = nullptr;
+    int* celems = nullptr;
+    std::vector<int> coff;
+    double *x = nullptr, *b = nullptr, *r = nullptr, *t = nullptr;
+};
+
 struct ice_newton {
     ice_disc* disc = nullptr;
     ice_newton_opts opt{};
@@ -180,6 +205,9 @@ struct ice_newton {
     double* sc = nullptr;       // Hessenberg, rotations, g, norms
     double* y = nullptr;        // m
     int* nsing = nullptr;
+    std::vector<MGLevel> mg;    // multigrid levels (level 0 = element constants)
+    double *mg_r = nullptr, *mg_t = nullptr;  // fine-level work vectors of the V-cycle
+    int neq = 0, nbasis = 0;
     int* ecolor = nullptr;      // element colour (multicolour Gauss-Seidel)
     int* celems = nullptr;      // elements grouped by colour
     std::vector<int> coff;      // colour offsets into celems
@@ -219,41 +247,47 @@ struct ice_newton {
         if (fp32()) NT_LAUNCH((ice::bsrT_apply<float, double>(nelem, nslots, bs, cols, J32, x, nullptr, w, st)));
         else NT_LAUNCH((ice::bsr_apply<double, double>(nelem, nslots, bs, cols, J64, x, nullptr, w, st)));
     }
+    void apply_sgs_fine(const double* v, double* out, hipStream_t st) {
+        const int nc = (int)coff.size() - 1;
+        const int* cols = ice::disc_jcols(disc);
+        for (int pass = 0; pass < 2; ++pass)
+            for (int k = 0; k < nc; ++k) {
+                const bool fwd = pass == 0;
+                const int c = fwd ? k : nc - 1 - k;
+                if (!fwd && c == nc - 1) continue;  // last colour: backward sweep leaves it unchanged
+                const int* el = celems + coff[c];
+                const int nr = coff[c + 1] - coff[c];
+                if (fp32())
+                    NT_LAUNCH((ice::sgsT_sweep<float, double>(fwd, nr, el, ecolor, c, nslots, bs, cols, J32,
+                                                              (const float*)Dinv, v, out, st)));
+                else
+                    NT_LAUNCH((ice::sgs_sweep<double, double>(fwd, nr, el, ecolor, c, nslots, bs, cols, J64,
+                                                              (const double*)Dinv, v, out, st)));
+            }
+    }
+
     void apply_M(const double* v, double* out, hipStream_t st) {
+        if (opt.preconditioner == ICE_PC_MULTIGRID) {
+            apply_mg(v, out, st);
+            return;
+        }
         if (opt.preconditioner == ICE_PC_BLOCK_SGS) {
-            const int nc = (int)coff.size() - 1;
-            const int* cols = ice::disc_jcols(disc);
-            for (int pass = 0; pass < 2; ++pass)
-                for (int k = 0; k < nc; ++k) {
-                    const bool fwd = pass == 0;
-                    const int c = fwd ? k : nc - 1 - k;
-                    if (!fwd && c == nc - 1) continue;  // last colour: backward sweep leaves it unchanged
-                    const int* el = celems + coff[c];
-                    const int nr = coff[c + 1] - coff[c];
-                    if (fp32())
-                        NT_LAUNCH((ice::sgsT_sweep<float, double>(fwd, nr, el, ecolor, c, nslots, bs, cols, J32,
-                                                                  (const float*)Dinv, v, out, st)));
-                    else
-                        NT_LAUNCH((ice::sgs_sweep<double, double>(fwd, nr, el, ecolor, c, nslots, bs, cols, J64,
-                                                                  (const double*)Dinv, v, out, st)));
-                }
+            apply_sgs_fine(v, out, st);
             return;
         }
         if (fp32()) NT_LAUNCH((ice::diagT_apply<float, double>(nelem, bs, (const float*)Dinv, v, out, st)));
         else NT_LAUNCH((ice::block_diag_apply<double, double>(nelem, bs, (const double*)Dinv, v, out, st)));
     }
 
-    // greedy colouring of the element face graph (neighbours differ)
-    void colour() {
-        std::vector<int> cols((size_t)nelem * nslots);
-        int ns = 0, b = 0;
-        if (ice_jacobian_layout(disc, &ns, &b, cols.data()) != ICE_OK) throw Fail{ICE_ERR_STATE, "no Jacobian layout"};
-        std::vector<int> col(nelem, -1);
+    // greedy colouring of a block graph (face neighbours differ); returns the
+    // colour offsets, fills the device colour and grouped-cell arrays
+    std::vector<int> colour_graph(int ncell, const std::vector<int>& cols, int** dcolour, int** dcells) {
+        std::vector<int> col(ncell, -1);
         int nc = 0;
-        for (int e = 0; e < nelem; ++e) {
+        for (int e = 0; e < ncell; ++e) {
             unsigned used = 0;
-            for (int s = 1; s < nslots; ++s) {
-                const int nb = cols[(size_t)e * nslots + s];
+            for (int s2 = 1; s2 < nslots; ++s2) {
+                const int nb = cols[(size_t)e * nslots + s2];
                 if (nb >= 0 && nb != e && col[nb] >= 0) used |= 1u << col[nb];
             }
             int c = 0;
@@ -261,15 +295,145 @@ struct ice_newton {
             col[e] = c;
             nc = std::max(nc, c + 1);
         }
-        coff.assign(nc + 1, 0);
-        for (int e = 0; e < nelem; ++e) ++coff[col[e] + 1];
-        for (int c = 0; c < nc; ++c) coff[c + 1] += coff[c];
-        std::vector<int> order(nelem), fill(coff.begin(), coff.end() - 1);
-        for (int e = 0; e < nelem; ++e) order[fill[col[e]]++] = e;
-        ecolor = alloc<int>(nelem);
-        celems = alloc<int>(nelem);
-        NT_HIP(hipMemcpy(ecolor, col.data(), nelem * sizeof(int), hipMemcpyHostToDevice));
-        NT_HIP(hipMemcpy(celems, order.data(), nelem * sizeof(int), hipMemcpyHostToDevice));
+        std::vector<int> off(nc + 1, 0);
+        for (int e = 0; e < ncell; ++e) ++off[col[e] + 1];
+        for (int c = 0; c < nc; ++c) off[c + 1] += off[c];
+        std::vector<int> order(ncell), fill(off.begin(), off.end() - 1);
+        for (int e = 0; e < ncell; ++e) order[fill[col[e]]++] = e;
+        *dcolour = alloc<int>(ncell);
+        *dcells = alloc<int>(ncell);
+        NT_HIP(hipMemcpy(*dcolour, col.data(), ncell * sizeof(int), hipMemcpyHostToDevice));
+        NT_HIP(hipMemcpy(*dcells, order.data(), ncell * sizeof(int), hipMemcpyHostToDevice));
+        return off;
+    }
+
+    void colour() {
+        std::vector<int> cols((size_t)nelem * nslots);
+        int ns = 0, b = 0;
+        if (ice_jacobian_layout(disc, &ns, &b, cols.data()) != ICE_OK) throw Fail{ICE_ERR_STATE, "no Jacobian layout"};
+        coff = colour_graph(nelem, cols, &ecolor, &celems);
+    }
+
+    // out = M^-1 rhs for one multicolour block SGS sweep with m x m blocks on level L
+    void sgs_level(const MGLevel& L, const double* rhs, double* out, hipStream_t st) {
+        const int nc = (int)L.coff.size() - 1;
+        for (int pass = 0; pass < 2; ++pass)
+            for (int k = 0; k < nc; ++k) {
+                const bool fwd = pass == 0;
+                const int c = fwd ? k : nc - 1 - k;
+                if (!fwd && c == nc - 1) continue;
+                NT_LAUNCH((ice::sgsT_sweep<float, double>(fwd, L.coff[c + 1] - L.coff[c], L.celems + L.coff[c],
+                                                          L.ecolor, c, nslots, neq, L.cols, L.A32, L.Dinv, rhs, out,
+                                                          st)));
+            }
+    }
+
+    // x = V-cycle approximation of A_l^-1 b (x need not be initialised)
+    void vcycle(int l, const double* b, double* x, hipStream_t st) {
+        MGLevel& L = mg[l];
+        const size_t nl = (size_t)L.n * neq;
+        sgs_level(L, b, x, st);
+        if (l + 1 == (int)mg.size()) {
+            for (int it = 0; it < 8; ++it) {
+                NT_LAUNCH((ice::bsrT_apply<float, double>(L.n, nslots, neq, L.cols, L.A32, x, b, L.r, st)));
+                sgs_level(L, L.r, L.t, st);
+                NT_LAUNCH(ice::launch_axpby2(nl, 1.0, L.t, 1.0, x, st));
+            }
+            return;
+        }
+        MGLevel& Cn = mg[l + 1];
+        NT_LAUNCH((ice::bsrT_apply<float, double>(L.n, nslots, neq, L.cols, L.A32, x, b, L.r, st)));
+        NT_LAUNCH(ice::mg_restrict_agg(Cn.n, neq, Cn.children, L.r, Cn.b, st));
+        vcycle(l + 1, Cn.b, Cn.x, st);
+        NT_LAUNCH(ice::mg_prolong_agg(L.n, neq, L.parent, Cn.x, x, st));
+        NT_LAUNCH((ice::bsrT_apply<float, double>(L.n, nslots, neq, L.cols, L.A32, x, b, L.r, st)));
+        sgs_level(L, L.r, L.t, st);
+        NT_LAUNCH(ice::launch_axpby2(nl, 1.0, L.t, 1.0, x, st));
+    }
+
+    // hierarchy structure: level 0 = the elements, then 2^ndim agglomerates
+    // down to at most 64 cells
+    void build_hierarchy() {
+        int nel[3], ne = 0, nbs = 0;
+        ice::disc_grid(disc, nel, &ne, &nbs);
+        neq = ne;
+        nbasis = nbs;
+        const int D = (nslots - 1) / 2;
+        MGLevel L0;
+        L0.n = nelem;
+        for (int k = 0; k < 3; ++k) L0.dims[k] = nel[k];
+        L0.hcols.resize((size_t)nelem * nslots);
+        int ns = 0, b = 0;
+        if (ice_jacobian_layout(disc, &ns, &b, L0.hcols.data()) != ICE_OK) throw Fail{ICE_ERR_STATE, "no layout"};
+        L0.cols = ice::disc_jcols(disc);
+        mg.push_back(std::move(L0));
+        while (mg.back().n > 64 && mg.size() < 16) {
+            MGLevel& F = mg.back();
+            MGLevel C;
+            for (int k = 0; k < 3; ++k) C.dims[k] = (k < D) ? (F.dims[k] + 1) / 2 : 1;
+            C.n = C.dims[0] * C.dims[1] * C.dims[2];
+            std::vector<int> par(F.n), ch((size_t)C.n * 8, -1), cnt(C.n, 0);
+            for (int c = 0; c < F.n; ++c) {
+                const int i = c % F.dims[0], j = (c / F.dims[0]) % F.dims[1], k = c / (F.dims[0] * F.dims[1]);
+                const int P = i / 2 + C.dims[0] * (j / 2 + C.dims[1] * (k / 2));
+                par[c] = P;
+                ch[(size_t)P * 8 + cnt[P]++] = c;
+            }
+            C.hcols.assign((size_t)C.n * nslots, -1);
+            for (int P = 0; P < C.n; ++P) C.hcols[(size_t)P * nslots] = P;
+            for (int c = 0; c < F.n; ++c)
+                for (int t = 1; t < nslots; ++t) {
+                    const int col = F.hcols[(size_t)c * nslots + t];
+                    if (col < 0 || par[col] == par[c]) continue;
+                    C.hcols[(size_t)par[c] * nslots + t] = par[col];
+                }
+            F.parent = alloc<int>(F.n);
+            NT_HIP(hipMemcpy(F.parent, par.data(), F.n * sizeof(int), hipMemcpyHostToDevice));
+            C.children = alloc<int>((size_t)C.n * 8);
+            NT_HIP(hipMemcpy(C.children, ch.data(), ch.size() * sizeof(int), hipMemcpyHostToDevice));
+            int* dc = alloc<int>(C.hcols.size());
+            NT_HIP(hipMemcpy(dc, C.hcols.data(), C.hcols.size() * sizeof(int), hipMemcpyHostToDevice));
+            C.cols = dc;
+            mg.push_back(std::move(C));
+        }
+        for (MGLevel& L : mg) {
+            const size_t nb2 = (size_t)L.n * nslots * neq * neq;
+            L.A64 = alloc<double>(nb2);
+            L.A32 = alloc<float>(nb2);
+            L.Dinv = alloc<float>((size_t)L.n * neq * neq);
+            for (double** v : {&L.x, &L.b, &L.r, &L.t}) *v = alloc<double>((size_t)L.n * neq);
+            L.coff = colour_graph(L.n, L.hcols, &L.ecolor, &L.celems);
+        }
+        mg_r = alloc<double>(n);
+        mg_t = alloc<double>(n);
+    }
+
+    // numeric setup of the hierarchy from the FP64 Jacobian
+    void factor_mg(hipStream_t st) {
+        for (size_t l = 0; l < mg.size(); ++l) {
+            MGLevel& L = mg[l];
+            if (l == 0)
+                NT_LAUNCH(ice::mg_galerkin_p0(nelem, nslots, nbasis, neq, J64, L.A64, st));
+            else
+                NT_LAUNCH(ice::mg_galerkin_agg(L.n, nslots, neq, L.children, mg[l - 1].cols, mg[l - 1].parent,
+                                               mg[l - 1].A64, L.A64, st));
+            NT_LAUNCH(ice::convert_transpose_fp32((size_t)L.n * nslots, neq, L.A64, L.A32, st));
+            NT_LAUNCH(ice::block_inverse<float>(L.n, nslots, neq, L.A64, L.Dinv, nsing, true, st));
+        }
+    }
+
+    // fine-level V-cycle: SGS, coarse correction through the element
+    // constants, SGS
+    void apply_mg(const double* v, double* out, hipStream_t st) {
+        const int* cols = ice::disc_jcols(disc);
+        apply_sgs_fine(v, out, st);
+        NT_LAUNCH((ice::bsrT_apply<float, double>(nelem, nslots, bs, cols, J32, out, v, mg_r, st)));
+        NT_LAUNCH(ice::mg_restrict_nodes(nelem, nbasis, neq, mg_r, mg[0].b, st));
+        vcycle(0, mg[0].b, mg[0].x, st);
+        NT_LAUNCH(ice::mg_prolong_nodes(nelem, nbasis, neq, mg[0].x, out, st));
+        NT_LAUNCH((ice::bsrT_apply<float, double>(nelem, nslots, bs, cols, J32, out, v, mg_r, st)));
+        apply_sgs_fine(mg_r, mg_t, st);
+        NT_LAUNCH(ice::launch_axpby2(n, 1.0, mg_t, 1.0, out, st));
     }
 
     void factor(hipStream_t st) {
@@ -280,6 +444,7 @@ struct ice_newton {
         } else {
             NT_LAUNCH(ice::block_inverse<double>(nelem, nslots, bs, J64, (double*)Dinv, nsing, false, st));
         }
+        if (opt.preconditioner == ICE_PC_MULTIGRID) factor_mg(st);
         factored = true;
     }
 
@@ -416,7 +581,8 @@ int ice_newton_create(ice_disc* d, const ice_newton_opts* opts, ice_newton** out
     if (o.gmres_restart < 1 || o.gmres_restart > 200 || o.kmax < 0 || o.gmres_maxit < 1 ||
         (o.jacobian_precision != ICE_PREC_FP32 && o.jacobian_precision != ICE_PREC_FP64) ||
         (o.linesearch < ICE_LS_NONE || o.linesearch > ICE_LS_CORRIGAN) ||
-        (o.preconditioner != ICE_PC_BLOCK_JACOBI && o.preconditioner != ICE_PC_BLOCK_SGS) ||
+        (o.preconditioner != ICE_PC_BLOCK_JACOBI && o.preconditioner != ICE_PC_BLOCK_SGS &&
+         o.preconditioner != ICE_PC_MULTIGRID) ||
         (o.forcing != ICE_FORCING_FIXED && o.forcing != ICE_FORCING_EW)) {
         ice::set_error("invalid Newton options");
         return ICE_ERR_ARG;
@@ -446,7 +612,11 @@ int ice_newton_create(ice_disc* d, const ice_newton_opts* opts, ice_newton** out
         s->sc = s->alloc<double>((size_t)(s->m + 1) * s->m + 2 * s->m + (s->m + 1) + 2);
         s->y = s->alloc<double>(s->m);
         s->nsing = s->alloc<int>(1);
-        if (o.preconditioner == ICE_PC_BLOCK_SGS) s->colour();
+        if (o.preconditioner != ICE_PC_BLOCK_JACOBI) s->colour();
+        if (o.preconditioner == ICE_PC_MULTIGRID) {
+            if (!s->fp32()) throw Fail{ICE_ERR_UNSUPPORTED, "the multigrid preconditioner runs with FP32 blocks"};
+            s->build_hierarchy();
+        }
         NT_HIP(hipHostMalloc((void**)&s->host, 8 * sizeof(double)));
         *out = s.release();
         return ICE_OK;

@@ -268,9 +268,11 @@ int ice_jacobian_apply(const ice_disc* disc, const double* blocks_dev, const dou
  * petsc_newton.hpp:174-183), picks alpha by the linesearch on
  * phi(alpha) = ||R(u - alpha du)||_2 (non-finite -> 1e9), sets u -= alpha du,
  * and stops when ||R||_2 <= tau_abs + tau_rel * ||R(u_0)||_2 or after kmax
- * steps.  The preconditioner is block Jacobi or one multicolour block
+ * steps.  The preconditioner is block Jacobi, one multicolour block
  * symmetric Gauss-Seidel sweep (elements coloured so that face neighbours
- * differ; the reference's default is PETSc's point SOR).  With ICE_PREC_FP32
+ * differ), or a V-cycle whose coarse levels are the Galerkin projections of
+ * the Jacobian onto element-wise constants and their 2^ndim agglomerates
+ * (the reference's default is PETSc's point SOR).  With ICE_PREC_FP32
  * the Krylov operator and the block-Jacobi factors
  * are FP32 copies of the FP64 blocks (FP64 vectors and accumulation) and the
  * correction is refined against the FP64 blocks: r = R - J du in FP64 until
@@ -282,6 +284,7 @@ int ice_jacobian_apply(const ice_disc* disc, const double* blocks_dev, const dou
 #define ICE_PREC_FP32 1
 #define ICE_PC_BLOCK_JACOBI 0
 #define ICE_PC_BLOCK_SGS 1
+#define ICE_PC_MULTIGRID 2
 #define ICE_FORCING_FIXED 0
 #define ICE_FORCING_EW 1
 #define ICE_NEWTON_MAX_HIST 64
@@ -301,8 +304,10 @@ typedef struct {
     double linear_rtol;       /* 1e-5 */
     int refine_max;           /* FP64 refinement passes (3) */
     int verbose;              /* 1: one line per Newton step on stdout */
-    int preconditioner;       /* ICE_PC_*: block Jacobi, or multicolour block
-                                 symmetric Gauss-Seidel (default) */
+    int preconditioner;       /* ICE_PC_*: block Jacobi, multicolour block
+                                 symmetric Gauss-Seidel (default), or a
+                                 multigrid V-cycle over element agglomerates
+                                 (P0 Galerkin coarse levels, SGS smoothing) */
     int forcing;              /* ICE_FORCING_*: linear tolerance linear_rtol at
                                  every step (default, as the reference's KSP),
                                  or Eisenstat-Walker choice 2 */

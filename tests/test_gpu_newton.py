@@ -87,7 +87,8 @@ def test_linear_solve_accuracy():
     b = torch.empty_like(u)
     jac.apply(x_true, b)
     for prec, pc, tol in [(abi.PREC_FP64, abi.PC_BLOCK_SGS, 1e-10), (abi.PREC_FP32, abi.PC_BLOCK_SGS, 1e-10),
-                          (abi.PREC_FP64, abi.PC_BLOCK_JACOBI, 1e-10), (abi.PREC_FP32, abi.PC_BLOCK_JACOBI, 1e-10)]:
+                          (abi.PREC_FP64, abi.PC_BLOCK_JACOBI, 1e-10), (abi.PREC_FP32, abi.PC_BLOCK_JACOBI, 1e-10),
+                          (abi.PREC_FP32, abi.PC_MULTIGRID, 1e-10)]:
         solver = engine.PetscNewton(disc, jacobian_precision=prec, linear_rtol=tol, gmres_restart=40,
                                     preconditioner=pc)
         x = torch.empty_like(u)
@@ -152,7 +153,8 @@ def test_c2_newton_full_size():
     from iceicle_amd import engine
     m, order, p, keep = workloads.ns_mms_2d()
     disc = engine.ConservationLawDDG(engine.FESpace(m, order), p)
-    solver = engine.PetscNewton(disc, tau_abs=1e-10, tau_rel=0.0, kmax=30, verbose=1)
+    solver = engine.PetscNewton(disc, tau_abs=1e-10, tau_rel=0.0, kmax=30, verbose=1,
+                                preconditioner=abi.PC_MULTIGRID, forcing=abi.FORCING_EW)
     u = torch.from_numpy(_freestream(disc.size)).cuda()
     rep = solver.solve(u)
     print({k: rep[k] for k in ("iterations", "rnorm", "seconds", "linear_iterations", "jacobian_seconds",
@@ -167,3 +169,23 @@ def test_c2_newton_full_size():
     # manufactured one at this resolution
     print(f"C2 L2 error vs manufactured solution {err:.3e}", flush=True)
     assert err < 1e-4
+
+
+def test_multigrid_newton_matches_dense():
+    """The agglomeration-multigrid preconditioner changes only the Krylov
+    iterations: the converged state is the oracle's."""
+    from iceicle_amd import engine
+    from oracle.oracle import Oracle
+    m, keep = _ns_mms(5, 2)
+    p = workloads.ns_mms_params()
+    o = Oracle(m, abi.space_desc(2), p)
+    u0 = _freestream(o.size)
+    u_ref, k_ref, _ = o.newton(u0, tau_abs=1e-12, kmax=20)
+    disc = engine.ConservationLawDDG(engine.FESpace(m, 2), p)
+    solver = engine.PetscNewton(disc, tau_abs=1e-12, tau_rel=0.0, kmax=20, linear_rtol=1e-10,
+                                preconditioner=abi.PC_MULTIGRID)
+    u = torch.from_numpy(u0).cuda()
+    rep = solver.solve(u)
+    print("multigrid", rep["rhist"], rep["linear_its"], flush=True)
+    assert rep["converged"]
+    assert cases.rel_inf(u.cpu().numpy(), u_ref) <= 1e-9
