@@ -557,6 +557,7 @@ int ice_disc_create(ice_space* s, const ice_disc_params* p, ice_disc** out) {
         ph.beta0 = (double)(order + 1) * (order + 1);
         ph.beta1 = p->interior_penalty ? 0.0 : 1.0 / std::max((double)(2 * order * (order + 1)), 1.0);
         ph.beta0_bdy = ph.beta0;
+        ice::set_derived(ph);
         d->ddgic = (!ns) && p->sigma_ic != 0.0;
         d->has_source = p->source.kind != ICE_FIELD_NONE;
         std::vector<ice_field> fields(2 * ICE_MAX_BCFLAG + 1);

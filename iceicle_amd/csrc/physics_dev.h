@@ -37,8 +37,28 @@ struct PhysT {
     // ConservationLawDDG
     R sigma_ic;
     R beta0, beta1, beta0_bdy;
+    // parameter-only factors of the flux formulas, computed once on the host
+    // (set_derived) so the pointwise fluxes multiply instead of divide
+    R p_fac;        // (gamma - 1) / (e_coeff Eu): p = p_fac rho e
+    R T_fac;        // gamma / ((gamma - 1) T_coeff): T = T_fac p / rho
+    R inv_e_coeff;  // 1 / e_coeff
+    R g_over_Pr;    // gamma / Pr
+    R inv_Re;       // 1 / Re
+    R inv_gamma;    // 1 / gamma
+    R vl_fac;       // 1 / (2 (gamma^2 - 1)), VanLeer energy split
 };
 using PhysParams = PhysT<double>;
+
+// fill the derived factors from the primary parameters
+inline void set_derived(PhysParams& p) {
+    p.p_fac = (p.gamma - 1) / (p.e_coeff * p.Eu);
+    p.T_fac = p.gamma / ((p.gamma - 1) * p.T_coeff);
+    p.inv_e_coeff = 1.0 / p.e_coeff;
+    p.g_over_Pr = p.gamma / p.Pr;
+    p.inv_Re = 1.0 / p.Re;
+    p.inv_gamma = 1.0 / p.gamma;
+    p.vl_fac = 1.0 / (2 * (p.gamma * p.gamma - 1));
+}
 
 // the same parameters rounded to another arithmetic type
 template <class R>
@@ -70,6 +90,13 @@ inline PhysT<R> phys_cast(const PhysParams& p) {
     q.beta0 = (R)p.beta0;
     q.beta1 = (R)p.beta1;
     q.beta0_bdy = (R)p.beta0_bdy;
+    q.p_fac = (R)p.p_fac;
+    q.T_fac = (R)p.T_fac;
+    q.inv_e_coeff = (R)p.inv_e_coeff;
+    q.g_over_Pr = (R)p.g_over_Pr;
+    q.inv_Re = (R)p.inv_Re;
+    q.inv_gamma = (R)p.inv_gamma;
+    q.vl_fac = (R)p.vl_fac;
     return q;
 }
 
@@ -156,8 +183,8 @@ struct NavierStokes {
         s.rhoE = u[D + 1];
         s.E = s.rhoE * s.irho;
         s.e = s.E - R(0.5) * p.e_coeff * s.vv;
-        s.p = fmax(R(1e-12), (p.gamma - 1) / (p.e_coeff * p.Eu) * s.rho * s.e);
-        s.T = s.p * s.irho * (p.gamma / ((p.gamma - 1) * p.T_coeff));
+        s.p = fmax(R(1e-12), p.p_fac * s.rho * s.e);
+        s.T = s.p * s.irho * p.T_fac;
         s.c = sqrt((p.gamma * p.Eu) * s.p * s.irho);
         return s;
     }
@@ -193,7 +220,7 @@ struct NavierStokes {
             for (int j = 0; j < D; ++j) tau[i][j] = mu * (gv[i][j] + gv[j][i]);
 #pragma unroll
         for (int i = 0; i < D; ++i) tau[i][i] -= R(2.0) / R(3.0) * mu * div;
-        const R iec = R(1.0) / p.e_coeff, kq = mu * p.gamma / p.Pr;
+        const R iec = p.inv_e_coeff, kq = mu * p.g_over_Pr;
 #pragma unroll
         for (int j = 0; j < D; ++j) {
             R qj = gE[j] * iec;
@@ -217,7 +244,7 @@ struct NavierStokes {
         if constexpr (FULL) {
             R tau[D][D], gE[D], q[D];
             viscous(p, s, g, viscosity(p, s.T), tau, gE, q);
-            const R iRe = R(1.0) / p.Re, ecRe = p.e_coeff * iRe;
+            const R iRe = p.inv_Re, ecRe = p.e_coeff * iRe;
 #pragma unroll
             for (int j = 0; j < D; ++j) {
                 R ef = q[j];
@@ -252,9 +279,9 @@ struct NavierStokes {
             const R fm = SL.rho * SL.c * (mL + 1) * (mL + 1) / R(4.0);
             f[0] = fm;
 #pragma unroll
-            for (int d = 0; d < D; ++d) f[1 + d] = fm * (SL.v[d] + n[d] * (-vnL + 2 * SL.c) / g);
+            for (int d = 0; d < D; ++d) f[1 + d] = fm * (SL.v[d] + n[d] * (-vnL + 2 * SL.c) * p.inv_gamma);
             const R t = (g - 1) * vnL + 2 * SL.c;
-            f[D + 1] = fm * ((SL.vv - vnL * vnL) / 2 + t * t / (2 * (g * g - 1)));
+            f[D + 1] = fm * ((SL.vv - vnL * vnL) / 2 + t * t * p.vl_fac);
         }
         if (mR < -1) {
             f[0] += SR.rho * vnR;
@@ -265,9 +292,9 @@ struct NavierStokes {
             const R fm = -SR.rho * SR.c * (mR - 1) * (mR - 1) / R(4.0);
             f[0] += fm;
 #pragma unroll
-            for (int d = 0; d < D; ++d) f[1 + d] += fm * (SR.v[d] + n[d] * (-vnR - 2 * SR.c) / g);
+            for (int d = 0; d < D; ++d) f[1 + d] += fm * (SR.v[d] + n[d] * (-vnR - 2 * SR.c) * p.inv_gamma);
             const R t = (g - 1) * vnR - 2 * SR.c;
-            f[D + 1] += fm * ((SR.vv - vnR * vnR) / 2 + t * t / (2 * (g * g - 1)));
+            f[D + 1] += fm * ((SR.vv - vnR * vnR) / 2 + t * t * p.vl_fac);
         }
     }
 
@@ -282,7 +309,7 @@ struct NavierStokes {
             const R mu = viscosity(p, s.T);
             R tau[D][D], gE[D], q[D];
             viscous(p, s, g, mu, tau, gE, q);
-            const R iRe = R(1.0) / p.Re, ecRe = p.e_coeff * iRe, kE = mu * p.gamma / p.Pr;
+            const R iRe = p.inv_Re, ecRe = p.e_coeff * iRe, kE = mu * p.g_over_Pr;
 #pragma unroll
             for (int j = 0; j < D; ++j) {
                 R ef = p.textbook_energy ? q[j] * n[j] : kE * gE[j] * n[j];

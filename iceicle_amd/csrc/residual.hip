@@ -84,14 +84,14 @@ struct TraceRec {
 // trace_kernel: out[v] = u, out[(1 + d) M + v] = gx/2 + bh hn on interior
 // faces, gx on boundary faces.
 template <int D, int N, int M, class VT>
-__device__ __forceinline__ void face_trace_sf(const DevSpace& S, const VT* Ue, int j, const int* pt,
+__device__ __forceinline__ void face_trace_sf(const DevSpace& S, const VT* Ltab, const VT* Ue, int j, const int* pt,
                                               const double (&X)[1 << D][D], const double* nrm, bool interior,
                                               double bh, VT* __restrict__ out) {
     constexpr int NB = Pow<N, D>::value;
     constexpr int NT = NB / N;  // tangential tensor points
     constexpr int NH = D * (D + 1) / 2;
     const double* xs = S.tab;
-    const VT* L0 = tab_of<VT>(S) + (N + 2);
+    const VT* L0 = Ltab;
     const VT* L1 = L0 + (N + 2) * N;
     const VT* L2 = L1 + (N + 2) * N;
     const int k = j % D;
@@ -291,29 +291,51 @@ __global__ __launch_bounds__(256, ICE_TRACE_MINW) void trace_kernel(DevSpace S, 
     constexpr bool GRAD = TraceRec<PH, D>::NEED_GRAD;
     constexpr int NV = TraceRec<PH, D>::NV;
     __shared__ VT sU[EPB * NB * M];
+    __shared__ VT sL[3 * (N + 2) * N];  // 1D values, first and second derivatives
+    __shared__ VT sOut[EPB * TPE * NV * M];  // the tile's trace records, written out contiguously
     const int e0 = blockIdx.x * EPB;
+    const int le = threadIdx.x / TPE;
+    const int t = threadIdx.x - le * TPE;
+    const int e = e0 + le;
+    const bool active = threadIdx.x < EPB * TPE && e < S.nelem;
+    const int j = t / NFQ, qp = t - j * NFQ;
+    // prologue: all global loads (coefficients, tables, this point's face
+    // data and element vertices) issued before the first barrier
     {
         const size_t base = (size_t)e0 * NB * M;
         const size_t total = (size_t)S.nelem * NB * M;
         for (int idx = threadIdx.x; idx < EPB * NB * M; idx += blockDim.x)
             sU[idx] = (VT)((base + idx < total) ? U[base + idx] : 0.0);
+        const VT* tv = tab_of<VT>(S) + (N + 2);
+        for (int idx = threadIdx.x; idx < 3 * (N + 2) * N; idx += blockDim.x) sL[idx] = tv[idx];
     }
-    __syncthreads();
-    const int le = threadIdx.x / TPE;
-    const int t = threadIdx.x - le * TPE;
-    const int e = e0 + le;
-    if (threadIdx.x >= EPB * TPE || e >= S.nelem) return;
-    const int j = t / NFQ, qp = t - j * NFQ;
-    int pt[D];
-    {
+    int pt[D] = {};
+    double X[1 << D][D];
+    double nrm[D] = {}, h = 1.0;
+    int ef = -1, q = qp;
+    if (active) {
         const signed char* fp = S.fpt + ((j * S.norient + 0) * NFQ + qp) * D;
 #pragma unroll
         for (int d = 0; d < D; ++d) pt[d] = fp[d];
+        if constexpr (GRAD) {
+            load_vertices<D>(S.X, e, X);
+            ef = S.efaces[(size_t)e * 2 * D + j];
+            // the face point q of this element point qp (inverse orientation permutation)
+            if (ef >= 0 && (ef >> 4) < S.nfi) q = S.finv[(j * S.norient + (ef & 7)) * NFQ + qp];
+            if (ef >= 0) {
+                const double* g = S.fgeo + ((size_t)(ef >> 4) * NFQ + q) * (D + 2);
+#pragma unroll
+                for (int d = 0; d < D; ++d) nrm[d] = g[d];
+                h = g[D + 1];
+            }
+        }
     }
-    const VT* L0 = tab_of<VT>(S) + (N + 2);
-    VT* out = T + (((size_t)e * 2 * D + j) * NFQ + qp) * NV * M;
+    __syncthreads();
+    const VT* L0 = sL;
+    VT* out = sOut + (size_t)threadIdx.x * NV * M;
     const VT* Ue = sU + le * NB * M;
-    if constexpr (!GRAD) {
+    if (!active) {
+    } else if constexpr (!GRAD) {
         VT a0[D][N];
 #pragma unroll
         for (int d = 0; d < D; ++d)
@@ -337,29 +359,16 @@ __global__ __launch_bounds__(256, ICE_TRACE_MINW) void trace_kernel(DevSpace S, 
 #pragma unroll
         for (int v = 0; v < M; ++v) out[v] = u[v];
     } else {
-        // face row, side and orientation of this element face; the face point
-        // q of this element point qp (inverse of the orientation permutation)
-        const int ef = S.efaces[(size_t)e * 2 * D + j];
-        const int row = ef >> 4;
-        const bool interior = ef >= 0 && row < S.nfi;
+        const bool interior = ef >= 0 && (ef >> 4) < S.nfi;
         const int side = (ef >> 3) & 1;
-        const int orient = ef & 7;
-        const int q = interior ? S.finv[(j * S.norient + orient) * NFQ + qp] : qp;
-        double X[1 << D][D];
-        load_vertices<D>(S.X, e, X);
-        double nrm[D], h = 1.0;
-        if (ef >= 0) {
-            const double* g = S.fgeo + ((size_t)row * NFQ + q) * (D + 2);
-#pragma unroll
-            for (int d = 0; d < D; ++d) nrm[d] = g[d];
-            h = g[D + 1];
-        } else {
-#pragma unroll
-            for (int d = 0; d < D; ++d) nrm[d] = 0.0;
-        }
         const double bh = (side ? 1.0 : -1.0) * (double)ph.beta1 * h;
-        face_trace_sf<D, N, M, VT>(S, Ue, j, pt, X, nrm, interior, bh, out);
+        face_trace_sf<D, N, M, VT>(S, sL, Ue, j, pt, X, nrm, interior, bh, out);
     }
+    __syncthreads();
+    // records of elements e0 .. e0 + ne - 1 are one contiguous range of T
+    const int ne = min(EPB, S.nelem - e0);
+    VT* dst = T + (size_t)e0 * TPE * NV * M;
+    for (int idx = threadIdx.x; idx < ne * TPE * NV * M; idx += blockDim.x) dst[idx] = sOut[idx];
 }
 
 // Numerical fluxes at face points from stored traces: interior points first
