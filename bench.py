@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-newton", action="store_true", help="skip the Newton time-to-solution leg")
+    ap.add_argument("--workload", default="ns2d", choices=["ns2d", "euler3d"],
+                    help="ns2d: the headline 2D NS MMS workload; euler3d: 3D Euler vortex 64^3 P3 "
+                         "(residual throughput and RK3-SSP step time)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="only run 20 device copies of an MB-sized buffer (HBM counter calibration)")
     return ap.parse_args()
@@ -65,6 +68,76 @@ def cpu_baseline(mesh, order, params, u, seconds):
     return n, el, o.size
 
 
+def euler3d(args):
+    """3D Euler isentropic vortex on 64^3 hexes, P3 (83,886,080 unknowns):
+    FP64 residual throughput, then RK3-SSP steps (textbook stages, fixed dt
+    from CFL 0.2 at the initial state) timed over `steps` steps."""
+    import torch
+    from iceicle_amd import abi, engine, workloads
+    n = 64 if args.n == 128 else args.n
+    mesh, order, params, ic = workloads.euler_vortex_3d(n, args.order)
+    space = engine.FESpace(mesh, order)
+    disc = engine.ConservationLawDDG(space, params)
+    u = torch.empty(disc.size, dtype=torch.float64, device="cuda")
+    engine.project(disc, ic, u)
+    u = torch.from_numpy(workloads.density_perturbed(u.cpu().numpy(), disc.neq, workloads.SEED0 + 3)).cuda()
+    r = torch.empty_like(u)
+    for _ in range(3):
+        engine.form_residual(space, disc, u, r)
+    torch.cuda.synchronize()
+    nres = max(5, min(args.steps, 50))
+    disc.profile_begin(nres)
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(nres):
+        engine.form_residual(space, disc, u, r)
+    stop.record()
+    torch.cuda.synchronize()
+    res_ms = start.elapsed_time(stop) / nres
+    stage_ms, ncalls = disc.profile_end()
+    assert bool(torch.isfinite(r).all())
+    # mixed-precision residual (FP32 interpolation, fluxes and face data)
+    r32 = torch.empty_like(u)
+    for _ in range(3):
+        engine.form_residual(space, disc, u, r32, precision="mixed")
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(nres):
+        engine.form_residual(space, disc, u, r32, precision="mixed")
+    stop.record()
+    torch.cuda.synchronize()
+    mixed_ms = start.elapsed_time(stop) / nres
+    mixed_err = float((r32 - r).abs().max() / r.abs().max())
+    # explicit steps at a fixed dt (CFL 0.2 over the element size and the
+    # initial maximum wave speed |v| + c ~ 2.9 + 1.1)
+    h = 40.0 / n
+    dt = 0.2 * h / (2 * order + 1) / 4.0
+    stepper = engine.ExplicitStepper(disc, abi.stepper_desc(abi.STEP_RK3_SSP, dt=dt, ntime=0))
+    nsteps = max(2, min(args.steps // 4, 50))
+    stepper.step(u)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(nsteps):
+        stepper.step(u)
+    stop.record()
+    torch.cuda.synchronize()
+    step_ms = start.elapsed_time(stop) / nsteps
+    assert bool(torch.isfinite(u).all())
+    ndof = disc.size
+    print(json.dumps({
+        "metric": "residual GDOF/s, 3D Euler vortex (FP64, 1 GPU)", "value": round(ndof / (res_ms * 1e-3) / 1e9, 4),
+        "unit": "GDOF/s", "n_gpus": 1, "ms_per_residual": round(res_ms, 4),
+        "kernels_ms": {k: round(v / ncalls, 4) for k, v in zip(("trace", "flux", "element"), stage_ms)},
+        "mixed": {"value": round(ndof / (mixed_ms * 1e-3) / 1e9, 4), "ms_per_residual": round(mixed_ms, 4),
+                  "rel_err_vs_fp64": mixed_err},
+        "rk3_ssp_step_ms": round(step_ms, 4), "rk3_steps_timed": nsteps, "dt": dt,
+        "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"3D Euler isentropic vortex {n}^3 hexes, P{order}, {order + 1}^3 GL, "
+                               f"coupled periodic faces, density perturbed U(-0.01, 0.01)",
+                   "elements": space.nelem, "unknowns": ndof},
+    }))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -82,6 +155,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes_read": 8 * n, "calibrate_copy_bytes_written": 8 * n, "copies": 20}))
         return
+
+    if args.workload == "euler3d":
+        return euler3d(args)
 
     mesh, order, params, coords = workloads.ns_mms_2d(args.n, args.order)
     space = engine.FESpace(mesh, order)

@@ -116,11 +116,11 @@ struct ice_space {
     std::vector<double> tab;          // host copy of the S tables
     std::vector<int4> fint, fbdy;
     std::vector<int> efaces;
-    std::vector<signed char> fpt, fvo, fperm, finv;
+    std::vector<signed char> fpt, fvo, fperm, finv, ftens;
     std::vector<double> X;
     std::vector<int> jcols;           // [nelem][2*ndim+1] block columns of the element-block Jacobian
     ice::DevSpace dev{};
-    void* dmem[12] = {};
+    void* dmem[13] = {};
     ~ice_space() {
         for (void* p : dmem)
             if (p) (void)hipFree(p);
@@ -344,6 +344,18 @@ int ice_space_create(const ice_mesh_desc* md, const ice_space_desc* sd, ice_spac
                     s->fperm[((size_t)j * no + o) * s->nfq + q] = (signed char)hit;
                     s->finv[((size_t)j * no + o) * s->nfq + hit] = (signed char)q;
                 }
+        // face point q of orientation o -> its index in the element's tensor
+        // order of the face's tangential axes (increasing axis, slowest first)
+        s->ftens.assign((size_t)2 * D * no * s->nfq, 0);
+        for (int j = 0; j < 2 * D; ++j)
+            for (int o = 0; o < no; ++o)
+                for (int q = 0; q < s->nfq; ++q) {
+                    const signed char* a = &s->fpt[(((size_t)j * no + o) * s->nfq + q) * D];
+                    int idx = 0;
+                    for (int d = 0; d < D; ++d)
+                        if (d != j % D) idx = idx * n + a[d];
+                    s->ftens[((size_t)j * no + o) * s->nfq + q] = (signed char)idx;
+                }
         s->fvo.assign(2 * D * 4, 0);
         for (int j = 0; j < 2 * D; ++j) {
             int v[4];
@@ -442,6 +454,8 @@ int ice_space_create(const ice_mesh_desc* md, const ice_space_desc* sd, ice_spac
         }
         dv.fperm = (const signed char*)s->dmem[8];
         dv.finv = (const signed char*)s->dmem[9];
+        s->dmem[12] = upload(s->ftens);
+        dv.ftens = (const signed char*)s->dmem[12];
         {
             const size_t ng = (size_t)(nfi + nfb) * s->nfq * (D + 2);
             ICE_HIP(hipMalloc(&s->dmem[10], std::max<size_t>(ng, 1) * sizeof(double)));
@@ -617,7 +631,7 @@ int ice_residual(ice_disc* d, const double* u, double* r, void* stream) {
 int ice_residual_mixed(ice_disc* d, const double* u, double* r, void* stream) {
     if (!d || !u || !r) return fail(ICE_ERR_ARG, "null argument");
     const ice_space* s = d->space;
-    if (s->ndim != 2) return fail(ICE_ERR_UNSUPPORTED, "the mixed-precision residual covers ndim = 2");
+    if (s->ndim < 2) return fail(ICE_ERR_UNSUPPORTED, "the mixed-precision residual covers ndim = 2 and 3");
     hipEvent_t* ev = nullptr;
     if (d->profiling && d->prof_n < d->prof_max) ev = &d->prof_ev[(size_t)4 * d->prof_n++];
     const int rc = ice::residual_mixed(s->ndim, s->p, d->prm.flux, s->dev, d->ph, d->dd, u, r, (hipStream_t)stream, ev);

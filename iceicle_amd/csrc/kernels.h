@@ -31,6 +31,8 @@ struct DevSpace {
     const double* fgeo;      // [nfi + nfb][nfq][ndim + 2] unit normal, w sqrt(g), DDG h per face point
     const signed char* fperm;  // [2*ndim][norient][nfq] face point q -> the element's canonical point
     const signed char* finv;   // inverse of fperm
+    const signed char* ftens;  // [2*ndim][norient][nfq] face point q -> its index in the element's
+                               // tensor order of the face's tangential axes
 };
 
 // face info word: fnl | fnr << 3 | orient << 6 | bctype << 9 | pdim << 13

@@ -31,7 +31,7 @@
 #include "physics_dev.h"
 
 // Minimum waves per SIMD requested from the register allocator for the
-// three residual kernels (build-time tuning knobs; 1 = no constraint).
+// residual kernels (build-time tuning knobs; 1 = no constraint).
 #ifndef ICE_TRACE_MINW
 #define ICE_TRACE_MINW 1
 #endif
@@ -40,6 +40,9 @@
 #endif
 #ifndef ICE_ELEM_MINW
 #define ICE_ELEM_MINW 1
+#endif
+#ifndef ICE_ELEM3_MINW
+#define ICE_ELEM3_MINW 1
 #endif
 
 namespace ice {
@@ -1096,6 +1099,456 @@ __global__ __launch_bounds__(256, ICE_ELEM_MINW) void element_kernel_2d(DevSpace
     for (int v = 0; v < M; ++v) Re[v] = r[v];
 }
 
+// 3D element kernel by sum factorisation.  Thread k of an element is the
+// tensor triple (k0, k1, k2) (dimension 0 slowest).  Each element owns one
+// LDS buffer of three [v][point] arrays that every stage reuses in place: a
+// stage reads its inputs into registers, the workgroup synchronises, then
+// the stage writes its outputs.
+//  1a (a, b, q2):   A0 = sum_c U l_c(x_q2),  A1 = sum_c U l_c'(x_q2)
+//  1b (a, q1, q2):  B00 = sum_b A0 l_b(x_q1), B01 = sum_b A0 l_b'(x_q1), B10 = sum_b A1 l_b(x_q1)
+//  1c (q0, q1, q2): u = sum_a B00 l_a(x_q0), du/dxi = (sum_a B00 l_a', sum_a B01 l_a, sum_a B10 l_a);
+//                   geometry, physical flux, reference-frame flux G_a = w det F A^T / det
+//  2a (q0, q1, i2): C0 = sum_q2 G0 l_i2, C1 = sum_q2 G1 l_i2, C2 = sum_q2 G2 l_i2'
+//  2b (q0, i1, i2): E0 = sum_q1 C0 l_i1, E1 = sum_q1 C1 l_i1' + C2 l_i1
+//  2c (i0, i1, i2): sum_q0 E0 l_i0' + E1 l_i0 (the domain integral,
+//                   conservation_law.hpp:244-314), minus the source load,
+//                   plus the lifted face fluxes.
+// Fluxes that use no gradient (Euler) skip A1, B01, B10 and the derivatives.
+// Lifting: the element's face fluxes are staged in LDS in the element's own
+// tensor order of each face's tangential axes (ftens), so the trace basis at
+// a face node factorises: sum_{pa,pb} Q l_sa(x_pa) l_sb(x_pb).  A thread
+// lifts only the faces its node lies on (the trace of every other basis
+// function vanishes there).  All global loads that depend on nothing else
+// (coefficients, vertices, face slots, tables) are issued before the first
+// barrier; the face fluxes, whose addresses come from the face slots, are
+// loaded into registers before phase 1a and stored to LDS after it.
+template <int P, class PH>
+__global__ __launch_bounds__(256, ICE_ELEM3_MINW) void element_kernel_3d(DevSpace S, typename PH::Params ph,
+                                                                         const double* __restrict__ srcload,
+                                                                         const double* __restrict__ U,
+                                                                         const typename PH::real* __restrict__ Q,
+                                                                         const typename PH::real* __restrict__ Pic,
+                                                                         double* __restrict__ R,
+                                                                         unsigned long long* __restrict__ wave,
+                                                                         double* __restrict__ wave_part) {
+    constexpr int D = 3;
+    using TL = ElemTile<D, P>;
+    constexpr int N = TL::N, NB = TL::NB, EPB = TL::EPB;
+    constexpr int NN = N * N;
+    constexpr int NFQ = NN;
+    constexpr int M = PH::M;
+    constexpr bool GRAD = TraceRec<PH, D>::NEED_GRAD;
+    constexpr int NBP = NB + 1;  // padded point stride of one [v][point] array
+    constexpr int ARR = M * NBP;
+    constexpr int EBUF = 3 * ARR;
+    constexpr int LIFT = 2 * D * NFQ * M;
+    constexpr int NV = 1 << D;
+    constexpr int NT2 = (N + 2) * N;
+    constexpr int QIT = (EPB * LIFT + 255) / 256;
+    using VT = typename PH::real;
+    __shared__ VT sbuf[EPB * EBUF];
+    __shared__ VT sQ[EPB * LIFT];
+    __shared__ VT sL[2 * NT2];
+    __shared__ double sLd[NT2];
+    __shared__ double sX[EPB * NV * D];
+    __shared__ int sEF[EPB * 2 * D];
+    __shared__ signed char sFT[2 * D * 8 * NFQ];
+    const int e0 = blockIdx.x * EPB;
+    const int le = threadIdx.x / NB;
+    const int k = threadIdx.x - le * NB;
+    const int k0 = k / NN, k1 = (k / N) % N, k2 = k % N;
+    const int e = e0 + le;
+    const bool active = (threadIdx.x < TL::NT) && (e < S.nelem);
+    const int lec = le < EPB ? le : 0;
+    VT* B = sbuf + (size_t)lec * EBUF;
+    // prologue: every global load that depends on nothing else
+    {
+        const size_t base = (size_t)e0 * NB * M;
+        const size_t total = (size_t)S.nelem * NB * M;
+        for (int idx = threadIdx.x; idx < EPB * NB * M; idx += blockDim.x) {
+            const int l = idx / (NB * M), rem = idx - l * (NB * M);
+            const int node = rem / M, v = rem - node * M;
+            sbuf[l * EBUF + v * NBP + node] = (VT)((base + idx < total) ? U[base + idx] : 0.0);
+        }
+        const size_t xb = (size_t)e0 * NV * D, xt = (size_t)S.nelem * NV * D;
+        for (int idx = threadIdx.x; idx < EPB * NV * D; idx += blockDim.x)
+            sX[idx] = (xb + idx < xt) ? S.X[xb + idx] : 0.0;
+        const size_t fb = (size_t)e0 * 2 * D, ft = (size_t)S.nelem * 2 * D;
+        for (int idx = threadIdx.x; idx < EPB * 2 * D; idx += blockDim.x)
+            sEF[idx] = (fb + idx < ft) ? S.efaces[fb + idx] : -1;
+        const VT* tv = tab_of<VT>(S) + (N + 2);
+        for (int idx = threadIdx.x; idx < 2 * NT2; idx += blockDim.x) sL[idx] = tv[idx];
+        for (int idx = threadIdx.x; idx < NT2; idx += blockDim.x) sLd[idx] = S.tab[(N + 2) + idx];
+        for (int idx = threadIdx.x; idx < 2 * D * 8 * NFQ && idx < 2 * D * S.norient * NFQ; idx += blockDim.x)
+            sFT[idx] = S.ftens[idx];
+    }
+    __syncthreads();
+    // the element face fluxes, loaded now and staged after phase 1a:
+    // sQ[l][j][t][v] with t the point's tensor index on the face
+    VT qv[QIT];
+    int qdst[QIT];
+#pragma unroll
+    for (int it = 0; it < QIT; ++it) {
+        const int idx = threadIdx.x + it * 256;
+        const int l = idx / LIFT, rem = idx - l * LIFT;
+        const int j = rem / (NFQ * M), w = rem - j * (NFQ * M);
+        const int fq = w / M, v = w - fq * M;
+        const int ef = (idx < EPB * LIFT) ? sEF[l * 2 * D + j] : -1;
+        qv[it] = (ef >= 0) ? Q[(size_t)(ef >> 4) * NFQ * M + w] : (VT)0;
+        qdst[it] = (ef >= 0) ? l * LIFT + (j * NFQ + sFT[(j * S.norient + (ef & 7)) * NFQ + fq]) * M + v : -1;
+    }
+    const double* xs = S.tab;
+    const double* wgl = S.tab + (N + 2) + 3 * (N + 2) * N;
+    const VT* L0 = sL;
+    const VT* L1 = sL + NT2;
+    VT o0[M], o1[M], o2[M];  // stage outputs held across the barrier
+    // 1a: (a = k0, b = k1, q2 = k2)
+    if (active) {
+        VT l0[N], l1[N];
+#pragma unroll
+        for (int c = 0; c < N; ++c) {
+            l0[c] = L0[k2 * N + c];
+            l1[c] = L1[k2 * N + c];
+        }
+        const VT* Ua = B + k0 * NN + k1 * N;
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            VT t0 = 0, t1 = 0;
+#pragma unroll
+            for (int c = 0; c < N; ++c) {
+                const VT x = Ua[v * NBP + c];
+                t0 += x * l0[c];
+                if constexpr (GRAD) t1 += x * l1[c];
+            }
+            o0[v] = t0;
+            o1[v] = t1;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < QIT; ++it)
+        if (qdst[it] >= 0) sQ[qdst[it]] = qv[it];
+    if (active) {
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            B[v * NBP + k] = o0[v];
+            if constexpr (GRAD) B[ARR + v * NBP + k] = o1[v];
+        }
+    }
+    __syncthreads();
+    // 1b: (a = k0, q1 = k1, q2 = k2)
+    if (active) {
+        VT l0[N], l1[N];
+#pragma unroll
+        for (int b = 0; b < N; ++b) {
+            l0[b] = L0[k1 * N + b];
+            l1[b] = L1[k1 * N + b];
+        }
+        const VT* A0 = B + k0 * NN + k2;
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            VT s00 = 0, s01 = 0, s10 = 0;
+#pragma unroll
+            for (int b = 0; b < N; ++b) {
+                const VT x = A0[v * NBP + b * N];
+                s00 += x * l0[b];
+                if constexpr (GRAD) {
+                    s01 += x * l1[b];
+                    s10 += A0[ARR + v * NBP + b * N] * l0[b];
+                }
+            }
+            o0[v] = s00;
+            o1[v] = s01;
+            o2[v] = s10;
+        }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            B[v * NBP + k] = o0[v];
+            if constexpr (GRAD) {
+                B[ARR + v * NBP + k] = o1[v];
+                B[2 * ARR + v * NBP + k] = o2[v];
+            }
+        }
+    }
+    __syncthreads();
+    // 1c: (q0 = k0, q1 = k1, q2 = k2)
+    double lam_w = 0.0, vis_w = 0.0;
+    if (active) {
+        double X[NV][D];
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+#pragma unroll
+            for (int d = 0; d < D; ++d) X[i][d] = sX[(lec * NV + i) * D + d];
+        const double xi[D] = {xs[k0], xs[k1], xs[k2]};
+        const double wq = wgl[k0] * wgl[k1] * wgl[k2];
+        double J[D][D], Hm[D][D][D], A[D][D];
+        q1_map<D>(X, xi, nullptr, J, Hm);
+        const double det = det_adj<D>(J, A);
+        const double detc = fmax(0.0, det);
+        const double detg = (det == 0.0) ? 1.0 : det;
+        const double idetg = 1.0 / detg;
+        VT l0[N], l1[N];
+#pragma unroll
+        for (int a = 0; a < N; ++a) {
+            l0[a] = L0[k0 * N + a];
+            l1[a] = L1[k0 * N + a];
+        }
+        VT ai[D][D];
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int j = 0; j < D; ++j) ai[a][j] = (VT)(A[a][j] * idetg);
+        const VT* Bq = B + k1 * N + k2;
+        VT u[M], gx[M * D], F[M * D];
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            VT s = 0, g0 = 0, g1 = 0, g2 = 0;
+#pragma unroll
+            for (int a = 0; a < N; ++a) {
+                const VT x = Bq[v * NBP + a * NN];
+                s += x * l0[a];
+                if constexpr (GRAD) {
+                    g0 += x * l1[a];
+                    g1 += Bq[ARR + v * NBP + a * NN] * l0[a];
+                    g2 += Bq[2 * ARR + v * NBP + a * NN] * l0[a];
+                }
+            }
+            u[v] = s;
+            // grad_x = grad_xi A / det (finite_element.hpp:220-258)
+#pragma unroll
+            for (int j = 0; j < D; ++j) gx[v * D + j] = g0 * ai[0][j] + g1 * ai[1][j] + g2 * ai[2][j];
+        }
+        PH::flux(ph, u, gx, F);
+        if (wave) {
+            lam_w = PH::wave_speed(ph, u);
+            vis_w = PH::visc_at(ph, u);
+            if (e == S.nelem - 1 && k == NB - 1) wave[1] = (unsigned long long)__double_as_longlong(lam_w);
+        }
+        const double fac = detc * wq * idetg;
+        VT ga[D][D];
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int j = 0; j < D; ++j) ga[a][j] = (VT)(A[a][j] * fac);
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            o0[v] = F[v * D + 0] * ga[0][0] + F[v * D + 1] * ga[0][1] + F[v * D + 2] * ga[0][2];
+            o1[v] = F[v * D + 0] * ga[1][0] + F[v * D + 1] * ga[1][1] + F[v * D + 2] * ga[1][2];
+            o2[v] = F[v * D + 0] * ga[2][0] + F[v * D + 1] * ga[2][1] + F[v * D + 2] * ga[2][2];
+        }
+    }
+    __shared__ double red[2 * 4];
+    if (wave) {
+        for (int off = 32; off > 0; off >>= 1) {
+            lam_w = fmax(lam_w, __shfl_xor(lam_w, off));
+            vis_w = fmax(vis_w, __shfl_xor(vis_w, off));
+        }
+        if ((threadIdx.x & 63) == 0) {
+            red[2 * (threadIdx.x >> 6)] = lam_w;
+            red[2 * (threadIdx.x >> 6) + 1] = vis_w;
+        }
+    }
+    __syncthreads();
+    if (wave && threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { a = fmax(a, red[2 * w]); b = fmax(b, red[2 * w + 1]); }
+        wave_part[2 * blockIdx.x] = a;
+        wave_part[2 * blockIdx.x + 1] = b;
+    }
+    if (active) {
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            B[v * NBP + k] = o0[v];
+            B[ARR + v * NBP + k] = o1[v];
+            B[2 * ARR + v * NBP + k] = o2[v];
+        }
+    }
+    __syncthreads();
+    // 2a: (q0 = k0, q1 = k1, i2 = k2)
+    if (active) {
+        VT l0[N], l1[N];
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+            l0[q] = L0[q * N + k2];
+            l1[q] = L1[q * N + k2];
+        }
+        const VT* G = B + k0 * NN + k1 * N;
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            VT c0 = 0, c1 = 0, c2 = 0;
+#pragma unroll
+            for (int q = 0; q < N; ++q) {
+                c0 += G[v * NBP + q] * l0[q];
+                c1 += G[ARR + v * NBP + q] * l0[q];
+                c2 += G[2 * ARR + v * NBP + q] * l1[q];
+            }
+            o0[v] = c0;
+            o1[v] = c1;
+            o2[v] = c2;
+        }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            B[v * NBP + k] = o0[v];
+            B[ARR + v * NBP + k] = o1[v];
+            B[2 * ARR + v * NBP + k] = o2[v];
+        }
+    }
+    __syncthreads();
+    // 2b: (q0 = k0, i1 = k1, i2 = k2)
+    if (active) {
+        VT l0[N], l1[N];
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+            l0[q] = L0[q * N + k1];
+            l1[q] = L1[q * N + k1];
+        }
+        const VT* C = B + k0 * NN + k2;
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            VT s0 = 0, s1 = 0;
+#pragma unroll
+            for (int q = 0; q < N; ++q) {
+                s0 += C[v * NBP + q * N] * l0[q];
+                s1 += C[ARR + v * NBP + q * N] * l1[q] + C[2 * ARR + v * NBP + q * N] * l0[q];
+            }
+            o0[v] = s0;
+            o1[v] = s1;
+        }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            B[v * NBP + k] = o0[v];
+            B[ARR + v * NBP + k] = o1[v];
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    // 2c: (i0 = k0, i1 = k1, i2 = k2)
+    double r[M];
+    {
+        VT l0[N], l1[N];
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+            l0[q] = L0[q * N + k0];
+            l1[q] = L1[q * N + k0];
+        }
+        const VT* E = B + k1 * N + k2;
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            VT s = 0;
+#pragma unroll
+            for (int q = 0; q < N; ++q) s += E[v * NBP + q * NN] * l1[q] + E[ARR + v * NBP + q * NN] * l0[q];
+            r[v] = (double)s;
+        }
+        if (srcload) {
+            const double* sl = srcload + ((size_t)e * NB + k) * M;
+#pragma unroll
+            for (int v = 0; v < M; ++v) r[v] -= sl[v];
+        }
+    }
+    // lifting of the faces this thread's node lies on
+    const int kd[D] = {k0, k1, k2};
+    unsigned mask = 0;
+#pragma unroll
+    for (int j = 0; j < 2 * D; ++j)
+        if (kd[j % D] == ((j < D) ? 0 : N - 1)) mask |= 1u << j;
+    while (mask) {
+        const int j = __builtin_ctz(mask);
+        mask &= mask - 1;
+        const int ef = sEF[lec * 2 * D + j];
+        if (ef < 0) continue;
+        const int kn = j % D;
+        const int sa = (kn == 0) ? k1 : k0, sb = (kn == 2) ? k1 : k2;
+        // the node's normal-axis basis function at the face's endpoint (S index n or n + 1)
+        const double en = sLd[((j < D) ? N : N + 1) * N + kd[kn]];
+        const double sgn = ((ef >> 3) & 1) ? -en : en;
+        double la[N], lb[N];
+#pragma unroll
+        for (int p = 0; p < N; ++p) {
+            la[p] = sLd[p * N + sa];
+            lb[p] = sLd[p * N + sb];
+        }
+        const VT* Qf = sQ + lec * LIFT + j * NFQ * M;
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            double acc = 0.0;
+#pragma unroll
+            for (int pa = 0; pa < N; ++pa) {
+                double t = 0.0;
+#pragma unroll
+                for (int pb = 0; pb < N; ++pb) t += (double)Qf[(pa * N + pb) * M + v] * lb[pb];
+                acc += t * la[pa];
+            }
+            r[v] += sgn * acc;
+        }
+    }
+    if constexpr (PH::HAS_HOMOGENEITY) {
+        if (Pic) {
+            const double* Ld = S.tab + (N + 2);
+            const double* Ld1 = Ld + (N + 2) * N;
+            double X[NV][D];
+#pragma unroll
+            for (int i = 0; i < NV; ++i)
+#pragma unroll
+                for (int d = 0; d < D; ++d) X[i][d] = sX[(lec * NV + i) * D + d];
+#pragma unroll 1
+            for (int j = 0; j < 2 * D; ++j) {
+                const int ef = sEF[lec * 2 * D + j];
+                if (ef < 0) continue;
+                const size_t slot = (size_t)(ef >> 4);
+                const signed char* fp = S.fpt + ((size_t)(j * S.norient + (ef & 7)) * NFQ) * D;
+                const VT* Ps = Pic + slot * NFQ * M * D;
+#pragma unroll 1
+                for (int fq = 0; fq < NFQ; ++fq) {
+                    double xi[D], cg[D];
+#pragma unroll
+                    for (int d = 0; d < D; ++d) xi[d] = xs[fp[fq * D + d]];
+#pragma unroll
+                    for (int a = 0; a < D; ++a) {
+                        double t = 1.0;
+#pragma unroll
+                        for (int d = 0; d < D; ++d) {
+                            const int sidx = fp[fq * D + d];
+                            t *= (d == a) ? Ld1[sidx * N + kd[d]] : Ld[sidx * N + kd[d]];
+                        }
+                        cg[a] = t;
+                    }
+                    double J[D][D], Hm[D][D][D], A[D][D];
+                    q1_map<D>(X, xi, nullptr, J, Hm);
+                    double det = det_adj<D>(J, A);
+                    det = (det == 0.0) ? 1.0 : det;
+                    double gphi[D];
+#pragma unroll
+                    for (int jj = 0; jj < D; ++jj) {
+                        double sacc = 0;
+#pragma unroll
+                        for (int a = 0; a < D; ++a) sacc += cg[a] * A[a][jj];
+                        gphi[jj] = sacc / det;
+                    }
+#pragma unroll
+                    for (int v = 0; v < M; ++v) {
+                        double sacc = 0;
+#pragma unroll
+                        for (int d = 0; d < D; ++d) sacc += (double)Ps[(fq * M + v) * D + d] * gphi[d];
+                        r[v] -= sacc;
+                    }
+                }
+            }
+        }
+    }
+    double* Re = R + ((size_t)e * NB + k) * M;
+#pragma unroll
+    for (int v = 0; v < M; ++v) Re[v] = r[v];
+}
+
 // Fold the per-block maxima of one residual into the running maxima
 // (lambda_max of BurgersFlux never resets, burgers.hpp:89; visc_max, navier_stokes.hpp:964-966).
 __global__ void wave_reduce_kernel(const double* __restrict__ part, int nblk, unsigned long long* __restrict__ wave) {
@@ -1318,8 +1771,11 @@ int launch_residual(const DevSpace& S, const typename PH::Params& ph, const Disc
     if constexpr (D == 2) {
         element_kernel_2d<P, PH><<<nblk, 256, 0, st>>>(S, ph, d.srcload, U, reinterpret_cast<const VT*>(d.Q),
                                                          reinterpret_cast<const VT*>(d.Pic), R, d.wave, d.wave_part);
+    } else if constexpr (D == 3) {
+        element_kernel_3d<P, PH><<<nblk, 256, 0, st>>>(S, ph, d.srcload, U, reinterpret_cast<const VT*>(d.Q),
+                                                         reinterpret_cast<const VT*>(d.Pic), R, d.wave, d.wave_part);
     } else {
-        static_assert(std::is_same<VT, double>::value || D == 2, "the mixed-precision residual is 2D only");
+        static_assert(std::is_same<VT, double>::value || D > 1, "the mixed-precision residual is 2D and 3D");
         if constexpr (std::is_same<VT, double>::value)
             element_kernel<D, P, PH><<<nblk, 256, 0, st>>>(S, ph, d.srcload, U, d.Q, d.Pic, R, d.wave, d.wave_part);
     }
@@ -1359,15 +1815,22 @@ int residual_fp64(int ndim, int p, int flux, const DevSpace& S, const PhysParams
 }
 
 // Mixed precision: FP32 interpolation, fluxes and face data, FP64 geometry,
-// source load and residual (2D).
+// source load and residual (2D and 3D).
+template <int D>
+int residual_mixed_dim(int p, int flux, const DevSpace& S, const PhysParams& ph, const DiscDev& dd, const double* U,
+                       double* R, hipStream_t st, hipEvent_t* ev) {
+    const PhysT<float> pf = phys_cast<float>(ph);
+    if (flux == ICE_FLUX_BURGERS) return dispatch_order<Burgers<D, float>, D>(p, S, pf, dd, U, R, st, ev);
+    if (flux == ICE_FLUX_NAVIER_STOKES) return dispatch_order<NavierStokes<D, true, float>, D>(p, S, pf, dd, U, R, st, ev);
+    if (flux == ICE_FLUX_EULER) return dispatch_order<NavierStokes<D, false, float>, D>(p, S, pf, dd, U, R, st, ev);
+    return -1;
+}
+
 int residual_mixed(int ndim, int p, int flux, const DevSpace& S, const PhysParams& ph, const DiscDev& dd,
                    const double* U, double* R, hipStream_t st, hipEvent_t* ev) {
-    if (ndim != 2) return -2;
-    const PhysT<float> pf = phys_cast<float>(ph);
-    if (flux == ICE_FLUX_BURGERS) return dispatch_order<Burgers<2, float>, 2>(p, S, pf, dd, U, R, st, ev);
-    if (flux == ICE_FLUX_NAVIER_STOKES) return dispatch_order<NavierStokes<2, true, float>, 2>(p, S, pf, dd, U, R, st, ev);
-    if (flux == ICE_FLUX_EULER) return dispatch_order<NavierStokes<2, false, float>, 2>(p, S, pf, dd, U, R, st, ev);
-    return -1;
+    if (ndim == 2) return residual_mixed_dim<2>(p, flux, S, ph, dd, U, R, st, ev);
+    if (ndim == 3) return residual_mixed_dim<3>(p, flux, S, ph, dd, U, R, st, ev);
+    return -2;
 }
 
 }  // namespace ice

@@ -198,7 +198,7 @@ def current_stream():
 
 def form_residual(space, disc, u, res, stream=None, precision="fp64"):
     """res <- R(u) on the GPU; precision "fp64" (the reference's arithmetic)
-    or "mixed" (FP32 interpolation and fluxes, FP64 geometry and output, 2D)."""
+    or "mixed" (FP32 interpolation and fluxes, FP64 geometry and output, 2D and 3D)."""
     _require_cuda(u, "u")
     _require_cuda(res, "res")
     if u.numel() != disc.size or res.numel() != disc.size:

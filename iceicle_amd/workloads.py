@@ -75,6 +75,25 @@ def burgers_2d(n=32, order=2, mu=0.01, periodic_mode=abi.PERIODIC_COUPLED, seed=
     return m, order, p, abi.Field.make("SINUSOIDS", ic)
 
 
+def euler_vortex_3d(n=64, order=3, half=20.0):
+    """Workload: 3D Euler on n^3 hexes of [-half, half]^3 with periodic faces
+    coupled, P{order}; isentropic vortex (examples/conservation_law/
+    isentropic_vortex.lua coefficients: K = 5, centre (-10, -10), free stream
+    (2, 2), c_inf = 1) extruded in z.  Returns (mesh, order, params, ic)."""
+    m = abi.mesh_desc(3, [n, n, n], [-half] * 3, [half] * 3, ["PERIODIC"] * 6,
+                      periodic_mode=abi.PERIODIC_COUPLED)
+    p = abi.default_params(abi.FLUX_EULER)
+    return m, order, p, abi.Field.make("ISENTROPIC_VORTEX", [-10.0, -10.0, 5.0, 1.0, 2.0, 2.0])
+
+
+def density_perturbed(u, neq, seed, amp=0.01):
+    """Multiply the density coefficients by 1 + U(-amp, amp) (seeded)."""
+    u = np.array(u, dtype=np.float64).reshape(-1, neq)
+    rng = np.random.default_rng(seed)
+    u[:, 0] *= 1.0 + rng.uniform(-amp, amp, size=u.shape[0])
+    return u.ravel()
+
+
 def mms_state_perturbed(space, neq, seed, amp=1e-3):
     """Nodal interpolant of the manufactured state with a small seeded
     perturbation, so the residual is far from equilibrium."""

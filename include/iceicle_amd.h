@@ -217,7 +217,7 @@ int ice_residual(ice_disc* disc, const double* u_dev, double* res_dev, void* str
  * throughout): the same R(u) with the trace interpolation, the physical and
  * numerical fluxes, the volume contractions and the face data in FP32, and
  * the geometric Jacobians, source load, face lifting sums and R in FP64.
- * ndim = 2 only (ICE_ERR_UNSUPPORTED otherwise).  Expect ~1e-7 of the
+ * ndim = 2 and 3 (ICE_ERR_UNSUPPORTED for ndim = 1).  Expect ~1e-7 of the
  * largest flux terms in absolute error. */
 int ice_residual_mixed(ice_disc* disc, const double* u_dev, double* r_dev, void* stream);
 

@@ -64,7 +64,7 @@ def test_burgers_1d(order):
     assert _run(1, [6], order, p, ["DIRICHLET", "DIRICHLET"], perturb=0.2, bcflag=[0, 1]) <= TOL
 
 
-@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
 def test_burgers_3d(order):
     p = cases.burgers_params(3, a=(0.2, 0.1, -0.3))
     bct = ["PERIODIC", "DIRICHLET", "NEUMANN", "PERIODIC", "EXTRAPOLATION", "DIRICHLET"]
@@ -84,18 +84,26 @@ def test_ns_2d_walls_riemann(order):
     assert _run(2, [4, 3], order, p, bct, perturb=0.1) <= TOL
 
 
-@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
 def test_euler_3d_coupled(order):
     p = cases.ns_params(3, flux=abi.FLUX_EULER)
     bct = ["PERIODIC", "SLIP_WALL", "PERIODIC", "PERIODIC", "SLIP_WALL", "PERIODIC"]
     assert _run(3, [3, 3, 2], order, p, bct, perturb=0.1, coupled=True) <= TOL
 
 
-@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("order", [1, 2, 3])
 def test_ns_3d(order):
     p = cases.ns_params(3)
     bct = ["DIRICHLET", "SLIP_WALL", "RIEMANN", "EXTRAPOLATION", "NO_SLIP_ISOTHERMAL", "DIRICHLET"]
     assert _run(3, [2, 3, 2], order, p, bct, perturb=0.1) <= TOL
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_burgers_3d_ddgic(order):
+    """The DDGIC lifting term (sigma_ic != 0) of the 3D element kernel."""
+    p = cases.burgers_params(3, mu=0.05, a=(0.3, 0.0, -0.2), sigma_ic=0.5)
+    bct = ["DIRICHLET", "PERIODIC", "NEUMANN", "DIRICHLET", "PERIODIC", "EXTRAPOLATION"]
+    assert _run(3, [2, 3, 3], order, p, bct, perturb=0.1, coupled=True, bcflag=[0, 0, 0, 1, 0, 0]) <= TOL
 
 
 # ---- mixed precision (FP32 interpolation, fluxes and face data; FP64
@@ -108,11 +116,12 @@ MIXED_TOL = 2e-5
 def _run_mixed(nel, order, params, bct, seed=0, state=None):
     from iceicle_amd import engine
     from oracle.oracle import Oracle
-    m = abi.mesh_desc(2, nel, [0.0, 0.0], [1.0, 1.0], bct)
+    nd = len(nel)
+    m = abi.mesh_desc(nd, nel, [0.0] * nd, [1.0] * nd, bct)
     coords = cases.perturbed_coords(m, 0.1, seed + 11)
-    m = abi.mesh_desc(2, nel, [0.0, 0.0], [1.0, 1.0], bct, coord=coords)
+    m = abi.mesh_desc(nd, nel, [0.0] * nd, [1.0] * nd, bct, coord=coords)
     o = Oracle(m, abi.space_desc(order), params)
-    u = cases.random_state(params.flux, 2, o.nelem, o.nbasis, seed) if state is None else state
+    u = cases.random_state(params.flux, nd, o.nelem, o.nbasis, seed) if state is None else state
     r_ref = o.residual(u)
     space = engine.FESpace(m, order)
     disc = engine.ConservationLawDDG(space, params)
@@ -121,7 +130,7 @@ def _run_mixed(nel, order, params, bct, seed=0, state=None):
     engine.form_residual(space, disc, ud, rd, precision="mixed")
     torch.cuda.synchronize()
     err = cases.rel_inf(rd.cpu().numpy(), r_ref)
-    print(f"mixed p={order} flux={params.flux} err={err:.3e}", flush=True)
+    print(f"mixed ndim={nd} p={order} flux={params.flux} err={err:.3e}", flush=True)
     return err
 
 
@@ -142,12 +151,25 @@ def test_mixed_burgers(order):
     assert _run_mixed([4, 4], order, p, ["DIRICHLET", "NEUMANN", "EXTRAPOLATION", "DIRICHLET"]) <= MIXED_TOL
 
 
-def test_mixed_rejects_3d():
+@pytest.mark.parametrize("order", [2, 3])
+def test_mixed_euler_3d(order):
+    p = cases.ns_params(3, flux=abi.FLUX_EULER)
+    bct = ["SLIP_WALL", "RIEMANN", "DIRICHLET", "EXTRAPOLATION", "SLIP_WALL", "DIRICHLET"]
+    assert _run_mixed([3, 2, 2], order, p, bct) <= MIXED_TOL
+
+
+def test_mixed_ns_3d():
+    p = cases.ns_params(3)
+    bct = ["DIRICHLET", "SLIP_WALL", "RIEMANN", "EXTRAPOLATION", "NO_SLIP_ISOTHERMAL", "DIRICHLET"]
+    assert _run_mixed([2, 2, 3], 2, p, bct) <= MIXED_TOL
+
+
+def test_mixed_rejects_1d():
     from iceicle_amd import engine
-    m = abi.mesh_desc(3, [2, 2, 2], [0] * 3, [1] * 3, ["EXTRAPOLATION"] * 6)
+    m = abi.mesh_desc(1, [4], [0], [1], ["EXTRAPOLATION"] * 2)
     disc = engine.ConservationLawDDG(engine.FESpace(m, 1), abi.default_params(abi.FLUX_EULER))
     u = torch.ones(disc.size, dtype=torch.float64, device="cuda")
-    with pytest.raises(engine.IceError, match="ndim = 2"):
+    with pytest.raises(engine.IceError, match="ndim = 2 and 3"):
         engine.form_residual(disc.space, disc, u, torch.empty_like(u), precision="mixed")
 
 
