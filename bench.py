@@ -44,8 +44,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-newton", action="store_true", help="skip the Newton time-to-solution leg")
     ap.add_argument("--workload", default="ns2d", choices=["ns2d", "euler3d"],
-                    help="ns2d: the headline 2D NS MMS workload; euler3d: 3D Euler vortex 64^3 P3 "
-                         "(residual throughput and RK3-SSP step time)")
+                    help="ns2d: the headline 2D NS MMS workload (with the 3D Euler leg appended); "
+                         "euler3d: only the 3D Euler vortex 64^3 P3 leg")
+    ap.add_argument("--no-euler3d", action="store_true", help="skip the 3D Euler vortex leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="only run 20 device copies of an MB-sized buffer (HBM counter calibration)")
     return ap.parse_args()
@@ -68,14 +69,14 @@ def cpu_baseline(mesh, order, params, u, seconds):
     return n, el, o.size
 
 
-def euler3d(args):
+def euler3d(args, n=64, order=3, nres=30, nsteps=50):
     """3D Euler isentropic vortex on 64^3 hexes, P3 (83,886,080 unknowns):
-    FP64 residual throughput, then RK3-SSP steps (textbook stages, fixed dt
-    from CFL 0.2 at the initial state) timed over `steps` steps."""
+    FP64 and mixed residual throughput, then RK3-SSP steps (textbook stages,
+    fixed dt from CFL 0.2 at the initial state) timed over `nsteps` steps."""
     import torch
     from iceicle_amd import abi, engine, workloads
-    n = 64 if args.n == 128 else args.n
-    mesh, order, params, ic = workloads.euler_vortex_3d(n, args.order)
+    print(f"euler3d {n}^3 P{order} ...", file=sys.stderr, flush=True)
+    mesh, order, params, ic = workloads.euler_vortex_3d(n, order)
     space = engine.FESpace(mesh, order)
     disc = engine.ConservationLawDDG(space, params)
     u = torch.empty(disc.size, dtype=torch.float64, device="cuda")
@@ -85,7 +86,6 @@ def euler3d(args):
     for _ in range(3):
         engine.form_residual(space, disc, u, r)
     torch.cuda.synchronize()
-    nres = max(5, min(args.steps, 50))
     disc.profile_begin(nres)
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     start.record()
@@ -113,7 +113,6 @@ def euler3d(args):
     h = 40.0 / n
     dt = 0.2 * h / (2 * order + 1) / 4.0
     stepper = engine.ExplicitStepper(disc, abi.stepper_desc(abi.STEP_RK3_SSP, dt=dt, ntime=0))
-    nsteps = max(2, min(args.steps // 4, 50))
     stepper.step(u)
     torch.cuda.synchronize()
     start.record()
@@ -124,7 +123,8 @@ def euler3d(args):
     step_ms = start.elapsed_time(stop) / nsteps
     assert bool(torch.isfinite(u).all())
     ndof = disc.size
-    print(json.dumps({
+    del stepper, disc, space
+    return {
         "metric": "residual GDOF/s, 3D Euler vortex (FP64, 1 GPU)", "value": round(ndof / (res_ms * 1e-3) / 1e9, 4),
         "unit": "GDOF/s", "n_gpus": 1, "ms_per_residual": round(res_ms, 4),
         "kernels_ms": {k: round(v / ncalls, 4) for k, v in zip(("trace", "flux", "element"), stage_ms)},
@@ -134,8 +134,8 @@ def euler3d(args):
         "higher_is_better": True, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"3D Euler isentropic vortex {n}^3 hexes, P{order}, {order + 1}^3 GL, "
                                f"coupled periodic faces, density perturbed U(-0.01, 0.01)",
-                   "elements": space.nelem, "unknowns": ndof},
-    }))
+                   "elements": n ** 3, "unknowns": ndof},
+    }
 
 
 def main():
@@ -157,7 +157,8 @@ def main():
         return
 
     if args.workload == "euler3d":
-        return euler3d(args)
+        print(json.dumps(euler3d(args)))
+        return
 
     mesh, order, params, coords = workloads.ns_mms_2d(args.n, args.order)
     space = engine.FESpace(mesh, order)
@@ -313,6 +314,10 @@ def main():
     out["mixed"] = mixed
     if newton is not None:
         out["newton"] = newton
+    if not args.no_euler3d:
+        del disc, space
+        torch.cuda.empty_cache()
+        out["euler3d"] = euler3d(args)
     if not args.no_cpu_baseline:
         n, el, size = cpu_baseline(mesh, order, params, u_host, args.cpu_seconds)
         out["cpu_baseline"] = {

@@ -592,11 +592,8 @@ int ice_disc_create(ice_space* s, const ice_disc_params* p, ice_disc** out) {
             throw ice::Error{ICE_ERR_HIP, "table setup launch failed"};
         ICE_HIP(hipMalloc(&d->d_wave, 4 * sizeof(unsigned long long)));
         ICE_HIP(hipMemset(d->d_wave, 0, 4 * sizeof(unsigned long long)));
-        {
-            const int nb = s->nb, epb = std::max(1, 256 / nb);
-            const size_t nblk = (size_t)(s->mesh.nelem + epb - 1) / epb;
-            ICE_HIP(hipMalloc(&d->d_wave_part, 2 * nblk * sizeof(double)));
-        }
+        // per-workgroup maxima of the element kernels: at most one workgroup per element
+        ICE_HIP(hipMalloc(&d->d_wave_part, 2 * std::max<size_t>(s->mesh.nelem, 1) * sizeof(double)));
         ICE_HIP(hipDeviceSynchronize());
         d->dd.wave = nullptr;  // wave-speed bookkeeping only while a CFL stepper uses it
         d->dd.wave_part = d->d_wave_part;

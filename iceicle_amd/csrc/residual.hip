@@ -44,6 +44,11 @@
 #ifndef ICE_ELEM3_MINW
 #define ICE_ELEM3_MINW 1
 #endif
+// target threads per workgroup of the 3D element kernel (whole elements per
+// workgroup, rounded up to full waves)
+#ifndef ICE_ELEM3_TPB
+#define ICE_ELEM3_TPB 256
+#endif
 
 namespace ice {
 
@@ -1122,8 +1127,17 @@ __global__ __launch_bounds__(256, ICE_ELEM_MINW) void element_kernel_2d(DevSpace
 // (coefficients, vertices, face slots, tables) are issued before the first
 // barrier; the face fluxes, whose addresses come from the face slots, are
 // loaded into registers before phase 1a and stored to LDS after it.
-template <int P, class PH>
-__global__ __launch_bounds__(256, ICE_ELEM3_MINW) void element_kernel_3d(DevSpace S, typename PH::Params ph,
+template <int P, int TPB>
+struct Elem3Tile {
+    static constexpr int N = P + 1;
+    static constexpr int NB = N * N * N;
+    static constexpr int EPB = (TPB / NB) > 0 ? (TPB / NB) : 1;
+    static constexpr int NT = EPB * NB;
+    static constexpr int BLOCK = ((NT + 63) / 64) * 64;
+};
+
+template <int P, class PH, int TPB>
+__global__ __launch_bounds__((Elem3Tile<P, TPB>::BLOCK), ICE_ELEM3_MINW) void element_kernel_3d(DevSpace S, typename PH::Params ph,
                                                                          const double* __restrict__ srcload,
                                                                          const double* __restrict__ U,
                                                                          const typename PH::real* __restrict__ Q,
@@ -1132,8 +1146,8 @@ __global__ __launch_bounds__(256, ICE_ELEM3_MINW) void element_kernel_3d(DevSpac
                                                                          unsigned long long* __restrict__ wave,
                                                                          double* __restrict__ wave_part) {
     constexpr int D = 3;
-    using TL = ElemTile<D, P>;
-    constexpr int N = TL::N, NB = TL::NB, EPB = TL::EPB;
+    using TL = Elem3Tile<P, TPB>;
+    constexpr int N = TL::N, NB = TL::NB, EPB = TL::EPB, BLOCK = TL::BLOCK;
     constexpr int NN = N * N;
     constexpr int NFQ = NN;
     constexpr int M = PH::M;
@@ -1144,7 +1158,7 @@ __global__ __launch_bounds__(256, ICE_ELEM3_MINW) void element_kernel_3d(DevSpac
     constexpr int LIFT = 2 * D * NFQ * M;
     constexpr int NV = 1 << D;
     constexpr int NT2 = (N + 2) * N;
-    constexpr int QIT = (EPB * LIFT + 255) / 256;
+    constexpr int QIT = (EPB * LIFT + BLOCK - 1) / BLOCK;
     using VT = typename PH::real;
     __shared__ VT sbuf[EPB * EBUF];
     __shared__ VT sQ[EPB * LIFT];
@@ -1189,7 +1203,7 @@ __global__ __launch_bounds__(256, ICE_ELEM3_MINW) void element_kernel_3d(DevSpac
     int qdst[QIT];
 #pragma unroll
     for (int it = 0; it < QIT; ++it) {
-        const int idx = threadIdx.x + it * 256;
+        const int idx = threadIdx.x + it * BLOCK;
         const int l = idx / LIFT, rem = idx - l * LIFT;
         const int j = rem / (NFQ * M), w = rem - j * (NFQ * M);
         const int fq = w / M, v = w - fq * M;
@@ -1767,13 +1781,16 @@ int launch_residual(const DevSpace& S, const typename PH::Params& ph, const Disc
     if (nbi + nbb > 0) flux_kernel<D, P, PH><<<nbi + nbb, 256, 0, st>>>(S, ph, d, T, nbi);
     if (ev) (void)hipEventRecord(ev[2], st);
     using TL = ElemTile<D, P>;
-    const int nblk = (S.nelem + TL::EPB - 1) / TL::EPB;
+    int nblk = (S.nelem + TL::EPB - 1) / TL::EPB;
     if constexpr (D == 2) {
         element_kernel_2d<P, PH><<<nblk, 256, 0, st>>>(S, ph, d.srcload, U, reinterpret_cast<const VT*>(d.Q),
                                                          reinterpret_cast<const VT*>(d.Pic), R, d.wave, d.wave_part);
     } else if constexpr (D == 3) {
-        element_kernel_3d<P, PH><<<nblk, 256, 0, st>>>(S, ph, d.srcload, U, reinterpret_cast<const VT*>(d.Q),
-                                                         reinterpret_cast<const VT*>(d.Pic), R, d.wave, d.wave_part);
+        using T3 = Elem3Tile<P, ICE_ELEM3_TPB>;
+        nblk = (S.nelem + T3::EPB - 1) / T3::EPB;
+        element_kernel_3d<P, PH, ICE_ELEM3_TPB><<<nblk, T3::BLOCK, 0, st>>>(
+            S, ph, d.srcload, U, reinterpret_cast<const VT*>(d.Q), reinterpret_cast<const VT*>(d.Pic), R, d.wave,
+            d.wave_part);
     } else {
         static_assert(std::is_same<VT, double>::value || D > 1, "the mixed-precision residual is 2D and 3D");
         if constexpr (std::is_same<VT, double>::value)
