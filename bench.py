@@ -43,10 +43,12 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-newton", action="store_true", help="skip the Newton time-to-solution leg")
-    ap.add_argument("--workload", default="ns2d", choices=["ns2d", "euler3d"],
+    ap.add_argument("--workload", default="ns2d", choices=["ns2d", "euler3d", "warmstart"],
                     help="ns2d: the headline 2D NS MMS workload (with the 3D Euler leg appended); "
                          "euler3d: only the 3D Euler vortex 64^3 P3 leg")
     ap.add_argument("--no-euler3d", action="store_true", help="skip the 3D Euler vortex leg")
+    ap.add_argument("--no-warmstart", action="store_true", help="skip the learned warm-start leg")
+    ap.add_argument("--warmstart-train", type=int, default=4000, help="GPU-generated training solves")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="only run 20 device copies of an MB-sized buffer (HBM counter calibration)")
     return ap.parse_args()
@@ -138,6 +140,62 @@ def euler3d(args, n=64, order=3, nres=30, nsteps=50):
     }
 
 
+def warmstart_leg(ntrain, ntest=100, epochs=300):
+    """Learned warm start on the steady 1D viscous Burgers layer family
+    (256 elements, P3, 1024 unknowns; nu log-uniform in [1e-3, 1e-2], uL, uR
+    uniform in [0.5, 2]): ntrain GPU Newton solves from the plain guess,
+    Adam training of the 3 -> 256 -> 256 -> 1024 network on the device, then
+    ntest unseen problems solved from the plain guess and from the network's
+    guess (plain-guess fallback), both to ||R||_2 < 1e-10 with the same
+    Newton (Wolfe linesearch, exact block-tridiagonal preconditioner)."""
+    import torch
+    from iceicle_amd import abi, warmstart as W, workloads
+    print(f"warm start: generating {ntrain} solves ...", file=sys.stderr, flush=True)
+    params = W.sample_params(ntrain + ntest, seed=workloads.SEED0 + 4)
+    train, test = params[:ntrain], params[ntrain:]
+    gen = W.BurgersLayer1D(linesearch=abi.LS_NONE)
+    t0 = time.perf_counter()
+    sols, reps = W.generate(gen, train)
+    t_gen = time.perf_counter() - t0
+    ok = np.all(np.isfinite(sols), axis=1)
+    print(f"warm start: {int(ok.sum())}/{ntrain} converged in {t_gen:.1f} s; first report "
+          f"{ {k: reps[0][k] for k in ('converged', 'iterations', 'rhist')} }", file=sys.stderr, flush=True)
+    if not ok.any():
+        raise RuntimeError("no training solve converged")
+    fam = W.BurgersLayer1D()
+    ws = W.WarmStart(fam, seed=1)
+    hist, t_train = ws.train(train, sols, abi.mlp_train_opts(batch=64, epochs=epochs, lr=2e-3, lr_decay=0.99, seed=3))
+    print(f"warm start: trained in {t_train:.1f} s, evaluating {ntest} ...", file=sys.stderr, flush=True)
+    plain_its, plain_s, plain_ok, warm_its, warm_s, warm_ok, used = [], [], 0, [], [], 0, 0
+    for prm in test:
+        fam.set(*prm)
+        u = torch.from_numpy(fam.plain_guess(prm[1], prm[2])).cuda()
+        t1 = time.perf_counter()
+        rep = fam.solve(u)
+        plain_s.append(time.perf_counter() - t1)
+        plain_its.append(rep["iterations"])
+        plain_ok += int(rep["converged"])
+        uw, rep, how = ws.solve(*prm)
+        warm_s.append(rep["seconds_total"])
+        warm_its.append(rep["iterations"])
+        warm_ok += int(rep["converged"])
+        used += int(how == "network")
+    return {
+        "problem": "steady 1D viscous Burgers layer, 256 elements, P3, 1024 unknowns, Dirichlet uL/uR",
+        "network": "3 -> 256 -> 256 -> 1024, tanh, FP32, device GEMMs on matrix cores, Adam",
+        "train_solves": int(ntrain), "train_converged": int(ok.sum()), "generate_seconds": round(t_gen, 2),
+        "train_seconds": round(t_train, 2), "epochs": epochs,
+        "loss_first_last": [float(f"{hist[0]:.4e}"), float(f"{hist[-1]:.4e}")],
+        "test_problems": int(ntest),
+        "plain": {"converged": plain_ok, "mean_newton_steps": round(float(np.mean(plain_its)), 3),
+                  "mean_seconds": round(float(np.mean(plain_s)), 5)},
+        "warm": {"converged": warm_ok, "network_guess_used": used,
+                 "mean_newton_steps": round(float(np.mean(warm_its)), 3),
+                 "mean_seconds": round(float(np.mean(warm_s)), 5)},
+        "time_ratio_warm_over_plain": round(float(np.mean(warm_s) / np.mean(plain_s)), 4),
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -158,6 +216,9 @@ def main():
 
     if args.workload == "euler3d":
         print(json.dumps(euler3d(args)))
+        return
+    if args.workload == "warmstart":
+        print(json.dumps(warmstart_leg(args.warmstart_train)))
         return
 
     mesh, order, params, coords = workloads.ns_mms_2d(args.n, args.order)
@@ -318,6 +379,8 @@ def main():
         del disc, space
         torch.cuda.empty_cache()
         out["euler3d"] = euler3d(args)
+    if not args.no_warmstart:
+        out["warmstart"] = warmstart_leg(args.warmstart_train)
     if not args.no_cpu_baseline:
         n, el, size = cpu_baseline(mesh, order, params, u_host, args.cpu_seconds)
         out["cpu_baseline"] = {

@@ -177,6 +177,7 @@ PREC_FP32 = 1
 PC_BLOCK_JACOBI = 0
 PC_BLOCK_SGS = 1
 PC_MULTIGRID = 2
+PC_LINE = 3
 FORCING_FIXED = 0
 FORCING_EW = 1
 NEWTON_MAX_HIST = 64
@@ -207,3 +208,15 @@ class NewtonReport(C.Structure):
                     jacobian_seconds=self.jacobian_seconds, linear_seconds=self.linear_seconds,
                     linesearch_seconds=self.linesearch_seconds, rhist=list(self.rhist[:k]),
                     alpha=list(self.alpha[:k]), linear_its=list(self.linear_its[:k]))
+
+
+class MlpTrainOpts(C.Structure):
+    _fields_ = [("batch", C.c_int), ("epochs", C.c_int), ("lr", C.c_double), ("lr_decay", C.c_double),
+                ("beta1", C.c_double), ("beta2", C.c_double), ("eps", C.c_double), ("seed", C.c_ulonglong)]
+
+
+def mlp_train_opts(batch=256, epochs=100, lr=1e-3, lr_decay=1.0, beta1=0.9, beta2=0.999, eps=1e-8, seed=0):
+    o = MlpTrainOpts()
+    o.batch, o.epochs, o.lr, o.lr_decay = batch, epochs, lr, lr_decay
+    o.beta1, o.beta2, o.eps, o.seed = beta1, beta2, eps, seed
+    return o

@@ -514,6 +514,38 @@ int ice_space_qp_coords(const ice_space* s, double* xq) {
     return ICE_OK;
 }
 
+// Flux parameters of a discretization (reference nondimensionalisation,
+// navier_stokes.hpp:129-213; DDG coefficients, conservation_law.hpp:518-527).
+static void derive_phys(const ice_disc_params* p, int order, ice::PhysParams& ph) {
+    std::memset(&ph, 0, sizeof(ph));
+    ph.mu = p->mu;
+    for (int k = 0; k < 3; ++k) { ph.a[k] = p->a[k]; ph.b[k] = p->b[k]; }
+    ph.gamma = p->gamma;
+    ph.Pr = p->Pr;
+    const double* r = p->ref;  // t, l, rho, u, e, p, T, mu
+    ph.Re = r[2] * r[3] * r[1] / r[7];
+    ph.Eu = r[5] / (r[2] * r[3] * r[3]);
+    ph.e_coeff = r[3] * r[3] / r[4];
+    const double cp = p->gamma / (p->gamma - 1) * p->Rgas;
+    ph.T_coeff = cp * r[6] * r[2] / r[5];
+    ph.visc_kind = p->viscosity_kind;
+    ph.visc_mu = p->visc_mu;
+    ph.mu_ratio = 1.716e-5 / r[7];
+    ph.T0r = 273.1 / r[6];
+    ph.Tsr = 110.5 / r[6];
+    for (int k = 0; k < ICE_MAX_BCFLAG; ++k) ph.isoT[k] = p->isothermal_T[k];
+    ph.fs_rho = p->freestream[0];
+    ph.fs_u = p->freestream[1];
+    for (int k = 0; k < 3; ++k) ph.fs_dir[k] = p->freestream[2 + k];
+    ph.fs_T = p->freestream[5];
+    ph.textbook_energy = p->ns_face_energy_textbook;
+    ph.sigma_ic = p->sigma_ic;
+    ph.beta0 = (double)(order + 1) * (order + 1);
+    ph.beta1 = p->interior_penalty ? 0.0 : 1.0 / std::max((double)(2 * order * (order + 1)), 1.0);
+    ph.beta0_bdy = ph.beta0;
+    ice::set_derived(ph);
+}
+
 int ice_disc_create(ice_space* s, const ice_disc_params* p, ice_disc** out) {
     if (!s || !p || !out) return fail(ICE_ERR_ARG, "null argument");
     *out = nullptr;
@@ -543,35 +575,7 @@ int ice_disc_create(ice_space* s, const ice_disc_params* p, ice_disc** out) {
         d->space = s;
         d->prm = *p;
         d->neq = ns ? D + 2 : 1;
-        ice::PhysParams& ph = d->ph;
-        std::memset(&ph, 0, sizeof(ph));
-        ph.mu = p->mu;
-        for (int k = 0; k < 3; ++k) { ph.a[k] = p->a[k]; ph.b[k] = p->b[k]; }
-        ph.gamma = p->gamma;
-        ph.Pr = p->Pr;
-        const double* r = p->ref;  // t, l, rho, u, e, p, T, mu
-        ph.Re = r[2] * r[3] * r[1] / r[7];
-        ph.Eu = r[5] / (r[2] * r[3] * r[3]);
-        ph.e_coeff = r[3] * r[3] / r[4];
-        const double cp = p->gamma / (p->gamma - 1) * p->Rgas;
-        ph.T_coeff = cp * r[6] * r[2] / r[5];
-        ph.visc_kind = p->viscosity_kind;
-        ph.visc_mu = p->visc_mu;
-        ph.mu_ratio = 1.716e-5 / r[7];
-        ph.T0r = 273.1 / r[6];
-        ph.Tsr = 110.5 / r[6];
-        for (int k = 0; k < ICE_MAX_BCFLAG; ++k) ph.isoT[k] = p->isothermal_T[k];
-        ph.fs_rho = p->freestream[0];
-        ph.fs_u = p->freestream[1];
-        for (int k = 0; k < 3; ++k) ph.fs_dir[k] = p->freestream[2 + k];
-        ph.fs_T = p->freestream[5];
-        ph.textbook_energy = p->ns_face_energy_textbook;
-        ph.sigma_ic = p->sigma_ic;
-        const int order = s->p;
-        ph.beta0 = (double)(order + 1) * (order + 1);
-        ph.beta1 = p->interior_penalty ? 0.0 : 1.0 / std::max((double)(2 * order * (order + 1)), 1.0);
-        ph.beta0_bdy = ph.beta0;
-        ice::set_derived(ph);
+        derive_phys(p, s->p, d->ph);
         d->ddgic = (!ns) && p->sigma_ic != 0.0;
         d->has_source = p->source.kind != ICE_FIELD_NONE;
         std::vector<ice_field> fields(2 * ICE_MAX_BCFLAG + 1);
@@ -613,6 +617,33 @@ int ice_disc_create(ice_space* s, const ice_disc_params* p, ice_disc** out) {
 int ice_disc_destroy(ice_disc* d) {
     delete d;
     return ICE_OK;
+}
+
+int ice_disc_update(ice_disc* d, const ice_disc_params* p) {
+    if (!d || !p) return fail(ICE_ERR_ARG, "null argument");
+    const ice_space* s = d->space;
+    if (p->flux != d->prm.flux) return fail(ICE_ERR_ARG, "the flux family of a discretization is fixed");
+    const bool ddgic = p->flux == ICE_FLUX_BURGERS && p->sigma_ic != 0.0;
+    const int has_source = p->source.kind != ICE_FIELD_NONE;
+    if (ddgic != d->ddgic || has_source != d->has_source)
+        return fail(ICE_ERR_ARG, "DDGIC and the source term must stay on or off as at creation");
+    try {
+        derive_phys(p, s->p, d->ph);
+        d->prm = *p;
+        std::vector<ice_field> fields(2 * ICE_MAX_BCFLAG + 1);
+        for (int k = 0; k < ICE_MAX_BCFLAG; ++k) {
+            fields[k] = p->dirichlet[k];
+            fields[ICE_MAX_BCFLAG + k] = p->neumann[k];
+        }
+        fields[2 * ICE_MAX_BCFLAG] = p->source;
+        ICE_HIP(hipMemcpy(d->d_fields, fields.data(), fields.size() * sizeof(ice_field), hipMemcpyHostToDevice));
+        if (ice::setup_disc_tables(s->ndim, s->p, d->neq, s->dev, d->d_fields, d->d_bval, d->d_srcload, nullptr) != 0)
+            throw ice::Error{ICE_ERR_HIP, "table setup launch failed"};
+        ICE_HIP(hipDeviceSynchronize());
+        return ICE_OK;
+    } catch (const ice::Error& e) {
+        return fail(e.code, e.msg);
+    }
 }
 
 int ice_residual(ice_disc* d, const double* u, double* r, void* stream) {

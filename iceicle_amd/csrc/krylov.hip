@@ -106,23 +106,15 @@ template int bsr_apply<float, double>(int, int, int, const int*, const float*, c
 
 namespace {
 
-template <class T>
-__global__ __launch_bounds__(256) void block_inverse_kernel(int nelem, int nslots, int bs, const double* __restrict__ J,
-                                                            T* __restrict__ Dinv, int* __restrict__ nsingular,
-                                                            int transposed) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    double* A = reinterpret_cast<double*>(smem_raw);  // bs * bs
-    int* perm = reinterpret_cast<int*>(A + (size_t)bs * bs);
-    __shared__ int s_piv;
-    __shared__ int s_bad;
-    const int e = blockIdx.x;
+// In-place inverse of the bs x bs matrix A in LDS by Gauss-Jordan elimination
+// with partial pivoting (largest |A[i][k]|, lowest row on ties), by the whole
+// workgroup; perm (LDS, bs ints) records the row interchanges.  Returns
+// false, uniformly across the workgroup, when a pivot is zero.
+__device__ bool gj_inverse_lds(double* A, int* perm, int bs, int* s_piv, int* s_bad) {
     const int tid = threadIdx.x;
-    const double* src = J + (size_t)e * nslots * bs * bs;
-    for (int k = tid; k < bs * bs; k += blockDim.x) A[k] = src[k];
-    if (tid == 0) s_bad = 0;
+    if (tid == 0) *s_bad = 0;
     __syncthreads();
     for (int k = 0; k < bs; ++k) {
-        // pivot search by the first wave: largest |A[i][k]|, i >= k, lowest index on ties
         if (tid < 64) {
             double best = -1.0;
             int bi = k;
@@ -136,14 +128,14 @@ __global__ __launch_bounds__(256) void block_inverse_kernel(int nelem, int nslot
                 if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
             }
             if (tid == 0) {
-                s_piv = bi;
+                *s_piv = bi;
                 perm[k] = bi;
-                if (!(best > 0.0)) s_bad = 1;
+                if (!(best > 0.0)) *s_bad = 1;
             }
         }
         __syncthreads();
-        if (s_bad) break;
-        const int pr = s_piv;
+        if (*s_bad) return false;
+        const int pr = *s_piv;
         if (pr != k)
             for (int c = tid; c < bs; c += blockDim.x) {
                 const double t = A[(size_t)k * bs + c];
@@ -161,22 +153,13 @@ __global__ __launch_bounds__(256) void block_inverse_kernel(int nelem, int nslot
         // eliminate column k from every other row
         for (int idx = tid; idx < bs * bs; idx += blockDim.x) {
             const int i = idx / bs, c = idx % bs;
-            if (i == k) continue;
-            const double f = A[(size_t)i * bs + k];
-            const double pk = A[(size_t)k * bs + c];
-            if (c == k) continue;
-            A[(size_t)i * bs + c] -= f * pk;
+            if (i == k || c == k) continue;
+            A[(size_t)i * bs + c] -= A[(size_t)i * bs + k] * A[(size_t)k * bs + c];
         }
         __syncthreads();
         for (int i = tid; i < bs; i += blockDim.x)
             if (i != k) A[(size_t)i * bs + k] = -A[(size_t)i * bs + k] * A[(size_t)k * bs + k];
         __syncthreads();
-    }
-    T* dst = Dinv + (size_t)e * bs * bs;
-    if (s_bad) {
-        for (int k = tid; k < bs * bs; k += blockDim.x) dst[k] = (T)((k / bs == k % bs) ? 1.0 : 0.0);
-        if (tid == 0) atomicAdd(nsingular, 1);
-        return;
     }
     // undo the row interchanges as column interchanges, last first
     for (int k = bs - 1; k >= 0; --k) {
@@ -189,10 +172,127 @@ __global__ __launch_bounds__(256) void block_inverse_kernel(int nelem, int nslot
             }
         __syncthreads();
     }
+    return true;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void block_inverse_kernel(int nelem, int nslots, int bs, const double* __restrict__ J,
+                                                            T* __restrict__ Dinv, int* __restrict__ nsingular,
+                                                            int transposed) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* A = reinterpret_cast<double*>(smem_raw);  // bs * bs
+    int* perm = reinterpret_cast<int*>(A + (size_t)bs * bs);
+    __shared__ int s_piv;
+    __shared__ int s_bad;
+    const int e = blockIdx.x;
+    const int tid = threadIdx.x;
+    const double* src = J + (size_t)e * nslots * bs * bs;
+    for (int k = tid; k < bs * bs; k += blockDim.x) A[k] = src[k];
+    __syncthreads();
+    const bool ok = gj_inverse_lds(A, perm, bs, &s_piv, &s_bad);
+    T* dst = Dinv + (size_t)e * bs * bs;
+    if (!ok) {
+        for (int k = tid; k < bs * bs; k += blockDim.x) dst[k] = (T)((k / bs == k % bs) ? 1.0 : 0.0);
+        if (tid == 0) atomicAdd(nsingular, 1);
+        return;
+    }
     if (transposed) {
         for (int k = tid; k < bs * bs; k += blockDim.x) dst[k] = (T)A[(size_t)(k % bs) * bs + k / bs];
     } else {
         for (int k = tid; k < bs * bs; k += blockDim.x) dst[k] = (T)A[k];
+    }
+}
+
+// Block LU of a block-tridiagonal matrix (a chain of elements e = 0..n-1,
+// slot 1 = element e-1, slot 2 = element e+1), one workgroup walking the
+// chain: S_0 = D_0, S_e = D_e - L_e W_{e-1}, W_e = S_e^-1 U_e, with
+// L_e = J[e][1], U_e = J[e][2].  Stores S_e^-1 and W_e.  A singular S_e is
+// replaced by the identity and counted.
+__global__ __launch_bounds__(256) void line_factor_kernel(int nelem, int nslots, int bs, const double* __restrict__ J,
+                                                          double* __restrict__ Sinv, double* __restrict__ W,
+                                                          int* __restrict__ nsingular) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* A = reinterpret_cast<double*>(smem_raw);
+    double* Wp = A + (size_t)bs * bs;
+    int* perm = reinterpret_cast<int*>(Wp + (size_t)bs * bs);
+    __shared__ int s_piv, s_bad;
+    const int tid = threadIdx.x;
+    const size_t bb = (size_t)bs * bs;
+    for (int e = 0; e < nelem; ++e) {
+        const double* Je = J + (size_t)e * nslots * bb;
+        for (int idx = tid; idx < bs * bs; idx += blockDim.x) {
+            const int i = idx / bs, c = idx % bs;
+            double sacc = Je[idx];
+            if (e > 0) {
+                const double* L = Je + bb;
+                for (int k = 0; k < bs; ++k) sacc -= L[(size_t)i * bs + k] * Wp[(size_t)k * bs + c];
+            }
+            A[idx] = sacc;
+        }
+        __syncthreads();
+        if (!gj_inverse_lds(A, perm, bs, &s_piv, &s_bad)) {
+            for (int idx = tid; idx < bs * bs; idx += blockDim.x) A[idx] = (idx / bs == idx % bs) ? 1.0 : 0.0;
+            if (tid == 0) atomicAdd(nsingular, 1);
+            __syncthreads();
+        }
+        for (int idx = tid; idx < bs * bs; idx += blockDim.x) Sinv[e * bb + idx] = A[idx];
+        if (e + 1 < nelem) {
+            const double* Uu = Je + 2 * bb;
+            for (int idx = tid; idx < bs * bs; idx += blockDim.x) {
+                const int i = idx / bs, c = idx % bs;
+                double sacc = 0.0;
+                for (int k = 0; k < bs; ++k) sacc += A[(size_t)i * bs + k] * Uu[(size_t)k * bs + c];
+                Wp[idx] = sacc;
+                W[e * bb + idx] = sacc;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// x = (L U)^-1 b with the factors of line_factor_kernel: forward
+// y_e = S_e^-1 (b_e - L_e y_{e-1}), backward x_e = y_e - W_e x_{e+1}.
+// One wave; each lane owns rows i, i + 64, ... of the current block.
+__global__ __launch_bounds__(64) void line_solve_kernel(int nelem, int nslots, int bs, const double* __restrict__ J,
+                                                        const double* __restrict__ Sinv, const double* __restrict__ W,
+                                                        const double* __restrict__ b, double* __restrict__ x) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* t = reinterpret_cast<double*>(smem_raw);
+    double* yp = t + bs;
+    const int tid = threadIdx.x;
+    const size_t bb = (size_t)bs * bs;
+    for (int e = 0; e < nelem; ++e) {
+        const double* L = J + ((size_t)e * nslots + 1) * bb;
+        for (int i = tid; i < bs; i += 64) {
+            double sacc = b[(size_t)e * bs + i];
+            if (e > 0)
+                for (int k = 0; k < bs; ++k) sacc -= L[(size_t)i * bs + k] * yp[k];
+            t[i] = sacc;
+        }
+        __syncthreads();
+        const double* Si = Sinv + e * bb;
+        for (int i = tid; i < bs; i += 64) {
+            double sacc = 0.0;
+            for (int k = 0; k < bs; ++k) sacc += Si[(size_t)i * bs + k] * t[k];
+            x[(size_t)e * bs + i] = sacc;
+            t[bs + i] = sacc;  // yp
+        }
+        __syncthreads();
+    }
+    // yp holds x_{n-1} = y_{n-1}
+    for (int e = nelem - 2; e >= 0; --e) {
+        const double* We = W + e * bb;
+        for (int i = tid; i < bs; i += 64) {
+            double sacc = x[(size_t)e * bs + i];
+            for (int k = 0; k < bs; ++k) sacc -= We[(size_t)i * bs + k] * yp[k];
+            t[i] = sacc;
+        }
+        __syncthreads();
+        for (int i = tid; i < bs; i += 64) {
+            x[(size_t)e * bs + i] = t[i];
+            yp[i] = t[i];
+        }
+        __syncthreads();
     }
 }
 
@@ -246,6 +346,21 @@ int block_inverse(int nelem, int nslots, int bs, const double* J, T* Dinv, int* 
 }
 template int block_inverse<float>(int, int, int, const double*, float*, int*, bool, hipStream_t);
 template int block_inverse<double>(int, int, int, const double*, double*, int*, bool, hipStream_t);
+
+int line_factor(int nelem, int nslots, int bs, const double* J, double* Sinv, double* W, int* nsingular,
+                hipStream_t st) {
+    const size_t sh = 2 * (size_t)bs * bs * sizeof(double) + (size_t)bs * sizeof(int);
+    if (sh > 160 * 1024 || nslots < 3) return -1;
+    line_factor_kernel<<<1, 256, sh, st>>>(nelem, nslots, bs, J, Sinv, W, nsingular);
+    return (int)hipGetLastError();
+}
+
+int line_solve(int nelem, int nslots, int bs, const double* J, const double* Sinv, const double* W, const double* b,
+               double* x, hipStream_t st) {
+    if (nslots < 3) return -1;
+    line_solve_kernel<<<1, 64, 2 * (size_t)bs * sizeof(double), st>>>(nelem, nslots, bs, J, Sinv, W, b, x);
+    return (int)hipGetLastError();
+}
 
 template <class T, class V>
 int block_diag_apply(int nelem, int bs, const T* D, const V* x, V* y, hipStream_t st) {
@@ -377,9 +492,11 @@ __global__ void scale_kernel(size_t n, const double* __restrict__ s, int inv, do
 }
 
 // y = a x + b y
+// y = a x + b y; a zero coefficient means the operand is not read (BLAS
+// convention), so (0, y, 0, y) clears y whatever it held, NaN included
 __global__ void axpby_kernel2(size_t n, double a, const double* __restrict__ x, double b, double* __restrict__ y) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        y[i] = a * x[i] + (b == 0.0 ? 0.0 : b * y[i]);
+        y[i] = (a == 0.0 ? 0.0 : a * x[i]) + (b == 0.0 ? 0.0 : b * y[i]);
 }
 
 // Arnoldi bookkeeping of step k (one thread): column k of the Hessenberg

@@ -35,6 +35,10 @@ template <class T, class V>
 int sgsT_sweep(bool forward, int nel, const int* elems, const int* ecolor, int colour, int nslots, int bs,
                const int* cols, const T* JT, const T* DT, const V* rhs, V* y, hipStream_t st);
 int convert_transpose_fp32(size_t nblocks, int bs, const double* a, float* b, hipStream_t st);
+int line_factor(int nelem, int nslots, int bs, const double* J, double* Sinv, double* W, int* nsingular,
+                hipStream_t st);
+int line_solve(int nelem, int nslots, int bs, const double* J, const double* Sinv, const double* W, const double* b,
+               double* x, hipStream_t st);
 int mg_galerkin_p0(int nelem, int nslots, int nb, int m, const double* J, double* A0, hipStream_t st);
 int mg_galerkin_agg(int ncoarse, int nslots, int m, const int* children, const int* fcols, const int* parent,
                     const double* Af, double* Ac, hipStream_t st);
@@ -205,6 +209,8 @@ struct ice_newton {
     double* sc = nullptr;       // Hessenberg, rotations, g, norms
     double* y = nullptr;        // m
     int* nsing = nullptr;
+    double* line_Sinv = nullptr;  // block-tridiagonal factors (line preconditioner)
+    double* line_W = nullptr;
     std::vector<MGLevel> mg;    // multigrid levels (level 0 = element constants)
     double *mg_r = nullptr, *mg_t = nullptr;  // fine-level work vectors of the V-cycle
     int neq = 0, nbasis = 0;
@@ -267,6 +273,10 @@ struct ice_newton {
     }
 
     void apply_M(const double* v, double* out, hipStream_t st) {
+        if (opt.preconditioner == ICE_PC_LINE) {
+            NT_LAUNCH(ice::line_solve(nelem, nslots, bs, J64, line_Sinv, line_W, v, out, st));
+            return;
+        }
         if (opt.preconditioner == ICE_PC_MULTIGRID) {
             apply_mg(v, out, st);
             return;
@@ -438,6 +448,12 @@ struct ice_newton {
 
     void factor(hipStream_t st) {
         NT_HIP(hipMemsetAsync(nsing, 0, sizeof(int), st));
+        if (opt.preconditioner == ICE_PC_LINE) {
+            if (fp32()) NT_LAUNCH(ice::convert_transpose_fp32((size_t)nelem * nslots, bs, J64, J32, st));
+            NT_LAUNCH(ice::line_factor(nelem, nslots, bs, J64, line_Sinv, line_W, nsing, st));
+            factored = true;
+            return;
+        }
         if (fp32()) {
             NT_LAUNCH(ice::convert_transpose_fp32((size_t)nelem * nslots, bs, J64, J32, st));
             NT_LAUNCH(ice::block_inverse<float>(nelem, nslots, bs, J64, (float*)Dinv, nsing, true, st));
@@ -582,7 +598,7 @@ int ice_newton_create(ice_disc* d, const ice_newton_opts* opts, ice_newton** out
         (o.jacobian_precision != ICE_PREC_FP32 && o.jacobian_precision != ICE_PREC_FP64) ||
         (o.linesearch < ICE_LS_NONE || o.linesearch > ICE_LS_CORRIGAN) ||
         (o.preconditioner != ICE_PC_BLOCK_JACOBI && o.preconditioner != ICE_PC_BLOCK_SGS &&
-         o.preconditioner != ICE_PC_MULTIGRID) ||
+         o.preconditioner != ICE_PC_MULTIGRID && o.preconditioner != ICE_PC_LINE) ||
         (o.forcing != ICE_FORCING_FIXED && o.forcing != ICE_FORCING_EW)) {
         ice::set_error("invalid Newton options");
         return ICE_ERR_ARG;
@@ -612,7 +628,23 @@ int ice_newton_create(ice_disc* d, const ice_newton_opts* opts, ice_newton** out
         s->sc = s->alloc<double>((size_t)(s->m + 1) * s->m + 2 * s->m + (s->m + 1) + 2);
         s->y = s->alloc<double>(s->m);
         s->nsing = s->alloc<int>(1);
-        if (o.preconditioner != ICE_PC_BLOCK_JACOBI) s->colour();
+        if (o.preconditioner == ICE_PC_LINE) {
+            // the elements must form one open chain in index order: slot 1
+            // (face 0) is element e - 1 and slot 2 (face 1) element e + 1
+            std::vector<int> hc((size_t)s->nelem * s->nslots);
+            NT_HIP(hipMemcpy(hc.data(), ice::disc_jcols(d), hc.size() * sizeof(int), hipMemcpyDeviceToHost));
+            bool chain = s->nslots == 3;
+            for (int e = 0; chain && e < s->nelem; ++e) {
+                const int l = hc[(size_t)e * 3 + 1], r = hc[(size_t)e * 3 + 2];
+                chain = (l == (e > 0 ? e - 1 : -1)) && (r == (e + 1 < s->nelem ? e + 1 : -1));
+            }
+            if (!chain)
+                throw Fail{ICE_ERR_UNSUPPORTED, "the line preconditioner needs a non-periodic 1D mesh"};
+            s->line_Sinv = s->alloc<double>((size_t)s->nelem * s->bs * s->bs);
+            s->line_W = s->alloc<double>((size_t)s->nelem * s->bs * s->bs);
+        } else if (o.preconditioner != ICE_PC_BLOCK_JACOBI) {
+            s->colour();
+        }
         if (o.preconditioner == ICE_PC_MULTIGRID) {
             if (!s->fp32()) throw Fail{ICE_ERR_UNSUPPORTED, "the multigrid preconditioner runs with FP32 blocks"};
             s->build_hierarchy();

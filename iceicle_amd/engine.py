@@ -36,6 +36,7 @@ _SIGS = {
     "ice_disc_destroy": ([P], C.c_int),
     "ice_residual": ([P, P, P, P], C.c_int),
     "ice_residual_mixed": ([P, P, P, P], C.c_int),
+    "ice_disc_update": ([P, C.POINTER(abi.DiscParams)], C.c_int),
     "ice_profile_begin": ([P, C.c_int], C.c_int),
     "ice_profile_end": ([P, C.POINTER(C.c_double), C.POINTER(C.c_int)], C.c_int),
     "ice_field_eval_points": ([C.POINTER(abi.Field), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
@@ -59,6 +60,14 @@ _SIGS = {
     "ice_newton_destroy": ([P], C.c_int),
     "ice_newton_solve": ([P, P, C.POINTER(abi.NewtonReport), P], C.c_int),
     "ice_newton_linear_solve": ([P, P, P, P, C.POINTER(C.c_int), C.POINTER(C.c_double), P], C.c_int),
+    "ice_mlp_create": ([C.c_int, C.POINTER(C.c_int), C.c_ulonglong, C.POINTER(P)], C.c_int),
+    "ice_mlp_destroy": ([P], C.c_int),
+    "ice_mlp_num_params": ([P, C.POINTER(C.c_longlong)], C.c_int),
+    "ice_mlp_get_params": ([P, P], C.c_int),
+    "ice_mlp_set_params": ([P, P], C.c_int),
+    "ice_mlp_forward": ([P, P, C.c_int, P, P], C.c_int),
+    "ice_mlp_gradient": ([P, P, P, C.c_int, C.POINTER(C.c_double), P, P], C.c_int),
+    "ice_mlp_train": ([P, P, P, C.c_int, C.POINTER(abi.MlpTrainOpts), P, P], C.c_int),
 }
 
 
@@ -156,9 +165,14 @@ class FESpace:
         return field_eval(field, self.ndim, neq, xn).ravel()
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().ice_space_destroy(self.h)
-            self.h = None
+        # at interpreter exit the module may be torn down first; the driver
+        # releases device memory with the process then
+        try:
+            if getattr(self, "h", None):
+                lib().ice_space_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
 
 
 class ConservationLawDDG:
@@ -174,6 +188,12 @@ class ConservationLawDDG:
         self.neq = 1 if params.flux == abi.FLUX_BURGERS else space.ndim + 2
         self.size = space.nelem * space.nbasis * self.neq
 
+    def update(self, params):
+        """Replace the parameters in place (same flux family; DDGIC and the
+        source term stay on or off as at creation)."""
+        check(lib().ice_disc_update(self.h, C.byref(params)))
+        self.params = params
+
     def profile_begin(self, max_calls):
         check(lib().ice_profile_begin(self.h, max_calls))
 
@@ -186,9 +206,14 @@ class ConservationLawDDG:
         return [ms[0], ms[1], ms[2]], n.value
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().ice_disc_destroy(self.h)
-            self.h = None
+        # at interpreter exit the module may be torn down first; the driver
+        # releases device memory with the process then
+        try:
+            if getattr(self, "h", None):
+                lib().ice_disc_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
 
 
 def current_stream():
@@ -261,9 +286,14 @@ class ExplicitStepper:
         return dict(time=t.value, dt=dt.value, itime=it.value, done=bool(done.value))
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().ice_stepper_destroy(self.h)
-            self.h = None
+        # at interpreter exit the module may be torn down first; the driver
+        # releases device memory with the process then
+        try:
+            if getattr(self, "h", None):
+                lib().ice_stepper_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
 
 
 class BlockJacobian:
@@ -341,6 +371,86 @@ class PetscNewton:
         return its.value, rel.value
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().ice_newton_destroy(self.h)
-            self.h = None
+        # at interpreter exit the module may be torn down first; the driver
+        # releases device memory with the process then
+        try:
+            if getattr(self, "h", None):
+                lib().ice_newton_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def _require_f32(t, name, cols=None):
+    import torch
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()):
+        raise IceError(f"{name} must be a contiguous float32 CUDA tensor")
+    if cols is not None and (t.dim() != 2 or t.shape[1] != cols):
+        raise IceError(f"{name} must have shape [batch, {cols}]")
+
+
+class MLP:
+    """Fully connected tanh network trained and evaluated by the library's
+    device kernels (warm-start predictor; no reference counterpart)."""
+
+    def __init__(self, widths, seed=0):
+        self.widths = [int(w) for w in widths]
+        arr = (C.c_int * len(self.widths))(*self.widths)
+        h = P()
+        check(lib().ice_mlp_create(len(self.widths) - 1, arr, seed, C.byref(h)))
+        self.h = h
+        n = C.c_longlong()
+        check(lib().ice_mlp_num_params(self.h, C.byref(n)))
+        self.nparams = n.value
+
+    def get_params(self):
+        out = np.empty(self.nparams, dtype=np.float32)
+        check(lib().ice_mlp_get_params(self.h, out.ctypes.data_as(P)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.nparams:
+            raise IceError(f"expected {self.nparams} parameters")
+        check(lib().ice_mlp_set_params(self.h, p.ctypes.data_as(P)))
+
+    def forward(self, x, y=None, stream=None):
+        import torch
+        _require_f32(x, "x", self.widths[0])
+        if y is None:
+            y = torch.empty((x.shape[0], self.widths[-1]), dtype=torch.float32, device=x.device)
+        _require_f32(y, "y", self.widths[-1])
+        check(lib().ice_mlp_forward(self.h, _ptr(x), x.shape[0], _ptr(y), stream or current_stream()))
+        return y
+
+    def gradient(self, x, t, stream=None):
+        """(loss, gradient as a device float32 vector in the packed layout)."""
+        import torch
+        _require_f32(x, "x", self.widths[0])
+        _require_f32(t, "t", self.widths[-1])
+        g = torch.empty(self.nparams, dtype=torch.float32, device=x.device)
+        loss = C.c_double()
+        check(lib().ice_mlp_gradient(self.h, _ptr(x), _ptr(t), x.shape[0], C.byref(loss), _ptr(g),
+                                     stream or current_stream()))
+        return loss.value, g
+
+    def train(self, x, t, opts, stream=None):
+        """Adam training; returns the per-epoch mean loss."""
+        _require_f32(x, "x", self.widths[0])
+        _require_f32(t, "t", self.widths[-1])
+        if x.shape[0] != t.shape[0]:
+            raise IceError("x and t must have the same number of rows")
+        hist = np.zeros(max(opts.epochs, 1))
+        check(lib().ice_mlp_train(self.h, _ptr(x), _ptr(t), x.shape[0], C.byref(opts),
+                                  hist.ctypes.data_as(P), stream or current_stream()))
+        return hist[:opts.epochs]
+
+    def __del__(self):
+        # at interpreter exit the module may be torn down first; the driver
+        # releases device memory with the process then
+        try:
+            if getattr(self, "h", None):
+                lib().ice_mlp_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass

@@ -209,6 +209,15 @@ int ice_space_qp_coords(const ice_space* space, double* xq);
 int ice_disc_create(ice_space* space, const ice_disc_params* params, ice_disc** out);
 int ice_disc_destroy(ice_disc* disc);
 
+/* Replace the parameters of an existing discretization in place (flux
+ * coefficients, boundary data, DDG options) and rebuild its boundary-value
+ * and source tables, so that many problems that differ only in parameters
+ * reuse one space, one discretization and one solver.  The flux family and
+ * whether DDGIC and the source term are on must stay as at creation
+ * (ICE_ERR_ARG otherwise).  No reference counterpart: the reference builds a
+ * new ConservationLawDDG per problem (conservation_law.cpp:287-346). */
+int ice_disc_update(ice_disc* disc, const ice_disc_params* params);
+
 /* form_residual (form_residual.hpp:56-268): res <- R(u), FP64, device pointers
  * in fe_layout_right order, both nelem*nbasis*neq doubles. */
 int ice_residual(ice_disc* disc, const double* u_dev, double* res_dev, void* stream);
@@ -285,6 +294,7 @@ int ice_jacobian_apply(const ice_disc* disc, const double* blocks_dev, const dou
 #define ICE_PC_BLOCK_JACOBI 0
 #define ICE_PC_BLOCK_SGS 1
 #define ICE_PC_MULTIGRID 2
+#define ICE_PC_LINE 3
 #define ICE_FORCING_FIXED 0
 #define ICE_FORCING_EW 1
 #define ICE_NEWTON_MAX_HIST 64
@@ -307,7 +317,9 @@ typedef struct {
     int preconditioner;       /* ICE_PC_*: block Jacobi, multicolour block
                                  symmetric Gauss-Seidel (default), or a
                                  multigrid V-cycle over element agglomerates
-                                 (P0 Galerkin coarse levels, SGS smoothing) */
+                                 (P0 Galerkin coarse levels, SGS smoothing), or
+                                 the exact block-tridiagonal (line) solve of a
+                                 non-periodic 1D mesh (FP64 blocks) */
     int forcing;              /* ICE_FORCING_*: linear tolerance linear_rtol at
                                  every step (default, as the reference's KSP),
                                  or Eisenstat-Walker choice 2 */
@@ -396,6 +408,44 @@ int ice_space_node_coords(const ice_space* space, double* xn);
  * `face_nr` seen with orientation `orientation` (TraceSpace transform_xiL /
  * transform_xiR, hypercube_face.hpp:230-252): xi[nfqp][ndim], host-only. */
 int ice_trace_points(int ndim, int order, int face_nr, int orientation, double* xi);
+
+/* ---- warm-start network (no reference counterpart) ----------------------
+ * A fully connected MLP, tanh on hidden layers and identity on the output,
+ * trained and evaluated on the device by this library's own kernels
+ * (FP32 matrix-core GEMMs).  It predicts DG coefficients of a steady solution
+ * from problem parameters, as a Newton initial guess; Newton's FP64 residual
+ * decides whether the guess is used.  Arrays are FP32, row-major:
+ * inputs [batch][widths[0]], outputs [batch][widths[nlayers]]; parameters are
+ * packed layer by layer as W_l [widths[l]][widths[l+1]] then b_l.  Loss: mean
+ * squared error over batch x outputs.  A handle is not thread-safe. */
+typedef struct ice_mlp ice_mlp;
+typedef struct {
+    int batch;         /* minibatch rows (rows beyond the last full batch of an epoch are skipped) */
+    int epochs;
+    double lr;         /* Adam step, multiplied by lr_decay after every epoch */
+    double lr_decay;
+    double beta1, beta2, eps;
+    unsigned long long seed;  /* minibatch shuffling */
+} ice_mlp_train_opts;
+
+/* Glorot-uniform weights (seeded), zero biases.  widths has nlayers+1 entries. */
+int ice_mlp_create(int nlayers, const int* widths, unsigned long long seed, ice_mlp** out);
+int ice_mlp_destroy(ice_mlp* mlp);
+int ice_mlp_num_params(ice_mlp* mlp, long long* n);
+/* host copies of the packed parameters; set also resets the Adam moments */
+int ice_mlp_get_params(ice_mlp* mlp, float* host);
+int ice_mlp_set_params(ice_mlp* mlp, const float* host);
+/* y_dev <- network(x_dev), device pointers */
+int ice_mlp_forward(ice_mlp* mlp, const float* x_dev, int batch, float* y_dev, void* stream);
+/* loss (host, may be null) and the gradient of the loss in the packed
+ * parameter layout (device, may be null) at targets t_dev; synchronous when
+ * loss is requested */
+int ice_mlp_gradient(ice_mlp* mlp, const float* x_dev, const float* t_dev, int batch, double* loss, float* grad_dev,
+                     void* stream);
+/* Adam training over n rows x_dev/t_dev (device); loss_hist (host, may be
+ * null) receives the mean minibatch loss of each epoch.  Synchronous. */
+int ice_mlp_train(ice_mlp* mlp, const float* x_dev, const float* t_dev, int n, const ice_mlp_train_opts* opts,
+                  double* loss_hist, void* stream);
 
 #ifdef __cplusplus
 }

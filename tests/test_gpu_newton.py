@@ -91,7 +91,7 @@ def test_linear_solve_accuracy():
                           (abi.PREC_FP32, abi.PC_MULTIGRID, 1e-10)]:
         solver = engine.PetscNewton(disc, jacobian_precision=prec, linear_rtol=tol, gmres_restart=40,
                                     preconditioner=pc)
-        x = torch.empty_like(u)
+        x = torch.full_like(u, float("nan"))  # old contents of the output must not leak in
         its, rel = solver.linear_solve(b, x, u=u)
         # the residual the solver reports is the FP64 one
         r = torch.empty_like(u)
