@@ -43,11 +43,12 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-newton", action="store_true", help="skip the Newton time-to-solution leg")
-    ap.add_argument("--workload", default="ns2d", choices=["ns2d", "euler3d", "warmstart"],
+    ap.add_argument("--workload", default="ns2d", choices=["ns2d", "euler3d", "warmstart", "batch"],
                     help="ns2d: the headline 2D NS MMS workload (with the 3D Euler leg appended); "
                          "euler3d: only the 3D Euler vortex 64^3 P3 leg")
     ap.add_argument("--no-euler3d", action="store_true", help="skip the 3D Euler vortex leg")
     ap.add_argument("--no-warmstart", action="store_true", help="skip the learned warm-start leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances leg")
     ap.add_argument("--warmstart-train", type=int, default=4000, help="GPU-generated training solves")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="only run 20 device copies of an MB-sized buffer (HBM counter calibration)")
@@ -114,7 +115,7 @@ def euler3d(args, n=64, order=3, nres=30, nsteps=50):
     # initial maximum wave speed |v| + c ~ 2.9 + 1.1)
     h = 40.0 / n
     dt = 0.2 * h / (2 * order + 1) / 4.0
-    stepper = engine.ExplicitStepper(disc, abi.stepper_desc(abi.STEP_RK3_SSP, dt=dt, ntime=0))
+    stepper = engine.ExplicitStepper(disc, abi.stepper_desc(abi.STEP_RK3_SSP, dt=dt, ntime=1 << 40))
     stepper.step(u)
     torch.cuda.synchronize()
     start.record()
@@ -196,6 +197,72 @@ def warmstart_leg(ntrain, ntest=100, epochs=300):
     }
 
 
+def batch_leg(ninst=512, reps=20, nsteps=10):
+    """512 independent 2D viscous Burgers instances (N ~ U{16..96}, p ~
+    U{1..4}, nu log-uniform on [1e-3, 1e-1], seeded sinusoidal ICs), one
+    batched space per order with per-element viscosity and CFL step: FP64 and
+    mixed residual throughput over the whole batch, RK3-SSP steps eager and
+    as one captured HIP graph, and the same residuals evaluated instance by
+    instance (one space per instance) for comparison."""
+    import torch
+    from iceicle_amd import batch as B, engine, workloads
+    print(f"batched: {ninst} instances ...", file=sys.stderr, flush=True)
+    insts = B.sample_instances(ninst, seed=workloads.SEED0 + 3)
+    t0 = time.perf_counter()
+    bt = B.Batch(insts)
+    t_setup = time.perf_counter() - t0
+    tiles = bt.autotune(reps=5)
+    outs = [torch.empty_like(g.u) for g in bt.groups]
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn, n):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        ev0.record()
+        for _ in range(n):
+            fn()
+        ev1.record()
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / n
+
+    ms64 = timed(lambda: bt.residual(outs), reps)
+    r64 = [o.clone() for o in outs]
+    ms32 = timed(lambda: bt.residual(outs, precision="mixed"), reps)
+    err32 = max(float((o - r).abs().max() / r.abs().max()) for o, r in zip(outs, r64))
+    # per-instance evaluation of the same residuals: one space per instance
+    singles = []
+    for inst in insts:
+        sp = engine.FESpace(B.mesh(inst["N"]), inst["p"])
+        dc = engine.ConservationLawDDG(sp, B.params(inst["nu"]))
+        uu = torch.zeros(dc.size, dtype=torch.float64, device="cuda")
+        singles.append((sp, dc, uu, torch.empty_like(uu)))
+    for k, (sp, dc, uu, rr) in enumerate(singles):
+        uu.copy_(bt.instance_state(k))
+    ms_single = timed(lambda: [engine.form_residual(sp, dc, uu, rr) for sp, dc, uu, rr in singles], 3)
+    same = all(torch.equal(rr, outs_k) for (sp, dc, uu, rr), outs_k in
+               zip(singles, [r64[bt.index[k][0]][bt.groups[bt.index[k][0]].instance_slice(bt.index[k][1])]
+                             for k in range(ninst)]))
+    del singles
+    # RK3-SSP steps of the whole batch: eager launches, then one graph
+    ms_eager = timed(bt.step, nsteps)
+    graph = bt.graph(nsteps)
+    ms_graph = timed(graph.launch, 3) / nsteps
+    finite = all(bool(torch.isfinite(g.u).all()) for g in bt.groups)
+    return {
+        "instances": ninst, "unknowns": int(bt.size), "groups": {g.order: len(g.insts) for g in bt.groups},
+        "setup_seconds": round(t_setup, 2),
+        "tiles": {str(k): {"best": v[0], "ms": [round(x, 4) for x in v[1]]} for k, v in tiles.items()},
+        "residual_fp64": {"ms": round(ms64, 4), "GDOF_s": round(bt.size / (ms64 * 1e-3) / 1e9, 3)},
+        "residual_mixed": {"ms": round(ms32, 4), "GDOF_s": round(bt.size / (ms32 * 1e-3) / 1e9, 3),
+                           "rel_err_vs_fp64": float(f"{err32:.3e}")},
+        "residual_per_instance": {"ms": round(ms_single, 3), "GDOF_s": round(bt.size / (ms_single * 1e-3) / 1e9, 3),
+                                  "bit_identical_to_batched": bool(same)},
+        "rk3_ssp_step_ms": {"eager": round(ms_eager, 4), "graph": round(ms_graph, 4), "graph_steps": nsteps},
+        "state_finite": finite,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -219,6 +286,9 @@ def main():
         return
     if args.workload == "warmstart":
         print(json.dumps(warmstart_leg(args.warmstart_train)))
+        return
+    if args.workload == "batch":
+        print(json.dumps(batch_leg()))
         return
 
     mesh, order, params, coords = workloads.ns_mms_2d(args.n, args.order)
@@ -381,6 +451,8 @@ def main():
         out["euler3d"] = euler3d(args)
     if not args.no_warmstart:
         out["warmstart"] = warmstart_leg(args.warmstart_train)
+    if not args.no_batch:
+        out["batched"] = batch_leg()
     if not args.no_cpu_baseline:
         n, el, size = cpu_baseline(mesh, order, params, u_host, args.cpu_seconds)
         out["cpu_baseline"] = {

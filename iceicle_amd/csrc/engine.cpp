@@ -113,6 +113,7 @@ struct ice_space {
     ice::HostMesh mesh;
     int ndim = 0, p = 0, n = 0, nb = 0, nq = 0, nfq = 0;
     bool coupled = false;
+    int batch = 1;                    // meshes concatenated into this space
     std::vector<double> tab;          // host copy of the S tables
     std::vector<int4> fint, fbdy;
     std::vector<int> efaces;
@@ -145,12 +146,13 @@ struct ice_disc {
     double* d_srcload = nullptr;
     unsigned long long* d_wave = nullptr;
     double* d_wave_part = nullptr;
+    double* d_emu = nullptr;  // per-element viscosity (batched instances)
     int wave_users = 0;       // CFL steppers bound to this discretization
     ice::DiscDev dd{};
     ~ice_disc() {
         for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
         for (void* p : {(void*)d_fields, (void*)d_Q, (void*)d_Pic, (void*)d_bval, (void*)d_srcload, (void*)d_wave,
-                        (void*)d_wave_part, (void*)d_T})
+                        (void*)d_wave_part, (void*)d_T, (void*)d_emu})
             if (p) (void)hipFree(p);
     }
 };
@@ -179,7 +181,8 @@ int disc_dims(const ice_disc* d, int* nelem, int* nslots, int* bs) {
     *nelem = s->mesh.nelem;
     *nslots = 2 * s->ndim + 1;
     *bs = s->nb * d->neq;
-    return (s->ndim > 2 || d->ddgic) ? -1 : 0;
+    // the FD Jacobian kernels use the discretization's viscosity, not per-element ones
+    return (s->ndim > 2 || d->ddgic || d->d_emu) ? -1 : 0;
 }
 
 const int* disc_jcols(const ice_disc* d) { return d->space->dev.jcols; }
@@ -253,17 +256,97 @@ int ice_mesh_export(const ice_mesh_desc* d, ice_mesh_counts* counts, double* coo
     }
 }
 
-int ice_space_create(const ice_mesh_desc* md, const ice_space_desc* sd, ice_space** out) {
-    if (!md || !sd || !out) return fail(ICE_ERR_ARG, "null argument");
+// Interior (and coupled periodic pair) faces and boundary faces of one
+// structured mesh as device records, element ids local to the mesh.
+static void face_lists(const ice::HostMesh& m, bool coupled, std::vector<int4>& fint, std::vector<int4>& fbdy) {
+    const int D = m.ndim;
+    for (int f = 0; f < m.n_interior; ++f) {
+        const auto& fc = m.faces[f];
+        fint.push_back(int4{fc.elemL, fc.elemR, ice::pack_face_info(fc.fnl, fc.fnr, fc.orient, ICE_BC_INTERIOR, 0), 0});
+    }
+    if (coupled) {
+        for (int f = m.n_interior; f < (int)m.faces.size(); ++f) {
+            const auto& fa = m.faces[f];
+            if (fa.bctype != ICE_BC_PERIODIC || fa.fnl >= D) continue;
+            const auto& fb = m.faces[fa.bcflag];
+            const int idim = fa.fnl;
+            const int nfv = (D > 1) ? (1 << (D - 1)) : 1;
+            int va[4];
+            for (int k = 0; k < nfv; ++k) {
+                // partner of A's node on B's side: same ordinates, last index along idim
+                int r = fa.nodes[k], ijk[3] = {0, 0, 0}, stride = 1, idx = 0;
+                for (int d = 0; d < D; ++d) { ijk[d] = r % m.nnd[d]; r /= m.nnd[d]; }
+                ijk[idim] = m.nnd[idim] - 1;
+                for (int d = 0; d < D; ++d) { idx += ijk[d] * stride; stride *= m.nnd[d]; }
+                va[k] = idx;
+            }
+            const int orient = ice::face_orientation(D, fb.nodes, va);
+            if (orient < 0) throw ice::Error{ICE_ERR_ARG, "periodic faces do not match"};
+            fint.push_back(int4{fb.elemL, fa.elemL, ice::pack_face_info(fb.fnl, fa.fnl, orient, ICE_BC_INTERIOR, idim + 1), 0});
+        }
+    }
+    for (int f = m.n_interior; f < (int)m.faces.size(); ++f) {
+        const auto& fc = m.faces[f];
+        if (coupled && fc.bctype == ICE_BC_PERIODIC) continue;
+        fbdy.push_back(int4{fc.elemL, ice::pack_face_info(fc.fnl, 0, 0, fc.bctype, 0), fc.bcflag, 0});
+    }
+}
+
+// Several meshes of the same dimension and periodic mode concatenated into
+// one space (elements of mesh k after those of meshes 0..k-1); one mesh is
+// the ordinary space.
+static int space_create(const ice_mesh_desc* mds, int nmesh, const ice_space_desc* sd, ice_space** out) {
+    if (!mds || !sd || !out || nmesh < 1) return fail(ICE_ERR_ARG, "null argument");
     *out = nullptr;
     if (sd->order < 1 || sd->order > ICE_MAX_ORDER) return fail(ICE_ERR_UNSUPPORTED, "order must be 1..ICE_MAX_ORDER");
     if (sd->geometry_order != 1) return fail(ICE_ERR_UNSUPPORTED, "only geometry order 1 is supported");
-    if (md->periodic_mode != ICE_PERIODIC_REFERENCE && md->periodic_mode != ICE_PERIODIC_COUPLED)
-        return fail(ICE_ERR_ARG, "bad periodic_mode");
+    for (int k = 0; k < nmesh; ++k) {
+        if (mds[k].periodic_mode != ICE_PERIODIC_REFERENCE && mds[k].periodic_mode != ICE_PERIODIC_COUPLED)
+            return fail(ICE_ERR_ARG, "bad periodic_mode");
+        if (mds[k].periodic_mode != mds[0].periodic_mode || mds[k].ndim != mds[0].ndim)
+            return fail(ICE_ERR_ARG, "batched meshes must share ndim and periodic_mode");
+    }
     try {
         auto s = std::make_unique<ice_space>();
-        s->coupled = md->periodic_mode == ICE_PERIODIC_COUPLED;
-        s->mesh = ice::build_mesh(*md, s->coupled);
+        s->coupled = mds[0].periodic_mode == ICE_PERIODIC_COUPLED;
+        std::vector<ice::HostMesh> meshes;
+        for (int k = 0; k < nmesh; ++k) meshes.push_back(ice::build_mesh(mds[k], s->coupled));
+        if (nmesh == 1) {
+            s->mesh = meshes[0];
+        } else {
+            // merged host mesh: element, node and face ids offset; the
+            // structured dimensions are left 0 (no single grid)
+            ice::HostMesh& mm = s->mesh;
+            mm.ndim = meshes[0].ndim;
+            for (int d = 0; d < 3; ++d) {
+                mm.nel[d] = 0;
+                mm.nnd[d] = 0;
+                mm.period[d] = meshes[0].period[d];
+            }
+            int noff = 0, eoff = 0, foff = 0;
+            for (const auto& mk : meshes) {
+                for (int d = 0; d < mm.ndim; ++d)
+                    if (s->coupled && std::abs(mk.period[d] - mm.period[d]) > 1e-14 * std::max(1.0, std::abs(mm.period[d])))
+                        throw ice::Error{ICE_ERR_ARG, "coupled periodic batched meshes must share the box extents"};
+                mm.coord.insert(mm.coord.end(), mk.coord.begin(), mk.coord.end());
+                for (int c : mk.conn) mm.conn.push_back(c + noff);
+                for (auto f : mk.faces) {
+                    f.elemL += eoff;
+                    if (f.elemR >= 0) f.elemR += eoff;
+                    if (f.bctype == ICE_BC_PERIODIC) f.bcflag += foff;
+                    for (int v = 0; v < ice::MAX_NFV; ++v) f.nodes[v] += noff;
+                    mm.faces.push_back(f);
+                }
+                mm.n_interior += mk.n_interior;
+                mm.n_pairs += mk.n_pairs;
+                noff += mk.nnode;
+                eoff += mk.nelem;
+                foff += (int)mk.faces.size();
+            }
+            mm.nelem = eoff;
+            mm.nnode = noff;
+        }
+        s->batch = nmesh;
         const ice::HostMesh& m = s->mesh;
         const int D = m.ndim, p = sd->order, n = p + 1;
         s->ndim = D;
@@ -362,38 +445,22 @@ int ice_space_create(const ice_mesh_desc* md, const ice_space_desc* sd, ice_spac
             ice::face_vertex_order(D, j, v);
             for (int k = 0; k < ((D > 1) ? (1 << (D - 1)) : 1); ++k) s->fvo[j * 4 + k] = (signed char)v[k];
         }
-        // device face lists
-        s->efaces.assign((size_t)m.nelem * 2 * D, -1);
-        for (int f = 0; f < m.n_interior; ++f) {
-            const auto& fc = m.faces[f];
-            s->fint.push_back(int4{fc.elemL, fc.elemR, ice::pack_face_info(fc.fnl, fc.fnr, fc.orient, ICE_BC_INTERIOR, 0), 0});
-        }
-        if (s->coupled) {
-            for (int f = m.n_interior; f < (int)m.faces.size(); ++f) {
-                const auto& fa = m.faces[f];
-                if (fa.bctype != ICE_BC_PERIODIC || fa.fnl >= D) continue;
-                const auto& fb = m.faces[fa.bcflag];
-                const int idim = fa.fnl;
-                const int nfv = (D > 1) ? (1 << (D - 1)) : 1;
-                int va[4];
-                for (int k = 0; k < nfv; ++k) {
-                    // partner of A's node on B's side: same ordinates, last index along idim
-                    int r = fa.nodes[k], ijk[3] = {0, 0, 0}, stride = 1, idx = 0;
-                    for (int d = 0; d < D; ++d) { ijk[d] = r % m.nnd[d]; r /= m.nnd[d]; }
-                    ijk[idim] = m.nnd[idim] - 1;
-                    for (int d = 0; d < D; ++d) { idx += ijk[d] * stride; stride *= m.nnd[d]; }
-                    va[k] = idx;
-                }
-                const int orient = ice::face_orientation(D, fb.nodes, va);
-                if (orient < 0) throw ice::Error{ICE_ERR_ARG, "periodic faces do not match"};
-                s->fint.push_back(int4{fb.elemL, fa.elemL, ice::pack_face_info(fb.fnl, fa.fnl, orient, ICE_BC_INTERIOR, idim + 1), 0});
+        // device face lists: every mesh's interior faces (and coupled
+        // periodic pairs), then every mesh's boundary faces, element ids
+        // offset by the elements of the meshes before it
+        {
+            int eoff = 0;
+            std::vector<std::vector<int4>> bd(meshes.size());
+            for (size_t k = 0; k < meshes.size(); ++k) {
+                std::vector<int4> fi;
+                face_lists(meshes[k], s->coupled, fi, bd[k]);
+                for (int4 r : fi) s->fint.push_back(int4{r.x + eoff, r.y + eoff, r.z, r.w});
+                for (int4& r : bd[k]) r.x += eoff;
+                eoff += meshes[k].nelem;
             }
+            for (const auto& b : bd) s->fbdy.insert(s->fbdy.end(), b.begin(), b.end());
         }
-        for (int f = m.n_interior; f < (int)m.faces.size(); ++f) {
-            const auto& fc = m.faces[f];
-            if (s->coupled && fc.bctype == ICE_BC_PERIODIC) continue;
-            s->fbdy.push_back(int4{fc.elemL, ice::pack_face_info(fc.fnl, 0, 0, fc.bctype, 0), fc.bcflag, 0});
-        }
+        s->efaces.assign((size_t)m.nelem * 2 * D, -1);
         const int nfi = (int)s->fint.size(), nfb = (int)s->fbdy.size();
         for (int f = 0; f < nfi; ++f) {
             const int4 r = s->fint[f];
@@ -471,6 +538,14 @@ int ice_space_create(const ice_mesh_desc* md, const ice_space_desc* sd, ice_spac
     } catch (const std::exception& e) {
         return fail(ICE_ERR_ALLOC, e.what());
     }
+}
+
+int ice_space_create(const ice_mesh_desc* md, const ice_space_desc* sd, ice_space** out) {
+    return space_create(md, 1, sd, out);
+}
+
+int ice_space_create_batch(const ice_mesh_desc* mds, int nmesh, const ice_space_desc* sd, ice_space** out) {
+    return space_create(mds, nmesh, sd, out);
 }
 
 int ice_space_destroy(ice_space* s) {
@@ -644,6 +719,76 @@ int ice_disc_update(ice_disc* d, const ice_disc_params* p) {
     } catch (const ice::Error& e) {
         return fail(e.code, e.msg);
     }
+}
+
+int ice_disc_set_element_viscosity(ice_disc* d, const double* mu) {
+    if (!d) return fail(ICE_ERR_ARG, "null argument");
+    if (d->prm.flux != ICE_FLUX_BURGERS) return fail(ICE_ERR_UNSUPPORTED, "per-element viscosity is a Burgers option");
+    if (d->wave_users > 0) return fail(ICE_ERR_STATE, "a CFL stepper uses this discretization's viscosity");
+    const size_t ne = (size_t)d->space->mesh.nelem;
+    try {
+        if (!mu) {
+            if (d->d_emu) ICE_HIP(hipFree(d->d_emu));
+            d->d_emu = nullptr;
+        } else {
+            if (!d->d_emu) ICE_HIP(hipMalloc(&d->d_emu, std::max<size_t>(ne, 1) * sizeof(double)));
+            ICE_HIP(hipMemcpy(d->d_emu, mu, ne * sizeof(double), hipMemcpyHostToDevice));
+        }
+        d->dd.emu = d->d_emu;
+        return ICE_OK;
+    } catch (const ice::Error& e) {
+        return fail(e.code, e.msg);
+    }
+}
+
+int ice_disc_set_tile(ice_disc* d, int tile) {
+    if (!d) return fail(ICE_ERR_ARG, "null argument");
+    if (tile < 0 || tile > 2) return fail(ICE_ERR_ARG, "tile must be 0, 1 or 2");
+    d->dd.tile = tile;
+    return ICE_OK;
+}
+
+int ice_disc_autotune(ice_disc* d, const double* u, double* r, int reps, int* best, double* ms, void* stream) {
+    if (!d || !u || !r || reps < 1) return fail(ICE_ERR_ARG, "bad argument");
+    const ice_space* s = d->space;
+    if (s->ndim < 2) {  // one element kernel shape in 1D
+        if (best) *best = 0;
+        return ICE_OK;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return fail(ICE_ERR_HIP, "event create failed");
+    int pick = 0;
+    double tbest = 1e300;
+    int rc = ICE_OK;
+    for (int tile = 0; tile < 3 && rc == ICE_OK; ++tile) {
+        d->dd.tile = tile;
+        float t = 0.f;
+        bool ok = true;
+        for (int k = 0; k < 2 && ok; ++k)
+            ok = ice::residual_fp64(s->ndim, s->p, d->prm.flux, s->dev, d->ph, d->dd, u, r, st, nullptr) == 0;
+        ok = ok && hipEventRecord(e0, st) == hipSuccess;
+        for (int k = 0; k < reps && ok; ++k)
+            ok = ice::residual_fp64(s->ndim, s->p, d->prm.flux, s->dev, d->ph, d->dd, u, r, st, nullptr) == 0;
+        ok = ok && hipEventRecord(e1, st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+             hipEventElapsedTime(&t, e0, e1) == hipSuccess;
+        if (!ok) {
+            rc = fail(ICE_ERR_HIP, "autotune timing failed");
+            break;
+        }
+        const double per = (double)t / reps;
+        if (ms) ms[tile] = per;
+        if (per < tbest) {
+            tbest = per;
+            pick = tile;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    d->dd.tile = pick;
+    if (best) *best = pick;
+    return rc;
 }
 
 int ice_residual(ice_disc* d, const double* u, double* r, void* stream) {
@@ -855,11 +1000,13 @@ struct ice_stepper {
     size_t n = 0;
     double* buf[5] = {};      // res, k1, k2, k3, ustage / uold
     ice::StepState* st = nullptr;
+    double* dte = nullptr;    // per-element step sizes (batched instances), or null
     ~ice_stepper() {
         if (tracks_wave && disc && --disc->wave_users == 0) disc->dd.wave = nullptr;
         for (double* b : buf)
             if (b) (void)hipFree(b);
         if (st) (void)hipFree(st);
+        if (dte) (void)hipFree(dte);
     }
 };
 
@@ -871,6 +1018,7 @@ int step_once(ice_stepper* S, double* u, hipStream_t st) {
     const size_t n = S->n;
     double *res = S->buf[0], *k1 = S->buf[1], *k2 = S->buf[2], *k3 = S->buf[3], *us = S->buf[4];
     const double* dt = &S->st->dt;
+    const size_t per = (size_t)sp->nb * d->neq;
     int rc = 0;
     auto R = [&](const double* x) {
         rc |= ice::residual_fp64(sp->ndim, sp->p, d->prm.flux, sp->dev, d->ph, d->dd, x, res, st, nullptr);
@@ -879,6 +1027,11 @@ int step_once(ice_stepper* S, double* u, hipStream_t st) {
     auto copy = [&](const double* x, double* y) {
         rc |= (int)hipMemcpyAsync(y, x, n * sizeof(double), hipMemcpyDeviceToDevice, st);
     };
+    // y += c dt x with the stepper's step size, or each element's own
+    auto axpy = [&](double c, const double* x, double* y) {
+        if (S->dte) rc |= ice::launch_axpy_edt(n, c, S->dte, per, S->st, x, y, st);
+        else rc |= ice::launch_axpy_dt(n, c, dt, x, y, st);
+    };
     rc |= ice::launch_step_begin(S->st, d->d_wave, S->ctl, st);
     switch (S->desc.kind) {
     case ICE_STEP_RK3_SSP:
@@ -886,24 +1039,24 @@ int step_once(ice_stepper* S, double* u, hipStream_t st) {
             // ssp_rk3.hpp:132-190 with every stage residual evaluated at u^n
             R(u);
             Minv(k1);
-            rc |= ice::launch_axpy_dt(n, 1.0 / 6.0, dt, k1, u, st);
-            rc |= ice::launch_axpy_dt(n, 1.0 / 6.0, dt, k1, u, st);
-            rc |= ice::launch_axpy_dt(n, 2.0 / 3.0, dt, k1, u, st);
+            axpy(1.0 / 6.0, k1, u);
+            axpy(1.0 / 6.0, k1, u);
+            axpy(2.0 / 3.0, k1, u);
         } else {
             R(u);
             Minv(k1);
             copy(u, us);
-            rc |= ice::launch_axpy_dt(n, 1.0, dt, k1, us, st);
+            axpy(1.0, k1, us);
             R(us);
             Minv(k2);
             copy(u, us);
-            rc |= ice::launch_axpy_dt(n, 0.25, dt, k1, us, st);
-            rc |= ice::launch_axpy_dt(n, 0.25, dt, k2, us, st);
+            axpy(0.25, k1, us);
+            axpy(0.25, k2, us);
             R(us);
             Minv(k3);
-            rc |= ice::launch_axpy_dt(n, 1.0 / 6.0, dt, k1, u, st);
-            rc |= ice::launch_axpy_dt(n, 1.0 / 6.0, dt, k2, u, st);
-            rc |= ice::launch_axpy_dt(n, 2.0 / 3.0, dt, k3, u, st);
+            axpy(1.0 / 6.0, k1, u);
+            axpy(1.0 / 6.0, k2, u);
+            axpy(2.0 / 3.0, k3, u);
         }
         break;
     case ICE_STEP_RK3_TVD: {
@@ -915,14 +1068,14 @@ int step_once(ice_stepper* S, double* u, hipStream_t st) {
             R(u);
             Minv(k1);
             rc |= ice::launch_axpby(n, alfa[s], us, beta[s], u, st);
-            rc |= ice::launch_axpy_dt(n, beta[s], dt, k1, u, st);
+            axpy(beta[s], k1, u);
         }
         break;
     }
     default:  // ExplicitEuler (explicit_euler.hpp:87-138)
         R(u);
         Minv(k1);
-        rc |= ice::launch_axpy_dt(n, 1.0, dt, k1, u, st);
+        axpy(1.0, k1, u);
         break;
     }
     rc |= ice::launch_step_end(S->st, st);
@@ -938,6 +1091,7 @@ int ice_stepper_create(ice_disc* d, const ice_stepper_desc* desc, ice_stepper** 
     *out = nullptr;
     if (desc->kind < ICE_STEP_RK3_SSP || desc->kind > ICE_STEP_EXPLICIT_EULER) return fail(ICE_ERR_ARG, "bad stepper kind");
     if (!desc->use_cfl && !(desc->dt > 0)) return fail(ICE_ERR_ARG, "fixed time step must be positive");
+    if (desc->use_cfl && d->d_emu) return fail(ICE_ERR_UNSUPPORTED, "CFL steps with per-element viscosity");
     try {
         auto S = std::make_unique<ice_stepper>();
         S->disc = d;
@@ -990,6 +1144,80 @@ int ice_stepper_create(ice_disc* d, const ice_stepper_desc* desc, ice_stepper** 
     } catch (const ice::Error& e) {
         return fail(e.code, e.msg);
     }
+}
+
+int ice_stepper_set_element_dt(ice_stepper* s, const double* dt) {
+    if (!s) return fail(ICE_ERR_ARG, "null argument");
+    if (s->ctl.use_cfl) return fail(ICE_ERR_STATE, "a CFL stepper chooses its own step");
+    const size_t ne = (size_t)s->disc->space->mesh.nelem;
+    try {
+        if (!dt) {
+            if (s->dte) ICE_HIP(hipFree(s->dte));
+            s->dte = nullptr;
+            return ICE_OK;
+        }
+        for (size_t e = 0; e < ne; ++e)
+            if (!(dt[e] > 0)) return fail(ICE_ERR_ARG, "element step sizes must be positive");
+        if (!s->dte) ICE_HIP(hipMalloc(&s->dte, std::max<size_t>(ne, 1) * sizeof(double)));
+        ICE_HIP(hipMemcpy(s->dte, dt, ne * sizeof(double), hipMemcpyHostToDevice));
+        return ICE_OK;
+    } catch (const ice::Error& e) {
+        return fail(e.code, e.msg);
+    }
+}
+
+// ---- step graphs -------------------------------------------------------
+struct ice_step_graph_impl {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int ice_step_graph_create(ice_stepper* const* steppers, double* const* u, int n, int nsteps, ice_step_graph** out) {
+    if (!steppers || !u || !out || n < 1 || nsteps < 1) return fail(ICE_ERR_ARG, "bad argument");
+    *out = nullptr;
+    for (int k = 0; k < n; ++k) {
+        if (!steppers[k] || !u[k]) return fail(ICE_ERR_ARG, "null stepper or state");
+        if (steppers[k]->ctl.use_cfl) return fail(ICE_ERR_UNSUPPORTED, "graph capture needs fixed step sizes");
+        if (steppers[k]->disc->dd.wave) return fail(ICE_ERR_UNSUPPORTED, "graph capture needs fixed step sizes");
+    }
+    hipStream_t cs = nullptr;
+    if (hipStreamCreate(&cs) != hipSuccess) return fail(ICE_ERR_HIP, "stream create failed");
+    auto g = std::make_unique<ice_step_graph_impl>();
+    int rc = 0;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipStreamDestroy(cs);
+        return fail(ICE_ERR_HIP, "stream capture failed to start");
+    }
+    for (int t = 0; t < nsteps; ++t)
+        for (int k = 0; k < n; ++k) rc |= step_once(steppers[k], u[k], cs);
+    const hipError_t ec = hipStreamEndCapture(cs, &g->graph);
+    (void)hipStreamDestroy(cs);
+    if (rc != 0 || ec != hipSuccess || !g->graph) {
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        return fail(ICE_ERR_HIP, "step capture failed");
+    }
+    if (hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(g->graph);
+        return fail(ICE_ERR_HIP, "graph instantiation failed");
+    }
+    *out = reinterpret_cast<ice_step_graph*>(g.release());
+    return ICE_OK;
+}
+
+int ice_step_graph_launch(ice_step_graph* g, void* stream) {
+    if (!g) return fail(ICE_ERR_ARG, "null argument");
+    auto* G = reinterpret_cast<ice_step_graph_impl*>(g);
+    if (hipGraphLaunch(G->exec, (hipStream_t)stream) != hipSuccess) return fail(ICE_ERR_HIP, "graph launch failed");
+    return ICE_OK;
+}
+
+int ice_step_graph_destroy(ice_step_graph* g) {
+    if (!g) return ICE_OK;
+    auto* G = reinterpret_cast<ice_step_graph_impl*>(g);
+    if (G->exec) (void)hipGraphExecDestroy(G->exec);
+    if (G->graph) (void)hipGraphDestroy(G->graph);
+    delete G;
+    return ICE_OK;
 }
 
 int ice_stepper_destroy(ice_stepper* s) {

@@ -45,6 +45,8 @@ struct DiscDev {
     const ice_field* fields;  // [dirichlet x4 | neumann x4 | source]
     const double* bval;       // [nfb][nfq][neq] Dirichlet / Neumann values at boundary points, or null
     const double* srcload;    // [nelem][nbasis][neq] source load vector, or null
+    const double* emu;        // [nelem] per-element viscosity (batched Burgers instances), or null
+    int tile;                 // element-kernel workgroup: 0 = 256 threads, 1 = 128, 2 = 64 (whole elements)
     double* Q;                // [nfi + nfb][nfq][neq] weighted normal face fluxes
     double* Pic;              // [nfi + nfb][nfq][neq][ndim] DDGIC lifting vectors, or null
     double* T;                // [nelem][2*ndim][nfq][nv][neq] element face traces
@@ -82,6 +84,8 @@ int l2_error_terms(int ndim, int p, int neq, const DevSpace& S, const ice_field*
 
 int launch_minv(int ndim, int p, int neq, const DevSpace& S, const double* r, double* k, hipStream_t st);
 int launch_axpy_dt(size_t n, double coef, const double* dtp, const double* x, double* y, hipStream_t st);
+int launch_axpy_edt(size_t n, double coef, const double* dte, size_t per, const StepState* stp, const double* x,
+                    double* y, hipStream_t st);
 int launch_axpby(size_t n, double a, const double* x, double b, double* y, hipStream_t st);
 int launch_step_begin(StepState* s, unsigned long long* wave, const StepControl& c, hipStream_t st);
 int launch_step_end(StepState* s, hipStream_t st);

@@ -366,6 +366,7 @@ struct ice_newton {
     void build_hierarchy() {
         int nel[3], ne = 0, nbs = 0;
         ice::disc_grid(disc, nel, &ne, &nbs);
+        if (nel[0] <= 0) throw Fail{ICE_ERR_UNSUPPORTED, "the multigrid preconditioner needs a single structured mesh"};
         neq = ne;
         nbasis = nbs;
         const int D = (nslots - 1) / 2;

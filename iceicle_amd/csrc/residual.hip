@@ -74,6 +74,15 @@ __device__ __forceinline__ const double* tab_of<double>(const DevSpace& S) { ret
 template <>
 __device__ __forceinline__ const float* tab_of<float>(const DevSpace& S) { return S.tabf; }
 
+// Flux parameters of element e: a per-element viscosity (batched instances)
+// replaces the discretization's when given
+template <class P>
+__device__ __forceinline__ P local_params(const P& p, const double* __restrict__ emu, int e) {
+    P q = p;
+    if (emu) q.mu = (decltype(q.mu))emu[e];
+    return q;
+}
+
 template <class PH, int D>
 struct TraceRec {
     // Euler needs only values: its diffusive flux is identically zero
@@ -382,7 +391,7 @@ __global__ __launch_bounds__(256, ICE_TRACE_MINW) void trace_kernel(DevSpace S, 
 // Numerical fluxes at face points from stored traces: interior points first
 // (blocks [0, nbi)), then boundary points.
 template <int D, int P, class PH>
-__global__ __launch_bounds__(256, ICE_FLUX_MINW) void flux_kernel(DevSpace S, typename PH::Params ph, DiscDev dd,
+__global__ __launch_bounds__(256, ICE_FLUX_MINW) void flux_kernel(DevSpace S, typename PH::Params ph0, DiscDev dd,
                                                                    const typename PH::real* __restrict__ T, int nbi) {
     using VT = typename PH::real;
     VT* const Qb = reinterpret_cast<VT*>(dd.Q);
@@ -397,6 +406,7 @@ __global__ __launch_bounds__(256, ICE_FLUX_MINW) void flux_kernel(DevSpace S, ty
         if (f >= S.nfi) return;
         const int4 rec = S.fint[f];
         const int eL = rec.x, eR = rec.y, info = rec.z;
+        const typename PH::Params ph = local_params(ph0, dd.emu, eL);
         const int fnl = info_fnl(info), fnr = info_fnr(info), orient = info_orient(info);
         const int qr = S.fperm[(fnr * S.norient + orient) * NFQ + q];
         const VT* tL = T + (((size_t)eL * 2 * D + fnl) * NFQ + q) * NV * M;
@@ -453,6 +463,7 @@ __global__ __launch_bounds__(256, ICE_FLUX_MINW) void flux_kernel(DevSpace S, ty
         if (b >= S.nfb) return;
         const int4 rec = S.fbdy[b];
         const int eL = rec.x, info = rec.y, bcflag = rec.z;
+        const typename PH::Params ph = local_params(ph0, dd.emu, eL);
         const int fnl = info_fnl(info), bctype = info_bctype(info);
         const size_t slot = (size_t)S.nfi + b;
         VT* Qf = Qb + (slot * NFQ + q) * M;
@@ -580,16 +591,22 @@ size_t trace_buffer_size(int ndim, int p, int flux, int nelem) {
 }
 
 // ---- elements: volume integral + face lifting -----------------------------
-template <int D, int P>
+// Whole elements per workgroup for a target of TPB threads, rounded up to
+// full waves
+template <int D, int P, int TPB = 256>
 struct ElemTile {
     static constexpr int N = P + 1;
     static constexpr int NB = Pow<N, D>::value;
-    static constexpr int EPB = (256 / NB) > 0 ? (256 / NB) : 1;
+    static constexpr int EPB = (TPB / NB) > 0 ? (TPB / NB) : 1;
     static constexpr int NT = EPB * NB;
+    static constexpr int BLOCK = ((NT + 63) / 64) * 64;
 };
+// workgroup size targets selected by DiscDev::tile
+constexpr int tile_tpb(int tile) { return tile == 1 ? 128 : tile == 2 ? 64 : 256; }
 
 template <int D, int P, class PH>
-__global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph, const double* __restrict__ srcload,
+__global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph0, const double* __restrict__ emu,
+                                                      const double* __restrict__ srcload,
                                                       const double* __restrict__ U, const double* __restrict__ Q,
                                                       const double* __restrict__ Pic, double* __restrict__ R,
                                                       unsigned long long* __restrict__ wave,
@@ -605,6 +622,7 @@ __global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph,
     const int k = threadIdx.x - le * NB;
     const int e = e0 + le;
     const bool active = (threadIdx.x < T::NT) && (e < S.nelem);
+    const PhysParams ph = local_params(ph0, emu, active ? e : 0);
     {
         const size_t base = (size_t)e0 * NB * M;
         const size_t total = (size_t)S.nelem * NB * M;
@@ -849,8 +867,9 @@ __global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph,
 // and the trace_integral lifting), 2N instead of N^2 multiply-adds per
 // point and basis function.  The lifting skips faces on which the basis
 // function's node does not lie (its trace there is zero).
-template <int P, class PH>
-__global__ __launch_bounds__(256, ICE_ELEM_MINW) void element_kernel_2d(DevSpace S, typename PH::Params ph,
+template <int P, class PH, int TPB>
+__global__ __launch_bounds__((ElemTile<2, P, TPB>::BLOCK), ICE_ELEM_MINW) void element_kernel_2d(DevSpace S, typename PH::Params ph0,
+                                                                        const double* __restrict__ emu,
                                                                         const double* __restrict__ srcload,
                                                                         const double* __restrict__ U,
                                                                         const typename PH::real* __restrict__ Q,
@@ -859,7 +878,7 @@ __global__ __launch_bounds__(256, ICE_ELEM_MINW) void element_kernel_2d(DevSpace
                                                          unsigned long long* __restrict__ wave,
                                                          double* __restrict__ wave_part) {
     constexpr int D = 2;
-    using TL = ElemTile<D, P>;
+    using TL = ElemTile<D, P, TPB>;
     constexpr int N = TL::N, NB = TL::NB, EPB = TL::EPB;
     constexpr int NFQ = N;
     constexpr int M = PH::M;
@@ -876,6 +895,7 @@ __global__ __launch_bounds__(256, ICE_ELEM_MINW) void element_kernel_2d(DevSpace
     const int k0 = k / N, k1 = k - (k / N) * N;
     const int e = e0 + le;
     const bool active = (threadIdx.x < TL::NT) && (e < S.nelem);
+    const typename PH::Params ph = local_params(ph0, emu, active ? e : 0);
     {
         const size_t base = (size_t)e0 * NB * M;
         const size_t total = (size_t)S.nelem * NB * M;
@@ -1128,16 +1148,11 @@ __global__ __launch_bounds__(256, ICE_ELEM_MINW) void element_kernel_2d(DevSpace
 // barrier; the face fluxes, whose addresses come from the face slots, are
 // loaded into registers before phase 1a and stored to LDS after it.
 template <int P, int TPB>
-struct Elem3Tile {
-    static constexpr int N = P + 1;
-    static constexpr int NB = N * N * N;
-    static constexpr int EPB = (TPB / NB) > 0 ? (TPB / NB) : 1;
-    static constexpr int NT = EPB * NB;
-    static constexpr int BLOCK = ((NT + 63) / 64) * 64;
-};
+using Elem3Tile = ElemTile<3, P, TPB>;
 
 template <int P, class PH, int TPB>
-__global__ __launch_bounds__((Elem3Tile<P, TPB>::BLOCK), ICE_ELEM3_MINW) void element_kernel_3d(DevSpace S, typename PH::Params ph,
+__global__ __launch_bounds__((Elem3Tile<P, TPB>::BLOCK), ICE_ELEM3_MINW) void element_kernel_3d(DevSpace S, typename PH::Params ph0,
+                                                                         const double* __restrict__ emu,
                                                                          const double* __restrict__ srcload,
                                                                          const double* __restrict__ U,
                                                                          const typename PH::real* __restrict__ Q,
@@ -1174,6 +1189,7 @@ __global__ __launch_bounds__((Elem3Tile<P, TPB>::BLOCK), ICE_ELEM3_MINW) void el
     const int e = e0 + le;
     const bool active = (threadIdx.x < TL::NT) && (e < S.nelem);
     const int lec = le < EPB ? le : 0;
+    const typename PH::Params ph = local_params(ph0, emu, active ? e : 0);
     VT* B = sbuf + (size_t)lec * EBUF;
     // prologue: every global load that depends on nothing else
     {
@@ -1782,19 +1798,30 @@ int launch_residual(const DevSpace& S, const typename PH::Params& ph, const Disc
     if (ev) (void)hipEventRecord(ev[2], st);
     using TL = ElemTile<D, P>;
     int nblk = (S.nelem + TL::EPB - 1) / TL::EPB;
-    if constexpr (D == 2) {
-        element_kernel_2d<P, PH><<<nblk, 256, 0, st>>>(S, ph, d.srcload, U, reinterpret_cast<const VT*>(d.Q),
-                                                         reinterpret_cast<const VT*>(d.Pic), R, d.wave, d.wave_part);
-    } else if constexpr (D == 3) {
-        using T3 = Elem3Tile<P, ICE_ELEM3_TPB>;
-        nblk = (S.nelem + T3::EPB - 1) / T3::EPB;
-        element_kernel_3d<P, PH, ICE_ELEM3_TPB><<<nblk, T3::BLOCK, 0, st>>>(
-            S, ph, d.srcload, U, reinterpret_cast<const VT*>(d.Q), reinterpret_cast<const VT*>(d.Pic), R, d.wave,
-            d.wave_part);
+    if constexpr (D == 2 || D == 3) {
+        auto go = [&](auto tpb) {
+            constexpr int T = decltype(tpb)::value;
+            using TT = ElemTile<D, P, T>;
+            nblk = (S.nelem + TT::EPB - 1) / TT::EPB;
+            if constexpr (D == 2)
+                element_kernel_2d<P, PH, T><<<nblk, TT::BLOCK, 0, st>>>(
+                    S, ph, d.emu, d.srcload, U, reinterpret_cast<const VT*>(d.Q), reinterpret_cast<const VT*>(d.Pic), R,
+                    d.wave, d.wave_part);
+            else
+                element_kernel_3d<P, PH, T><<<nblk, TT::BLOCK, 0, st>>>(
+                    S, ph, d.emu, d.srcload, U, reinterpret_cast<const VT*>(d.Q), reinterpret_cast<const VT*>(d.Pic), R,
+                    d.wave, d.wave_part);
+        };
+        switch (d.tile) {
+        case 1: go(std::integral_constant<int, tile_tpb(1)>{}); break;
+        case 2: go(std::integral_constant<int, tile_tpb(2)>{}); break;
+        default: go(std::integral_constant<int, (D == 3 ? ICE_ELEM3_TPB : 256)>{}); break;
+        }
     } else {
         static_assert(std::is_same<VT, double>::value || D > 1, "the mixed-precision residual is 2D and 3D");
         if constexpr (std::is_same<VT, double>::value)
-            element_kernel<D, P, PH><<<nblk, 256, 0, st>>>(S, ph, d.srcload, U, d.Q, d.Pic, R, d.wave, d.wave_part);
+            element_kernel<D, P, PH><<<nblk, 256, 0, st>>>(S, ph, d.emu, d.srcload, U, d.Q, d.Pic, R, d.wave,
+                                                            d.wave_part);
     }
     if (d.wave) wave_reduce_kernel<<<1, 256, 0, st>>>(d.wave_part, nblk, d.wave);
     if (ev) (void)hipEventRecord(ev[3], st);

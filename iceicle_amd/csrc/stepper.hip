@@ -168,6 +168,16 @@ __global__ void axpy_dt_kernel(size_t n, double coef, const double* __restrict__
         y[i] += a * x[i];
 }
 
+// y += coef * dt_e * x with a step size per element (batched instances);
+// per = entries per element; nothing once the stepper's run is done
+__global__ void axpy_edt_kernel(size_t n, double coef, const double* __restrict__ dte, size_t per,
+                                const StepState* __restrict__ stp, const double* __restrict__ x,
+                                double* __restrict__ y) {
+    if (stp->done) return;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        y[i] += coef * dte[i / per] * x[i];
+}
+
 // y = a x + b y  (axpby, fespan.hpp:312-330)
 __global__ void axpby_kernel(size_t n, double a, const double* __restrict__ x, double b, double* __restrict__ y) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -245,6 +255,12 @@ static unsigned grid_for(size_t n) {
 
 int launch_axpy_dt(size_t n, double coef, const double* dtp, const double* x, double* y, hipStream_t st) {
     axpy_dt_kernel<<<grid_for(n), 256, 0, st>>>(n, coef, dtp, x, y);
+    return (int)hipGetLastError();
+}
+
+int launch_axpy_edt(size_t n, double coef, const double* dte, size_t per, const StepState* stp, const double* x,
+                    double* y, hipStream_t st) {
+    axpy_edt_kernel<<<grid_for(n), 256, 0, st>>>(n, coef, dte, per, stp, x, y);
     return (int)hipGetLastError();
 }
 

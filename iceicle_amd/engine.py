@@ -37,6 +37,14 @@ _SIGS = {
     "ice_residual": ([P, P, P, P], C.c_int),
     "ice_residual_mixed": ([P, P, P, P], C.c_int),
     "ice_disc_update": ([P, C.POINTER(abi.DiscParams)], C.c_int),
+    "ice_space_create_batch": ([C.POINTER(abi.MeshDesc), C.c_int, C.POINTER(abi.SpaceDesc), C.POINTER(P)], C.c_int),
+    "ice_disc_set_element_viscosity": ([P, P], C.c_int),
+    "ice_disc_set_tile": ([P, C.c_int], C.c_int),
+    "ice_disc_autotune": ([P, P, P, C.c_int, C.POINTER(C.c_int), P, P], C.c_int),
+    "ice_stepper_set_element_dt": ([P, P], C.c_int),
+    "ice_step_graph_create": ([P, P, C.c_int, C.c_int, C.POINTER(P)], C.c_int),
+    "ice_step_graph_launch": ([P, P], C.c_int),
+    "ice_step_graph_destroy": ([P], C.c_int),
     "ice_profile_begin": ([P, C.c_int], C.c_int),
     "ice_profile_end": ([P, C.POINTER(C.c_double), C.POINTER(C.c_int)], C.c_int),
     "ice_field_eval_points": ([C.POINTER(abi.Field), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
@@ -137,11 +145,18 @@ class FESpace:
     """Mesh + reference tables + trace maps resident on the current GPU."""
 
     def __init__(self, mesh, order):
+        """`mesh` is one MeshDesc, or a list of them for a batched space
+        (the meshes concatenated, element ids of mesh k after meshes 0..k-1)."""
         L = lib()
         self._mesh = mesh
         self._space_desc = abi.space_desc(order)
         h = P()
-        check(L.ice_space_create(C.byref(mesh), C.byref(self._space_desc), C.byref(h)))
+        if isinstance(mesh, (list, tuple)):
+            arr = (abi.MeshDesc * len(mesh))(*mesh)
+            self._mesh_array = arr
+            check(L.ice_space_create_batch(arr, len(mesh), C.byref(self._space_desc), C.byref(h)))
+        else:
+            check(L.ice_space_create(C.byref(mesh), C.byref(self._space_desc), C.byref(h)))
         self.h = h
         info = abi.SpaceInfo()
         check(L.ice_space_query(self.h, C.byref(info)))
@@ -187,6 +202,30 @@ class ConservationLawDDG:
         self.h = h
         self.neq = 1 if params.flux == abi.FLUX_BURGERS else space.ndim + 2
         self.size = space.nelem * space.nbasis * self.neq
+
+    def set_element_viscosity(self, mu):
+        """Per-element viscosity (Burgers; host array [nelem]) or None."""
+        if mu is None:
+            check(lib().ice_disc_set_element_viscosity(self.h, None))
+            return
+        mu = np.ascontiguousarray(mu, dtype=np.float64)
+        if mu.size != self.space.nelem:
+            raise IceError(f"expected {self.space.nelem} viscosities")
+        check(lib().ice_disc_set_element_viscosity(self.h, mu.ctypes.data_as(P)))
+
+    def set_tile(self, tile):
+        check(lib().ice_disc_set_tile(self.h, int(tile)))
+
+    def autotune(self, u, r, reps=10, stream=None):
+        """Time the element-kernel tiles on this state, keep the fastest;
+        returns (best tile, [ms per residual for tiles 0, 1, 2])."""
+        _require_cuda(u, "u")
+        _require_cuda(r, "r")
+        best = C.c_int()
+        ms = np.zeros(3)
+        check(lib().ice_disc_autotune(self.h, _ptr(u), _ptr(r), reps, C.byref(best), ms.ctypes.data_as(P),
+                                      stream or current_stream()))
+        return best.value, ms.tolist()
 
     def update(self, params):
         """Replace the parameters in place (same flux family; DDGIC and the
@@ -279,6 +318,16 @@ class ExplicitStepper:
         t = C.c_double()
         check(lib().ice_stepper_solve(self.h, _ptr(u), stream or current_stream(), C.byref(n), C.byref(t)))
         return n.value, t.value
+
+    def set_element_dt(self, dt):
+        """Per-element step sizes (host array [nelem]) or None."""
+        if dt is None:
+            check(lib().ice_stepper_set_element_dt(self.h, None))
+            return
+        dt = np.ascontiguousarray(dt, dtype=np.float64)
+        if dt.size != self.disc.space.nelem:
+            raise IceError(f"expected {self.disc.space.nelem} step sizes")
+        check(lib().ice_stepper_set_element_dt(self.h, dt.ctypes.data_as(P)))
 
     def status(self):
         t, dt, it, done = C.c_double(), C.c_double(), C.c_longlong(), C.c_int()
@@ -451,6 +500,35 @@ class MLP:
         try:
             if getattr(self, "h", None):
                 lib().ice_mlp_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class StepGraph:
+    """nsteps steps of several steppers (each on its own device state)
+    captured into one HIP graph; launch() replays them all."""
+
+    def __init__(self, steppers, states, nsteps):
+        if len(steppers) != len(states):
+            raise IceError("one state per stepper")
+        for u in states:
+            _require_cuda(u, "u")
+        self._keep = (list(steppers), list(states))
+        sa = (P * len(steppers))(*[s.h for s in steppers])
+        ua = (P * len(states))(*[u.data_ptr() for u in states])
+        h = P()
+        check(lib().ice_step_graph_create(sa, ua, len(steppers), nsteps, C.byref(h)))
+        self.h = h
+        self.nsteps = nsteps
+
+    def launch(self, stream=None):
+        check(lib().ice_step_graph_launch(self.h, stream or current_stream()))
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().ice_step_graph_destroy(self.h)
                 self.h = None
         except Exception:
             pass

@@ -198,6 +198,16 @@ int ice_mesh_export(const ice_mesh_desc* mesh, ice_mesh_counts* counts,
  * trace maps are built on the host and uploaded to the current HIP device. */
 int ice_space_create(const ice_mesh_desc* mesh, const ice_space_desc* space,
                      ice_space** out);
+
+/* Batched space: nmesh independent meshes (same ndim and periodic mode; with
+ * coupled periodic faces, the same box extents) concatenated into one space,
+ * the elements of mesh k following those of meshes 0..k-1 and no face
+ * connecting two meshes.  One residual launch then evaluates every instance;
+ * with ice_disc_set_element_viscosity each instance keeps its own viscosity.
+ * No reference counterpart (the reference solves one mesh per process).
+ * The multigrid and line preconditioners reject batched spaces. */
+int ice_space_create_batch(const ice_mesh_desc* meshes, int nmesh, const ice_space_desc* space,
+                           ice_space** out);
 int ice_space_destroy(ice_space* space);
 int ice_space_query(const ice_space* space, ice_space_info* info);
 
@@ -217,6 +227,23 @@ int ice_disc_destroy(ice_disc* disc);
  * (ICE_ERR_ARG otherwise).  No reference counterpart: the reference builds a
  * new ConservationLawDDG per problem (conservation_law.cpp:287-346). */
 int ice_disc_update(ice_disc* disc, const ice_disc_params* params);
+
+/* Burgers only: a viscosity per element (host array [nelem]) replacing
+ * params.mu in the volume and face fluxes, so that the instances of a
+ * batched space keep their own viscosity; null restores params.mu.  The
+ * device Jacobian is unavailable while it is set.  No reference counterpart. */
+int ice_disc_set_element_viscosity(ice_disc* disc, const double* mu);
+
+/* Element-kernel workgroup shape of this discretization's residual (its p,
+ * number of variables and dimension are fixed): tile 0 = whole elements in
+ * 256 threads (default), 1 = 128, 2 = 64.  Results are identical; only speed
+ * changes.  ice_disc_autotune times `reps` FP64 residuals at u for each tile
+ * on the stream, keeps the fastest (*best) and writes the per-residual
+ * milliseconds to ms[3] (either may be null); r is overwritten.
+ * Synchronous.  No reference counterpart. */
+int ice_disc_set_tile(ice_disc* disc, int tile);
+int ice_disc_autotune(ice_disc* disc, const double* u_dev, double* r_dev, int reps, int* best, double* ms,
+                      void* stream);
 
 /* form_residual (form_residual.hpp:56-268): res <- R(u), FP64, device pointers
  * in fe_layout_right order, both nelem*nbasis*neq doubles. */
@@ -377,6 +404,21 @@ typedef struct ice_stepper ice_stepper;
 
 int ice_stepper_create(ice_disc* disc, const ice_stepper_desc* desc, ice_stepper** out);
 int ice_stepper_destroy(ice_stepper* stepper);
+
+/* Per-element step sizes (host array [nelem], positive) for a fixed-step
+ * stepper, e.g. each batched instance's own CFL step; null restores the
+ * descriptor's dt.  No reference counterpart. */
+int ice_stepper_set_element_dt(ice_stepper* stepper, const double* dt);
+
+/* One HIP graph holding nsteps steps of every stepper, each advancing its own
+ * device state u[k] (fixed step sizes only): a single graph launch then runs
+ * all of them.  The state pointers are baked into the graph.  No reference
+ * counterpart. */
+typedef struct ice_step_graph ice_step_graph;
+int ice_step_graph_create(ice_stepper* const* steppers, double* const* u_dev, int n, int nsteps,
+                          ice_step_graph** out);
+int ice_step_graph_launch(ice_step_graph* graph, void* stream);
+int ice_step_graph_destroy(ice_step_graph* graph);
 /* One step of u (device, fe_layout_right) on `stream`; time step, time and
  * step count live on the device, so consecutive steps need no host sync. */
 int ice_stepper_step(ice_stepper* stepper, double* u_dev, void* stream);

@@ -133,3 +133,17 @@ def test_warm_start_features_and_sampling():
     assert f.dtype == np.float32 and f.min() >= -1.0 - 1e-6 and f.max() <= 1.0 + 1e-6
     corners = warmstart.features([[1e-3, 0.5, 2.0], [1e-2, 2.0, 0.5]])
     assert np.allclose(corners, [[-1, -1, 1], [1, 1, -1]], atol=1e-6)
+
+
+def test_batch_instance_sampling():
+    from iceicle_amd import batch
+    insts = batch.sample_instances(512, seed=3)
+    again = batch.sample_instances(512, seed=3)
+    assert all(a["N"] == b["N"] and a["p"] == b["p"] and a["nu"] == b["nu"] for a, b in zip(insts, again))
+    assert min(i["N"] for i in insts) >= 16 and max(i["N"] for i in insts) <= 96
+    assert {i["p"] for i in insts} == {1, 2, 3, 4}
+    assert all(1e-3 <= i["nu"] <= 1e-1 for i in insts)
+    # the initial condition is the stated sum of sinusoids
+    x, y = np.array([0.1, 0.3]), np.array([0.2, 0.7])
+    amp = np.array([1.0, 0.0, 0.0, 0.0])
+    assert np.allclose(batch.initial_condition(x, y, amp), np.sin(2 * np.pi * x))
