@@ -204,46 +204,66 @@ __global__ __launch_bounds__(256) void block_inverse_kernel(int nelem, int nslot
 }
 
 // Block LU of a block-tridiagonal matrix (a chain of elements e = 0..n-1,
-// slot 1 = element e-1, slot 2 = element e+1), one workgroup walking the
-// chain: S_0 = D_0, S_e = D_e - L_e W_{e-1}, W_e = S_e^-1 U_e, with
-// L_e = J[e][1], U_e = J[e][2].  Stores S_e^-1 and W_e.  A singular S_e is
-// replaced by the identity and counted.
-__global__ __launch_bounds__(256) void line_factor_kernel(int nelem, int nslots, int bs, const double* __restrict__ J,
+// slot 1 = element e-1, slot 2 = element e+1), one wave walking the chain:
+// S_0 = D_0, S_e = D_e - L_e W_{e-1}, W_e = S_e^-1 U_e, with L_e = J[e][1],
+// U_e = J[e][2].  Stores S_e^-1 and W_e.  A singular S_e is replaced by the
+// identity and counted.  The chain is sequential, so its cost is latency:
+// the three blocks of element e+1 are loaded into registers while element e
+// is factored and stored to LDS (double buffer) at the end of the step.
+constexpr int LINE_STAGE = 12;  // staged values per lane: 3 bs^2 <= 64 * 12 (bs <= 16)
+__global__ __launch_bounds__(64) void line_factor_kernel(int nelem, int nslots, int bs, const double* __restrict__ J,
                                                           double* __restrict__ Sinv, double* __restrict__ W,
                                                           int* __restrict__ nsingular) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int bb = bs * bs;
     double* A = reinterpret_cast<double*>(smem_raw);
-    double* Wp = A + (size_t)bs * bs;
-    int* perm = reinterpret_cast<int*>(Wp + (size_t)bs * bs);
+    double* Wp = A + bb;
+    double* blk = Wp + bb;  // [2][D | L | U]
+    int* perm = reinterpret_cast<int*>(blk + 6 * bb);
     __shared__ int s_piv, s_bad;
     const int tid = threadIdx.x;
-    const size_t bb = (size_t)bs * bs;
+    auto src = [&](int e, int k) {  // k-th of element e's D, L, U values (slots 0, 1, 2)
+        return J[(size_t)e * nslots * bb + k];
+    };
+    for (int k = tid; k < 3 * bb; k += 64) blk[k] = src(0, k);
+    __syncthreads();
+    double stage[LINE_STAGE];
     for (int e = 0; e < nelem; ++e) {
-        const double* Je = J + (size_t)e * nslots * bb;
-        for (int idx = tid; idx < bs * bs; idx += blockDim.x) {
-            const int i = idx / bs, c = idx % bs;
-            double sacc = Je[idx];
-            if (e > 0) {
-                const double* L = Je + bb;
-                for (int k = 0; k < bs; ++k) sacc -= L[(size_t)i * bs + k] * Wp[(size_t)k * bs + c];
+        const double* cur = blk + (e & 1) * 3 * bb;
+        if (e + 1 < nelem) {
+#pragma unroll
+            for (int r = 0; r < LINE_STAGE; ++r) {
+                const int k = tid + r * 64;
+                if (k < 3 * bb) stage[r] = src(e + 1, k);
             }
+        }
+        for (int idx = tid; idx < bb; idx += 64) {
+            const int i = idx / bs, c = idx % bs;
+            double sacc = cur[idx];
+            if (e > 0)
+                for (int k = 0; k < bs; ++k) sacc -= cur[bb + i * bs + k] * Wp[k * bs + c];
             A[idx] = sacc;
         }
         __syncthreads();
         if (!gj_inverse_lds(A, perm, bs, &s_piv, &s_bad)) {
-            for (int idx = tid; idx < bs * bs; idx += blockDim.x) A[idx] = (idx / bs == idx % bs) ? 1.0 : 0.0;
+            for (int idx = tid; idx < bb; idx += 64) A[idx] = (idx / bs == idx % bs) ? 1.0 : 0.0;
             if (tid == 0) atomicAdd(nsingular, 1);
             __syncthreads();
         }
-        for (int idx = tid; idx < bs * bs; idx += blockDim.x) Sinv[e * bb + idx] = A[idx];
+        for (int idx = tid; idx < bb; idx += 64) Sinv[(size_t)e * bb + idx] = A[idx];
         if (e + 1 < nelem) {
-            const double* Uu = Je + 2 * bb;
-            for (int idx = tid; idx < bs * bs; idx += blockDim.x) {
+            for (int idx = tid; idx < bb; idx += 64) {
                 const int i = idx / bs, c = idx % bs;
                 double sacc = 0.0;
-                for (int k = 0; k < bs; ++k) sacc += A[(size_t)i * bs + k] * Uu[(size_t)k * bs + c];
+                for (int k = 0; k < bs; ++k) sacc += A[i * bs + k] * cur[2 * bb + k * bs + c];
                 Wp[idx] = sacc;
-                W[e * bb + idx] = sacc;
+                W[(size_t)e * bb + idx] = sacc;
+            }
+            double* nxt = blk + ((e + 1) & 1) * 3 * bb;
+#pragma unroll
+            for (int r = 0; r < LINE_STAGE; ++r) {
+                const int k = tid + r * 64;
+                if (k < 3 * bb) nxt[k] = stage[r];
             }
         }
         __syncthreads();
@@ -252,47 +272,270 @@ __global__ __launch_bounds__(256) void line_factor_kernel(int nelem, int nslots,
 
 // x = (L U)^-1 b with the factors of line_factor_kernel: forward
 // y_e = S_e^-1 (b_e - L_e y_{e-1}), backward x_e = y_e - W_e x_{e+1}.
-// One wave; each lane owns rows i, i + 64, ... of the current block.
+// One wave; lane i owns row i of the current block (bs <= 16).  The next
+// element's blocks are loaded into registers a step ahead, as in the
+// factorisation.
 __global__ __launch_bounds__(64) void line_solve_kernel(int nelem, int nslots, int bs, const double* __restrict__ J,
                                                         const double* __restrict__ Sinv, const double* __restrict__ W,
                                                         const double* __restrict__ b, double* __restrict__ x) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int bb = bs * bs;
     double* t = reinterpret_cast<double*>(smem_raw);
     double* yp = t + bs;
     const int tid = threadIdx.x;
-    const size_t bb = (size_t)bs * bs;
+    const bool row = tid < bs;
+    // lane i holds row i of L_e and S_e^-1 (forward) or W_e (backward), and b_e[i]
+    double Lr[16], Sr[16], bi = 0.0;
+    double nLr[16], nSr[16], nbi = 0.0;
+    auto load_fwd = [&](int e, double* l, double* sr, double& bv) {
+        if (!row) return;
+        const double* L = J + ((size_t)e * nslots + 1) * bb + tid * bs;
+        const double* Si = Sinv + (size_t)e * bb + tid * bs;
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if (k < bs) {
+                l[k] = L[k];
+                sr[k] = Si[k];
+            }
+        bv = b[(size_t)e * bs + tid];
+    };
+    load_fwd(0, Lr, Sr, bi);
     for (int e = 0; e < nelem; ++e) {
-        const double* L = J + ((size_t)e * nslots + 1) * bb;
-        for (int i = tid; i < bs; i += 64) {
-            double sacc = b[(size_t)e * bs + i];
+        if (e + 1 < nelem) load_fwd(e + 1, nLr, nSr, nbi);
+        if (row) {
+            double sacc = bi;
             if (e > 0)
-                for (int k = 0; k < bs; ++k) sacc -= L[(size_t)i * bs + k] * yp[k];
-            t[i] = sacc;
+#pragma unroll
+                for (int k = 0; k < 16; ++k)
+                    if (k < bs) sacc -= Lr[k] * yp[k];
+            t[tid] = sacc;
         }
         __syncthreads();
-        const double* Si = Sinv + e * bb;
-        for (int i = tid; i < bs; i += 64) {
+        if (row) {
             double sacc = 0.0;
-            for (int k = 0; k < bs; ++k) sacc += Si[(size_t)i * bs + k] * t[k];
-            x[(size_t)e * bs + i] = sacc;
-            t[bs + i] = sacc;  // yp
+#pragma unroll
+            for (int k = 0; k < 16; ++k)
+                if (k < bs) sacc += Sr[k] * t[k];
+            x[(size_t)e * bs + tid] = sacc;
+            t[bs + tid] = sacc;  // yp
         }
         __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            Lr[k] = nLr[k];
+            Sr[k] = nSr[k];
+        }
+        bi = nbi;
     }
-    // yp holds x_{n-1} = y_{n-1}
+    // yp holds x_{n-1} = y_{n-1}; lane i re-reads the y_e[i] it wrote itself
+    auto load_bwd = [&](int e, double* w, double& yv) {
+        if (!row || e < 0) return;
+        const double* We = W + (size_t)e * bb + tid * bs;
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if (k < bs) w[k] = We[k];
+        yv = x[(size_t)e * bs + tid];
+    };
+    load_bwd(nelem - 2, Lr, bi);
     for (int e = nelem - 2; e >= 0; --e) {
-        const double* We = W + e * bb;
-        for (int i = tid; i < bs; i += 64) {
-            double sacc = x[(size_t)e * bs + i];
-            for (int k = 0; k < bs; ++k) sacc -= We[(size_t)i * bs + k] * yp[k];
+        load_bwd(e - 1, nLr, nbi);
+        if (row) {
+            double sacc = bi;
+#pragma unroll
+            for (int k = 0; k < 16; ++k)
+                if (k < bs) sacc -= Lr[k] * yp[k];
+            t[tid] = sacc;
+        }
+        __syncthreads();
+        if (row) {
+            x[(size_t)e * bs + tid] = t[tid];
+            yp[tid] = t[tid];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 16; ++k) Lr[k] = nLr[k];
+        bi = nbi;
+    }
+}
+
+// The same factorisation and solve for small blocks (bs <= 5, e.g. scalar
+// 1D problems) by a single lane holding everything in registers: the chain
+// is a sequence of dependent steps, and without LDS round trips and barriers
+// each step is a few hundred register operations.  The next element's blocks
+// are loaded a step ahead.  Gauss-Jordan on [S | I] with partial pivoting
+// (largest |S[i][k]|, lowest row on ties) by selects, so no register array
+// is indexed at run time.
+template <int BS>
+__global__ __launch_bounds__(64) void line_factor_small_kernel(int nelem, int nslots, const double* __restrict__ J,
+                                                               double* __restrict__ Sinv, double* __restrict__ W,
+                                                               int* __restrict__ nsingular) {
+    if (threadIdx.x != 0) return;
+    constexpr int BB = BS * BS;
+    double Wp[BB] = {};
+    double cD[BB], cL[BB], cU[BB], nD[BB], nL[BB], nU[BB];
+    auto load = [&](int e, double* d, double* l, double* u) {
+        const double* Je = J + (size_t)e * nslots * BB;
+#pragma unroll
+        for (int k = 0; k < BB; ++k) {
+            d[k] = Je[k];
+            l[k] = Je[BB + k];
+            u[k] = Je[2 * BB + k];
+        }
+    };
+    load(0, cD, cL, cU);
+    for (int e = 0; e < nelem; ++e) {
+        if (e + 1 < nelem) load(e + 1, nD, nL, nU);
+        double A[BS][2 * BS];
+#pragma unroll
+        for (int i = 0; i < BS; ++i)
+#pragma unroll
+            for (int c = 0; c < BS; ++c) {
+                double sacc = cD[i * BS + c];
+                if (e > 0)
+#pragma unroll
+                    for (int k = 0; k < BS; ++k) sacc -= cL[i * BS + k] * Wp[k * BS + c];
+                A[i][c] = sacc;
+                A[i][BS + c] = (i == c) ? 1.0 : 0.0;
+            }
+        bool bad = false;
+#pragma unroll
+        for (int k = 0; k < BS; ++k) {
+            int piv = k;
+            double best = fabs(A[k][k]);
+#pragma unroll
+            for (int i = k + 1; i < BS; ++i) {
+                const double a = fabs(A[i][k]);
+                if (a > best) { best = a; piv = i; }
+            }
+            if (!(best > 0.0)) bad = true;
+#pragma unroll
+            for (int i = k + 1; i < BS; ++i)
+#pragma unroll
+                for (int c = 0; c < 2 * BS; ++c) {
+                    const double tk = A[k][c], ti = A[i][c];
+                    A[k][c] = (piv == i) ? ti : tk;
+                    A[i][c] = (piv == i) ? tk : ti;
+                }
+            const double ip = 1.0 / A[k][k];
+#pragma unroll
+            for (int c = 0; c < 2 * BS; ++c) A[k][c] *= ip;
+#pragma unroll
+            for (int i = 0; i < BS; ++i) {
+                if (i == k) continue;
+                const double f = A[i][k];
+#pragma unroll
+                for (int c = 0; c < 2 * BS; ++c) A[i][c] -= f * A[k][c];
+            }
+        }
+        if (bad) {
+#pragma unroll
+            for (int i = 0; i < BS; ++i)
+#pragma unroll
+                for (int c = 0; c < BS; ++c) A[i][BS + c] = (i == c) ? 1.0 : 0.0;
+            atomicAdd(nsingular, 1);
+        }
+        double* So = Sinv + (size_t)e * BB;
+#pragma unroll
+        for (int i = 0; i < BS; ++i)
+#pragma unroll
+            for (int c = 0; c < BS; ++c) So[i * BS + c] = A[i][BS + c];
+        if (e + 1 < nelem) {
+            double* Wo = W + (size_t)e * BB;
+#pragma unroll
+            for (int i = 0; i < BS; ++i)
+#pragma unroll
+                for (int c = 0; c < BS; ++c) {
+                    double sacc = 0.0;
+#pragma unroll
+                    for (int k = 0; k < BS; ++k) sacc += A[i][BS + k] * cU[k * BS + c];
+                    Wp[i * BS + c] = sacc;
+                    Wo[i * BS + c] = sacc;
+                }
+#pragma unroll
+            for (int k = 0; k < BB; ++k) {
+                cD[k] = nD[k];
+                cL[k] = nL[k];
+                cU[k] = nU[k];
+            }
+        }
+    }
+}
+
+template <int BS>
+__global__ __launch_bounds__(64) void line_solve_small_kernel(int nelem, int nslots, const double* __restrict__ J,
+                                                              const double* __restrict__ Sinv,
+                                                              const double* __restrict__ W, const double* __restrict__ b,
+                                                              double* __restrict__ x) {
+    if (threadIdx.x != 0) return;
+    constexpr int BB = BS * BS;
+    double yp[BS] = {};
+    double cL[BB], cS[BB], cb[BS], nL[BB], nS[BB], nb[BS];
+    auto load = [&](int e, double* l, double* si, double* bv) {
+        const double* L = J + ((size_t)e * nslots + 1) * BB;
+#pragma unroll
+        for (int k = 0; k < BB; ++k) {
+            l[k] = L[k];
+            si[k] = Sinv[(size_t)e * BB + k];
+        }
+#pragma unroll
+        for (int k = 0; k < BS; ++k) bv[k] = b[(size_t)e * BS + k];
+    };
+    load(0, cL, cS, cb);
+    for (int e = 0; e < nelem; ++e) {
+        if (e + 1 < nelem) load(e + 1, nL, nS, nb);
+        double t[BS];
+#pragma unroll
+        for (int i = 0; i < BS; ++i) {
+            double sacc = cb[i];
+            if (e > 0)
+#pragma unroll
+                for (int k = 0; k < BS; ++k) sacc -= cL[i * BS + k] * yp[k];
             t[i] = sacc;
         }
-        __syncthreads();
-        for (int i = tid; i < bs; i += 64) {
-            x[(size_t)e * bs + i] = t[i];
-            yp[i] = t[i];
+#pragma unroll
+        for (int i = 0; i < BS; ++i) {
+            double sacc = 0.0;
+#pragma unroll
+            for (int k = 0; k < BS; ++k) sacc += cS[i * BS + k] * t[k];
+            yp[i] = sacc;
+            x[(size_t)e * BS + i] = sacc;
         }
-        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < BB; ++k) {
+            cL[k] = nL[k];
+            cS[k] = nS[k];
+        }
+#pragma unroll
+        for (int k = 0; k < BS; ++k) cb[k] = nb[k];
+    }
+    // backward: x_e = y_e - W_e x_{e+1}; yp holds x_{n-1}
+    double cW[BB], cy[BS], nW[BB], ny[BS];
+    auto loadb = [&](int e, double* w, double* yv) {
+#pragma unroll
+        for (int k = 0; k < BB; ++k) w[k] = W[(size_t)e * BB + k];
+#pragma unroll
+        for (int k = 0; k < BS; ++k) yv[k] = x[(size_t)e * BS + k];
+    };
+    if (nelem >= 2) loadb(nelem - 2, cW, cy);
+    for (int e = nelem - 2; e >= 0; --e) {
+        if (e >= 1) loadb(e - 1, nW, ny);
+        double xe[BS];
+#pragma unroll
+        for (int i = 0; i < BS; ++i) {
+            double sacc = cy[i];
+#pragma unroll
+            for (int k = 0; k < BS; ++k) sacc -= cW[i * BS + k] * yp[k];
+            xe[i] = sacc;
+        }
+#pragma unroll
+        for (int i = 0; i < BS; ++i) {
+            yp[i] = xe[i];
+            x[(size_t)e * BS + i] = xe[i];
+        }
+#pragma unroll
+        for (int k = 0; k < BB; ++k) cW[k] = nW[k];
+#pragma unroll
+        for (int k = 0; k < BS; ++k) cy[k] = ny[k];
     }
 }
 
@@ -349,16 +592,32 @@ template int block_inverse<double>(int, int, int, const double*, double*, int*, 
 
 int line_factor(int nelem, int nslots, int bs, const double* J, double* Sinv, double* W, int* nsingular,
                 hipStream_t st) {
-    const size_t sh = 2 * (size_t)bs * bs * sizeof(double) + (size_t)bs * sizeof(int);
-    if (sh > 160 * 1024 || nslots < 3) return -1;
-    line_factor_kernel<<<1, 256, sh, st>>>(nelem, nslots, bs, J, Sinv, W, nsingular);
+    const size_t sh = 8 * (size_t)bs * bs * sizeof(double) + (size_t)bs * sizeof(int);
+    if (bs > 16 || nslots < 3 || nelem < 1) return -1;
+    switch (bs) {
+    case 1: line_factor_small_kernel<1><<<1, 64, 0, st>>>(nelem, nslots, J, Sinv, W, nsingular); break;
+    case 2: line_factor_small_kernel<2><<<1, 64, 0, st>>>(nelem, nslots, J, Sinv, W, nsingular); break;
+    case 3: line_factor_small_kernel<3><<<1, 64, 0, st>>>(nelem, nslots, J, Sinv, W, nsingular); break;
+    case 4: line_factor_small_kernel<4><<<1, 64, 0, st>>>(nelem, nslots, J, Sinv, W, nsingular); break;
+    case 5: line_factor_small_kernel<5><<<1, 64, 0, st>>>(nelem, nslots, J, Sinv, W, nsingular); break;
+    default: line_factor_kernel<<<1, 64, sh, st>>>(nelem, nslots, bs, J, Sinv, W, nsingular); break;
+    }
     return (int)hipGetLastError();
 }
 
 int line_solve(int nelem, int nslots, int bs, const double* J, const double* Sinv, const double* W, const double* b,
                double* x, hipStream_t st) {
-    if (nslots < 3) return -1;
-    line_solve_kernel<<<1, 64, 2 * (size_t)bs * sizeof(double), st>>>(nelem, nslots, bs, J, Sinv, W, b, x);
+    if (bs > 16 || nslots < 3 || nelem < 1) return -1;
+    switch (bs) {
+    case 1: line_solve_small_kernel<1><<<1, 64, 0, st>>>(nelem, nslots, J, Sinv, W, b, x); break;
+    case 2: line_solve_small_kernel<2><<<1, 64, 0, st>>>(nelem, nslots, J, Sinv, W, b, x); break;
+    case 3: line_solve_small_kernel<3><<<1, 64, 0, st>>>(nelem, nslots, J, Sinv, W, b, x); break;
+    case 4: line_solve_small_kernel<4><<<1, 64, 0, st>>>(nelem, nslots, J, Sinv, W, b, x); break;
+    case 5: line_solve_small_kernel<5><<<1, 64, 0, st>>>(nelem, nslots, J, Sinv, W, b, x); break;
+    default:
+        line_solve_kernel<<<1, 64, 2 * (size_t)bs * sizeof(double), st>>>(nelem, nslots, bs, J, Sinv, W, b, x);
+        break;
+    }
     return (int)hipGetLastError();
 }
 

@@ -65,6 +65,27 @@ def test_line_preconditioner_solves_exactly():
     assert its <= 2 and rel <= 1e-12 and err <= 1e-9
 
 
+def test_line_preconditioner_ns_1d():
+    """Navier-Stokes 1D, P3: 12 x 12 blocks."""
+    from iceicle_amd import engine
+    p = cases.ns_params(1)
+    m = abi.mesh_desc(1, [30], [0.0], [1.0], ["DIRICHLET", "EXTRAPOLATION"])
+    disc = engine.ConservationLawDDG(engine.FESpace(m, 3), p)
+    u = torch.from_numpy(cases.random_state(p.flux, 1, 30, 4, 3)).cuda()
+    jac = engine.BlockJacobian(disc)
+    jac.assemble(u)
+    x_true = torch.from_numpy(np.random.default_rng(4).normal(size=disc.size)).cuda()
+    b = torch.empty_like(u)
+    jac.apply(x_true, b)
+    solver = engine.PetscNewton(disc, jacobian_precision=abi.PREC_FP64, preconditioner=abi.PC_LINE,
+                                linear_rtol=1e-12)
+    x = torch.full_like(u, float("nan"))
+    its, rel = solver.linear_solve(b, x, u=u)
+    err = float((x - x_true).norm() / x_true.norm())
+    print(f"line preconditioner NS 1D: {its} iterations, rel {rel:.2e}, x err {err:.2e}", flush=True)
+    assert its <= 2 and rel <= 1e-12 and err <= 1e-8
+
+
 def test_line_preconditioner_rejects_non_chains():
     from iceicle_amd import engine
     p = cases.burgers_params(1, mu=0.01, a=(1.0, 0, 0))
