@@ -29,8 +29,8 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the FP64 residual kernels on the default workload
 # (128 x 128, P3), from rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by the
 # same counters on a 256 MiB copy (x2 and x1): profiles/residual_ns2d_p3_128_hbm_counters.md
-MEASURED_TRAFFIC_128_P3 = {"trace": (6891 * 2 + 24634) * 1024, "flux": (14759 * 2 + 4176) * 1024,
-                           "element": (11032 * 2 + 8192) * 1024}
+MEASURED_TRAFFIC_128_P3 = {"trace": (6893 * 2 + 24576) * 1024, "flux": (14758 * 2 + 4368) * 1024,
+                           "element": (11019 * 2 + 8192) * 1024}
 
 
 def parse():
