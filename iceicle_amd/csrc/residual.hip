@@ -889,6 +889,18 @@ __global__ __launch_bounds__((ElemTile<2, P, TPB>::BLOCK), ICE_ELEM_MINW) void e
     __shared__ VT sT[EPB * 2 * NB * M];
     VT* sU = sUG;
     VT* sG = sUG;
+    // face fluxes in each face's tensor order, vertices, face slots, FP64
+    // tables and the source load, all staged before the first barrier
+    constexpr int LIFT = 2 * D * NFQ * M;
+    constexpr int BLOCK = TL::BLOCK;
+    constexpr int QIT = (EPB * LIFT + BLOCK - 1) / BLOCK;
+    constexpr int NT2 = (N + 2) * N;
+    __shared__ VT sQ[EPB * LIFT];
+    __shared__ double sX[EPB * 4 * D];
+    __shared__ int sEF[EPB * 2 * D];
+    __shared__ double sLd[NT2];
+    __shared__ signed char sFT[2 * D * 2 * NFQ];  // two orientation codes in 2D
+    __shared__ double sSL[EPB * NB * M];
     const int e0 = blockIdx.x * EPB;
     const int le = threadIdx.x / NB;
     const int k = threadIdx.x - le * NB;
@@ -901,8 +913,32 @@ __global__ __launch_bounds__((ElemTile<2, P, TPB>::BLOCK), ICE_ELEM_MINW) void e
         const size_t total = (size_t)S.nelem * NB * M;
         for (int idx = threadIdx.x; idx < EPB * NB * M; idx += blockDim.x)
             sU[idx] = (VT)((base + idx < total) ? U[base + idx] : 0.0);
+        if (srcload)
+            for (int idx = threadIdx.x; idx < EPB * NB * M; idx += blockDim.x)
+                sSL[idx] = (base + idx < total) ? srcload[base + idx] : 0.0;
+        const size_t xb = (size_t)e0 * 4 * D, xt = (size_t)S.nelem * 4 * D;
+        for (int idx = threadIdx.x; idx < EPB * 4 * D; idx += blockDim.x) sX[idx] = (xb + idx < xt) ? S.X[xb + idx] : 0.0;
+        const size_t fb = (size_t)e0 * 2 * D, ft = (size_t)S.nelem * 2 * D;
+        for (int idx = threadIdx.x; idx < EPB * 2 * D; idx += blockDim.x)
+            sEF[idx] = (fb + idx < ft) ? S.efaces[fb + idx] : -1;
+        for (int idx = threadIdx.x; idx < NT2; idx += blockDim.x) sLd[idx] = S.tab[(N + 2) + idx];
+        for (int idx = threadIdx.x; idx < 2 * D * 2 * NFQ && idx < 2 * D * S.norient * NFQ; idx += blockDim.x)
+            sFT[idx] = S.ftens[idx];
     }
     __syncthreads();
+    // the element face fluxes, loaded now and staged after phase 1a
+    VT qv[QIT];
+    int qdst[QIT];
+#pragma unroll
+    for (int it = 0; it < QIT; ++it) {
+        const int idx = threadIdx.x + it * BLOCK;
+        const int l = idx / LIFT, rem = idx - l * LIFT;
+        const int j = rem / (NFQ * M), w = rem - j * (NFQ * M);
+        const int fq = w / M, v = w - fq * M;
+        const int ef = (idx < EPB * LIFT) ? sEF[l * 2 * D + j] : -1;
+        qv[it] = (ef >= 0) ? Q[(size_t)(ef >> 4) * NFQ * M + w] : (VT)0;
+        qdst[it] = (ef >= 0) ? l * LIFT + (j * NFQ + sFT[(j * S.norient + (ef & 7)) * NFQ + fq]) * M + v : -1;
+    }
     const double* xs = S.tab;
     const double* wgl = S.tab + (N + 2) + 3 * (N + 2) * N;
     const VT* L0 = tab_of<VT>(S) + (N + 2);
@@ -932,11 +968,20 @@ __global__ __launch_bounds__((ElemTile<2, P, TPB>::BLOCK), ICE_ELEM_MINW) void e
         }
     }
     __syncthreads();
+#pragma unroll
+    for (int it = 0; it < QIT; ++it)
+        if (qdst[it] >= 0) sQ[qdst[it]] = qv[it];
     double X[1 << D][D];
+    {
+        const int lec = le < EPB ? le : 0;
+#pragma unroll
+        for (int vv = 0; vv < (1 << D); ++vv)
+#pragma unroll
+            for (int d = 0; d < D; ++d) X[vv][d] = sX[(lec * 4 + vv) * D + d];
+    }
     double lam_w = 0.0, vis_w = 0.0;
     // phase 1b: (q0 = k0, q1 = k1)
     if (active) {
-        load_vertices<D>(S.X, e, X);
         double xi[D] = {xs[k0], xs[k1]};
         const double wq = wgl[k0] * wgl[k1];
         double x[D], J[D][D], Hm[D][D][D], A[D][D];
@@ -1048,34 +1093,42 @@ __global__ __launch_bounds__((ElemTile<2, P, TPB>::BLOCK), ICE_ELEM_MINW) void e
         }
     }
     if (srcload) {
-        const double* sl = srcload + ((size_t)e * NB + k) * M;
 #pragma unroll
-        for (int v = 0; v < M; ++v) r[v] -= sl[v];
+        for (int v = 0; v < M; ++v) r[v] -= sSL[(le * NB + k) * M + v];
     }
     const int kd[D] = {k0, k1};
+    // lifting of the faces this thread's node lies on, from the staged
+    // fluxes: the trace basis at the node factorises into the normal-axis
+    // endpoint value and l_s(x_p) along the face
+    unsigned mask = 0;
+#pragma unroll
+    for (int j = 0; j < 2 * D; ++j)
+        if (kd[j % D] == ((j < D) ? 0 : N - 1)) mask |= 1u << j;
+    while (mask) {
+        const int j = __builtin_ctz(mask);
+        mask &= mask - 1;
+        const int ef = sEF[le * 2 * D + j];
+        if (ef < 0) continue;
+        const int kn = j % D;
+        const int st = kd[1 - kn];
+        const double en = sLd[((j < D) ? N : N + 1) * N + kd[kn]];
+        const double sgn = ((ef >> 3) & 1) ? -en : en;
+        const VT* Qf = sQ + le * LIFT + j * NFQ * M;
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            double acc = 0.0;
+#pragma unroll
+            for (int p = 0; p < NFQ; ++p) acc += (double)Qf[p * M + v] * sLd[p * N + st];
+            r[v] += sgn * acc;
+        }
+    }
 #pragma unroll 1
     for (int j = 0; j < 2 * D; ++j) {
-        const int ef = S.efaces[(size_t)e * 2 * D + j];
+        const int ef = sEF[le * 2 * D + j];
         if (ef < 0) continue;
         const size_t slot = (size_t)(ef >> 4);
-        const int side = (ef >> 3) & 1;
         const int orient = ef & 7;
-        const double sgn = side ? -1.0 : 1.0;
         const signed char* fp = S.fpt + ((size_t)(j * S.norient + orient) * NFQ) * D;
-        const int kn = j % D;
-        const bool on_face = kd[kn] == ((j < D) ? 0 : N - 1);
-        if (on_face) {
-            const VT* Qs = Q + slot * NFQ * M;
-            const double* Ld = S.tab + (N + 2);
-#pragma unroll
-            for (int fq = 0; fq < NFQ; ++fq) {
-                double phi = 1.0;
-#pragma unroll
-                for (int d = 0; d < D; ++d) phi *= Ld[fp[fq * D + d] * N + kd[d]];
-#pragma unroll
-                for (int v = 0; v < M; ++v) r[v] += sgn * (double)Qs[fq * M + v] * phi;
-            }
-        }
         if constexpr (PH::HAS_HOMOGENEITY) {
             if (Pic) {
                 const VT* Ps = Pic + slot * NFQ * M * D;
