@@ -43,13 +43,14 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-newton", action="store_true", help="skip the Newton time-to-solution leg")
-    ap.add_argument("--workload", default="ns2d", choices=["ns2d", "euler3d", "warmstart", "batch"],
+    ap.add_argument("--workload", default="ns2d", choices=["ns2d", "euler3d", "warmstart", "warmstart2d", "batch"],
                     help="ns2d: the headline 2D NS MMS workload (with the 3D Euler leg appended); "
                          "euler3d: only the 3D Euler vortex 64^3 P3 leg")
     ap.add_argument("--no-euler3d", action="store_true", help="skip the 3D Euler vortex leg")
     ap.add_argument("--no-warmstart", action="store_true", help="skip the learned warm-start leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances leg")
-    ap.add_argument("--warmstart-train", type=int, default=4000, help="GPU-generated training solves")
+    ap.add_argument("--warmstart-train", type=int, default=4000, help="GPU-generated training solves (1D)")
+    ap.add_argument("--warmstart2d-train", type=int, default=500, help="GPU-generated training solves (2D)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="only run 20 device copies of an MB-sized buffer (HBM counter calibration)")
     return ap.parse_args()
@@ -141,20 +142,24 @@ def euler3d(args, n=64, order=3, nres=30, nsteps=50):
     }
 
 
-def warmstart_leg(ntrain, ntest=100, epochs=300):
-    """Learned warm start on the steady 1D viscous Burgers layer family
-    (256 elements, P3, 1024 unknowns; nu log-uniform in [1e-3, 1e-2], uL, uR
-    uniform in [0.5, 2]): ntrain GPU Newton solves from the plain guess,
-    Adam training of the 3 -> 256 -> 256 -> 1024 network on the device, then
-    ntest unseen problems solved from the plain guess and from the network's
-    guess (plain-guess fallback), both to ||R||_2 < 1e-10 with the same
-    Newton (Wolfe linesearch, exact block-tridiagonal preconditioner)."""
+def warmstart_leg(ntrain, ntest=100, epochs=300, dim=1):
+    """Learned warm start on the steady viscous Burgers layer family (nu
+    log-uniform in [1e-3, 1e-2], uL, uR uniform in [0.5, 2]).  1D: 256
+    elements, P3, 1024 unknowns, Newton with the exact block-tridiagonal
+    preconditioner; 2D: 64 x 64, P3, 65,536 unknowns, Newton with the
+    agglomeration multigrid and Eisenstat-Walker forcing.  ntrain GPU Newton
+    solves from the plain guess, Adam training of the 3 -> 256 -> 256 ->
+    unknowns network on the device, then ntest unseen problems solved from
+    the plain guess and from the network's guess (plain-guess fallback),
+    both to ||R||_2 < 1e-10 with the same Newton (Wolfe linesearch)."""
     import torch
     from iceicle_amd import abi, warmstart as W, workloads
-    print(f"warm start: generating {ntrain} solves ...", file=sys.stderr, flush=True)
-    params = W.sample_params(ntrain + ntest, seed=workloads.SEED0 + 4)
+    print(f"warm start {dim}D: generating {ntrain} solves ...", file=sys.stderr, flush=True)
+    params = W.sample_params(ntrain + ntest, seed=workloads.SEED0 + 4 + 10 * (dim - 1))
     train, test = params[:ntrain], params[ntrain:]
-    gen = W.BurgersLayer1D(linesearch=abi.LS_NONE)
+    Fam = W.BurgersLayer1D if dim == 1 else W.BurgersLayer2D
+    # 1D generation runs without a linesearch (every member converges, in fewer steps)
+    gen = Fam(linesearch=abi.LS_NONE) if dim == 1 else Fam()
     t0 = time.perf_counter()
     sols, reps = W.generate(gen, train)
     t_gen = time.perf_counter() - t0
@@ -163,7 +168,7 @@ def warmstart_leg(ntrain, ntest=100, epochs=300):
           f"{ {k: reps[0][k] for k in ('converged', 'iterations', 'rhist')} }", file=sys.stderr, flush=True)
     if not ok.any():
         raise RuntimeError("no training solve converged")
-    fam = W.BurgersLayer1D()
+    fam = Fam() if dim == 1 else gen
     ws = W.WarmStart(fam, seed=1)
     hist, t_train = ws.train(train, sols, abi.mlp_train_opts(batch=64, epochs=epochs, lr=2e-3, lr_decay=0.99, seed=3))
     print(f"warm start: trained in {t_train:.1f} s, evaluating {ntest} ...", file=sys.stderr, flush=True)
@@ -181,9 +186,11 @@ def warmstart_leg(ntrain, ntest=100, epochs=300):
         warm_its.append(rep["iterations"])
         warm_ok += int(rep["converged"])
         used += int(how == "network")
+    problem = ("steady 1D viscous Burgers layer, 256 elements, P3, 1024 unknowns, Dirichlet uL/uR" if dim == 1 else
+               "steady 2D viscous Burgers layers, 64x64 quads, P3, 65536 unknowns, Dirichlet uL inflow / uR outflow")
     return {
-        "problem": "steady 1D viscous Burgers layer, 256 elements, P3, 1024 unknowns, Dirichlet uL/uR",
-        "network": "3 -> 256 -> 256 -> 1024, tanh, FP32, device GEMMs on matrix cores, Adam",
+        "problem": problem,
+        "network": f"3 -> 256 -> 256 -> {fam.size}, tanh, FP32, device GEMMs on matrix cores, Adam",
         "train_solves": int(ntrain), "train_converged": int(ok.sum()), "generate_seconds": round(t_gen, 2),
         "train_seconds": round(t_train, 2), "epochs": epochs,
         "loss_first_last": [float(f"{hist[0]:.4e}"), float(f"{hist[-1]:.4e}")],
@@ -286,6 +293,9 @@ def main():
         return
     if args.workload == "warmstart":
         print(json.dumps(warmstart_leg(args.warmstart_train)))
+        return
+    if args.workload == "warmstart2d":
+        print(json.dumps(warmstart_leg(args.warmstart2d_train, ntest=50, dim=2)))
         return
     if args.workload == "batch":
         print(json.dumps(batch_leg()))
@@ -451,6 +461,7 @@ def main():
         out["euler3d"] = euler3d(args)
     if not args.no_warmstart:
         out["warmstart"] = warmstart_leg(args.warmstart_train)
+        out["warmstart_2d"] = warmstart_leg(args.warmstart2d_train, ntest=50, dim=2)
     if not args.no_batch:
         out["batched"] = batch_leg()
     if not args.no_cpu_baseline:

@@ -95,6 +95,31 @@ class BurgersLayer1D:
         return self.solver.solve(u)
 
 
+class BurgersLayer2D(BurgersLayer1D):
+    """The 2D member: F(u) = (u^2/2, u^2/2) - nu grad u on [0, 1]^2 with
+    u = uL on the inflow sides (x = 0, y = 0) and u = uR on the outflow sides
+    (x = 1, y = 1); the layers form against the outflow sides and the
+    corner.  Newton with GMRES and the agglomeration multigrid (FP32 blocks,
+    FP64 refinement), Eisenstat-Walker forcing."""
+
+    def __init__(self, nelem=64, order=3, tau_abs=1e-10, kmax=60, linesearch=abi.LS_WOLFE):
+        self.mesh = abi.mesh_desc(2, [nelem, nelem], [0.0, 0.0], [1.0, 1.0], ["DIRICHLET"] * 4,
+                                  bcflag=[0, 0, 1, 1])
+        self.space = engine.FESpace(self.mesh, order)
+        xn = self.space.node_coords()
+        self.xn = 0.5 * (xn[..., 0] + xn[..., 1]).reshape(-1)  # the plain guess ramps along x + y
+        self.disc = engine.ConservationLawDDG(self.space, self.params(NU_RANGE[0], 1.0, 1.0))
+        self.size = self.disc.size
+        self.solver = engine.PetscNewton(self.disc, tau_abs=tau_abs, tau_rel=0.0, kmax=kmax, linesearch=linesearch,
+                                         preconditioner=abi.PC_MULTIGRID, forcing=abi.FORCING_EW)
+
+    @staticmethod
+    def params(nu, ul, ur):
+        p = BurgersLayer1D.params(nu, ul, ur)
+        p.b[1] = 1.0
+        return p
+
+
 def generate(fam, params, verbose=False):
     """Solve each problem from the plain guess.  Returns (solutions [n][size]
     FP64 with NaN rows for failures, reports)."""

@@ -147,3 +147,26 @@ def test_warm_start_cuts_newton_iterations():
     print(f"Newton steps plain {plain_its} (mean {np.mean(plain_its):.2f}), warm {warm_its} "
           f"(mean {np.mean(warm_its):.2f}); guess used {used}", flush=True)
     assert np.mean(warm_its) < np.mean(plain_its)
+
+
+def test_warm_start_2d_cuts_newton_iterations():
+    """The 2D family (16 x 16, P3, multigrid Newton): train on 96 solves,
+    solve 6 unseen problems from the network's guess."""
+    from iceicle_amd import warmstart
+    fam = warmstart.BurgersLayer2D(nelem=16)
+    params = warmstart.sample_params(102, seed=workloads.SEED0 + 7)
+    train, test = params[:96], params[96:]
+    sols, reps = warmstart.generate(fam, train)
+    assert np.all(np.isfinite(sols))
+    ws = warmstart.WarmStart(fam, seed=4)
+    hist, secs = ws.train(train, sols, abi.mlp_train_opts(batch=16, epochs=200, lr=2e-3, lr_decay=0.985, seed=5))
+    plain_its, warm_its = [], []
+    for prm in test:
+        fam.set(*prm)
+        u = torch.from_numpy(fam.plain_guess(prm[1], prm[2])).cuda()
+        plain_its.append(fam.solve(u)["iterations"])
+        uw, rep, how = ws.solve(*prm)
+        assert rep["converged"] and rep["rnorm"] < 1e-10
+        warm_its.append(rep["iterations"])
+    print(f"2D: loss {hist[0]:.3e} -> {hist[-1]:.3e}; Newton steps plain {plain_its}, warm {warm_its}", flush=True)
+    assert np.mean(warm_its) < np.mean(plain_its)
