@@ -1083,11 +1083,11 @@ __global__ __launch_bounds__(RT_THREADS) void diagT_kernel(int nelem, int bs, co
 // workgroup owns whole elements (epb of them; its threads loop over their
 // rows), because the multiplication by the factored diagonal block needs the
 // element's complete partial-sum vector.
-template <class T, class V, int NS, bool FWD>
+template <class T, class V, int NS, bool FWD, class TD>
 __global__ __launch_bounds__(RT_THREADS) void sgsT_kernel(int nel, int epb, const int* __restrict__ elems,
                                                           const int* __restrict__ ecolor, int colour, int bs,
                                                           const int* __restrict__ cols, const T* __restrict__ JT,
-                                                          const T* __restrict__ DT, const V* __restrict__ rhs,
+                                                          const TD* __restrict__ DT, const V* __restrict__ rhs,
                                                           V* __restrict__ y) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     V* sx = reinterpret_cast<V*>(smem_raw);  // [epb][NS][bs] neighbour values
@@ -1131,7 +1131,7 @@ __global__ __launch_bounds__(RT_THREADS) void sgsT_kernel(int nel, int epb, cons
         const int le = g / bs, r = g - le * bs;
         const int e = elems[l0 + le];
         const V* tv = st + (size_t)le * bs;
-        const T* Dc = DT + (size_t)e * bs * bs + r;
+        const TD* Dc = DT + (size_t)e * bs * bs + r;
         V acc0 = 0, acc1 = 0;
         int c = 0;
         for (; c + 1 < bs; c += 2) {
@@ -1144,13 +1144,14 @@ __global__ __launch_bounds__(RT_THREADS) void sgsT_kernel(int nel, int epb, cons
     }
 }
 
-__global__ void convert_transpose_kernel(size_t nblocks, int bs, const double* __restrict__ a, float* __restrict__ b) {
+template <class TO>
+__global__ void convert_transpose_kernel(size_t nblocks, int bs, const double* __restrict__ a, TO* __restrict__ b) {
     const size_t bb = (size_t)bs * bs;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nblocks * bb;
          i += (size_t)gridDim.x * blockDim.x) {
         const size_t blk = i / bb, k = i - blk * bb;
         const int c = (int)(k / bs), r = (int)(k - (size_t)c * bs);  // destination [c][r]
-        b[i] = (float)a[blk * bb + (size_t)r * bs + c];
+        b[i] = (TO)a[blk * bb + (size_t)r * bs + c];
     }
 }
 
@@ -1175,6 +1176,8 @@ int bsrT_apply(int nelem, int nslots, int bs, const int* cols, const T* JT, cons
 }
 template int bsrT_apply<float, double>(int, int, int, const int*, const float*, const double*, const double*, double*,
                                        hipStream_t);
+template int bsrT_apply<_Float16, double>(int, int, int, const int*, const _Float16*, const double*, const double*, double*,
+                                       hipStream_t);
 
 template <class T, class V>
 int diagT_apply(int nelem, int bs, const T* DT, const V* x, V* y, hipStream_t st) {
@@ -1186,9 +1189,9 @@ int diagT_apply(int nelem, int bs, const T* DT, const V* x, V* y, hipStream_t st
 template int diagT_apply<float, double>(int, int, const float*, const double*, double*, hipStream_t);
 template int diagT_apply<double, double>(int, int, const double*, const double*, double*, hipStream_t);
 
-template <class T, class V>
+template <class T, class V, class TD>
 int sgsT_sweep(bool forward, int nel, const int* elems, const int* ecolor, int colour, int nslots, int bs,
-               const int* cols, const T* JT, const T* DT, const V* rhs, V* y, hipStream_t st) {
+               const int* cols, const T* JT, const TD* DT, const V* rhs, V* y, hipStream_t st) {
     if (nel <= 0) return 0;
     const int epb = std::max(1, RT_THREADS / bs);
     const int grid = (nel + epb - 1) / epb;
@@ -1197,10 +1200,10 @@ int sgsT_sweep(bool forward, int nel, const int* elems, const int* ecolor, int c
 #define ICE_SGST(NSV)                                                                                               \
     case NSV:                                                                                                       \
         if (forward)                                                                                                \
-            sgsT_kernel<T, V, NSV, true><<<grid, RT_THREADS, sh, st>>>(nel, epb, elems, ecolor, colour, bs, cols,  \
+            sgsT_kernel<T, V, NSV, true, TD><<<grid, RT_THREADS, sh, st>>>(nel, epb, elems, ecolor, colour, bs, cols,  \
                                                                        JT, DT, rhs, y);                             \
         else                                                                                                        \
-            sgsT_kernel<T, V, NSV, false><<<grid, RT_THREADS, sh, st>>>(nel, epb, elems, ecolor, colour, bs, cols, \
+            sgsT_kernel<T, V, NSV, false, TD><<<grid, RT_THREADS, sh, st>>>(nel, epb, elems, ecolor, colour, bs, cols, \
                                                                         JT, DT, rhs, y);                            \
         break;
     switch (nslots) {
@@ -1212,13 +1215,21 @@ int sgsT_sweep(bool forward, int nel, const int* elems, const int* ecolor, int c
 #undef ICE_SGST
     return (int)hipGetLastError();
 }
-template int sgsT_sweep<float, double>(bool, int, const int*, const int*, int, int, int, const int*, const float*,
-                                       const float*, const double*, double*, hipStream_t);
-template int sgsT_sweep<double, double>(bool, int, const int*, const int*, int, int, int, const int*, const double*,
-                                        const double*, const double*, double*, hipStream_t);
+template int sgsT_sweep<float, double, float>(bool, int, const int*, const int*, int, int, int, const int*,
+                                              const float*, const float*, const double*, double*, hipStream_t);
+// FP16 off-diagonal blocks, FP32 factored diagonal blocks (inverse entries can exceed the FP16 range)
+template int sgsT_sweep<_Float16, double, float>(bool, int, const int*, const int*, int, int, int, const int*,
+                                                 const _Float16*, const float*, const double*, double*, hipStream_t);
+template int sgsT_sweep<double, double, double>(bool, int, const int*, const int*, int, int, int, const int*,
+                                                const double*, const double*, const double*, double*, hipStream_t);
 
 int convert_transpose_fp32(size_t nblocks, int bs, const double* a, float* b, hipStream_t st) {
-    convert_transpose_kernel<<<8192, 256, 0, st>>>(nblocks, bs, a, b);
+    convert_transpose_kernel<float><<<8192, 256, 0, st>>>(nblocks, bs, a, b);
+    return (int)hipGetLastError();
+}
+
+int convert_transpose_fp16(size_t nblocks, int bs, const double* a, _Float16* b, hipStream_t st) {
+    convert_transpose_kernel<_Float16><<<8192, 256, 0, st>>>(nblocks, bs, a, b);
     return (int)hipGetLastError();
 }
 

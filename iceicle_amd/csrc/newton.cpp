@@ -31,10 +31,11 @@ int bsrT_apply(int nelem, int nslots, int bs, const int* cols, const T* JT, cons
                hipStream_t st);
 template <class T, class V>
 int diagT_apply(int nelem, int bs, const T* DT, const V* x, V* y, hipStream_t st);
-template <class T, class V>
+template <class T, class V, class TD>
 int sgsT_sweep(bool forward, int nel, const int* elems, const int* ecolor, int colour, int nslots, int bs,
-               const int* cols, const T* JT, const T* DT, const V* rhs, V* y, hipStream_t st);
+               const int* cols, const T* JT, const TD* DT, const V* rhs, V* y, hipStream_t st);
 int convert_transpose_fp32(size_t nblocks, int bs, const double* a, float* b, hipStream_t st);
+int convert_transpose_fp16(size_t nblocks, int bs, const double* a, _Float16* b, hipStream_t st);
 int line_factor(int nelem, int nslots, int bs, const double* J, double* Sinv, double* W, int* nsingular,
                 hipStream_t st);
 int line_solve(int nelem, int nslots, int bs, const double* J, const double* Sinv, const double* W, const double* b,
@@ -209,6 +210,7 @@ struct ice_newton {
     double* sc = nullptr;       // Hessenberg, rotations, g, norms
     double* y = nullptr;        // m
     int* nsing = nullptr;
+    _Float16* J16 = nullptr;      // FP16 copy of the blocks for the smoother (column-major per block)
     double* line_Sinv = nullptr;  // block-tridiagonal factors (line preconditioner)
     double* line_W = nullptr;
     std::vector<MGLevel> mg;    // multigrid levels (level 0 = element constants)
@@ -263,8 +265,11 @@ struct ice_newton {
                 if (!fwd && c == nc - 1) continue;  // last colour: backward sweep leaves it unchanged
                 const int* el = celems + coff[c];
                 const int nr = coff[c + 1] - coff[c];
-                if (fp32())
-                    NT_LAUNCH((ice::sgsT_sweep<float, double>(fwd, nr, el, ecolor, c, nslots, bs, cols, J32,
+                if (J16)
+                    NT_LAUNCH((ice::sgsT_sweep<_Float16, double, float>(fwd, nr, el, ecolor, c, nslots, bs, cols,
+                                                                        J16, (const float*)Dinv, v, out, st)));
+                else if (fp32())
+                    NT_LAUNCH((ice::sgsT_sweep<float, double, float>(fwd, nr, el, ecolor, c, nslots, bs, cols, J32,
                                                               (const float*)Dinv, v, out, st)));
                 else
                     NT_LAUNCH((ice::sgs_sweep<double, double>(fwd, nr, el, ecolor, c, nslots, bs, cols, J64,
@@ -332,7 +337,7 @@ struct ice_newton {
                 const bool fwd = pass == 0;
                 const int c = fwd ? k : nc - 1 - k;
                 if (!fwd && c == nc - 1) continue;
-                NT_LAUNCH((ice::sgsT_sweep<float, double>(fwd, L.coff[c + 1] - L.coff[c], L.celems + L.coff[c],
+                NT_LAUNCH((ice::sgsT_sweep<float, double, float>(fwd, L.coff[c + 1] - L.coff[c], L.celems + L.coff[c],
                                                           L.ecolor, c, nslots, neq, L.cols, L.A32, L.Dinv, rhs, out,
                                                           st)));
             }
@@ -437,12 +442,17 @@ struct ice_newton {
     // constants, SGS
     void apply_mg(const double* v, double* out, hipStream_t st) {
         const int* cols = ice::disc_jcols(disc);
+        // the fine residuals inside the preconditioner read the smoother's blocks
+        auto fine_residual = [&](const double* x) {
+            if (J16) NT_LAUNCH((ice::bsrT_apply<_Float16, double>(nelem, nslots, bs, cols, J16, x, v, mg_r, st)));
+            else NT_LAUNCH((ice::bsrT_apply<float, double>(nelem, nslots, bs, cols, J32, x, v, mg_r, st)));
+        };
         apply_sgs_fine(v, out, st);
-        NT_LAUNCH((ice::bsrT_apply<float, double>(nelem, nslots, bs, cols, J32, out, v, mg_r, st)));
+        fine_residual(out);
         NT_LAUNCH(ice::mg_restrict_nodes(nelem, nbasis, neq, mg_r, mg[0].b, st));
         vcycle(0, mg[0].b, mg[0].x, st);
         NT_LAUNCH(ice::mg_prolong_nodes(nelem, nbasis, neq, mg[0].x, out, st));
-        NT_LAUNCH((ice::bsrT_apply<float, double>(nelem, nslots, bs, cols, J32, out, v, mg_r, st)));
+        fine_residual(out);
         apply_sgs_fine(mg_r, mg_t, st);
         NT_LAUNCH(ice::launch_axpby2(n, 1.0, mg_t, 1.0, out, st));
     }
@@ -457,6 +467,7 @@ struct ice_newton {
         }
         if (fp32()) {
             NT_LAUNCH(ice::convert_transpose_fp32((size_t)nelem * nslots, bs, J64, J32, st));
+            if (J16) NT_LAUNCH(ice::convert_transpose_fp16((size_t)nelem * nslots, bs, J64, J16, st));
             NT_LAUNCH(ice::block_inverse<float>(nelem, nslots, bs, J64, (float*)Dinv, nsing, true, st));
         } else {
             NT_LAUNCH(ice::block_inverse<double>(nelem, nslots, bs, J64, (double*)Dinv, nsing, false, st));
@@ -583,6 +594,7 @@ int ice_newton_opts_default(ice_newton_opts* o) {
     o->verbose = 0;
     o->preconditioner = ICE_PC_BLOCK_SGS;
     o->forcing = ICE_FORCING_FIXED;
+    o->smoother_precision = ICE_PREC_FP32;
     return ICE_OK;
 }
 
@@ -600,7 +612,11 @@ int ice_newton_create(ice_disc* d, const ice_newton_opts* opts, ice_newton** out
         (o.linesearch < ICE_LS_NONE || o.linesearch > ICE_LS_CORRIGAN) ||
         (o.preconditioner != ICE_PC_BLOCK_JACOBI && o.preconditioner != ICE_PC_BLOCK_SGS &&
          o.preconditioner != ICE_PC_MULTIGRID && o.preconditioner != ICE_PC_LINE) ||
-        (o.forcing != ICE_FORCING_FIXED && o.forcing != ICE_FORCING_EW)) {
+        (o.forcing != ICE_FORCING_FIXED && o.forcing != ICE_FORCING_EW) ||
+        (o.smoother_precision != ICE_PREC_FP32 && o.smoother_precision != ICE_PREC_FP16) ||
+        (o.smoother_precision == ICE_PREC_FP16 &&
+         (o.jacobian_precision != ICE_PREC_FP32 ||
+          (o.preconditioner != ICE_PC_BLOCK_SGS && o.preconditioner != ICE_PC_MULTIGRID)))) {
         ice::set_error("invalid Newton options");
         return ICE_ERR_ARG;
     }
@@ -617,6 +633,7 @@ int ice_newton_create(ice_disc* d, const ice_newton_opts* opts, ice_newton** out
         s->J64 = s->alloc<double>(s->nblkvals);
         if (s->fp32()) {
             s->J32 = s->alloc<float>(s->nblkvals);
+            if (o.smoother_precision == ICE_PREC_FP16) s->J16 = s->alloc<_Float16>(s->nblkvals);
             s->Dinv = s->alloc<float>((size_t)s->nelem * s->bs * s->bs);
         } else {
             s->Dinv = s->alloc<double>((size_t)s->nelem * s->bs * s->bs);

@@ -318,6 +318,7 @@ int ice_jacobian_apply(const ice_disc* disc, const double* blocks_dev, const dou
 #define ICE_LS_CORRIGAN 2
 #define ICE_PREC_FP64 0
 #define ICE_PREC_FP32 1
+#define ICE_PREC_FP16 2
 #define ICE_PC_BLOCK_JACOBI 0
 #define ICE_PC_BLOCK_SGS 1
 #define ICE_PC_MULTIGRID 2
@@ -350,6 +351,11 @@ typedef struct {
     int forcing;              /* ICE_FORCING_*: linear tolerance linear_rtol at
                                  every step (default, as the reference's KSP),
                                  or Eisenstat-Walker choice 2 */
+    int smoother_precision;   /* ICE_PREC_FP32 (default) or ICE_PREC_FP16: the
+                                 fine-level blocks the block-SGS smoother and
+                                 the multigrid's fine residuals read (a fixed
+                                 preconditioner either way; the Krylov operator
+                                 keeps jacobian_precision); with FP32 blocks */
 } ice_newton_opts;
 
 typedef struct {

@@ -86,11 +86,15 @@ def test_linear_solve_accuracy():
     x_true = torch.from_numpy(np.random.default_rng(1).normal(size=disc.size)).cuda()
     b = torch.empty_like(u)
     jac.apply(x_true, b)
-    for prec, pc, tol in [(abi.PREC_FP64, abi.PC_BLOCK_SGS, 1e-10), (abi.PREC_FP32, abi.PC_BLOCK_SGS, 1e-10),
-                          (abi.PREC_FP64, abi.PC_BLOCK_JACOBI, 1e-10), (abi.PREC_FP32, abi.PC_BLOCK_JACOBI, 1e-10),
-                          (abi.PREC_FP32, abi.PC_MULTIGRID, 1e-10)]:
+    for prec, pc, tol, sm in [(abi.PREC_FP64, abi.PC_BLOCK_SGS, 1e-10, abi.PREC_FP32),
+                              (abi.PREC_FP32, abi.PC_BLOCK_SGS, 1e-10, abi.PREC_FP32),
+                              (abi.PREC_FP64, abi.PC_BLOCK_JACOBI, 1e-10, abi.PREC_FP32),
+                              (abi.PREC_FP32, abi.PC_BLOCK_JACOBI, 1e-10, abi.PREC_FP32),
+                              (abi.PREC_FP32, abi.PC_MULTIGRID, 1e-10, abi.PREC_FP32),
+                              (abi.PREC_FP32, abi.PC_BLOCK_SGS, 1e-10, abi.PREC_FP16),
+                              (abi.PREC_FP32, abi.PC_MULTIGRID, 1e-10, abi.PREC_FP16)]:
         solver = engine.PetscNewton(disc, jacobian_precision=prec, linear_rtol=tol, gmres_restart=40,
-                                    preconditioner=pc)
+                                    preconditioner=pc, smoother_precision=sm)
         x = torch.full_like(u, float("nan"))  # old contents of the output must not leak in
         its, rel = solver.linear_solve(b, x, u=u)
         # the residual the solver reports is the FP64 one
@@ -98,7 +102,8 @@ def test_linear_solve_accuracy():
         jac.apply(x, r)
         rel_true = float((r - b).norm() / b.norm())
         xerr = float((x - x_true).norm() / x_true.norm())
-        print(f"prec={prec} pc={pc} its={its} rel={rel:.2e} true rel={rel_true:.2e} x err={xerr:.2e}", flush=True)
+        print(f"prec={prec} pc={pc} smoother={sm} its={its} rel={rel:.2e} true rel={rel_true:.2e} x err={xerr:.2e}",
+              flush=True)
         assert rel <= tol and rel_true <= 2 * tol
         assert xerr <= 1e-6
 
@@ -154,7 +159,8 @@ def test_c2_newton_full_size():
     m, order, p, keep = workloads.ns_mms_2d()
     disc = engine.ConservationLawDDG(engine.FESpace(m, order), p)
     solver = engine.PetscNewton(disc, tau_abs=1e-10, tau_rel=0.0, kmax=30, verbose=1,
-                                preconditioner=abi.PC_MULTIGRID, forcing=abi.FORCING_EW)
+                                preconditioner=abi.PC_MULTIGRID, forcing=abi.FORCING_EW,
+                                smoother_precision=abi.PREC_FP16)
     u = torch.from_numpy(_freestream(disc.size)).cuda()
     rep = solver.solve(u)
     print({k: rep[k] for k in ("iterations", "rnorm", "seconds", "linear_iterations", "jacobian_seconds",
@@ -169,6 +175,17 @@ def test_c2_newton_full_size():
     # manufactured one at this resolution
     print(f"C2 L2 error vs manufactured solution {err:.3e}", flush=True)
     assert err < 1e-4
+
+
+def test_fp16_smoother_rejected_without_fp32_blocks():
+    from iceicle_amd import engine
+    m, keep = _ns_mms(4, 2)
+    disc = engine.ConservationLawDDG(engine.FESpace(m, 2), workloads.ns_mms_params())
+    with pytest.raises(engine.IceError, match="invalid Newton options"):
+        engine.PetscNewton(disc, jacobian_precision=abi.PREC_FP64, preconditioner=abi.PC_BLOCK_SGS,
+                           smoother_precision=abi.PREC_FP16)
+    with pytest.raises(engine.IceError, match="invalid Newton options"):
+        engine.PetscNewton(disc, preconditioner=abi.PC_BLOCK_JACOBI, smoother_precision=abi.PREC_FP16)
 
 
 def test_multigrid_newton_matches_dense():
