@@ -1228,8 +1228,31 @@ int convert_transpose_fp32(size_t nblocks, int bs, const double* a, float* b, hi
     return (int)hipGetLastError();
 }
 
-int convert_transpose_fp16(size_t nblocks, int bs, const double* a, _Float16* b, hipStream_t st) {
-    convert_transpose_kernel<_Float16><<<8192, 256, 0, st>>>(nblocks, bs, a, b);
+// FP32 (and optionally FP16) column-major copies of row-major FP64 blocks,
+// one workgroup per block: the block is read once, coalesced, into LDS and
+// both copies are written coalesced from it.
+__global__ __launch_bounds__(256) void convert_transpose_tile_kernel(int bs, const double* __restrict__ a,
+                                                                      float* __restrict__ b32,
+                                                                      _Float16* __restrict__ b16) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* t = reinterpret_cast<double*>(smem_raw);
+    const size_t bb = (size_t)bs * bs;
+    const double* src = a + (size_t)blockIdx.x * bb;
+    for (int k = threadIdx.x; k < (int)bb; k += blockDim.x) t[k] = src[k];
+    __syncthreads();
+    float* d32 = b32 + (size_t)blockIdx.x * bb;
+    for (int k = threadIdx.x; k < (int)bb; k += blockDim.x) {
+        const int c = k / bs, r = k - c * bs;  // destination [c][r]
+        const float v = (float)t[r * bs + c];
+        d32[k] = v;
+        if (b16) b16[(size_t)blockIdx.x * bb + k] = (_Float16)v;
+    }
+}
+
+int convert_transpose_blocks(size_t nblocks, int bs, const double* a, float* b32, _Float16* b16, hipStream_t st) {
+    const size_t sh = (size_t)bs * bs * sizeof(double);
+    if (sh > 160 * 1024 || nblocks == 0) return nblocks == 0 ? 0 : -1;
+    convert_transpose_tile_kernel<<<(unsigned)nblocks, 256, sh, st>>>(bs, a, b32, b16);
     return (int)hipGetLastError();
 }
 

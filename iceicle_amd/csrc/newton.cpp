@@ -35,7 +35,7 @@ template <class T, class V, class TD>
 int sgsT_sweep(bool forward, int nel, const int* elems, const int* ecolor, int colour, int nslots, int bs,
                const int* cols, const T* JT, const TD* DT, const V* rhs, V* y, hipStream_t st);
 int convert_transpose_fp32(size_t nblocks, int bs, const double* a, float* b, hipStream_t st);
-int convert_transpose_fp16(size_t nblocks, int bs, const double* a, _Float16* b, hipStream_t st);
+int convert_transpose_blocks(size_t nblocks, int bs, const double* a, float* b32, _Float16* b16, hipStream_t st);
 int line_factor(int nelem, int nslots, int bs, const double* J, double* Sinv, double* W, int* nsingular,
                 hipStream_t st);
 int line_solve(int nelem, int nslots, int bs, const double* J, const double* Sinv, const double* W, const double* b,
@@ -466,8 +466,7 @@ struct ice_newton {
             return;
         }
         if (fp32()) {
-            NT_LAUNCH(ice::convert_transpose_fp32((size_t)nelem * nslots, bs, J64, J32, st));
-            if (J16) NT_LAUNCH(ice::convert_transpose_fp16((size_t)nelem * nslots, bs, J64, J16, st));
+            NT_LAUNCH(ice::convert_transpose_blocks((size_t)nelem * nslots, bs, J64, J32, J16, st));
             NT_LAUNCH(ice::block_inverse<float>(nelem, nslots, bs, J64, (float*)Dinv, nsing, true, st));
         } else {
             NT_LAUNCH(ice::block_inverse<double>(nelem, nslots, bs, J64, (double*)Dinv, nsing, false, st));
