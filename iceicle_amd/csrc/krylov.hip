@@ -1249,6 +1249,28 @@ __global__ __launch_bounds__(256) void convert_transpose_tile_kernel(int bs, con
     }
 }
 
+// x = A^-1 b with the dense inverse stored column-major (the multigrid's
+// coarsest level): one thread per row, coalesced column reads
+__global__ __launch_bounds__(256) void dense_inverse_apply_kernel(int n, const double* __restrict__ inv,
+                                                                  const double* __restrict__ b,
+                                                                  double* __restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s0 = 0.0, s1 = 0.0;
+    int j = 0;
+    for (; j + 1 < n; j += 2) {
+        s0 += inv[(size_t)j * n + i] * b[j];
+        s1 += inv[(size_t)(j + 1) * n + i] * b[j + 1];
+    }
+    if (j < n) s0 += inv[(size_t)j * n + i] * b[j];
+    x[i] = s0 + s1;
+}
+
+int dense_inverse_apply(int n, const double* inv, const double* b, double* x, hipStream_t st) {
+    dense_inverse_apply_kernel<<<(n + 255) / 256, 256, 0, st>>>(n, inv, b, x);
+    return (int)hipGetLastError();
+}
+
 int convert_transpose_blocks(size_t nblocks, int bs, const double* a, float* b32, _Float16* b16, hipStream_t st) {
     const size_t sh = (size_t)bs * bs * sizeof(double);
     if (sh > 160 * 1024 || nblocks == 0) return nblocks == 0 ? 0 : -1;

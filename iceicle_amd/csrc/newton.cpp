@@ -36,6 +36,7 @@ int sgsT_sweep(bool forward, int nel, const int* elems, const int* ecolor, int c
                const int* cols, const T* JT, const TD* DT, const V* rhs, V* y, hipStream_t st);
 int convert_transpose_fp32(size_t nblocks, int bs, const double* a, float* b, hipStream_t st);
 int convert_transpose_blocks(size_t nblocks, int bs, const double* a, float* b32, _Float16* b16, hipStream_t st);
+int dense_inverse_apply(int n, const double* inv, const double* b, double* x, hipStream_t st);
 int line_factor(int nelem, int nslots, int bs, const double* J, double* Sinv, double* W, int* nsingular,
                 hipStream_t st);
 int line_solve(int nelem, int nslots, int bs, const double* J, const double* Sinv, const double* W, const double* b,
@@ -215,6 +216,9 @@ struct ice_newton {
     double* line_W = nullptr;
     std::vector<MGLevel> mg;    // multigrid levels (level 0 = element constants)
     double *mg_r = nullptr, *mg_t = nullptr;  // fine-level work vectors of the V-cycle
+    double* coarse_inv = nullptr;  // dense inverse of the coarsest level (column-major), when it is small
+    int coarse_n = 0;
+    bool coarse_direct = false;    // this factorisation's coarsest level is solved exactly
     int neq = 0, nbasis = 0;
     int* ecolor = nullptr;      // element colour (multicolour Gauss-Seidel)
     int* celems = nullptr;      // elements grouped by colour
@@ -347,6 +351,10 @@ struct ice_newton {
     void vcycle(int l, const double* b, double* x, hipStream_t st) {
         MGLevel& L = mg[l];
         const size_t nl = (size_t)L.n * neq;
+        if (l + 1 == (int)mg.size() && coarse_direct) {
+            NT_LAUNCH(ice::dense_inverse_apply(coarse_n, coarse_inv, b, x, st));
+            return;
+        }
         sgs_level(L, b, x, st);
         if (l + 1 == (int)mg.size()) {
             for (int it = 0; it < 8; ++it) {
@@ -422,6 +430,61 @@ struct ice_newton {
         }
         mg_r = alloc<double>(n);
         mg_t = alloc<double>(n);
+        coarse_n = mg.back().n * neq;
+        if (coarse_n <= 1024) coarse_inv = alloc<double>((size_t)coarse_n * coarse_n);
+    }
+
+    // dense inverse of the coarsest level by Gauss-Jordan with partial
+    // pivoting on the host (at most 1024 unknowns), uploaded column-major;
+    // false if the level is singular (the V-cycle then keeps its SGS sweeps)
+    bool factor_coarse(hipStream_t st) {
+        if (!coarse_inv) return false;
+        const MGLevel& L = mg.back();
+        const int cn = coarse_n, mm = neq * neq;
+        std::vector<double> blk((size_t)L.n * nslots * mm);
+        NT_HIP(hipMemcpyAsync(blk.data(), L.A64, blk.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        NT_HIP(hipStreamSynchronize(st));
+        std::vector<double> a((size_t)cn * cn, 0.0), inv((size_t)cn * cn, 0.0);
+        for (int P = 0; P < L.n; ++P)
+            for (int t = 0; t < nslots; ++t) {
+                const int col = L.hcols[(size_t)P * nslots + t];
+                if (col < 0) continue;
+                const double* B = blk.data() + ((size_t)P * nslots + t) * mm;
+                for (int r = 0; r < neq; ++r)
+                    for (int c = 0; c < neq; ++c) a[(size_t)(P * neq + r) * cn + col * neq + c] += B[r * neq + c];
+            }
+        for (int i = 0; i < cn; ++i) inv[(size_t)i * cn + i] = 1.0;
+        for (int k = 0; k < cn; ++k) {
+            int p = k;
+            for (int i = k + 1; i < cn; ++i)
+                if (std::abs(a[(size_t)i * cn + k]) > std::abs(a[(size_t)p * cn + k])) p = i;
+            if (!(std::abs(a[(size_t)p * cn + k]) > 0.0)) return false;
+            if (p != k)
+                for (int j = 0; j < cn; ++j) {
+                    std::swap(a[(size_t)k * cn + j], a[(size_t)p * cn + j]);
+                    std::swap(inv[(size_t)k * cn + j], inv[(size_t)p * cn + j]);
+                }
+            const double d = 1.0 / a[(size_t)k * cn + k];
+            for (int j = 0; j < cn; ++j) {
+                a[(size_t)k * cn + j] *= d;
+                inv[(size_t)k * cn + j] *= d;
+            }
+            for (int i = 0; i < cn; ++i) {
+                if (i == k) continue;
+                const double f = a[(size_t)i * cn + k];
+                if (f == 0.0) continue;
+                for (int j = 0; j < cn; ++j) {
+                    a[(size_t)i * cn + j] -= f * a[(size_t)k * cn + j];
+                    inv[(size_t)i * cn + j] -= f * inv[(size_t)k * cn + j];
+                }
+            }
+        }
+        std::vector<double> cm((size_t)cn * cn);
+        for (int i = 0; i < cn; ++i)
+            for (int j = 0; j < cn; ++j) cm[(size_t)j * cn + i] = inv[(size_t)i * cn + j];
+        NT_HIP(hipMemcpyAsync(coarse_inv, cm.data(), cm.size() * sizeof(double), hipMemcpyHostToDevice, st));
+        NT_HIP(hipStreamSynchronize(st));
+        return true;
     }
 
     // numeric setup of the hierarchy from the FP64 Jacobian
@@ -436,6 +499,7 @@ struct ice_newton {
             NT_LAUNCH(ice::convert_transpose_fp32((size_t)L.n * nslots, neq, L.A64, L.A32, st));
             NT_LAUNCH(ice::block_inverse<float>(L.n, nslots, neq, L.A64, L.Dinv, nsing, true, st));
         }
+        coarse_direct = factor_coarse(st);
     }
 
     // fine-level V-cycle: SGS, coarse correction through the element
