@@ -1043,20 +1043,22 @@ int step_once(ice_stepper* S, double* u, hipStream_t st) {
             axpy(1.0 / 6.0, k1, u);
             axpy(2.0 / 3.0, k1, u);
         } else {
+            // ssp_rk3.hpp:164-190; each stage combination in one pass:
+            // u1 = u + dt k1, u2 = u + dt (k1 + k2)/4, u = u + dt (k1/6 + k2/6 + 2 k3/3)
+            auto combine = [&](int nk, const double* const* kk, const double* c, double* out) {
+                rc |= ice::launch_rk_combine(n, u, nk, kk, c, dt, S->dte, per, S->st, out, st);
+            };
+            const double* k12[3] = {k1, k2, k3};
+            const double c1[1] = {1.0}, c2[2] = {0.25, 0.25}, c3[3] = {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0};
             R(u);
             Minv(k1);
-            copy(u, us);
-            axpy(1.0, k1, us);
+            combine(1, k12, c1, us);
             R(us);
             Minv(k2);
-            copy(u, us);
-            axpy(0.25, k1, us);
-            axpy(0.25, k2, us);
+            combine(2, k12, c2, us);
             R(us);
             Minv(k3);
-            axpy(1.0 / 6.0, k1, u);
-            axpy(1.0 / 6.0, k2, u);
-            axpy(2.0 / 3.0, k3, u);
+            combine(3, k12, c3, u);
         }
         break;
     case ICE_STEP_RK3_TVD: {

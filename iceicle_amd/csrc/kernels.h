@@ -86,6 +86,8 @@ int launch_minv(int ndim, int p, int neq, const DevSpace& S, const double* r, do
 int launch_axpy_dt(size_t n, double coef, const double* dtp, const double* x, double* y, hipStream_t st);
 int launch_axpy_edt(size_t n, double coef, const double* dte, size_t per, const StepState* stp, const double* x,
                     double* y, hipStream_t st);
+int launch_rk_combine(size_t n, const double* base, int nk, const double* const* k, const double* c, const double* dtp,
+                      const double* dte, size_t per, const StepState* stp, double* out, hipStream_t st);
 int launch_axpby(size_t n, double a, const double* x, double b, double* y, hipStream_t st);
 int launch_step_begin(StepState* s, unsigned long long* wave, const StepControl& c, hipStream_t st);
 int launch_step_end(StepState* s, hipStream_t st);

@@ -178,6 +178,25 @@ __global__ void axpy_edt_kernel(size_t n, double coef, const double* __restrict_
         y[i] += coef * dte[i / per] * x[i];
 }
 
+// out = base + dt_e (c0 k0 + c1 k1 + c2 k2): one pass for a whole RK stage
+// combination (nk = 1..3 stage vectors), the step size read on the device
+// (the stepper's, or each element's; zero once the run is done).  out may
+// alias base.
+__global__ void rk_combine_kernel(size_t n, const double* __restrict__ base, int nk, const double* __restrict__ k0,
+                                  const double* __restrict__ k1, const double* __restrict__ k2, double c0, double c1,
+                                  double c2, const double* __restrict__ dtp, const double* __restrict__ dte,
+                                  size_t per, const StepState* __restrict__ stp, double* out) {
+    const bool done = stp && stp->done;
+    const double dts = dtp ? *dtp : 1.0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double dt = done ? 0.0 : (dte ? dte[i / per] : dts);
+        double s = c0 * k0[i];
+        if (nk > 1) s += c1 * k1[i];
+        if (nk > 2) s += c2 * k2[i];
+        out[i] = base[i] + dt * s;
+    }
+}
+
 // y = a x + b y  (axpby, fespan.hpp:312-330)
 __global__ void axpby_kernel(size_t n, double a, const double* __restrict__ x, double b, double* __restrict__ y) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -261,6 +280,14 @@ int launch_axpy_dt(size_t n, double coef, const double* dtp, const double* x, do
 int launch_axpy_edt(size_t n, double coef, const double* dte, size_t per, const StepState* stp, const double* x,
                     double* y, hipStream_t st) {
     axpy_edt_kernel<<<grid_for(n), 256, 0, st>>>(n, coef, dte, per, stp, x, y);
+    return (int)hipGetLastError();
+}
+
+int launch_rk_combine(size_t n, const double* base, int nk, const double* const* k, const double* c, const double* dtp,
+                      const double* dte, size_t per, const StepState* stp, double* out, hipStream_t st) {
+    rk_combine_kernel<<<grid_for(n), 256, 0, st>>>(n, base, nk, k[0], nk > 1 ? k[1] : nullptr, nk > 2 ? k[2] : nullptr,
+                                                    c[0], nk > 1 ? c[1] : 0.0, nk > 2 ? c[2] : 0.0, dtp, dte, per, stp,
+                                                    out);
     return (int)hipGetLastError();
 }
 
