@@ -294,6 +294,42 @@ __device__ __forceinline__ void face_trace_sf(const DevSpace& S, const VT* Ltab,
     }
 }
 
+// Trace record of one element at its tensor point pt on its face j (Ue the
+// element's coefficients, L the 1D value / derivative tables): u only when
+// the flux has no diffusive part, else as face_trace_sf.
+template <int D, int N, int M, class VT, bool GRAD>
+__device__ __forceinline__ void element_point_trace(const DevSpace& S, const VT* L, const VT* Ue, int j, const int* pt,
+                                                    const double (&X)[1 << D][D], const double* nrm, bool interior,
+                                                    double bh, VT* __restrict__ out) {
+    constexpr int NB = Pow<N, D>::value;
+    if constexpr (!GRAD) {
+        VT a0[D][N];
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+#pragma unroll
+            for (int i = 0; i < N; ++i) a0[d][i] = L[pt[d] * N + i];
+        VT u[M];
+#pragma unroll
+        for (int v = 0; v < M; ++v) u[v] = 0;
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            int id[D];
+            int r = i;
+#pragma unroll
+            for (int d = D - 1; d >= 0; --d) { id[d] = r % N; r /= N; }
+            VT c0 = 1;
+#pragma unroll
+            for (int d = 0; d < D; ++d) c0 *= a0[d][id[d]];
+#pragma unroll
+            for (int v = 0; v < M; ++v) u[v] += Ue[i * M + v] * c0;
+        }
+#pragma unroll
+        for (int v = 0; v < M; ++v) out[v] = u[v];
+    } else {
+        face_trace_sf<D, N, M, VT>(S, L, Ue, j, pt, X, nrm, interior, bh, out);
+    }
+}
+
 template <int D, int P, class PH>
 __global__ __launch_bounds__(256, ICE_TRACE_MINW) void trace_kernel(DevSpace S, typename PH::Params ph,
                                                                      const double* __restrict__ U,
@@ -348,38 +384,12 @@ __global__ __launch_bounds__(256, ICE_TRACE_MINW) void trace_kernel(DevSpace S, 
         }
     }
     __syncthreads();
-    const VT* L0 = sL;
-    VT* out = sOut + (size_t)threadIdx.x * NV * M;
-    const VT* Ue = sU + le * NB * M;
-    if (!active) {
-    } else if constexpr (!GRAD) {
-        VT a0[D][N];
-#pragma unroll
-        for (int d = 0; d < D; ++d)
-#pragma unroll
-            for (int i = 0; i < N; ++i) a0[d][i] = L0[pt[d] * N + i];
-        VT u[M];
-#pragma unroll
-        for (int v = 0; v < M; ++v) u[v] = 0;
-#pragma unroll
-        for (int i = 0; i < NB; ++i) {
-            int id[D];
-            int r = i;
-#pragma unroll
-            for (int d = D - 1; d >= 0; --d) { id[d] = r % N; r /= N; }
-            VT c0 = 1;
-#pragma unroll
-            for (int d = 0; d < D; ++d) c0 *= a0[d][id[d]];
-#pragma unroll
-            for (int v = 0; v < M; ++v) u[v] += Ue[i * M + v] * c0;
-        }
-#pragma unroll
-        for (int v = 0; v < M; ++v) out[v] = u[v];
-    } else {
+    if (active) {
         const bool interior = ef >= 0 && (ef >> 4) < S.nfi;
         const int side = (ef >> 3) & 1;
         const double bh = (side ? 1.0 : -1.0) * (double)ph.beta1 * h;
-        face_trace_sf<D, N, M, VT>(S, sL, Ue, j, pt, X, nrm, interior, bh, out);
+        element_point_trace<D, N, M, VT, GRAD>(S, sL, sU + le * NB * M, j, pt, X, nrm, interior, bh,
+                                               sOut + (size_t)threadIdx.x * NV * M);
     }
     __syncthreads();
     // records of elements e0 .. e0 + ne - 1 are one contiguous range of T
@@ -388,14 +398,145 @@ __global__ __launch_bounds__(256, ICE_TRACE_MINW) void trace_kernel(DevSpace S, 
     for (int idx = threadIdx.x; idx < ne * TPE * NV * M; idx += blockDim.x) dst[idx] = sOut[idx];
 }
 
+// Numerical flux (and DDGIC lifting vector) at point q of interior face f
+// from the two trace records of the point (left element tL, right tR), ph
+// the left element's flux parameters.
+template <int D, class PH>
+__device__ __forceinline__ void interior_face_point(const DevSpace& S, const typename PH::Params& ph, const DiscDev& dd,
+                                                    int f, int q, int NFQ, const typename PH::real* tL,
+                                                    const typename PH::real* tR) {
+    using VT = typename PH::real;
+    constexpr int M = PH::M;
+    constexpr int NV = TraceRec<PH, D>::NV;
+    VT* const Qb = reinterpret_cast<VT*>(dd.Q);
+    VT* const Pb = reinterpret_cast<VT*>(dd.Pic);
+    const double* g = S.fgeo + ((size_t)f * NFQ + q) * (D + 2);
+    VT nrm[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) nrm[d] = (VT)g[d];
+    const VT ws = (VT)g[D], h = (VT)g[D + 1];
+    VT uL[M], uR[M], gs[M * D];
+#pragma unroll
+    for (int v = 0; v < M; ++v) {
+        uL[v] = tL[v];
+        uR[v] = tR[v];
+    }
+    if constexpr (NV > 1) {
+#pragma unroll
+        for (int v = 0; v < M; ++v)
+#pragma unroll
+            for (int d = 0; d < D; ++d) gs[v * D + d] = tL[(1 + d) * M + v] + tR[(1 + d) * M + v];
+    } else {
+#pragma unroll
+        for (int k = 0; k < M * D; ++k) gs[k] = VT(0);
+    }
+    VT Qv[M];
+    interior_point_flux_split<D, PH>(ph, uL, uR, gs, nrm, h, ws, Qv);
+    VT* Qf = Qb + ((size_t)f * NFQ + q) * M;
+#pragma unroll
+    for (int v = 0; v < M; ++v) Qf[v] = Qv[v];
+    if constexpr (PH::HAS_HOMOGENEITY) {
+        if (dd.Pic) {
+            VT uavg[M], G[M * D * M * D];
+#pragma unroll
+            for (int v = 0; v < M; ++v) uavg[v] = VT(0.5) * (uL[v] + uR[v]);
+            PH::homogeneity(ph, uavg, G);
+            VT* Pf = Pb + ((size_t)f * NFQ + q) * M * D;
+#pragma unroll
+            for (int v = 0; v < M; ++v)
+#pragma unroll
+                for (int s = 0; s < D; ++s) {
+                    VT acc = 0;
+#pragma unroll
+                    for (int k = 0; k < D; ++k)
+#pragma unroll
+                        for (int r = 0; r < M; ++r)
+                            acc += G[((v * D + k) * M + r) * D + s] * nrm[k] * (uR[r] - uL[r]);
+                    Pf[v * D + s] = VT(0.5) * ph.sigma_ic * acc * ws;
+                }
+        }
+    }
+}
+
+// Numerical flux (and DDGIC lifting vector) at point q of boundary face b
+// from the interior element's trace record tL (not read on Neumann faces).
+template <int D, class PH>
+__device__ __forceinline__ void boundary_face_point(const DevSpace& S, const typename PH::Params& ph, const DiscDev& dd,
+                                                    int b, int q, int NFQ, int bctype, int bcflag,
+                                                    const typename PH::real* tL) {
+    using VT = typename PH::real;
+    constexpr int M = PH::M;
+    constexpr int NV = TraceRec<PH, D>::NV;
+    VT* const Qb = reinterpret_cast<VT*>(dd.Q);
+    VT* const Pb = reinterpret_cast<VT*>(dd.Pic);
+    const size_t slot = (size_t)S.nfi + b;
+    VT* Qf = Qb + (slot * NFQ + q) * M;
+    VT* Pf = nullptr;
+    if constexpr (PH::HAS_HOMOGENEITY) {
+        if (dd.Pic) {
+            Pf = Pb + (slot * NFQ + q) * M * D;
+#pragma unroll
+            for (int k = 0; k < M * D; ++k) Pf[k] = VT(0);
+        }
+    }
+    const bool general = !(bctype == ICE_BC_DIRICHLET || bctype == ICE_BC_NEUMANN ||
+                           bctype == ICE_BC_EXTRAPOLATION || bctype == ICE_BC_SPACETIME_FUTURE);
+    if (general && !PH::IMPLEMENTS_BCS) {
+#pragma unroll
+        for (int v = 0; v < M; ++v) Qf[v] = VT(0);
+        return;
+    }
+    const double* g = S.fgeo + (slot * NFQ + q) * (D + 2);
+    VT nrm[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) nrm[d] = (VT)g[d];
+    const VT ws = (VT)g[D], hb = (VT)g[D + 1];
+    VT uL[M] = {}, gL[M * D] = {};
+    if (bctype != ICE_BC_NEUMANN) {
+#pragma unroll
+        for (int v = 0; v < M; ++v) uL[v] = tL[v];
+        if constexpr (NV > 1) {
+#pragma unroll
+            for (int v = 0; v < M; ++v)
+#pragma unroll
+                for (int d = 0; d < D; ++d) gL[v * D + d] = tL[(1 + d) * M + v];
+        }
+    }
+    VT bv[M];
+#pragma unroll
+    for (int v = 0; v < M; ++v) bv[v] = (VT)dd.bval[((size_t)b * NFQ + q) * M + v];
+    VT Qv[M], uB[M];
+    boundary_point_flux<D, PH>(ph, bctype, bcflag, uL, gL, bv, nrm, hb, ws, Qv, uB);
+#pragma unroll
+    for (int v = 0; v < M; ++v) Qf[v] = Qv[v];
+    if constexpr (PH::HAS_HOMOGENEITY) {
+        if (Pf && bctype == ICE_BC_DIRICHLET) {
+            VT uavg[M], G[M * D * M * D];
+#pragma unroll
+            for (int v = 0; v < M; ++v) uavg[v] = VT(0.5) * (uL[v] + uB[v]);
+            PH::homogeneity(ph, uavg, G);
+#pragma unroll
+            for (int v = 0; v < M; ++v)
+#pragma unroll
+                for (int s = 0; s < D; ++s) {
+                    VT acc = 0;
+#pragma unroll
+                    for (int k = 0; k < D; ++k)
+#pragma unroll
+                        for (int r = 0; r < M; ++r)
+                            acc += G[((v * D + k) * M + r) * D + s] * nrm[k] * (uB[r] - uL[r]);
+                    Pf[v * D + s] = ph.sigma_ic * acc * ws;
+                }
+        }
+    }
+}
+
 // Numerical fluxes at face points from stored traces: interior points first
 // (blocks [0, nbi)), then boundary points.
 template <int D, int P, class PH>
 __global__ __launch_bounds__(256, ICE_FLUX_MINW) void flux_kernel(DevSpace S, typename PH::Params ph0, DiscDev dd,
                                                                    const typename PH::real* __restrict__ T, int nbi) {
     using VT = typename PH::real;
-    VT* const Qb = reinterpret_cast<VT*>(dd.Q);
-    VT* const Pb = reinterpret_cast<VT*>(dd.Pic);
     constexpr int N = P + 1;
     constexpr int NFQ = Pow<N, D - 1>::value;
     constexpr int M = PH::M;
@@ -406,126 +547,19 @@ __global__ __launch_bounds__(256, ICE_FLUX_MINW) void flux_kernel(DevSpace S, ty
         if (f >= S.nfi) return;
         const int4 rec = S.fint[f];
         const int eL = rec.x, eR = rec.y, info = rec.z;
-        const typename PH::Params ph = local_params(ph0, dd.emu, eL);
         const int fnl = info_fnl(info), fnr = info_fnr(info), orient = info_orient(info);
         const int qr = S.fperm[(fnr * S.norient + orient) * NFQ + q];
         const VT* tL = T + (((size_t)eL * 2 * D + fnl) * NFQ + q) * NV * M;
         const VT* tR = T + (((size_t)eR * 2 * D + fnr) * NFQ + qr) * NV * M;
-        const double* g = S.fgeo + ((size_t)f * NFQ + q) * (D + 2);
-        VT nrm[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) nrm[d] = (VT)g[d];
-        const VT ws = (VT)g[D], h = (VT)g[D + 1];
-        VT uL[M], uR[M], gs[M * D];
-#pragma unroll
-        for (int v = 0; v < M; ++v) {
-            uL[v] = tL[v];
-            uR[v] = tR[v];
-        }
-        if constexpr (NV > 1) {
-#pragma unroll
-            for (int v = 0; v < M; ++v)
-#pragma unroll
-                for (int d = 0; d < D; ++d) gs[v * D + d] = tL[(1 + d) * M + v] + tR[(1 + d) * M + v];
-        } else {
-#pragma unroll
-            for (int k = 0; k < M * D; ++k) gs[k] = VT(0);
-        }
-        VT Qv[M];
-        interior_point_flux_split<D, PH>(ph, uL, uR, gs, nrm, h, ws, Qv);
-        VT* Qf = Qb + ((size_t)f * NFQ + q) * M;
-#pragma unroll
-        for (int v = 0; v < M; ++v) Qf[v] = Qv[v];
-        if constexpr (PH::HAS_HOMOGENEITY) {
-            if (dd.Pic) {
-                VT uavg[M], G[M * D * M * D];
-#pragma unroll
-                for (int v = 0; v < M; ++v) uavg[v] = VT(0.5) * (uL[v] + uR[v]);
-                PH::homogeneity(ph, uavg, G);
-                VT* Pf = Pb + ((size_t)f * NFQ + q) * M * D;
-#pragma unroll
-                for (int v = 0; v < M; ++v)
-#pragma unroll
-                    for (int s = 0; s < D; ++s) {
-                        VT acc = 0;
-#pragma unroll
-                        for (int k = 0; k < D; ++k)
-#pragma unroll
-                            for (int r = 0; r < M; ++r)
-                                acc += G[((v * D + k) * M + r) * D + s] * nrm[k] * (uR[r] - uL[r]);
-                        Pf[v * D + s] = VT(0.5) * ph.sigma_ic * acc * ws;
-                    }
-            }
-        }
+        interior_face_point<D, PH>(S, local_params(ph0, dd.emu, eL), dd, f, q, NFQ, tL, tR);
     } else {
         const int tid = (blockIdx.x - nbi) * blockDim.x + threadIdx.x;
         const int b = tid / NFQ, q = tid - b * NFQ;
         if (b >= S.nfb) return;
         const int4 rec = S.fbdy[b];
-        const int eL = rec.x, info = rec.y, bcflag = rec.z;
-        const typename PH::Params ph = local_params(ph0, dd.emu, eL);
-        const int fnl = info_fnl(info), bctype = info_bctype(info);
-        const size_t slot = (size_t)S.nfi + b;
-        VT* Qf = Qb + (slot * NFQ + q) * M;
-        VT* Pf = nullptr;
-        if constexpr (PH::HAS_HOMOGENEITY) {
-            if (dd.Pic) {
-                Pf = Pb + (slot * NFQ + q) * M * D;
-#pragma unroll
-                for (int k = 0; k < M * D; ++k) Pf[k] = VT(0);
-            }
-        }
-        const bool general = !(bctype == ICE_BC_DIRICHLET || bctype == ICE_BC_NEUMANN ||
-                               bctype == ICE_BC_EXTRAPOLATION || bctype == ICE_BC_SPACETIME_FUTURE);
-        if (general && !PH::IMPLEMENTS_BCS) {
-#pragma unroll
-            for (int v = 0; v < M; ++v) Qf[v] = VT(0);
-            return;
-        }
-        const double* g = S.fgeo + (slot * NFQ + q) * (D + 2);
-        VT nrm[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) nrm[d] = (VT)g[d];
-        const VT ws = (VT)g[D], hb = (VT)g[D + 1];
-        VT uL[M] = {}, gL[M * D] = {};
-        if (bctype != ICE_BC_NEUMANN) {
-            const VT* tL = T + (((size_t)eL * 2 * D + fnl) * NFQ + q) * NV * M;
-#pragma unroll
-            for (int v = 0; v < M; ++v) uL[v] = tL[v];
-            if constexpr (NV > 1) {
-#pragma unroll
-                for (int v = 0; v < M; ++v)
-#pragma unroll
-                    for (int d = 0; d < D; ++d) gL[v * D + d] = tL[(1 + d) * M + v];
-            }
-        }
-        VT bv[M];
-#pragma unroll
-        for (int v = 0; v < M; ++v) bv[v] = (VT)dd.bval[((size_t)b * NFQ + q) * M + v];
-        VT Qv[M], uB[M];
-        boundary_point_flux<D, PH>(ph, bctype, bcflag, uL, gL, bv, nrm, hb, ws, Qv, uB);
-#pragma unroll
-        for (int v = 0; v < M; ++v) Qf[v] = Qv[v];
-        if constexpr (PH::HAS_HOMOGENEITY) {
-            if (Pf && bctype == ICE_BC_DIRICHLET) {
-                VT uavg[M], G[M * D * M * D];
-#pragma unroll
-                for (int v = 0; v < M; ++v) uavg[v] = VT(0.5) * (uL[v] + uB[v]);
-                PH::homogeneity(ph, uavg, G);
-#pragma unroll
-                for (int v = 0; v < M; ++v)
-#pragma unroll
-                    for (int s = 0; s < D; ++s) {
-                        VT acc = 0;
-#pragma unroll
-                        for (int k = 0; k < D; ++k)
-#pragma unroll
-                            for (int r = 0; r < M; ++r)
-                                acc += G[((v * D + k) * M + r) * D + s] * nrm[k] * (uB[r] - uL[r]);
-                        Pf[v * D + s] = ph.sigma_ic * acc * ws;
-                    }
-            }
-        }
+        const int eL = rec.x, info = rec.y;
+        const VT* tL = T + (((size_t)eL * 2 * D + info_fnl(info)) * NFQ + q) * NV * M;
+        boundary_face_point<D, PH>(S, local_params(ph0, dd.emu, eL), dd, b, q, NFQ, info_bctype(info), rec.z, tL);
     }
 }
 
