@@ -745,6 +745,7 @@ int ice_disc_set_tile(ice_disc* d, int tile) {
     if (!d) return fail(ICE_ERR_ARG, "null argument");
     if (tile < 0 || tile > 2) return fail(ICE_ERR_ARG, "tile must be 0, 1 or 2");
     d->dd.tile = tile;
+    d->dd.tile_mixed = tile;
     return ICE_OK;
 }
 
@@ -759,34 +760,44 @@ int ice_disc_autotune(ice_disc* d, const double* u, double* r, int reps, int* be
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         return fail(ICE_ERR_HIP, "event create failed");
-    int pick = 0;
-    double tbest = 1e300;
+    int pick = 0, pick_mixed = 0;
     int rc = ICE_OK;
-    for (int tile = 0; tile < 3 && rc == ICE_OK; ++tile) {
-        d->dd.tile = tile;
-        float t = 0.f;
-        bool ok = true;
-        for (int k = 0; k < 2 && ok; ++k)
-            ok = ice::residual_fp64(s->ndim, s->p, d->prm.flux, s->dev, d->ph, d->dd, u, r, st, nullptr) == 0;
-        ok = ok && hipEventRecord(e0, st) == hipSuccess;
-        for (int k = 0; k < reps && ok; ++k)
-            ok = ice::residual_fp64(s->ndim, s->p, d->prm.flux, s->dev, d->ph, d->dd, u, r, st, nullptr) == 0;
-        ok = ok && hipEventRecord(e1, st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
-             hipEventElapsedTime(&t, e0, e1) == hipSuccess;
-        if (!ok) {
-            rc = fail(ICE_ERR_HIP, "autotune timing failed");
-            break;
-        }
-        const double per = (double)t / reps;
-        if (ms) ms[tile] = per;
-        if (per < tbest) {
-            tbest = per;
-            pick = tile;
+    // FP64 residual, then the mixed one (its own tile: FP32 tiles hold half
+    // the bytes, so the fastest shape can differ)
+    for (int prec = 0; prec < 2 && rc == ICE_OK; ++prec) {
+        double tbest = 1e300;
+        for (int tile = 0; tile < 3 && rc == ICE_OK; ++tile) {
+            d->dd.tile = tile;
+            d->dd.tile_mixed = tile;
+            auto once = [&]() {
+                return (prec == 0 ? ice::residual_fp64(s->ndim, s->p, d->prm.flux, s->dev, d->ph, d->dd, u, r, st,
+                                                       nullptr)
+                                  : ice::residual_mixed(s->ndim, s->p, d->prm.flux, s->dev, d->ph, d->dd, u, r, st,
+                                                        nullptr)) == 0;
+            };
+            float t = 0.f;
+            bool ok = true;
+            for (int k = 0; k < 2 && ok; ++k) ok = once();
+            ok = ok && hipEventRecord(e0, st) == hipSuccess;
+            for (int k = 0; k < reps && ok; ++k) ok = once();
+            ok = ok && hipEventRecord(e1, st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+                 hipEventElapsedTime(&t, e0, e1) == hipSuccess;
+            if (!ok) {
+                rc = fail(ICE_ERR_HIP, "autotune timing failed");
+                break;
+            }
+            const double per = (double)t / reps;
+            if (ms && prec == 0) ms[tile] = per;
+            if (per < tbest) {
+                tbest = per;
+                (prec == 0 ? pick : pick_mixed) = tile;
+            }
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     d->dd.tile = pick;
+    d->dd.tile_mixed = pick_mixed;
     if (best) *best = pick;
     return rc;
 }

@@ -47,6 +47,7 @@ struct DiscDev {
     const double* srcload;    // [nelem][nbasis][neq] source load vector, or null
     const double* emu;        // [nelem] per-element viscosity (batched Burgers instances), or null
     int tile;                 // element-kernel workgroup: 0 = 256 threads, 1 = 128, 2 = 64 (whole elements)
+    int tile_mixed;           // the same for the mixed-precision residual
     double* Q;                // [nfi + nfb][nfq][neq] weighted normal face fluxes
     double* Pic;              // [nfi + nfb][nfq][neq][ndim] DDGIC lifting vectors, or null
     double* T;                // [nelem][2*ndim][nfq][nv][neq] element face traces

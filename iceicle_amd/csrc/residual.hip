@@ -917,10 +917,16 @@ __global__ __launch_bounds__((ElemTile<2, P, TPB>::BLOCK), ICE_ELEM_MINW) void e
     constexpr int NFQ = N;
     constexpr int M = PH::M;
     using VT = typename PH::real;
-    // the staged coefficients are dead after phase 1a, so the reference-frame
-    // fluxes of phase 1b reuse their storage
+    // The staged coefficients are dead after phase 1a, so the reference-frame
+    // fluxes G of phase 1b reuse their storage sUG.  FP64 (LOWLDS): sUG also
+    // holds the phase-1a derivative contractions T1 (second half) until phase
+    // 1b has read them, sT holds T0 then S0, the second test contraction is
+    // summed from G in phase 2b and the source load is read from global
+    // memory (34 KB for a 16-element P3 workgroup instead of 50 KB).  FP32:
+    // sT holds T0, T1 then S0, S1 and the source load is staged.
+    constexpr bool LOWLDS = std::is_same<VT, double>::value;
     __shared__ VT sUG[EPB * NB * M * D];
-    __shared__ VT sT[EPB * 2 * NB * M];
+    __shared__ VT sT[EPB * (LOWLDS ? 1 : 2) * NB * M];
     VT* sU = sUG;
     VT* sG = sUG;
     // face fluxes in each face's tensor order, vertices, face slots, FP64
@@ -934,7 +940,7 @@ __global__ __launch_bounds__((ElemTile<2, P, TPB>::BLOCK), ICE_ELEM_MINW) void e
     __shared__ int sEF[EPB * 2 * D];
     __shared__ double sLd[NT2];
     __shared__ signed char sFT[2 * D * 2 * NFQ];  // two orientation codes in 2D
-    __shared__ double sSL[EPB * NB * M];
+    __shared__ double sSL[LOWLDS ? 1 : EPB * NB * M];
     const int e0 = blockIdx.x * EPB;
     const int le = threadIdx.x / NB;
     const int k = threadIdx.x - le * NB;
@@ -947,7 +953,7 @@ __global__ __launch_bounds__((ElemTile<2, P, TPB>::BLOCK), ICE_ELEM_MINW) void e
         const size_t total = (size_t)S.nelem * NB * M;
         for (int idx = threadIdx.x; idx < EPB * NB * M; idx += blockDim.x)
             sU[idx] = (VT)((base + idx < total) ? U[base + idx] : 0.0);
-        if (srcload)
+        if (!LOWLDS && srcload)
             for (int idx = threadIdx.x; idx < EPB * NB * M; idx += blockDim.x)
                 sSL[idx] = (base + idx < total) ? srcload[base + idx] : 0.0;
         const size_t xb = (size_t)e0 * 4 * D, xt = (size_t)S.nelem * 4 * D;
@@ -977,8 +983,8 @@ __global__ __launch_bounds__((ElemTile<2, P, TPB>::BLOCK), ICE_ELEM_MINW) void e
     const double* wgl = S.tab + (N + 2) + 3 * (N + 2) * N;
     const VT* L0 = tab_of<VT>(S) + (N + 2);
     const VT* L1 = L0 + (N + 2) * N;
-    VT* T0 = sT + (size_t)le * 2 * NB * M;  // [a][q1][v]
-    VT* T1 = T0 + NB * M;
+    VT* T0 = sT + (size_t)le * (LOWLDS ? 1 : 2) * NB * M;  // [a][q1][v]
+    VT* T1 = LOWLDS ? sUG + (size_t)EPB * NB * M + (size_t)le * NB * M : T0 + NB * M;
     // phase 1a: (a = k0, q1 = k1)
     if (active) {
         const VT* Ua = sU + ((size_t)le * NB + k0 * N) * M;
@@ -1014,6 +1020,7 @@ __global__ __launch_bounds__((ElemTile<2, P, TPB>::BLOCK), ICE_ELEM_MINW) void e
             for (int d = 0; d < D; ++d) X[vv][d] = sX[(lec * 4 + vv) * D + d];
     }
     double lam_w = 0.0, vis_w = 0.0;
+    VT gq[M * D];  // this point's reference-frame fluxes (FP64: stored once T1 is dead)
     // phase 1b: (q0 = k0, q1 = k1)
     if (active) {
         double xi[D] = {xs[k0], xs[k1]};
@@ -1056,7 +1063,6 @@ __global__ __launch_bounds__((ElemTile<2, P, TPB>::BLOCK), ICE_ELEM_MINW) void e
         }
         const double fac = detc * wq * idetg;
         const VT ga[D][D] = {{(VT)(A[0][0] * fac), (VT)(A[0][1] * fac)}, {(VT)(A[1][0] * fac), (VT)(A[1][1] * fac)}};
-        VT* Gq = sG + ((size_t)le * NB + k) * M * D;
 #pragma unroll
         for (int v = 0; v < M; ++v)
 #pragma unroll
@@ -1064,8 +1070,13 @@ __global__ __launch_bounds__((ElemTile<2, P, TPB>::BLOCK), ICE_ELEM_MINW) void e
                 VT sacc = 0;
 #pragma unroll
                 for (int j = 0; j < D; ++j) sacc += F[v * D + j] * ga[a][j];
-                Gq[v * D + a] = sacc;
+                gq[v * D + a] = sacc;
             }
+    }
+    if (!LOWLDS && active) {
+        VT* Gq = sG + ((size_t)le * NB + k) * M * D;
+#pragma unroll
+        for (int v = 0; v < M * D; ++v) Gq[v] = gq[v];
     }
     __shared__ double red[2 * 4];
     if (wave) {
@@ -1079,13 +1090,29 @@ __global__ __launch_bounds__((ElemTile<2, P, TPB>::BLOCK), ICE_ELEM_MINW) void e
         }
     }
     __syncthreads();
+    if constexpr (LOWLDS) {
+        if (active) {
+            VT* Gq = sG + ((size_t)le * NB + k) * M * D;
+#pragma unroll
+            for (int v = 0; v < M * D; ++v) Gq[v] = gq[v];
+        }
+        __syncthreads();
+    }
     if (wave && threadIdx.x == 0) {
         double a = 0.0, b = 0.0;
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { a = fmax(a, red[2 * w]); b = fmax(b, red[2 * w + 1]); }
         wave_part[2 * blockIdx.x] = a;
         wave_part[2 * blockIdx.x + 1] = b;
     }
-    // phase 2a: (q0 = k0, i1 = k1); T0/T1 reused as S0/S1 [q0][i1][v]
+    // the source load (FP64: read now, subtracted in phase 2b)
+    double sl[M];
+#pragma unroll
+    for (int v = 0; v < M; ++v)
+        sl[v] = !srcload       ? 0.0
+                : !LOWLDS      ? sSL[(le * NB + k) * M + v]
+                : active       ? srcload[((size_t)e * NB + k) * M + v]
+                               : 0.0;
+    // phase 2a: (q0 = k0, i1 = k1); T0 (and T1 in FP32) reused as S0 (S1) [q0][i1][v]
     if (active) {
         VT l0[N], l1[N];
 #pragma unroll
@@ -1100,10 +1127,10 @@ __global__ __launch_bounds__((ElemTile<2, P, TPB>::BLOCK), ICE_ELEM_MINW) void e
 #pragma unroll
             for (int q1 = 0; q1 < N; ++q1) {
                 s0 += Gr[(q1 * M + v) * D + 0] * l0[q1];
-                s1 += Gr[(q1 * M + v) * D + 1] * l1[q1];
+                if constexpr (!LOWLDS) s1 += Gr[(q1 * M + v) * D + 1] * l1[q1];
             }
             T0[k * M + v] = s0;
-            T1[k * M + v] = s1;
+            if constexpr (!LOWLDS) T1[k * M + v] = s1;
         }
     }
     __syncthreads();
@@ -1117,18 +1144,31 @@ __global__ __launch_bounds__((ElemTile<2, P, TPB>::BLOCK), ICE_ELEM_MINW) void e
             l0[q0] = L0[q0 * N + k0];
             l1[q0] = L1[q0 * N + k0];
         }
+        // FP64: S1[q0][i1] = sum_q1 G1[q0][q1] L1[q1][i1], summed here from G
+        VT m1[N];
+#pragma unroll
+        for (int q1 = 0; q1 < N; ++q1) m1[q1] = L1[q1 * N + k1];
+        const VT* Ge = sG + (size_t)le * NB * M * D;
 #pragma unroll
         for (int v = 0; v < M; ++v) {
             VT sacc = 0;
 #pragma unroll
-            for (int q0 = 0; q0 < N; ++q0)
-                sacc += T0[(q0 * N + k1) * M + v] * l1[q0] + T1[(q0 * N + k1) * M + v] * l0[q0];
+            for (int q0 = 0; q0 < N; ++q0) {
+                VT s1 = 0;
+                if constexpr (LOWLDS) {
+#pragma unroll
+                    for (int q1 = 0; q1 < N; ++q1) s1 += Ge[((q0 * N + q1) * M + v) * D + 1] * m1[q1];
+                } else {
+                    s1 = T1[(q0 * N + k1) * M + v];
+                }
+                sacc += T0[(q0 * N + k1) * M + v] * l1[q0] + s1 * l0[q0];
+            }
             r[v] = sacc;
         }
     }
     if (srcload) {
 #pragma unroll
-        for (int v = 0; v < M; ++v) r[v] -= sSL[(le * NB + k) * M + v];
+        for (int v = 0; v < M; ++v) r[v] -= sl[v];
     }
     const int kd[D] = {k0, k1};
     // lifting of the faces this thread's node lies on, from the staged
@@ -1899,7 +1939,7 @@ int launch_residual(const DevSpace& S, const typename PH::Params& ph, const Disc
                     S, ph, d.emu, d.srcload, U, reinterpret_cast<const VT*>(d.Q), reinterpret_cast<const VT*>(d.Pic), R,
                     d.wave, d.wave_part);
         };
-        switch (d.tile) {
+        switch (std::is_same<VT, double>::value ? d.tile : d.tile_mixed) {
         case 1: go(std::integral_constant<int, tile_tpb(1)>{}); break;
         case 2: go(std::integral_constant<int, tile_tpb(2)>{}); break;
         default: go(std::integral_constant<int, (D == 3 ? ICE_ELEM3_TPB : 256)>{}); break;

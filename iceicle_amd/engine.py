@@ -217,8 +217,10 @@ class ConservationLawDDG:
         check(lib().ice_disc_set_tile(self.h, int(tile)))
 
     def autotune(self, u, r, reps=10, stream=None):
-        """Time the element-kernel tiles on this state, keep the fastest;
-        returns (best tile, [ms per residual for tiles 0, 1, 2])."""
+        """Time the element-kernel tiles on this state and keep the fastest,
+        for the FP64 residual and (2D, 3D) separately for the mixed one;
+        returns the FP64 residual's (best tile, [ms per residual for tiles
+        0, 1, 2])."""
         _require_cuda(u, "u")
         _require_cuda(r, "r")
         best = C.c_int()

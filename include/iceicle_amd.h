@@ -237,10 +237,13 @@ int ice_disc_set_element_viscosity(ice_disc* disc, const double* mu);
 /* Element-kernel workgroup shape of this discretization's residual (its p,
  * number of variables and dimension are fixed): tile 0 = whole elements in
  * 256 threads (default), 1 = 128, 2 = 64.  Results are identical; only speed
- * changes.  ice_disc_autotune times `reps` FP64 residuals at u for each tile
- * on the stream, keeps the fastest (*best) and writes the per-residual
- * milliseconds to ms[3] (either may be null); r is overwritten.
- * Synchronous.  No reference counterpart. */
+ * changes.  ice_disc_set_tile sets the shape of both the FP64 and the
+ * mixed-precision residual.  ice_disc_autotune times `reps` FP64 residuals at
+ * u for each tile on the stream, keeps the fastest (*best) and writes the
+ * per-residual milliseconds to ms[3] (either may be null); then, in 2D and
+ * 3D, times the mixed-precision residual the same way and keeps its fastest
+ * tile for that residual.  r is overwritten.  Synchronous.  No reference
+ * counterpart. */
 int ice_disc_set_tile(ice_disc* disc, int tile);
 int ice_disc_autotune(ice_disc* disc, const double* u_dev, double* r_dev, int reps, int* best, double* ms,
                       void* stream);
