@@ -1051,6 +1051,50 @@ __global__ __launch_bounds__(RT_THREADS) void bsrT_kernel(int nelem, int bs, con
     y[g] = b ? b[g] - acc : acc;
 }
 
+// The same product with one wave per element block row, for block sizes bs
+// = VEC * RG with RG a power of two (VEC = the 16 bytes of one load in T
+// entries): lane (rg, cg) owns rows VEC rg .. VEC rg + VEC - 1 and columns
+// cg, cg + CG, ... (CG = 64 / RG), so each load instruction of the wave reads
+// CG whole consecutive columns of the column-major block; the partial sums
+// are reduced across the column groups with cross-lane shuffles.
+template <class T, class V, int NS>
+__global__ __launch_bounds__(256) void bsrT_wave_kernel(int nelem, int bs, int lrg, const int* __restrict__ cols,
+                                                        const T* __restrict__ JT, const V* __restrict__ x,
+                                                        const V* __restrict__ b, V* __restrict__ y) {
+    constexpr int VEC = 16 / (int)sizeof(T);
+    typedef T vec_t __attribute__((ext_vector_type(VEC)));
+    const int lane = threadIdx.x & 63;
+    const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (e >= nelem) return;  // uniform across the wave
+    const int rg = lane & ((1 << lrg) - 1), cg = lane >> lrg;
+    const int CG = 64 >> lrg;
+    V acc[VEC];
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) acc[k] = 0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int col = cols[(size_t)e * NS + s];
+        if (col < 0) continue;
+        const T* Jb = JT + ((size_t)e * NS + s) * bs * bs + rg * VEC;
+        const V* xv = x + (size_t)col * bs;
+#pragma unroll 4
+        for (int c = cg; c < bs; c += CG) {
+            const V xc = xv[c];
+            const vec_t jv = *reinterpret_cast<const vec_t*>(Jb + (size_t)c * bs);
+#pragma unroll
+            for (int k = 0; k < VEC; ++k) acc[k] += (V)jv[k] * xc;
+        }
+    }
+    for (int off = 1 << lrg; off < 64; off <<= 1)
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) acc[k] += __shfl_xor(acc[k], off);
+    if (cg == 0) {
+        const size_t g = (size_t)e * bs + rg * VEC;
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) y[g + k] = b ? b[g + k] - acc[k] : acc[k];
+    }
+}
+
 // y = D x with column-major dense blocks D[e][c][r]
 template <class T, class V>
 __global__ __launch_bounds__(RT_THREADS) void diagT_kernel(int nelem, int bs, const T* __restrict__ DT,
@@ -1083,6 +1127,87 @@ __global__ __launch_bounds__(RT_THREADS) void diagT_kernel(int nelem, int bs, co
 // workgroup owns whole elements (epb of them; its threads loop over their
 // rows), because the multiplication by the factored diagonal block needs the
 // element's complete partial-sum vector.
+// The same colour pass with one wave per element (block sizes bs = VEC RG,
+// RG a power of two, as bsrT_wave_kernel): the off-diagonal products with
+// 16-byte loads of whole columns, reduced across the column groups with
+// shuffles into the wave's partial-sum vector in LDS, then the product with
+// the factored diagonal block the same way.  Neighbour values are read from
+// y directly: the colour test excludes every element written in this pass.
+template <class T, class V, int NS, bool FWD, class TD>
+__global__ __launch_bounds__(256) void sgsT_wave_kernel(int nel, int bs, int lrg, int lrgd,
+                                                        const int* __restrict__ elems, const int* __restrict__ ecolor,
+                                                        int colour, const int* __restrict__ cols,
+                                                        const T* __restrict__ JT, const TD* __restrict__ DT,
+                                                        const V* __restrict__ rhs, V* __restrict__ y) {
+    constexpr int VEC = 16 / (int)sizeof(T);
+    constexpr int VECD = 16 / (int)sizeof(TD);
+    typedef T vec_t __attribute__((ext_vector_type(VEC)));
+    typedef TD vecd_t __attribute__((ext_vector_type(VECD)));
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    V* stv = reinterpret_cast<V*>(smem_raw) + (size_t)w * bs;  // this wave's partial sums
+    const int l = blockIdx.x * 4 + w;
+    const bool live = l < nel;  // uniform across the wave
+    const int e = live ? elems[l] : 0;
+    if (live) {
+        const int rg = lane & ((1 << lrg) - 1), cg = lane >> lrg;
+        const int CG = 64 >> lrg;
+        V acc[VEC];
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) acc[k] = 0;
+#pragma unroll
+        for (int s = 1; s < NS; ++s) {
+            const int col = cols[(size_t)e * NS + s];
+            if (col < 0 || !(FWD ? ecolor[col] < colour : ecolor[col] > colour)) continue;
+            const T* Jb = JT + ((size_t)e * NS + s) * bs * bs + rg * VEC;
+            const V* xv = y + (size_t)col * bs;
+#pragma unroll 4
+            for (int c = cg; c < bs; c += CG) {
+                const V xc = xv[c];
+                const vec_t jv = *reinterpret_cast<const vec_t*>(Jb + (size_t)c * bs);
+#pragma unroll
+                for (int k = 0; k < VEC; ++k) acc[k] += (V)jv[k] * xc;
+            }
+        }
+        for (int off = 1 << lrg; off < 64; off <<= 1)
+#pragma unroll
+            for (int k = 0; k < VEC; ++k) acc[k] += __shfl_xor(acc[k], off);
+        if (cg == 0) {
+#pragma unroll
+            for (int k = 0; k < VEC; ++k) {
+                const int r = rg * VEC + k;
+                stv[r] = FWD ? rhs[(size_t)e * bs + r] - acc[k] : acc[k];
+            }
+        }
+    }
+    __syncthreads();
+    if (live) {
+        const int rg = lane & ((1 << lrgd) - 1), cg = lane >> lrgd;
+        const int CG = 64 >> lrgd;
+        V acc[VECD];
+#pragma unroll
+        for (int k = 0; k < VECD; ++k) acc[k] = 0;
+        const TD* Db = DT + (size_t)e * bs * bs + rg * VECD;
+#pragma unroll 4
+        for (int c = cg; c < bs; c += CG) {
+            const V tc = stv[c];
+            const vecd_t dv = *reinterpret_cast<const vecd_t*>(Db + (size_t)c * bs);
+#pragma unroll
+            for (int k = 0; k < VECD; ++k) acc[k] += (V)dv[k] * tc;
+        }
+        for (int off = 1 << lrgd; off < 64; off <<= 1)
+#pragma unroll
+            for (int k = 0; k < VECD; ++k) acc[k] += __shfl_xor(acc[k], off);
+        if (cg == 0) {
+#pragma unroll
+            for (int k = 0; k < VECD; ++k) {
+                const size_t i = (size_t)e * bs + rg * VECD + k;
+                y[i] = FWD ? acc[k] : y[i] - acc[k];
+            }
+        }
+    }
+}
+
 template <class T, class V, int NS, bool FWD, class TD>
 __global__ __launch_bounds__(RT_THREADS) void sgsT_kernel(int nel, int epb, const int* __restrict__ elems,
                                                           const int* __restrict__ ecolor, int colour, int bs,
@@ -1163,6 +1288,19 @@ int span_elems(int bs) { return (RT_THREADS + bs - 2) / bs + 1; }
 template <class T, class V>
 int bsrT_apply(int nelem, int nslots, int bs, const int* cols, const T* JT, const V* x, const V* b, V* y,
                hipStream_t st) {
+    constexpr int VEC = 16 / (int)sizeof(T);
+    const int rgn = bs / VEC;
+    if (bs >= 16 && bs % VEC == 0 && rgn <= 64 && (rgn & (rgn - 1)) == 0) {
+        const int lrg = __builtin_ctz(rgn);
+        const int wgrid = (nelem + 3) / 4;
+        switch (nslots) {
+        case 3: bsrT_wave_kernel<T, V, 3><<<wgrid, 256, 0, st>>>(nelem, bs, lrg, cols, JT, x, b, y); break;
+        case 5: bsrT_wave_kernel<T, V, 5><<<wgrid, 256, 0, st>>>(nelem, bs, lrg, cols, JT, x, b, y); break;
+        case 7: bsrT_wave_kernel<T, V, 7><<<wgrid, 256, 0, st>>>(nelem, bs, lrg, cols, JT, x, b, y); break;
+        default: return -1;
+        }
+        return (int)hipGetLastError();
+    }
     const long ntot = (long)nelem * bs;
     const int grid = (int)((ntot + RT_THREADS - 1) / RT_THREADS);
     const size_t sh = (size_t)span_elems(bs) * nslots * bs * sizeof(V);
@@ -1193,6 +1331,33 @@ template <class T, class V, class TD>
 int sgsT_sweep(bool forward, int nel, const int* elems, const int* ecolor, int colour, int nslots, int bs,
                const int* cols, const T* JT, const TD* DT, const V* rhs, V* y, hipStream_t st) {
     if (nel <= 0) return 0;
+    {
+        constexpr int VEC = 16 / (int)sizeof(T), VECD = 16 / (int)sizeof(TD);
+        const int rgn = bs / VEC, rgd = bs / VECD;
+        auto pow2 = [](int v) { return v > 0 && v <= 64 && (v & (v - 1)) == 0; };
+        if (bs >= 16 && bs % VEC == 0 && bs % VECD == 0 && pow2(rgn) && pow2(rgd)) {
+            const int lrg = __builtin_ctz(rgn), lrgd = __builtin_ctz(rgd);
+            const int wgrid = (nel + 3) / 4;
+            const size_t wsh = (size_t)4 * bs * sizeof(V);
+#define ICE_SGSW(NSV)                                                                                           \
+    case NSV:                                                                                                   \
+        if (forward)                                                                                            \
+            sgsT_wave_kernel<T, V, NSV, true, TD><<<wgrid, 256, wsh, st>>>(nel, bs, lrg, lrgd, elems, ecolor,   \
+                                                                         colour, cols, JT, DT, rhs, y);        \
+        else                                                                                                    \
+            sgsT_wave_kernel<T, V, NSV, false, TD><<<wgrid, 256, wsh, st>>>(nel, bs, lrg, lrgd, elems, ecolor,  \
+                                                                          colour, cols, JT, DT, rhs, y);       \
+        break;
+            switch (nslots) {
+                ICE_SGSW(3)
+                ICE_SGSW(5)
+                ICE_SGSW(7)
+            default: return -1;
+            }
+#undef ICE_SGSW
+            return (int)hipGetLastError();
+        }
+    }
     const int epb = std::max(1, RT_THREADS / bs);
     const int grid = (nel + epb - 1) / epb;
     const size_t sh = (size_t)epb * (nslots + 1) * bs * sizeof(V);
