@@ -31,8 +31,10 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the FP64 residual kernels on the default workload
 # (128 x 128, P3), from rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by the
 # same counters on a 256 MiB copy (x2 and x1): profiles/residual_ns2d_p3_128_hbm_counters.md
-MEASURED_TRAFFIC_128_P3 = {"trace": (6893 * 2 + 24576) * 1024, "flux": (14758 * 2 + 4368) * 1024,
-                           "element": (11019 * 2 + 8192) * 1024}
+# (element kernel: per workgroup tile 0 / 1 / 2 = 256 / 128 / 64 threads)
+MEASURED_TRAFFIC_128_P3 = {"trace": (6893 * 2 + 24576) * 1024, "flux": (14738 * 2 + 4368) * 1024,
+                           "element": {0: (11040 * 2 + 8192) * 1024, 1: (11103 * 2 + 8192) * 1024,
+                                       2: int((11359.5 * 2 + 8192) * 1024)}}
 
 
 def parse():
@@ -56,6 +58,11 @@ def parse():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="only run 20 device copies of an MB-sized buffer (HBM counter calibration)")
     return ap.parse_args()
+
+
+def traffic_of(kernel, tile):
+    t = MEASURED_TRAFFIC_128_P3[kernel]
+    return t[tile] if isinstance(t, dict) else t
 
 
 def kernel_event_pass(engine, space, disc, u, r, precision, nres):
@@ -472,7 +479,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": (MEASURED_TRAFFIC_128_P3[dom] if (args.n, order) == (128, 3) else None),
+            "traffic": (traffic_of(dom, tile) if (args.n, order) == (128, 3) else None),
             "bytes_per_launch": int(kbytes),
         },
         "residual_roofline": {"bytes": int(b_alg), "achieved_GBs": round(res_gbs, 2),
