@@ -447,28 +447,33 @@ __device__ __forceinline__ bool boundary_point_flux(const typename PH::Params& p
         for (int v = 0; v < M; ++v) Q[v] = R(0);
         return false;
     }
-    if (bctype == ICE_BC_EXTRAPOLATION || bctype == ICE_BC_SPACETIME_FUTURE) {
-        PH::conv(ph, uL, uL, nrm, fadv);
-        PH::diff(ph, uL, gL, nrm, fvis);
+    // One call of each flux on value arrays for every boundary type:
+    // extrapolation takes uB = uL, the interior gradient and uL as the state
+    // by value selects (identical values to separate calls with uL and gL;
+    // calls with different array arguments per branch would be merged into
+    // one call through a pointer select and move the arrays to scratch).
+    const bool extrap = bctype == ICE_BC_EXTRAPOLATION || bctype == ICE_BC_SPACETIME_FUTURE;
+    if (bctype == ICE_BC_DIRICHLET) {
+#pragma unroll
+        for (int v = 0; v < M; ++v) uB[v] = bv[v];
+    } else if (extrap) {
 #pragma unroll
         for (int v = 0; v < M; ++v) uB[v] = uL[v];
     } else {
-        if (bctype == ICE_BC_DIRICHLET) {
-#pragma unroll
-            for (int v = 0; v < M; ++v) uB[v] = bv[v];
-        } else {
-            PH::apply_bc(ph, uL, gL, nrm, bctype, bcflag, uB);
-        }
-        PH::conv(ph, uL, uB, nrm, fadv);
-#pragma unroll
-        for (int v = 0; v < M; ++v) {
-            const R jump = uB[v] - uL[v];
-#pragma unroll
-            for (int d = 0; d < D; ++d) gddg[v * D + d] = ph.beta0_bdy * jump / hb * nrm[d] + gL[v * D + d];
-            uavg[v] = R(0.5) * (uL[v] + uB[v]);
-        }
-        PH::diff(ph, uavg, gddg, nrm, fvis);
+        PH::apply_bc(ph, uL, gL, nrm, bctype, bcflag, uB);
     }
+    PH::conv(ph, uL, uB, nrm, fadv);
+#pragma unroll
+    for (int v = 0; v < M; ++v) {
+        const R jump = uB[v] - uL[v];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const R gd = ph.beta0_bdy * jump / hb * nrm[d] + gL[v * D + d];
+            gddg[v * D + d] = extrap ? gL[v * D + d] : gd;
+        }
+        uavg[v] = extrap ? uL[v] : R(0.5) * (uL[v] + uB[v]);
+    }
+    PH::diff(ph, uavg, gddg, nrm, fvis);
 #pragma unroll
     for (int v = 0; v < M; ++v) Q[v] = fvis[v] * ws - fadv[v] * ws;
     if (uB_out) {

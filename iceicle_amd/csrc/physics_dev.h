@@ -375,7 +375,13 @@ struct NavierStokes {
             prim[0] = sL.rho;
 #pragma unroll
             for (int d = 0; d < D; ++d) prim[1 + d] = 0;
-            prim[D + 1] = p.isoT[bcflag & (ICE_MAX_BCFLAG - 1)];
+            // wall temperature by an unrolled select: a runtime index into
+            // the parameter copy would move the whole copy to scratch memory
+            const int wall = bcflag & (ICE_MAX_BCFLAG - 1);
+            R Tw = p.isoT[0];
+#pragma unroll
+            for (int k = 1; k < ICE_MAX_BCFLAG; ++k) Tw = (wall == k) ? p.isoT[k] : Tw;
+            prim[D + 1] = Tw;
             cons_from_rho_u_T(p, prim, uR);
         } else {  // RIEMANN (Carlson, NASA/TM-2011-217181 as in :1014-1125)
             const R g = p.gamma;

@@ -532,8 +532,10 @@ __device__ __forceinline__ void boundary_face_point(const DevSpace& S, const typ
 }
 
 // Numerical fluxes at face points from stored traces: interior points first
-// (blocks [0, nbi)), then boundary points.
-template <int D, int P, class PH>
+// (blocks [0, nbi)), then boundary points.  BDY = false compiles the boundary
+// branch out (meshes without boundary faces): the boundary-condition code
+// holds more registers than the interior flux (Euler: 75 against 50 VGPRs).
+template <int D, int P, class PH, bool BDY>
 __global__ __launch_bounds__(256, ICE_FLUX_MINW) void flux_kernel(DevSpace S, typename PH::Params ph0, DiscDev dd,
                                                                    const typename PH::real* __restrict__ T, int nbi) {
     using VT = typename PH::real;
@@ -552,7 +554,7 @@ __global__ __launch_bounds__(256, ICE_FLUX_MINW) void flux_kernel(DevSpace S, ty
         const VT* tL = T + (((size_t)eL * 2 * D + fnl) * NFQ + q) * NV * M;
         const VT* tR = T + (((size_t)eR * 2 * D + fnr) * NFQ + qr) * NV * M;
         interior_face_point<D, PH>(S, local_params(ph0, dd.emu, eL), dd, f, q, NFQ, tL, tR);
-    } else {
+    } else if constexpr (BDY) {
         const int tid = (blockIdx.x - nbi) * blockDim.x + threadIdx.x;
         const int b = tid / NFQ, q = tid - b * NFQ;
         if (b >= S.nfb) return;
@@ -1921,7 +1923,10 @@ int launch_residual(const DevSpace& S, const typename PH::Params& ph, const Disc
         trace_kernel<D, P, PH><<<(S.nelem + EPB - 1) / EPB, 256, 0, st>>>(S, ph, U, T);
     }
     if (ev) (void)hipEventRecord(ev[1], st);
-    if (nbi + nbb > 0) flux_kernel<D, P, PH><<<nbi + nbb, 256, 0, st>>>(S, ph, d, T, nbi);
+    if (nbb > 0)
+        flux_kernel<D, P, PH, true><<<nbi + nbb, 256, 0, st>>>(S, ph, d, T, nbi);
+    else if (nbi > 0)
+        flux_kernel<D, P, PH, false><<<nbi, 256, 0, st>>>(S, ph, d, T, nbi);
     if (ev) (void)hipEventRecord(ev[2], st);
     using TL = ElemTile<D, P>;
     int nblk = (S.nelem + TL::EPB - 1) / TL::EPB;
