@@ -1415,24 +1415,38 @@ __global__ __launch_bounds__(256) void convert_transpose_tile_kernel(int bs, con
 }
 
 // x = A^-1 b with the dense inverse stored column-major (the multigrid's
-// coarsest level): one thread per row, coalesced column reads
-__global__ __launch_bounds__(256) void dense_inverse_apply_kernel(int n, const double* __restrict__ inv,
-                                                                  const double* __restrict__ b,
-                                                                  double* __restrict__ x) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// coarsest level): a workgroup owns 64 rows, its 16 waves split the columns
+// (coalesced column reads, one lane per row), and the 16 partial sums of a
+// row are added in a fixed order from LDS
+__global__ __launch_bounds__(1024) void dense_inverse_apply_kernel(int n, const double* __restrict__ inv,
+                                                                   const double* __restrict__ b,
+                                                                   double* __restrict__ x) {
+    __shared__ double part[16][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
+    const int len = (n + 15) / 16;
+    const int j0 = w * len, j1 = min(n, j0 + len);
     double s0 = 0.0, s1 = 0.0;
-    int j = 0;
-    for (; j + 1 < n; j += 2) {
-        s0 += inv[(size_t)j * n + i] * b[j];
-        s1 += inv[(size_t)(j + 1) * n + i] * b[j + 1];
+    if (i < n) {
+        int j = j0;
+        for (; j + 1 < j1; j += 2) {
+            s0 += inv[(size_t)j * n + i] * b[j];
+            s1 += inv[(size_t)(j + 1) * n + i] * b[j + 1];
+        }
+        if (j < j1) s0 += inv[(size_t)j * n + i] * b[j];
     }
-    if (j < n) s0 += inv[(size_t)j * n + i] * b[j];
-    x[i] = s0 + s1;
+    part[w][lane] = s0 + s1;
+    __syncthreads();
+    if (w == 0 && i < n) {
+        double sacc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) sacc += part[k][lane];
+        x[i] = sacc;
+    }
 }
 
 int dense_inverse_apply(int n, const double* inv, const double* b, double* x, hipStream_t st) {
-    dense_inverse_apply_kernel<<<(n + 255) / 256, 256, 0, st>>>(n, inv, b, x);
+    dense_inverse_apply_kernel<<<(n + 63) / 64, 1024, 0, st>>>(n, inv, b, x);
     return (int)hipGetLastError();
 }
 
