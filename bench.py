@@ -443,9 +443,9 @@ def main():
             "jacobian_seconds": round(rep["jacobian_seconds"], 4),
             "linear_seconds": round(rep["linear_seconds"], 4),
             "initial_guess": "uniform free stream",
-            "linear_solver": "GMRES(30) with an agglomeration multigrid V-cycle (P0 Galerkin coarse levels, "
-                             "block SGS smoothing on FP16 fine blocks), FP32 operator + FP64 refinement, "
-                             "Eisenstat-Walker forcing",
+            "linear_solver": "GMRES(30) (classical Gram-Schmidt, one pass) with an agglomeration multigrid "
+                             "V-cycle (P0 Galerkin coarse levels, block SGS smoothing on FP16 fine blocks), "
+                             "FP32 operator + FP64 refinement, Eisenstat-Walker forcing",
         }
         del solver
 

@@ -190,7 +190,8 @@ class NewtonOpts(C.Structure):
                 ("ls_c1", C.c_double), ("ls_c2", C.c_double), ("ls_alpha_min", C.c_double),
                 ("jacobian_precision", C.c_int), ("gmres_restart", C.c_int), ("gmres_maxit", C.c_int),
                 ("linear_rtol", C.c_double), ("refine_max", C.c_int), ("verbose", C.c_int),
-                ("preconditioner", C.c_int), ("forcing", C.c_int), ("smoother_precision", C.c_int)]
+                ("preconditioner", C.c_int), ("forcing", C.c_int), ("smoother_precision", C.c_int),
+                ("gmres_cgs_refine", C.c_int)]
 
 
 class NewtonReport(C.Structure):

@@ -762,7 +762,7 @@ __global__ void axpby_kernel2(size_t n, double a, const double* __restrict__ x, 
 // matrix from the two Gram-Schmidt passes, the norm of the new vector,
 // Givens rotations, residual estimate.  sc layout: see GmresScalars.
 __global__ void arnoldi_kernel(int k, int m, const double* __restrict__ h1, const double* __restrict__ h2,
-                               double* __restrict__ sc) {
+                               int refine, double* __restrict__ sc) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     double* H = sc;                       // (m+1) x m, column-major: H[j + (m+1) k]
     double* cs = H + (size_t)(m + 1) * m;
@@ -770,8 +770,8 @@ __global__ void arnoldi_kernel(int k, int m, const double* __restrict__ h1, cons
     double* g = sn + m;                   // m+1
     double* hn = g + m + 1;               // [0] = ||v_{k+1}|| before normalisation, [1] = |g_{k+1}|
     double* col = H + (size_t)(m + 1) * k;
-    for (int j = 0; j <= k; ++j) col[j] = h1[j] + h2[j];
-    const double hk1 = sqrt(fmax(h2[k + 1], 0.0));  // h2[k+1] = squared norm after the second pass
+    for (int j = 0; j <= k; ++j) col[j] = refine ? h1[j] + h2[j] : h1[j];
+    const double hk1 = sqrt(fmax(h2[k + 1], 0.0));  // h2[k+1] = squared norm after the last pass
     col[k + 1] = hk1;
     hn[0] = hk1;
     for (int j = 0; j < k; ++j) {
@@ -856,8 +856,8 @@ int launch_axpby2(size_t n, double a, const double* x, double b, double* y, hipS
     return (int)hipGetLastError();
 }
 
-int launch_arnoldi(int k, int m, const double* h1, const double* h2, double* sc, hipStream_t st) {
-    arnoldi_kernel<<<1, 64, 0, st>>>(k, m, h1, h2, sc);
+int launch_arnoldi(int k, int m, const double* h1, const double* h2, int refine, double* sc, hipStream_t st) {
+    arnoldi_kernel<<<1, 64, 0, st>>>(k, m, h1, h2, refine, sc);
     return (int)hipGetLastError();
 }
 

@@ -60,7 +60,7 @@ int launch_norm2(size_t n, const double* w, double* part, double* out, hipStream
 int launch_combine(size_t n, int nv, const double* V, size_t ldv, const double* y, double* x, hipStream_t st);
 int launch_scale(size_t n, const double* s, int inv, double* x, hipStream_t st);
 int launch_axpby2(size_t n, double a, const double* x, double b, double* y, hipStream_t st);
-int launch_arnoldi(int k, int m, const double* h1, const double* h2, double* sc, hipStream_t st);
+int launch_arnoldi(int k, int m, const double* h1, const double* h2, int refine, double* sc, hipStream_t st);
 int launch_hessenberg_solve(int k, int m, const double* sc, double* y, hipStream_t st);
 int launch_gmres_init(int m, const double* nrm2, double* sc, hipStream_t st);
 template <class T, class V>
@@ -539,8 +539,9 @@ struct ice_newton {
         factored = true;
     }
 
-    // Restarted GMRES (right preconditioning, classical Gram-Schmidt with one
-    // reorthogonalisation pass): x += correction solving A x = b to
+    // Restarted GMRES (right preconditioning, classical Gram-Schmidt: one
+    // pass as PETSc's default, or with one reorthogonalisation pass when
+    // gmres_cgs_refine = 1): x += correction solving A x = b to
     // ||b - A x|| <= rtol ||b|| or maxit iterations.  x must hold 0 on entry.
     int gmres(const double* b, double* x, double rtol, int maxit, hipStream_t st, double* est) {
         int total = 0;
@@ -563,10 +564,14 @@ struct ice_newton {
                 apply_A(z, Vj(k + 1), st);
                 double* w = Vj(k + 1);
                 NT_LAUNCH(ice::launch_mdot(n, k + 1, V, n, w, part, npart, h1, st));
-                NT_LAUNCH(ice::launch_maxpy_norm(n, k + 1, V, n, h1, w, part, nullptr, st));
-                NT_LAUNCH(ice::launch_mdot(n, k + 1, V, n, w, part, npart, h2, st));
-                NT_LAUNCH(ice::launch_maxpy_norm(n, k + 1, V, n, h2, w, part, h2 + k + 1, st));
-                NT_LAUNCH(ice::launch_arnoldi(k, m, h1, h2, sc, st));
+                if (opt.gmres_cgs_refine) {
+                    NT_LAUNCH(ice::launch_maxpy_norm(n, k + 1, V, n, h1, w, part, nullptr, st));
+                    NT_LAUNCH(ice::launch_mdot(n, k + 1, V, n, w, part, npart, h2, st));
+                    NT_LAUNCH(ice::launch_maxpy_norm(n, k + 1, V, n, h2, w, part, h2 + k + 1, st));
+                } else {
+                    NT_LAUNCH(ice::launch_maxpy_norm(n, k + 1, V, n, h1, w, part, h2 + k + 1, st));
+                }
+                NT_LAUNCH(ice::launch_arnoldi(k, m, h1, h2, opt.gmres_cgs_refine, sc, st));
                 NT_LAUNCH(ice::launch_scale(n, hn, 1, w, st));
                 ++k;
                 ++total;
@@ -658,6 +663,7 @@ int ice_newton_opts_default(ice_newton_opts* o) {
     o->preconditioner = ICE_PC_BLOCK_SGS;
     o->forcing = ICE_FORCING_FIXED;
     o->smoother_precision = ICE_PREC_FP32;
+    o->gmres_cgs_refine = 0;
     return ICE_OK;
 }
 
@@ -677,6 +683,7 @@ int ice_newton_create(ice_disc* d, const ice_newton_opts* opts, ice_newton** out
          o.preconditioner != ICE_PC_MULTIGRID && o.preconditioner != ICE_PC_LINE) ||
         (o.forcing != ICE_FORCING_FIXED && o.forcing != ICE_FORCING_EW) ||
         (o.smoother_precision != ICE_PREC_FP32 && o.smoother_precision != ICE_PREC_FP16) ||
+        (o.gmres_cgs_refine != 0 && o.gmres_cgs_refine != 1) ||
         (o.smoother_precision == ICE_PREC_FP16 &&
          (o.jacobian_precision != ICE_PREC_FP32 ||
           (o.preconditioner != ICE_PC_BLOCK_SGS && o.preconditioner != ICE_PC_MULTIGRID)))) {

@@ -359,6 +359,11 @@ typedef struct {
                                  the multigrid's fine residuals read (a fixed
                                  preconditioner either way; the Krylov operator
                                  keeps jacobian_precision); with FP32 blocks */
+    int gmres_cgs_refine;     /* GMRES orthogonalisation by classical
+                                 Gram-Schmidt: 0 (default) one pass, PETSc's
+                                 GMRES default (KSP_GMRES_CGS_REFINE_NEVER)
+                                 that the reference's KSP runs; 1 with one
+                                 reorthogonalisation pass */
 } ice_newton_opts;
 
 typedef struct {

@@ -86,15 +86,18 @@ def test_linear_solve_accuracy():
     x_true = torch.from_numpy(np.random.default_rng(1).normal(size=disc.size)).cuda()
     b = torch.empty_like(u)
     jac.apply(x_true, b)
-    for prec, pc, tol, sm in [(abi.PREC_FP64, abi.PC_BLOCK_SGS, 1e-10, abi.PREC_FP32),
-                              (abi.PREC_FP32, abi.PC_BLOCK_SGS, 1e-10, abi.PREC_FP32),
-                              (abi.PREC_FP64, abi.PC_BLOCK_JACOBI, 1e-10, abi.PREC_FP32),
-                              (abi.PREC_FP32, abi.PC_BLOCK_JACOBI, 1e-10, abi.PREC_FP32),
-                              (abi.PREC_FP32, abi.PC_MULTIGRID, 1e-10, abi.PREC_FP32),
-                              (abi.PREC_FP32, abi.PC_BLOCK_SGS, 1e-10, abi.PREC_FP16),
-                              (abi.PREC_FP32, abi.PC_MULTIGRID, 1e-10, abi.PREC_FP16)]:
+    for prec, pc, tol, sm, cgs in [(abi.PREC_FP64, abi.PC_BLOCK_SGS, 1e-10, abi.PREC_FP32, 1),
+                                   (abi.PREC_FP32, abi.PC_BLOCK_SGS, 1e-10, abi.PREC_FP32, 1),
+                                   (abi.PREC_FP64, abi.PC_BLOCK_JACOBI, 1e-10, abi.PREC_FP32, 1),
+                                   (abi.PREC_FP32, abi.PC_BLOCK_JACOBI, 1e-10, abi.PREC_FP32, 1),
+                                   (abi.PREC_FP32, abi.PC_MULTIGRID, 1e-10, abi.PREC_FP32, 1),
+                                   (abi.PREC_FP32, abi.PC_BLOCK_SGS, 1e-10, abi.PREC_FP16, 1),
+                                   (abi.PREC_FP32, abi.PC_MULTIGRID, 1e-10, abi.PREC_FP16, 1),
+                                   # one Gram-Schmidt pass (PETSc's default orthogonalisation)
+                                   (abi.PREC_FP64, abi.PC_BLOCK_SGS, 1e-10, abi.PREC_FP32, 0),
+                                   (abi.PREC_FP32, abi.PC_MULTIGRID, 1e-10, abi.PREC_FP16, 0)]:
         solver = engine.PetscNewton(disc, jacobian_precision=prec, linear_rtol=tol, gmres_restart=40,
-                                    preconditioner=pc, smoother_precision=sm)
+                                    preconditioner=pc, smoother_precision=sm, gmres_cgs_refine=cgs)
         x = torch.full_like(u, float("nan"))  # old contents of the output must not leak in
         its, rel = solver.linear_solve(b, x, u=u)
         # the residual the solver reports is the FP64 one
@@ -102,7 +105,7 @@ def test_linear_solve_accuracy():
         jac.apply(x, r)
         rel_true = float((r - b).norm() / b.norm())
         xerr = float((x - x_true).norm() / x_true.norm())
-        print(f"prec={prec} pc={pc} smoother={sm} its={its} rel={rel:.2e} true rel={rel_true:.2e} x err={xerr:.2e}",
+        print(f"prec={prec} pc={pc} smoother={sm} cgs_refine={cgs} its={its} rel={rel:.2e} true rel={rel_true:.2e} x err={xerr:.2e}",
               flush=True)
         assert rel <= tol and rel_true <= 2 * tol
         assert xerr <= 1e-6
