@@ -342,7 +342,7 @@ def main():
         engine.form_residual(space, disc, u, r)
     torch.cuda.synchronize()
     # element-kernel workgroup shape chosen by timing (FP64 and mixed separately)
-    tile, tile_ms = disc.autotune(u, r, reps=20)
+    tile, tile_ms = disc.autotune(u, r, reps=50)
 
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
