@@ -929,7 +929,8 @@ __global__ __launch_bounds__((ElemTile<2, P, TPB>::BLOCK), ICE_ELEM_MINW) void e
     // FP64: node strides are padded to an odd number of entries and element
     // strides by one more, so that the lanes of a phase, which differ in
     // node and element, fall on different LDS banks (power-of-two strides
-    // put every element's row on the same bank; 21.4 -> 20.1 us at C2).  The
+    // put every element's row on the same bank; 2D NS 128 x 128 P3: 21.4 ->
+    // 20.1 us).  The
     // FP32 tiles measured slightly slower padded and keep the dense layout.
     constexpr bool LOWLDS = std::is_same<VT, double>::value;
     constexpr bool PAD = LOWLDS;

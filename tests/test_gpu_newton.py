@@ -156,7 +156,7 @@ def test_mms_convergence_rate():
                 assert rate >= order + 0.5
 
 
-def test_c2_newton_full_size():
+def test_ns_mms_128_p3_newton_full_size():
     """2D NS MMS 128 x 128 P3 (1,048,576 unknowns) from free stream to ||R||_2 < 1e-10."""
     from iceicle_amd import engine
     m, order, p, keep = workloads.ns_mms_2d()
@@ -176,7 +176,7 @@ def test_c2_newton_full_size():
     # the reference's face energy flux leaves an O(1) consistency error (see
     # test_mms_convergence_rate), so the discrete solution sits ~3e-5 from the
     # manufactured one at this resolution
-    print(f"C2 L2 error vs manufactured solution {err:.3e}", flush=True)
+    print(f"NS MMS 128x128 P3 L2 error vs manufactured solution {err:.3e}", flush=True)
     assert err < 1e-4
 
 

@@ -173,7 +173,7 @@ def test_mixed_rejects_1d():
         engine.form_residual(disc.space, disc, u, torch.empty_like(u), precision="mixed")
 
 
-def test_mixed_c2_against_fp64():
+def test_mixed_ns_mms_128_p3_against_fp64():
     """Full-size 128 x 128 P3 workload: mixed vs FP64 device residual (the
     FP64 one matches the oracle to 1e-15).  From the uniform free-stream
     state (the Newton initial guess) R is far from zero and the bound is the
@@ -193,5 +193,5 @@ def test_mixed_c2_against_fp64():
         engine.form_residual(space, disc, u, r32, precision="mixed")
         torch.cuda.synchronize()
         err = float((r32 - r64).abs().max() / r64.abs().max())
-        print(f"C2 {name}: mixed vs fp64 rel err {err:.3e}", flush=True)
+        print(f"NS MMS 128x128 P3 {name}: mixed vs fp64 rel err {err:.3e}", flush=True)
         assert err <= tol

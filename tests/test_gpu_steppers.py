@@ -57,7 +57,7 @@ def test_projection_and_l2_error():
 
 @pytest.mark.parametrize("compat", [True, False])
 @pytest.mark.parametrize("periodic_mode", [abi.PERIODIC_REFERENCE, abi.PERIODIC_COUPLED])
-def test_burgers_rk3_ssp_cfl_c1(compat, periodic_mode):
+def test_burgers_rk3_ssp_cfl_32x32_p2(compat, periodic_mode):
     """2D viscous Burgers, 32x32 periodic, P2, RK3-SSP CFL 0.2, 100 steps."""
     from iceicle_amd import engine
     mesh, order, params, ic = workloads.burgers_2d(32, 2, periodic_mode=periodic_mode)
@@ -68,7 +68,7 @@ def test_burgers_rk3_ssp_cfl_c1(compat, periodic_mode):
     st = engine.ExplicitStepper(disc, abi.stepper_desc(abi.STEP_RK3_SSP, reference_compat=compat, cfl=0.2, ntime=100))
     n, t = st.solve(u)
     err = cases.rel_inf(u.cpu().numpy(), ref)
-    print(f"C1 compat={compat} mode={periodic_mode} steps={n} t={t:.6e} t_ref={t_ref:.6e} err={err:.2e}")
+    print(f"Burgers 32x32 P2 compat={compat} mode={periodic_mode} steps={n} t={t:.6e} t_ref={t_ref:.6e} err={err:.2e}")
     assert n == n_ref == 100
     assert abs(t - t_ref) <= 1e-13 * t_ref
     assert err <= 1e-11
