@@ -38,6 +38,11 @@
 #ifndef ICE_FLUX_MINW
 #define ICE_FLUX_MINW 1
 #endif
+// trace-kernel workgroup target (whole waves for one element when an
+// element has more trace points)
+#ifndef ICE_TRACE_TPB
+#define ICE_TRACE_TPB 256
+#endif
 #ifndef ICE_ELEM_MINW
 #define ICE_ELEM_MINW 1
 #endif
@@ -330,8 +335,8 @@ __device__ __forceinline__ void element_point_trace(const DevSpace& S, const VT*
     }
 }
 
-template <int D, int P, class PH>
-__global__ __launch_bounds__(256, ICE_TRACE_MINW) void trace_kernel(DevSpace S, typename PH::Params ph,
+template <int D, int P, class PH, int TPB>
+__global__ __launch_bounds__(TPB, ICE_TRACE_MINW) void trace_kernel(DevSpace S, typename PH::Params ph,
                                                                      const double* __restrict__ U,
                                                                      typename PH::real* __restrict__ T) {
     using VT = typename PH::real;
@@ -340,7 +345,8 @@ __global__ __launch_bounds__(256, ICE_TRACE_MINW) void trace_kernel(DevSpace S, 
     constexpr int NFQ = Pow<N, D - 1>::value;
     constexpr int M = PH::M;
     constexpr int TPE = 2 * D * NFQ;                       // trace points per element
-    constexpr int EPB = (256 / TPE) > 0 ? (256 / TPE) : 1;  // elements per block
+    static_assert(TPB >= TPE, "one thread per trace point of an element");
+    constexpr int EPB = TPB / TPE;  // elements per block
     constexpr bool GRAD = TraceRec<PH, D>::NEED_GRAD;
     constexpr int NV = TraceRec<PH, D>::NV;
     __shared__ VT sU[EPB * NB * M];
@@ -1932,8 +1938,9 @@ int launch_residual(const DevSpace& S, const typename PH::Params& ph, const Disc
     const int nbb = (int)(((long)S.nfb * NFQ + 255) / 256);
     {
         constexpr int TPE = 2 * D * NFQ;
-        constexpr int EPB = (256 / TPE) > 0 ? (256 / TPE) : 1;
-        trace_kernel<D, P, PH><<<(S.nelem + EPB - 1) / EPB, 256, 0, st>>>(S, ph, U, T);
+        constexpr int TTPB = (TPE <= ICE_TRACE_TPB) ? ICE_TRACE_TPB : ((TPE + 63) / 64) * 64;
+        constexpr int EPB = TTPB / TPE;
+        trace_kernel<D, P, PH, TTPB><<<(S.nelem + EPB - 1) / EPB, TTPB, 0, st>>>(S, ph, U, T);
     }
     if (ev) (void)hipEventRecord(ev[1], st);
     if (nbb > 0)
