@@ -28,6 +28,9 @@ FLUX_BURGERS = 0
 FLUX_NAVIER_STOKES = 1
 FLUX_EULER = 2
 
+DDGIC_SOURCE = 0
+DDGIC_PUBLISHED = 1
+
 VISC_CONSTANT = 0
 VISC_DIMENSIONLESS_SUTHERLAND = 1
 
@@ -93,6 +96,7 @@ class DiscParams(C.Structure):
         ("ns_face_energy_textbook", C.c_int),
         ("sigma_ic", C.c_double),
         ("interior_penalty", C.c_int),
+        ("ddgic_form", C.c_int),
         ("dirichlet", Field * ICE_MAX_BCFLAG),
         ("neumann", Field * ICE_MAX_BCFLAG),
         ("source", Field),

@@ -615,6 +615,12 @@ static void derive_phys(const ice_disc_params* p, int order, ice::PhysParams& ph
     ph.fs_T = p->freestream[5];
     ph.textbook_energy = p->ns_face_energy_textbook;
     ph.sigma_ic = p->sigma_ic;
+    // interface correction (conservation_law.hpp:580-610, :750-780): SOURCE
+    // keeps the average operator's 1/2 per side and the Dirichlet term,
+    // PUBLISHED applies sigma per side and no boundary term
+    const bool pub = p->ddgic_form == ICE_DDGIC_PUBLISHED;
+    ph.sigma_face = pub ? p->sigma_ic : 0.5 * p->sigma_ic;
+    ph.sigma_bdy = pub ? 0.0 : p->sigma_ic;
     ph.beta0 = (double)(order + 1) * (order + 1);
     ph.beta1 = p->interior_penalty ? 0.0 : 1.0 / std::max((double)(2 * order * (order + 1)), 1.0);
     ph.beta0_bdy = ph.beta0;
@@ -627,6 +633,8 @@ int ice_disc_create(ice_space* s, const ice_disc_params* p, ice_disc** out) {
     const int D = s->ndim;
     if (p->flux != ICE_FLUX_BURGERS && p->flux != ICE_FLUX_NAVIER_STOKES && p->flux != ICE_FLUX_EULER)
         return fail(ICE_ERR_ARG, "unknown flux family");
+    if (p->ddgic_form != ICE_DDGIC_SOURCE && p->ddgic_form != ICE_DDGIC_PUBLISHED)
+        return fail(ICE_ERR_ARG, "ddgic_form must be ICE_DDGIC_SOURCE or ICE_DDGIC_PUBLISHED");
     const bool ns = p->flux != ICE_FLUX_BURGERS;
     // boundary types the reference defines for this flux family
     for (const auto& f : s->mesh.faces) {
@@ -691,6 +699,13 @@ int ice_disc_create(ice_space* s, const ice_disc_params* p, ice_disc** out) {
 
 int ice_disc_destroy(ice_disc* d) {
     delete d;
+    return ICE_OK;
+}
+
+int ice_disc_query(const ice_disc* d, int* neq, int64_t* size) {
+    if (!d) return fail(ICE_ERR_ARG, "null discretization");
+    if (neq) *neq = d->neq;
+    if (size) *size = (int64_t)d->space->dev.nelem * d->space->nb * d->neq;
     return ICE_OK;
 }
 

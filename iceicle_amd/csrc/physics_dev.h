@@ -36,6 +36,7 @@ struct PhysT {
     int textbook_energy;
     // ConservationLawDDG
     R sigma_ic;
+    R sigma_face, sigma_bdy;  // DDGIC factors per interior side and on Dirichlet faces
     R beta0, beta1, beta0_bdy;
     // parameter-only factors of the flux formulas, computed once on the host
     // (set_derived) so the pointwise fluxes multiply instead of divide
@@ -87,6 +88,8 @@ inline PhysT<R> phys_cast(const PhysParams& p) {
     q.fs_T = (R)p.fs_T;
     q.textbook_energy = p.textbook_energy;
     q.sigma_ic = (R)p.sigma_ic;
+    q.sigma_face = (R)p.sigma_face;
+    q.sigma_bdy = (R)p.sigma_bdy;
     q.beta0 = (R)p.beta0;
     q.beta1 = (R)p.beta1;
     q.beta0_bdy = (R)p.beta0_bdy;

@@ -458,7 +458,7 @@ __device__ __forceinline__ void interior_face_point(const DevSpace& S, const typ
 #pragma unroll
                         for (int r = 0; r < M; ++r)
                             acc += G[((v * D + k) * M + r) * D + s] * nrm[k] * (uR[r] - uL[r]);
-                    Pf[v * D + s] = VT(0.5) * ph.sigma_ic * acc * ws;
+                    Pf[v * D + s] = ph.sigma_face * acc * ws;
                 }
         }
     }
@@ -531,7 +531,7 @@ __device__ __forceinline__ void boundary_face_point(const DevSpace& S, const typ
 #pragma unroll
                         for (int r = 0; r < M; ++r)
                             acc += G[((v * D + k) * M + r) * D + s] * nrm[k] * (uB[r] - uL[r]);
-                    Pf[v * D + s] = ph.sigma_ic * acc * ws;
+                    Pf[v * D + s] = ph.sigma_bdy * acc * ws;
                 }
         }
     }

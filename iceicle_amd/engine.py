@@ -34,6 +34,7 @@ _SIGS = {
     "ice_space_qp_coords": ([P, C.POINTER(C.c_double)], C.c_int),
     "ice_disc_create": ([P, C.POINTER(abi.DiscParams), C.POINTER(P)], C.c_int),
     "ice_disc_destroy": ([P], C.c_int),
+    "ice_disc_query": ([P, C.POINTER(C.c_int), C.POINTER(C.c_int64)], C.c_int),
     "ice_residual": ([P, P, P, P], C.c_int),
     "ice_residual_mixed": ([P, P, P, P], C.c_int),
     "ice_disc_update": ([P, C.POINTER(abi.DiscParams)], C.c_int),

@@ -149,6 +149,15 @@ typedef struct ice_space_info {
 
 /* Discretization parameters: ConservationLawDDG public members
  * (conservation_law.hpp:132-170) plus the flux functor coefficients. */
+/* DDGIC interface-correction forms.  SOURCE follows conservation_law.hpp
+ * as written: sigma_ic/2 times each side's test gradient on interior faces
+ * (:580-610) and sigma_ic on Dirichlet faces (:750-780).  PUBLISHED is the
+ * form that reproduces the DDGIC column of doc/sine_diffusion_1d_study.md:45-55
+ * for P1 to the printed digits: sigma_ic on each side of interior faces and
+ * no boundary term. */
+#define ICE_DDGIC_SOURCE 0
+#define ICE_DDGIC_PUBLISHED 1
+
 typedef struct ice_disc_params {
     int flux;                 /* ICE_FLUX_* */
     /* BurgersCoefficients (burgers.hpp:14-40) */
@@ -170,6 +179,8 @@ typedef struct ice_disc_params {
     /* ConservationLawDDG */
     double sigma_ic;          /* DDGIC multiplier */
     int interior_penalty;     /* beta1 = 0 */
+    int ddgic_form;           /* ICE_DDGIC_*: which reading of the interface
+                                 correction term sigma_ic scales */
     ice_field dirichlet[ICE_MAX_BCFLAG];
     ice_field neumann[ICE_MAX_BCFLAG];
     ice_field source;         /* user_source; kind NONE = no source */
@@ -218,6 +229,12 @@ int ice_space_qp_coords(const ice_space* space, double* xq);
 /* Discretization (ConservationLawDDG constructor + public members). */
 int ice_disc_create(ice_space* space, const ice_disc_params* params, ice_disc** out);
 int ice_disc_destroy(ice_disc* disc);
+
+/* Host-only query: the number of conserved components (nv_comp,
+ * conservation_law.hpp:129-130) and the length in doubles of a state or
+ * residual vector, nelem * nbasis * neq (fe_layout_right size,
+ * layout_right.hpp:91-215); either pointer may be null. */
+int ice_disc_query(const ice_disc* disc, int* neq, int64_t* size);
 
 /* Replace the parameters of an existing discretization in place (flux
  * coefficients, boundary data, DDG options) and rebuild its boundary-value

@@ -51,6 +51,7 @@ struct Problem final : ProblemBase {
     Problem(FESpace<D>&& space, Disc&& d, const ice_disc_params& p) : fs(std::move(space)), disc(std::move(d)), prm(p) {
         disc.interior_penalty = p.interior_penalty != 0;
         disc.sigma_ic = p.sigma_ic;
+        disc.ddgic_published = p.ddgic_form == ICE_DDGIC_PUBLISHED;
         for (int k = 0; k < ICE_MAX_BCFLAG; ++k) {
             ice_field fd = p.dirichlet[k], fn = p.neumann[k];
             disc.dirichlet_callbacks.push_back([fd](const double* x, double* out) { ice_field_eval(&fd, D, neq, x, out); });
@@ -380,6 +381,71 @@ int ice_oracle_vanleer(const ice_disc_params* p, int ndim, const double* uL, con
     else if (ndim == 3) run(std::integral_constant<int, 3>{});
     else return -1;
     return 0;
+}
+
+// Diffusive numerical flux at a face point (navier_stokes.hpp:1241-1289 for
+// NS, burgers.hpp:206-220 for Burgers) for the consistency checks against
+// the physical flux.
+int ice_oracle_diff_flux(const ice_disc_params* p, int ndim, const double* u, const double* gradu, const double* n,
+                         double* f) {
+    auto run = [&](auto dimc) {
+        constexpr int D = decltype(dimc)::value;
+        if (p->flux == ICE_FLUX_BURGERS) {
+            BurgersCoefficients c; c.mu = p->mu;
+            for (int d = 0; d < 3; ++d) { c.a[d] = p->a[d]; c.b[d] = p->b[d]; }
+            BurgersDiffusionFlux<D>{&c}(u, gradu, n, f);
+        } else {
+            NSPhysics<D> ph; setup_physics<D>(ph, p);
+            NSDiffusionFlux<D, true> df{&ph};
+            df.textbook_energy = p->ns_face_energy_textbook != 0;
+            df(u, gradu, n, f);
+        }
+    };
+    if (ndim == 1) run(std::integral_constant<int, 1>{});
+    else if (ndim == 2) run(std::integral_constant<int, 2>{});
+    else if (ndim == 3) run(std::integral_constant<int, 3>{});
+    else return -1;
+    return 0;
+}
+// Burgers physical flux (burgers.hpp:72-91); lambda_out gets the wave speed norm
+int ice_oracle_burgers_flux(const ice_disc_params* p, int ndim, const double* u, const double* gradu, double* F,
+                            double* lambda_out) {
+    auto run = [&](auto dimc) {
+        constexpr int D = decltype(dimc)::value;
+        BurgersCoefficients c; c.mu = p->mu;
+        for (int d = 0; d < 3; ++d) { c.a[d] = p->a[d]; c.b[d] = p->b[d]; }
+        BurgersFlux<D> fl{&c};
+        fl(u, gradu, F);
+        *lambda_out = fl.lambda_max;
+    };
+    if (ndim == 1) run(std::integral_constant<int, 1>{});
+    else if (ndim == 2) run(std::integral_constant<int, 2>{});
+    else if (ndim == 3) run(std::integral_constant<int, 3>{});
+    else return -1;
+    return 0;
+}
+// NS boundary state (apply_bc, navier_stokes.hpp:975-1189); FULL_NS picks the
+// wall-general mapping
+int ice_oracle_apply_bc(const ice_disc_params* p, int ndim, const double* uL, const double* graduL, const double* n,
+                        int bctype, int bcflag, double* uR, double* graduR) {
+    try {
+        auto run = [&](auto dimc) {
+            constexpr int D = decltype(dimc)::value;
+            NSPhysics<D> ph; setup_physics<D>(ph, p);
+            double ul[D + 2];
+            for (int k = 0; k < D + 2; ++k) ul[k] = uL[k];
+            if (p->flux == ICE_FLUX_EULER) NSFlux<D, false>{&ph}.apply_bc(ul, graduL, n, bctype, bcflag, uR, graduR);
+            else NSFlux<D, true>{&ph}.apply_bc(ul, graduL, n, bctype, bcflag, uR, graduR);
+        };
+        if (ndim == 1) run(std::integral_constant<int, 1>{});
+        else if (ndim == 2) run(std::integral_constant<int, 2>{});
+        else if (ndim == 3) run(std::integral_constant<int, 3>{});
+        else return -1;
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
 }
 
 }  // extern "C"

@@ -256,6 +256,11 @@ struct ConservationLawDDG {
     DF diff_flux;
     bool interior_penalty = false;
     double sigma_ic = 0.0;
+    // reading of the interface correction: false = conservation_law.hpp as
+    // written (sigma/2 per side, Dirichlet term); true = sigma per side and no
+    // boundary term, which reproduces the published DDGIC P1 column
+    // (doc/sine_diffusion_1d_study.md:45-55)
+    bool ddgic_published = false;
     std::vector<Callback> dirichlet_callbacks, neumann_callbacks;
     std::optional<Callback> user_source;
 
@@ -362,6 +367,9 @@ struct ConservationLawDDG {
                 for (int v = 0; v < neq; ++v) RR[i * neq + v] -= (fvis[v] - fadv[v]) * biR[i];
             if constexpr (DF::has_homogeneity) {
                 if (sigma_ic != 0.0) {
+                    // SOURCE: the 0.5 of the average operator (:600-603);
+                    // PUBLISHED: sigma_ic on each side
+                    const double ic_side = ddgic_published ? 1.0 : 0.5;
                     double Gt[neq * D * neq * D];
                     diff_flux.homogeneity_tensor(uavg, Gt);
                     for (int it = 0; it < nb; ++it)
@@ -371,8 +379,8 @@ struct ConservationLawDDG {
                                     const double jr = uR[r] - uL[r];
                                     for (int s = 0; s < D; ++s) {
                                         const double ic = sigma_ic * Gt[((ie * D + k) * neq + r) * D + s] * nrm[k] * jr * w * sqrtg;
-                                        RL[it * neq + ie] -= ic * 0.5 * gL[it * D + s];
-                                        RR[it * neq + ie] -= ic * 0.5 * gR[it * D + s];
+                                        RL[it * neq + ie] -= ic * ic_side * gL[it * D + s];
+                                        RR[it * neq + ie] -= ic * ic_side * gR[it * D + s];
                                     }
                                 }
                 }
@@ -458,7 +466,7 @@ struct ConservationLawDDG {
             for (int i = 0; i < nb; ++i)
                 for (int v = 0; v < neq; ++v) RL[i * neq + v] += (fvis[v] - fadv[v]) * biL[i];
             if constexpr (DF::has_homogeneity) {
-                if (t.bctype == BC_DIRICHLET && sigma_ic != 0.0) {
+                if (t.bctype == BC_DIRICHLET && sigma_ic != 0.0 && !ddgic_published) {
                     double Gt[neq * D * neq * D];
                     diff_flux.homogeneity_tensor(uavg, Gt);
                     for (int it = 0; it < nb; ++it)

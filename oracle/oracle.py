@@ -48,6 +48,9 @@ def lib():
             ("ice_oracle_element_jacobian", [C.c_int, dp, dp, dp]),
             ("ice_oracle_ns_flux", [C.POINTER(abi.DiscParams), C.c_int, dp, dp, dp]),
             ("ice_oracle_vanleer", [C.POINTER(abi.DiscParams), C.c_int, dp, dp, dp, dp]),
+            ("ice_oracle_diff_flux", [C.POINTER(abi.DiscParams), C.c_int, dp, dp, dp, dp]),
+            ("ice_oracle_burgers_flux", [C.POINTER(abi.DiscParams), C.c_int, dp, dp, dp, dp]),
+            ("ice_oracle_apply_bc", [C.POINTER(abi.DiscParams), C.c_int, dp, dp, dp, C.c_int, C.c_int, dp, dp]),
             ("ice_oracle_trace_ref", [C.c_int, C.c_int, C.c_int, dp, dp]),
             ("ice_oracle_jacobian", [P, dp, dp, ip, dp]),
             ("ice_oracle_rk_solve", [P, dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
@@ -251,3 +254,29 @@ def vanleer(params, ndim, uL, uR, n):
     f = np.zeros(ndim + 2)
     _check(lib().ice_oracle_vanleer(C.byref(params), ndim, _dp(arrs[0]), _dp(arrs[1]), _dp(arrs[2]), _dp(f)))
     return f
+
+
+def diff_flux(params, ndim, u, gradu, n):
+    neq = 1 if params.flux == abi.FLUX_BURGERS else ndim + 2
+    arrs = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (u, gradu, n)]
+    f = np.zeros(neq)
+    _check(lib().ice_oracle_diff_flux(C.byref(params), ndim, _dp(arrs[0]), _dp(arrs[1]), _dp(arrs[2]), _dp(f)))
+    return f
+
+
+def burgers_flux(params, ndim, u, gradu):
+    """Physical Burgers flux F[d] and the wave-speed norm it records."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (u, gradu)]
+    F = np.zeros(ndim)
+    lam = np.zeros(1)
+    _check(lib().ice_oracle_burgers_flux(C.byref(params), ndim, _dp(arrs[0]), _dp(arrs[1]), _dp(F), _dp(lam)))
+    return F, float(lam[0])
+
+
+def apply_bc(params, ndim, uL, graduL, n, bctype, bcflag=0):
+    arrs = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (uL, graduL, n)]
+    uR = np.zeros(ndim + 2)
+    gR = np.zeros((ndim + 2) * ndim)
+    _check(lib().ice_oracle_apply_bc(C.byref(params), ndim, _dp(arrs[0]), _dp(arrs[1]), _dp(arrs[2]),
+                                     abi.BC[bctype] if isinstance(bctype, str) else bctype, bcflag, _dp(uR), _dp(gR)))
+    return uR, gR.reshape(ndim + 2, ndim)

@@ -1,0 +1,251 @@
+"""GPU parity at the benchmark workloads' real sizes and for the behaviours the
+reference states without a test of its own: the 2D NS manufactured problem
+at 128 x 128 P3 against the oracle, the 3D Euler vortex (RK3-SSP steps
+against the oracle on a small mesh, size-independent properties at 64^3),
+the inverse mass matrix at 3D P3/P4 and on a singular element, the detJ
+clamp of an inverted element, interior penalty, Sutherland viscosity, and
+the published DDGIC column through the device stepper."""
+import math
+
+import numpy as np
+import pytest
+
+from iceicle_amd import abi, workloads
+from tests import cases
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _gpu_residual(space, disc, u, precision="fp64"):
+    from iceicle_amd import engine
+    ud = torch.from_numpy(np.ascontiguousarray(u)).cuda() if isinstance(u, np.ndarray) else u
+    rd = torch.full_like(ud, float("nan"))
+    engine.form_residual(space, disc, ud, rd, precision=precision)
+    torch.cuda.synchronize()
+    return rd
+
+
+def test_ns_mms_128_p3_fp64_vs_oracle():
+    """The benchmark workload itself (128 x 128, 5% perturbed, P3, 1,048,576
+    unknowns): FP64 device residual against the oracle on the benchmark state
+    (manufactured solution perturbed by 1e-3) and on the free stream."""
+    from iceicle_amd import engine
+    from oracle.oracle import Oracle
+    m, order, p, keep = workloads.ns_mms_2d()
+    space = engine.FESpace(m, order)
+    disc = engine.ConservationLawDDG(space, p)
+    o = Oracle(m, abi.space_desc(order), p)
+    fs = np.tile([1.0, 0.3, 0.0, 1.0 / 1.4 / 0.4 + 0.045], disc.size // 4)
+    for name, state in [("bench state", workloads.mms_state_perturbed(space, 4, workloads.SEED0 + 2)),
+                        ("free stream", fs)]:
+        print(f"NS MMS 128x128 P3 {name}: oracle residual ...", flush=True)
+        ref = o.residual(state)
+        r = _gpu_residual(space, disc, state).cpu().numpy()
+        err = cases.rel_inf(r, ref)
+        print(f"NS MMS 128x128 P3 {name}: fp64 vs oracle rel err {err:.3e}", flush=True)
+        assert err <= TOL
+
+
+def _vortex_3d(n, order, half=20.0):
+    m, order, p, ic = workloads.euler_vortex_3d(n=n, order=order, half=half)
+    return m, order, p, ic
+
+
+def test_euler_vortex_3d_16_p3_residual_vs_oracle():
+    """3D Euler vortex (the C3 physics and boundary coupling) at 16^3 P3,
+    density perturbed: FP64 residual against the oracle."""
+    from iceicle_amd import engine
+    from oracle.oracle import Oracle
+    m, order, p, ic = _vortex_3d(16, 3)
+    o = Oracle(m, abi.space_desc(order), p)
+    space = engine.FESpace(m, order)
+    disc = engine.ConservationLawDDG(space, p)
+    u0 = workloads.density_perturbed(o.project(ic), disc.neq, workloads.SEED0 + 3)
+    ref = o.residual(u0)
+    err = cases.rel_inf(_gpu_residual(space, disc, u0).cpu().numpy(), ref)
+    print(f"Euler vortex 16^3 P3: fp64 vs oracle rel err {err:.3e}", flush=True)
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("compat", [False, True])
+def test_euler_vortex_3d_rk3_ssp_vs_oracle(compat):
+    """RK3-SSP with the CFL step (explicit_utils.hpp:61-90, Euler lambda_max)
+    on the 3D vortex, 4^3 hexes P3 with coupled periodic faces, 6 steps."""
+    from iceicle_amd import engine
+    from oracle.oracle import Oracle
+    m, order, p, ic = _vortex_3d(4, 3)
+    o = Oracle(m, abi.space_desc(order), p)
+    space = engine.FESpace(m, order)
+    disc = engine.ConservationLawDDG(space, p)
+    u0 = workloads.density_perturbed(o.project(ic), disc.neq, workloads.SEED0 + 3)
+    ref, n_ref, t_ref = o.rk_solve(u0, kind=0, compat=compat, cfl=0.2, ntime=6)
+    u = torch.from_numpy(u0).cuda()
+    st = engine.ExplicitStepper(disc, abi.stepper_desc(abi.STEP_RK3_SSP, reference_compat=compat, cfl=0.2, ntime=6))
+    n, t = st.solve(u)
+    err = cases.rel_inf(u.cpu().numpy(), ref)
+    print(f"Euler vortex 4^3 P3 RK3-SSP compat={compat}: steps={n} t={t:.6e} t_ref={t_ref:.6e} err={err:.2e}",
+          flush=True)
+    assert n == n_ref == 6
+    assert abs(t - t_ref) <= 1e-13 * t_ref
+    assert err <= 1e-11
+
+
+def test_euler_vortex_3d_64_p3_properties():
+    """The C3 workload at its real size (64^3 hexes, P3, 83,886,080 unknowns),
+    through properties that hold at any size: a uniform state is steady
+    (R = 0); with coupled periodic faces every face flux enters two elements
+    with opposite signs and the Lagrange basis sums to one, so the sum of R
+    over all coefficients of each component is zero; the mixed residual
+    stays within its bound of the FP64 one; one RK3-SSP step keeps the state
+    finite and conserves the total of each component's coefficients times
+    the element masses (checked through sum R = 0 for each stage)."""
+    from iceicle_amd import engine
+    m, order, p, ic = _vortex_3d(64, 3)
+    space = engine.FESpace(m, order)
+    disc = engine.ConservationLawDDG(space, p)
+    nc = disc.size // 5
+    print("Euler vortex 64^3 P3: uniform state", flush=True)
+    w = torch.tensor([1.0, 2.0, 2.0, 0.0, 1.0 / 0.4 / 1.4 + 4.0], dtype=torch.float64, device="cuda")
+    u = w.repeat(nc)
+    r = _gpu_residual(space, disc, u)
+    assert float(r.abs().max()) < 1e-11 * 4.0
+    print("Euler vortex 64^3 P3: vortex state", flush=True)
+    engine.project(disc, ic, u)
+    torch.cuda.synchronize()
+    u = torch.from_numpy(workloads.density_perturbed(u.cpu().numpy(), 5, workloads.SEED0 + 3)).cuda()
+    r = _gpu_residual(space, disc, u)
+    rv = r.view(-1, 5)
+    sums = rv.sum(dim=0)
+    scale = rv.abs().sum(dim=0)
+    print(f"sum R per component {sums.cpu().numpy()} (scale {scale.cpu().numpy()})", flush=True)
+    assert bool(torch.all(sums.abs() <= 1e-12 * scale))
+    rm = _gpu_residual(space, disc, u, precision="mixed")
+    err = float((rm - r).abs().max() / r.abs().max())
+    print(f"Euler vortex 64^3 P3: mixed vs fp64 rel err {err:.3e}", flush=True)
+    assert err <= 2e-5
+    st = engine.ExplicitStepper(disc, abi.stepper_desc(abi.STEP_RK3_SSP, cfl=0.2, ntime=1))
+    n, t = st.solve(u)
+    assert n == 1 and t > 0 and bool(torch.isfinite(u).all())
+
+
+@pytest.mark.parametrize("ndim,order,flux", [(3, 3, abi.FLUX_EULER), (3, 4, abi.FLUX_BURGERS)])
+def test_mass_inverse_3d_high_order(ndim, order, flux):
+    """The inverse mass kernel's element tiles at 64 and 125 nodes per element."""
+    from iceicle_amd import engine
+    from oracle.oracle import Oracle
+    m = abi.mesh_desc(ndim, [3] * ndim, [0] * ndim, [1] * ndim, ["EXTRAPOLATION"] * (2 * ndim))
+    coords = workloads.perturbed_nodes(m, 0.2, 4)
+    m = abi.mesh_desc(ndim, [3] * ndim, [0] * ndim, [1] * ndim, ["EXTRAPOLATION"] * (2 * ndim), coord=coords)
+    prm = abi.default_params(flux)
+    o = Oracle(m, abi.space_desc(order), prm)
+    space = engine.FESpace(m, order)
+    disc = engine.ConservationLawDDG(space, prm)
+    r = np.random.default_rng(1).normal(size=o.size)
+    ref = o.mass_apply(r)
+    out = torch.empty(o.size, dtype=torch.float64, device="cuda")
+    engine.mass_inverse_apply(disc, torch.from_numpy(r).cuda(), out)
+    torch.cuda.synchronize()
+    err = cases.rel_inf(out.cpu().numpy(), ref)
+    print(f"mass inverse ndim={ndim} p={order} err={err:.2e}", flush=True)
+    assert err <= 1e-12
+
+
+def test_mass_inverse_singular_element():
+    """tvd_rk3.hpp:190-204: an element whose mass matrix is singular (here a
+    1D element of zero length) gets 1e-8 r instead of M^-1 r."""
+    from iceicle_amd import engine
+    from oracle.oracle import Oracle
+    coords = np.array([[0.0], [0.3], [0.3], [1.0]])
+    m = abi.mesh_desc(1, [3], [0.0], [1.0], ["EXTRAPOLATION"] * 2, coord=coords)
+    prm = abi.default_params(abi.FLUX_BURGERS)
+    o = Oracle(m, abi.space_desc(2), prm)
+    space = engine.FESpace(m, 2)
+    disc = engine.ConservationLawDDG(space, prm)
+    r = np.random.default_rng(2).normal(size=o.size)
+    ref = o.mass_apply(r)
+    np.testing.assert_allclose(ref[3:6], 1e-8 * r[3:6], rtol=1e-15)
+    out = torch.empty(o.size, dtype=torch.float64, device="cuda")
+    engine.mass_inverse_apply(disc, torch.from_numpy(r).cuda(), out)
+    torch.cuda.synchronize()
+    assert cases.rel_inf(out.cpu().numpy(), ref) <= 1e-12
+
+
+def _compare(m, order, params, seed=0, state=None):
+    from iceicle_amd import engine
+    from oracle.oracle import Oracle
+    o = Oracle(m, abi.space_desc(order), params)
+    u = cases.random_state(params.flux, m.ndim, o.nelem, o.nbasis, seed) if state is None else state
+    ref = o.residual(u)
+    space = engine.FESpace(m, order)
+    disc = engine.ConservationLawDDG(space, params)
+    return cases.rel_inf(_gpu_residual(space, disc, u).cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_inverted_element_detj_clamp(order):
+    """conservation_law.hpp:270 clamps detJ at zero in the volume integral
+    (finite_element.hpp:237 guards the divide): an interior node pushed past
+    its neighbours inverts parts of four elements."""
+    m = abi.mesh_desc(2, [3, 3], [0, 0], [1, 1], ["DIRICHLET"] * 4)
+    coords = workloads.perturbed_nodes(m, 0.0, 0)
+    # node (1, 1) (index 5 of the 4 x 4 nodes) moved beyond node (2, 2)
+    coords[5] = [0.8, 0.75]
+    m = abi.mesh_desc(2, [3, 3], [0, 0], [1, 1], ["DIRICHLET"] * 4, coord=coords)
+    p = cases.burgers_params(2, a=(0.3, -0.2, 0))
+    err = _compare(m, order, p)
+    print(f"inverted element P{order}: err {err:.3e}", flush=True)
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_interior_penalty(order):
+    # beta1 = 0 (conservation_law.hpp:536-537)
+    m = abi.mesh_desc(2, [4, 3], [0, 0], [1, 1], ["PERIODIC"] * 4, periodic_mode=abi.PERIODIC_COUPLED)
+    coords = workloads.perturbed_nodes(m, 0.15, 3)
+    m = abi.mesh_desc(2, [4, 3], [0, 0], [1, 1], ["PERIODIC"] * 4, coord=coords, periodic_mode=abi.PERIODIC_COUPLED)
+    p = cases.burgers_params(2, mu=0.05, interior_penalty=True)
+    assert _compare(m, order, p) <= TOL
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_sutherland_viscosity(ndim):
+    # DimensionlessSutherlands (navier_stokes.hpp:271-301), with non-unit
+    # reference quantities so the ratios differ from one
+    p = cases.ns_params(ndim)
+    p.viscosity_kind = abi.VISC_DIMENSIONLESS_SUTHERLAND
+    p.ref[6] = 300.0 / 287.052874 * 0.0025  # reference temperature
+    p.ref[7] = 1.716e-5 / 0.01              # reference viscosity
+    bct = ["DIRICHLET", "SLIP_WALL", "EXTRAPOLATION", "NO_SLIP_ISOTHERMAL"] if ndim == 2 else \
+        ["DIRICHLET", "SLIP_WALL", "EXTRAPOLATION", "EXTRAPOLATION", "NO_SLIP_ISOTHERMAL", "DIRICHLET"]
+    nel = [3, 3] if ndim == 2 else [2, 2, 2]
+    m = abi.mesh_desc(ndim, nel, [0] * ndim, [1] * ndim, bct)
+    coords = workloads.perturbed_nodes(m, 0.1, 5)
+    m = abi.mesh_desc(ndim, nel, [0] * ndim, [1] * ndim, bct, coord=coords)
+    assert _compare(m, 2, p) <= TOL
+
+
+@pytest.mark.parametrize("nx", [4, 8])
+def test_ddgic_published_column_on_device(nx):
+    """The published DDGIC P1 errors (doc/sine_diffusion_1d_study.md:51-52)
+    through the device residual, inverse mass and RK3-TVD stepper."""
+    from iceicle_amd import engine
+    published = {4: 0.0824449545, 8: 0.027170573}
+    m = abi.mesh_desc(1, [nx], [0.0], [2 * math.pi], ["DIRICHLET", "DIRICHLET"])
+    prm = abi.default_params(abi.FLUX_BURGERS)
+    prm.mu = 1.0
+    prm.sigma_ic = 0.5
+    prm.ddgic_form = abi.DDGIC_PUBLISHED
+    prm.dirichlet[0] = abi.Field.make("CONSTANT", [0.0])
+    space = engine.FESpace(m, 1)
+    disc = engine.ConservationLawDDG(space, prm)
+    u = torch.empty(disc.size, dtype=torch.float64, device="cuda")
+    engine.project(disc, abi.Field.make("SINE", [1, 1, 0]), u)
+    dt = 1e-4 * (2 * math.pi / nx) ** 2
+    st = engine.ExplicitStepper(disc, abi.stepper_desc(abi.STEP_RK3_TVD, dt=dt, tfinal=1.0))
+    n, t = st.solve(u)
+    err = engine.l2_error(disc, abi.Field.make("EXP_DECAY_SINE", [1.0, 1.0]), u)
+    print(f"1D heat DDGIC nx={nx} P1: L2={err:.9g} (published {published[nx]})", flush=True)
+    assert abs(err - published[nx]) / published[nx] < 4e-7
