@@ -5,15 +5,23 @@ Navier-Stokes manufactured steady problem, 128 x 128 quads with seeded 5%
 interior node perturbation, P3 Lagrange, 4 x 4 Gauss-Legendre, 4 conserved
 variables, VanLeer + DDG viscous fluxes, Dirichlet boundaries, manufactured
 source.  One step = one FP64 residual evaluation R(u) (form_residual) over
-the whole mesh with u and R resident in HBM.  After the timed steps, one
+the whole mesh with u and R resident in HBM.  After the timed steps, the
 Newton-linesearch solve of the same problem from a uniform free-stream state
 to ||R||_2 < 1e-10 is timed (the metric's second part, under "newton").
 
-Prints one JSON line: value = residual GDOF/s of the timed pass (K
-residuals, no per-kernel events); roofline = the slowest kernel's
-algorithmic bytes over its duration from HIP events in a second pass of the
-same K residuals, against 8 TB/s;
-cpu_baseline = the CPU oracle on full residuals of the same workload.
+Prints one JSON line:
+  value     residual GDOF/s of the timed pass (K residuals back to back, no
+            per-kernel events);
+  roofline  the residual's algorithmic bytes B_alg = 2 s m E nb + 8 d N_vert
+            + 4 2^d E + 12 F (state in, residual out, vertices, P1
+            connectivity, face records) over the timed residual, against
+            8 TB/s; traffic = the HBM bytes of the residual's kernels from
+            rocprofv3 FETCH_SIZE / WRITE_SIZE counters taken in this run
+            (child processes), scaled by the same counters on a device copy;
+  kernels   per-kernel durations (HIP events in a second pass), the bytes
+            each kernel moves by design and their measured traffic;
+  cpu_baseline  the CPU oracle (restated reference loop) on all host cores,
+            with the single-core figure beside it.
 """
 import argparse
 import json
@@ -27,15 +35,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-
-# HBM bytes per launch of the FP64 residual kernels on the default workload
-# (128 x 128, P3), from rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by the
-# same counters on a 256 MiB copy (x2 and x1): profiles/residual_ns2d_p3_128_hbm_counters.md
-# (element kernel: per workgroup tile 0 / 1 / 2 = 256 / 128 / 64 threads)
-MEASURED_TRAFFIC_128_P3 = {"trace": (6893 * 2 + 24576) * 1024, "flux": (14738 * 2 + 4368) * 1024,
-                           "element": {0: (11040 * 2 + 8192) * 1024, 1: (11103 * 2 + 8192) * 1024,
-                                       2: int((11359.5 * 2 + 8192) * 1024)}}
-
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -57,12 +56,12 @@ def parse():
     ap.add_argument("--warmstart2d-train", type=int, default=500, help="GPU-generated training solves (2D)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="only run 20 device copies of an MB-sized buffer (HBM counter calibration)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 counter runs that measure the residual kernels' HBM traffic")
+    ap.add_argument("--no-c1", action="store_true", help="skip the 2D Burgers 32x32 P2 RK3-SSP leg")
+    ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--tile", type=int, default=0, help=argparse.SUPPRESS)
     return ap.parse_args()
-
-
-def traffic_of(kernel, tile):
-    t = MEASURED_TRAFFIC_128_P3[kernel]
-    return t[tile] if isinstance(t, dict) else t
 
 
 def kernel_event_pass(engine, space, disc, u, r, precision, nres):
@@ -84,21 +83,243 @@ def kernel_event_pass(engine, space, disc, u, r, precision, nres):
     return stage_ms, ncalls, start.elapsed_time(stop) / nres
 
 
-def cpu_baseline(mesh, order, params, u, seconds):
-    """The oracle (serial FP64 restatement of the reference path) on the same
-    workload, timed on this host: repeated full residuals until `seconds`."""
+KERNEL_KEYS = (("trace", "trace_kernel"), ("flux", "flux_kernel"), ("element", "element_kernel"))
+
+
+def traffic_probe(args):
+    """Child of measure_traffic, run under rocprofv3 --pmc: a few FP64
+    residuals of the headline workload with the element tile the parent
+    chose, then device copies of a known size for the counter scale."""
+    import torch
+    from iceicle_amd import engine, workloads
+    mesh, order, params, _ = workloads.ns_mms_2d(args.n, args.order)
+    space = engine.FESpace(mesh, order)
+    disc = engine.ConservationLawDDG(space, params)
+    disc.set_tile(args.tile)
+    u = torch.from_numpy(workloads.mms_state_perturbed(space, disc.neq, workloads.SEED0 + 2)).cuda()
+    r = torch.empty_like(u)
+    for _ in range(5):
+        engine.form_residual(space, disc, u, r)
+    torch.cuda.synchronize()
+    a = torch.ones(COPY_MB * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print("traffic probe done", file=sys.stderr, flush=True)
+
+
+COPY_MB = 256
+
+
+def _counter_means(csv_dir, counter):
+    """Mean counter value per dispatch, by kernel-name prefix, from a
+    rocprofv3 counter_collection.csv."""
+    import csv
+    import glob
+    files = glob.glob(os.path.join(csv_dir, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise RuntimeError(f"no counter_collection.csv under {csv_dir}")
+    sums = {}
+    with open(files[0]) as fh:
+        for row in csv.DictReader(fh):
+            if row.get("Counter_Name") != counter:
+                continue
+            name = row["Kernel_Name"]
+            key = None
+            for k, prefix in KERNEL_KEYS:
+                if prefix in name:
+                    key = k
+            if key is None and "copyBuffer" in name or (key is None and "copy" in name.lower()):
+                key = "copy"
+            if key is None:
+                continue
+            tot, n = sums.get(key, (0.0, 0))
+            sums[key] = (tot + float(row["Counter_Value"]), n + 1)
+    return {k: tot / n for k, (tot, n) in sums.items()}
+
+
+def measure_traffic(args, tile, timeout=150):
+    """HBM bytes per launch of the residual kernels, measured in this run:
+    two child processes of this script under rocprofv3, one counting
+    FETCH_SIZE and one WRITE_SIZE (KiB per dispatch), each scaled by what the
+    same counter reads for a device copy of COPY_MB MiB (bytes read and
+    written known).  Returns (per-kernel bytes, details) or (None, reason)."""
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return None, "rocprofv3 not found"
+    out = {}
+    tmp = tempfile.mkdtemp(prefix="ice_traffic_")
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = os.path.join(tmp, counter)
+        cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "c", "--",
+               sys.executable, os.path.abspath(__file__), "--traffic-probe", "--tile", str(tile),
+               "--n", str(args.n), "--order", str(args.order)]
+        print(f"traffic: {counter} child run ...", file=sys.stderr, flush=True)
+        proc = subprocess.Popen(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, start_new_session=True)
+        try:
+            _, err = proc.communicate(timeout=timeout)
+        except subprocess.TimeoutExpired:
+            os.killpg(proc.pid, signal.SIGKILL)
+            proc.communicate()
+            return None, f"rocprofv3 {counter} run timed out"
+        try:
+            out[counter] = _counter_means(d, counter)
+        except Exception as e:  # noqa: BLE001
+            tail = err.decode(errors="replace")[-300:] if err else ""
+            return None, f"rocprofv3 {counter}: {e}; rc={proc.returncode} {tail}"
+    copy_bytes = COPY_MB * (1 << 20)
+    fc, wc = out["FETCH_SIZE"].get("copy"), out["WRITE_SIZE"].get("copy")
+    if not fc or not wc:
+        return None, "copy calibration kernel not found in the counter output"
+    f_scale = copy_bytes / (fc * 1024.0)  # HBM bytes read per FETCH_SIZE byte
+    w_scale = copy_bytes / (wc * 1024.0)
+    per = {}
+    for k, _ in KERNEL_KEYS:
+        if k in out["FETCH_SIZE"] and k in out["WRITE_SIZE"]:
+            per[k] = int(out["FETCH_SIZE"][k] * 1024 * f_scale + out["WRITE_SIZE"][k] * 1024 * w_scale)
+    details = {"fetch_kib": {k: round(v, 1) for k, v in out["FETCH_SIZE"].items()},
+               "write_kib": {k: round(v, 1) for k, v in out["WRITE_SIZE"].items()},
+               "scale_fetch": round(f_scale, 4), "scale_write": round(w_scale, 4),
+               "calibration": f"{COPY_MB} MiB device copy (reads and writes {copy_bytes} bytes)"}
+    shutil.rmtree(tmp, ignore_errors=True)
+    return per, details
+
+
+def cpu_info():
+    """Host CPU model string and logical core count."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_baseline_leg(mesh, order, params, u, u_free, checks, gdofs, seconds, out):
+    """The CPU baseline: the oracle (FP64 restatement of the reference loop)
+    on the same workload and state, timed on this host, first on one core
+    for about `seconds`, then the element/face-parallel variant (bit-identical
+    to the serial loop) on every thread OpenMP is given for about as long.
+    Its residuals are also the checker of the device results gathered by
+    the GPU legs (`checks`): the relative max-norm errors go into `out`."""
     from iceicle_amd import abi
-    from oracle.oracle import Oracle
-    o = Oracle(mesh, abi.space_desc(order), params)
-    o.residual(u)  # warm
-    n, t0 = 0, time.perf_counter()
-    while True:
-        o.residual(u)
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or n >= 100:
-            break
-    return n, el, o.size
+    from oracle import oracle as O
+    print("cpu baseline (oracle) ...", file=sys.stderr, flush=True)
+    o = O.Oracle(mesh, abi.space_desc(order), params)
+
+    def timed(fn):
+        fn(u)  # warm
+        n, t0 = 0, time.perf_counter()
+        while True:
+            fn(u)
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= seconds or n >= 400:
+                return n, el
+
+    threads = O.max_threads()
+    n1, s1 = timed(o.residual)
+    nm, sm = timed(lambda x: o.residual_parallel(x, threads))
+
+    def err(a, ref):
+        return float(f"{np.max(np.abs(a - ref)) / np.max(np.abs(ref)):.3e}")
+
+    ref = o.residual_parallel(u, threads)
+    ref_fs = o.residual_parallel(u_free, threads)
+    out["rel_err_vs_oracle"] = err(checks["fp64"], ref)
+    out["rel_err_vs_oracle_free_stream"] = err(checks["fp64_free_stream"], ref_fs)
+    out["mixed"]["rel_err_vs_oracle"] = err(checks["mixed"], ref)
+    out["mixed"]["rel_err_vs_oracle_free_stream"] = err(checks["mixed_free_stream"], ref_fs)
+    model, ncpu = cpu_info()
+    multi = o.size * nm / sm / 1e9
+    return {
+        "value": round(multi, 6),
+        "unit": "GDOF/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{nm} full FP64 residuals of the same workload and state on {threads} host threads "
+                  f"({sm:.1f} s; element/face-parallel oracle loop, bit-identical to the serial one)",
+        "single_core": {"value": round(o.size * n1 / s1 / 1e9, 6), "sample": f"{n1} residuals ({s1:.1f} s)"},
+        "cpu_model": model, "nproc": ncpu,
+        "gpu_speedup": round(gdofs / multi, 1),
+    }
+
+
+def c1_cpu_leg(g):
+    """C1's CPU baseline: the oracle's RK3-SSP solve of the same 100 CFL
+    steps on one core (the reference loop is serial), and the device's
+    final state and residual checked against it."""
+    from iceicle_amd import abi
+    from oracle import oracle as O
+    o = O.Oracle(g["mesh"], abi.space_desc(g["order"]), g["params"])
+    r_ref = o.residual(g["u0"])
+    t0 = time.perf_counter()
+    ref, n_ref, t_ref = o.rk_solve(g["u0"], kind=0, compat=False, cfl=0.2, ntime=g["nsteps"])
+    cpu_s = time.perf_counter() - t0
+    return {
+        "rk3_ssp_100_steps_seconds": round(cpu_s, 3), "cores": 1, "kind": "port",
+        "gpu_speedup": round(cpu_s / g["gpu_seconds"], 1), "oracle_steps": n_ref,
+        "final_state_rel_err_vs_oracle": float(f"{np.max(np.abs(g['u_end'] - ref)) / np.max(np.abs(ref)):.3e}"),
+        "residual_rel_err_vs_oracle": float(f"{np.max(np.abs(g['r0'] - r_ref)) / np.max(np.abs(r_ref)):.3e}"),
+    }
+
+
+def c1_leg(nres=200, nsteps=100):
+    """2D viscous Burgers (nu 0.01, b = (1, 1)) on a coupled periodic 32 x 32
+    grid, P2, 3 x 3 GL, IC the L2 projection of a seeded sum of 4
+    sinusoids: FP64 residual throughput, and the RK3-SSP solve of 100 steps
+    at CFL 0.2 (device-side CFL step) timed on the GPU.  Returns the leg's
+    record and the device results for the CPU-baseline leg."""
+    import torch
+    from iceicle_amd import abi, engine, workloads
+    print("C1 Burgers 32x32 P2 ...", file=sys.stderr, flush=True)
+    mesh, order, params, ic = workloads.burgers_2d(32, 2)
+    space = engine.FESpace(mesh, order)
+    disc = engine.ConservationLawDDG(space, params)
+    u = torch.empty(disc.size, dtype=torch.float64, device="cuda")
+    engine.project(disc, ic, u)
+    u0 = u.cpu().numpy()
+    r = torch.empty_like(u)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for _ in range(20):
+        engine.form_residual(space, disc, u, r)
+    torch.cuda.synchronize()
+    ev0.record()
+    for _ in range(nres):
+        engine.form_residual(space, disc, u, r)
+    ev1.record()
+    torch.cuda.synchronize()
+    res_ms = ev0.elapsed_time(ev1) / nres
+    r0 = r.cpu().numpy()
+    # warm the stepper once, then time the full 100-step solve
+    st = engine.ExplicitStepper(disc, abi.stepper_desc(abi.STEP_RK3_SSP, cfl=0.2, ntime=nsteps))
+    st.solve(torch.from_numpy(u0).cuda())
+    st = engine.ExplicitStepper(disc, abi.stepper_desc(abi.STEP_RK3_SSP, cfl=0.2, ntime=nsteps))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    n, tend = st.solve(u)
+    torch.cuda.synchronize()
+    gpu_s = time.perf_counter() - t0
+    ndof = disc.size
+    rec = {
+        "workload": "2D viscous Burgers nu 0.01, b = (1, 1), 32x32 quads, coupled periodic, P2, 3x3 GL, "
+                    "IC L2 projection of 4 seeded sinusoids, RK3-SSP CFL 0.2, 100 steps",
+        "unknowns": ndof,
+        "residual": {"GDOF_s": round(ndof / (res_ms * 1e-3) / 1e9, 4), "ms": round(res_ms, 5)},
+        "rk3_ssp_100_steps": {"gpu_seconds": round(gpu_s, 5), "steps": n, "t_end": tend},
+    }
+    g = {"mesh": mesh, "order": order, "params": params, "u0": u0, "r0": r0, "u_end": u.cpu().numpy(),
+         "nsteps": nsteps, "gpu_seconds": gpu_s}
+    return rec, g
 
 
 def euler3d(args, n=64, order=3, nres=30, nsteps=50):
@@ -317,6 +538,9 @@ def main():
         print(json.dumps({"calibrate_copy_bytes_read": 8 * n, "calibrate_copy_bytes_written": 8 * n, "copies": 20}))
         return
 
+    if args.traffic_probe:
+        traffic_probe(args)
+        return
     if args.workload == "euler3d":
         print(json.dumps(euler3d(args)))
         return
@@ -358,7 +582,8 @@ def main():
     gdofs = ndof / (ms_per_step * 1e-3) / 1e9
     assert bool(torch.isfinite(r).all())
 
-    # per-kernel compulsory (unique) bytes of one launch
+    # bytes each kernel moves by its design (state, geometry, face records,
+    # the intermediate traces and face fluxes), for the per-kernel figures
     info = space.info
     D, nb, nfq = space.ndim, space.nbasis, space.nfqp
     E, nfi, nfb = space.nelem, info.nface_interior, info.nface_boundary
@@ -367,30 +592,30 @@ def main():
     nv = 1 + D  # trace record: value + the element's share of the DDG gradient
     geo_rec = 8 * (D + 2)  # normal, w sqrt(g), h per face point
     trace_bytes = 8 * E * 2 * D * nfq * nv * neq
-    kernels = {
-        # element face traces: state, vertices, face links and face-point
-        # geometry read, traces written
-        "trace": (stage_ms[0] / ncalls,
-                  8 * E * nb * neq + 8 * E * nvert * D + 4 * 2 * D * E + geo_rec * E * 2 * D * nfq + trace_bytes),
-        # face fluxes: traces of both sides, face records and geometry,
-        # boundary values read, weighted normal fluxes written
-        "flux": (stage_ms[1] / ncalls,
-                 trace_bytes + 16 * F + geo_rec * F * nfq + 8 * nfb * nfq * neq + 8 * F * nfq * neq),
-        # elements: state, vertices, face links, face fluxes and source load
-        # read, residual written
-        "element": (stage_ms[2] / ncalls,
-                    8 * E * nb * neq + 8 * E * nvert * D + 4 * 2 * D * E + 8 * F * nfq * neq
-                    + 8 * E * nb * neq + 8 * E * nb * neq),
+    design_bytes = {
+        "trace": 8 * E * nb * neq + 8 * E * nvert * D + 4 * 2 * D * E + geo_rec * E * 2 * D * nfq + trace_bytes,
+        "flux": trace_bytes + 16 * F + geo_rec * F * nfq + 8 * nfb * nfq * neq + 8 * F * nfq * neq,
+        "element": 8 * E * nb * neq + 8 * E * nvert * D + 4 * 2 * D * E + 8 * F * nfq * neq
+                   + 8 * E * nb * neq + 8 * E * nb * neq,
     }
-    dom = max(kernels, key=lambda k: kernels[k][0])
-    kms, kbytes = kernels[dom]
-    achieved = kbytes / (kms * 1e-3) / 1e9
+    kms = {k: stage_ms[n] / ncalls for n, k in enumerate(("trace", "flux", "element"))}
     nnode = (args.n + 1) ** 2
     b_alg = workloads.residual_bytes(D, order, neq, E, nnode, F)
     res_gbs = b_alg / (ms_per_step * 1e-3) / 1e9
 
+    # device results checked against the oracle in the CPU-baseline leg
+    checks = {"fp64": r.cpu().numpy()}
+
+    traffic, traffic_info = (None, "skipped (--no-traffic)")
+    if not args.no_traffic:
+        traffic, traffic_info = measure_traffic(args, tile)
+        print(f"traffic: {traffic} {traffic_info if traffic is None else ''}", file=sys.stderr, flush=True)
+    kernels = {k: {"ms": round(kms[k], 6), "design_bytes": int(design_bytes[k]),
+                   "design_GBs": round(design_bytes[k] / (kms[k] * 1e-3) / 1e9, 1),
+                   "hbm_traffic_bytes": (traffic or {}).get(k)} for k in kms}
+
     # mixed-precision residual on the same state: throughput and its error
-    # against the FP64 residual (which matches the CPU oracle to 1e-15)
+    # against the FP64 oracle (and the device FP64 residual)
     for _ in range(args.warmup):
         engine.form_residual(space, disc, u, r, precision="mixed")
     r64 = r.clone()
@@ -404,50 +629,58 @@ def main():
     mixed_ms = start.elapsed_time(stop) / args.steps
     mstage, mcalls, _ = kernel_event_pass(engine, space, disc, u, r, "mixed", args.steps)
     mixed_err = float((r - r64).abs().max() / r64.abs().max())
-    # the same comparison from the uniform free-stream state, where R is far from zero
-    uf = torch.from_numpy(np.tile([1.0, 0.3, 0.0, 1.0 / 1.4 / 0.4 + 0.045], disc.size // 4)).cuda()
+    checks["mixed"] = r.cpu().numpy()
+    # the same residuals from the uniform free-stream state, where R is far from zero
+    fs_state = np.tile([1.0, 0.3, 0.0, 1.0 / 1.4 / 0.4 + 0.045], disc.size // 4)
+    uf = torch.from_numpy(fs_state).cuda()
     rf64, rf32 = torch.empty_like(uf), torch.empty_like(uf)
     engine.form_residual(space, disc, uf, rf64)
     engine.form_residual(space, disc, uf, rf32, precision="mixed")
     torch.cuda.synchronize()
-    mixed_err_fs = float((rf32 - rf64).abs().max() / rf64.abs().max())
+    checks["fp64_free_stream"] = rf64.cpu().numpy()
+    checks["mixed_free_stream"] = rf32.cpu().numpy()
     mixed = {
         "value": round(ndof / (mixed_ms * 1e-3) / 1e9, 4),
         "unit": "GDOF/s",
         "ms_per_step": round(mixed_ms, 6),
         "rel_err_vs_fp64": float(f"{mixed_err:.3e}"),
-        "rel_err_vs_fp64_free_stream": float(f"{mixed_err_fs:.3e}"),
         "kernels_ms": {k: round(v / mcalls, 6) for k, v in zip(("trace", "flux", "element"), mstage)},
         "arithmetic": "FP32 interpolation, fluxes and face data; FP64 geometry, source load and output",
     }
 
+    def newton_leg(label, **kw):
+        # Newton-linesearch from a uniform free-stream state (rho 1, u 0.3,
+        # v 0, p 1/gamma) to ||R||_2 < 1e-10
+        un = torch.from_numpy(fs_state).cuda()
+        solver = engine.PetscNewton(disc, tau_abs=1e-10, tau_rel=0.0, kmax=30, forcing=abi.FORCING_EW,
+                                    preconditioner=abi.PC_MULTIGRID, **kw)
+        print(f"newton ({label}) ...", file=sys.stderr, flush=True)
+        rep = solver.solve(un)
+        print(f"newton ({label}) done: {rep['iterations']} steps, {rep['seconds']:.2f} s", file=sys.stderr,
+              flush=True)
+        del solver
+        return {
+            "seconds": round(rep["seconds"], 4), "converged": rep["converged"], "steps": rep["iterations"],
+            "residual_norms": [float(f"{v:.3e}") for v in rep["rhist"]],
+            "alpha": [round(a, 4) for a in rep["alpha"]],
+            "r0": float(f"{rep['r0']:.4e}"), "linear_iterations": rep["linear_iterations"],
+            "jacobian_seconds": round(rep["jacobian_seconds"], 4), "linear_seconds": round(rep["linear_seconds"], 4),
+            "linesearch_seconds": round(rep["linesearch_seconds"], 4),
+        }
+
     newton = None
     if not args.no_newton:
-        # Newton-linesearch from a uniform free-stream state (rho 1, u 0.3,
-        # v 0, p 1/gamma) to ||R||_2 < 1e-10, FP32 Jacobian blocks and
-        # preconditioner with FP64 residual and refinement
-        fs = np.tile([1.0, 0.3, 0.0, 1.0 / 1.4 / 0.4 + 0.045], disc.size // 4)
-        un = torch.from_numpy(fs).cuda()
-        solver = engine.PetscNewton(disc, tau_abs=1e-10, tau_rel=0.0, kmax=30, forcing=abi.FORCING_EW,
-                                    preconditioner=abi.PC_MULTIGRID, smoother_precision=abi.PREC_FP16)
-        print("newton solve ...", file=sys.stderr, flush=True)
-        rep = solver.solve(un)
-        print(f"newton done: {rep['iterations']} steps, {rep['seconds']:.2f} s", file=sys.stderr, flush=True)
-        newton = {
-            "seconds": round(rep["seconds"], 4),
-            "converged": rep["converged"],
-            "steps": rep["iterations"],
-            "residual_norms": [float(f"{v:.3e}") for v in rep["rhist"]],
-            "r0": float(f"{rep['r0']:.4e}"),
-            "linear_iterations": rep["linear_iterations"],
-            "jacobian_seconds": round(rep["jacobian_seconds"], 4),
-            "linear_seconds": round(rep["linear_seconds"], 4),
-            "initial_guess": "uniform free stream",
-            "linear_solver": "GMRES(30) (classical Gram-Schmidt, one pass) with an agglomeration multigrid "
-                             "V-cycle (P0 Galerkin coarse levels, block SGS smoothing on FP16 fine blocks), "
-                             "FP32 operator + FP64 refinement, Eisenstat-Walker forcing",
-        }
-        del solver
+        newton = newton_leg("Wolfe, FP32 blocks", linesearch=abi.LS_WOLFE, smoother_precision=abi.PREC_FP16)
+        newton["initial_guess"] = "uniform free stream"
+        newton["linesearch"] = "Wolfe (nonlinear_solver_utils.hpp:139-170)"
+        newton["linear_solver"] = ("GMRES(30) (classical Gram-Schmidt, one pass) with an agglomeration multigrid "
+                                   "V-cycle (P0 Galerkin coarse levels, block SGS smoothing on FP16 fine blocks), "
+                                   "FP32 Jacobian blocks + FP64 refinement, Eisenstat-Walker forcing")
+        newton["fp64_jacobian"] = newton_leg("Wolfe, FP64 blocks", linesearch=abi.LS_WOLFE,
+                                             jacobian_precision=abi.PREC_FP64, smoother_precision=abi.PREC_FP32)
+        newton["fp64_jacobian"]["linear_solver"] = "the same with FP64 Jacobian blocks and FP32 smoother blocks"
+        newton["no_linesearch"] = newton_leg("full steps, FP32 blocks", linesearch=abi.LS_NONE,
+                                             smoother_precision=abi.PREC_FP16)
 
     out = {
         "metric": "residual GDOF/s (FP64 vs mixed, 1 GPU); Newton time to ||R||<1e-10",
@@ -468,22 +701,22 @@ def main():
                         f"{order + 1}x{order + 1} GL, FP64 residual",
             "elements": E, "unknowns": ndof, "neq": neq, "order": order,
         },
-        "kernels_ms": {k: round(v[0], 6) for k, v in kernels.items()},
-        "kernel_bytes": {k: int(v[1]) for k, v in kernels.items()},
+        "kernels": kernels,
         "element_tile": {"best": tile, "ms": [round(t, 5) for t in tile_ms],
                          "threads": [256, 128, 64]},
         "roofline": {
             "bound": "hbm",
-            "kernel": dom,
-            "achieved": round(achieved, 2),
+            "kernel": "residual = trace_kernel + flux_kernel + element_kernel_2d (3 launches, timed pass)",
+            "achieved": round(res_gbs, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": (traffic_of(dom, tile) if (args.n, order) == (128, 3) else None),
-            "bytes_per_launch": int(kbytes),
+            "frac": round(res_gbs / HBM_PEAK_GBS, 5),
+            "traffic": (int(sum(traffic.values())) if traffic and len(traffic) == 3 else None),
+            "bytes_per_launch": int(b_alg),
+            "bytes_formula": "2*8*m*E*nb + 8*d*N_vert + 4*2^d*E + 12*(F_i+F_b)",
+            "traffic_source": traffic_info if traffic is None else
+            dict(traffic_info, how="rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE child runs of this script"),
         },
-        "residual_roofline": {"bytes": int(b_alg), "achieved_GBs": round(res_gbs, 2),
-                              "frac": round(res_gbs / HBM_PEAK_GBS, 5)},
     }
     out["mixed"] = mixed
     if newton is not None:
@@ -497,16 +730,15 @@ def main():
         out["warmstart_2d"] = warmstart_leg(args.warmstart2d_train, ntest=50, dim=2)
     if not args.no_batch:
         out["batched"] = batch_leg()
+    c1 = None
+    if not args.no_c1:
+        c1, c1_gpu = c1_leg()
+        out["c1_burgers"] = c1
     if not args.no_cpu_baseline:
-        n, el, size = cpu_baseline(mesh, order, params, u_host, args.cpu_seconds)
-        out["cpu_baseline"] = {
-            "value": round(size * n / el / 1e9, 6),
-            "unit": "GDOF/s",
-            "cores": 1,
-            "kind": "port",
-            "sample": f"{n} full residuals of the same workload on the host CPU ({el:.1f} s)",
-            "gpu_speedup": round(gdofs / (size * n / el / 1e9), 1),
-        }
+        out["cpu_baseline"] = cpu_baseline_leg(mesh, order, params, u_host, fs_state, checks, gdofs,
+                                               args.cpu_seconds, out)
+        if c1 is not None:
+            c1["cpu_baseline"] = c1_cpu_leg(c1_gpu)
     print(json.dumps(out))
 
 

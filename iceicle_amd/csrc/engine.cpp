@@ -121,7 +121,7 @@ struct ice_space {
     std::vector<double> X;
     std::vector<int> jcols;           // [nelem][2*ndim+1] block columns of the element-block Jacobian
     ice::DevSpace dev{};
-    void* dmem[13] = {};
+    void* dmem[14] = {};
     ~ice_space() {
         for (void* p : dmem)
             if (p) (void)hipFree(p);
@@ -486,6 +486,37 @@ static int space_create(const ice_mesh_desc* mds, int nmesh, const ice_space_des
                 }
             }
         }
+        // Jacobian face colouring (greedy, in face order): the lowest colour
+        // not yet used by a face of either element
+        std::vector<int> jorder(std::max(nfi + nfb, 1));
+        auto colour_faces = [&](int nf, auto elems, int* coff, int& ncol, int base) {
+            std::vector<unsigned> used((size_t)m.nelem, 0u);
+            std::vector<int> col(nf);
+            ncol = 0;
+            for (int f = 0; f < nf; ++f) {
+                const auto [a, b] = elems(f);
+                const unsigned busy = used[a] | used[b];
+                int c = 0;
+                while (busy & (1u << c)) ++c;
+                if (c >= ice::DevSpace::MAX_FACE_COLOURS) throw ice::Error{ICE_ERR_UNSUPPORTED, "face colouring overflow"};
+                col[f] = c;
+                used[a] |= 1u << c;
+                used[b] |= 1u << c;
+                ncol = std::max(ncol, c + 1);
+            }
+            std::fill(coff, coff + ice::DevSpace::MAX_FACE_COLOURS + 1, 0);
+            for (int f = 0; f < nf; ++f) ++coff[col[f] + 1];
+            for (int c = 0; c < ice::DevSpace::MAX_FACE_COLOURS; ++c) coff[c + 1] += coff[c];
+            std::vector<int> fill(coff, coff + ice::DevSpace::MAX_FACE_COLOURS);
+            for (int f = 0; f < nf; ++f) jorder[base + fill[col[f]]++] = f;
+        };
+        ice::DevSpace& dvc = s->dev;
+        colour_faces(nfi, [&](int f) { return std::pair<int, int>(s->fint[f].x, s->fint[f].y); }, dvc.coff_i,
+                     dvc.ncol_i, 0);
+        colour_faces(nfb, [&](int f) { return std::pair<int, int>(s->fbdy[f].x, s->fbdy[f].x); }, dvc.coff_b,
+                     dvc.ncol_b, nfi);
+        s->dmem[13] = upload(jorder);
+        dvc.jorder = (const int*)s->dmem[13];
         // upload
         s->dmem[7] = upload(s->jcols);
         s->dmem[0] = upload(s->X);

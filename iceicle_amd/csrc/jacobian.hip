@@ -13,7 +13,11 @@
 // pair re-evaluates just the pointwise numerical flux from the stored base
 // traces.  The flux differences are lifted to block entries in LDS; blocks
 // coupling two different elements belong to exactly one face and are
-// stored directly, diagonal blocks are accumulated with atomics.
+// stored directly.  Diagonal blocks are owned by their element: the volume
+// kernel writes them first, then the boundary and interior faces add their
+// contributions one face colour per launch (no two faces of a colour share
+// an element), so every entry is summed in the same order on every run and
+// no atomics are needed.
 #include <hip/hip_runtime.h>
 
 #include "dg_common.h"
@@ -123,14 +127,13 @@ __device__ __forceinline__ void phys_basis(const PointGeo<D>& g, const double* g
     }
 }
 
-__device__ __forceinline__ void atomic_add_f64(double* a, double v) { atomicAdd(a, v); }
-
 }  // namespace
 
 // ---- interior faces --------------------------------------------------------
+// One workgroup per face of one colour (jorder[base + blockIdx.x]).
 template <int D, int P, class PH>
 __global__ __launch_bounds__(256) void jac_face_kernel(DevSpace S, PhysParams ph, const double* __restrict__ U,
-                                                       double* __restrict__ J, int bslots) {
+                                                       double* __restrict__ J, int bslots, int base) {
     constexpr int N = P + 1;
     constexpr int NB = Pow<N, D>::value;
     constexpr int NFQ = Pow<N, D - 1>::value;
@@ -141,11 +144,14 @@ __global__ __launch_bounds__(256) void jac_face_kernel(DevSpace S, PhysParams ph
     __shared__ double su[2][NFQ][M], sg[2][NFQ][M * D], shn[2][NFQ][M * D];
     __shared__ double snrm[NFQ][D], sh[NFQ], sws[NFQ];
     __shared__ int spt[2][NFQ][D];
+    // basis value, physical gradient and normal Hessian term of every basis
+    // function at every face point, per side
+    __shared__ double bphi[2][NFQ][NB], bgx[2][NFQ][NB][D], bhn[2][NFQ][NB][D];
     __shared__ double sQ0[NFQ][M];
     __shared__ double sdQ[NC][NFQ][M];
     __shared__ double sred[256];
     __shared__ double seps;
-    const int f = blockIdx.x;
+    const int f = S.jorder[base + blockIdx.x];
     const int4 rec = S.fint[f];
     const int eL = rec.x, eR = rec.y, info = rec.z;
     const int fnl = info_fnl(info), fnr = info_fnr(info), orient = info_orient(info), pdim = info_pdim(info);
@@ -187,6 +193,18 @@ __global__ __launch_bounds__(256) void jac_face_kernel(DevSpace S, PhysParams ph
         }
     }
     __syncthreads();
+    for (int t = tid; t < 2 * NFQ * NB; t += blockDim.x) {
+        const int side = t / (NFQ * NB), q = (t / NB) % NFQ, i = t % NB;
+        double phi, gref[D], href[D * D], gx[D], hn[D];
+        basis_at<D, N>(S.tab, spt[side][q], i, phi, gref, href);
+        phys_basis<D>(sgeo[side][q], gref, href, snrm[q], gx, hn);
+        bphi[side][q][i] = phi;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            bgx[side][q][i][d] = gx[d];
+            bhn[side][q][i][d] = hn[d];
+        }
+    }
     if (tid < NFQ) {
         double Q[M];
         interior_point_flux<D, PH>(ph, su[0][tid], sg[0][tid], shn[0][tid], su[1][tid], sg[1][tid], shn[1][tid],
@@ -200,11 +218,7 @@ __global__ __launch_bounds__(256) void jac_face_kernel(DevSpace S, PhysParams ph
     for (int r = tid; r < NC; r += blockDim.x) {
         const int side = r / NBM, iv = r % NBM, i = iv / M, v = iv % M;
         double acc = 0;
-        for (int q = 0; q < NFQ; ++q) {
-            double phi, gref[D];
-            basis_at<D, N>(S.tab, spt[side][q], i, phi, gref, nullptr);
-            acc += sQ0[q][v] * phi;
-        }
+        for (int q = 0; q < NFQ; ++q) acc += sQ0[q][v] * bphi[side][q][i];
         if (side) sR += acc * acc;
         else sL += acc * acc;
     }
@@ -229,9 +243,6 @@ __global__ __launch_bounds__(256) void jac_face_kernel(DevSpace S, PhysParams ph
     for (int it = tid; it < NC * NFQ; it += blockDim.x) {
         const int c = it / NFQ, q = it % NFQ;
         const int side = c / NBM, iw = c % NBM, i = iw / M, w = iw % M;
-        double phi, gref[D], href[D * D], gx[D], hn[D];
-        basis_at<D, N>(S.tab, spt[side][q], i, phi, gref, href);
-        phys_basis<D>(sgeo[side][q], gref, href, snrm[q], gx, hn);
         double u[2][M], g[2][M * D], h[2][M * D];
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
@@ -240,11 +251,11 @@ __global__ __launch_bounds__(256) void jac_face_kernel(DevSpace S, PhysParams ph
 #pragma unroll
             for (int k = 0; k < M * D; ++k) { g[s][k] = sg[s][q][k]; h[s][k] = shn[s][q][k]; }
         }
-        u[side][w] += eps * phi;
+        u[side][w] += eps * bphi[side][q][i];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            g[side][w * D + d] += eps * gx[d];
-            h[side][w * D + d] += eps * hn[d];
+            g[side][w * D + d] += eps * bgx[side][q][i][d];
+            h[side][w * D + d] += eps * bhn[side][q][i][d];
         }
         double Q[M];
         interior_point_flux<D, PH>(ph, u[0], g[0], h[0], u[1], g[1], h[1], snrm[q], sh[q], sws[q], Q);
@@ -252,28 +263,31 @@ __global__ __launch_bounds__(256) void jac_face_kernel(DevSpace S, PhysParams ph
         for (int v = 0; v < M; ++v) sdQ[c][q][v] = Q[v] - sQ0[q][v];
     }
     __syncthreads();
-    // lift: row (side', i', v'), column c
+    // lift: row (side rs, i, v), column c; the left rows first, then the
+    // right rows (with periodic coupling across a single element both sides
+    // can be the same element, whose diagonal block then takes both in turn)
     const size_t bs = NBM;
     const double ieps = 1.0 / eps;
-    for (int ent = tid; ent < NC * NC; ent += blockDim.x) {
-        const int r = ent / NC, c = ent % NC;
-        const int rs = r / NBM, iv = r % NBM, i = iv / M, v = iv % M;
-        const int cs = c / NBM, cw = c % NBM;
-        double acc = 0;
-#pragma unroll
-        for (int q = 0; q < NFQ; ++q) {
-            double phi, gref[D];
-            basis_at<D, N>(S.tab, spt[rs][q], i, phi, gref, nullptr);
-            acc += sdQ[c][q][v] * phi;
-        }
-        const double val = (rs ? -acc : acc) * ieps;
+#pragma unroll 1
+    for (int rs = 0; rs < 2; ++rs) {
         const int erow = rs ? eR : eL;
-        if (rs == cs) {
-            atomic_add_f64(J + ((size_t)erow * bslots) * bs * bs + (size_t)iv * bs + cw, val);
-        } else {
-            const int slot = 1 + (rs ? fnr : fnl);
-            J[((size_t)erow * bslots + slot) * bs * bs + (size_t)iv * bs + cw] = val;
+        for (int ent = tid; ent < NBM * NC; ent += blockDim.x) {
+            const int iv = ent / NC, c = ent % NC;
+            const int i = iv / M, v = iv % M;
+            const int cs = c / NBM, cw = c % NBM;
+            double acc = 0;
+#pragma unroll
+            for (int q = 0; q < NFQ; ++q) acc += sdQ[c][q][v] * bphi[rs][q][i];
+            const double val = (rs ? -acc : acc) * ieps;
+            if (rs == cs) {
+                double* a = J + ((size_t)erow * bslots) * bs * bs + (size_t)iv * bs + cw;
+                *a += val;
+            } else {
+                const int slot = 1 + (rs ? fnr : fnl);
+                J[((size_t)erow * bslots + slot) * bs * bs + (size_t)iv * bs + cw] = val;
+            }
         }
+        __syncthreads();
     }
 }
 
@@ -281,7 +295,7 @@ __global__ __launch_bounds__(256) void jac_face_kernel(DevSpace S, PhysParams ph
 template <int D, int P, class PH>
 __global__ __launch_bounds__(256) void jac_bface_kernel(DevSpace S, PhysParams ph, const double* __restrict__ bval,
                                                         const double* __restrict__ U, double* __restrict__ J,
-                                                        int bslots) {
+                                                        int bslots, int base) {
     constexpr int N = P + 1;
     constexpr int NB = Pow<N, D>::value;
     constexpr int NFQ = Pow<N, D - 1>::value;
@@ -291,11 +305,12 @@ __global__ __launch_bounds__(256) void jac_bface_kernel(DevSpace S, PhysParams p
     __shared__ double su[NFQ][M], sg[NFQ][M * D];
     __shared__ double snrm[NFQ][D], sh[NFQ], sws[NFQ];
     __shared__ int spt[NFQ][D];
+    __shared__ double bphi[NFQ][NB], bgx[NFQ][NB][D];
     __shared__ double sQ0[NFQ][M];
     __shared__ double sdQ[NBM][NFQ][M];
     __shared__ double sred[256];
     __shared__ double seps;
-    const int b = blockIdx.x;
+    const int b = S.jorder[S.nfi + base + blockIdx.x];
     const int4 rec = S.fbdy[b];
     const int eL = rec.x, info = rec.y, bcflag = rec.z;
     const int fnl = info_fnl(info), bctype = info_bctype(info);
@@ -338,15 +353,21 @@ __global__ __launch_bounds__(256) void jac_bface_kernel(DevSpace S, PhysParams p
         for (int v = 0; v < M; ++v) sQ0[q][v] = Q[v];
     }
     __syncthreads();
+    for (int t = tid; t < NFQ * NB; t += blockDim.x) {
+        const int q = t / NB, i = t % NB;
+        double phi, gref[D], gx[D];
+        basis_at<D, N>(S.tab, spt[q], i, phi, gref, nullptr);
+        phys_basis<D>(sgeo[q], gref, nullptr, snrm[q], gx, nullptr);
+        bphi[q][i] = phi;
+#pragma unroll
+        for (int d = 0; d < D; ++d) bgx[q][i][d] = gx[d];
+    }
+    __syncthreads();
     double sL = 0;
     for (int r = tid; r < NBM; r += blockDim.x) {
         const int i = r / M, v = r % M;
         double acc = 0;
-        for (int q = 0; q < NFQ; ++q) {
-            double phi, gref[D];
-            basis_at<D, N>(S.tab, spt[q], i, phi, gref, nullptr);
-            acc += sQ0[q][v] * phi;
-        }
+        for (int q = 0; q < NFQ; ++q) acc += sQ0[q][v] * bphi[q][i];
         sL += acc * acc;
     }
     sred[tid] = sL;
@@ -361,17 +382,14 @@ __global__ __launch_bounds__(256) void jac_bface_kernel(DevSpace S, PhysParams p
     for (int it = tid; it < NBM * NFQ; it += blockDim.x) {
         const int c = it / NFQ, q = it % NFQ;
         const int i = c / M, w = c % M;
-        double phi, gref[D], gx[D];
-        basis_at<D, N>(S.tab, spt[q], i, phi, gref, nullptr);
-        phys_basis<D>(sgeo[q], gref, nullptr, snrm[q], gx, nullptr);
         double u[M], g[M * D];
 #pragma unroll
         for (int v = 0; v < M; ++v) u[v] = su[q][v];
 #pragma unroll
         for (int k = 0; k < M * D; ++k) g[k] = sg[q][k];
-        u[w] += eps * phi;
+        u[w] += eps * bphi[q][i];
 #pragma unroll
-        for (int d = 0; d < D; ++d) g[w * D + d] += eps * gx[d];
+        for (int d = 0; d < D; ++d) g[w * D + d] += eps * bgx[q][i][d];
         double Q[M];
         boundary_point_flux<D, PH>(ph, bctype, bcflag, u, g, bval + ((size_t)b * NFQ + q) * M, snrm[q], sh[q], sws[q],
                                    Q, nullptr);
@@ -386,12 +404,8 @@ __global__ __launch_bounds__(256) void jac_bface_kernel(DevSpace S, PhysParams p
         const int i = r / M, v = r % M;
         double acc = 0;
 #pragma unroll
-        for (int q = 0; q < NFQ; ++q) {
-            double phi, gref[D];
-            basis_at<D, N>(S.tab, spt[q], i, phi, gref, nullptr);
-            acc += sdQ[c][q][v] * phi;
-        }
-        atomic_add_f64(J + ((size_t)eL * bslots) * bs * bs + (size_t)r * bs + c, acc * ieps);
+        for (int q = 0; q < NFQ; ++q) acc += sdQ[c][q][v] * bphi[q][i];
+        J[((size_t)eL * bslots) * bs * bs + (size_t)r * bs + c] += acc * ieps;
     }
 }
 
@@ -406,6 +420,7 @@ __global__ __launch_bounds__(256) void jac_elem_kernel(DevSpace S, PhysParams ph
     __shared__ double sdFdu[NB][M][D][M];
     __shared__ double sdFdg[NB][M][D][M][D];
     __shared__ double sgx[NB][NB][D];  // [q][i][d] physical basis gradients
+    __shared__ double sphi[NB][NB];    // [q][j] basis values
     __shared__ double sdw[NB];
     const int e = blockIdx.x;
     const int tid = threadIdx.x;
@@ -439,6 +454,7 @@ __global__ __launch_bounds__(256) void jac_elem_kernel(DevSpace S, PhysParams ph
         for (int i = 0; i < NB; ++i) {
             double phi, gref[D];
             basis_at<D, N>(S.tab, pt, i, phi, gref, nullptr);
+            sphi[q][i] = phi;
             double gxi[D];
 #pragma unroll
             for (int j = 0; j < D; ++j) {
@@ -493,17 +509,7 @@ __global__ __launch_bounds__(256) void jac_elem_kernel(DevSpace S, PhysParams ph
         const int i = r / M, v = r % M, j = c / M, w = c % M;
         double acc = 0;
         for (int q = 0; q < NB; ++q) {
-            int qd[D];
-            int t = q;
-#pragma unroll
-            for (int d = D - 1; d >= 0; --d) { qd[d] = t % N; t /= N; }
-            int jd[D];
-            t = j;
-#pragma unroll
-            for (int d = D - 1; d >= 0; --d) { jd[d] = t % N; t /= N; }
-            double phij = 1.0;
-#pragma unroll
-            for (int d = 0; d < D; ++d) phij *= L0[qd[d] * N + jd[d]];
+            const double phij = sphi[q][j];
             double s = 0;
 #pragma unroll
             for (int d = 0; d < D; ++d) {
@@ -514,12 +520,21 @@ __global__ __launch_bounds__(256) void jac_elem_kernel(DevSpace S, PhysParams ph
             }
             acc += s * sdw[q];
         }
-        atomic_add_f64(J + ((size_t)e * bslots) * bs * bs + (size_t)r * bs + c, acc);
+        // first writer of the diagonal block (the faces add to it afterwards)
+        J[((size_t)e * bslots) * bs * bs + (size_t)r * bs + c] = acc;
     }
 }
 
-__global__ void fill_zero_kernel(size_t n, double* __restrict__ a) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = 0.0;
+// zero the off-diagonal slots without a neighbour (boundary faces): the
+// block products multiply them by zero, so they must hold finite values
+__global__ void zero_missing_slots_kernel(int nelem, int bslots, size_t bsq, const int* __restrict__ cols,
+                                          double* __restrict__ J) {
+    const int e = blockIdx.x;
+    for (int s = 1; s < bslots; ++s) {
+        if (cols[(size_t)e * bslots + s] >= 0) continue;
+        double* blk = J + ((size_t)e * bslots + s) * bsq;
+        for (size_t k = threadIdx.x; k < bsq; k += blockDim.x) blk[k] = 0.0;
+    }
 }
 
 __global__ void to_fp32_kernel(size_t n, const double* __restrict__ a, float* __restrict__ b) {
@@ -531,9 +546,19 @@ template <int D, int P, class PH>
 int launch_jac(const DevSpace& S, const PhysParams& ph, const DiscDev& dd, const double* U, double* J,
                hipStream_t st) {
     const int bslots = 2 * D + 1;
-    if (S.nfi > 0) jac_face_kernel<D, P, PH><<<S.nfi, 256, 0, st>>>(S, ph, U, J, bslots);
-    if (S.nfb > 0) jac_bface_kernel<D, P, PH><<<S.nfb, 256, 0, st>>>(S, ph, dd.bval, U, J, bslots);
+    constexpr int NBM = Pow<P + 1, D>::value * PH::M;
+    zero_missing_slots_kernel<<<S.nelem, 256, 0, st>>>(S.nelem, bslots, (size_t)NBM * NBM, S.jcols, J);
+    // diagonal blocks: volume term first, then boundary faces and interior
+    // faces one colour at a time (fixed summation order, no atomics)
     jac_elem_kernel<D, P, PH><<<S.nelem, 256, 0, st>>>(S, ph, U, J, bslots);
+    for (int c = 0; c < S.ncol_b; ++c) {
+        const int n = S.coff_b[c + 1] - S.coff_b[c];
+        if (n > 0) jac_bface_kernel<D, P, PH><<<n, 256, 0, st>>>(S, ph, dd.bval, U, J, bslots, S.coff_b[c]);
+    }
+    for (int c = 0; c < S.ncol_i; ++c) {
+        const int n = S.coff_i[c + 1] - S.coff_i[c];
+        if (n > 0) jac_face_kernel<D, P, PH><<<n, 256, 0, st>>>(S, ph, U, J, bslots, S.coff_i[c]);
+    }
     return (int)hipGetLastError();
 }
 
@@ -551,7 +576,7 @@ int jac_order(int p, const DevSpace& S, const PhysParams& ph, const DiscDev& dd,
 
 int jacobian_fp64(int ndim, int p, int flux, const DevSpace& S, const PhysParams& ph, const DiscDev& dd,
                   const double* U, double* J, size_t nvals, hipStream_t st) {
-    fill_zero_kernel<<<4096, 256, 0, st>>>(nvals, J);
+    (void)nvals;
     if (flux == ICE_FLUX_BURGERS) {
         if (ndim == 1) return jac_order<Burgers<1>, 1>(p, S, ph, dd, U, J, st);
         if (ndim == 2) return jac_order<Burgers<2>, 2>(p, S, ph, dd, U, J, st);

@@ -33,6 +33,16 @@ struct DevSpace {
     const signed char* finv;   // inverse of fperm
     const signed char* ftens;  // [2*ndim][norient][nfq] face point q -> its index in the element's
                                // tensor order of the face's tangential axes
+    // Face colouring for the Jacobian assembly: no two faces of one colour
+    // share an element, so a launch per colour adds into diagonal blocks
+    // without atomics and in a fixed order.  jorder lists the interior faces
+    // colour by colour, then the boundary faces colour by colour; colour c of
+    // the interior faces is jorder[coff_i[c] .. coff_i[c + 1]), of the boundary
+    // faces jorder[nfi + coff_b[c] .. nfi + coff_b[c + 1]).
+    static constexpr int MAX_FACE_COLOURS = 16;
+    const int* jorder;
+    int ncol_i, ncol_b;
+    int coff_i[MAX_FACE_COLOURS + 1], coff_b[MAX_FACE_COLOURS + 1];
 };
 
 // face info word: fnl | fnr << 3 | orient << 6 | bctype << 9 | pdim << 13
@@ -111,7 +121,7 @@ int face_geometry_table(int ndim, int p, const DevSpace& S, double* out, hipStre
 size_t trace_buffer_size(int ndim, int p, int flux, int nelem);
 
 // Finite-difference element-block Jacobian into J[nelem][2*ndim+1][bs][bs]
-// (zeroed first); ndim <= 2, no DDGIC.
+// (every block written; the same entries on every run); ndim <= 2, no DDGIC.
 int jacobian_fp64(int ndim, int p, int flux, const DevSpace& S, const PhysParams& ph, const DiscDev& dd,
                   const double* U, double* J, size_t nvals, hipStream_t st);
 

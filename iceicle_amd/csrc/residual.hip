@@ -1938,7 +1938,11 @@ int launch_residual(const DevSpace& S, const typename PH::Params& ph, const Disc
     const int nbb = (int)(((long)S.nfb * NFQ + 255) / 256);
     {
         constexpr int TPE = 2 * D * NFQ;
-        constexpr int TTPB = (TPE <= ICE_TRACE_TPB) ? ICE_TRACE_TPB : ((TPE + 63) / 64) * 64;
+        // whole elements per workgroup, rounded up to whole waves only once:
+        // a wave that would hold no trace point is not launched (3D P3:
+        // 2 x 96 points in 192 threads rather than 256)
+        constexpr int EPW = (TPE <= ICE_TRACE_TPB) ? ICE_TRACE_TPB / TPE : 1;
+        constexpr int TTPB = ((EPW * TPE + 63) / 64) * 64;
         constexpr int EPB = TTPB / TPE;
         trace_kernel<D, P, PH, TTPB><<<(S.nelem + EPB - 1) / EPB, TTPB, 0, st>>>(S, ph, U, T);
     }

@@ -8,6 +8,7 @@
 #include <memory>
 #include <string>
 
+#include "parallel.hpp"
 #include "solvers.hpp"
 #include "../iceicle_amd/csrc/fields.h"
 
@@ -21,6 +22,7 @@ struct ProblemBase {
     virtual ~ProblemBase() = default;
     virtual void info(int* out) const = 0;
     virtual void residual(const double* u, double* r) = 0;
+    virtual void residual_parallel(const double* u, double* r, int nthreads) = 0;
     virtual void faces(int* recs) const = 0;
     virtual void mesh_arrays(double* coord, int* conn) const = 0;
     virtual void project(const ice_field* f, double* u) const = 0;
@@ -68,6 +70,11 @@ struct Problem final : ProblemBase {
         out[8] = fs.mesh.nnode; out[9] = (int)fs.periodic.size();
     }
     void residual(const double* u, double* r) override { form_residual(fs, disc, u, r); }
+    ParallelPlan<D> plan;
+    std::vector<double> fbuf;
+    void residual_parallel(const double* u, double* r, int nthreads) override {
+        form_residual_parallel(fs, disc, u, r, plan, fbuf, nthreads);
+    }
     void faces(int* recs) const override {
         constexpr int NFV = Mesh<D>::NFV;
         int k = 0;
@@ -273,6 +280,10 @@ void ice_oracle_destroy(void* h) { delete static_cast<Handle*>(h); }
 
 int ice_oracle_info(void* h, int* out) { ORACLE_TRY(static_cast<Handle*>(h)->prob->info(out)) }
 int ice_oracle_residual(void* h, const double* u, double* r) { ORACLE_TRY(static_cast<Handle*>(h)->prob->residual(u, r)) }
+int ice_oracle_residual_parallel(void* h, const double* u, double* r, int nthreads) {
+    ORACLE_TRY(static_cast<Handle*>(h)->prob->residual_parallel(u, r, nthreads))
+}
+int ice_oracle_max_threads() { return omp_get_max_threads(); }
 int ice_oracle_faces(void* h, int* recs) { ORACLE_TRY(static_cast<Handle*>(h)->prob->faces(recs)) }
 int ice_oracle_mesh(void* h, double* coord, int* conn) { ORACLE_TRY(static_cast<Handle*>(h)->prob->mesh_arrays(coord, conn)) }
 int ice_oracle_project(void* h, const ice_field* f, double* u) { ORACLE_TRY(static_cast<Handle*>(h)->prob->project(f, u)) }

@@ -16,6 +16,11 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 
+def max_threads():
+    """Host threads OpenMP would use (OMP_NUM_THREADS, else the cores)."""
+    return int(lib().ice_oracle_max_threads())
+
+
 def lib():
     global _LIB
     if _LIB is None:
@@ -33,6 +38,7 @@ def lib():
         for name, args in [
             ("ice_oracle_info", [P, ip]),
             ("ice_oracle_residual", [P, dp, dp]),
+            ("ice_oracle_residual_parallel", [P, dp, dp, C.c_int]),
             ("ice_oracle_faces", [P, ip]),
             ("ice_oracle_mesh", [P, dp, ip]),
             ("ice_oracle_project", [P, C.POINTER(abi.Field), dp]),
@@ -63,6 +69,7 @@ def lib():
         L.ice_oracle_l2_error.restype = C.c_double
         L.ice_oracle_newton.argtypes = [P, dp, C.c_double, C.c_double, C.c_int, dp, C.c_int]
         L.ice_oracle_newton.restype = C.c_int
+        L.ice_oracle_max_threads.restype = C.c_int
         L.ice_oracle_cfl_dt.argtypes = [P, C.c_double]
         L.ice_oracle_cfl_dt.restype = C.c_double
         _LIB = L
@@ -106,6 +113,15 @@ class Oracle:
         u = np.ascontiguousarray(u, dtype=np.float64)
         r = np.empty(self.size)
         _check(lib().ice_oracle_residual(self.h, _dp(u), _dp(r)))
+        return r
+
+    def residual_parallel(self, u, nthreads=0):
+        """The same residual, bit for bit, from the element/face-parallel loop
+        on nthreads host threads (0: every available core)."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        r = np.empty(self.size)
+        n = nthreads if nthreads > 0 else max_threads()
+        _check(lib().ice_oracle_residual_parallel(self.h, _dp(u), _dp(r), int(n)))
         return r
 
     def faces(self):

@@ -7,6 +7,7 @@
 #include "../../oracle/capi.cpp"
 
 #include <cstdio>
+#include <cstring>
 #include <vector>
 
 namespace {
@@ -78,6 +79,9 @@ void run(const char* name, const ice_mesh_desc& m, int order, const ice_disc_par
     const int n = info[1] * info[2] * info[3];
     std::vector<double> u = state(p.flux, m.ndim, n), r(n), out(n);
     expect(ice_oracle_residual(h, u.data(), r.data()) == 0, name);
+    // the all-core variant is bit-identical to the serial loop
+    expect(ice_oracle_residual_parallel(h, u.data(), out.data(), 4) == 0, name);
+    expect(std::memcmp(r.data(), out.data(), n * sizeof(double)) == 0, name);
     expect(ice_oracle_mass_apply(h, r.data(), out.data()) == 0, name);
     std::vector<int> recs(info[4] * (6 + (1 << (m.ndim - 1))));
     expect(ice_oracle_faces(h, recs.data()) == 0, name);

@@ -51,6 +51,12 @@ def _compare(m, order, params, u):
     torch.cuda.synchronize()
     assert torch.equal(rd, r2)
     assert cases.rel_inf(rd.cpu().numpy(), res_ref) <= 1e-12
+    # owner-ordered accumulation: a second assembly gives the same bits
+    first = jac.blocks.clone()
+    jac.blocks.fill_(float("nan"))
+    jac.assemble(ud, rd)
+    torch.cuda.synchronize()
+    assert torch.equal(jac.blocks, first)
     blk = jac.blocks.cpu().numpy()
     scale = np.max(np.abs(blk_ref))
     err = float(np.max(np.abs(blk - blk_ref)) / scale)

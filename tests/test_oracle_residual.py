@@ -174,3 +174,32 @@ def test_rk3_ssp_textbook_third_order():
     errs = [np.abs(o.rk3_ssp(u0, T / n, n) - ref).max() for n in (4, 8)]
     rate = math.log2(errs[0] / errs[1])
     assert 2.7 < rate < 3.4, (errs, rate)
+
+
+@pytest.mark.parametrize("case", ["ns2d_dirichlet", "burgers_ddgic_mixed", "burgers_periodic_ref", "euler3d"])
+def test_parallel_oracle_bit_identical_to_serial(case):
+    """The all-core oracle (the CPU baseline on every host core) sums each
+    element's face and volume contributions in the serial loop's order, so
+    its residual equals the serial one bit for bit."""
+    from iceicle_amd import workloads
+    from tests import cases
+    if case == "ns2d_dirichlet":
+        mesh, order, params, _ = workloads.ns_mms_2d(6, 3)
+        o = O.Oracle(mesh, abi.space_desc(order), params)
+    elif case == "burgers_ddgic_mixed":
+        p = cases.burgers_params(2, sigma_ic=0.5, a=(0.3, -0.2, 0))
+        m = abi.mesh_desc(2, [5, 4], [0, 0], [1, 1], ["DIRICHLET", "NEUMANN", "EXTRAPOLATION", "DIRICHLET"],
+                          bcflag=[0, 1, 0, 1])
+        o = O.Oracle(m, abi.space_desc(2), p)
+    elif case == "burgers_periodic_ref":
+        m = abi.mesh_desc(2, [4, 5], [0, 0], [1, 1], ["PERIODIC"] * 4)
+        o = O.Oracle(m, abi.space_desc(3), cases.burgers_params(2))
+    else:
+        mesh, order, params, _ = workloads.euler_vortex_3d(3, 2)
+        o = O.Oracle(mesh, abi.space_desc(order), params)
+    neq = o.size // (o.nelem * o.nbasis)
+    ndim = 3 if case == "euler3d" else 2
+    u = cases.random_state(abi.FLUX_BURGERS if neq == 1 else abi.FLUX_EULER, ndim, o.nelem, o.nbasis, 4)
+    r1 = o.residual(u)
+    for nt in (1, 3, 8):
+        assert np.array_equal(o.residual_parallel(u, nt), r1), (case, nt)
