@@ -166,6 +166,34 @@ int disc_residual(ice_disc* d, const double* u, double* r, hipStream_t st) {
     return residual_fp64(s->ndim, s->p, d->prm.flux, s->dev, d->ph, d->dd, u, r, st, nullptr);
 }
 
+// FP64 residual whose per-call buffers (element face traces, face fluxes,
+// DDGIC lifting vectors, per-workgroup wave-speed maxima) are carved from
+// `region` (cap doubles, 256-byte aligned) instead of the discretization's
+// own.  *need receives the doubles required; with region == nullptr only
+// the size is computed.  Every carved entry the kernels read is written
+// earlier in the same call, so the region needs no clearing.
+int disc_residual_in(ice_disc* d, const double* u, double* r, hipStream_t st, double* region, size_t cap,
+                     size_t* need) {
+    const ice_space* s = d->space;
+    const int D = s->ndim;
+    auto up = [](size_t n) { return (n + 31) / 32 * 32; };  // 256-byte sub-buffers
+    const size_t rows = (size_t)(s->dev.nfi + s->dev.nfb) * s->nfq * d->neq;
+    const size_t nT = up(std::max<size_t>(trace_buffer_size(D, s->p, d->prm.flux, s->mesh.nelem), 1));
+    const size_t nQ = up(std::max<size_t>(rows, 1));
+    const size_t nP = d->ddgic ? up(rows * D) : 0;
+    const size_t nW = d->dd.wave ? up(2 * std::max<size_t>(s->mesh.nelem, 1)) : 0;
+    const size_t total = nT + nQ + nP + nW;
+    if (need) *need = total;
+    if (!region) return 0;
+    if (total > cap) return -2;
+    DiscDev dd = d->dd;
+    dd.T = region;
+    dd.Q = region + nT;
+    dd.Pic = d->ddgic ? region + nT + nQ : nullptr;
+    if (d->dd.wave) dd.wave_part = region + nT + nQ + nP;
+    return residual_fp64(D, s->p, d->prm.flux, s->dev, d->ph, dd, u, r, st, nullptr);
+}
+
 // residual (if r) and FD block Jacobian at u
 int disc_jacobian(ice_disc* d, const double* u, double* r, double* J, hipStream_t st) {
     const ice_space* s = d->space;

@@ -37,6 +37,8 @@ _SIGS = {
     "ice_disc_query": ([P, C.POINTER(C.c_int), C.POINTER(C.c_int64)], C.c_int),
     "ice_residual": ([P, P, P, P], C.c_int),
     "ice_residual_mixed": ([P, P, P, P], C.c_int),
+    "ice_residual_noalloc": ([P, P, P, P], C.c_int),
+    "ice_noalloc_scratch_bytes": ([P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
     "ice_disc_update": ([P, C.POINTER(abi.DiscParams)], C.c_int),
     "ice_space_create_batch": ([C.POINTER(abi.MeshDesc), C.c_int, C.POINTER(abi.SpaceDesc), C.POINTER(P)], C.c_int),
     "ice_disc_set_element_viscosity": ([P, P], C.c_int),

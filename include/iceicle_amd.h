@@ -277,6 +277,23 @@ int ice_residual(ice_disc* disc, const double* u_dev, double* res_dev, void* str
  * largest flux terms in absolute error. */
 int ice_residual_mixed(ice_disc* disc, const double* u_dev, double* r_dev, void* stream);
 
+/* Allocation-free FP64 residual.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same result as
+ * ice_residual (same kernels, same summation order), but the per-call
+ * buffers (face traces, face fluxes, DDGIC lifting vectors, wave-speed
+ * partials) live in one fixed array of ICE_NOALLOC_SCRATCH_MB MiB of device
+ * memory per GPU, held until the process ends; the call itself allocates
+ * nothing.  A discretization needing more scratch than that (e.g. 2D NS P3
+ * beyond ~190 x 190 elements) gets ICE_ERR_ARG: use ice_residual.  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU (other
+ * streams or threads) race; calls ordered on one stream do not. */
+#define ICE_NOALLOC_SCRATCH_MB 64
+int ice_residual_noalloc(ice_disc* disc, const double* u_dev, double* res_dev, void* stream);
+/* Scratch bytes ice_residual_noalloc needs for this discretization and the
+ * fixed maximum (either pointer may be null; a null disc reports the maximum). */
+int ice_noalloc_scratch_bytes(const ice_disc* disc, int64_t* need_bytes, int64_t* max_bytes);
+
 /* Inverse mass matrix per element and component: out = M_e^-1 r.  Replaces
  * calculate_mass_matrix + decompose_lu + sub_lu of the explicit steppers
  * (finite_element.hpp:586-611, ssp_rk3.hpp:142-163); singular element mass
