@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 counter runs that measure the residual kernels' HBM traffic")
     ap.add_argument("--no-c1", action="store_true", help="skip the 2D Burgers 32x32 P2 RK3-SSP leg")
+    ap.add_argument("--quick-legs", action="store_true",
+                    help="few repetitions in the 3D leg (for profiler runs)")
     ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--tile", type=int, default=0, help=argparse.SUPPRESS)
     return ap.parse_args()
@@ -114,13 +116,15 @@ COPY_MB = 256
 
 def _counter_means(csv_dir, counter):
     """Mean counter value per dispatch, by kernel-name prefix, from a
-    rocprofv3 counter_collection.csv."""
+    rocprofv3 counter_collection.csv.  For the calibration copy only the
+    largest copy dispatches count (the run's small host-to-device staging
+    copies use the same blit kernel)."""
     import csv
     import glob
     files = glob.glob(os.path.join(csv_dir, "**", "*counter_collection.csv"), recursive=True)
     if not files:
         raise RuntimeError(f"no counter_collection.csv under {csv_dir}")
-    sums = {}
+    rows = {}
     with open(files[0]) as fh:
         for row in csv.DictReader(fh):
             if row.get("Counter_Name") != counter:
@@ -130,13 +134,14 @@ def _counter_means(csv_dir, counter):
             for k, prefix in KERNEL_KEYS:
                 if prefix in name:
                     key = k
-            if key is None and "copyBuffer" in name or (key is None and "copy" in name.lower()):
+            if key is None and "copy" in name.lower():
                 key = "copy"
-            if key is None:
-                continue
-            tot, n = sums.get(key, (0.0, 0))
-            sums[key] = (tot + float(row["Counter_Value"]), n + 1)
-    return {k: tot / n for k, (tot, n) in sums.items()}
+            if key is not None:
+                rows.setdefault(key, []).append((int(row.get("Grid_Size", 0) or 0), float(row["Counter_Value"])))
+    if "copy" in rows:
+        big = max(g for g, _ in rows["copy"])
+        rows["copy"] = [(g, v) for g, v in rows["copy"] if g == big]
+    return {k: sum(v for _, v in r) / len(r) for k, r in rows.items()}
 
 
 def measure_traffic(args, tile, timeout=150):
@@ -349,18 +354,27 @@ def euler3d(args, n=64, order=3, nres=30, nsteps=50):
     res_ms = start.elapsed_time(stop) / nres
     stage_ms, ncalls, _ = kernel_event_pass(engine, space, disc, u, r, "fp64", nres)
     assert bool(torch.isfinite(r).all())
-    # mixed-precision residual (FP32 interpolation, fluxes and face data)
+    # mixed-precision residual (FP32 interpolation, fluxes and face data),
+    # with each element-kernel tile: 0-2 the VALU kernel in 256 / 128 / 64
+    # threads, 3 the kernel whose tensor contractions run on the matrix cores
     r32 = torch.empty_like(u)
-    for _ in range(3):
-        engine.form_residual(space, disc, u, r32, precision="mixed")
-    torch.cuda.synchronize()
-    start.record()
-    for _ in range(nres):
-        engine.form_residual(space, disc, u, r32, precision="mixed")
-    stop.record()
-    torch.cuda.synchronize()
-    mixed_ms = start.elapsed_time(stop) / nres
-    mixed_err = float((r32 - r).abs().max() / r.abs().max())
+    per_tile = {}
+    for t in range(4):
+        disc.set_tile(t)
+        for _ in range(3):
+            engine.form_residual(space, disc, u, r32, precision="mixed")
+        torch.cuda.synchronize()
+        start.record()
+        for _ in range(nres):
+            engine.form_residual(space, disc, u, r32, precision="mixed")
+        stop.record()
+        torch.cuda.synchronize()
+        per_tile[t] = (start.elapsed_time(stop) / nres, float((r32 - r).abs().max() / r.abs().max()))
+    mixed_tile = min(per_tile, key=lambda t: per_tile[t][0])
+    disc.set_tile(mixed_tile)
+    mstage, mcalls, _ = kernel_event_pass(engine, space, disc, u, r32, "mixed", nres)
+    mixed_ms, mixed_err = per_tile[mixed_tile]
+    valu_best = min((0, 1, 2), key=lambda t: per_tile[t][0])
     # explicit steps at a fixed dt (CFL 0.2 over the element size and the
     # initial maximum wave speed |v| + c ~ 2.9 + 1.1)
     h = 40.0 / n
@@ -382,7 +396,13 @@ def euler3d(args, n=64, order=3, nres=30, nsteps=50):
         "unit": "GDOF/s", "n_gpus": 1, "ms_per_residual": round(res_ms, 4),
         "kernels_ms": {k: round(v / ncalls, 4) for k, v in zip(("trace", "flux", "element"), stage_ms)},
         "mixed": {"value": round(ndof / (mixed_ms * 1e-3) / 1e9, 4), "ms_per_residual": round(mixed_ms, 4),
-                  "rel_err_vs_fp64": mixed_err},
+                  "rel_err_vs_fp64": mixed_err, "tile": mixed_tile,
+                  "kernels_ms": {k: round(v / mcalls, 4) for k, v in zip(("trace", "flux", "element"), mstage)},
+                  "element_kernel_matrix_cores": {
+                      "ms_per_residual": round(per_tile[3][0], 4), "rel_err_vs_fp64": per_tile[3][1],
+                      "instruction": "v_mfma_f32_4x4x1f32 (16 blocks of 4 x 4 per wave, one tensor line per lane)"},
+                  "element_kernel_valu_best": {"tile": valu_best, "ms_per_residual": round(per_tile[valu_best][0], 4),
+                                               "rel_err_vs_fp64": per_tile[valu_best][1]}},
         "rk3_ssp_step_ms": round(step_ms, 4), "rk3_steps_timed": nsteps, "dt": dt,
         "element_tile": {"best": tile, "ms": [round(t, 4) for t in tile_ms]},
         "higher_is_better": True, "dtype": "f64", "data": "synthetic",
@@ -542,7 +562,7 @@ def main():
         traffic_probe(args)
         return
     if args.workload == "euler3d":
-        print(json.dumps(euler3d(args)))
+        print(json.dumps(euler3d(args, nres=5, nsteps=2) if args.quick_legs else euler3d(args)))
         return
     if args.workload == "warmstart":
         print(json.dumps(warmstart_leg(args.warmstart_train)))
@@ -652,8 +672,8 @@ def main():
         # Newton-linesearch from a uniform free-stream state (rho 1, u 0.3,
         # v 0, p 1/gamma) to ||R||_2 < 1e-10
         un = torch.from_numpy(fs_state).cuda()
-        solver = engine.PetscNewton(disc, tau_abs=1e-10, tau_rel=0.0, kmax=30, forcing=abi.FORCING_EW,
-                                    preconditioner=abi.PC_MULTIGRID, **kw)
+        kw.setdefault("preconditioner", abi.PC_MULTIGRID)
+        solver = engine.PetscNewton(disc, tau_abs=1e-10, tau_rel=0.0, kmax=30, forcing=abi.FORCING_EW, **kw)
         print(f"newton ({label}) ...", file=sys.stderr, flush=True)
         rep = solver.solve(un)
         print(f"newton ({label}) done: {rep['iterations']} steps, {rep['seconds']:.2f} s", file=sys.stderr,
@@ -676,11 +696,19 @@ def main():
         newton["linear_solver"] = ("GMRES(30) (classical Gram-Schmidt, one pass) with an agglomeration multigrid "
                                    "V-cycle (P0 Galerkin coarse levels, block SGS smoothing on FP16 fine blocks), "
                                    "FP32 Jacobian blocks + FP64 refinement, Eisenstat-Walker forcing")
-        newton["fp64_jacobian"] = newton_leg("Wolfe, FP64 blocks", linesearch=abi.LS_WOLFE,
-                                             jacobian_precision=abi.PREC_FP64, smoother_precision=abi.PREC_FP32)
-        newton["fp64_jacobian"]["linear_solver"] = "the same with FP64 Jacobian blocks and FP32 smoother blocks"
         newton["no_linesearch"] = newton_leg("full steps, FP32 blocks", linesearch=abi.LS_NONE,
                                              smoother_precision=abi.PREC_FP16)
+        # FP32 Jacobian blocks + FP64 refinement against FP64 blocks, both
+        # with the multicolour block-SGS preconditioner (the multigrid
+        # hierarchy is built from the FP32 blocks only)
+        newton["block_sgs"] = {
+            "fp32_blocks_fp64_refinement": newton_leg("Wolfe, FP32 blocks, SGS", linesearch=abi.LS_WOLFE,
+                                                      preconditioner=abi.PC_BLOCK_SGS,
+                                                      smoother_precision=abi.PREC_FP32),
+            "fp64_blocks": newton_leg("Wolfe, FP64 blocks, SGS", linesearch=abi.LS_WOLFE,
+                                      preconditioner=abi.PC_BLOCK_SGS, jacobian_precision=abi.PREC_FP64,
+                                      smoother_precision=abi.PREC_FP32),
+        }
 
     out = {
         "metric": "residual GDOF/s (FP64 vs mixed, 1 GPU); Newton time to ||R||<1e-10",

@@ -5,6 +5,8 @@ exact structs, so a test builds one descriptor and hands it to both.
 """
 import ctypes as C
 
+import numpy as np
+
 ICE_MAX_BCFLAG = 4
 
 # status codes
@@ -149,7 +151,8 @@ def default_params(flux):
 def mesh_desc(ndim, nelem, xmin, xmax, bctype, bcflag=None, coord=None, periodic_mode=PERIODIC_REFERENCE):
     """Build a MeshDesc.  `bctype` entries may be names from BC or ints.
     `coord` (optional) is a C-contiguous float64 numpy array [nnode][ndim];
-    the caller keeps it alive while the descriptor is used."""
+    the descriptor keeps a reference to it, so it lives as long as the
+    descriptor (the C struct holds only its address)."""
     m = MeshDesc()
     m.ndim = ndim
     for d in range(ndim):
@@ -161,7 +164,9 @@ def mesh_desc(ndim, nelem, xmin, xmax, bctype, bcflag=None, coord=None, periodic
         m.bctype[k] = BC[b] if isinstance(b, str) else int(b)
         m.bcflag[k] = 0 if bcflag is None else int(bcflag[k])
     if coord is not None:
+        coord = np.ascontiguousarray(coord, dtype=np.float64)
         m.coord = coord.ctypes.data_as(C.POINTER(C.c_double))
+        m._coord_owner = coord
     m.periodic_mode = periodic_mode
     return m
 

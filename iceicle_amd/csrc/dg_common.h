@@ -403,14 +403,16 @@ __device__ __forceinline__ void interior_point_flux(const typename PH::Params& p
 template <int D, class PH>
 __device__ __forceinline__ void interior_point_flux_split(const typename PH::Params& ph, const typename PH::real* uL, const typename PH::real* uR,
                                                           const typename PH::real* gs, const typename PH::real* nrm, typename PH::real h, typename PH::real ws,
-                                                          typename PH::real* Q) {
+                                                          typename PH::real* Q, const typename PH::real* jmp = nullptr) {
+    // jmp: uR - uL formed in higher precision by the caller (FP32 fluxes),
+    // else the difference of the two arguments
     constexpr int M = PH::M;
     using R = typename PH::real;
     R fadv[M], fvis[M], gddg[M * D], uavg[M];
     PH::conv(ph, uL, uR, nrm, fadv);
 #pragma unroll
     for (int v = 0; v < M; ++v) {
-        const R jump = uR[v] - uL[v];
+        const R jump = jmp ? jmp[v] : uR[v] - uL[v];
 #pragma unroll
         for (int d = 0; d < D; ++d) gddg[v * D + d] = ph.beta0 * jump / h * nrm[d] + gs[v * D + d];
         uavg[v] = R(0.5) * (uL[v] + uR[v]);
@@ -427,7 +429,10 @@ __device__ __forceinline__ void interior_point_flux_split(const typename PH::Par
 template <int D, class PH>
 __device__ __forceinline__ bool boundary_point_flux(const typename PH::Params& ph, int bctype, int bcflag, const typename PH::real* uL,
                                                     const typename PH::real* gL, const typename PH::real* bv, const typename PH::real* nrm, typename PH::real hb,
-                                                    typename PH::real ws, typename PH::real* Q, typename PH::real* uB_out) {
+                                                    typename PH::real ws, typename PH::real* Q, typename PH::real* uB_out,
+                                                    const typename PH::real* jmp_dirichlet = nullptr) {
+    // jmp_dirichlet: bv - uL formed in higher precision by the caller (FP32
+    // fluxes on Dirichlet faces), else the difference of the arguments
     constexpr int M = PH::M;
     using R = typename PH::real;
     R fadv[M], fvis[M], gddg[M * D], uavg[M], uB[M];
@@ -465,7 +470,7 @@ __device__ __forceinline__ bool boundary_point_flux(const typename PH::Params& p
     PH::conv(ph, uL, uB, nrm, fadv);
 #pragma unroll
     for (int v = 0; v < M; ++v) {
-        const R jump = uB[v] - uL[v];
+        const R jump = (jmp_dirichlet && bctype == ICE_BC_DIRICHLET) ? jmp_dirichlet[v] : uB[v] - uL[v];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             const R gd = ph.beta0_bdy * jump / hb * nrm[d] + gL[v * D + d];

@@ -817,7 +817,7 @@ int ice_disc_set_element_viscosity(ice_disc* d, const double* mu) {
 
 int ice_disc_set_tile(ice_disc* d, int tile) {
     if (!d) return fail(ICE_ERR_ARG, "null argument");
-    if (tile < 0 || tile > 2) return fail(ICE_ERR_ARG, "tile must be 0, 1 or 2");
+    if (tile < 0 || tile > 3) return fail(ICE_ERR_ARG, "tile must be 0, 1, 2 or 3");
     d->dd.tile = tile;
     d->dd.tile_mixed = tile;
     return ICE_OK;
@@ -840,7 +840,9 @@ int ice_disc_autotune(ice_disc* d, const double* u, double* r, int reps, int* be
     // the bytes, so the fastest shape can differ)
     for (int prec = 0; prec < 2 && rc == ICE_OK; ++prec) {
         double tbest = 1e300;
-        for (int tile = 0; tile < 3 && rc == ICE_OK; ++tile) {
+        // tile 3 (matrix cores) exists for the mixed residual of 3D P3 Euler only
+        const bool mfma = prec == 1 && s->ndim == 3 && s->p == 3 && d->prm.flux == ICE_FLUX_EULER;
+        for (int tile = 0; tile < (mfma ? 4 : 3) && rc == ICE_OK; ++tile) {
             d->dd.tile = tile;
             d->dd.tile_mixed = tile;
             auto once = [&]() {

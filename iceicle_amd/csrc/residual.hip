@@ -18,6 +18,14 @@
 //     (element, basis function) and sums the volume term
 //     (domain_integral, :244-314), the source term and the lifted face fluxes
 //     of the element's 2*ndim faces, then writes R with one coalesced store.
+// Mixed precision (FP32 arithmetic) keeps each element's coefficients as FP32
+// differences from the element's first coefficient, dU_i = U_i - U_0 (formed
+// in FP64), with U_0 in FP64.  Derivatives see only dU (the basis gradients
+// sum to zero), values are U_0 + sum dU_i phi_i with the sum in FP32 and the
+// addition in FP64, and the face jumps uR - uL are formed in FP64 before the
+// DDG penalty multiplies them by beta0 / h.  FP32 rounding then scales with
+// the variation of u inside an element (~h |grad u|) instead of |u|, so the
+// derivative, jump and penalty terms keep FP64-like accuracy.
 // The face-flux buffer replaces the reference's scatter of per-face residual
 // blocks (form_residual.hpp:137-242): every element gathers its own faces, so
 // the result is deterministic and needs no atomics.
@@ -78,6 +86,20 @@ template <>
 __device__ __forceinline__ const double* tab_of<double>(const DevSpace& S) { return S.tab; }
 template <>
 __device__ __forceinline__ const float* tab_of<float>(const DevSpace& S) { return S.tabf; }
+
+// Staged coefficient idx of the tile starting at U[base] ([element][node][var]):
+// FP64 as is; FP32 as the difference from the element's first coefficient of
+// the same variable (dU = U_i - U_0, the mixed-precision representation)
+template <class VT, int NB, int M>
+__device__ __forceinline__ VT coef_stage(const double* __restrict__ U, size_t base, size_t total, int idx) {
+    if (base + idx >= total) return VT(0);
+    if constexpr (std::is_same<VT, double>::value) {
+        return U[base + idx];
+    } else {
+        const int l = idx / (NB * M), v = idx % M;
+        return (VT)(U[base + idx] - U[base + (size_t)l * NB * M + v]);
+    }
+}
 
 // Flux parameters of element e: a per-element viscosity (batched instances)
 // replaces the discretization's when given
@@ -364,7 +386,7 @@ __global__ __launch_bounds__(TPB, ICE_TRACE_MINW) void trace_kernel(DevSpace S, 
         const size_t base = (size_t)e0 * NB * M;
         const size_t total = (size_t)S.nelem * NB * M;
         for (int idx = threadIdx.x; idx < EPB * NB * M; idx += blockDim.x)
-            sU[idx] = (VT)((base + idx < total) ? U[base + idx] : 0.0);
+            sU[idx] = coef_stage<VT, NB, M>(U, base, total, idx);
         const VT* tv = tab_of<VT>(S) + (N + 2);
         for (int idx = threadIdx.x; idx < 3 * (N + 2) * N; idx += blockDim.x) sL[idx] = tv[idx];
     }
@@ -410,10 +432,12 @@ __global__ __launch_bounds__(TPB, ICE_TRACE_MINW) void trace_kernel(DevSpace S, 
 template <int D, class PH>
 __device__ __forceinline__ void interior_face_point(const DevSpace& S, const typename PH::Params& ph, const DiscDev& dd,
                                                     int f, int q, int NFQ, const typename PH::real* tL,
-                                                    const typename PH::real* tR) {
+                                                    const typename PH::real* tR, const double* U0L,
+                                                    const double* U0R) {
     using VT = typename PH::real;
     constexpr int M = PH::M;
     constexpr int NV = TraceRec<PH, D>::NV;
+    constexpr bool DELTA = std::is_same<VT, float>::value;  // FP32 traces hold u - U_0
     VT* const Qb = reinterpret_cast<VT*>(dd.Q);
     VT* const Pb = reinterpret_cast<VT*>(dd.Pic);
     const double* g = S.fgeo + ((size_t)f * NFQ + q) * (D + 2);
@@ -421,11 +445,19 @@ __device__ __forceinline__ void interior_face_point(const DevSpace& S, const typ
 #pragma unroll
     for (int d = 0; d < D; ++d) nrm[d] = (VT)g[d];
     const VT ws = (VT)g[D], h = (VT)g[D + 1];
-    VT uL[M], uR[M], gs[M * D];
+    VT uL[M], uR[M], gs[M * D], jmp[M];
 #pragma unroll
     for (int v = 0; v < M; ++v) {
-        uL[v] = tL[v];
-        uR[v] = tR[v];
+        if constexpr (DELTA) {
+            const double a = U0L[v] + (double)tL[v], b = U0R[v] + (double)tR[v];
+            uL[v] = (VT)a;
+            uR[v] = (VT)b;
+            jmp[v] = (VT)(b - a);
+        } else {
+            uL[v] = tL[v];
+            uR[v] = tR[v];
+            jmp[v] = uR[v] - uL[v];
+        }
     }
     if constexpr (NV > 1) {
 #pragma unroll
@@ -437,7 +469,7 @@ __device__ __forceinline__ void interior_face_point(const DevSpace& S, const typ
         for (int k = 0; k < M * D; ++k) gs[k] = VT(0);
     }
     VT Qv[M];
-    interior_point_flux_split<D, PH>(ph, uL, uR, gs, nrm, h, ws, Qv);
+    interior_point_flux_split<D, PH>(ph, uL, uR, gs, nrm, h, ws, Qv, jmp);
     VT* Qf = Qb + ((size_t)f * NFQ + q) * M;
 #pragma unroll
     for (int v = 0; v < M; ++v) Qf[v] = Qv[v];
@@ -457,7 +489,7 @@ __device__ __forceinline__ void interior_face_point(const DevSpace& S, const typ
                     for (int k = 0; k < D; ++k)
 #pragma unroll
                         for (int r = 0; r < M; ++r)
-                            acc += G[((v * D + k) * M + r) * D + s] * nrm[k] * (uR[r] - uL[r]);
+                            acc += G[((v * D + k) * M + r) * D + s] * nrm[k] * jmp[r];
                     Pf[v * D + s] = ph.sigma_face * acc * ws;
                 }
         }
@@ -469,10 +501,11 @@ __device__ __forceinline__ void interior_face_point(const DevSpace& S, const typ
 template <int D, class PH>
 __device__ __forceinline__ void boundary_face_point(const DevSpace& S, const typename PH::Params& ph, const DiscDev& dd,
                                                     int b, int q, int NFQ, int bctype, int bcflag,
-                                                    const typename PH::real* tL) {
+                                                    const typename PH::real* tL, const double* U0L) {
     using VT = typename PH::real;
     constexpr int M = PH::M;
     constexpr int NV = TraceRec<PH, D>::NV;
+    constexpr bool DELTA = std::is_same<VT, float>::value;  // FP32 traces hold u - U_0
     VT* const Qb = reinterpret_cast<VT*>(dd.Q);
     VT* const Pb = reinterpret_cast<VT*>(dd.Pic);
     const size_t slot = (size_t)S.nfi + b;
@@ -497,10 +530,18 @@ __device__ __forceinline__ void boundary_face_point(const DevSpace& S, const typ
 #pragma unroll
     for (int d = 0; d < D; ++d) nrm[d] = (VT)g[d];
     const VT ws = (VT)g[D], hb = (VT)g[D + 1];
-    VT uL[M] = {}, gL[M * D] = {};
+    VT uL[M] = {}, gL[M * D] = {}, jmpD[M] = {};
     if (bctype != ICE_BC_NEUMANN) {
 #pragma unroll
-        for (int v = 0; v < M; ++v) uL[v] = tL[v];
+        for (int v = 0; v < M; ++v) {
+            if constexpr (DELTA) {
+                const double a = U0L[v] + (double)tL[v];
+                uL[v] = (VT)a;
+                jmpD[v] = (VT)(dd.bval[((size_t)b * NFQ + q) * M + v] - a);
+            } else {
+                uL[v] = tL[v];
+            }
+        }
         if constexpr (NV > 1) {
 #pragma unroll
             for (int v = 0; v < M; ++v)
@@ -512,7 +553,7 @@ __device__ __forceinline__ void boundary_face_point(const DevSpace& S, const typ
 #pragma unroll
     for (int v = 0; v < M; ++v) bv[v] = (VT)dd.bval[((size_t)b * NFQ + q) * M + v];
     VT Qv[M], uB[M];
-    boundary_point_flux<D, PH>(ph, bctype, bcflag, uL, gL, bv, nrm, hb, ws, Qv, uB);
+    boundary_point_flux<D, PH>(ph, bctype, bcflag, uL, gL, bv, nrm, hb, ws, Qv, uB, DELTA ? jmpD : nullptr);
 #pragma unroll
     for (int v = 0; v < M; ++v) Qf[v] = Qv[v];
     if constexpr (PH::HAS_HOMOGENEITY) {
@@ -530,7 +571,7 @@ __device__ __forceinline__ void boundary_face_point(const DevSpace& S, const typ
                     for (int k = 0; k < D; ++k)
 #pragma unroll
                         for (int r = 0; r < M; ++r)
-                            acc += G[((v * D + k) * M + r) * D + s] * nrm[k] * (uB[r] - uL[r]);
+                            acc += G[((v * D + k) * M + r) * D + s] * nrm[k] * (DELTA ? jmpD[r] : uB[r] - uL[r]);
                     Pf[v * D + s] = ph.sigma_bdy * acc * ws;
                 }
         }
@@ -543,10 +584,12 @@ __device__ __forceinline__ void boundary_face_point(const DevSpace& S, const typ
 // holds more registers than the interior flux (Euler: 75 against 50 VGPRs).
 template <int D, int P, class PH, bool BDY>
 __global__ __launch_bounds__(256, ICE_FLUX_MINW) void flux_kernel(DevSpace S, typename PH::Params ph0, DiscDev dd,
-                                                                   const typename PH::real* __restrict__ T, int nbi) {
+                                                                   const typename PH::real* __restrict__ T, int nbi,
+                                                                   const double* __restrict__ U) {
     using VT = typename PH::real;
     constexpr int N = P + 1;
     constexpr int NFQ = Pow<N, D - 1>::value;
+    constexpr int NBM = Pow<N, D>::value * PH::M;  // one element's coefficients (U_0 = the first M)
     constexpr int M = PH::M;
     constexpr int NV = TraceRec<PH, D>::NV;
     if ((int)blockIdx.x < nbi) {
@@ -559,7 +602,8 @@ __global__ __launch_bounds__(256, ICE_FLUX_MINW) void flux_kernel(DevSpace S, ty
         const int qr = S.fperm[(fnr * S.norient + orient) * NFQ + q];
         const VT* tL = T + (((size_t)eL * 2 * D + fnl) * NFQ + q) * NV * M;
         const VT* tR = T + (((size_t)eR * 2 * D + fnr) * NFQ + qr) * NV * M;
-        interior_face_point<D, PH>(S, local_params(ph0, dd.emu, eL), dd, f, q, NFQ, tL, tR);
+        interior_face_point<D, PH>(S, local_params(ph0, dd.emu, eL), dd, f, q, NFQ, tL, tR, U + (size_t)eL * NBM,
+                                   U + (size_t)eR * NBM);
     } else if constexpr (BDY) {
         const int tid = (blockIdx.x - nbi) * blockDim.x + threadIdx.x;
         const int b = tid / NFQ, q = tid - b * NFQ;
@@ -567,7 +611,8 @@ __global__ __launch_bounds__(256, ICE_FLUX_MINW) void flux_kernel(DevSpace S, ty
         const int4 rec = S.fbdy[b];
         const int eL = rec.x, info = rec.y;
         const VT* tL = T + (((size_t)eL * 2 * D + info_fnl(info)) * NFQ + q) * NV * M;
-        boundary_face_point<D, PH>(S, local_params(ph0, dd.emu, eL), dd, b, q, NFQ, info_bctype(info), rec.z, tL);
+        boundary_face_point<D, PH>(S, local_params(ph0, dd.emu, eL), dd, b, q, NFQ, info_bctype(info), rec.z, tL,
+                                   U + (size_t)eL * NBM);
     }
 }
 
@@ -643,8 +688,10 @@ struct ElemTile {
     static constexpr int NT = EPB * NB;
     static constexpr int BLOCK = ((NT + 63) / 64) * 64;
 };
-// workgroup size targets selected by DiscDev::tile
+// workgroup size targets selected by DiscDev::tile (TILE_MFMA: the matrix-core
+// 3D P3 kernel of the mixed residual where it applies, else the default tile)
 constexpr int tile_tpb(int tile) { return tile == 1 ? 128 : tile == 2 ? 64 : 256; }
+constexpr int TILE_MFMA = 3;
 
 template <int D, int P, class PH>
 __global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph0, const double* __restrict__ emu,
@@ -960,6 +1007,7 @@ __global__ __launch_bounds__((ElemTile<2, P, TPB>::BLOCK), ICE_ELEM_MINW) void e
     __shared__ double sLd[NT2];
     __shared__ signed char sFT[2 * D * 2 * NFQ];  // two orientation codes in 2D
     __shared__ double sSL[LOWLDS ? 1 : EPB * NB * M];
+    __shared__ double sU0[LOWLDS ? 1 : EPB * M];  // FP32: each element's first coefficients
     const int e0 = blockIdx.x * EPB;
     const int le = threadIdx.x / NB;
     const int k = threadIdx.x - le * NB;
@@ -972,8 +1020,13 @@ __global__ __launch_bounds__((ElemTile<2, P, TPB>::BLOCK), ICE_ELEM_MINW) void e
         const size_t total = (size_t)S.nelem * NB * M;
         for (int idx = threadIdx.x; idx < EPB * NB * M; idx += blockDim.x) {
             const int l = idx / (NB * M), r = idx - l * (NB * M), nd = r / M, v = r - nd * M;
-            sU[l * ESU + nd * MS + v] = (VT)((base + idx < total) ? U[base + idx] : 0.0);
+            sU[l * ESU + nd * MS + v] = coef_stage<VT, NB, M>(U, base, total, idx);
         }
+        if (!LOWLDS)
+            for (int idx = threadIdx.x; idx < EPB * M; idx += blockDim.x) {
+                const size_t g = base + (size_t)(idx / M) * NB * M + idx % M;
+                sU0[idx] = g < total ? U[g] : 0.0;
+            }
         if (!LOWLDS && srcload)
             for (int idx = threadIdx.x; idx < EPB * NB * M; idx += blockDim.x)
                 sSL[idx] = (base + idx < total) ? srcload[base + idx] : 0.0;
@@ -1071,7 +1124,8 @@ __global__ __launch_bounds__((ElemTile<2, P, TPB>::BLOCK), ICE_ELEM_MINW) void e
                 sg0 += t0 * l1[a];
                 s1 += T1[(a * N + k1) * MS + v] * l0[a];
             }
-            u[v] = s0;
+            if constexpr (LOWLDS) u[v] = s0;
+            else u[v] = (VT)((double)s0 + sU0[le * M + v]);  // U_0 + sum dU_i phi_i
             // grad_x = grad_xi A / det (finite_element.hpp:220-258)
             gx[v * D + 0] = sg0 * a00 + s1 * a10;
             gx[v * D + 1] = sg0 * a01 + s1 * a11;
@@ -1330,6 +1384,8 @@ __global__ __launch_bounds__((Elem3Tile<P, TPB>::BLOCK), ICE_ELEM3_MINW) void el
     __shared__ double sX[EPB * NV * D];
     __shared__ int sEF[EPB * 2 * D];
     __shared__ signed char sFT[2 * D * 8 * NFQ];
+    constexpr bool DELTA = std::is_same<VT, float>::value;  // coefficients staged as U_i - U_0
+    __shared__ double sU0[DELTA ? EPB * M : 1];
     const int e0 = blockIdx.x * EPB;
     const int le = threadIdx.x / NB;
     const int k = threadIdx.x - le * NB;
@@ -1346,8 +1402,13 @@ __global__ __launch_bounds__((Elem3Tile<P, TPB>::BLOCK), ICE_ELEM3_MINW) void el
         for (int idx = threadIdx.x; idx < EPB * NB * M; idx += blockDim.x) {
             const int l = idx / (NB * M), rem = idx - l * (NB * M);
             const int node = rem / M, v = rem - node * M;
-            sbuf[l * EBUF + v * NBP + node] = (VT)((base + idx < total) ? U[base + idx] : 0.0);
+            sbuf[l * EBUF + v * NBP + node] = coef_stage<VT, NB, M>(U, base, total, idx);
         }
+        if (DELTA)
+            for (int idx = threadIdx.x; idx < EPB * M; idx += blockDim.x) {
+                const size_t g = base + (size_t)(idx / M) * NB * M + idx % M;
+                sU0[idx] = g < total ? U[g] : 0.0;
+            }
         const size_t xb = (size_t)e0 * NV * D, xt = (size_t)S.nelem * NV * D;
         for (int idx = threadIdx.x; idx < EPB * NV * D; idx += blockDim.x)
             sX[idx] = (xb + idx < xt) ? S.X[xb + idx] : 0.0;
@@ -1494,7 +1555,8 @@ __global__ __launch_bounds__((Elem3Tile<P, TPB>::BLOCK), ICE_ELEM3_MINW) void el
                     g2 += Bq[2 * ARR + v * NBP + a * NN] * l0[a];
                 }
             }
-            u[v] = s;
+            if constexpr (DELTA) u[v] = (VT)((double)s + sU0[lec * M + v]);  // U_0 + sum dU_i phi_i
+            else u[v] = s;
             // grad_x = grad_xi A / det (finite_element.hpp:220-258)
 #pragma unroll
             for (int j = 0; j < D; ++j) gx[v * D + j] = g0 * ai[0][j] + g1 * ai[1][j] + g2 * ai[2][j];
@@ -1727,6 +1789,310 @@ __global__ __launch_bounds__((Elem3Tile<P, TPB>::BLOCK), ICE_ELEM3_MINW) void el
     for (int v = 0; v < M; ++v) Re[v] = r[v];
 }
 
+// ---- 3D P3 element kernel on the matrix cores (FP32, fluxes without a
+// gradient) ------------------------------------------------------------------
+// The sum-factorised contractions of element_kernel_3d are 4 x 4 operators
+// applied along one axis of each element's [var][4][4][4] tensors.  Here
+// one lane owns one tensor line (4 entries along the contracted axis) and
+// the wave applies the operator to 64 lines at once with
+// v_mfma_f32_4x4x1f32 (16 blocks of 4 x 4 outer products): lane 4b + r
+// supplies operator row r, lane l its line's entry c, and after the four
+// K = 1 steps lane l holds its line's 4 outputs (register r = output r).
+// Each phase writes its outputs rotated, [ev][r][x][y] for the input line
+// [ev][x][y][:], so the next phase's contraction axis is again contiguous
+// (one 16-byte LDS read per line) and three rotations restore the natural
+// point order.  Interpolation to the quadrature points (3 phases), the flux
+// at each point (one thread per point, FP64 geometry), the test
+// contractions (2a: three arrays, 2b: two with the sum of the second and
+// third, 2c: one), then per node the source load, the face lifting and the
+// FP64 store, as element_kernel_3d.
+typedef float ice_floatx4 __attribute__((ext_vector_type(4)));
+
+template <int M>
+struct Elem3Mfma {
+    static constexpr int EPB = 4;                          // elements per workgroup
+    static constexpr int LINES = EPB * M * 16;             // tensor lines per array
+    static constexpr int NWAVE = (LINES / 64 > EPB) ? LINES / 64 : EPB;  // line waves, or a wave per element
+    static constexpr int BLOCK = 64 * NWAVE;
+    // LDS stride of one [ev] tensor: a multiple of 16 floats that is 16 mod
+    // 64, so the rotated stores of the 4 tensors a wave covers fall on
+    // distinct banks
+    static constexpr int EVS = 80;
+    static constexpr int ARR = EPB * M * EVS;              // one array of the workgroup
+};
+
+// 4 x 4 contraction of the lane's line x: returns out[r] = sum_c A[r][c] x[c]
+// when lane l passes a[c] = A[l % 4][c]
+__device__ __forceinline__ ice_floatx4 mfma_line(const float (&a)[4], const float (&x)[4], ice_floatx4 acc) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc = __builtin_amdgcn_mfma_f32_4x4x1f32(a[c], x[c], acc, 0, 0, 0);
+    return acc;
+}
+
+// line L = (ev, x, y) of an array with [ev] stride EVS: its 4 entries start at
+__device__ __forceinline__ int mfma_line_base(int L, int evs) { return (L >> 4) * evs + (L & 15) * 4; }
+// rotated output slot r of line L: [ev][r][x][y]
+__device__ __forceinline__ int mfma_rot_slot(int L, int r, int evs) { return (L >> 4) * evs + r * 16 + (L & 15); }
+
+template <class PH>
+__global__ __launch_bounds__(Elem3Mfma<PH::M>::BLOCK) void element_kernel_3d_mfma(
+    DevSpace S, typename PH::Params ph0, const double* __restrict__ emu, const double* __restrict__ srcload,
+    const double* __restrict__ U, const float* __restrict__ Q, double* __restrict__ R,
+    unsigned long long* __restrict__ wave, double* __restrict__ wave_part) {
+    constexpr int D = 3, N = 4, NN = 16, NB = 64, NFQ = 16;
+    constexpr int M = PH::M;
+    using TL = Elem3Mfma<M>;
+    constexpr int EPB = TL::EPB, LINES = TL::LINES, EVS = TL::EVS, ARR = TL::ARR, BLOCK = TL::BLOCK;
+    constexpr int NV = 1 << D;
+    constexpr int NT2 = (N + 2) * N;
+    constexpr int LIFT = 2 * D * NFQ * M;
+    static_assert(!TraceRec<PH, D>::NEED_GRAD, "fluxes without a gradient");
+    static_assert((LINES % 64) == 0, "whole waves of tensor lines");
+    __shared__ __align__(16) float s0[3 * ARR];
+    __shared__ __align__(16) float s1[3 * ARR];
+    __shared__ float sQ[EPB * LIFT];
+    __shared__ float sL[2 * NT2];
+    __shared__ double sLd[NT2];
+    __shared__ double sX[EPB * NV * D];
+    __shared__ int sEF[EPB * 2 * D];
+    __shared__ signed char sFT[2 * D * 8 * NFQ];
+    __shared__ double red[2 * (BLOCK / 64)];
+    __shared__ double sU0[EPB * M];  // each element's first coefficients (coefficients staged as U_i - U_0)
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
+    const int e0 = blockIdx.x * EPB;
+    // prologue: coefficients as [ev][node] (ev = element * M + var), tables,
+    // vertices, face slots
+    {
+        const size_t base = (size_t)e0 * NB * M, total = (size_t)S.nelem * NB * M;
+        for (int idx = tid; idx < EPB * NB * M; idx += BLOCK) {
+            const int l = idx / (NB * M), rem = idx - l * (NB * M);
+            const int node = rem / M, v = rem - node * M;
+            s0[(l * M + v) * EVS + node] = coef_stage<float, NB, M>(U, base, total, idx);
+        }
+        for (int idx = tid; idx < EPB * M; idx += BLOCK) {
+            const size_t g = base + (size_t)(idx / M) * NB * M + idx % M;
+            sU0[idx] = g < total ? U[g] : 0.0;
+        }
+        const size_t xb = (size_t)e0 * NV * D, xt = (size_t)S.nelem * NV * D;
+        for (int idx = tid; idx < EPB * NV * D; idx += BLOCK) sX[idx] = (xb + idx < xt) ? S.X[xb + idx] : 0.0;
+        const size_t fb = (size_t)e0 * 2 * D, ft = (size_t)S.nelem * 2 * D;
+        for (int idx = tid; idx < EPB * 2 * D; idx += BLOCK) sEF[idx] = (fb + idx < ft) ? S.efaces[fb + idx] : -1;
+        for (int idx = tid; idx < 2 * NT2; idx += BLOCK) sL[idx] = S.tabf[(N + 2) + idx];
+        for (int idx = tid; idx < NT2; idx += BLOCK) sLd[idx] = S.tab[(N + 2) + idx];
+        for (int idx = tid; idx < 2 * D * 8 * NFQ && idx < 2 * D * S.norient * NFQ; idx += BLOCK)
+            sFT[idx] = S.ftens[idx];
+    }
+    __syncthreads();
+    // face fluxes in the face's tensor order (as element_kernel_3d): loaded
+    // into registers now, stored to LDS after the first contraction phase,
+    // so their latency overlaps it
+    constexpr int QIT = (EPB * LIFT + BLOCK - 1) / BLOCK;
+    float qv[QIT];
+    int qdst[QIT];
+#pragma unroll
+    for (int it = 0; it < QIT; ++it) {
+        const int idx = tid + it * BLOCK;
+        const int l = idx / LIFT, rem = idx - l * LIFT;
+        const int j = rem / (NFQ * M), w = rem - j * (NFQ * M);
+        const int fq = w / M, v = w - fq * M;
+        const int ef = (idx < EPB * LIFT) ? sEF[l * 2 * D + j] : -1;  // faces without a slot are not lifted
+        qv[it] = (ef >= 0) ? Q[(size_t)(ef >> 4) * NFQ * M + w] : 0.f;
+        qdst[it] = (ef >= 0) ? l * LIFT + (j * NFQ + sFT[(j * S.norient + (ef & 7)) * NFQ + fq]) * M + v : -1;
+    }
+    // per-lane operator rows: interpolation a[c] = L0[point r][basis c],
+    // test contractions with L0^T and L1^T: a[c] = L[point c][basis r]
+    const int r4 = lane & 3;
+    float aI[4], aT0[4], aT1[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        aI[c] = sL[r4 * N + c];
+        aT0[c] = sL[c * N + r4];
+        aT1[c] = sL[NT2 + c * N + r4];
+    }
+    // one rotation pass: lines of `in` (any count, uniform per wave) -> `out`
+    auto rotate = [&](const float* in, float* out, const float (&a)[4]) {
+#pragma unroll 1
+        for (int g = wv; g < LINES / 64; g += BLOCK / 64) {
+            const int L = g * 64 + lane;
+            const float4 xv = *reinterpret_cast<const float4*>(in + mfma_line_base(L, EVS));
+            const float x[4] = {xv.x, xv.y, xv.z, xv.w};
+            ice_floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+            acc = mfma_line(a, x, acc);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) out[mfma_rot_slot(L, r, EVS)] = acc[r];
+        }
+    };
+    // 1a, 1b, 1c: u at the quadrature points, natural order, in s1[0]
+    rotate(s0, s1, aI);
+#pragma unroll
+    for (int it = 0; it < QIT; ++it)
+        if (qdst[it] >= 0) sQ[qdst[it]] = qv[it];
+    __syncthreads();
+    rotate(s1, s0, aI);
+    __syncthreads();
+    rotate(s0, s1, aI);
+    __syncthreads();
+    // flux at each point: contravariant fluxes G_a = F . adj(J)[a] detc w / det into s0[a]
+    const double* xs = S.tab;
+    const double* wgl = S.tab + (N + 2) + 3 * (N + 2) * N;
+    double lam_w = 0.0, vis_w = 0.0;
+    for (int t = tid; t < EPB * NB; t += BLOCK) {
+        const int l = t / NB, k = t - l * NB;
+        const int e = e0 + l;
+        if (e >= S.nelem) continue;
+        const int k0 = k / NN, k1 = (k / N) % N, k2 = k % N;
+        const typename PH::Params ph = local_params(ph0, emu, e);
+        double X[NV][D];
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+#pragma unroll
+            for (int d = 0; d < D; ++d) X[i][d] = sX[(l * NV + i) * D + d];
+        const double xi[D] = {xs[k0], xs[k1], xs[k2]};
+        const double wq = wgl[k0] * wgl[k1] * wgl[k2];
+        double J[D][D], Hm[D][D][D], A[D][D];
+        q1_map<D>(X, xi, nullptr, J, Hm);
+        const double det = det_adj<D>(J, A);
+        const double detc = fmax(0.0, det);
+        const double detg = (det == 0.0) ? 1.0 : det;
+        const double fac = detc * wq / detg;
+        float u[M], gx[M * D] = {}, F[M * D];
+#pragma unroll
+        for (int v = 0; v < M; ++v) u[v] = (float)((double)s1[(l * M + v) * EVS + k] + sU0[l * M + v]);
+        PH::flux(ph, u, gx, F);
+        if (wave) {
+            lam_w = fmax(lam_w, (double)PH::wave_speed(ph, u));
+            vis_w = fmax(vis_w, (double)PH::visc_at(ph, u));
+            if (e == S.nelem - 1 && k == NB - 1) wave[1] = (unsigned long long)__double_as_longlong((double)PH::wave_speed(ph, u));
+        }
+        float ga[D][D];
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int j = 0; j < D; ++j) ga[a][j] = (float)(A[a][j] * fac);
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int v = 0; v < M; ++v)
+                s0[a * ARR + (l * M + v) * EVS + k] = F[v * D + 0] * ga[a][0] + F[v * D + 1] * ga[a][1] +
+                                                     F[v * D + 2] * ga[a][2];
+    }
+    if (wave) {
+        for (int off = 32; off > 0; off >>= 1) {
+            lam_w = fmax(lam_w, __shfl_xor(lam_w, off));
+            vis_w = fmax(vis_w, __shfl_xor(vis_w, off));
+        }
+        if (lane == 0) {
+            red[2 * wv] = lam_w;
+            red[2 * wv + 1] = vis_w;
+        }
+    }
+    __syncthreads();
+    if (wave && tid == 0) {
+        double a = 0.0, b = 0.0;
+        for (int w = 0; w < BLOCK / 64; ++w) { a = fmax(a, red[2 * w]); b = fmax(b, red[2 * w + 1]); }
+        wave_part[2 * blockIdx.x] = a;
+        wave_part[2 * blockIdx.x + 1] = b;
+    }
+    // 2a: contract q2 of G0, G1 (values) and G2 (derivatives) -> s1[a]
+#pragma unroll 1
+    for (int g = wv; g < 3 * LINES / 64; g += BLOCK / 64) {
+        const int a = g / (LINES / 64), L = (g - a * (LINES / 64)) * 64 + lane;
+        const float4 xv = *reinterpret_cast<const float4*>(s0 + a * ARR + mfma_line_base(L, EVS));
+        const float x[4] = {xv.x, xv.y, xv.z, xv.w};
+        ice_floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = (a == 2) ? mfma_line(aT1, x, acc) : mfma_line(aT0, x, acc);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) s1[a * ARR + mfma_rot_slot(L, r, EVS)] = acc[r];
+    }
+    __syncthreads();
+    // 2b: contract q1: T0 = L0^T G0' -> s0[0]; T12 = L1^T G1' + L0^T G2' -> s0[1]
+#pragma unroll 1
+    for (int g = wv; g < 2 * LINES / 64; g += BLOCK / 64) {
+        const int o = g / (LINES / 64), L = (g - o * (LINES / 64)) * 64 + lane;
+        ice_floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+        const float4 xv = *reinterpret_cast<const float4*>(s1 + o * ARR + mfma_line_base(L, EVS));
+        const float x[4] = {xv.x, xv.y, xv.z, xv.w};
+        if (o == 0) {
+            acc = mfma_line(aT0, x, acc);
+        } else {
+            acc = mfma_line(aT1, x, acc);
+            const float4 yv = *reinterpret_cast<const float4*>(s1 + 2 * ARR + mfma_line_base(L, EVS));
+            const float y[4] = {yv.x, yv.y, yv.z, yv.w};
+            acc = mfma_line(aT0, y, acc);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) s0[o * ARR + mfma_rot_slot(L, r, EVS)] = acc[r];
+    }
+    __syncthreads();
+    // 2c: contract q0: R = L1^T T0' + L0^T T12' -> s1[0], natural node order
+#pragma unroll 1
+    for (int g = wv; g < LINES / 64; g += BLOCK / 64) {
+        const int L = g * 64 + lane;
+        const float4 xv = *reinterpret_cast<const float4*>(s0 + mfma_line_base(L, EVS));
+        const float4 yv = *reinterpret_cast<const float4*>(s0 + ARR + mfma_line_base(L, EVS));
+        const float x[4] = {xv.x, xv.y, xv.z, xv.w}, y[4] = {yv.x, yv.y, yv.z, yv.w};
+        ice_floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = mfma_line(aT1, x, acc);
+        acc = mfma_line(aT0, y, acc);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) s1[mfma_rot_slot(L, r, EVS)] = acc[r];
+    }
+    __syncthreads();
+    // per node: volume term, source load, lifting, FP64 store (element_kernel_3d)
+    for (int t = tid; t < EPB * NB; t += BLOCK) {
+        const int l = t / NB, k = t - l * NB;
+        const int e = e0 + l;
+        if (e >= S.nelem) continue;
+        const int k0 = k / NN, k1 = (k / N) % N, k2 = k % N;
+        double r[M];
+#pragma unroll
+        for (int v = 0; v < M; ++v) r[v] = (double)s1[(l * M + v) * EVS + k];
+        if (srcload) {
+            const double* sl = srcload + ((size_t)e * NB + k) * M;
+#pragma unroll
+            for (int v = 0; v < M; ++v) r[v] -= sl[v];
+        }
+        const int kd[D] = {k0, k1, k2};
+        unsigned mask = 0;
+#pragma unroll
+        for (int j = 0; j < 2 * D; ++j)
+            if (kd[j % D] == ((j < D) ? 0 : N - 1)) mask |= 1u << j;
+        while (mask) {
+            const int j = __builtin_ctz(mask);
+            mask &= mask - 1;
+            const int ef = sEF[l * 2 * D + j];
+            if (ef < 0) continue;
+            const int kn = j % D;
+            const int sa = (kn == 0) ? k1 : k0, sb = (kn == 2) ? k1 : k2;
+            const double en = sLd[((j < D) ? N : N + 1) * N + kd[kn]];
+            const double sgn = ((ef >> 3) & 1) ? -en : en;
+            double la[N], lb[N];
+#pragma unroll
+            for (int p = 0; p < N; ++p) {
+                la[p] = sLd[p * N + sa];
+                lb[p] = sLd[p * N + sb];
+            }
+            const float* Qf = sQ + l * LIFT + j * NFQ * M;
+#pragma unroll
+            for (int v = 0; v < M; ++v) {
+                double acc = 0.0;
+#pragma unroll
+                for (int pa = 0; pa < N; ++pa) {
+                    double tt = 0.0;
+#pragma unroll
+                    for (int pb = 0; pb < N; ++pb) tt += (double)Qf[(pa * N + pb) * M + v] * lb[pb];
+                    acc += tt * la[pa];
+                }
+                r[v] += sgn * acc;
+            }
+        }
+        double* Re = R + ((size_t)e * NB + k) * M;
+#pragma unroll
+        for (int v = 0; v < M; ++v) Re[v] = r[v];
+    }
+}
+
 // Fold the per-block maxima of one residual into the running maxima
 // (lambda_max of BurgersFlux never resets, burgers.hpp:89; visc_max, navier_stokes.hpp:964-966).
 __global__ void wave_reduce_kernel(const double* __restrict__ part, int nblk, unsigned long long* __restrict__ wave) {
@@ -1948,9 +2314,9 @@ int launch_residual(const DevSpace& S, const typename PH::Params& ph, const Disc
     }
     if (ev) (void)hipEventRecord(ev[1], st);
     if (nbb > 0)
-        flux_kernel<D, P, PH, true><<<nbi + nbb, 256, 0, st>>>(S, ph, d, T, nbi);
+        flux_kernel<D, P, PH, true><<<nbi + nbb, 256, 0, st>>>(S, ph, d, T, nbi, U);
     else if (nbi > 0)
-        flux_kernel<D, P, PH, false><<<nbi, 256, 0, st>>>(S, ph, d, T, nbi);
+        flux_kernel<D, P, PH, false><<<nbi, 256, 0, st>>>(S, ph, d, T, nbi, U);
     if (ev) (void)hipEventRecord(ev[2], st);
     using TL = ElemTile<D, P>;
     int nblk = (S.nelem + TL::EPB - 1) / TL::EPB;
@@ -1968,7 +2334,20 @@ int launch_residual(const DevSpace& S, const typename PH::Params& ph, const Disc
                     S, ph, d.emu, d.srcload, U, reinterpret_cast<const VT*>(d.Q), reinterpret_cast<const VT*>(d.Pic), R,
                     d.wave, d.wave_part);
         };
-        switch (std::is_same<VT, double>::value ? d.tile : d.tile_mixed) {
+        const int tile = std::is_same<VT, double>::value ? d.tile : d.tile_mixed;
+        if constexpr (D == 3 && P == 3 && std::is_same<VT, float>::value && !TraceRec<PH, D>::NEED_GRAD &&
+                      !PH::HAS_HOMOGENEITY) {
+            if (tile == TILE_MFMA) {
+                using TM = Elem3Mfma<PH::M>;
+                nblk = (S.nelem + TM::EPB - 1) / TM::EPB;
+                element_kernel_3d_mfma<PH><<<nblk, TM::BLOCK, 0, st>>>(
+                    S, ph, d.emu, d.srcload, U, reinterpret_cast<const float*>(d.Q), R, d.wave, d.wave_part);
+                if (d.wave) wave_reduce_kernel<<<1, 256, 0, st>>>(d.wave_part, nblk, d.wave);
+                if (ev) (void)hipEventRecord(ev[3], st);
+                return (int)hipGetLastError();
+            }
+        }
+        switch (tile) {
         case 1: go(std::integral_constant<int, tile_tpb(1)>{}); break;
         case 2: go(std::integral_constant<int, tile_tpb(2)>{}); break;
         default: go(std::integral_constant<int, (D == 3 ? ICE_ELEM3_TPB : 256)>{}); break;

@@ -253,14 +253,17 @@ int ice_disc_set_element_viscosity(ice_disc* disc, const double* mu);
 
 /* Element-kernel workgroup shape of this discretization's residual (its p,
  * number of variables and dimension are fixed): tile 0 = whole elements in
- * 256 threads (default), 1 = 128, 2 = 64.  Results are identical; only speed
- * changes.  ice_disc_set_tile sets the shape of both the FP64 and the
- * mixed-precision residual.  ice_disc_autotune times `reps` FP64 residuals at
- * u for each tile on the stream, keeps the fastest (*best) and writes the
- * per-residual milliseconds to ms[3] (either may be null); then, in 2D and
- * 3D, times the mixed-precision residual the same way and keeps its fastest
- * tile for that residual.  r is overwritten.  Synchronous.  No reference
- * counterpart. */
+ * 256 threads (default), 1 = 128, 2 = 64; results are identical, only speed
+ * changes.  Tile 3 selects, for the mixed-precision residual of 3D P3 Euler,
+ * the element kernel whose tensor contractions run on the matrix cores
+ * (v_mfma_f32_4x4x1f32; same FP32 arithmetic, its own summation order), and
+ * the default shape elsewhere.  ice_disc_set_tile sets the shape of both the
+ * FP64 and the mixed-precision residual.  ice_disc_autotune times `reps` FP64
+ * residuals at u for each tile 0-2 on the stream, keeps the fastest (*best)
+ * and writes the per-residual milliseconds to ms[3] (either may be null);
+ * then, in 2D and 3D, times the mixed-precision residual the same way (tile 3
+ * included where it applies) and keeps its fastest tile for that residual.
+ * r is overwritten.  Synchronous.  No reference counterpart. */
 int ice_disc_set_tile(ice_disc* disc, int tile);
 int ice_disc_autotune(ice_disc* disc, const double* u_dev, double* r_dev, int reps, int* best, double* ms,
                       void* stream);

@@ -8,6 +8,7 @@
 #include <memory>
 #include <string>
 
+#include "mdg.hpp"
 #include "parallel.hpp"
 #include "solvers.hpp"
 #include "../iceicle_amd/csrc/fields.h"
@@ -39,6 +40,10 @@ struct ProblemBase {
     // blocks in BSR slot order: per row e, slot 0 = (e,e), slot 1+j = neighbour across local face j
     virtual void jacobian(const double* u, double* res, int* cols, double* blocks) = 0;
     virtual int newton(double* u, double tau_abs, double tau_rel, int kmax, double* rhist, int nhist) = 0;
+    // MDG interface conservation on selected traces: the selected nodes
+    // (count in *nsel, ids in nodes when non-null), and the residual
+    virtual void mdg_nodeset(const int* traces, int ntr, int* nsel, int* nodes) const = 0;
+    virtual void mdg_residual(const int* traces, int ntr, const double* u, double* res) const = 0;
 };
 
 template <int D, class PF, class CF, class DF>
@@ -156,6 +161,15 @@ struct Problem final : ProblemBase {
     }
     void mass_apply(const double* r, double* out) const override {
         const_cast<Problem*>(this)->mi().apply_add(fs, neq, r, out);
+    }
+    void mdg_nodeset(const int* traces, int ntr, int* nsel, int* nodes) const override {
+        NodeSet<D> ns(fs, std::vector<int>(traces, traces + ntr));
+        *nsel = (int)ns.selected.size();
+        if (nodes) std::copy(ns.selected.begin(), ns.selected.end(), nodes);
+    }
+    void mdg_residual(const int* traces, int ntr, const double* u, double* res) const override {
+        NodeSet<D> ns(fs, std::vector<int>(traces, traces + ntr));
+        form_mdg_residual(fs, disc, ns, u, res);
     }
     int newton(double* u, double tau_abs, double tau_rel, int kmax, double* rhist, int nhist) override {
         return newton_dense(fs, disc, u, tau_abs, tau_rel, kmax, rhist, nhist);
@@ -305,6 +319,12 @@ int ice_oracle_qp_coords(void* h, double* xq) { ORACLE_TRY(static_cast<Handle*>(
 int ice_oracle_mass_apply(void* h, const double* r, double* out) { ORACLE_TRY(static_cast<Handle*>(h)->prob->mass_apply(r, out)) }
 int ice_oracle_jacobian(void* h, const double* u, double* res, int* cols, double* blocks) {
     ORACLE_TRY(static_cast<Handle*>(h)->prob->jacobian(u, res, cols, blocks))
+}
+int ice_oracle_mdg_nodeset(void* h, const int* traces, int ntr, int* nsel, int* nodes) {
+    ORACLE_TRY(static_cast<Handle*>(h)->prob->mdg_nodeset(traces, ntr, nsel, nodes))
+}
+int ice_oracle_mdg_residual(void* h, const int* traces, int ntr, const double* u, double* res) {
+    ORACLE_TRY(static_cast<Handle*>(h)->prob->mdg_residual(traces, ntr, u, res))
 }
 int ice_oracle_newton(void* h, double* u, double tau_abs, double tau_rel, int kmax, double* rhist, int nhist) {
     try {

@@ -39,6 +39,8 @@ def lib():
             ("ice_oracle_info", [P, ip]),
             ("ice_oracle_residual", [P, dp, dp]),
             ("ice_oracle_residual_parallel", [P, dp, dp, C.c_int]),
+            ("ice_oracle_mdg_nodeset", [P, ip, C.c_int, ip, ip]),
+            ("ice_oracle_mdg_residual", [P, ip, C.c_int, dp, dp]),
             ("ice_oracle_faces", [P, ip]),
             ("ice_oracle_mesh", [P, dp, ip]),
             ("ice_oracle_project", [P, C.POINTER(abi.Field), dp]),
@@ -114,6 +116,27 @@ class Oracle:
         r = np.empty(self.size)
         _check(lib().ice_oracle_residual(self.h, _dp(u), _dp(r)))
         return r
+
+    def mdg_nodeset(self, traces):
+        """Global node ids of the MDG degrees of freedom of the selected traces
+        (node_set_layout.hpp:46-83); trace t < n_interior is interior face t,
+        else boundary face t - n_interior."""
+        tr = np.ascontiguousarray(traces, dtype=np.int32)
+        n = C.c_int()
+        _check(lib().ice_oracle_mdg_nodeset(self.h, _ip(tr), tr.size, C.byref(n), None))
+        out = np.empty(n.value, dtype=np.int32)
+        _check(lib().ice_oracle_mdg_nodeset(self.h, _ip(tr), tr.size, C.byref(n), _ip(out)))
+        return out
+
+    def mdg_residual(self, traces, u):
+        """Interface-conservation residual [selected node][neq] (form_mdg_residual,
+        form_residual.hpp:270-327)."""
+        tr = np.ascontiguousarray(traces, dtype=np.int32)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        nsel = self.mdg_nodeset(tr).size
+        out = np.empty(nsel * self.neq)
+        _check(lib().ice_oracle_mdg_residual(self.h, _ip(tr), tr.size, _dp(u), _dp(out)))
+        return out.reshape(nsel, self.neq)
 
     def residual_parallel(self, u, nthreads=0):
         """The same residual, bit for bit, from the element/face-parallel loop
