@@ -521,11 +521,37 @@ def batch_leg(ninst=512, reps=20, nsteps=10):
                zip(singles, [r64[bt.index[k][0]][bt.groups[bt.index[k][0]].instance_slice(bt.index[k][1])]
                              for k in range(ninst)]))
     del singles
-    # RK3-SSP steps of the whole batch: eager launches, then one graph
+    # RK3-SSP steps of the whole batch: eager launches, then one graph (the
+    # order groups chained, or as concurrent branches)
     ms_eager = timed(bt.step, nsteps)
     graph = bt.graph(nsteps)
     ms_graph = timed(graph.launch, 3) / nsteps
+    del graph
+    graph = bt.graph(nsteps, concurrent=True)
+    ms_graph_conc = timed(graph.launch, 3) / nsteps
+    del graph
     finite = all(bool(torch.isfinite(g.u).all()) for g in bt.groups)
+    # FP32 states: residuals, steps, and the FP64 check
+    bt.reset()
+    bt.reset_f32()
+    ms32s = timed(lambda: bt.residual_f32(outs), reps)
+    rm = [o.clone() for o in outs]
+    bt.residual_f32(outs, check_fp64=True)
+    torch.cuda.synchronize()
+    err_check = max(float((m - c).abs().max() / c.abs().max()) for m, c in zip(rm, outs))
+    for _ in range(nsteps):
+        bt.step()
+        bt.step_f32()
+    torch.cuda.synchronize()
+    drift = max(float((g.u32.double() - g.u).abs().max() / g.u.abs().max()) for g in bt.groups)
+    ms_eager32 = timed(bt.step_f32, nsteps)
+    graph = bt.graph_f32(nsteps)
+    ms_graph32 = timed(graph.launch, 3) / nsteps
+    del graph
+    graph = bt.graph_f32(nsteps, concurrent=True)
+    ms_graph32c = timed(graph.launch, 3) / nsteps
+    del graph
+    finite32 = all(bool(torch.isfinite(g.u32).all()) for g in bt.groups)
     return {
         "instances": ninst, "unknowns": int(bt.size), "groups": {g.order: len(g.insts) for g in bt.groups},
         "setup_seconds": round(t_setup, 2),
@@ -535,8 +561,17 @@ def batch_leg(ninst=512, reps=20, nsteps=10):
                            "rel_err_vs_fp64": float(f"{err32:.3e}")},
         "residual_per_instance": {"ms": round(ms_single, 3), "GDOF_s": round(bt.size / (ms_single * 1e-3) / 1e9, 3),
                                   "bit_identical_to_batched": bool(same)},
-        "rk3_ssp_step_ms": {"eager": round(ms_eager, 4), "graph": round(ms_graph, 4), "graph_steps": nsteps},
+        "rk3_ssp_step_ms": {"eager": round(ms_eager, 4), "graph": round(ms_graph, 4),
+                            "graph_concurrent_orders": round(ms_graph_conc, 4), "graph_steps": nsteps},
         "state_finite": finite,
+        "fp32_state": {
+            "residual_mixed_ms": round(ms32s, 4), "GDOF_s": round(bt.size / (ms32s * 1e-3) / 1e9, 3),
+            "rel_err_vs_fp64_check": float(f"{err_check:.3e}"),
+            "rk3_ssp_step_ms": {"eager": round(ms_eager32, 4), "graph": round(ms_graph32, 4),
+                                "graph_concurrent_orders": round(ms_graph32c, 4)},
+            f"state_drift_vs_fp64_after_{nsteps}_steps": float(f"{drift:.3e}"),
+            "state_finite": finite32,
+        },
     }
 
 

@@ -142,8 +142,8 @@ class Batch:
         for g in self.groups:
             g.stepper.step(g.u)
 
-    def graph(self, nsteps):
-        return engine.StepGraph([g.stepper for g in self.groups], [g.u for g in self.groups], nsteps)
+    def graph(self, nsteps, concurrent=False):
+        return engine.StepGraph([g.stepper for g in self.groups], [g.u for g in self.groups], nsteps, concurrent)
 
     def reset(self):
         import torch
@@ -154,3 +154,29 @@ class Batch:
         gi, pos = self.index[i]
         g = self.groups[gi]
         return g.u[g.instance_slice(pos)]
+
+    # ---- FP32 states ----------------------------------------------------
+    # Each group also holds an FP32 copy of its state (u32), stepped by the
+    # same stepper with FP32 stage buffers; the FP64 residual of the widened
+    # FP32 state is the check against the FP64 discretization.
+
+    def reset_f32(self):
+        import torch
+        for g in self.groups:
+            g.u32 = torch.from_numpy(g.u0).to(device=g.u.device, dtype=torch.float32)
+
+    def step_f32(self):
+        for g in self.groups:
+            g.stepper.step(g.u32)
+
+    def graph_f32(self, nsteps, concurrent=False):
+        return engine.StepGraph([g.stepper for g in self.groups], [g.u32 for g in self.groups], nsteps, concurrent)
+
+    def residual_f32(self, outs, check_fp64=False):
+        for g, r in zip(self.groups, outs):
+            engine.form_residual(g.space, g.disc, g.u32, r, precision="fp64" if check_fp64 else "mixed")
+
+    def instance_state_f32(self, i):
+        gi, pos = self.index[i]
+        g = self.groups[gi]
+        return g.u32[g.instance_slice(pos)]

@@ -16,7 +16,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libiceicle_amd.so")
 
 SOURCES = ["mesh.cpp", "engine.cpp", "residual.hip", "stepper.hip", "jacobian.hip", "krylov.hip", "newton.cpp",
-           "mlp.hip", "noalloc.hip"]
+           "mlp.hip", "noalloc.hip", "mdg.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("ICE_OFFLOAD_ARCH", "gfx950")

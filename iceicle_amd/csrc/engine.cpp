@@ -8,6 +8,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/iceicle_amd.h"
@@ -147,12 +148,13 @@ struct ice_disc {
     unsigned long long* d_wave = nullptr;
     double* d_wave_part = nullptr;
     double* d_emu = nullptr;  // per-element viscosity (batched instances)
+    double* d_u64 = nullptr;  // FP64 copy of an FP32 state for the FP64 residual check, made on first use
     int wave_users = 0;       // CFL steppers bound to this discretization
     ice::DiscDev dd{};
     ~ice_disc() {
         for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
         for (void* p : {(void*)d_fields, (void*)d_Q, (void*)d_Pic, (void*)d_bval, (void*)d_srcload, (void*)d_wave,
-                        (void*)d_wave_part, (void*)d_T, (void*)d_emu})
+                        (void*)d_wave_part, (void*)d_T, (void*)d_emu, (void*)d_u64})
             if (p) (void)hipFree(p);
     }
 };
@@ -160,6 +162,11 @@ struct ice_disc {
 namespace ice {
 
 const DevSpace& space_dev(const ice_space* s) { return s->dev; }
+const HostMesh& space_mesh(const ice_space* s) { return s->mesh; }
+int space_batch(const ice_space* s) { return s->batch; }
+bool space_coupled(const ice_space* s) { return s->coupled; }
+const PhysParams& disc_phys(const ice_disc* d) { return d->ph; }
+int disc_flux(const ice_disc* d) { return d->prm.flux; }
 
 int disc_residual(ice_disc* d, const double* u, double* r, hipStream_t st) {
     const ice_space* s = d->space;
@@ -366,7 +373,6 @@ static int space_create(const ice_mesh_desc* mds, int nmesh, const ice_space_des
                     mm.faces.push_back(f);
                 }
                 mm.n_interior += mk.n_interior;
-                mm.n_pairs += mk.n_pairs;
                 noff += mk.nnode;
                 eoff += mk.nelem;
                 foff += (int)mk.faces.size();
@@ -1086,11 +1092,14 @@ struct ice_stepper {
     ice::StepControl ctl{};
     size_t n = 0;
     double* buf[5] = {};      // res, k1, k2, k3, ustage / uold
+    float* fbuf[4] = {};      // FP32 states: k1, k2, k3, ustage (allocated by the first FP32 step)
     ice::StepState* st = nullptr;
     double* dte = nullptr;    // per-element step sizes (batched instances), or null
     ~ice_stepper() {
         if (tracks_wave && disc && --disc->wave_users == 0) disc->dd.wave = nullptr;
         for (double* b : buf)
+            if (b) (void)hipFree(b);
+        for (float* b : fbuf)
             if (b) (void)hipFree(b);
         if (st) (void)hipFree(st);
         if (dte) (void)hipFree(dte);
@@ -1169,6 +1178,68 @@ int step_once(ice_stepper* S, double* u, hipStream_t st) {
     }
     rc |= ice::launch_step_end(S->st, st);
     return rc;
+}
+
+// FP32 state: the stage residuals are the mixed-precision residual read from
+// the FP32 coefficients (FP64 output), the stage increments M^-1 R are kept
+// in FP32 and every stage combination is formed in FP64 and rounded once.
+// Textbook SSP-RK3 and explicit Euler.
+int step_once(ice_stepper* S, float* u, hipStream_t st) {
+    ice_disc* d = S->disc;
+    const ice_space* sp = d->space;
+    const size_t n = S->n;
+    double* res = S->buf[0];
+    float *k1 = S->fbuf[0], *k2 = S->fbuf[1], *k3 = S->fbuf[2], *us = S->fbuf[3];
+    const double* dt = &S->st->dt;
+    const size_t per = (size_t)sp->nb * d->neq;
+    int rc = 0;
+    auto R = [&](const float* x) {
+        rc |= ice::residual_mixed_f32state(sp->ndim, sp->p, d->prm.flux, sp->dev, d->ph, d->dd, x, res, st);
+    };
+    auto Minv = [&](float* k) { rc |= ice::launch_minv_f32(sp->ndim, sp->p, d->neq, sp->dev, res, k, st); };
+    auto combine = [&](int nk, const float* const* kk, const double* c, float* out) {
+        rc |= ice::launch_rk_combine_f32(n, u, nk, kk, c, dt, S->dte, per, S->st, out, st);
+    };
+    const float* kk[3] = {k1, k2, k3};
+    const double c1[1] = {1.0}, c2[2] = {0.25, 0.25}, c3[3] = {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0};
+    rc |= ice::launch_step_begin(S->st, d->d_wave, S->ctl, st);
+    R(u);
+    Minv(k1);
+    if (S->desc.kind == ICE_STEP_RK3_SSP) {
+        combine(1, kk, c1, us);
+        R(us);
+        Minv(k2);
+        combine(2, kk, c2, us);
+        R(us);
+        Minv(k3);
+        combine(3, kk, c3, u);
+    } else {
+        combine(1, kk, c1, u);
+    }
+    rc |= ice::launch_step_end(S->st, st);
+    return rc;
+}
+
+// FP32-state steps: the stepper kinds and spaces they cover, and the FP32
+// buffers, allocated once.
+int prepare_f32(ice_stepper* S) {
+    const ice_space* sp = S->disc->space;
+    if (sp->ndim < 2) return fail(ICE_ERR_UNSUPPORTED, "FP32 states cover ndim = 2 and 3");
+    const bool ok = (S->desc.kind == ICE_STEP_RK3_SSP && !S->desc.reference_compat) ||
+                    S->desc.kind == ICE_STEP_EXPLICIT_EULER;
+    if (!ok) return fail(ICE_ERR_UNSUPPORTED, "FP32 states step with textbook RK3-SSP or explicit Euler");
+    if (!S->fbuf[0]) {
+        try {
+            for (float*& b : S->fbuf) ICE_HIP(hipMalloc(&b, std::max<size_t>(S->n, 1) * sizeof(float)));
+        } catch (const ice::Error& e) {
+            for (float*& b : S->fbuf) {
+                if (b) (void)hipFree(b);
+                b = nullptr;
+            }
+            return fail(e.code, e.msg);
+        }
+    }
+    return ICE_OK;
 }
 
 }  // namespace
@@ -1261,26 +1332,74 @@ struct ice_step_graph_impl {
     hipGraphExec_t exec = nullptr;
 };
 
-int ice_step_graph_create(ice_stepper* const* steppers, double* const* u, int n, int nsteps, ice_step_graph** out) {
+}  // extern "C"
+
+namespace {
+
+// Capture nsteps steps of every stepper into one graph.  With concurrent set
+// and every stepper on its own discretization and state, each stepper's
+// steps form a branch of their own (forked from and joined to the capture
+// stream by events), so the graph may run the branches side by side;
+// otherwise all steps are one chain in stepper order.
+template <class UT>
+int make_step_graph(ice_stepper* const* steppers, UT* const* u, int n, int nsteps, int concurrent,
+                    ice_step_graph** out) {
     if (!steppers || !u || !out || n < 1 || nsteps < 1) return fail(ICE_ERR_ARG, "bad argument");
     *out = nullptr;
     for (int k = 0; k < n; ++k) {
         if (!steppers[k] || !u[k]) return fail(ICE_ERR_ARG, "null stepper or state");
         if (steppers[k]->ctl.use_cfl) return fail(ICE_ERR_UNSUPPORTED, "graph capture needs fixed step sizes");
         if (steppers[k]->disc->dd.wave) return fail(ICE_ERR_UNSUPPORTED, "graph capture needs fixed step sizes");
+        if constexpr (std::is_same<UT, float>::value) {
+            const int rc = prepare_f32(steppers[k]);
+            if (rc != ICE_OK) return rc;
+        }
     }
-    hipStream_t cs = nullptr;
-    if (hipStreamCreate(&cs) != hipSuccess) return fail(ICE_ERR_HIP, "stream create failed");
+    bool branches = concurrent && n > 1;
+    for (int a = 0; a < n && branches; ++a)
+        for (int b = a + 1; b < n && branches; ++b)
+            if (steppers[a]->disc == steppers[b]->disc || u[a] == u[b] || steppers[a] == steppers[b])
+                branches = false;
+    std::vector<hipStream_t> streams(branches ? n + 1 : 1, nullptr);
+    std::vector<hipEvent_t> events(branches ? n + 1 : 0, nullptr);
+    auto release = [&]() {
+        for (hipEvent_t e : events)
+            if (e) (void)hipEventDestroy(e);
+        for (hipStream_t s : streams)
+            if (s) (void)hipStreamDestroy(s);
+    };
+    for (hipStream_t& s : streams)
+        if (hipStreamCreate(&s) != hipSuccess) {
+            release();
+            return fail(ICE_ERR_HIP, "stream create failed");
+        }
+    for (hipEvent_t& e : events)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            release();
+            return fail(ICE_ERR_HIP, "event create failed");
+        }
+    hipStream_t cs = streams[0];
     auto g = std::make_unique<ice_step_graph_impl>();
     int rc = 0;
     if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-        (void)hipStreamDestroy(cs);
+        release();
         return fail(ICE_ERR_HIP, "stream capture failed to start");
     }
-    for (int t = 0; t < nsteps; ++t)
-        for (int k = 0; k < n; ++k) rc |= step_once(steppers[k], u[k], cs);
+    if (branches) {
+        rc |= (int)hipEventRecord(events[0], cs);
+        for (int k = 0; k < n; ++k) {
+            hipStream_t bs = streams[k + 1];
+            rc |= (int)hipStreamWaitEvent(bs, events[0], 0);
+            for (int t = 0; t < nsteps; ++t) rc |= step_once(steppers[k], u[k], bs);
+            rc |= (int)hipEventRecord(events[k + 1], bs);
+            rc |= (int)hipStreamWaitEvent(cs, events[k + 1], 0);
+        }
+    } else {
+        for (int t = 0; t < nsteps; ++t)
+            for (int k = 0; k < n; ++k) rc |= step_once(steppers[k], u[k], cs);
+    }
     const hipError_t ec = hipStreamEndCapture(cs, &g->graph);
-    (void)hipStreamDestroy(cs);
+    release();
     if (rc != 0 || ec != hipSuccess || !g->graph) {
         if (g->graph) (void)hipGraphDestroy(g->graph);
         return fail(ICE_ERR_HIP, "step capture failed");
@@ -1291,6 +1410,22 @@ int ice_step_graph_create(ice_stepper* const* steppers, double* const* u, int n,
     }
     *out = reinterpret_cast<ice_step_graph*>(g.release());
     return ICE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ice_step_graph_create(ice_stepper* const* steppers, double* const* u, int n, int nsteps, ice_step_graph** out) {
+    return make_step_graph(steppers, u, n, nsteps, 0, out);
+}
+
+int ice_step_graph_create_ex(ice_stepper* const* steppers, void* const* u, int state_fp32, int n, int nsteps,
+                             int concurrent, ice_step_graph** out) {
+    if (!u) return fail(ICE_ERR_ARG, "bad argument");
+    if (state_fp32)
+        return make_step_graph(steppers, reinterpret_cast<float* const*>(u), n, nsteps, concurrent, out);
+    return make_step_graph(steppers, reinterpret_cast<double* const*>(u), n, nsteps, concurrent, out);
 }
 
 int ice_step_graph_launch(ice_step_graph* g, void* stream) {
@@ -1317,6 +1452,37 @@ int ice_stepper_destroy(ice_stepper* s) {
 int ice_stepper_step(ice_stepper* s, double* u, void* stream) {
     if (!s || !u) return fail(ICE_ERR_ARG, "null argument");
     if (step_once(s, u, (hipStream_t)stream) != 0) return fail(ICE_ERR_HIP, "step launch failed");
+    return ICE_OK;
+}
+
+int ice_stepper_step_f32(ice_stepper* s, float* u, void* stream) {
+    if (!s || !u) return fail(ICE_ERR_ARG, "null argument");
+    const int rc = prepare_f32(s);
+    if (rc != ICE_OK) return rc;
+    if (step_once(s, u, (hipStream_t)stream) != 0) return fail(ICE_ERR_HIP, "step launch failed");
+    return ICE_OK;
+}
+
+int ice_residual_f32state(ice_disc* d, const float* u, double* r, int check_fp64, void* stream) {
+    if (!d || !u || !r) return fail(ICE_ERR_ARG, "null argument");
+    const ice_space* s = d->space;
+    if (s->ndim < 2) return fail(ICE_ERR_UNSUPPORTED, "FP32 states cover ndim = 2 and 3");
+    hipStream_t st = (hipStream_t)stream;
+    int rc = 0;
+    if (check_fp64) {
+        const size_t n = (size_t)s->mesh.nelem * s->nb * d->neq;
+        if (!d->d_u64) {
+            if (hipMalloc(&d->d_u64, std::max<size_t>(n, 1) * sizeof(double)) != hipSuccess) {
+                d->d_u64 = nullptr;
+                return fail(ICE_ERR_HIP, "allocation of the FP64 copy of the state failed");
+            }
+        }
+        rc = ice::launch_widen(n, u, d->d_u64, st);
+        if (rc == 0) rc = ice::residual_fp64(s->ndim, s->p, d->prm.flux, s->dev, d->ph, d->dd, d->d_u64, r, st, nullptr);
+    } else {
+        rc = ice::residual_mixed_f32state(s->ndim, s->p, d->prm.flux, s->dev, d->ph, d->dd, u, r, st);
+    }
+    if (rc != 0) return fail(ICE_ERR_HIP, std::string("residual launch failed: ") + hipGetErrorString((hipError_t)rc));
     return ICE_OK;
 }
 

@@ -6,6 +6,12 @@
 
 namespace ice {
 const DevSpace& space_dev(const ice_space* s);
+struct HostMesh;
+const HostMesh& space_mesh(const ice_space* s);
+int space_batch(const ice_space* s);
+bool space_coupled(const ice_space* s);  // periodic faces coupled as interior pairs
+const PhysParams& disc_phys(const ice_disc* d);
+int disc_flux(const ice_disc* d);
 const ice_space* disc_space(const ice_disc* d);
 int disc_neq(const ice_disc* d);
 void set_error(const std::string& m);

@@ -94,12 +94,17 @@ int l2_error_terms(int ndim, int p, int neq, const DevSpace& S, const ice_field*
                    hipStream_t st);
 
 int launch_minv(int ndim, int p, int neq, const DevSpace& S, const double* r, double* k, hipStream_t st);
+int launch_minv_f32(int ndim, int p, int neq, const DevSpace& S, const double* r, float* k, hipStream_t st);
+int launch_rk_combine_f32(size_t n, const float* base, int nk, const float* const* k, const double* c,
+                          const double* dtp, const double* dte, size_t per, const StepState* stp, float* out,
+                          hipStream_t st);
 int launch_axpy_dt(size_t n, double coef, const double* dtp, const double* x, double* y, hipStream_t st);
 int launch_axpy_edt(size_t n, double coef, const double* dte, size_t per, const StepState* stp, const double* x,
                     double* y, hipStream_t st);
 int launch_rk_combine(size_t n, const double* base, int nk, const double* const* k, const double* c, const double* dtp,
                       const double* dte, size_t per, const StepState* stp, double* out, hipStream_t st);
 int launch_axpby(size_t n, double a, const double* x, double b, double* y, hipStream_t st);
+int launch_widen(size_t n, const float* x, double* y, hipStream_t st);  // y = (double)x
 int launch_step_begin(StepState* s, unsigned long long* wave, const StepControl& c, hipStream_t st);
 int launch_step_end(StepState* s, hipStream_t st);
 
@@ -109,6 +114,9 @@ int residual_fp64(int ndim, int p, int flux, const DevSpace& S, const PhysParams
 // ndim 2 only (-2 otherwise)
 int residual_mixed(int ndim, int p, int flux, const DevSpace& S, const PhysParams& ph, const DiscDev& dd,
                    const double* U, double* R, hipStream_t st, hipEvent_t* ev = nullptr);
+// The same with the coefficients read from an FP32 state (2D and 3D)
+int residual_mixed_f32state(int ndim, int p, int flux, const DevSpace& S, const PhysParams& ph, const DiscDev& dd,
+                            const float* U, double* R, hipStream_t st);
 
 // One-time tables of u-independent terms: boundary values at the boundary
 // face quadrature points and the source load vector sum_q s(x_q) phi_i w detJ.

@@ -42,9 +42,6 @@ struct HostMesh {
     std::vector<int> conn;            // [nelem][2^ndim], element-local tensor order
     std::vector<HostFace> faces;      // reference order: interior, then boundary
     int n_interior = 0;
-    // coupled periodic mode: pairs appended after the interior faces, and the
-    // periodic boundary faces dropped from the boundary range
-    int n_pairs = 0;
 };
 
 // Sign that flips the first trace axis so that face maps keep outward normals

@@ -90,14 +90,14 @@ __device__ __forceinline__ const float* tab_of<float>(const DevSpace& S) { retur
 // Staged coefficient idx of the tile starting at U[base] ([element][node][var]):
 // FP64 as is; FP32 as the difference from the element's first coefficient of
 // the same variable (dU = U_i - U_0, the mixed-precision representation)
-template <class VT, int NB, int M>
-__device__ __forceinline__ VT coef_stage(const double* __restrict__ U, size_t base, size_t total, int idx) {
+template <class VT, int NB, int M, class UT>
+__device__ __forceinline__ VT coef_stage(const UT* __restrict__ U, size_t base, size_t total, int idx) {
     if (base + idx >= total) return VT(0);
     if constexpr (std::is_same<VT, double>::value) {
-        return U[base + idx];
+        return (VT)U[base + idx];
     } else {
         const int l = idx / (NB * M), v = idx % M;
-        return (VT)(U[base + idx] - U[base + (size_t)l * NB * M + v]);
+        return (VT)((double)U[base + idx] - (double)U[base + (size_t)l * NB * M + v]);
     }
 }
 
@@ -357,9 +357,9 @@ __device__ __forceinline__ void element_point_trace(const DevSpace& S, const VT*
     }
 }
 
-template <int D, int P, class PH, int TPB>
+template <int D, int P, class PH, int TPB, class UT = double>
 __global__ __launch_bounds__(TPB, ICE_TRACE_MINW) void trace_kernel(DevSpace S, typename PH::Params ph,
-                                                                     const double* __restrict__ U,
+                                                                     const UT* __restrict__ U,
                                                                      typename PH::real* __restrict__ T) {
     using VT = typename PH::real;
     constexpr int N = P + 1;
@@ -386,7 +386,7 @@ __global__ __launch_bounds__(TPB, ICE_TRACE_MINW) void trace_kernel(DevSpace S, 
         const size_t base = (size_t)e0 * NB * M;
         const size_t total = (size_t)S.nelem * NB * M;
         for (int idx = threadIdx.x; idx < EPB * NB * M; idx += blockDim.x)
-            sU[idx] = coef_stage<VT, NB, M>(U, base, total, idx);
+            sU[idx] = coef_stage<VT, NB, M, UT>(U, base, total, idx);
         const VT* tv = tab_of<VT>(S) + (N + 2);
         for (int idx = threadIdx.x; idx < 3 * (N + 2) * N; idx += blockDim.x) sL[idx] = tv[idx];
     }
@@ -429,11 +429,11 @@ __global__ __launch_bounds__(TPB, ICE_TRACE_MINW) void trace_kernel(DevSpace S, 
 // Numerical flux (and DDGIC lifting vector) at point q of interior face f
 // from the two trace records of the point (left element tL, right tR), ph
 // the left element's flux parameters.
-template <int D, class PH>
+template <int D, class PH, class UT>
 __device__ __forceinline__ void interior_face_point(const DevSpace& S, const typename PH::Params& ph, const DiscDev& dd,
                                                     int f, int q, int NFQ, const typename PH::real* tL,
-                                                    const typename PH::real* tR, const double* U0L,
-                                                    const double* U0R) {
+                                                    const typename PH::real* tR, const UT* U0L,
+                                                    const UT* U0R) {
     using VT = typename PH::real;
     constexpr int M = PH::M;
     constexpr int NV = TraceRec<PH, D>::NV;
@@ -449,7 +449,7 @@ __device__ __forceinline__ void interior_face_point(const DevSpace& S, const typ
 #pragma unroll
     for (int v = 0; v < M; ++v) {
         if constexpr (DELTA) {
-            const double a = U0L[v] + (double)tL[v], b = U0R[v] + (double)tR[v];
+            const double a = (double)U0L[v] + (double)tL[v], b = (double)U0R[v] + (double)tR[v];
             uL[v] = (VT)a;
             uR[v] = (VT)b;
             jmp[v] = (VT)(b - a);
@@ -498,10 +498,10 @@ __device__ __forceinline__ void interior_face_point(const DevSpace& S, const typ
 
 // Numerical flux (and DDGIC lifting vector) at point q of boundary face b
 // from the interior element's trace record tL (not read on Neumann faces).
-template <int D, class PH>
+template <int D, class PH, class UT>
 __device__ __forceinline__ void boundary_face_point(const DevSpace& S, const typename PH::Params& ph, const DiscDev& dd,
                                                     int b, int q, int NFQ, int bctype, int bcflag,
-                                                    const typename PH::real* tL, const double* U0L) {
+                                                    const typename PH::real* tL, const UT* U0L) {
     using VT = typename PH::real;
     constexpr int M = PH::M;
     constexpr int NV = TraceRec<PH, D>::NV;
@@ -535,7 +535,7 @@ __device__ __forceinline__ void boundary_face_point(const DevSpace& S, const typ
 #pragma unroll
         for (int v = 0; v < M; ++v) {
             if constexpr (DELTA) {
-                const double a = U0L[v] + (double)tL[v];
+                const double a = (double)U0L[v] + (double)tL[v];
                 uL[v] = (VT)a;
                 jmpD[v] = (VT)(dd.bval[((size_t)b * NFQ + q) * M + v] - a);
             } else {
@@ -582,10 +582,10 @@ __device__ __forceinline__ void boundary_face_point(const DevSpace& S, const typ
 // (blocks [0, nbi)), then boundary points.  BDY = false compiles the boundary
 // branch out (meshes without boundary faces): the boundary-condition code
 // holds more registers than the interior flux (Euler: 75 against 50 VGPRs).
-template <int D, int P, class PH, bool BDY>
+template <int D, int P, class PH, bool BDY, class UT = double>
 __global__ __launch_bounds__(256, ICE_FLUX_MINW) void flux_kernel(DevSpace S, typename PH::Params ph0, DiscDev dd,
                                                                    const typename PH::real* __restrict__ T, int nbi,
-                                                                   const double* __restrict__ U) {
+                                                                   const UT* __restrict__ U) {
     using VT = typename PH::real;
     constexpr int N = P + 1;
     constexpr int NFQ = Pow<N, D - 1>::value;
@@ -602,7 +602,7 @@ __global__ __launch_bounds__(256, ICE_FLUX_MINW) void flux_kernel(DevSpace S, ty
         const int qr = S.fperm[(fnr * S.norient + orient) * NFQ + q];
         const VT* tL = T + (((size_t)eL * 2 * D + fnl) * NFQ + q) * NV * M;
         const VT* tR = T + (((size_t)eR * 2 * D + fnr) * NFQ + qr) * NV * M;
-        interior_face_point<D, PH>(S, local_params(ph0, dd.emu, eL), dd, f, q, NFQ, tL, tR, U + (size_t)eL * NBM,
+        interior_face_point<D, PH, UT>(S, local_params(ph0, dd.emu, eL), dd, f, q, NFQ, tL, tR, U + (size_t)eL * NBM,
                                    U + (size_t)eR * NBM);
     } else if constexpr (BDY) {
         const int tid = (blockIdx.x - nbi) * blockDim.x + threadIdx.x;
@@ -611,7 +611,7 @@ __global__ __launch_bounds__(256, ICE_FLUX_MINW) void flux_kernel(DevSpace S, ty
         const int4 rec = S.fbdy[b];
         const int eL = rec.x, info = rec.y;
         const VT* tL = T + (((size_t)eL * 2 * D + info_fnl(info)) * NFQ + q) * NV * M;
-        boundary_face_point<D, PH>(S, local_params(ph0, dd.emu, eL), dd, b, q, NFQ, info_bctype(info), rec.z, tL,
+        boundary_face_point<D, PH, UT>(S, local_params(ph0, dd.emu, eL), dd, b, q, NFQ, info_bctype(info), rec.z, tL,
                                    U + (size_t)eL * NBM);
     }
 }
@@ -956,11 +956,11 @@ __global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph0
 // and the trace_integral lifting), 2N instead of N^2 multiply-adds per
 // point and basis function.  The lifting skips faces on which the basis
 // function's node does not lie (its trace there is zero).
-template <int P, class PH, int TPB>
+template <int P, class PH, int TPB, class UT = double>
 __global__ __launch_bounds__((ElemTile<2, P, TPB>::BLOCK), ICE_ELEM_MINW) void element_kernel_2d(DevSpace S, typename PH::Params ph0,
                                                                         const double* __restrict__ emu,
                                                                         const double* __restrict__ srcload,
-                                                                        const double* __restrict__ U,
+                                                                        const UT* __restrict__ U,
                                                                         const typename PH::real* __restrict__ Q,
                                                                         const typename PH::real* __restrict__ Pic,
                                                                         double* __restrict__ R,
@@ -1020,12 +1020,12 @@ __global__ __launch_bounds__((ElemTile<2, P, TPB>::BLOCK), ICE_ELEM_MINW) void e
         const size_t total = (size_t)S.nelem * NB * M;
         for (int idx = threadIdx.x; idx < EPB * NB * M; idx += blockDim.x) {
             const int l = idx / (NB * M), r = idx - l * (NB * M), nd = r / M, v = r - nd * M;
-            sU[l * ESU + nd * MS + v] = coef_stage<VT, NB, M>(U, base, total, idx);
+            sU[l * ESU + nd * MS + v] = coef_stage<VT, NB, M, UT>(U, base, total, idx);
         }
         if (!LOWLDS)
             for (int idx = threadIdx.x; idx < EPB * M; idx += blockDim.x) {
                 const size_t g = base + (size_t)(idx / M) * NB * M + idx % M;
-                sU0[idx] = g < total ? U[g] : 0.0;
+                sU0[idx] = g < total ? (double)U[g] : 0.0;
             }
         if (!LOWLDS && srcload)
             for (int idx = threadIdx.x; idx < EPB * NB * M; idx += blockDim.x)
@@ -1352,11 +1352,11 @@ __global__ __launch_bounds__((ElemTile<2, P, TPB>::BLOCK), ICE_ELEM_MINW) void e
 template <int P, int TPB>
 using Elem3Tile = ElemTile<3, P, TPB>;
 
-template <int P, class PH, int TPB>
+template <int P, class PH, int TPB, class UT = double>
 __global__ __launch_bounds__((Elem3Tile<P, TPB>::BLOCK), ICE_ELEM3_MINW) void element_kernel_3d(DevSpace S, typename PH::Params ph0,
                                                                          const double* __restrict__ emu,
                                                                          const double* __restrict__ srcload,
-                                                                         const double* __restrict__ U,
+                                                                         const UT* __restrict__ U,
                                                                          const typename PH::real* __restrict__ Q,
                                                                          const typename PH::real* __restrict__ Pic,
                                                                          double* __restrict__ R,
@@ -1402,12 +1402,12 @@ __global__ __launch_bounds__((Elem3Tile<P, TPB>::BLOCK), ICE_ELEM3_MINW) void el
         for (int idx = threadIdx.x; idx < EPB * NB * M; idx += blockDim.x) {
             const int l = idx / (NB * M), rem = idx - l * (NB * M);
             const int node = rem / M, v = rem - node * M;
-            sbuf[l * EBUF + v * NBP + node] = coef_stage<VT, NB, M>(U, base, total, idx);
+            sbuf[l * EBUF + v * NBP + node] = coef_stage<VT, NB, M, UT>(U, base, total, idx);
         }
         if (DELTA)
             for (int idx = threadIdx.x; idx < EPB * M; idx += blockDim.x) {
                 const size_t g = base + (size_t)(idx / M) * NB * M + idx % M;
-                sU0[idx] = g < total ? U[g] : 0.0;
+                sU0[idx] = g < total ? (double)U[g] : 0.0;
             }
         const size_t xb = (size_t)e0 * NV * D, xt = (size_t)S.nelem * NV * D;
         for (int idx = threadIdx.x; idx < EPB * NV * D; idx += blockDim.x)
@@ -1834,10 +1834,10 @@ __device__ __forceinline__ int mfma_line_base(int L, int evs) { return (L >> 4) 
 // rotated output slot r of line L: [ev][r][x][y]
 __device__ __forceinline__ int mfma_rot_slot(int L, int r, int evs) { return (L >> 4) * evs + r * 16 + (L & 15); }
 
-template <class PH>
+template <class PH, class UT = double>
 __global__ __launch_bounds__(Elem3Mfma<PH::M>::BLOCK) void element_kernel_3d_mfma(
     DevSpace S, typename PH::Params ph0, const double* __restrict__ emu, const double* __restrict__ srcload,
-    const double* __restrict__ U, const float* __restrict__ Q, double* __restrict__ R,
+    const UT* __restrict__ U, const float* __restrict__ Q, double* __restrict__ R,
     unsigned long long* __restrict__ wave, double* __restrict__ wave_part) {
     constexpr int D = 3, N = 4, NN = 16, NB = 64, NFQ = 16;
     constexpr int M = PH::M;
@@ -1868,11 +1868,11 @@ __global__ __launch_bounds__(Elem3Mfma<PH::M>::BLOCK) void element_kernel_3d_mfm
         for (int idx = tid; idx < EPB * NB * M; idx += BLOCK) {
             const int l = idx / (NB * M), rem = idx - l * (NB * M);
             const int node = rem / M, v = rem - node * M;
-            s0[(l * M + v) * EVS + node] = coef_stage<float, NB, M>(U, base, total, idx);
+            s0[(l * M + v) * EVS + node] = coef_stage<float, NB, M, UT>(U, base, total, idx);
         }
         for (int idx = tid; idx < EPB * M; idx += BLOCK) {
             const size_t g = base + (size_t)(idx / M) * NB * M + idx % M;
-            sU0[idx] = g < total ? U[g] : 0.0;
+            sU0[idx] = g < total ? (double)U[g] : 0.0;
         }
         const size_t xb = (size_t)e0 * NV * D, xt = (size_t)S.nelem * NV * D;
         for (int idx = tid; idx < EPB * NV * D; idx += BLOCK) sX[idx] = (xb + idx < xt) ? S.X[xb + idx] : 0.0;
@@ -2290,8 +2290,8 @@ int setup_disc_tables(int ndim, int p, int neq, const DevSpace& S, const ice_fie
 // PH fixes the arithmetic of the trace, flux and 2D element kernels (double,
 // or float for the mixed-precision residual); geometry, the source load and
 // the residual itself stay FP64.
-template <int D, int P, class PH>
-int launch_residual(const DevSpace& S, const typename PH::Params& ph, const DiscDev& dd, const double* U, double* R,
+template <int D, int P, class PH, class UT>
+int launch_residual(const DevSpace& S, const typename PH::Params& ph, const DiscDev& dd, const UT* U, double* R,
                     hipStream_t st, hipEvent_t* ev) {
     using VT = typename PH::real;
     constexpr int N = P + 1;
@@ -2310,13 +2310,13 @@ int launch_residual(const DevSpace& S, const typename PH::Params& ph, const Disc
         constexpr int EPW = (TPE <= ICE_TRACE_TPB) ? ICE_TRACE_TPB / TPE : 1;
         constexpr int TTPB = ((EPW * TPE + 63) / 64) * 64;
         constexpr int EPB = TTPB / TPE;
-        trace_kernel<D, P, PH, TTPB><<<(S.nelem + EPB - 1) / EPB, TTPB, 0, st>>>(S, ph, U, T);
+        trace_kernel<D, P, PH, TTPB, UT><<<(S.nelem + EPB - 1) / EPB, TTPB, 0, st>>>(S, ph, U, T);
     }
     if (ev) (void)hipEventRecord(ev[1], st);
     if (nbb > 0)
-        flux_kernel<D, P, PH, true><<<nbi + nbb, 256, 0, st>>>(S, ph, d, T, nbi, U);
+        flux_kernel<D, P, PH, true, UT><<<nbi + nbb, 256, 0, st>>>(S, ph, d, T, nbi, U);
     else if (nbi > 0)
-        flux_kernel<D, P, PH, false><<<nbi, 256, 0, st>>>(S, ph, d, T, nbi, U);
+        flux_kernel<D, P, PH, false, UT><<<nbi, 256, 0, st>>>(S, ph, d, T, nbi, U);
     if (ev) (void)hipEventRecord(ev[2], st);
     using TL = ElemTile<D, P>;
     int nblk = (S.nelem + TL::EPB - 1) / TL::EPB;
@@ -2326,11 +2326,11 @@ int launch_residual(const DevSpace& S, const typename PH::Params& ph, const Disc
             using TT = ElemTile<D, P, T>;
             nblk = (S.nelem + TT::EPB - 1) / TT::EPB;
             if constexpr (D == 2)
-                element_kernel_2d<P, PH, T><<<nblk, TT::BLOCK, 0, st>>>(
+                element_kernel_2d<P, PH, T, UT><<<nblk, TT::BLOCK, 0, st>>>(
                     S, ph, d.emu, d.srcload, U, reinterpret_cast<const VT*>(d.Q), reinterpret_cast<const VT*>(d.Pic), R,
                     d.wave, d.wave_part);
             else
-                element_kernel_3d<P, PH, T><<<nblk, TT::BLOCK, 0, st>>>(
+                element_kernel_3d<P, PH, T, UT><<<nblk, TT::BLOCK, 0, st>>>(
                     S, ph, d.emu, d.srcload, U, reinterpret_cast<const VT*>(d.Q), reinterpret_cast<const VT*>(d.Pic), R,
                     d.wave, d.wave_part);
         };
@@ -2340,7 +2340,7 @@ int launch_residual(const DevSpace& S, const typename PH::Params& ph, const Disc
             if (tile == TILE_MFMA) {
                 using TM = Elem3Mfma<PH::M>;
                 nblk = (S.nelem + TM::EPB - 1) / TM::EPB;
-                element_kernel_3d_mfma<PH><<<nblk, TM::BLOCK, 0, st>>>(
+                element_kernel_3d_mfma<PH, UT><<<nblk, TM::BLOCK, 0, st>>>(
                     S, ph, d.emu, d.srcload, U, reinterpret_cast<const float*>(d.Q), R, d.wave, d.wave_part);
                 if (d.wave) wave_reduce_kernel<<<1, 256, 0, st>>>(d.wave_part, nblk, d.wave);
                 if (ev) (void)hipEventRecord(ev[3], st);
@@ -2354,6 +2354,7 @@ int launch_residual(const DevSpace& S, const typename PH::Params& ph, const Disc
         }
     } else {
         static_assert(std::is_same<VT, double>::value || D > 1, "the mixed-precision residual is 2D and 3D");
+        static_assert(std::is_same<UT, double>::value, "FP32 states are 2D and 3D");
         if constexpr (std::is_same<VT, double>::value)
             element_kernel<D, P, PH><<<nblk, 256, 0, st>>>(S, ph, d.emu, d.srcload, U, d.Q, d.Pic, R, d.wave,
                                                             d.wave_part);
@@ -2363,14 +2364,14 @@ int launch_residual(const DevSpace& S, const typename PH::Params& ph, const Disc
     return (int)hipGetLastError();
 }
 
-template <class PH, int D>
-int dispatch_order(int p, const DevSpace& S, const typename PH::Params& ph, const DiscDev& dd, const double* U,
+template <class PH, int D, class UT = double>
+int dispatch_order(int p, const DevSpace& S, const typename PH::Params& ph, const DiscDev& dd, const UT* U,
                    double* R, hipStream_t st, hipEvent_t* ev) {
     switch (p) {
-    case 1: return launch_residual<D, 1, PH>(S, ph, dd, U, R, st, ev);
-    case 2: return launch_residual<D, 2, PH>(S, ph, dd, U, R, st, ev);
-    case 3: return launch_residual<D, 3, PH>(S, ph, dd, U, R, st, ev);
-    case 4: return launch_residual<D, 4, PH>(S, ph, dd, U, R, st, ev);
+    case 1: return launch_residual<D, 1, PH, UT>(S, ph, dd, U, R, st, ev);
+    case 2: return launch_residual<D, 2, PH, UT>(S, ph, dd, U, R, st, ev);
+    case 3: return launch_residual<D, 3, PH, UT>(S, ph, dd, U, R, st, ev);
+    case 4: return launch_residual<D, 4, PH, UT>(S, ph, dd, U, R, st, ev);
     default: return -1;
     }
 }
@@ -2395,20 +2396,30 @@ int residual_fp64(int ndim, int p, int flux, const DevSpace& S, const PhysParams
 
 // Mixed precision: FP32 interpolation, fluxes and face data, FP64 geometry,
 // source load and residual (2D and 3D).
-template <int D>
-int residual_mixed_dim(int p, int flux, const DevSpace& S, const PhysParams& ph, const DiscDev& dd, const double* U,
+template <int D, class UT>
+int residual_mixed_dim(int p, int flux, const DevSpace& S, const PhysParams& ph, const DiscDev& dd, const UT* U,
                        double* R, hipStream_t st, hipEvent_t* ev) {
     const PhysT<float> pf = phys_cast<float>(ph);
-    if (flux == ICE_FLUX_BURGERS) return dispatch_order<Burgers<D, float>, D>(p, S, pf, dd, U, R, st, ev);
-    if (flux == ICE_FLUX_NAVIER_STOKES) return dispatch_order<NavierStokes<D, true, float>, D>(p, S, pf, dd, U, R, st, ev);
-    if (flux == ICE_FLUX_EULER) return dispatch_order<NavierStokes<D, false, float>, D>(p, S, pf, dd, U, R, st, ev);
+    if (flux == ICE_FLUX_BURGERS) return dispatch_order<Burgers<D, float>, D, UT>(p, S, pf, dd, U, R, st, ev);
+    if (flux == ICE_FLUX_NAVIER_STOKES)
+        return dispatch_order<NavierStokes<D, true, float>, D, UT>(p, S, pf, dd, U, R, st, ev);
+    if (flux == ICE_FLUX_EULER) return dispatch_order<NavierStokes<D, false, float>, D, UT>(p, S, pf, dd, U, R, st, ev);
     return -1;
 }
 
 int residual_mixed(int ndim, int p, int flux, const DevSpace& S, const PhysParams& ph, const DiscDev& dd,
                    const double* U, double* R, hipStream_t st, hipEvent_t* ev) {
-    if (ndim == 2) return residual_mixed_dim<2>(p, flux, S, ph, dd, U, R, st, ev);
-    if (ndim == 3) return residual_mixed_dim<3>(p, flux, S, ph, dd, U, R, st, ev);
+    if (ndim == 2) return residual_mixed_dim<2, double>(p, flux, S, ph, dd, U, R, st, ev);
+    if (ndim == 3) return residual_mixed_dim<3, double>(p, flux, S, ph, dd, U, R, st, ev);
+    return -2;
+}
+
+// The mixed residual of an FP32 state (the same kernels reading FP32
+// coefficients; differences and element values still formed in FP64)
+int residual_mixed_f32state(int ndim, int p, int flux, const DevSpace& S, const PhysParams& ph, const DiscDev& dd,
+                            const float* U, double* R, hipStream_t st) {
+    if (ndim == 2) return residual_mixed_dim<2, float>(p, flux, S, ph, dd, U, R, st, nullptr);
+    if (ndim == 3) return residual_mixed_dim<3, float>(p, flux, S, ph, dd, U, R, st, nullptr);
     return -2;
 }
 

@@ -59,8 +59,8 @@ __device__ __forceinline__ double q1_det(const double* __restrict__ Xe, const do
 #ifndef ICE_MINV_TPB
 #define ICE_MINV_TPB 128
 #endif
-template <int D, int P, int M, int TPB>
-__global__ __launch_bounds__(TPB) void minv_kernel(DevSpace S, const double* __restrict__ r, double* __restrict__ k) {
+template <int D, int P, int M, int TPB, class KT = double>
+__global__ __launch_bounds__(TPB) void minv_kernel(DevSpace S, const double* __restrict__ r, KT* __restrict__ k) {
     constexpr int N = P + 1;
     constexpr int NB = PowS<N, D>::value;
     static_assert(TPB >= NB, "one thread per node of an element");
@@ -158,10 +158,10 @@ __global__ __launch_bounds__(TPB) void minv_kernel(DevSpace S, const double* __r
         const size_t o = ((size_t)e * NB + kk) * M;
         if (sing[le]) {
 #pragma unroll
-            for (int v = 0; v < M; ++v) k[o + v] = 1e-8 * r[o + v];
+            for (int v = 0; v < M; ++v) k[o + v] = (KT)(1e-8 * r[o + v]);
         } else {
 #pragma unroll
-            for (int v = 0; v < M; ++v) k[o + v] = cur[(le * NB + kk) * M + v];
+            for (int v = 0; v < M; ++v) k[o + v] = (KT)cur[(le * NB + kk) * M + v];
         }
     }
 }
@@ -188,18 +188,20 @@ __global__ void axpy_edt_kernel(size_t n, double coef, const double* __restrict_
 // combination (nk = 1..3 stage vectors), the step size read on the device
 // (the stepper's, or each element's; zero once the run is done).  out may
 // alias base.
-__global__ void rk_combine_kernel(size_t n, const double* __restrict__ base, int nk, const double* __restrict__ k0,
-                                  const double* __restrict__ k1, const double* __restrict__ k2, double c0, double c1,
+// (T = float for FP32 states: the combination is formed in FP64 and rounded once)
+template <class T>
+__global__ void rk_combine_kernel(size_t n, const T* __restrict__ base, int nk, const T* __restrict__ k0,
+                                  const T* __restrict__ k1, const T* __restrict__ k2, double c0, double c1,
                                   double c2, const double* __restrict__ dtp, const double* __restrict__ dte,
-                                  size_t per, const StepState* __restrict__ stp, double* out) {
+                                  size_t per, const StepState* __restrict__ stp, T* out) {
     const bool done = stp && stp->done;
     const double dts = dtp ? *dtp : 1.0;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const double dt = done ? 0.0 : (dte ? dte[i / per] : dts);
-        double s = c0 * k0[i];
-        if (nk > 1) s += c1 * k1[i];
-        if (nk > 2) s += c2 * k2[i];
-        out[i] = base[i] + dt * s;
+        double s = c0 * (double)k0[i];
+        if (nk > 1) s += c1 * (double)k1[i];
+        if (nk > 2) s += c2 * (double)k2[i];
+        out[i] = (T)((double)base[i] + dt * s);
     }
 }
 
@@ -244,19 +246,19 @@ __global__ void step_end_kernel(StepState* st) {
     }
 }
 
-template <int D, int P, int M>
-int launch_minv_t(const DevSpace& S, const double* r, double* k, hipStream_t st) {
+template <int D, int P, int M, class KT>
+int launch_minv_t(const DevSpace& S, const double* r, KT* k, hipStream_t st) {
     constexpr int NB = PowS<P + 1, D>::value;
     // workgroup: ICE_MINV_TPB threads, or whole waves for one element when
     // an element has more nodes
     constexpr int TPB = (NB <= ICE_MINV_TPB) ? ICE_MINV_TPB : ((NB + 63) / 64) * 64;
     constexpr int EPB = TPB / NB;
-    minv_kernel<D, P, M, TPB><<<(S.nelem + EPB - 1) / EPB, TPB, 0, st>>>(S, r, k);
+    minv_kernel<D, P, M, TPB, KT><<<(S.nelem + EPB - 1) / EPB, TPB, 0, st>>>(S, r, k);
     return (int)hipGetLastError();
 }
 
-template <int D, int M>
-int minv_order(int p, const DevSpace& S, const double* r, double* k, hipStream_t st) {
+template <int D, int M, class KT>
+int minv_order(int p, const DevSpace& S, const double* r, KT* k, hipStream_t st) {
     switch (p) {
     case 1: return launch_minv_t<D, 1, M>(S, r, k, st);
     case 2: return launch_minv_t<D, 2, M>(S, r, k, st);
@@ -266,7 +268,8 @@ int minv_order(int p, const DevSpace& S, const double* r, double* k, hipStream_t
     }
 }
 
-int launch_minv(int ndim, int p, int neq, const DevSpace& S, const double* r, double* k, hipStream_t st) {
+template <class KT>
+int minv_dispatch(int ndim, int p, int neq, const DevSpace& S, const double* r, KT* k, hipStream_t st) {
     if (ndim == 1 && neq == 1) return minv_order<1, 1>(p, S, r, k, st);
     if (ndim == 1 && neq == 3) return minv_order<1, 3>(p, S, r, k, st);
     if (ndim == 2 && neq == 1) return minv_order<2, 1>(p, S, r, k, st);
@@ -274,6 +277,15 @@ int launch_minv(int ndim, int p, int neq, const DevSpace& S, const double* r, do
     if (ndim == 3 && neq == 1) return minv_order<3, 1>(p, S, r, k, st);
     if (ndim == 3 && neq == 5) return minv_order<3, 5>(p, S, r, k, st);
     return -1;
+}
+
+int launch_minv(int ndim, int p, int neq, const DevSpace& S, const double* r, double* k, hipStream_t st) {
+    return minv_dispatch(ndim, p, neq, S, r, k, st);
+}
+
+int launch_minv_f32(int ndim, int p, int neq, const DevSpace& S, const double* r, float* k, hipStream_t st) {
+    if (ndim == 1) return -1;  // FP32 states are 2D and 3D
+    return minv_dispatch(ndim, p, neq, S, r, k, st);
 }
 
 static unsigned grid_for(size_t n) {
@@ -294,9 +306,28 @@ int launch_axpy_edt(size_t n, double coef, const double* dte, size_t per, const 
 
 int launch_rk_combine(size_t n, const double* base, int nk, const double* const* k, const double* c, const double* dtp,
                       const double* dte, size_t per, const StepState* stp, double* out, hipStream_t st) {
-    rk_combine_kernel<<<grid_for(n), 256, 0, st>>>(n, base, nk, k[0], nk > 1 ? k[1] : nullptr, nk > 2 ? k[2] : nullptr,
-                                                    c[0], nk > 1 ? c[1] : 0.0, nk > 2 ? c[2] : 0.0, dtp, dte, per, stp,
-                                                    out);
+    rk_combine_kernel<double><<<grid_for(n), 256, 0, st>>>(n, base, nk, k[0], nk > 1 ? k[1] : nullptr,
+                                                            nk > 2 ? k[2] : nullptr, c[0], nk > 1 ? c[1] : 0.0,
+                                                            nk > 2 ? c[2] : 0.0, dtp, dte, per, stp, out);
+    return (int)hipGetLastError();
+}
+
+int launch_rk_combine_f32(size_t n, const float* base, int nk, const float* const* k, const double* c,
+                          const double* dtp, const double* dte, size_t per, const StepState* stp, float* out,
+                          hipStream_t st) {
+    rk_combine_kernel<float><<<grid_for(n), 256, 0, st>>>(n, base, nk, k[0], nk > 1 ? k[1] : nullptr,
+                                                           nk > 2 ? k[2] : nullptr, c[0], nk > 1 ? c[1] : 0.0,
+                                                           nk > 2 ? c[2] : 0.0, dtp, dte, per, stp, out);
+    return (int)hipGetLastError();
+}
+
+__global__ void widen_kernel(size_t n, const float* __restrict__ x, double* __restrict__ y) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        y[i] = (double)x[i];
+}
+
+int launch_widen(size_t n, const float* x, double* y, hipStream_t st) {
+    widen_kernel<<<grid_for(n), 256, 0, st>>>(n, x, y);
     return (int)hipGetLastError();
 }
 

@@ -39,6 +39,10 @@ _SIGS = {
     "ice_residual_mixed": ([P, P, P, P], C.c_int),
     "ice_residual_noalloc": ([P, P, P, P], C.c_int),
     "ice_noalloc_scratch_bytes": ([P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
+    "ice_mdg_select": ([P, C.POINTER(C.c_int), C.c_int, C.POINTER(P)], C.c_int),
+    "ice_mdg_query": ([P, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+    "ice_mdg_residual": ([P, P, P, P, P], C.c_int),
+    "ice_mdg_destroy": ([P], C.c_int),
     "ice_disc_update": ([P, C.POINTER(abi.DiscParams)], C.c_int),
     "ice_space_create_batch": ([C.POINTER(abi.MeshDesc), C.c_int, C.POINTER(abi.SpaceDesc), C.POINTER(P)], C.c_int),
     "ice_disc_set_element_viscosity": ([P, P], C.c_int),
@@ -46,6 +50,7 @@ _SIGS = {
     "ice_disc_autotune": ([P, P, P, C.c_int, C.POINTER(C.c_int), P, P], C.c_int),
     "ice_stepper_set_element_dt": ([P, P], C.c_int),
     "ice_step_graph_create": ([P, P, C.c_int, C.c_int, C.POINTER(P)], C.c_int),
+    "ice_step_graph_create_ex": ([P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(P)], C.c_int),
     "ice_step_graph_launch": ([P, P], C.c_int),
     "ice_step_graph_destroy": ([P], C.c_int),
     "ice_profile_begin": ([P, C.c_int], C.c_int),
@@ -60,6 +65,8 @@ _SIGS = {
     "ice_stepper_create": ([P, C.POINTER(abi.StepperDesc), C.POINTER(P)], C.c_int),
     "ice_stepper_destroy": ([P], C.c_int),
     "ice_stepper_step": ([P, P, P], C.c_int),
+    "ice_stepper_step_f32": ([P, P, P], C.c_int),
+    "ice_residual_f32state": ([P, P, P, C.c_int, P], C.c_int),
     "ice_stepper_solve": ([P, P, P, C.POINTER(C.c_longlong), C.POINTER(C.c_double)], C.c_int),
     "ice_stepper_status": ([P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                             C.POINTER(C.c_int)], C.c_int),
@@ -138,10 +145,11 @@ def trace_points(ndim, order, face_nr, orientation):
     return xi
 
 
-def _require_cuda(t, name):
+def _require_cuda(t, name, dtype=None):
     import torch
-    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
-        raise IceError(f"{name} must be a contiguous float64 CUDA tensor")
+    dtype = torch.float64 if dtype is None else dtype
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == dtype and t.is_contiguous()):
+        raise IceError(f"{name} must be a contiguous {str(dtype).replace('torch.', '')} CUDA tensor")
 
 
 class FESpace:
@@ -267,20 +275,64 @@ def current_stream():
 
 def form_residual(space, disc, u, res, stream=None, precision="fp64"):
     """res <- R(u) on the GPU; precision "fp64" (the reference's arithmetic)
-    or "mixed" (FP32 interpolation and fluxes, FP64 geometry and output, 2D and 3D)."""
-    _require_cuda(u, "u")
+    or "mixed" (FP32 interpolation and fluxes, FP64 geometry and output, 2D and 3D).
+    u may be an FP32 state (2D and 3D): "mixed" is then the residual FP32
+    steps use, "fp64" the FP64 residual of the widened state (the check of an
+    FP32 solution against the FP64 discretization).  res is float64."""
+    import torch
     _require_cuda(res, "res")
+    if isinstance(u, torch.Tensor) and u.dtype == torch.float32:
+        _require_cuda(u, "u", torch.float32)
+    else:
+        _require_cuda(u, "u")
     if u.numel() != disc.size or res.numel() != disc.size:
         raise IceError(f"u/res must have {disc.size} entries")
-    st = current_stream() if stream is None else stream
-    fn = {"fp64": lib().ice_residual, "mixed": lib().ice_residual_mixed}.get(precision)
-    if fn is None:
+    if precision not in ("fp64", "mixed"):
         raise IceError(f"unknown precision {precision!r}")
+    st = current_stream() if stream is None else stream
+    if u.dtype == torch.float32:
+        check(lib().ice_residual_f32state(disc.h, _ptr(u), _ptr(res), 1 if precision == "fp64" else 0, st))
+        return
+    fn = lib().ice_residual if precision == "fp64" else lib().ice_residual_mixed
     check(fn(disc.h, C.c_void_p(u.data_ptr()), C.c_void_p(res.data_ptr()), st))
 
 
 def _ptr(t):
     return C.c_void_p(t.data_ptr())
+
+
+class MDGSelection:
+    """Selected traces of the MDG interface-conservation residual
+    (nodeset_dof_map, node_set_layout.hpp:46-83): trace t < number of
+    interior faces is interior face t, else boundary face t - that number."""
+
+    def __init__(self, space, traces):
+        self.space = space
+        tr = np.ascontiguousarray(traces, dtype=np.int32)
+        h = P()
+        check(lib().ice_mdg_select(space.h, tr.ctypes.data_as(C.POINTER(C.c_int)), tr.size, C.byref(h)))
+        self.h = h
+        n = C.c_int()
+        check(lib().ice_mdg_query(self.h, C.byref(n), None))
+        self.nodes = np.empty(n.value, dtype=np.int32)
+        check(lib().ice_mdg_query(self.h, C.byref(n), self.nodes.ctypes.data_as(C.POINTER(C.c_int))))
+
+    def residual(self, disc, u, res=None, stream=None):
+        """form_mdg_residual: res[node][neq] on the device."""
+        import torch
+        _require_cuda(u, "u")
+        if res is None:
+            res = torch.empty((self.nodes.size, disc.neq), dtype=torch.float64, device=u.device)
+        check(lib().ice_mdg_residual(disc.h, self.h, _ptr(u), _ptr(res), stream or current_stream()))
+        return res
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().ice_mdg_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
 
 
 def mass_inverse_apply(disc, r, out, stream=None):
@@ -314,6 +366,13 @@ class ExplicitStepper:
         self.h = h
 
     def step(self, u, stream=None):
+        """One step of u: a float64 state, or a float32 one (2D and 3D,
+        textbook RK3-SSP or explicit Euler; ice_stepper_step_f32)."""
+        import torch
+        if isinstance(u, torch.Tensor) and u.dtype == torch.float32:
+            _require_cuda(u, "u", torch.float32)
+            check(lib().ice_stepper_step_f32(self.h, _ptr(u), stream or current_stream()))
+            return
         _require_cuda(u, "u")
         check(lib().ice_stepper_step(self.h, _ptr(u), stream or current_stream()))
 
@@ -514,16 +573,22 @@ class StepGraph:
     """nsteps steps of several steppers (each on its own device state)
     captured into one HIP graph; launch() replays them all."""
 
-    def __init__(self, steppers, states, nsteps):
+    def __init__(self, steppers, states, nsteps, concurrent=False):
+        """states: all float64, or all float32 (FP32-state steps).  With
+        concurrent, steppers on distinct discretizations and states become
+        independent branches of the graph (ice_step_graph_create_ex)."""
+        import torch
         if len(steppers) != len(states):
             raise IceError("one state per stepper")
+        fp32 = bool(states) and states[0].dtype == torch.float32
         for u in states:
-            _require_cuda(u, "u")
+            _require_cuda(u, "u", torch.float32 if fp32 else torch.float64)
         self._keep = (list(steppers), list(states))
         sa = (P * len(steppers))(*[s.h for s in steppers])
         ua = (P * len(states))(*[u.data_ptr() for u in states])
         h = P()
-        check(lib().ice_step_graph_create(sa, ua, len(steppers), nsteps, C.byref(h)))
+        check(lib().ice_step_graph_create_ex(sa, ua, int(fp32), len(steppers), nsteps, int(bool(concurrent)),
+                                             C.byref(h)))
         self.h = h
         self.nsteps = nsteps
 

@@ -468,11 +468,31 @@ int ice_stepper_set_element_dt(ice_stepper* stepper, const double* dt);
 typedef struct ice_step_graph ice_step_graph;
 int ice_step_graph_create(ice_stepper* const* steppers, double* const* u_dev, int n, int nsteps,
                           ice_step_graph** out);
+/* The same for FP64 (state_fp32 = 0, u_dev[k] double*) or FP32 states
+ * (state_fp32 = 1, u_dev[k] float*, stepped as ice_stepper_step_f32 does).
+ * With concurrent = 1 and every stepper on its own discretization and state,
+ * each stepper's steps are an independent branch of the graph, so steppers of
+ * different sizes (e.g. one per polynomial order of a batch) can share the
+ * device; otherwise the steps form one chain.  No reference counterpart. */
+int ice_step_graph_create_ex(ice_stepper* const* steppers, void* const* u_dev, int state_fp32, int n, int nsteps,
+                             int concurrent, ice_step_graph** out);
 int ice_step_graph_launch(ice_step_graph* graph, void* stream);
 int ice_step_graph_destroy(ice_step_graph* graph);
 /* One step of u (device, fe_layout_right) on `stream`; time step, time and
  * step count live on the device, so consecutive steps need no host sync. */
 int ice_stepper_step(ice_stepper* stepper, double* u_dev, void* stream);
+/* One step of an FP32 state u (device, fe_layout_right, 2D and 3D; textbook
+ * RK3-SSP or explicit Euler): stage residuals are the mixed-precision
+ * residual of the FP32 coefficients (FP64 output), increments M^-1 R are kept
+ * in FP32, each stage combination is formed in FP64 and rounded once to FP32.
+ * The first call allocates the stepper's FP32 stage buffers.  No reference
+ * counterpart (the reference's states are FP64). */
+int ice_stepper_step_f32(ice_stepper* stepper, float* u_dev, void* stream);
+/* Residual of an FP32 state: check_fp64 = 0 gives the mixed-precision residual
+ * the FP32 steps use; check_fp64 = 1 widens the state to FP64 (a device copy
+ * the discretization allocates on first use) and forms the FP64 residual, the
+ * check of an FP32 solution against the FP64 discretization.  r is FP64. */
+int ice_residual_f32state(ice_disc* disc, const float* u_dev, double* r_dev, int check_fp64, void* stream);
 /* solve(): one residual for the wave speeds, then steps until termination. */
 int ice_stepper_solve(ice_stepper* stepper, double* u_dev, void* stream, long long* nsteps, double* time);
 /* Synchronous read of the device-side time, last step size and step count. */
@@ -520,6 +540,22 @@ typedef struct {
     double beta1, beta2, eps;
     unsigned long long seed;  /* minibatch shuffling */
 } ice_mlp_train_opts;
+
+/* ---- MDG interface conservation (moving discontinuous Galerkin) ----------
+ * form_mdg_residual over a nodeset_dof_map (form_residual.hpp:270-327,
+ * node_set_layout.hpp:46-83) with ConservationLawDDG::interface_conservation
+ * (conservation_law.hpp:1178-1288).  A selection is a list of trace indices
+ * in the reference's FESpace::traces order (interior faces, then boundary
+ * faces, mesh face order); its degrees of freedom are the nodes of those
+ * traces minus every node of a boundary face, in ascending node order.
+ * Single meshes without coupled periodic faces.  ice_mdg_residual writes
+ * res[node][neq] (FP64, device) for the current u and mesh coordinates. */
+typedef struct ice_mdg_sel ice_mdg_sel;
+int ice_mdg_select(const ice_space* space, const int* traces, int ntraces, ice_mdg_sel** out);
+/* number of selected nodes, and their global node ids when nodes != NULL */
+int ice_mdg_query(const ice_mdg_sel* sel, int* nnodes, int* nodes);
+int ice_mdg_residual(ice_disc* disc, const ice_mdg_sel* sel, const double* u_dev, double* res_dev, void* stream);
+int ice_mdg_destroy(ice_mdg_sel* sel);
 
 /* Glorot-uniform weights (seeded), zero biases.  widths has nlayers+1 entries. */
 int ice_mlp_create(int nlayers, const int* widths, unsigned long long seed, ice_mlp** out);
