@@ -149,6 +149,8 @@ struct ice_disc {
     double* d_wave_part = nullptr;
     double* d_emu = nullptr;  // per-element viscosity (batched instances)
     double* d_u64 = nullptr;  // FP64 copy of an FP32 state for the FP64 residual check, made on first use
+    bool bval_values = false; // boundary data set from point values (not recomputed from the fields)
+    bool src_values = false;  // source load set from point values (likewise)
     int wave_users = 0;       // CFL steppers bound to this discretization
     ice::DiscDev dd{};
     ~ice_disc() {
@@ -171,6 +173,15 @@ int disc_flux(const ice_disc* d) { return d->prm.flux; }
 int disc_residual(ice_disc* d, const double* u, double* r, hipStream_t st) {
     const ice_space* s = d->space;
     return residual_fp64(s->ndim, s->p, d->prm.flux, s->dev, d->ph, d->dd, u, r, st, nullptr);
+}
+
+int disc_refresh_geometry(ice_disc* d, hipStream_t st) {
+    const ice_space* s = d->space;
+    int rc = face_geometry_table(s->ndim, s->p, s->dev, const_cast<double*>(s->dev.fgeo), st);
+    double* bv = d->bval_values ? nullptr : d->d_bval;
+    double* sl = d->src_values ? nullptr : d->d_srcload;
+    if (rc == 0 && (bv || sl)) rc = setup_disc_tables(s->ndim, s->p, d->neq, s->dev, d->d_fields, bv, sl, st);
+    return rc;
 }
 
 // FP64 residual whose per-call buffers (element face traces, face fluxes,
@@ -792,7 +803,8 @@ int ice_disc_update(ice_disc* d, const ice_disc_params* p) {
         }
         fields[2 * ICE_MAX_BCFLAG] = p->source;
         ICE_HIP(hipMemcpy(d->d_fields, fields.data(), fields.size() * sizeof(ice_field), hipMemcpyHostToDevice));
-        if (ice::setup_disc_tables(s->ndim, s->p, d->neq, s->dev, d->d_fields, d->d_bval, d->d_srcload, nullptr) != 0)
+        if (ice::setup_disc_tables(s->ndim, s->p, d->neq, s->dev, d->d_fields, d->bval_values ? nullptr : d->d_bval,
+                                   d->src_values ? nullptr : d->d_srcload, nullptr) != 0)
             throw ice::Error{ICE_ERR_HIP, "table setup launch failed"};
         ICE_HIP(hipDeviceSynchronize());
         return ICE_OK;
@@ -932,6 +944,72 @@ int ice_project(ice_disc* d, const ice_field* f, double* u, void* stream) {
     (void)hipFree(b);
     (void)hipFree(df);
     if (rc != 0) return fail(ICE_ERR_HIP, "projection failed");
+    return ICE_OK;
+}
+
+int ice_project_values(ice_disc* d, const double* vq, double* u, void* stream) {
+    if (!d || !vq || !u) return fail(ICE_ERR_ARG, "null argument");
+    const ice_space* s = d->space;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t n = (size_t)s->mesh.nelem * s->nb * d->neq;
+    double* b = nullptr;
+    if (hipMalloc(&b, std::max<size_t>(n, 1) * sizeof(double)) != hipSuccess)
+        return fail(ICE_ERR_ALLOC, "projection scratch allocation failed");
+    int rc = ice::load_from_values(s->ndim, s->p, d->neq, s->dev, vq, 0, b, st);
+    if (rc == 0) rc = ice::launch_minv(s->ndim, s->p, d->neq, s->dev, b, u, st);
+    if (rc == 0) rc = (int)hipStreamSynchronize(st);
+    (void)hipFree(b);
+    if (rc != 0) return fail(ICE_ERR_HIP, "projection failed");
+    return ICE_OK;
+}
+
+int ice_disc_set_source_values(ice_disc* d, const double* vq, void* stream) {
+    if (!d || !vq) return fail(ICE_ERR_ARG, "null argument");
+    const ice_space* s = d->space;
+    hipStream_t st = (hipStream_t)stream;
+    if (!d->d_srcload &&
+        hipMalloc(&d->d_srcload, std::max<size_t>((size_t)s->mesh.nelem * s->nb * d->neq, 1) * sizeof(double)) !=
+            hipSuccess) {
+        d->d_srcload = nullptr;
+        return fail(ICE_ERR_ALLOC, "source load allocation failed");
+    }
+    int rc = ice::load_from_values(s->ndim, s->p, d->neq, s->dev, vq, 1, d->d_srcload, st);
+    if (rc == 0) rc = (int)hipStreamSynchronize(st);
+    if (rc != 0) return fail(ICE_ERR_HIP, "source load failed");
+    d->src_values = true;
+    d->dd.srcload = d->d_srcload;
+    return ICE_OK;
+}
+
+int ice_disc_set_boundary_values(ice_disc* d, const double* bv, void* stream) {
+    if (!d || !bv) return fail(ICE_ERR_ARG, "null argument");
+    const ice_space* s = d->space;
+    const size_t n = (size_t)s->dev.nfb * s->nfq * d->neq;
+    if (n == 0) return ICE_OK;
+    hipStream_t st = (hipStream_t)stream;
+    int rc = (int)hipMemcpyAsync(d->d_bval, bv, n * sizeof(double), hipMemcpyDeviceToDevice, st);
+    if (rc == 0) rc = (int)hipStreamSynchronize(st);
+    if (rc != 0) return fail(ICE_ERR_HIP, "boundary value copy failed");
+    d->bval_values = true;
+    return ICE_OK;
+}
+
+int ice_space_bface_coords(const ice_space* s, double* xb, int* bctype, int* bcflag) {
+    if (!s) return fail(ICE_ERR_ARG, "null argument");
+    const int nfb = s->dev.nfb;
+    if (bctype || bcflag)
+        for (int b = 0; b < nfb; ++b) {
+            if (bctype) bctype[b] = (s->fbdy[b].y >> 9) & 15;
+            if (bcflag) bcflag[b] = s->fbdy[b].z;
+        }
+    if (!xb || nfb == 0) return ICE_OK;
+    const size_t n = (size_t)nfb * s->nfq * s->ndim;
+    double* dx = nullptr;
+    if (hipMalloc(&dx, n * sizeof(double)) != hipSuccess) return fail(ICE_ERR_ALLOC, "allocation failed");
+    int rc = ice::bface_coords(s->ndim, s->p, s->dev, dx, nullptr);
+    if (rc == 0) rc = (int)hipMemcpy(xb, dx, n * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(dx);
+    if (rc != 0) return fail(ICE_ERR_HIP, "boundary point evaluation failed");
     return ICE_OK;
 }
 

@@ -122,6 +122,12 @@ int residual_mixed_f32state(int ndim, int p, int flux, const DevSpace& S, const 
 // face quadrature points and the source load vector sum_q s(x_q) phi_i w detJ.
 int setup_disc_tables(int ndim, int p, int neq, const DevSpace& S, const ice_field* fields, double* bval,
                       double* srcload, hipStream_t st);
+// Load vector sum_q vals[e][q][v] phi_i w detJ (clamp: max(0, detJ)) from
+// values at the volume quadrature points; physical coordinates of the
+// boundary face quadrature points [nfb][nfq][ndim].
+int load_from_values(int ndim, int p, int neq, const DevSpace& S, const double* vals, int clamp, double* out,
+                     hipStream_t st);
+int bface_coords(int ndim, int p, const DevSpace& S, double* out, hipStream_t st);
 
 // Face geometry table of the space (see DevSpace::fgeo) and the size in
 // doubles of a discretization's face trace buffer.

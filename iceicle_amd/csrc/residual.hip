@@ -2187,6 +2187,79 @@ __global__ __launch_bounds__(256) void srcload_kernel(DevSpace S, const ice_fiel
     for (int v = 0; v < M; ++v) out[((size_t)e * NB + i) * M + v] = acc[v];
 }
 
+// The same load vector from values at the volume quadrature points,
+// vals[e][q][v] in the order of ice_space_qp_coords (clamp: max(0, detJ) as
+// the source load; 0: the raw detJ of the projection).
+template <int D, int P>
+__global__ __launch_bounds__(256) void load_values_kernel(DevSpace S, const double* __restrict__ vals, int M,
+                                                          int clamp, double* __restrict__ out) {
+    constexpr int N = P + 1;
+    constexpr int NB = Pow<N, D>::value;
+    const int e = blockIdx.x;
+    const int i = threadIdx.x;
+    if (i >= NB) return;
+    const double* xs = S.tab;
+    const double* L0 = S.tab + (N + 2);
+    const double* wgl = L0 + 3 * (N + 2) * N;
+    double X[1 << D][D];
+    load_vertices<D>(S.X, e, X);
+    int id[D];
+    {
+        int r = i;
+        for (int d = D - 1; d >= 0; --d) { id[d] = r % N; r /= N; }
+    }
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int q = 0; q < NB; ++q) {
+        int qd[D];
+        int r = q;
+        for (int d = D - 1; d >= 0; --d) { qd[d] = r % N; r /= N; }
+        double xi[D], wq = 1.0, phi = 1.0;
+        for (int d = 0; d < D; ++d) {
+            xi[d] = xs[qd[d]];
+            wq = (d == 0) ? wgl[qd[d]] : wq * wgl[qd[d]];
+            phi *= L0[qd[d] * N + id[d]];
+        }
+        double J[D][D], Hm[D][D][D], A[D][D];
+        q1_map<D>(X, xi, nullptr, J, Hm);
+        const double det = det_adj<D>(J, A);
+        const double detc = clamp ? fmax(0.0, det) : det;
+        const double* src = vals + ((size_t)e * NB + q) * M;
+        for (int v = 0; v < M; ++v) acc[v] += src[v] * phi * detc * wq;
+    }
+    for (int v = 0; v < M; ++v) out[((size_t)e * NB + i) * M + v] = acc[v];
+}
+
+// Physical coordinates of the boundary face quadrature points, out[b][q][d]
+// in the device boundary face order (the points bval_kernel evaluates at).
+template <int D, int P>
+__global__ __launch_bounds__(256) void bface_coords_kernel(DevSpace S, double* __restrict__ out) {
+    constexpr int N = P + 1;
+    constexpr int NFQ = Pow<N, D - 1>::value;
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = tid / NFQ, q = tid - b * NFQ;
+    if (b >= S.nfb) return;
+    const int4 rec = S.fbdy[b];
+    double X[1 << D][D], x[D], nrm[D];
+    load_vertices<D>(S.X, rec.x, X);
+    (void)face_geometry<D, N>(S, X, info_fnl(rec.y), q, x, nrm);
+    for (int d = 0; d < D; ++d) out[((size_t)b * NFQ + q) * D + d] = x[d];
+}
+
+template <int D, int P>
+int launch_load_values(const DevSpace& S, const double* vals, int M, int clamp, double* out, hipStream_t st) {
+    constexpr int NB = Pow<P + 1, D>::value;
+    load_values_kernel<D, P><<<S.nelem, ((NB + 63) / 64) * 64, 0, st>>>(S, vals, M, clamp, out);
+    return (int)hipGetLastError();
+}
+
+template <int D, int P>
+int launch_bface_coords(const DevSpace& S, double* out, hipStream_t st) {
+    constexpr int NFQ = Pow<P + 1, D - 1>::value;
+    const long nt = (long)S.nfb * NFQ;
+    if (nt > 0) bface_coords_kernel<D, P><<<(unsigned)((nt + 255) / 256), 256, 0, st>>>(S, out);
+    return (int)hipGetLastError();
+}
+
 template <int D, int P>
 int launch_setup(const DevSpace& S, const ice_field* fields, int M, double* bval, double* srcload, hipStream_t st) {
     constexpr int N = P + 1;
@@ -2272,6 +2345,26 @@ int l2_error_terms(int ndim, int p, int neq, const DevSpace& S, const ice_field*
     ICE_LCASE(2, 1) ICE_LCASE(2, 2) ICE_LCASE(2, 3) ICE_LCASE(2, 4)
     ICE_LCASE(3, 1) ICE_LCASE(3, 2) ICE_LCASE(3, 3) ICE_LCASE(3, 4)
 #undef ICE_LCASE
+    return -1;
+}
+
+int load_from_values(int ndim, int p, int neq, const DevSpace& S, const double* vals, int clamp, double* out,
+                     hipStream_t st) {
+    if (neq > 8) return -1;
+#define ICE_VCASE(DD, PP) if (ndim == DD && p == PP) return launch_load_values<DD, PP>(S, vals, neq, clamp, out, st);
+    ICE_VCASE(1, 1) ICE_VCASE(1, 2) ICE_VCASE(1, 3) ICE_VCASE(1, 4)
+    ICE_VCASE(2, 1) ICE_VCASE(2, 2) ICE_VCASE(2, 3) ICE_VCASE(2, 4)
+    ICE_VCASE(3, 1) ICE_VCASE(3, 2) ICE_VCASE(3, 3) ICE_VCASE(3, 4)
+#undef ICE_VCASE
+    return -1;
+}
+
+int bface_coords(int ndim, int p, const DevSpace& S, double* out, hipStream_t st) {
+#define ICE_BCASE(DD, PP) if (ndim == DD && p == PP) return launch_bface_coords<DD, PP>(S, out, st);
+    ICE_BCASE(1, 1) ICE_BCASE(1, 2) ICE_BCASE(1, 3) ICE_BCASE(1, 4)
+    ICE_BCASE(2, 1) ICE_BCASE(2, 2) ICE_BCASE(2, 3) ICE_BCASE(2, 4)
+    ICE_BCASE(3, 1) ICE_BCASE(3, 2) ICE_BCASE(3, 3) ICE_BCASE(3, 4)
+#undef ICE_BCASE
     return -1;
 }
 

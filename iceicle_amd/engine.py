@@ -42,6 +42,8 @@ _SIGS = {
     "ice_mdg_select": ([P, C.POINTER(C.c_int), C.c_int, C.POINTER(P)], C.c_int),
     "ice_mdg_query": ([P, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
     "ice_mdg_residual": ([P, P, P, P, P], C.c_int),
+    "ice_mdg_form_residual": ([P, P, P, P, P], C.c_int),
+    "ice_interface_error": ([P, P, C.POINTER(C.c_double), P], C.c_int),
     "ice_mdg_destroy": ([P], C.c_int),
     "ice_disc_update": ([P, C.POINTER(abi.DiscParams)], C.c_int),
     "ice_space_create_batch": ([C.POINTER(abi.MeshDesc), C.c_int, C.POINTER(abi.SpaceDesc), C.POINTER(P)], C.c_int),
@@ -62,6 +64,10 @@ _SIGS = {
     "ice_mass_inverse_apply": ([P, P, P, P], C.c_int),
     "ice_project": ([P, C.POINTER(abi.Field), P, P], C.c_int),
     "ice_l2_error": ([P, C.POINTER(abi.Field), P, C.POINTER(C.c_double), P], C.c_int),
+    "ice_project_values": ([P, P, P, P], C.c_int),
+    "ice_disc_set_source_values": ([P, P, P], C.c_int),
+    "ice_space_bface_coords": ([P, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+    "ice_disc_set_boundary_values": ([P, P, P], C.c_int),
     "ice_stepper_create": ([P, C.POINTER(abi.StepperDesc), C.POINTER(P)], C.c_int),
     "ice_stepper_destroy": ([P], C.c_int),
     "ice_stepper_step": ([P, P, P], C.c_int),
@@ -185,6 +191,17 @@ class FESpace:
         check(lib().ice_space_node_coords(self.h, _dp(xn)))
         return xn
 
+    def bface_coords(self):
+        """(xb [nfb][nfqp][ndim], bctype [nfb], bcflag [nfb]) of the boundary
+        faces in device order (ice_space_bface_coords)."""
+        nfb = self.info.nface_boundary
+        xb = np.empty((nfb, self.nfqp, self.ndim))
+        bt = np.empty(nfb, dtype=np.int32)
+        bf = np.empty(nfb, dtype=np.int32)
+        check(lib().ice_space_bface_coords(self.h, _dp(xb), bt.ctypes.data_as(C.POINTER(C.c_int)),
+                                           bf.ctypes.data_as(C.POINTER(C.c_int))))
+        return xb, bt, bf
+
     def interpolate(self, field, neq):
         """Nodal interpolant of an analytic field (host array, fe_layout_right)."""
         xn = self.node_coords().reshape(-1, self.ndim)
@@ -226,6 +243,22 @@ class ConservationLawDDG:
 
     def set_tile(self, tile):
         check(lib().ice_disc_set_tile(self.h, int(tile)))
+
+    def set_source_values(self, vq, stream=None):
+        """Source term from values at the volume quadrature points
+        (float64 CUDA tensor, [nelem][nqp][neq] in qp_coords order)."""
+        _require_cuda(vq, "vq")
+        if vq.numel() != self.space.nelem * self.space.nqp * self.neq:
+            raise IceError("source values must be [nelem][nqp][neq]")
+        check(lib().ice_disc_set_source_values(self.h, _ptr(vq), stream or current_stream()))
+
+    def set_boundary_values(self, bv, stream=None):
+        """Dirichlet states / Neumann data at the boundary face quadrature
+        points (float64 CUDA tensor, [nfb][nfqp][neq] in bface_coords order)."""
+        _require_cuda(bv, "bv")
+        if bv.numel() != self.space.info.nface_boundary * self.space.nfqp * self.neq:
+            raise IceError("boundary values must be [nface_boundary][nfqp][neq]")
+        check(lib().ice_disc_set_boundary_values(self.h, _ptr(bv), stream or current_stream()))
 
     def autotune(self, u, r, reps=10, stream=None):
         """Time the element-kernel tiles on this state and keep the fastest,
@@ -326,6 +359,26 @@ class MDGSelection:
         check(lib().ice_mdg_residual(disc.h, self.h, _ptr(u), _ptr(res), stream or current_stream()))
         return res
 
+    def form_residual(self, disc, u, res=None, stream=None):
+        """The combined residual over the selection (form_residual with a
+        nodeset, form_residual.hpp:380-424): u = [DG coefficients | selected
+        node coordinates [nnodes][ndim]] (float64 CUDA), returns [DG residual
+        | MDG residual [nnodes][neq]].  The selected nodes of the space's
+        device geometry move to the given coordinates and stay there."""
+        import torch
+        _require_cuda(u, "u")
+        nin = disc.size + self.nodes.size * self.space.ndim
+        if u.numel() != nin:
+            raise IceError(f"u must have {nin} entries (DG coefficients, then node coordinates)")
+        nout = disc.size + self.nodes.size * disc.neq
+        if res is None:
+            res = torch.empty(nout, dtype=torch.float64, device=u.device)
+        _require_cuda(res, "res")
+        if res.numel() != nout:
+            raise IceError(f"res must have {nout} entries")
+        check(lib().ice_mdg_form_residual(disc.h, self.h, _ptr(u), _ptr(res), stream or current_stream()))
+        return res
+
     def __del__(self):
         try:
             if getattr(self, "h", None):
@@ -340,6 +393,25 @@ def mass_inverse_apply(disc, r, out, stream=None):
     _require_cuda(r, "r")
     _require_cuda(out, "out")
     check(lib().ice_mass_inverse_apply(disc.h, _ptr(r), _ptr(out), stream or current_stream()))
+
+
+def interface_error(disc, u, stream=None):
+    """ic_error (l2_error.hpp:279-410): sum over interior faces of |jump of
+    the normal physical flux| w sqrt(g), over points and equations."""
+    _require_cuda(u, "u")
+    e = C.c_double()
+    check(lib().ice_interface_error(disc.h, _ptr(u), C.byref(e), stream or current_stream()))
+    return e.value
+
+
+def project_values(disc, vq, u, stream=None):
+    """u <- L2 projection of values at the volume quadrature points
+    (float64 CUDA tensors; vq [nelem][nqp][neq] in qp_coords order)."""
+    _require_cuda(vq, "vq")
+    _require_cuda(u, "u")
+    if vq.numel() != disc.space.nelem * disc.space.nqp * disc.neq or u.numel() != disc.size:
+        raise IceError("vq must be [nelem][nqp][neq] and u the discretization's size")
+    check(lib().ice_project_values(disc.h, _ptr(vq), _ptr(u), stream or current_stream()))
 
 
 def project(disc, field, u, stream=None):

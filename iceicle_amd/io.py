@@ -59,6 +59,21 @@ class HostGeometry:
         """Physical points of reference points xi [npts][d] in every element: [nelem][npts][d]."""
         return np.einsum("pn,end->epd", self.q1_shape(xi), self.X)
 
+    def det_jacobian(self, xi):
+        """det(dx/dxi) of the Q1 map at reference points xi in every element: [nelem][npts]."""
+        xi = np.atleast_2d(np.asarray(xi, dtype=np.float64))
+        npts, D = xi.shape[0], self.ndim
+        J = np.empty((self.nelem, npts, D, D))
+        for a in range(D):
+            sh = np.ones((npts, 1))
+            for d in range(D):
+                s = xi[:, d]
+                r = (np.stack([np.full_like(s, -0.5), np.full_like(s, 0.5)], axis=1) if d == a
+                     else np.stack([(1.0 - s) / 2.0, (1.0 + s) / 2.0], axis=1))
+                sh = (sh[:, :, None] * r[:, None, :]).reshape(npts, -1)
+            J[:, :, :, a] = np.einsum("pn,end->epd", sh, self.X)
+        return np.linalg.det(J)
+
     def basis(self, xi):
         """Lagrange basis on the equispaced nodes -1 + 2j/p (lagrange_1d.hpp:19-87) at xi: [npts][nbasis]."""
         p = self.order

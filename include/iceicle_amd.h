@@ -308,6 +308,31 @@ int ice_mass_inverse_apply(ice_disc* disc, const double* r_dev, double* out_dev,
  * as used by projection_initialization, initialization.hpp:20-30).  Synchronous. */
 int ice_project(ice_disc* disc, const ice_field* f, double* u_dev, void* stream);
 
+/* Callback data given as values at points (the reference's std::function
+ * callbacks: initial conditions, initialization.hpp:37-104; Dirichlet and
+ * Neumann callbacks and the user source, conservation_law.hpp:700-730,
+ * 300-312; set from Lua functions by solvers_lua_interface.hpp).  A host
+ * caller evaluates its function at the points these report and hands the
+ * values over on the device:
+ *  - ice_project_values: L2 projection of values vq_dev[nelem][nqp][neq] at
+ *    the volume quadrature points (ice_space_qp_coords order); synchronous;
+ *  - ice_disc_set_source_values: the source term from the same kind of
+ *    values (the load is formed once, with max(0, detJ) as the volume
+ *    integral's source term); synchronous;
+ *  - ice_space_bface_coords: per device boundary face b (ice_space_info's
+ *    nface_boundary faces, in device order) its boundary type and flag, and the physical coordinates xb[b][q][ndim] of its quadrature
+ *    points (any of the three outputs may be NULL);
+ *  - ice_disc_set_boundary_values: bv_dev[b][q][neq], the Dirichlet state
+ *    on Dirichlet faces and the Neumann data on Neumann faces (other faces
+ *    ignore it); synchronous.
+ * Values set this way replace the analytic fields' and are kept when the
+ * geometry or the parameters are refreshed (ice_disc_update,
+ * ice_mdg_form_residual).  No reference counterpart as entry points. */
+int ice_project_values(ice_disc* disc, const double* vq_dev, double* u_dev, void* stream);
+int ice_disc_set_source_values(ice_disc* disc, const double* vq_dev, void* stream);
+int ice_space_bface_coords(const ice_space* space, double* xb, int* bctype, int* bcflag);
+int ice_disc_set_boundary_values(ice_disc* disc, const double* bv_dev, void* stream);
+
 /* L2 error of u against an analytic field (l2_error.hpp:37-94).  Synchronous. */
 int ice_l2_error(ice_disc* disc, const ice_field* f, const double* u_dev, double* err, void* stream);
 
@@ -555,7 +580,22 @@ int ice_mdg_select(const ice_space* space, const int* traces, int ntraces, ice_m
 /* number of selected nodes, and their global node ids when nodes != NULL */
 int ice_mdg_query(const ice_mdg_sel* sel, int* nnodes, int* nodes);
 int ice_mdg_residual(ice_disc* disc, const ice_mdg_sel* sel, const double* u_dev, double* res_dev, void* stream);
+/* The combined residual over the selection (form_residual with a
+ * nodeset_dof_map, form_residual.hpp:380-424): u_dev = [DG coefficients
+ * (fe_layout_right) | coordinates of the selected nodes [nnodes][ndim]],
+ * res_dev = [DG residual | MDG residual [nnodes][neq]].  The selected nodes
+ * are first moved to the given coordinates in the space's device geometry
+ * (element vertices, face geometry table, source load), which keeps them for
+ * every later call on the space, as the reference's mesh keeps them; host-side
+ * quantities taken at creation (a stepper's CFL reference length) are not
+ * updated. */
+int ice_mdg_form_residual(ice_disc* disc, const ice_mdg_sel* sel, const double* u_dev, double* res_dev,
+                          void* stream);
 int ice_mdg_destroy(ice_mdg_sel* sel);
+/* The interface error (ic_error, l2_error.hpp:279-410): the sum over interior
+ * faces, face points and equations of |jump of the normal physical flux|
+ * times w sqrt(g), gradients kept at every order.  Synchronous; *err host. */
+int ice_interface_error(ice_disc* disc, const double* u_dev, double* err, void* stream);
 
 /* Glorot-uniform weights (seeded), zero biases.  widths has nlayers+1 entries. */
 int ice_mlp_create(int nlayers, const int* widths, unsigned long long seed, ice_mlp** out);
