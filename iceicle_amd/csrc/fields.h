@@ -29,7 +29,8 @@ ICE_HD inline void ice_isentropic_vortex(const double* p, int ndim, const double
     const double f = exp(0.5 * (1.0 - r2));
     const double u = uinf - beta / (2.0 * pi) * f * dy;
     const double v = vinf + beta / (2.0 * pi) * f * dx;
-    const double T = 1.0 - (gamma - 1.0) * beta * beta / (8.0 * gamma * pi * pi) * exp(1.0 - r2);
+    /* T / T_inf with c_inf^2 = gamma p_inf / rho_inf = 1 (the deck's T_hat) */
+    const double T = 1.0 - (gamma - 1.0) * beta * beta / (8.0 * pi * pi) * exp(1.0 - r2);
     const double rho = pow(T, 1.0 / (gamma - 1.0));
     const double pr = pow(rho, gamma) / gamma;  /* p = rho^gamma / gamma  (c_inf = 1) */
     out[0] = rho;
