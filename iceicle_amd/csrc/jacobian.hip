@@ -243,22 +243,41 @@ __global__ __launch_bounds__(256) void jac_face_kernel(DevSpace S, PhysParams ph
     for (int it = tid; it < NC * NFQ; it += blockDim.x) {
         const int c = it / NFQ, q = it % NFQ;
         const int side = c / NBM, iw = c % NBM, i = iw / M, w = iw % M;
-        double u[2][M], g[2][M * D], h[2][M * D];
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-#pragma unroll
-            for (int v = 0; v < M; ++v) u[s][v] = su[s][q][v];
-#pragma unroll
-            for (int k = 0; k < M * D; ++k) { g[s][k] = sg[s][q][k]; h[s][k] = shn[s][q][k]; }
-        }
-        u[side][w] += eps * bphi[side][q][i];
+        // both sides' records with column c's perturbation added by selects
+        // (a runtime index into the register arrays would move them to scratch)
+        double uL[M], gL[M * D], hL[M * D], uR[M], gR[M * D], hR[M * D];
+        const double du = eps * bphi[side][q][i];
+        double dg[D], dh[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            g[side][w * D + d] += eps * bgx[side][q][i][d];
-            h[side][w * D + d] += eps * bhn[side][q][i][d];
+            dg[d] = eps * bgx[side][q][i][d];
+            dh[d] = eps * bhn[side][q][i][d];
+        }
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            const bool hitL = side == 0 && v == w, hitR = side == 1 && v == w;
+            uL[v] = su[0][q][v];
+            uR[v] = su[1][q][v];
+            if (hitL) uL[v] += du;
+            if (hitR) uR[v] += du;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                gL[v * D + d] = sg[0][q][v * D + d];
+                hL[v * D + d] = shn[0][q][v * D + d];
+                gR[v * D + d] = sg[1][q][v * D + d];
+                hR[v * D + d] = shn[1][q][v * D + d];
+                if (hitL) {
+                    gL[v * D + d] += dg[d];
+                    hL[v * D + d] += dh[d];
+                }
+                if (hitR) {
+                    gR[v * D + d] += dg[d];
+                    hR[v * D + d] += dh[d];
+                }
+            }
         }
         double Q[M];
-        interior_point_flux<D, PH>(ph, u[0], g[0], h[0], u[1], g[1], h[1], snrm[q], sh[q], sws[q], Q);
+        interior_point_flux<D, PH>(ph, uL, gL, hL, uR, gR, hR, snrm[q], sh[q], sws[q], Q);
 #pragma unroll
         for (int v = 0; v < M; ++v) sdQ[c][q][v] = Q[v] - sQ0[q][v];
     }
@@ -268,23 +287,42 @@ __global__ __launch_bounds__(256) void jac_face_kernel(DevSpace S, PhysParams ph
     // can be the same element, whose diagonal block then takes both in turn)
     const size_t bs = NBM;
     const double ieps = 1.0 / eps;
+    // each thread takes LU entries a pass and issues the loads of its
+    // diagonal-block entries together before adding to them (the adds wait
+    // on one round trip instead of LU)
+    constexpr int ENT = NBM * NC;
+    constexpr int LU = 8;
 #pragma unroll 1
     for (int rs = 0; rs < 2; ++rs) {
         const int erow = rs ? eR : eL;
-        for (int ent = tid; ent < NBM * NC; ent += blockDim.x) {
-            const int iv = ent / NC, c = ent % NC;
-            const int i = iv / M, v = iv % M;
-            const int cs = c / NBM, cw = c % NBM;
-            double acc = 0;
+        double* Jd = J + ((size_t)erow * bslots) * bs * bs;
+        double* Jo = J + ((size_t)erow * bslots + 1 + (rs ? fnr : fnl)) * bs * bs;
+#pragma unroll 1
+        for (int e0 = tid; e0 < ENT; e0 += blockDim.x * LU) {
+            double val[LU], old[LU];
+            int off[LU];
+            bool diag[LU], ok[LU];
 #pragma unroll
-            for (int q = 0; q < NFQ; ++q) acc += sdQ[c][q][v] * bphi[rs][q][i];
-            const double val = (rs ? -acc : acc) * ieps;
-            if (rs == cs) {
-                double* a = J + ((size_t)erow * bslots) * bs * bs + (size_t)iv * bs + cw;
-                *a += val;
-            } else {
-                const int slot = 1 + (rs ? fnr : fnl);
-                J[((size_t)erow * bslots + slot) * bs * bs + (size_t)iv * bs + cw] = val;
+            for (int k = 0; k < LU; ++k) {
+                const int ent = e0 + k * (int)blockDim.x;
+                ok[k] = ent < ENT;
+                const int en = ok[k] ? ent : 0;
+                const int iv = en / NC, c = en % NC;
+                const int i = iv / M, v = iv % M;
+                const int cs = c / NBM, cw = c % NBM;
+                double acc = 0;
+#pragma unroll
+                for (int q = 0; q < NFQ; ++q) acc += sdQ[c][q][v] * bphi[rs][q][i];
+                val[k] = (rs ? -acc : acc) * ieps;
+                diag[k] = rs == cs;
+                off[k] = iv * (int)bs + cw;
+                old[k] = (ok[k] && diag[k]) ? Jd[off[k]] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < LU; ++k) {
+                if (!ok[k]) continue;
+                if (diag[k]) Jd[off[k]] = old[k] + val[k];
+                else Jo[off[k]] = val[k];
             }
         }
         __syncthreads();
@@ -422,6 +460,7 @@ __global__ __launch_bounds__(256) void jac_elem_kernel(DevSpace S, PhysParams ph
     __shared__ double sgx[NB][NB][D];  // [q][i][d] physical basis gradients
     __shared__ double sphi[NB][NB];    // [q][j] basis values
     __shared__ double sdw[NB];
+    __shared__ double su[NB][M], sg[NB][M * D], sF[NB][M * D], seps[NB];  // state, gradient, flux, eps per point
     const int e = blockIdx.x;
     const int tid = threadIdx.x;
     const double* xs = S.tab;
@@ -472,34 +511,47 @@ __global__ __launch_bounds__(256) void jac_elem_kernel(DevSpace S, PhysParams ph
                 for (int j = 0; j < D; ++j) g[v * D + j] += c * gxi[j];
             }
         }
-        double F[M * D], Fp[M * D];
+        double F[M * D];
         PH::flux(ph, u, g, F);
         double fro = 0;
 #pragma unroll
         for (int k = 0; k < M * D; ++k) fro += F[k] * F[k];
-        const double eps = fmax(SQRT_EPS, SQRT_EPS * sqrt(fro));  // conservation_law.hpp:366-369
-        const double ieps = 1.0 / eps;
+        seps[q] = fmax(SQRT_EPS, SQRT_EPS * sqrt(fro));  // conservation_law.hpp:366-369
 #pragma unroll
-        for (int w = 0; w < M; ++w) {
-            const double uo = u[w];
-            u[w] += eps;
-            PH::flux(ph, u, g, Fp);
+        for (int v = 0; v < M; ++v) su[q][v] = u[v];
+#pragma unroll
+        for (int k = 0; k < M * D; ++k) {
+            sg[q][k] = g[k];
+            sF[q][k] = F[k];
+        }
+    }
+    __syncthreads();
+    // one perturbed flux per (point, perturbation): M value perturbations,
+    // then M * D gradient perturbations, spread over the workgroup
+    constexpr int NPERT = M + M * D;
+    for (int t = tid; t < NB * NPERT; t += blockDim.x) {
+        const int q = t / NPERT, k = t % NPERT;
+        double u[M], g[M * D], Fp[M * D];
+#pragma unroll
+        for (int v = 0; v < M; ++v) u[v] = su[q][v];
+#pragma unroll
+        for (int j = 0; j < M * D; ++j) g[j] = sg[q][j];
+        const double eps = seps[q];
+        const double ieps = 1.0 / eps;
+        if (k < M) u[k] += eps;
+        else g[k - M] += eps;
+        PH::flux(ph, u, g, Fp);
+        if (k < M) {
 #pragma unroll
             for (int v = 0; v < M; ++v)
 #pragma unroll
-                for (int d = 0; d < D; ++d) sdFdu[q][v][d][w] = (Fp[v * D + d] - F[v * D + d]) * ieps;
-            u[w] = uo;
+                for (int d = 0; d < D; ++d) sdFdu[q][v][d][k] = (Fp[v * D + d] - sF[q][v * D + d]) * ieps;
+        } else {
+            const int w = (k - M) / D, jd = (k - M) % D;
 #pragma unroll
-            for (int jd = 0; jd < D; ++jd) {
-                const double go = g[w * D + jd];
-                g[w * D + jd] += eps;
-                PH::flux(ph, u, g, Fp);
+            for (int v = 0; v < M; ++v)
 #pragma unroll
-                for (int v = 0; v < M; ++v)
-#pragma unroll
-                    for (int d = 0; d < D; ++d) sdFdg[q][v][d][w][jd] = (Fp[v * D + d] - F[v * D + d]) * ieps;
-                g[w * D + jd] = go;
-            }
+                for (int d = 0; d < D; ++d) sdFdg[q][v][d][w][jd] = (Fp[v * D + d] - sF[q][v * D + d]) * ieps;
         }
     }
     __syncthreads();
