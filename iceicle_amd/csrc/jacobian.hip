@@ -626,18 +626,34 @@ int jac_order(int p, const DevSpace& S, const PhysParams& ph, const DiscDev& dd,
     }
 }
 
+// 3D: orders whose per-workgroup tables (the face kernel's flux
+// differences for all 2 nb m columns, the element kernel's flux derivatives
+// and basis gradients) fit the 160 KB of LDS of a workgroup: P1 and P2
+template <class PH>
+int jac_order3(int p, const DevSpace& S, const PhysParams& ph, const DiscDev& dd, const double* U, double* J,
+               hipStream_t st) {
+    switch (p) {
+    case 1: return launch_jac<3, 1, PH>(S, ph, dd, U, J, st);
+    case 2: return launch_jac<3, 2, PH>(S, ph, dd, U, J, st);
+    default: return -2;
+    }
+}
+
 int jacobian_fp64(int ndim, int p, int flux, const DevSpace& S, const PhysParams& ph, const DiscDev& dd,
                   const double* U, double* J, size_t nvals, hipStream_t st) {
     (void)nvals;
     if (flux == ICE_FLUX_BURGERS) {
         if (ndim == 1) return jac_order<Burgers<1>, 1>(p, S, ph, dd, U, J, st);
         if (ndim == 2) return jac_order<Burgers<2>, 2>(p, S, ph, dd, U, J, st);
+        if (ndim == 3) return jac_order3<Burgers<3>>(p, S, ph, dd, U, J, st);
     } else if (flux == ICE_FLUX_NAVIER_STOKES) {
         if (ndim == 1) return jac_order<NavierStokes<1, true>, 1>(p, S, ph, dd, U, J, st);
         if (ndim == 2) return jac_order<NavierStokes<2, true>, 2>(p, S, ph, dd, U, J, st);
+        if (ndim == 3) return jac_order3<NavierStokes<3, true>>(p, S, ph, dd, U, J, st);
     } else if (flux == ICE_FLUX_EULER) {
         if (ndim == 1) return jac_order<NavierStokes<1, false>, 1>(p, S, ph, dd, U, J, st);
         if (ndim == 2) return jac_order<NavierStokes<2, false>, 2>(p, S, ph, dd, U, J, st);
+        if (ndim == 3) return jac_order3<NavierStokes<3, false>>(p, S, ph, dd, U, J, st);
     }
     return -1;
 }

@@ -185,13 +185,20 @@ struct Problem final : ProblemBase {
                 nbr[(size_t)t.elemL * S + 1 + t.fnl] = t.elemR;
                 nbr[(size_t)t.elemR * S + 1 + t.fnr] = t.elemL;
             }
+        // the (row, column) block goes to the first slot naming that column:
+        // where two faces of an element meet the same neighbour (a periodic
+        // direction two elements wide, or one element wide for the diagonal)
+        // the later slots hold zeros, so the slots of a row sum to the row's
+        // blocks and the product over slots is the matrix product
         for (int e = 0; e < E; ++e)
             for (int sl = 0; sl < S; ++sl) {
                 const int c = nbr[(size_t)e * S + sl];
                 cols[(size_t)e * S + sl] = c;
+                bool first = true;
+                for (int s0 = 0; s0 < sl; ++s0) first = first && nbr[(size_t)e * S + s0] != c;
                 double* dst = blocks + ((size_t)e * S + sl) * bs * bs;
-                auto it = (c >= 0) ? B.find({e, c}) : B.end();
-                for (int k = 0; k < bs * bs; ++k) dst[k] = (it != B.end() && (sl == 0 || c != e)) ? it->second[k] : 0.0;
+                auto it = (c >= 0 && first) ? B.find({e, c}) : B.end();
+                for (int k = 0; k < bs * bs; ++k) dst[k] = (it != B.end()) ? it->second[k] : 0.0;
             }
     }
     // RK3SSP/RK3TVD/ExplicitEuler ::solve (ssp_rk3.hpp:208-233, tvd_rk3.hpp:240-270,

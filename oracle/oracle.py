@@ -193,7 +193,10 @@ class Oracle:
 
     def jacobian(self, u):
         """(res, cols[nelem][2 ndim + 1], blocks[nelem][2 ndim + 1][bs][bs]) by the
-        reference's finite-difference block Jacobian."""
+        reference's finite-difference block Jacobian.  Slot 0 is the diagonal,
+        slot 1 + f the neighbour across face f; a neighbour met across several
+        faces has its whole block in the first such slot and zeros in the
+        others."""
         u = np.ascontiguousarray(u, dtype=np.float64)
         S = 2 * self.ndim + 1
         bs = self.nbasis * self.neq

@@ -31,6 +31,20 @@ def _mesh(ndim, nel, bct, perturb, seed, bcflag=None, coupled=False):
     return m, coords
 
 
+def _by_pair(cols, blocks):
+    """{(row, column): sum of the row's slot blocks naming that column}: the
+    device keeps one block per face, the oracle one per neighbour, and the
+    two agree as sums when two faces of an element meet the same neighbour."""
+    out = {}
+    for e in range(cols.shape[0]):
+        for s in range(cols.shape[1]):
+            c = int(cols[e, s])
+            if c >= 0:
+                key = (e, c)
+                out[key] = out[key] + blocks[e, s] if key in out else blocks[e, s].copy()
+    return out
+
+
 def _compare(m, order, params, u):
     from iceicle_amd import engine
     from oracle.oracle import Oracle
@@ -59,7 +73,9 @@ def _compare(m, order, params, u):
     assert torch.equal(jac.blocks, first)
     blk = jac.blocks.cpu().numpy()
     scale = np.max(np.abs(blk_ref))
-    err = float(np.max(np.abs(blk - blk_ref)) / scale)
+    got, want = _by_pair(cols_ref, blk), _by_pair(cols_ref, blk_ref)
+    assert got.keys() == want.keys()
+    err = max(float(np.max(np.abs(got[k] - want[k]))) for k in want) / scale
     print(f"jacobian ndim={m.ndim} p={order} flux={params.flux} rel err {err:.2e} (scale {scale:.3e})", flush=True)
     assert np.all(np.isfinite(blk))
     assert err <= TOL
@@ -93,6 +109,16 @@ def test_burgers_2d_coupled_periodic(order):
     p = cases.burgers_params(2, mu=0.01)
     m, keep = _mesh(2, [3, 4], ["PERIODIC"] * 4, 0.15, 2, coupled=True)
     u = cases.random_state(p.flux, 2, 12, (order + 1) ** 2, 8)
+    _compare(m, order, p, u)
+
+
+@pytest.mark.parametrize("nel,order", [([2, 3], 2), ([1, 3], 1)])
+def test_burgers_2d_narrow_coupled_periodic(nel, order):
+    """Coupled periodic x two elements wide (both x faces meet the same
+    neighbour) and one element wide (both x faces meet the element itself)."""
+    p = cases.burgers_params(2, mu=0.02, a=(0.3, -0.2, 0))
+    m, keep = _mesh(2, nel, ["PERIODIC"] * 4, 0.0, 0, coupled=True)
+    u = cases.random_state(p.flux, 2, nel[0] * nel[1], (order + 1) ** 2, 31)
     _compare(m, order, p, u)
 
 
@@ -137,6 +163,35 @@ def test_ns_2d_isothermal_wall():
     _compare(m, 2, p, u)
 
 
+@pytest.mark.parametrize("order", [1, 2])
+def test_euler_3d(order):
+    """3D (P1, P2): walls, Riemann, Dirichlet and extrapolation faces on a
+    perturbed 2 x 3 x 2 mesh."""
+    p = cases.ns_params(3, flux=abi.FLUX_EULER)
+    m, keep = _mesh(3, [2, 3, 2], ["SLIP_WALL", "RIEMANN", "DIRICHLET", "EXTRAPOLATION", "SLIP_WALL", "DIRICHLET"],
+                    0.1, 21)
+    nb = (order + 1) ** 3
+    u = cases.random_state(p.flux, 3, 12, nb, 17)
+    _compare(m, order, p, u)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_ns_3d(order):
+    p = cases.ns_params(3)
+    m, keep = _mesh(3, [2, 2, 2], ["DIRICHLET"] * 6, 0.1, 23)
+    u = cases.random_state(p.flux, 3, 8, (order + 1) ** 3, 19)
+    _compare(m, order, p, u)
+
+
+def test_burgers_3d_coupled_periodic():
+    """Coupled periodic faces two elements wide in x and y: both faces of an
+    element in those directions meet the same neighbour."""
+    p = cases.burgers_params(3, mu=0.02, a=(0.3, -0.2, 0.1))
+    m, keep = _mesh(3, [2, 2, 3], ["PERIODIC"] * 6, 0.0, 0, coupled=True)
+    u = cases.random_state(p.flux, 3, 12, 27, 29)
+    _compare(m, 2, p, u)
+
+
 def test_unsupported_cases_raise():
     from iceicle_amd import engine
     p = cases.burgers_params(2, sigma_ic=0.5)
@@ -146,6 +201,10 @@ def test_unsupported_cases_raise():
     u = torch.zeros(disc.size, dtype=torch.float64, device="cuda")
     with pytest.raises(engine.IceError, match="DDGIC"):
         jac.assemble(u)
+    m3, keep3 = _mesh(3, [2, 2, 2], ["DIRICHLET"] * 6, 0.0, 0)
+    d3 = engine.ConservationLawDDG(engine.FESpace(m3, 3), cases.burgers_params(3))
+    with pytest.raises(engine.IceError, match="P1 and P2 in 3D"):
+        engine.BlockJacobian(d3).assemble(torch.zeros(d3.size, dtype=torch.float64, device="cuda"))
 
 
 def test_ns_mms_128_p3_directional_derivative():

@@ -1,0 +1,77 @@
+"""The oracle's finite-difference block Jacobian (form_petsc_jacobian_fd,
+form_petsc_jacobian.hpp:55-317) pinned by what the mathematics fixes: its
+slot blocks, summed as a block-sparse product, give J v equal to a central
+difference of the oracle's own residual along v, and for a linear law
+(advection-diffusion through Burgers with no nonlinear velocity) J v equals
+R(u + v) - R(u) to rounding.  The meshes include coupled periodic directions
+one and two elements wide, where several faces of an element meet the same
+neighbour."""
+import numpy as np
+import pytest
+
+from iceicle_amd import abi
+from oracle import oracle as O
+from tests import cases
+
+
+def _oracle(ndim, nel, order, params, bct, coupled=True):
+    m = abi.mesh_desc(ndim, nel, [0.0] * ndim, [1.0] * ndim, bct,
+                      periodic_mode=abi.PERIODIC_COUPLED if coupled else abi.PERIODIC_REFERENCE)
+    return O.Oracle(m, abi.space_desc(order), params)
+
+
+def _apply(cols, blocks, x):
+    bs = blocks.shape[2]
+    y = np.zeros_like(x)
+    for e in range(cols.shape[0]):
+        for s in range(cols.shape[1]):
+            c = cols[e, s]
+            if c >= 0:
+                y[e * bs:(e + 1) * bs] += blocks[e, s] @ x[c * bs:(c + 1) * bs]
+    return y
+
+
+CASES = [
+    (2, [2, 3], 2, ["PERIODIC"] * 4),
+    (2, [1, 3], 1, ["PERIODIC"] * 4),
+    (3, [2, 2, 3], 1, ["PERIODIC"] * 6),
+    (2, [3, 2], 2, ["DIRICHLET", "NEUMANN", "PERIODIC", "PERIODIC"]),
+]
+
+
+@pytest.mark.parametrize("ndim,nel,order,bct", CASES)
+def test_jv_matches_central_difference(ndim, nel, order, bct):
+    p = cases.burgers_params(ndim, mu=0.02, a=(0.3, -0.2, 0.1), b=(0.4, 0.3, -0.2))
+    o = _oracle(ndim, nel, order, p, bct)
+    nb = (order + 1) ** ndim
+    u = cases.random_state(p.flux, ndim, int(np.prod(nel)), nb, 41)
+    res, cols, blocks = o.jacobian(u)
+    assert np.allclose(res, o.residual(u), rtol=0, atol=1e-14 * np.abs(res).max())
+    v = np.random.default_rng(7).normal(size=u.size)
+    h = 1e-5
+    fd = (o.residual(u + h * v) - o.residual(u - h * v)) / (2 * h)
+    jv = _apply(cols, blocks, v)
+    scale = np.abs(fd).max()
+    # forward-difference columns: error ~ sqrt(machine eps) relative
+    assert np.abs(jv - fd).max() <= 1e-6 * scale, np.abs(jv - fd).max() / scale
+
+
+@pytest.mark.parametrize("ndim,nel,order,bct", CASES[:3])
+def test_linear_law_jacobian_is_the_operator(ndim, nel, order, bct):
+    p = cases.burgers_params(ndim, mu=0.05, a=(1.0, 0.5, -0.25), b=(0.0, 0.0, 0.0))
+    o = _oracle(ndim, nel, order, p, bct)
+    nb = (order + 1) ** ndim
+    u = cases.random_state(p.flux, ndim, int(np.prod(nel)), nb, 43)
+    _, cols, blocks = o.jacobian(u)
+    v = np.random.default_rng(9).normal(size=u.size)
+    want = o.residual(u + v) - o.residual(u)
+    got = _apply(cols, blocks, v)
+    assert np.abs(got - want).max() <= 1e-6 * np.abs(want).max()
+    # a repeated neighbour's block sits in its first slot only
+    for e in range(cols.shape[0]):
+        seen = set()
+        for s in range(cols.shape[1]):
+            c = int(cols[e, s])
+            if c in seen:
+                assert not blocks[e, s].any()
+            seen.add(c)
