@@ -109,16 +109,16 @@ __device__ __forceinline__ void load_vertices(const double* __restrict__ Xall, i
         for (int d = 0; d < D; ++d) X[k][d] = p[k * D + d];
 }
 
-// Face map from the left element's face vertices at face quadrature point
-// fq: physical point x, unit normal and sqrt(det(J^T J)).
+// Face map from the left element's face vertices FX[k] = X[fvo[fnl][k]]
+// (k < 2^(D-1)) at face quadrature point fq: physical point x, unit normal
+// and sqrt(det(J^T J)).
 // (HypercubeFace::transform/Jacobian + calc_ortho/normalize/rootRiemannMetric;
 //  1D: PointFace conventions, HypercubeTransformations.hpp:1358-1456.)
 template <int D, int N>
-__device__ __forceinline__ double face_geometry(const DevSpace& S, const double (&X)[1 << D][D], int fnl, int fq,
-                                                double* x, double* nrm) {
+__device__ __forceinline__ double face_geometry_fv(const DevSpace& S, const double (&FX)[1 << (D - 1)][D], int fnl,
+                                                   int fq, double* x, double* nrm) {
     if constexpr (D == 1) {
-        const int v = S.fvo[fnl * 4 + 0];
-        x[0] = X[v][0];
+        x[0] = FX[0][0];
         nrm[0] = (fnl == 0) ? -1.0 : 1.0;
         return 1.0;
     } else {
@@ -140,7 +140,6 @@ __device__ __forceinline__ double face_geometry(const DevSpace& S, const double 
         }
 #pragma unroll
         for (int k = 0; k < NFV; ++k) {
-            const int v = S.fvo[fnl * 4 + k];
             double psi = 1.0, dpsi[TD];
 #pragma unroll
             for (int t = 0; t < TD; ++t) {
@@ -161,9 +160,9 @@ __device__ __forceinline__ double face_geometry(const DevSpace& S, const double 
             }
 #pragma unroll
             for (int i = 0; i < D; ++i) {
-                x[i] += psi * X[v][i];
+                x[i] += psi * FX[k][i];
 #pragma unroll
-                for (int t = 0; t < TD; ++t) Jf[i][t] += dpsi[t] * X[v][i];
+                for (int t = 0; t < TD; ++t) Jf[i][t] += dpsi[t] * FX[k][i];
             }
         }
         double sg;
@@ -191,6 +190,31 @@ __device__ __forceinline__ double face_geometry(const DevSpace& S, const double 
 #pragma unroll
         for (int d = 0; d < D; ++d) nrm[d] /= nn;
         return sg;
+    }
+}
+
+// The same from all the element's vertices X.
+template <int D, int N>
+__device__ __forceinline__ double face_geometry(const DevSpace& S, const double (&X)[1 << D][D], int fnl, int fq,
+                                                double* x, double* nrm) {
+    double FX[1 << (D - 1)][D];
+#pragma unroll
+    for (int k = 0; k < (1 << (D - 1)); ++k)
+#pragma unroll
+        for (int d = 0; d < D; ++d) FX[k][d] = X[S.fvo[fnl * 4 + k]][d];
+    return face_geometry_fv<D, N>(S, FX, fnl, fq, x, nrm);
+}
+
+// Face vertices of element e's face fnl read straight from the vertex table
+// (no element-sized register array indexed at run time).
+template <int D>
+__device__ __forceinline__ void load_face_vertices(const DevSpace& S, int e, int fnl, double (&FX)[1 << (D - 1)][D]) {
+    const double* p = S.X + (size_t)e * (1 << D) * D;
+#pragma unroll
+    for (int k = 0; k < (1 << (D - 1)); ++k) {
+        const int v = S.fvo[fnl * 4 + k];
+#pragma unroll
+        for (int d = 0; d < D; ++d) FX[k][d] = p[v * D + d];
     }
 }
 

@@ -330,6 +330,14 @@ __device__ __forceinline__ void element_point_trace(const DevSpace& S, const VT*
                                                     double bh, VT* __restrict__ out) {
     constexpr int NB = Pow<N, D>::value;
     if constexpr (!GRAD) {
+        // The nodes include the interval ends, so along the face normal only
+        // the node on the face has a nonzero basis value at the face (the
+        // others vanish at -1 exactly and are O(1e-16) at +1, where the
+        // cumulative equispaced nodes end one ulp short of 1): the sum runs
+        // over the N^(D-1) face nodes.
+        constexpr int NFN = NB / N;
+        const int kn = j % D;
+        const int inode = (j < D) ? 0 : N - 1;
         VT a0[D][N];
 #pragma unroll
         for (int d = 0; d < D; ++d)
@@ -338,18 +346,54 @@ __device__ __forceinline__ void element_point_trace(const DevSpace& S, const VT*
         VT u[M];
 #pragma unroll
         for (int v = 0; v < M; ++v) u[v] = 0;
+        // the normal axis as a compile-time constant, so every node offset
+        // but the face's is an immediate
+        auto face_sum = [&](auto KN) {
+            constexpr int K = decltype(KN)::value;
+            constexpr int SK = Pow<N, D - 1 - K>::value;  // node stride along the normal axis
+            const VT en = L[pt[K] * N + inode];
+            const VT* Ub = Ue + (size_t)inode * SK * M;
+            if constexpr (D == 3) {
+                // one tangential index per pass (a fully unrolled face issues
+                // all its loads at once and halves the occupancy)
+                constexpr int TA = (K == 0) ? 1 : 0, TB = (K == 2) ? 1 : 2;
+                constexpr int SA = Pow<N, 2 - TA>::value, SB = Pow<N, 2 - TB>::value;
+#pragma unroll 1
+                for (int ia = 0; ia < N; ++ia) {
+                    const VT ca = en * L[pt[TA] * N + ia];
+                    const VT* Ua = Ub + ia * SA * M;
 #pragma unroll
-        for (int i = 0; i < NB; ++i) {
-            int id[D];
-            int r = i;
+                    for (int ib = 0; ib < N; ++ib) {
+                        const VT c0 = ca * a0[TB][ib];
 #pragma unroll
-            for (int d = D - 1; d >= 0; --d) { id[d] = r % N; r /= N; }
-            VT c0 = 1;
+                        for (int v = 0; v < M; ++v) u[v] += Ua[ib * SB * M + v] * c0;
+                    }
+                }
+                return;
+            }
 #pragma unroll
-            for (int d = 0; d < D; ++d) c0 *= a0[d][id[d]];
+            for (int t = 0; t < NFN; ++t) {
+                // t enumerates the tangential indices in tensor order
+                int node = 0, r = t;
+                VT c0 = en;
 #pragma unroll
-            for (int v = 0; v < M; ++v) u[v] += Ue[i * M + v] * c0;
-        }
+                for (int d = D - 1; d >= 0; --d) {
+                    if (d == K) continue;
+                    const int id = r % N;
+                    r /= N;
+                    int sd = 1;
+#pragma unroll
+                    for (int dd = d + 1; dd < D; ++dd) sd *= N;
+                    node += id * sd;
+                    c0 *= a0[d][id];
+                }
+#pragma unroll
+                for (int v = 0; v < M; ++v) u[v] += Ub[node * M + v] * c0;
+            }
+        };
+        if (kn == 0) face_sum(std::integral_constant<int, 0>{});
+        if constexpr (D > 1) if (kn == 1) face_sum(std::integral_constant<int, (D > 1 ? 1 : 0)>{});
+        if constexpr (D > 2) if (kn == 2) face_sum(std::integral_constant<int, (D > 2 ? 2 : 0)>{});
 #pragma unroll
         for (int v = 0; v < M; ++v) out[v] = u[v];
     } else {
@@ -429,18 +473,19 @@ __global__ __launch_bounds__(TPB, ICE_TRACE_MINW) void trace_kernel(DevSpace S, 
 // Numerical flux (and DDGIC lifting vector) at point q of interior face f
 // from the two trace records of the point (left element tL, right tR), ph
 // the left element's flux parameters.
+// g: the point's unit normal, w sqrt(g) and DDG distance h (a row of the
+// face geometry table, or the same quantities formed from the vertices).
 template <int D, class PH, class UT>
 __device__ __forceinline__ void interior_face_point(const DevSpace& S, const typename PH::Params& ph, const DiscDev& dd,
                                                     int f, int q, int NFQ, const typename PH::real* tL,
                                                     const typename PH::real* tR, const UT* U0L,
-                                                    const UT* U0R) {
+                                                    const UT* U0R, const double* g) {
     using VT = typename PH::real;
     constexpr int M = PH::M;
     constexpr int NV = TraceRec<PH, D>::NV;
     constexpr bool DELTA = std::is_same<VT, float>::value;  // FP32 traces hold u - U_0
     VT* const Qb = reinterpret_cast<VT*>(dd.Q);
     VT* const Pb = reinterpret_cast<VT*>(dd.Pic);
-    const double* g = S.fgeo + ((size_t)f * NFQ + q) * (D + 2);
     VT nrm[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) nrm[d] = (VT)g[d];
@@ -602,8 +647,23 @@ __global__ __launch_bounds__(256, ICE_FLUX_MINW) void flux_kernel(DevSpace S, ty
         const int qr = S.fperm[(fnr * S.norient + orient) * NFQ + q];
         const VT* tL = T + (((size_t)eL * 2 * D + fnl) * NFQ + q) * NV * M;
         const VT* tR = T + (((size_t)eR * 2 * D + fnr) * NFQ + qr) * NV * M;
+        // 3D fluxes without a diffusive part need only the normal and w sqrt(g),
+        // formed here from the left element's face vertices (96 B per face,
+        // shared by its points) rather than read from the table (40 B per point)
+        double geo[D + 2];
+        if constexpr (D == 3 && NV == 1) {
+            double FX[1 << (D - 1)][D], x[D];
+            load_face_vertices<D>(S, eL, fnl, FX);
+            const double sg = face_geometry_fv<D, N>(S, FX, fnl, q, x, geo);
+            geo[D] = face_weight<N, D>(S, q) * sg;
+            geo[D + 1] = 1.0;  // the DDG distance multiplies the (zero) diffusive flux only
+        } else {
+            const double* g = S.fgeo + ((size_t)f * NFQ + q) * (D + 2);
+#pragma unroll
+            for (int k = 0; k < D + 2; ++k) geo[k] = g[k];
+        }
         interior_face_point<D, PH, UT>(S, local_params(ph0, dd.emu, eL), dd, f, q, NFQ, tL, tR, U + (size_t)eL * NBM,
-                                   U + (size_t)eR * NBM);
+                                   U + (size_t)eR * NBM, geo);
     } else if constexpr (BDY) {
         const int tid = (blockIdx.x - nbi) * blockDim.x + threadIdx.x;
         const int b = tid / NFQ, q = tid - b * NFQ;
@@ -1343,9 +1403,10 @@ __global__ __launch_bounds__((ElemTile<2, P, TPB>::BLOCK), ICE_ELEM_MINW) void e
 // Fluxes that use no gradient (Euler) skip A1, B01, B10 and the derivatives.
 // Lifting: the element's face fluxes are staged in LDS in the element's own
 // tensor order of each face's tangential axes (ftens), so the trace basis at
-// a face node factorises: sum_{pa,pb} Q l_sa(x_pa) l_sb(x_pb).  A thread
-// lifts only the faces its node lies on (the trace of every other basis
-// function vanishes there).  All global loads that depend on nothing else
+// a face node factorises: sum_pa l_sa(x_pa) sum_pb Q l_sb(x_pb).  The inner
+// sums replace the face-flux rows in place during phase 1b (one thread per
+// row), and a thread lifts only the faces its node lies on (the trace of
+// every other basis function vanishes there).  All global loads that depend on nothing else
 // (coefficients, vertices, face slots, tables) are issued before the first
 // barrier; the face fluxes, whose addresses come from the face slots, are
 // loaded into registers before phase 1a and stored to LDS after it.
@@ -1378,7 +1439,13 @@ __global__ __launch_bounds__((Elem3Tile<P, TPB>::BLOCK), ICE_ELEM3_MINW) void el
     constexpr int QIT = (EPB * LIFT + BLOCK - 1) / BLOCK;
     using VT = typename PH::real;
     __shared__ VT sbuf[EPB * EBUF];
-    __shared__ VT sQ[EPB * LIFT];
+    // Fluxes without a gradient leave arrays 1 and 2 of the element buffer
+    // unused until phase 1c: the face fluxes are staged there and lifted
+    // before 1c, which keeps the workgroup's LDS to the element buffers
+    // (3D P3 FP64: 4 waves per SIMD instead of 3)
+    constexpr bool QALIAS = !GRAD && LIFT <= 2 * ARR && !PH::HAS_HOMOGENEITY;
+    constexpr int QST = QALIAS ? EBUF : LIFT;  // element stride of the staged fluxes
+    __shared__ VT sQarr[QALIAS ? 1 : EPB * LIFT];
     __shared__ VT sL[2 * NT2];
     __shared__ double sLd[NT2];
     __shared__ double sX[EPB * NV * D];
@@ -1395,6 +1462,7 @@ __global__ __launch_bounds__((Elem3Tile<P, TPB>::BLOCK), ICE_ELEM3_MINW) void el
     const int lec = le < EPB ? le : 0;
     const typename PH::Params ph = local_params(ph0, emu, active ? e : 0);
     VT* B = sbuf + (size_t)lec * EBUF;
+    VT* const sQ = QALIAS ? sbuf + ARR : sQarr;
     // prologue: every global load that depends on nothing else
     {
         const size_t base = (size_t)e0 * NB * M;
@@ -1434,12 +1502,45 @@ __global__ __launch_bounds__((Elem3Tile<P, TPB>::BLOCK), ICE_ELEM3_MINW) void el
         const int fq = w / M, v = w - fq * M;
         const int ef = (idx < EPB * LIFT) ? sEF[l * 2 * D + j] : -1;
         qv[it] = (ef >= 0) ? Q[(size_t)(ef >> 4) * NFQ * M + w] : (VT)0;
-        qdst[it] = (ef >= 0) ? l * LIFT + (j * NFQ + sFT[(j * S.norient + (ef & 7)) * NFQ + fq]) * M + v : -1;
+        qdst[it] = (ef >= 0) ? l * QST + (j * NFQ + sFT[(j * S.norient + (ef & 7)) * NFQ + fq]) * M + v : -1;
     }
     const double* xs = S.tab;
     const double* wgl = S.tab + (N + 2) + 3 * (N + 2) * N;
     const VT* L0 = sL;
     const VT* L1 = sL + NT2;
+    // lifting of the faces this thread's node lies on, added to r
+    const int kd[D] = {k0, k1, k2};
+    auto lift = [&](double (&r)[M]) {
+        unsigned mask = 0;
+#pragma unroll
+        for (int j = 0; j < 2 * D; ++j)
+            if (kd[j % D] == ((j < D) ? 0 : N - 1)) mask |= 1u << j;
+        while (mask) {
+            const int j = __builtin_ctz(mask);
+            mask &= mask - 1;
+            const int ef = sEF[lec * 2 * D + j];
+            if (ef < 0) continue;
+            const int kn = j % D;
+            const int sa = (kn == 0) ? k1 : k0, sb = (kn == 2) ? k1 : k2;
+            // the node's normal-axis basis function at the face's endpoint (S index n or n + 1)
+            const double en = sLd[((j < D) ? N : N + 1) * N + kd[kn]];
+            const double sgn = ((ef >> 3) & 1) ? -en : en;
+            double la[N];
+#pragma unroll
+            for (int p = 0; p < N; ++p) la[p] = sLd[p * N + sa];
+            const VT* Zf = sQ + lec * QST + j * NFQ * M + sb * M;  // rows contracted along pb
+#pragma unroll
+            for (int v = 0; v < M; ++v) {
+                double acc = 0.0;
+#pragma unroll
+                for (int pa = 0; pa < N; ++pa) acc += (double)Zf[pa * N * M + v] * la[pa];
+                r[v] += sgn * acc;
+            }
+        }
+    };
+    double r[M];
+#pragma unroll
+    for (int v = 0; v < M; ++v) r[v] = 0.0;
     VT o0[M], o1[M], o2[M];  // stage outputs held across the barrier
     // 1a: (a = k0, b = k1, q2 = k2)
     if (active) {
@@ -1475,6 +1576,27 @@ __global__ __launch_bounds__((Elem3Tile<P, TPB>::BLOCK), ICE_ELEM3_MINW) void el
         }
     }
     __syncthreads();
+    // face-flux rows contracted along the face's second tangential axis, in
+    // place: sQ[l][j][pa][sb] = sum_pb Q[pa][pb] l_sb(x_pb); the lifting then
+    // sums over pa only
+    for (int tz = threadIdx.x; tz < EPB * 2 * D * N; tz += BLOCK) {
+        const int l = tz / (2 * D * N), rem = tz - l * (2 * D * N);
+        VT* Qr = sQ + l * QST + rem * N * M;  // rem = j * N + pa
+        VT row[N][M];
+#pragma unroll
+        for (int pb = 0; pb < N; ++pb)
+#pragma unroll
+            for (int v = 0; v < M; ++v) row[pb][v] = Qr[pb * M + v];
+#pragma unroll
+        for (int sb = 0; sb < N; ++sb)
+#pragma unroll
+            for (int v = 0; v < M; ++v) {
+                double z = 0.0;
+#pragma unroll
+                for (int pb = 0; pb < N; ++pb) z += (double)row[pb][v] * sLd[pb * N + sb];
+                Qr[sb * M + v] = (VT)z;
+            }
+    }
     // 1b: (a = k0, q1 = k1, q2 = k2)
     if (active) {
         VT l0[N], l1[N];
@@ -1503,6 +1625,7 @@ __global__ __launch_bounds__((Elem3Tile<P, TPB>::BLOCK), ICE_ELEM3_MINW) void el
     }
     __syncthreads();
     if (active) {
+        if constexpr (QALIAS) lift(r);  // the staged fluxes are overwritten in 1c
 #pragma unroll
         for (int v = 0; v < M; ++v) {
             B[v * NBP + k] = o0[v];
@@ -1672,7 +1795,6 @@ __global__ __launch_bounds__((Elem3Tile<P, TPB>::BLOCK), ICE_ELEM3_MINW) void el
     __syncthreads();
     if (!active) return;
     // 2c: (i0 = k0, i1 = k1, i2 = k2)
-    double r[M];
     {
         VT l0[N], l1[N];
 #pragma unroll
@@ -1686,7 +1808,7 @@ __global__ __launch_bounds__((Elem3Tile<P, TPB>::BLOCK), ICE_ELEM3_MINW) void el
             VT s = 0;
 #pragma unroll
             for (int q = 0; q < N; ++q) s += E[v * NBP + q * NN] * l1[q] + E[ARR + v * NBP + q * NN] * l0[q];
-            r[v] = (double)s;
+            r[v] += (double)s;
         }
         if (srcload) {
             const double* sl = srcload + ((size_t)e * NB + k) * M;
@@ -1694,42 +1816,7 @@ __global__ __launch_bounds__((Elem3Tile<P, TPB>::BLOCK), ICE_ELEM3_MINW) void el
             for (int v = 0; v < M; ++v) r[v] -= sl[v];
         }
     }
-    // lifting of the faces this thread's node lies on
-    const int kd[D] = {k0, k1, k2};
-    unsigned mask = 0;
-#pragma unroll
-    for (int j = 0; j < 2 * D; ++j)
-        if (kd[j % D] == ((j < D) ? 0 : N - 1)) mask |= 1u << j;
-    while (mask) {
-        const int j = __builtin_ctz(mask);
-        mask &= mask - 1;
-        const int ef = sEF[lec * 2 * D + j];
-        if (ef < 0) continue;
-        const int kn = j % D;
-        const int sa = (kn == 0) ? k1 : k0, sb = (kn == 2) ? k1 : k2;
-        // the node's normal-axis basis function at the face's endpoint (S index n or n + 1)
-        const double en = sLd[((j < D) ? N : N + 1) * N + kd[kn]];
-        const double sgn = ((ef >> 3) & 1) ? -en : en;
-        double la[N], lb[N];
-#pragma unroll
-        for (int p = 0; p < N; ++p) {
-            la[p] = sLd[p * N + sa];
-            lb[p] = sLd[p * N + sb];
-        }
-        const VT* Qf = sQ + lec * LIFT + j * NFQ * M;
-#pragma unroll
-        for (int v = 0; v < M; ++v) {
-            double acc = 0.0;
-#pragma unroll
-            for (int pa = 0; pa < N; ++pa) {
-                double t = 0.0;
-#pragma unroll
-                for (int pb = 0; pb < N; ++pb) t += (double)Qf[(pa * N + pb) * M + v] * lb[pb];
-                acc += t * la[pa];
-            }
-            r[v] += sgn * acc;
-        }
-    }
+    if constexpr (!QALIAS) lift(r);
     if constexpr (PH::HAS_HOMOGENEITY) {
         if (Pic) {
             const double* Ld = S.tab + (N + 2);
