@@ -397,6 +397,31 @@ __device__ __forceinline__ double face_weight(const DevSpace& S, int fq) {
 }
 
 
+// DDGIC lifting vector at one face point (the interface correction of
+// conservation_law.hpp:580-610, Dirichlet faces :750-780):
+// P[v][s] = sigma sum_k sum_r G(uavg)[v][k][r][s] n_k jmp_r w sqrt(g); each
+// element adjacent to the point subtracts P . grad phi_i there.
+template <int D, class PH>
+__device__ __forceinline__ void ddgic_lift(const typename PH::Params& ph, typename PH::real sigma,
+                                           const typename PH::real* uavg, const typename PH::real* nrm,
+                                           const typename PH::real* jmp, typename PH::real ws, typename PH::real* P) {
+    constexpr int M = PH::M;
+    using R = typename PH::real;
+    R G[M * D * M * D];
+    PH::homogeneity(ph, uavg, G);
+#pragma unroll
+    for (int v = 0; v < M; ++v)
+#pragma unroll
+        for (int s = 0; s < D; ++s) {
+            R acc = 0;
+#pragma unroll
+            for (int k = 0; k < D; ++k)
+#pragma unroll
+                for (int r = 0; r < M; ++r) acc += G[((v * D + k) * M + r) * D + s] * nrm[k] * jmp[r];
+            P[v * D + s] = sigma * acc * ws;
+        }
+}
+
 // DDG single-valued gradient and weighted normal flux at one interior face
 // point (conservation_law.hpp:502-575): Q = (F_visc - F_conv) w sqrt(g).
 template <int D, class PH>

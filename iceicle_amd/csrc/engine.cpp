@@ -228,7 +228,7 @@ int disc_dims(const ice_disc* d, int* nelem, int* nslots, int* bs) {
     *nslots = 2 * s->ndim + 1;
     *bs = s->nb * d->neq;
     // the FD Jacobian kernels use the discretization's viscosity, not per-element ones
-    return ((s->ndim > 2 && s->p > 2) || d->ddgic || d->d_emu) ? -1 : 0;
+    return ((s->ndim > 2 && s->p > 2) || d->d_emu) ? -1 : 0;
 }
 
 const int* disc_jcols(const ice_disc* d) { return d->space->dev.jcols; }
@@ -1052,7 +1052,6 @@ int ice_jacobian(ice_disc* d, const double* u, double* r, double* blocks, void* 
     const ice_space* s = d->space;
     if (s->ndim > 2 && s->p > 2)
         return fail(ICE_ERR_UNSUPPORTED, "the device Jacobian covers P1 and P2 in 3D (per-face tables in LDS)");
-    if (d->ddgic) return fail(ICE_ERR_UNSUPPORTED, "the device Jacobian does not cover the DDGIC term (sigma_ic != 0)");
     hipStream_t st = (hipStream_t)stream;
     if (r) {
         const int rc = ice::residual_fp64(s->ndim, s->p, d->prm.flux, s->dev, d->ph, d->dd, u, r, st, nullptr);

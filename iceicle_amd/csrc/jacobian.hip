@@ -149,6 +149,13 @@ __global__ __launch_bounds__(256) void jac_face_kernel(DevSpace S, PhysParams ph
     __shared__ double bphi[2][NFQ][NB], bgx[2][NFQ][NB][D], bhn[2][NFQ][NB][D];
     __shared__ double sQ0[NFQ][M];
     __shared__ double sdQ[NC][NFQ][M];
+    // DDGIC lifting vectors (fluxes with a homogeneity tensor): base and
+    // per-column differences, lifted with the basis gradients of each side
+    constexpr bool IC = PH::HAS_HOMOGENEITY;
+    constexpr int MD = M * D;
+    __shared__ double sP0[IC ? NFQ : 1][IC ? MD : 1];
+    __shared__ double sdP[IC ? NC : 1][IC ? NFQ : 1][IC ? MD : 1];
+    const bool ddgic = IC && ph.sigma_face != 0.0;
     __shared__ double sred[256];
     __shared__ double seps;
     const int f = S.jorder[base + blockIdx.x];
@@ -211,14 +218,36 @@ __global__ __launch_bounds__(256) void jac_face_kernel(DevSpace S, PhysParams ph
                                    snrm[tid], sh[tid], sws[tid], Q);
 #pragma unroll
         for (int v = 0; v < M; ++v) sQ0[tid][v] = Q[v];
+        if constexpr (IC) {
+            if (ddgic) {
+                double uavg[M], jmp[M], Pv[MD];
+#pragma unroll
+                for (int v = 0; v < M; ++v) {
+                    uavg[v] = 0.5 * (su[0][tid][v] + su[1][tid][v]);
+                    jmp[v] = su[1][tid][v] - su[0][tid][v];
+                }
+                ddgic_lift<D, PH>(ph, ph.sigma_face, uavg, snrm[tid], jmp, sws[tid], Pv);
+#pragma unroll
+                for (int k = 0; k < MD; ++k) sP0[tid][k] = Pv[k];
+            }
+        }
     }
     __syncthreads();
-    // eps = max(sqrt eps, sqrt eps * max(|resL|, |resR|)) (form_petsc_jacobian.hpp:191)
+    // eps = max(sqrt eps, sqrt eps * max(|resL|, |resR|)) (form_petsc_jacobian.hpp:191),
+    // the face's residual on either side: +-Q lifted with the trace basis,
+    // minus the DDGIC vectors against the side's basis gradients
     double sL = 0, sR = 0;
     for (int r = tid; r < NC; r += blockDim.x) {
         const int side = r / NBM, iv = r % NBM, i = iv / M, v = iv % M;
         double acc = 0;
         for (int q = 0; q < NFQ; ++q) acc += sQ0[q][v] * bphi[side][q][i];
+        if (side) acc = -acc;
+        if constexpr (IC) {
+            if (ddgic)
+                for (int q = 0; q < NFQ; ++q)
+#pragma unroll
+                    for (int d = 0; d < D; ++d) acc -= sP0[q][v * D + d] * bgx[side][q][i][d];
+        }
         if (side) sR += acc * acc;
         else sL += acc * acc;
     }
@@ -280,6 +309,19 @@ __global__ __launch_bounds__(256) void jac_face_kernel(DevSpace S, PhysParams ph
         interior_point_flux<D, PH>(ph, uL, gL, hL, uR, gR, hR, snrm[q], sh[q], sws[q], Q);
 #pragma unroll
         for (int v = 0; v < M; ++v) sdQ[c][q][v] = Q[v] - sQ0[q][v];
+        if constexpr (IC) {
+            if (ddgic) {
+                double uavg[M], jmp[M], Pv[MD];
+#pragma unroll
+                for (int v = 0; v < M; ++v) {
+                    uavg[v] = 0.5 * (uL[v] + uR[v]);
+                    jmp[v] = uR[v] - uL[v];
+                }
+                ddgic_lift<D, PH>(ph, ph.sigma_face, uavg, snrm[q], jmp, sws[q], Pv);
+#pragma unroll
+                for (int k = 0; k < MD; ++k) sdP[c][q][k] = Pv[k] - sP0[q][k];
+            }
+        }
     }
     __syncthreads();
     // lift: row (side rs, i, v), column c; the left rows first, then the
@@ -313,7 +355,15 @@ __global__ __launch_bounds__(256) void jac_face_kernel(DevSpace S, PhysParams ph
                 double acc = 0;
 #pragma unroll
                 for (int q = 0; q < NFQ; ++q) acc += sdQ[c][q][v] * bphi[rs][q][i];
-                val[k] = (rs ? -acc : acc) * ieps;
+                if (rs) acc = -acc;
+                if constexpr (IC) {
+                    if (ddgic)
+#pragma unroll
+                        for (int q = 0; q < NFQ; ++q)
+#pragma unroll
+                            for (int d = 0; d < D; ++d) acc -= sdP[c][q][v * D + d] * bgx[rs][q][i][d];
+                }
+                val[k] = acc * ieps;
                 diag[k] = rs == cs;
                 off[k] = iv * (int)bs + cw;
                 old[k] = (ok[k] && diag[k]) ? Jd[off[k]] : 0.0;
@@ -346,6 +396,10 @@ __global__ __launch_bounds__(256) void jac_bface_kernel(DevSpace S, PhysParams p
     __shared__ double bphi[NFQ][NB], bgx[NFQ][NB][D];
     __shared__ double sQ0[NFQ][M];
     __shared__ double sdQ[NBM][NFQ][M];
+    constexpr bool IC = PH::HAS_HOMOGENEITY;
+    constexpr int MD = M * D;
+    __shared__ double sP0[IC ? NFQ : 1][IC ? MD : 1];
+    __shared__ double sdP[IC ? NBM : 1][IC ? NFQ : 1][IC ? MD : 1];
     __shared__ double sred[256];
     __shared__ double seps;
     const int b = S.jorder[S.nfi + base + blockIdx.x];
@@ -355,6 +409,8 @@ __global__ __launch_bounds__(256) void jac_bface_kernel(DevSpace S, PhysParams p
     const bool general = !(bctype == ICE_BC_DIRICHLET || bctype == ICE_BC_NEUMANN || bctype == ICE_BC_EXTRAPOLATION ||
                            bctype == ICE_BC_SPACETIME_FUTURE);
     if (bctype == ICE_BC_NEUMANN || (general && !PH::IMPLEMENTS_BCS)) return;  // no dependence on u
+    // the DDGIC Dirichlet-face term (the source form's sigma_bdy)
+    const bool ddgic = IC && ph.sigma_bdy != 0.0 && bctype == ICE_BC_DIRICHLET;
     const int tid = threadIdx.x;
     if (tid < NFQ) {
         const int q = tid;
@@ -384,11 +440,24 @@ __global__ __launch_bounds__(256) void jac_bface_kernel(DevSpace S, PhysParams p
         for (int d = 0; d < D; ++d) snrm[q][d] = nrm[d];
         sh[q] = ddg_h_boundary<D>(nrm, x, cL);
         sws[q] = face_weight<N, D>(S, q) * sqrtg;
-        double Q[M];
+        double Q[M], uB[M];
         boundary_point_flux<D, PH>(ph, bctype, bcflag, u, g, bval + ((size_t)b * NFQ + q) * M, nrm, sh[q], sws[q], Q,
-                                   nullptr);
+                                   uB);
 #pragma unroll
         for (int v = 0; v < M; ++v) sQ0[q][v] = Q[v];
+        if constexpr (IC) {
+            if (ddgic) {
+                double uavg[M], jmp[M], Pv[MD];
+#pragma unroll
+                for (int v = 0; v < M; ++v) {
+                    uavg[v] = 0.5 * (u[v] + uB[v]);
+                    jmp[v] = uB[v] - u[v];
+                }
+                ddgic_lift<D, PH>(ph, ph.sigma_bdy, uavg, nrm, jmp, sws[q], Pv);
+#pragma unroll
+                for (int k = 0; k < MD; ++k) sP0[q][k] = Pv[k];
+            }
+        }
     }
     __syncthreads();
     for (int t = tid; t < NFQ * NB; t += blockDim.x) {
@@ -406,6 +475,12 @@ __global__ __launch_bounds__(256) void jac_bface_kernel(DevSpace S, PhysParams p
         const int i = r / M, v = r % M;
         double acc = 0;
         for (int q = 0; q < NFQ; ++q) acc += sQ0[q][v] * bphi[q][i];
+        if constexpr (IC) {
+            if (ddgic)
+                for (int q = 0; q < NFQ; ++q)
+#pragma unroll
+                    for (int d = 0; d < D; ++d) acc -= sP0[q][v * D + d] * bgx[q][i][d];
+        }
         sL += acc * acc;
     }
     sred[tid] = sL;
@@ -428,11 +503,24 @@ __global__ __launch_bounds__(256) void jac_bface_kernel(DevSpace S, PhysParams p
         u[w] += eps * bphi[q][i];
 #pragma unroll
         for (int d = 0; d < D; ++d) g[w * D + d] += eps * bgx[q][i][d];
-        double Q[M];
+        double Q[M], uB[M];
         boundary_point_flux<D, PH>(ph, bctype, bcflag, u, g, bval + ((size_t)b * NFQ + q) * M, snrm[q], sh[q], sws[q],
-                                   Q, nullptr);
+                                   Q, uB);
 #pragma unroll
         for (int v = 0; v < M; ++v) sdQ[c][q][v] = Q[v] - sQ0[q][v];
+        if constexpr (IC) {
+            if (ddgic) {
+                double uavg[M], jmp[M], Pv[MD];
+#pragma unroll
+                for (int v = 0; v < M; ++v) {
+                    uavg[v] = 0.5 * (u[v] + uB[v]);
+                    jmp[v] = uB[v] - u[v];
+                }
+                ddgic_lift<D, PH>(ph, ph.sigma_bdy, uavg, snrm[q], jmp, sws[q], Pv);
+#pragma unroll
+                for (int k = 0; k < MD; ++k) sdP[c][q][k] = Pv[k] - sP0[q][k];
+            }
+        }
     }
     __syncthreads();
     const size_t bs = NBM;
@@ -443,6 +531,13 @@ __global__ __launch_bounds__(256) void jac_bface_kernel(DevSpace S, PhysParams p
         double acc = 0;
 #pragma unroll
         for (int q = 0; q < NFQ; ++q) acc += sdQ[c][q][v] * bphi[q][i];
+        if constexpr (IC) {
+            if (ddgic)
+#pragma unroll
+                for (int q = 0; q < NFQ; ++q)
+#pragma unroll
+                    for (int d = 0; d < D; ++d) acc -= sdP[c][q][v * D + d] * bgx[q][i][d];
+        }
         J[((size_t)eL * bslots) * bs * bs + (size_t)r * bs + c] += acc * ieps;
     }
 }

@@ -520,23 +520,10 @@ __device__ __forceinline__ void interior_face_point(const DevSpace& S, const typ
     for (int v = 0; v < M; ++v) Qf[v] = Qv[v];
     if constexpr (PH::HAS_HOMOGENEITY) {
         if (dd.Pic) {
-            VT uavg[M], G[M * D * M * D];
+            VT uavg[M];
 #pragma unroll
             for (int v = 0; v < M; ++v) uavg[v] = VT(0.5) * (uL[v] + uR[v]);
-            PH::homogeneity(ph, uavg, G);
-            VT* Pf = Pb + ((size_t)f * NFQ + q) * M * D;
-#pragma unroll
-            for (int v = 0; v < M; ++v)
-#pragma unroll
-                for (int s = 0; s < D; ++s) {
-                    VT acc = 0;
-#pragma unroll
-                    for (int k = 0; k < D; ++k)
-#pragma unroll
-                        for (int r = 0; r < M; ++r)
-                            acc += G[((v * D + k) * M + r) * D + s] * nrm[k] * jmp[r];
-                    Pf[v * D + s] = ph.sigma_face * acc * ws;
-                }
+            ddgic_lift<D, PH>(ph, ph.sigma_face, uavg, nrm, jmp, ws, Pb + ((size_t)f * NFQ + q) * M * D);
         }
     }
 }
@@ -603,22 +590,13 @@ __device__ __forceinline__ void boundary_face_point(const DevSpace& S, const typ
     for (int v = 0; v < M; ++v) Qf[v] = Qv[v];
     if constexpr (PH::HAS_HOMOGENEITY) {
         if (Pf && bctype == ICE_BC_DIRICHLET) {
-            VT uavg[M], G[M * D * M * D];
+            VT uavg[M], jb[M];
 #pragma unroll
-            for (int v = 0; v < M; ++v) uavg[v] = VT(0.5) * (uL[v] + uB[v]);
-            PH::homogeneity(ph, uavg, G);
-#pragma unroll
-            for (int v = 0; v < M; ++v)
-#pragma unroll
-                for (int s = 0; s < D; ++s) {
-                    VT acc = 0;
-#pragma unroll
-                    for (int k = 0; k < D; ++k)
-#pragma unroll
-                        for (int r = 0; r < M; ++r)
-                            acc += G[((v * D + k) * M + r) * D + s] * nrm[k] * (DELTA ? jmpD[r] : uB[r] - uL[r]);
-                    Pf[v * D + s] = ph.sigma_bdy * acc * ws;
-                }
+            for (int v = 0; v < M; ++v) {
+                uavg[v] = VT(0.5) * (uL[v] + uB[v]);
+                jb[v] = DELTA ? jmpD[v] : uB[v] - uL[v];
+            }
+            ddgic_lift<D, PH>(ph, ph.sigma_bdy, uavg, nrm, jb, ws, Pf);
         }
     }
 }

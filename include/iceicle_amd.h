@@ -353,8 +353,10 @@ int ice_jacobian_layout(const ice_disc* disc, int* nslots, int* block_size, int*
  * (FP64), and the residual into r_dev unless it is NULL.  The face blocks
  * use the reference's eps = max(sqrt(eps_mach), sqrt(eps_mach) * max over the
  * face's element residual norms), the element blocks the per-quadrature-point
- * eps of domain_integral_jacobian.  ndim <= 2 and sigma_ic = 0 only
- * (ICE_ERR_UNSUPPORTED otherwise). */
+ * eps of domain_integral_jacobian; with sigma_ic != 0 the face blocks
+ * include the DDGIC lifting (both forms).  1D and 2D at P1-P4, 3D at P1 and
+ * P2 (ICE_ERR_UNSUPPORTED above P2 in 3D: the face kernel keeps its
+ * per-column flux differences in LDS). */
 int ice_jacobian(ice_disc* disc, const double* u_dev, double* r_dev, double* blocks_dev, void* stream);
 /* y = J x for blocks in the layout above (FP64 blocks, FP64 vectors). */
 int ice_jacobian_apply(const ice_disc* disc, const double* blocks_dev, const double* x_dev, double* y_dev, void* stream);

@@ -192,15 +192,32 @@ def test_burgers_3d_coupled_periodic():
     _compare(m, 2, p, u)
 
 
+@pytest.mark.parametrize("form", [abi.DDGIC_SOURCE, abi.DDGIC_PUBLISHED])
+@pytest.mark.parametrize("order", [1, 3])
+def test_burgers_2d_ddgic(order, form):
+    """DDGIC lifting in the face blocks: interior faces (sigma on each side or
+    half of it) and, in the source form, the Dirichlet-face term; Dirichlet,
+    Neumann and extrapolation faces on a perturbed mesh."""
+    p = cases.burgers_params(2, mu=0.05, a=(0.4, -0.3, 0), sigma_ic=0.5)
+    p.ddgic_form = form
+    bct = ["DIRICHLET", "NEUMANN", "DIRICHLET", "EXTRAPOLATION"]
+    m, keep = _mesh(2, [3, 4], bct, 0.15, 11 + order, bcflag=[0, 1, 1, 0])
+    u = cases.random_state(p.flux, 2, 12, (order + 1) ** 2, 21 + order)
+    _compare(m, order, p, u)
+
+
+def test_burgers_ddgic_1d_and_3d():
+    p1 = cases.burgers_params(1, mu=0.03, sigma_ic=0.5)
+    m1, keep1 = _mesh(1, [5], ["DIRICHLET", "NEUMANN"], 0.2, 4, bcflag=[1, 1])
+    _compare(m1, 2, p1, cases.random_state(p1.flux, 1, 5, 3, 23))
+    p3 = cases.burgers_params(3, mu=0.02, a=(0.3, -0.2, 0.1), sigma_ic=0.5)
+    m3, keep3 = _mesh(3, [2, 2, 3], ["DIRICHLET", "PERIODIC", "NEUMANN", "DIRICHLET", "PERIODIC", "EXTRAPOLATION"],
+                      0.0, 0, bcflag=[0, 0, 1, 1, 0, 0], coupled=True)
+    _compare(m3, 2, p3, cases.random_state(p3.flux, 3, 12, 27, 25))
+
+
 def test_unsupported_cases_raise():
     from iceicle_amd import engine
-    p = cases.burgers_params(2, sigma_ic=0.5)
-    m, keep = _mesh(2, [3, 3], ["DIRICHLET"] * 4, 0.0, 0)
-    disc = engine.ConservationLawDDG(engine.FESpace(m, 1), p)
-    jac = engine.BlockJacobian(disc)
-    u = torch.zeros(disc.size, dtype=torch.float64, device="cuda")
-    with pytest.raises(engine.IceError, match="DDGIC"):
-        jac.assemble(u)
     m3, keep3 = _mesh(3, [2, 2, 2], ["DIRICHLET"] * 6, 0.0, 0)
     d3 = engine.ConservationLawDDG(engine.FESpace(m3, 3), cases.burgers_params(3))
     with pytest.raises(engine.IceError, match="P1 and P2 in 3D"):
