@@ -31,7 +31,18 @@ PUBLISHED_DDGIC = {
 }
 
 
-def sine_diffusion_l2(nx, p, sigma=0.0, ddgic_form=abi.DDGIC_SOURCE):
+# doc/sine_diffusion_1d_study.md:57-66 (Interior Penalty, --interior_penalty:
+# the DDG flux without its Hessian term, beta_1 = 0), rows nx = 2..16
+PUBLISHED_IP = {
+    2: [0.262893571, 0.0475606773, 0.0290106132, 0.000952407674],
+    4: [0.0636081844, 0.0175930309, 0.00140523714, 0.000111911005],
+    8: [0.0165579431, 0.00390484158, 8.63017364e-05, 6.16326363e-06],
+    16: [0.00417741741, 0.000942627192, 5.38785297e-06, 3.73127053e-07],
+}
+IP_TOL = {1: 4e-7, 2: 1e-6, 3: 1e-6, 4: 1e-4}
+
+
+def sine_diffusion_l2(nx, p, sigma=0.0, ddgic_form=abi.DDGIC_SOURCE, interior_penalty=False):
     """sine_diffusion_1d.cpp:23-184: heat equation through Burgers (mu = 1),
     Dirichlet 0 on [0, 2 pi], IC sin x (L2 projection), RK3-TVD with
     dt = 1e-4 (2 pi / nx)^2 to t = 1, L2 error against exp(-1) sin x."""
@@ -40,6 +51,7 @@ def sine_diffusion_l2(nx, p, sigma=0.0, ddgic_form=abi.DDGIC_SOURCE):
     prm.mu = 1.0
     prm.sigma_ic = sigma
     prm.ddgic_form = ddgic_form
+    prm.interior_penalty = int(interior_penalty)
     prm.dirichlet[0] = abi.Field.make("CONSTANT", [0.0])
     o = O.Oracle(m, abi.space_desc(p), prm)
     u = o.project(abi.Field.make("SINE", [1.0, 1.0, 0.0]))
@@ -53,6 +65,19 @@ def test_published_sine_diffusion_table(nx, p):
     err = sine_diffusion_l2(nx, p)
     ref = PUBLISHED_DDG[nx][p - 1]
     assert abs(err - ref) / ref <= PIN_TOL[p], (nx, p, err, ref)
+
+
+@pytest.mark.parametrize("nx", [2, 4, 8, 16])
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+def test_published_interior_penalty_table(nx, p):
+    """The interior-penalty column (beta_1 = 0: no Hessian term in the DDG
+    gradient; P1 coincides with standard DDG) to the same digits as the
+    standard-DDG column."""
+    err = sine_diffusion_l2(nx, p, interior_penalty=True)
+    ref = PUBLISHED_IP[nx][p - 1]
+    assert abs(err - ref) / ref <= IP_TOL[p], (nx, p, err, ref)
+    if p >= 2 and nx >= 8:
+        assert abs(err - PUBLISHED_DDG[nx][p - 1]) / ref > 1e-2  # the Hessian term matters
 
 
 @pytest.mark.parametrize("nx", [2, 4, 8, 16])
