@@ -166,6 +166,173 @@ __global__ __launch_bounds__(TPB) void minv_kernel(DevSpace S, const double* __r
     }
 }
 
+// The same in 3D with one tensor line of N nodes per thread: the passes run
+// along axes 2, 1, 0 (B^-T), scale, then 0, 1, 2 (B^-1), so the two passes
+// meet on axis 0 in one thread's registers and four LDS transposes (in
+// place: a thread rewrites the entries of its own line) replace the six
+// LDS round trips of the node-per-thread kernel; the residual is read and
+// the result written as contiguous lines of N nodes.
+template <int P, int M, class KT = double>
+struct MinvLines {
+    static constexpr int N = P + 1, NN = N * N, NB = NN * N;
+    static constexpr int PL = NN + (N == 4 ? 4 : 1);  // plane stride (N = 4: conflict-free transposes)
+    static constexpr int VS = N * PL;
+    static constexpr int EBUF = M * VS;
+    static constexpr int EPB = 128 / NN;
+    static constexpr int BLOCK = ((EPB * NN + 63) / 64) * 64;
+};
+
+template <int P, int M, class KT = double>
+__global__ __launch_bounds__((MinvLines<P, M, KT>::BLOCK)) void minv_lines_kernel(DevSpace S,
+                                                                                  const double* __restrict__ r,
+                                                                                  KT* __restrict__ k) {
+    constexpr int D = 3;
+    using LL = MinvLines<P, M, KT>;
+    constexpr int N = LL::N, NN = LL::NN, NB = LL::NB, PL = LL::PL, VS = LL::VS, EBUF = LL::EBUF, EPB = LL::EPB;
+    constexpr int NV = 1 << D;
+    __shared__ double buf[EPB * EBUF];
+    __shared__ double sX[EPB * NV * D];
+    __shared__ double sB[N * N];
+    __shared__ int sing[EPB];
+    const int e0 = blockIdx.x * EPB;
+    const int le = threadIdx.x / NN;
+    const int t = threadIdx.x - le * NN;
+    const int a = t / N, b = t - (t / N) * N;  // the line's two fixed indices
+    const int e = e0 + le;
+    const bool active = threadIdx.x < EPB * NN && e < S.nelem;
+    const int lec = le < EPB ? le : 0;
+    double* const B = buf + (size_t)lec * EBUF;
+    const double* xs = S.tab;
+    const double* wgl = S.tab + (N + 2) + 3 * (N + 2) * N;
+    double y[N][M];
+    {
+        const size_t xb = (size_t)e0 * NV * D, xt = (size_t)S.nelem * NV * D;
+        for (int idx = threadIdx.x; idx < EPB * NV * D; idx += blockDim.x)
+            sX[idx] = (xb + idx < xt) ? S.X[xb + idx] : 0.0;
+        for (int idx = threadIdx.x; idx < N * N; idx += blockDim.x) sB[idx] = wgl[N + idx];  // Binv[i][q]
+        if (threadIdx.x < EPB) sing[threadIdx.x] = 0;
+        // the line (a, b, c), c < N, of the residual
+        const double* rl = r + ((size_t)(active ? e : 0) * NB + a * NN + b * N) * M;
+        double in[N][M];
+#pragma unroll
+        for (int c = 0; c < N; ++c)
+#pragma unroll
+            for (int v = 0; v < M; ++v) in[c][v] = active ? rl[c * M + v] : 0.0;
+        __syncthreads();
+        // B^-T along axis 2: y[q] = sum_i Binv[i][q] in[i]
+#pragma unroll
+        for (int q = 0; q < N; ++q)
+#pragma unroll
+            for (int v = 0; v < M; ++v) {
+                double acc = 0;
+#pragma unroll
+                for (int i = 0; i < N; ++i) acc += sB[i * N + q] * in[i][v];
+                y[q][v] = acc;
+            }
+    }
+    if (active) {
+#pragma unroll
+        for (int v = 0; v < M; ++v)
+#pragma unroll
+            for (int q = 0; q < N; ++q) B[v * VS + a * PL + b * N + q] = y[q][v];
+    }
+    __syncthreads();
+    // B^-T along axis 1: line (a, ., b)
+    if (active) {
+        double in[N][M];
+#pragma unroll
+        for (int v = 0; v < M; ++v)
+#pragma unroll
+            for (int i = 0; i < N; ++i) in[i][v] = B[v * VS + a * PL + i * N + b];
+#pragma unroll
+        for (int v = 0; v < M; ++v)
+#pragma unroll
+            for (int q = 0; q < N; ++q) {
+                double acc = 0;
+#pragma unroll
+                for (int i = 0; i < N; ++i) acc += sB[i * N + q] * in[i][v];
+                B[v * VS + a * PL + q * N + b] = acc;
+            }
+    }
+    __syncthreads();
+    // B^-T along axis 0: line (., a, b); scale by 1 / (w_q detJ_q); B^-1 along
+    // axis 0 in the same registers
+    if (active) {
+        double in[N][M];
+#pragma unroll
+        for (int v = 0; v < M; ++v)
+#pragma unroll
+            for (int i = 0; i < N; ++i) in[i][v] = B[v * VS + i * PL + a * N + b];
+        const double* Xe = sX + lec * NV * D;
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+            const double xi[D] = {xs[q], xs[a], xs[b]};
+            const double det = q1_det<D>(Xe, xi);
+            if (det == 0.0) sing[lec] = 1;
+            const double sc = 1.0 / (wgl[q] * wgl[a] * wgl[b] * det);
+#pragma unroll
+            for (int v = 0; v < M; ++v) {
+                double acc = 0;
+#pragma unroll
+                for (int i = 0; i < N; ++i) acc += sB[i * N + q] * in[i][v];
+                y[q][v] = acc * sc;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < M; ++v)
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                double acc = 0;
+#pragma unroll
+                for (int q = 0; q < N; ++q) acc += sB[i * N + q] * y[q][v];
+                B[v * VS + i * PL + a * N + b] = acc;
+            }
+    }
+    __syncthreads();
+    // B^-1 along axis 1: line (a, ., b)
+    if (active) {
+        double in[N][M];
+#pragma unroll
+        for (int v = 0; v < M; ++v)
+#pragma unroll
+            for (int q = 0; q < N; ++q) in[q][v] = B[v * VS + a * PL + q * N + b];
+#pragma unroll
+        for (int v = 0; v < M; ++v)
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                double acc = 0;
+#pragma unroll
+                for (int q = 0; q < N; ++q) acc += sB[i * N + q] * in[q][v];
+                B[v * VS + a * PL + i * N + b] = acc;
+            }
+    }
+    __syncthreads();
+    if (!active) return;
+    // B^-1 along axis 2: line (a, b, .), written out as a line of the result
+    double in[N][M];
+#pragma unroll
+    for (int v = 0; v < M; ++v)
+#pragma unroll
+        for (int q = 0; q < N; ++q) in[q][v] = B[v * VS + a * PL + b * N + q];
+    const size_t o = ((size_t)e * NB + a * NN + b * N) * M;
+    if (sing[lec]) {
+#pragma unroll
+        for (int c = 0; c < N; ++c)
+#pragma unroll
+            for (int v = 0; v < M; ++v) k[o + c * M + v] = (KT)(1e-8 * r[o + c * M + v]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int v = 0; v < M; ++v) {
+                double acc = 0;
+#pragma unroll
+                for (int q = 0; q < N; ++q) acc += sB[i * N + q] * in[q][v];
+                k[o + i * M + v] = (KT)acc;
+            }
+    }
+}
+
 // y += coef * (*dt) * x   (axpy, fespan.hpp:283-300, with the step size read on the device)
 __global__ void axpy_dt_kernel(size_t n, double coef, const double* __restrict__ dtp, const double* __restrict__ x,
                                double* __restrict__ y) {
@@ -248,6 +415,11 @@ __global__ void step_end_kernel(StepState* st) {
 
 template <int D, int P, int M, class KT>
 int launch_minv_t(const DevSpace& S, const double* r, KT* k, hipStream_t st) {
+    if constexpr (D == 3 && P >= 2) {
+        using LL = MinvLines<P, M, KT>;
+        minv_lines_kernel<P, M, KT><<<(S.nelem + LL::EPB - 1) / LL::EPB, LL::BLOCK, 0, st>>>(S, r, k);
+        return (int)hipGetLastError();
+    }
     constexpr int NB = PowS<P + 1, D>::value;
     // workgroup: ICE_MINV_TPB threads, or whole waves for one element when
     // an element has more nodes
