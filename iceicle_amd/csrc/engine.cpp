@@ -1217,22 +1217,22 @@ int step_once(ice_stepper* S, double* u, hipStream_t st) {
             axpy(1.0 / 6.0, k1, u);
             axpy(2.0 / 3.0, k1, u);
         } else {
-            // ssp_rk3.hpp:164-190; each stage combination in one pass:
+            // ssp_rk3.hpp:164-190; each stage's increment and combination
             // u1 = u + dt k1, u2 = u + dt (k1 + k2)/4, u = u + dt (k1/6 + k2/6 + 2 k3/3)
-            auto combine = [&](int nk, const double* const* kk, const double* c, double* out) {
-                rc |= ice::launch_rk_combine(n, u, nk, kk, c, dt, S->dte, per, S->st, out, st);
+            // in one pass over the state (3D) or two
+            auto minv_combine = [&](double* k, int nprev, const double* const* kk, const double* c, double* out,
+                                    bool store_k) {
+                rc |= ice::launch_minv_combine(sp->ndim, sp->p, d->neq, sp->dev, res, k, u, nprev, kk, c, dt, S->dte,
+                                               per, S->st, out, store_k, st);
             };
-            const double* k12[3] = {k1, k2, k3};
+            const double* k12[2] = {k1, k2};
             const double c1[1] = {1.0}, c2[2] = {0.25, 0.25}, c3[3] = {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0};
             R(u);
-            Minv(k1);
-            combine(1, k12, c1, us);
+            minv_combine(k1, 0, k12, c1, us, true);
             R(us);
-            Minv(k2);
-            combine(2, k12, c2, us);
+            minv_combine(k2, 1, k12, c2, us, true);
             R(us);
-            Minv(k3);
-            combine(3, k12, c3, u);
+            minv_combine(k3, 2, k12, c3, u, false);
         }
         break;
     case ICE_STEP_RK3_TVD: {
@@ -1274,25 +1274,23 @@ int step_once(ice_stepper* S, float* u, hipStream_t st) {
     auto R = [&](const float* x) {
         rc |= ice::residual_mixed_f32state(sp->ndim, sp->p, d->prm.flux, sp->dev, d->ph, d->dd, x, res, st);
     };
-    auto Minv = [&](float* k) { rc |= ice::launch_minv_f32(sp->ndim, sp->p, d->neq, sp->dev, res, k, st); };
-    auto combine = [&](int nk, const float* const* kk, const double* c, float* out) {
-        rc |= ice::launch_rk_combine_f32(n, u, nk, kk, c, dt, S->dte, per, S->st, out, st);
+    // each stage's increment and combination in one pass over the state (3D) or two
+    auto minv_combine = [&](float* k, int nprev, const float* const* kk, const double* c, float* out, bool store_k) {
+        rc |= ice::launch_minv_combine_f32(sp->ndim, sp->p, d->neq, sp->dev, res, k, u, nprev, kk, c, dt, S->dte, per,
+                                           S->st, out, store_k, st);
     };
-    const float* kk[3] = {k1, k2, k3};
+    const float* kk[2] = {k1, k2};
     const double c1[1] = {1.0}, c2[2] = {0.25, 0.25}, c3[3] = {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0};
     rc |= ice::launch_step_begin(S->st, d->d_wave, S->ctl, st);
     R(u);
-    Minv(k1);
     if (S->desc.kind == ICE_STEP_RK3_SSP) {
-        combine(1, kk, c1, us);
+        minv_combine(k1, 0, kk, c1, us, true);
         R(us);
-        Minv(k2);
-        combine(2, kk, c2, us);
+        minv_combine(k2, 1, kk, c2, us, true);
         R(us);
-        Minv(k3);
-        combine(3, kk, c3, u);
+        minv_combine(k3, 2, kk, c3, u, false);
     } else {
-        combine(1, kk, c1, u);
+        minv_combine(k1, 0, kk, c1, u, false);
     }
     rc |= ice::launch_step_end(S->st, st);
     return rc;

@@ -182,10 +182,30 @@ struct MinvLines {
     static constexpr int BLOCK = ((EPB * NN + 63) / 64) * 64;
 };
 
+// Optional stage combination formed by the kernel from the increment k_new
+// it computes: out = base + dt (c0 k0 + c1 k1 + cn k_new) over the nprev
+// earlier increments k0, k1, with the step of rk_combine_kernel (the
+// stepper's dt, or the element's, none once the run is done); k_new is
+// rounded to KT first, as when it is stored and read back.
+template <class KT>
+struct StageComb {
+    KT* out = nullptr;  // null: k = M^-1 r only
+    const KT* base = nullptr;
+    const KT* k0 = nullptr;
+    const KT* k1 = nullptr;
+    int nprev = 0;
+    double c0 = 0, c1 = 0, cn = 0;
+    const double* dtp = nullptr;
+    const double* dte = nullptr;
+    const StepState* stp = nullptr;
+    bool store_k = true;  // also store k_new (a later stage reads it)
+};
+
 template <int P, int M, class KT = double>
 __global__ __launch_bounds__((MinvLines<P, M, KT>::BLOCK)) void minv_lines_kernel(DevSpace S,
                                                                                   const double* __restrict__ r,
-                                                                                  KT* __restrict__ k) {
+                                                                                  KT* __restrict__ k,
+                                                                                  StageComb<KT> cb) {
     constexpr int D = 3;
     using LL = MinvLines<P, M, KT>;
     constexpr int N = LL::N, NN = LL::NN, NB = LL::NB, PL = LL::PL, VS = LL::VS, EBUF = LL::EBUF, EPB = LL::EPB;
@@ -315,21 +335,55 @@ __global__ __launch_bounds__((MinvLines<P, M, KT>::BLOCK)) void minv_lines_kerne
 #pragma unroll
         for (int q = 0; q < N; ++q) in[q][v] = B[v * VS + a * PL + b * N + q];
     const size_t o = ((size_t)e * NB + a * NN + b * N) * M;
-    if (sing[lec]) {
+    const bool sg = sing[lec] != 0;
+    KT kn[N][M];
 #pragma unroll
-        for (int c = 0; c < N; ++c)
+    for (int i = 0; i < N; ++i)
 #pragma unroll
-            for (int v = 0; v < M; ++v) k[o + c * M + v] = (KT)(1e-8 * r[o + c * M + v]);
-    } else {
+        for (int v = 0; v < M; ++v) {
+            double acc = 0;
+#pragma unroll
+            for (int q = 0; q < N; ++q) acc += sB[i * N + q] * in[q][v];
+            kn[i][v] = sg ? (KT)(1e-8 * r[o + i * M + v]) : (KT)acc;
+        }
+    if (!cb.out) {
 #pragma unroll
         for (int i = 0; i < N; ++i)
 #pragma unroll
-            for (int v = 0; v < M; ++v) {
-                double acc = 0;
+            for (int v = 0; v < M; ++v) k[o + i * M + v] = kn[i][v];
+        return;
+    }
+    // the line's base and earlier increments are read before anything is
+    // stored (out may be base: the last stage updates the state in place)
+    const double dt = (cb.stp && cb.stp->done) ? 0.0 : (cb.dte ? cb.dte[e] : *cb.dtp);
+    double sum[N][M];
 #pragma unroll
-                for (int q = 0; q < N; ++q) acc += sB[i * N + q] * in[q][v];
-                k[o + i * M + v] = (KT)acc;
-            }
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int v = 0; v < M; ++v) sum[i][v] = cb.nprev > 0 ? cb.c0 * (double)cb.k0[o + i * M + v] : 0.0;
+    if (cb.nprev > 1) {
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int v = 0; v < M; ++v) sum[i][v] += cb.c1 * (double)cb.k1[o + i * M + v];
+    }
+    double bv[N][M];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int v = 0; v < M; ++v) bv[i][v] = (double)cb.base[o + i * M + v];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            sum[i][v] += cb.cn * (double)kn[i][v];
+            cb.out[o + i * M + v] = (KT)(bv[i][v] + dt * sum[i][v]);
+        }
+    if (cb.store_k) {
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int v = 0; v < M; ++v) k[o + i * M + v] = kn[i][v];
     }
 }
 
@@ -417,7 +471,8 @@ template <int D, int P, int M, class KT>
 int launch_minv_t(const DevSpace& S, const double* r, KT* k, hipStream_t st) {
     if constexpr (D == 3 && P >= 2) {
         using LL = MinvLines<P, M, KT>;
-        minv_lines_kernel<P, M, KT><<<(S.nelem + LL::EPB - 1) / LL::EPB, LL::BLOCK, 0, st>>>(S, r, k);
+        minv_lines_kernel<P, M, KT><<<(S.nelem + LL::EPB - 1) / LL::EPB, LL::BLOCK, 0, st>>>(S, r, k,
+                                                                                             StageComb<KT>{});
         return (int)hipGetLastError();
     }
     constexpr int NB = PowS<P + 1, D>::value;
@@ -458,6 +513,67 @@ int launch_minv(int ndim, int p, int neq, const DevSpace& S, const double* r, do
 int launch_minv_f32(int ndim, int p, int neq, const DevSpace& S, const double* r, float* k, hipStream_t st) {
     if (ndim == 1) return -1;  // FP32 states are 2D and 3D
     return minv_dispatch(ndim, p, neq, S, r, k, st);
+}
+
+// k = M^-1 r and out = base + dt (c[0] k0 + .. + c[nprev] k): in one launch
+// where the 3D line kernel applies (P2-P4), else the inverse mass and the
+// stage combination one after the other.  k receives the increment (always
+// on the two-launch path; with the fused kernel only if store_k).
+template <int M, class KT>
+int minv_combine_order(int p, const DevSpace& S, const double* r, KT* k, const StageComb<KT>& cb, hipStream_t st) {
+    auto go = [&](auto PP) {
+        constexpr int P = decltype(PP)::value;
+        using LL = MinvLines<P, M, KT>;
+        minv_lines_kernel<P, M, KT><<<(S.nelem + LL::EPB - 1) / LL::EPB, LL::BLOCK, 0, st>>>(S, r, k, cb);
+        return (int)hipGetLastError();
+    };
+    switch (p) {
+    case 2: return go(std::integral_constant<int, 2>{});
+    case 3: return go(std::integral_constant<int, 3>{});
+    case 4: return go(std::integral_constant<int, 4>{});
+    default: return -1;
+    }
+}
+
+template <class KT>
+int minv_combine_t(int ndim, int p, int neq, const DevSpace& S, const double* r, KT* k, const KT* base, int nprev,
+                   const KT* const* kk, const double* c, const double* dtp, const double* dte, size_t per,
+                   const StepState* stp, KT* out, bool store_k, hipStream_t st) {
+    if (ndim == 3 && p >= 2 && (neq == 1 || neq == 5)) {
+        StageComb<KT> cb;
+        cb.out = out;
+        cb.base = base;
+        cb.k0 = nprev > 0 ? kk[0] : nullptr;
+        cb.k1 = nprev > 1 ? kk[1] : nullptr;
+        cb.nprev = nprev;
+        cb.c0 = nprev > 0 ? c[0] : 0.0;
+        cb.c1 = nprev > 1 ? c[1] : 0.0;
+        cb.cn = c[nprev];
+        cb.dtp = dtp;
+        cb.dte = dte;
+        cb.stp = stp;
+        cb.store_k = store_k;
+        return neq == 1 ? minv_combine_order<1, KT>(p, S, r, k, cb, st) : minv_combine_order<5, KT>(p, S, r, k, cb, st);
+    }
+    int rc = minv_dispatch(ndim, p, neq, S, r, k, st);
+    const KT* ks[3] = {nprev > 0 ? kk[0] : k, nprev > 1 ? kk[1] : k, k};
+    if constexpr (std::is_same<KT, double>::value)
+        rc |= launch_rk_combine(S.nelem * per, base, nprev + 1, ks, c, dtp, dte, per, stp, out, st);
+    else
+        rc |= launch_rk_combine_f32(S.nelem * per, base, nprev + 1, ks, c, dtp, dte, per, stp, out, st);
+    return rc;
+}
+
+int launch_minv_combine(int ndim, int p, int neq, const DevSpace& S, const double* r, double* k, const double* base,
+                        int nprev, const double* const* kk, const double* c, const double* dtp, const double* dte,
+                        size_t per, const StepState* stp, double* out, bool store_k, hipStream_t st) {
+    return minv_combine_t<double>(ndim, p, neq, S, r, k, base, nprev, kk, c, dtp, dte, per, stp, out, store_k, st);
+}
+
+int launch_minv_combine_f32(int ndim, int p, int neq, const DevSpace& S, const double* r, float* k, const float* base,
+                            int nprev, const float* const* kk, const double* c, const double* dtp, const double* dte,
+                            size_t per, const StepState* stp, float* out, bool store_k, hipStream_t st) {
+    return minv_combine_t<float>(ndim, p, neq, S, r, k, base, nprev, kk, c, dtp, dte, per, stp, out, store_k, st);
 }
 
 static unsigned grid_for(size_t n) {
