@@ -1219,11 +1219,11 @@ int step_once(ice_stepper* S, double* u, hipStream_t st) {
         } else {
             // ssp_rk3.hpp:164-190; each stage's increment and combination
             // u1 = u + dt k1, u2 = u + dt (k1 + k2)/4, u = u + dt (k1/6 + k2/6 + 2 k3/3)
-            // in one pass over the state (3D) or two
+            // in one pass over the state
             auto minv_combine = [&](double* k, int nprev, const double* const* kk, const double* c, double* out,
                                     bool store_k) {
                 rc |= ice::launch_minv_combine(sp->ndim, sp->p, d->neq, sp->dev, res, k, u, nprev, kk, c, dt, S->dte,
-                                               per, S->st, out, store_k, st);
+                                               S->st, out, store_k, st);
             };
             const double* k12[2] = {k1, k2};
             const double c1[1] = {1.0}, c2[2] = {0.25, 0.25}, c3[3] = {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0};
@@ -1265,18 +1265,16 @@ int step_once(ice_stepper* S, double* u, hipStream_t st) {
 int step_once(ice_stepper* S, float* u, hipStream_t st) {
     ice_disc* d = S->disc;
     const ice_space* sp = d->space;
-    const size_t n = S->n;
     double* res = S->buf[0];
     float *k1 = S->fbuf[0], *k2 = S->fbuf[1], *k3 = S->fbuf[2], *us = S->fbuf[3];
     const double* dt = &S->st->dt;
-    const size_t per = (size_t)sp->nb * d->neq;
     int rc = 0;
     auto R = [&](const float* x) {
         rc |= ice::residual_mixed_f32state(sp->ndim, sp->p, d->prm.flux, sp->dev, d->ph, d->dd, x, res, st);
     };
-    // each stage's increment and combination in one pass over the state (3D) or two
+    // each stage's increment and combination in one pass over the state
     auto minv_combine = [&](float* k, int nprev, const float* const* kk, const double* c, float* out, bool store_k) {
-        rc |= ice::launch_minv_combine_f32(sp->ndim, sp->p, d->neq, sp->dev, res, k, u, nprev, kk, c, dt, S->dte, per,
+        rc |= ice::launch_minv_combine_f32(sp->ndim, sp->p, d->neq, sp->dev, res, k, u, nprev, kk, c, dt, S->dte,
                                            S->st, out, store_k, st);
     };
     const float* kk[2] = {k1, k2};

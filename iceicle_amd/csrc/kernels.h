@@ -99,14 +99,14 @@ int launch_rk_combine_f32(size_t n, const float* base, int nk, const float* cons
                           const double* dtp, const double* dte, size_t per, const StepState* stp, float* out,
                           hipStream_t st);
 // k = M^-1 r and out = base + dt (c[0] kk[0] + .. + c[nprev - 1] kk[nprev - 1] + c[nprev] k), the
-// step of launch_rk_combine: one launch in 3D at P2-P4 (k stored only if store_k), else the
-// inverse mass (k always stored) then the combination
+// step of launch_rk_combine (dte: one step per element), in one launch; k is stored only if
+// store_k
 int launch_minv_combine(int ndim, int p, int neq, const DevSpace& S, const double* r, double* k, const double* base,
                         int nprev, const double* const* kk, const double* c, const double* dtp, const double* dte,
-                        size_t per, const StepState* stp, double* out, bool store_k, hipStream_t st);
+                        const StepState* stp, double* out, bool store_k, hipStream_t st);
 int launch_minv_combine_f32(int ndim, int p, int neq, const DevSpace& S, const double* r, float* k, const float* base,
                             int nprev, const float* const* kk, const double* c, const double* dtp, const double* dte,
-                            size_t per, const StepState* stp, float* out, bool store_k, hipStream_t st);
+                            const StepState* stp, float* out, bool store_k, hipStream_t st);
 int launch_axpy_dt(size_t n, double coef, const double* dtp, const double* x, double* y, hipStream_t st);
 int launch_axpy_edt(size_t n, double coef, const double* dte, size_t per, const StepState* stp, const double* x,
                     double* y, hipStream_t st);

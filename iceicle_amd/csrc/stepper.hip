@@ -51,6 +51,25 @@ __device__ __forceinline__ double q1_det(const double* __restrict__ Xe, const do
                J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
 }
 
+// Optional stage combination formed by the kernel from the increment k_new
+// it computes: out = base + dt (c0 k0 + c1 k1 + cn k_new) over the nprev
+// earlier increments k0, k1, with the step of rk_combine_kernel (the
+// stepper's dt, or the element's, none once the run is done); k_new is
+// rounded to KT first, as when it is stored and read back.
+template <class KT>
+struct StageComb {
+    KT* out = nullptr;  // null: k = M^-1 r only
+    const KT* base = nullptr;
+    const KT* k0 = nullptr;
+    const KT* k1 = nullptr;
+    int nprev = 0;
+    double c0 = 0, c1 = 0, cn = 0;
+    const double* dtp = nullptr;
+    const double* dte = nullptr;
+    const StepState* stp = nullptr;
+    bool store_k = true;  // also store k_new (a later stage reads it)
+};
+
 // k = M^-1 r for every element and component.  Elements whose mass matrix is
 // singular (a zero detJ at a quadrature point) get 1e-8 r, the fallback of
 // tvd_rk3.hpp:190-204.
@@ -60,7 +79,8 @@ __device__ __forceinline__ double q1_det(const double* __restrict__ Xe, const do
 #define ICE_MINV_TPB 128
 #endif
 template <int D, int P, int M, int TPB, class KT = double>
-__global__ __launch_bounds__(TPB) void minv_kernel(DevSpace S, const double* __restrict__ r, KT* __restrict__ k) {
+__global__ __launch_bounds__(TPB) void minv_kernel(DevSpace S, const double* __restrict__ r, KT* __restrict__ k,
+                                                   StageComb<KT> cb) {
     constexpr int N = P + 1;
     constexpr int NB = PowS<N, D>::value;
     static_assert(TPB >= NB, "one thread per node of an element");
@@ -81,6 +101,19 @@ __global__ __launch_bounds__(TPB) void minv_kernel(DevSpace S, const double* __r
         const size_t base = (size_t)e0 * NB * M, total = (size_t)S.nelem * NB * M;
         for (int idx = threadIdx.x; idx < EPB * NB * M; idx += blockDim.x)
             bufA[idx] = (base + idx < total) ? r[base + idx] : 0.0;
+    }
+    const size_t o = ((size_t)(active ? e : 0) * NB + kk) * M;
+    // the stage combination's base and earlier increments, loaded here so
+    // that their latency hides behind the contractions (every value is read
+    // before out, which may be base, is stored)
+    double sum[M], bv[M];
+    if (cb.out && active) {
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            sum[v] = cb.nprev > 0 ? cb.c0 * (double)cb.k0[o + v] : 0.0;
+            if (cb.nprev > 1) sum[v] += cb.c1 * (double)cb.k1[o + v];
+            bv[v] = (double)cb.base[o + v];
+        }
     }
     int kd[D];
     {
@@ -154,16 +187,22 @@ __global__ __launch_bounds__(TPB) void minv_kernel(DevSpace S, const double* __r
         __syncthreads();
         double* t = cur; cur = nxt; nxt = t;
     }
-    if (active) {
-        const size_t o = ((size_t)e * NB + kk) * M;
-        if (sing[le]) {
+    if (!active) return;
+    const bool sg = sing[le] != 0;
+    KT kn[M];
 #pragma unroll
-            for (int v = 0; v < M; ++v) k[o + v] = (KT)(1e-8 * r[o + v]);
-        } else {
+    for (int v = 0; v < M; ++v) kn[v] = sg ? (KT)(1e-8 * r[o + v]) : (KT)cur[(le * NB + kk) * M + v];
+    if (cb.out) {
+        const double dt = (cb.stp && cb.stp->done) ? 0.0 : (cb.dte ? cb.dte[e] : *cb.dtp);
 #pragma unroll
-            for (int v = 0; v < M; ++v) k[o + v] = (KT)cur[(le * NB + kk) * M + v];
+        for (int v = 0; v < M; ++v) {
+            sum[v] += cb.cn * (double)kn[v];
+            cb.out[o + v] = (KT)(bv[v] + dt * sum[v]);
         }
+        if (!cb.store_k) return;
     }
+#pragma unroll
+    for (int v = 0; v < M; ++v) k[o + v] = kn[v];
 }
 
 // The same in 3D with one tensor line of N nodes per thread: the passes run
@@ -180,25 +219,6 @@ struct MinvLines {
     static constexpr int EBUF = M * VS;
     static constexpr int EPB = 128 / NN;
     static constexpr int BLOCK = ((EPB * NN + 63) / 64) * 64;
-};
-
-// Optional stage combination formed by the kernel from the increment k_new
-// it computes: out = base + dt (c0 k0 + c1 k1 + cn k_new) over the nprev
-// earlier increments k0, k1, with the step of rk_combine_kernel (the
-// stepper's dt, or the element's, none once the run is done); k_new is
-// rounded to KT first, as when it is stored and read back.
-template <class KT>
-struct StageComb {
-    KT* out = nullptr;  // null: k = M^-1 r only
-    const KT* base = nullptr;
-    const KT* k0 = nullptr;
-    const KT* k1 = nullptr;
-    int nprev = 0;
-    double c0 = 0, c1 = 0, cn = 0;
-    const double* dtp = nullptr;
-    const double* dte = nullptr;
-    const StepState* stp = nullptr;
-    bool store_k = true;  // also store k_new (a later stage reads it)
 };
 
 template <int P, int M, class KT = double>
@@ -387,6 +407,150 @@ __global__ __launch_bounds__((MinvLines<P, M, KT>::BLOCK)) void minv_lines_kerne
     }
 }
 
+// The same in 2D: a thread holds the line (t, .) of N nodes, applies B^-T
+// along axis 1 in registers, and after one LDS transpose the column (., t):
+// B^-T along axis 0, the scaling and B^-1 along axis 0 without leaving
+// registers; a transpose back and B^-1 along axis 1 end on the line it
+// read.  Two barriers instead of the node-per-thread kernel's four LDS
+// passes, and N M loads and stores in flight per thread.
+template <int P, int M>
+struct MinvLines2 {
+    static constexpr int N = P + 1, NB = N * N;
+    static constexpr int PL = N + (N % 2 == 0 ? 1 : 0);  // odd line stride
+    static constexpr int VS = N * PL;
+    static constexpr int EBUF = M * VS;
+    static constexpr int EPB = 128 / N;
+    static constexpr int BLOCK = ((EPB * N + 63) / 64) * 64;
+};
+
+template <int P, int M, class KT = double>
+__global__ __launch_bounds__((MinvLines2<P, M>::BLOCK)) void minv_lines2_kernel(DevSpace S,
+                                                                                const double* __restrict__ r,
+                                                                                KT* __restrict__ k,
+                                                                                StageComb<KT> cb) {
+    constexpr int D = 2;
+    using LL = MinvLines2<P, M>;
+    constexpr int N = LL::N, NB = LL::NB, PL = LL::PL, VS = LL::VS, EBUF = LL::EBUF, EPB = LL::EPB;
+    constexpr int NV = 1 << D;
+    __shared__ double buf[EPB * EBUF];
+    __shared__ double sX[EPB * NV * D];
+    __shared__ double sB[N * N];
+    __shared__ int sing[EPB];
+    const int e0 = blockIdx.x * EPB;
+    const int le = threadIdx.x / N;
+    const int t = threadIdx.x - le * N;
+    const int e = e0 + le;
+    const bool active = threadIdx.x < EPB * N && e < S.nelem;
+    const int lec = le < EPB ? le : 0;
+    double* const B = buf + (size_t)lec * EBUF;
+    const double* xs = S.tab;
+    const double* wgl = S.tab + (N + 2) + 3 * (N + 2) * N;
+    const size_t o = ((size_t)(active ? e : 0) * NB + t * N) * M;  // line (t, .)
+    {
+        const size_t xb = (size_t)e0 * NV * D, xt = (size_t)S.nelem * NV * D;
+        for (int idx = threadIdx.x; idx < EPB * NV * D; idx += blockDim.x)
+            sX[idx] = (xb + idx < xt) ? S.X[xb + idx] : 0.0;
+        for (int idx = threadIdx.x; idx < N * N; idx += blockDim.x) sB[idx] = wgl[N + idx];  // Binv[i][q]
+        if (threadIdx.x < EPB) sing[threadIdx.x] = 0;
+        double in[N][M];
+#pragma unroll
+        for (int c = 0; c < N; ++c)
+#pragma unroll
+            for (int v = 0; v < M; ++v) in[c][v] = active ? r[o + c * M + v] : 0.0;
+        __syncthreads();
+        // B^-T along axis 1: y[q] = sum_i Binv[i][q] in[i]
+        if (active) {
+#pragma unroll
+            for (int v = 0; v < M; ++v)
+#pragma unroll
+                for (int q = 0; q < N; ++q) {
+                    double acc = 0;
+#pragma unroll
+                    for (int i = 0; i < N; ++i) acc += sB[i * N + q] * in[i][v];
+                    B[v * VS + t * PL + q] = acc;
+                }
+        }
+    }
+    __syncthreads();
+    // column (., t): B^-T along axis 0, 1 / (w_q detJ_q), B^-1 along axis 0
+    if (active) {
+        double in[N][M], z[N][M];
+#pragma unroll
+        for (int v = 0; v < M; ++v)
+#pragma unroll
+            for (int i = 0; i < N; ++i) in[i][v] = B[v * VS + i * PL + t];
+        const double* Xe = sX + lec * NV * D;
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+            const double xi[D] = {xs[q], xs[t]};
+            const double det = q1_det<D>(Xe, xi);
+            if (det == 0.0) sing[lec] = 1;
+            const double sc = 1.0 / (wgl[q] * wgl[t] * det);
+#pragma unroll
+            for (int v = 0; v < M; ++v) {
+                double acc = 0;
+#pragma unroll
+                for (int i = 0; i < N; ++i) acc += sB[i * N + q] * in[i][v];
+                z[q][v] = acc * sc;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < M; ++v)
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                double acc = 0;
+#pragma unroll
+                for (int q = 0; q < N; ++q) acc += sB[i * N + q] * z[q][v];
+                B[v * VS + i * PL + t] = acc;
+            }
+    }
+    __syncthreads();
+    if (!active) return;
+    // line (t, .): B^-1 along axis 1
+    double in[N][M];
+#pragma unroll
+    for (int v = 0; v < M; ++v)
+#pragma unroll
+        for (int q = 0; q < N; ++q) in[q][v] = B[v * VS + t * PL + q];
+    const bool sg = sing[lec] != 0;
+    KT kn[N][M];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int v = 0; v < M; ++v) {
+            double acc = 0;
+#pragma unroll
+            for (int q = 0; q < N; ++q) acc += sB[i * N + q] * in[q][v];
+            kn[i][v] = sg ? (KT)(1e-8 * r[o + i * M + v]) : (KT)acc;
+        }
+    if (cb.out) {
+        // the line's base and earlier increments are read before anything
+        // is stored (out may be base)
+        const double dt = (cb.stp && cb.stp->done) ? 0.0 : (cb.dte ? cb.dte[e] : *cb.dtp);
+        double sum[N][M], bv[N][M];
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int v = 0; v < M; ++v) {
+                sum[i][v] = cb.nprev > 0 ? cb.c0 * (double)cb.k0[o + i * M + v] : 0.0;
+                if (cb.nprev > 1) sum[i][v] += cb.c1 * (double)cb.k1[o + i * M + v];
+                bv[i][v] = (double)cb.base[o + i * M + v];
+            }
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int v = 0; v < M; ++v) {
+                sum[i][v] += cb.cn * (double)kn[i][v];
+                cb.out[o + i * M + v] = (KT)(bv[i][v] + dt * sum[i][v]);
+            }
+        if (!cb.store_k) return;
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int v = 0; v < M; ++v) k[o + i * M + v] = kn[i][v];
+}
+
 // y += coef * (*dt) * x   (axpy, fespan.hpp:283-300, with the step size read on the device)
 __global__ void axpy_dt_kernel(size_t n, double coef, const double* __restrict__ dtp, const double* __restrict__ x,
                                double* __restrict__ y) {
@@ -467,42 +631,50 @@ __global__ void step_end_kernel(StepState* st) {
     }
 }
 
+// k = M^-1 r, and with cb.out set the stage combination in the same launch
+// (k is then stored only if cb.store_k)
 template <int D, int P, int M, class KT>
-int launch_minv_t(const DevSpace& S, const double* r, KT* k, hipStream_t st) {
+int launch_minv_t(const DevSpace& S, const double* r, KT* k, const StageComb<KT>& cb, hipStream_t st) {
     if constexpr (D == 3 && P >= 2) {
         using LL = MinvLines<P, M, KT>;
-        minv_lines_kernel<P, M, KT><<<(S.nelem + LL::EPB - 1) / LL::EPB, LL::BLOCK, 0, st>>>(S, r, k,
-                                                                                             StageComb<KT>{});
+        minv_lines_kernel<P, M, KT><<<(S.nelem + LL::EPB - 1) / LL::EPB, LL::BLOCK, 0, st>>>(S, r, k, cb);
         return (int)hipGetLastError();
     }
-    constexpr int NB = PowS<P + 1, D>::value;
-    // workgroup: ICE_MINV_TPB threads, or whole waves for one element when
-    // an element has more nodes
-    constexpr int TPB = (NB <= ICE_MINV_TPB) ? ICE_MINV_TPB : ((NB + 63) / 64) * 64;
-    constexpr int EPB = TPB / NB;
-    minv_kernel<D, P, M, TPB, KT><<<(S.nelem + EPB - 1) / EPB, TPB, 0, st>>>(S, r, k);
-    return (int)hipGetLastError();
+    else if constexpr (D == 2) {
+        using LL = MinvLines2<P, M>;
+        minv_lines2_kernel<P, M, KT><<<(S.nelem + LL::EPB - 1) / LL::EPB, LL::BLOCK, 0, st>>>(S, r, k, cb);
+        return (int)hipGetLastError();
+    } else {
+        constexpr int NB = PowS<P + 1, D>::value;
+        // workgroup: ICE_MINV_TPB threads, or whole waves for one element
+        // when an element has more nodes
+        constexpr int TPB = (NB <= ICE_MINV_TPB) ? ICE_MINV_TPB : ((NB + 63) / 64) * 64;
+        constexpr int EPB = TPB / NB;
+        minv_kernel<D, P, M, TPB, KT><<<(S.nelem + EPB - 1) / EPB, TPB, 0, st>>>(S, r, k, cb);
+        return (int)hipGetLastError();
+    }
 }
 
 template <int D, int M, class KT>
-int minv_order(int p, const DevSpace& S, const double* r, KT* k, hipStream_t st) {
+int minv_order(int p, const DevSpace& S, const double* r, KT* k, const StageComb<KT>& cb, hipStream_t st) {
     switch (p) {
-    case 1: return launch_minv_t<D, 1, M>(S, r, k, st);
-    case 2: return launch_minv_t<D, 2, M>(S, r, k, st);
-    case 3: return launch_minv_t<D, 3, M>(S, r, k, st);
-    case 4: return launch_minv_t<D, 4, M>(S, r, k, st);
+    case 1: return launch_minv_t<D, 1, M>(S, r, k, cb, st);
+    case 2: return launch_minv_t<D, 2, M>(S, r, k, cb, st);
+    case 3: return launch_minv_t<D, 3, M>(S, r, k, cb, st);
+    case 4: return launch_minv_t<D, 4, M>(S, r, k, cb, st);
     default: return -1;
     }
 }
 
 template <class KT>
-int minv_dispatch(int ndim, int p, int neq, const DevSpace& S, const double* r, KT* k, hipStream_t st) {
-    if (ndim == 1 && neq == 1) return minv_order<1, 1>(p, S, r, k, st);
-    if (ndim == 1 && neq == 3) return minv_order<1, 3>(p, S, r, k, st);
-    if (ndim == 2 && neq == 1) return minv_order<2, 1>(p, S, r, k, st);
-    if (ndim == 2 && neq == 4) return minv_order<2, 4>(p, S, r, k, st);
-    if (ndim == 3 && neq == 1) return minv_order<3, 1>(p, S, r, k, st);
-    if (ndim == 3 && neq == 5) return minv_order<3, 5>(p, S, r, k, st);
+int minv_dispatch(int ndim, int p, int neq, const DevSpace& S, const double* r, KT* k, hipStream_t st,
+                  const StageComb<KT>& cb = StageComb<KT>{}) {
+    if (ndim == 1 && neq == 1) return minv_order<1, 1>(p, S, r, k, cb, st);
+    if (ndim == 1 && neq == 3) return minv_order<1, 3>(p, S, r, k, cb, st);
+    if (ndim == 2 && neq == 1) return minv_order<2, 1>(p, S, r, k, cb, st);
+    if (ndim == 2 && neq == 4) return minv_order<2, 4>(p, S, r, k, cb, st);
+    if (ndim == 3 && neq == 1) return minv_order<3, 1>(p, S, r, k, cb, st);
+    if (ndim == 3 && neq == 5) return minv_order<3, 5>(p, S, r, k, cb, st);
     return -1;
 }
 
@@ -515,65 +687,38 @@ int launch_minv_f32(int ndim, int p, int neq, const DevSpace& S, const double* r
     return minv_dispatch(ndim, p, neq, S, r, k, st);
 }
 
-// k = M^-1 r and out = base + dt (c[0] k0 + .. + c[nprev] k): in one launch
-// where the 3D line kernel applies (P2-P4), else the inverse mass and the
-// stage combination one after the other.  k receives the increment (always
-// on the two-launch path; with the fused kernel only if store_k).
-template <int M, class KT>
-int minv_combine_order(int p, const DevSpace& S, const double* r, KT* k, const StageComb<KT>& cb, hipStream_t st) {
-    auto go = [&](auto PP) {
-        constexpr int P = decltype(PP)::value;
-        using LL = MinvLines<P, M, KT>;
-        minv_lines_kernel<P, M, KT><<<(S.nelem + LL::EPB - 1) / LL::EPB, LL::BLOCK, 0, st>>>(S, r, k, cb);
-        return (int)hipGetLastError();
-    };
-    switch (p) {
-    case 2: return go(std::integral_constant<int, 2>{});
-    case 3: return go(std::integral_constant<int, 3>{});
-    case 4: return go(std::integral_constant<int, 4>{});
-    default: return -1;
-    }
-}
-
+// k = M^-1 r and out = base + dt (c[0] k0 + .. + c[nprev] k) in one launch;
+// k receives the increment only if store_k (a later stage reads it).
 template <class KT>
 int minv_combine_t(int ndim, int p, int neq, const DevSpace& S, const double* r, KT* k, const KT* base, int nprev,
-                   const KT* const* kk, const double* c, const double* dtp, const double* dte, size_t per,
+                   const KT* const* kk, const double* c, const double* dtp, const double* dte,
                    const StepState* stp, KT* out, bool store_k, hipStream_t st) {
-    if (ndim == 3 && p >= 2 && (neq == 1 || neq == 5)) {
-        StageComb<KT> cb;
-        cb.out = out;
-        cb.base = base;
-        cb.k0 = nprev > 0 ? kk[0] : nullptr;
-        cb.k1 = nprev > 1 ? kk[1] : nullptr;
-        cb.nprev = nprev;
-        cb.c0 = nprev > 0 ? c[0] : 0.0;
-        cb.c1 = nprev > 1 ? c[1] : 0.0;
-        cb.cn = c[nprev];
-        cb.dtp = dtp;
-        cb.dte = dte;
-        cb.stp = stp;
-        cb.store_k = store_k;
-        return neq == 1 ? minv_combine_order<1, KT>(p, S, r, k, cb, st) : minv_combine_order<5, KT>(p, S, r, k, cb, st);
-    }
-    int rc = minv_dispatch(ndim, p, neq, S, r, k, st);
-    const KT* ks[3] = {nprev > 0 ? kk[0] : k, nprev > 1 ? kk[1] : k, k};
-    if constexpr (std::is_same<KT, double>::value)
-        rc |= launch_rk_combine(S.nelem * per, base, nprev + 1, ks, c, dtp, dte, per, stp, out, st);
-    else
-        rc |= launch_rk_combine_f32(S.nelem * per, base, nprev + 1, ks, c, dtp, dte, per, stp, out, st);
-    return rc;
+    StageComb<KT> cb;
+    cb.out = out;
+    cb.base = base;
+    cb.k0 = nprev > 0 ? kk[0] : nullptr;
+    cb.k1 = nprev > 1 ? kk[1] : nullptr;
+    cb.nprev = nprev;
+    cb.c0 = nprev > 0 ? c[0] : 0.0;
+    cb.c1 = nprev > 1 ? c[1] : 0.0;
+    cb.cn = c[nprev];
+    cb.dtp = dtp;
+    cb.dte = dte;
+    cb.stp = stp;
+    cb.store_k = store_k;
+    return minv_dispatch(ndim, p, neq, S, r, k, st, cb);
 }
 
 int launch_minv_combine(int ndim, int p, int neq, const DevSpace& S, const double* r, double* k, const double* base,
                         int nprev, const double* const* kk, const double* c, const double* dtp, const double* dte,
-                        size_t per, const StepState* stp, double* out, bool store_k, hipStream_t st) {
-    return minv_combine_t<double>(ndim, p, neq, S, r, k, base, nprev, kk, c, dtp, dte, per, stp, out, store_k, st);
+                        const StepState* stp, double* out, bool store_k, hipStream_t st) {
+    return minv_combine_t<double>(ndim, p, neq, S, r, k, base, nprev, kk, c, dtp, dte, stp, out, store_k, st);
 }
 
 int launch_minv_combine_f32(int ndim, int p, int neq, const DevSpace& S, const double* r, float* k, const float* base,
                             int nprev, const float* const* kk, const double* c, const double* dtp, const double* dte,
-                            size_t per, const StepState* stp, float* out, bool store_k, hipStream_t st) {
-    return minv_combine_t<float>(ndim, p, neq, S, r, k, base, nprev, kk, c, dtp, dte, per, stp, out, store_k, st);
+                            const StepState* stp, float* out, bool store_k, hipStream_t st) {
+    return minv_combine_t<float>(ndim, p, neq, S, r, k, base, nprev, kk, c, dtp, dte, stp, out, store_k, st);
 }
 
 static unsigned grid_for(size_t n) {
