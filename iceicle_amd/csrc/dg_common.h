@@ -422,6 +422,60 @@ __device__ __forceinline__ void ddgic_lift(const typename PH::Params& ph, typena
         }
 }
 
+// Physical gradient of the basis function with tensor index kd at the point
+// whose table rows are pt (rows 0..N-1 the nodes, N and N + 1 the interval
+// ends) in the element with vertices X
+template <int D, int N>
+__device__ __forceinline__ void basis_grad_rows(const double* __restrict__ tab, const double (&X)[1 << D][D],
+                                                const int* pt, const int* kd, double* gphi) {
+    const double* L0 = tab + (N + 2);
+    const double* L1 = L0 + (N + 2) * N;
+    double xi[D], cg[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) xi[d] = tab[pt[d]];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        double t = 1.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) t *= (d == a) ? L1[pt[d] * N + kd[d]] : L0[pt[d] * N + kd[d]];
+        cg[a] = t;
+    }
+    double J[D][D], Hm[D][D][D], A[D][D];
+    q1_map<D>(X, xi, nullptr, J, Hm);
+    double det = det_adj<D>(J, A);
+    det = (det == 0.0) ? 1.0 : det;
+#pragma unroll
+    for (int jj = 0; jj < D; ++jj) {
+        double sacc = 0;
+#pragma unroll
+        for (int a = 0; a < D; ++a) sacc += cg[a] * A[a][jj];
+        gphi[jj] = sacc / det;
+    }
+}
+
+// DDGIC test gradient at the face point with table rows fp: the basis
+// gradient there, or (mirror, the published reading of the interface
+// correction) its average with the gradient at the point reflected through
+// the element along the face normal, i.e. the same point of the opposite
+// face (the one axis whose row is an interval end, N <-> N + 1)
+template <int D, int N>
+__device__ __forceinline__ void ic_test_gradient(const double* __restrict__ tab, const double (&X)[1 << D][D],
+                                                 const signed char* fp, const int* kd, bool mirror, double* gphi) {
+    int pt[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) pt[d] = fp[d];
+    basis_grad_rows<D, N>(tab, X, pt, kd, gphi);
+    if (mirror) {
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            if (pt[d] >= N) pt[d] = 2 * N + 1 - pt[d];
+        double gm[D];
+        basis_grad_rows<D, N>(tab, X, pt, kd, gm);
+#pragma unroll
+        for (int d = 0; d < D; ++d) gphi[d] = 0.5 * (gphi[d] + gm[d]);
+    }
+}
+
 // DDG single-valued gradient and weighted normal flux at one interior face
 // point (conservation_law.hpp:502-575): Q = (F_visc - F_conv) w sqrt(g).
 template <int D, class PH>

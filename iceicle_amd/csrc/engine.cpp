@@ -693,10 +693,14 @@ static void derive_phys(const ice_disc_params* p, int order, ice::PhysParams& ph
     ph.sigma_ic = p->sigma_ic;
     // interface correction (conservation_law.hpp:580-610, :750-780): SOURCE
     // keeps the average operator's 1/2 per side and the Dirichlet term,
-    // PUBLISHED applies sigma per side and no boundary term
+    // PUBLISHED applies sigma per side, no boundary term, and each side's
+    // test gradient averaged between the face point and its mirror through
+    // the element along the face normal (the published DDGIC table,
+    // doc/sine_diffusion_1d_study.md:45-55)
     const bool pub = p->ddgic_form == ICE_DDGIC_PUBLISHED;
     ph.sigma_face = pub ? p->sigma_ic : 0.5 * p->sigma_ic;
     ph.sigma_bdy = pub ? 0.0 : p->sigma_ic;
+    ph.ic_mirror = pub ? 1 : 0;
     ph.beta0 = (double)(order + 1) * (order + 1);
     ph.beta1 = p->interior_penalty ? 0.0 : 1.0 / std::max((double)(2 * order * (order + 1)), 1.0);
     ph.beta0_bdy = ph.beta0;

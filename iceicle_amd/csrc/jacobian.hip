@@ -156,6 +156,10 @@ __global__ __launch_bounds__(256) void jac_face_kernel(DevSpace S, PhysParams ph
     __shared__ double sP0[IC ? NFQ : 1][IC ? MD : 1];
     __shared__ double sdP[IC ? NC : 1][IC ? NFQ : 1][IC ? MD : 1];
     const bool ddgic = IC && ph.sigma_face != 0.0;
+    // the lift's test gradients (ic_test_gradient): the face point's, or with
+    // ic_mirror their average with the mirror point's, whose geometry sgeom holds
+    __shared__ double btg[IC ? 2 : 1][IC ? NFQ : 1][IC ? NB : 1][IC ? D : 1];
+    __shared__ PointGeo<D> sgeom[IC ? 2 : 1][IC ? NFQ : 1];
     __shared__ double sred[256];
     __shared__ double seps;
     const int f = S.jorder[base + blockIdx.x];
@@ -181,6 +185,14 @@ __global__ __launch_bounds__(256) void jac_face_kernel(DevSpace S, PhysParams ph
             xi[d] = S.tab[pt[d]];
         }
         make_geo<D>(XS, xi, sgeo[side][q]);
+        if constexpr (IC) {
+            if (ddgic && ph.ic_mirror) {
+                double xim[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d) xim[d] = S.tab[pt[d] >= N ? 2 * N + 1 - pt[d] : pt[d]];
+                make_geo<D>(XS, xim, sgeom[side][q]);
+            }
+        }
         double u[M], g[M * D], hn[M * D];
         trace_eval<D, N, M, true>(S, U + (size_t)(side ? eR : eL) * NB * M, pt, XS, nrm, u, g, hn);
 #pragma unroll
@@ -210,6 +222,26 @@ __global__ __launch_bounds__(256) void jac_face_kernel(DevSpace S, PhysParams ph
         for (int d = 0; d < D; ++d) {
             bgx[side][q][i][d] = gx[d];
             bhn[side][q][i][d] = hn[d];
+        }
+        if constexpr (IC) {
+            if (ddgic) {
+                if (ph.ic_mirror) {
+                    int ptm[D];
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        const int pd = spt[side][q][d];
+                        ptm[d] = pd >= N ? 2 * N + 1 - pd : pd;
+                    }
+                    double phm, grm[D], hrm[D * D], gxm[D], hnm[D];
+                    basis_at<D, N>(S.tab, ptm, i, phm, grm, hrm);
+                    phys_basis<D>(sgeom[side][q], grm, hrm, snrm[q], gxm, hnm);
+#pragma unroll
+                    for (int d = 0; d < D; ++d) btg[side][q][i][d] = 0.5 * (gx[d] + gxm[d]);
+                } else {
+#pragma unroll
+                    for (int d = 0; d < D; ++d) btg[side][q][i][d] = gx[d];
+                }
+            }
         }
     }
     if (tid < NFQ) {
@@ -246,7 +278,7 @@ __global__ __launch_bounds__(256) void jac_face_kernel(DevSpace S, PhysParams ph
             if (ddgic)
                 for (int q = 0; q < NFQ; ++q)
 #pragma unroll
-                    for (int d = 0; d < D; ++d) acc -= sP0[q][v * D + d] * bgx[side][q][i][d];
+                    for (int d = 0; d < D; ++d) acc -= sP0[q][v * D + d] * btg[side][q][i][d];
         }
         if (side) sR += acc * acc;
         else sL += acc * acc;
@@ -361,7 +393,7 @@ __global__ __launch_bounds__(256) void jac_face_kernel(DevSpace S, PhysParams ph
 #pragma unroll
                         for (int q = 0; q < NFQ; ++q)
 #pragma unroll
-                            for (int d = 0; d < D; ++d) acc -= sdP[c][q][v * D + d] * bgx[rs][q][i][d];
+                            for (int d = 0; d < D; ++d) acc -= sdP[c][q][v * D + d] * btg[rs][q][i][d];
                 }
                 val[k] = acc * ieps;
                 diag[k] = rs == cs;

@@ -37,6 +37,7 @@ struct PhysT {
     // ConservationLawDDG
     R sigma_ic;
     R sigma_face, sigma_bdy;  // DDGIC factors per interior side and on Dirichlet faces
+    int ic_mirror;            // DDGIC test gradients averaged with the mirror point's
     R beta0, beta1, beta0_bdy;
     // parameter-only factors of the flux formulas, computed once on the host
     // (set_derived) so the pointwise fluxes multiply instead of divide
@@ -90,6 +91,7 @@ inline PhysT<R> phys_cast(const PhysParams& p) {
     q.sigma_ic = (R)p.sigma_ic;
     q.sigma_face = (R)p.sigma_face;
     q.sigma_bdy = (R)p.sigma_bdy;
+    q.ic_mirror = p.ic_mirror;
     q.beta0 = (R)p.beta0;
     q.beta1 = (R)p.beta1;
     q.beta0_bdy = (R)p.beta0_bdy;

@@ -1042,31 +1042,8 @@ __global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph0
                 const double* Ps = Pic + slot * NFQ * M * D;
 #pragma unroll 1
                 for (int fq = 0; fq < NFQ; ++fq) {
-                    double xi[D], cg[D];
-#pragma unroll
-                    for (int d = 0; d < D; ++d) xi[d] = xs[fp[fq * D + d]];
-#pragma unroll
-                    for (int a = 0; a < D; ++a) {
-                        double t = 1.0;
-#pragma unroll
-                        for (int d = 0; d < D; ++d) {
-                            const int s = fp[fq * D + d];
-                            t *= (d == a) ? L1[s * N + kd[d]] : L0[s * N + kd[d]];
-                        }
-                        cg[a] = t;
-                    }
-                    double J[D][D], Hm[D][D][D], A[D][D];
-                    q1_map<D>(X, xi, nullptr, J, Hm);
-                    double det = det_adj<D>(J, A);
-                    det = (det == 0.0) ? 1.0 : det;
                     double gphi[D];
-#pragma unroll
-                    for (int jj = 0; jj < D; ++jj) {
-                        double s = 0;
-#pragma unroll
-                        for (int a = 0; a < D; ++a) s += cg[a] * A[a][jj];
-                        gphi[jj] = s / det;
-                    }
+                    ic_test_gradient<D, N>(S.tab, X, fp + fq * D, kd, ph.ic_mirror != 0, gphi);
 #pragma unroll
                     for (int v = 0; v < M; ++v) {
                         double s = 0;
@@ -1419,35 +1396,10 @@ __global__ __launch_bounds__((ElemTile<2, P, TPB>::BLOCK), ICE_ELEM_MINW) void e
         if constexpr (PH::HAS_HOMOGENEITY) {
             if (Pic) {
                 const VT* Ps = Pic + slot * NFQ * M * D;
-                const double* Ld0 = S.tab + (N + 2);
-                const double* Ld1 = Ld0 + (N + 2) * N;
 #pragma unroll 1
                 for (int fq = 0; fq < NFQ; ++fq) {
-                    double xi[D], cg[D];
-#pragma unroll
-                    for (int d = 0; d < D; ++d) xi[d] = xs[fp[fq * D + d]];
-#pragma unroll
-                    for (int a = 0; a < D; ++a) {
-                        double t = 1.0;
-#pragma unroll
-                        for (int d = 0; d < D; ++d) {
-                            const int sidx = fp[fq * D + d];
-                            t *= (d == a) ? Ld1[sidx * N + kd[d]] : Ld0[sidx * N + kd[d]];
-                        }
-                        cg[a] = t;
-                    }
-                    double J[D][D], Hm[D][D][D], A[D][D];
-                    q1_map<D>(X, xi, nullptr, J, Hm);
-                    double det = det_adj<D>(J, A);
-                    det = (det == 0.0) ? 1.0 : det;
                     double gphi[D];
-#pragma unroll
-                    for (int jj = 0; jj < D; ++jj) {
-                        double sacc = 0;
-#pragma unroll
-                        for (int a = 0; a < D; ++a) sacc += cg[a] * A[a][jj];
-                        gphi[jj] = sacc / det;
-                    }
+                    ic_test_gradient<D, N>(S.tab, X, fp + fq * D, kd, ph.ic_mirror != 0, gphi);
 #pragma unroll
                     for (int v = 0; v < M; ++v) {
                         double sacc = 0;
@@ -1897,8 +1849,6 @@ __global__ __launch_bounds__((Elem3Tile<P, TPB>::BLOCK), ICE_ELEM3_MINW) void el
     if constexpr (!QALIAS) lift(r);
     if constexpr (PH::HAS_HOMOGENEITY) {
         if (Pic) {
-            const double* Ld = S.tab + (N + 2);
-            const double* Ld1 = Ld + (N + 2) * N;
             double X[NV][D];
 #pragma unroll
             for (int i = 0; i < NV; ++i)
@@ -1913,31 +1863,8 @@ __global__ __launch_bounds__((Elem3Tile<P, TPB>::BLOCK), ICE_ELEM3_MINW) void el
                 const VT* Ps = Pic + slot * NFQ * M * D;
 #pragma unroll 1
                 for (int fq = 0; fq < NFQ; ++fq) {
-                    double xi[D], cg[D];
-#pragma unroll
-                    for (int d = 0; d < D; ++d) xi[d] = xs[fp[fq * D + d]];
-#pragma unroll
-                    for (int a = 0; a < D; ++a) {
-                        double t = 1.0;
-#pragma unroll
-                        for (int d = 0; d < D; ++d) {
-                            const int sidx = fp[fq * D + d];
-                            t *= (d == a) ? Ld1[sidx * N + kd[d]] : Ld[sidx * N + kd[d]];
-                        }
-                        cg[a] = t;
-                    }
-                    double J[D][D], Hm[D][D][D], A[D][D];
-                    q1_map<D>(X, xi, nullptr, J, Hm);
-                    double det = det_adj<D>(J, A);
-                    det = (det == 0.0) ? 1.0 : det;
                     double gphi[D];
-#pragma unroll
-                    for (int jj = 0; jj < D; ++jj) {
-                        double sacc = 0;
-#pragma unroll
-                        for (int a = 0; a < D; ++a) sacc += cg[a] * A[a][jj];
-                        gphi[jj] = sacc / det;
-                    }
+                    ic_test_gradient<D, N>(S.tab, X, fp + fq * D, kd, ph.ic_mirror != 0, gphi);
 #pragma unroll
                     for (int v = 0; v < M; ++v) {
                         double sacc = 0;

@@ -150,11 +150,14 @@ typedef struct ice_space_info {
 /* Discretization parameters: ConservationLawDDG public members
  * (conservation_law.hpp:132-170) plus the flux functor coefficients. */
 /* DDGIC interface-correction forms.  SOURCE follows conservation_law.hpp
- * as written: sigma_ic/2 times each side's test gradient on interior faces
- * (:580-610) and sigma_ic on Dirichlet faces (:750-780).  PUBLISHED is the
- * form that reproduces the DDGIC column of doc/sine_diffusion_1d_study.md:45-55
- * for P1 to the printed digits: sigma_ic on each side of interior faces and
- * no boundary term. */
+ * as written: sigma_ic/2 times each side's test gradient at the face point
+ * on interior faces (:580-610) and sigma_ic on Dirichlet faces (:750-780).
+ * PUBLISHED is the form that reproduces the DDGIC table of
+ * doc/sine_diffusion_1d_study.md:45-55 (P1-P4) to the digits of its
+ * standard-DDG table: sigma_ic on each side of interior faces, no boundary
+ * term, and each side's test gradient averaged between the face point and
+ * its mirror through the element along the face normal (in 1D the mean of
+ * the derivative at the element's two ends). */
 #define ICE_DDGIC_SOURCE 0
 #define ICE_DDGIC_PUBLISHED 1
 

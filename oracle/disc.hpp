@@ -110,6 +110,10 @@ struct FESpace {
         std::array<double, D> shiftR{};  // added to the right centroid (coupled periodic)
         std::vector<double> xiL, xiR;    // [nfq][D]
         std::vector<double> BL, GL, HL, BR, GR, HR;
+        // each side's point mirrored through its element along the face
+        // normal (the opposite face) and the basis there: the published
+        // DDGIC reading's test gradients
+        std::vector<double> xiLm, xiRm, GLm, HLm, GRm, HRm;
     };
     std::vector<Trace> interior, boundary, periodic;
 
@@ -166,6 +170,16 @@ struct FESpace {
             trace_to_ref<D>(fnr, sR, &t.xiR[q * D]);
             basis.eval(&t.xiL[q * D], &t.BL[q * nb], &t.GL[q * nb * D], &t.HL[q * nb * D * D]);
             basis.eval(&t.xiR[q * D], &t.BR[q * nb], &t.GR[q * nb * D], &t.HR[q * nb * D * D]);
+        }
+        t.xiLm = t.xiL; t.xiRm = t.xiR;
+        t.GLm.resize(nfq * nb * D); t.HLm.resize(nfq * nb * D * D);
+        t.GRm.resize(nfq * nb * D); t.HRm.resize(nfq * nb * D * D);
+        std::vector<double> Bm(nb);
+        for (int q = 0; q < nfq; ++q) {
+            t.xiLm[q * D + fnl % D] = -t.xiL[q * D + fnl % D];
+            t.xiRm[q * D + fnr % D] = -t.xiR[q * D + fnr % D];
+            basis.eval(&t.xiLm[q * D], Bm.data(), &t.GLm[q * nb * D], &t.HLm[q * nb * D * D]);
+            basis.eval(&t.xiRm[q * D], Bm.data(), &t.GRm[q * nb * D], &t.HRm[q * nb * D * D]);
         }
         return t;
     }
@@ -257,9 +271,12 @@ struct ConservationLawDDG {
     bool interior_penalty = false;
     double sigma_ic = 0.0;
     // reading of the interface correction: false = conservation_law.hpp as
-    // written (sigma/2 per side, Dirichlet term); true = sigma per side and no
-    // boundary term, which reproduces the published DDGIC P1 column
-    // (doc/sine_diffusion_1d_study.md:45-55)
+    // written (sigma/2 per side, Dirichlet term, each side's test gradient at
+    // the face point); true = sigma per side, no boundary term, and each
+    // side's test gradient averaged between the face point and its mirror
+    // through the element along the face normal, which reproduces the
+    // published DDGIC table (doc/sine_diffusion_1d_study.md:45-55; in 1D the
+    // mean of the test function's derivative at the element's two ends)
     bool ddgic_published = false;
     std::vector<Callback> dirichlet_callbacks, neumann_callbacks;
     std::optional<Callback> user_source;
@@ -372,6 +389,13 @@ struct ConservationLawDDG {
                     const double ic_side = ddgic_published ? 1.0 : 0.5;
                     double Gt[neq * D * neq * D];
                     diff_flux.homogeneity_tensor(uavg, Gt);
+                    if (ddgic_published) {
+                        std::vector<double> gm(nb * D), hm(nb * D * D);
+                        side_eval(fs, t.elemL, &t.xiLm[q * D], &t.GLm[q * nb * D], &t.HLm[q * nb * D * D], gm.data(), hm.data());
+                        for (int i = 0; i < nb * D; ++i) gL[i] = 0.5 * (gL[i] + gm[i]);
+                        side_eval(fs, t.elemR, &t.xiRm[q * D], &t.GRm[q * nb * D], &t.HRm[q * nb * D * D], gm.data(), hm.data());
+                        for (int i = 0; i < nb * D; ++i) gR[i] = 0.5 * (gR[i] + gm[i]);
+                    }
                     for (int it = 0; it < nb; ++it)
                         for (int ie = 0; ie < neq; ++ie)
                             for (int k = 0; k < D; ++k)

@@ -206,11 +206,14 @@ def test_burgers_2d_ddgic(order, form):
     _compare(m, order, p, u)
 
 
-def test_burgers_ddgic_1d_and_3d():
+@pytest.mark.parametrize("form", [abi.DDGIC_SOURCE, abi.DDGIC_PUBLISHED])
+def test_burgers_ddgic_1d_and_3d(form):
     p1 = cases.burgers_params(1, mu=0.03, sigma_ic=0.5)
+    p1.ddgic_form = form
     m1, keep1 = _mesh(1, [5], ["DIRICHLET", "NEUMANN"], 0.2, 4, bcflag=[1, 1])
     _compare(m1, 2, p1, cases.random_state(p1.flux, 1, 5, 3, 23))
     p3 = cases.burgers_params(3, mu=0.02, a=(0.3, -0.2, 0.1), sigma_ic=0.5)
+    p3.ddgic_form = form
     m3, keep3 = _mesh(3, [2, 2, 3], ["DIRICHLET", "PERIODIC", "NEUMANN", "DIRICHLET", "PERIODIC", "EXTRAPOLATION"],
                       0.0, 0, bcflag=[0, 0, 1, 1, 0, 0], coupled=True)
     _compare(m3, 2, p3, cases.random_state(p3.flux, 3, 12, 27, 25))

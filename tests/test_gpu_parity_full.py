@@ -227,19 +227,22 @@ def test_sutherland_viscosity(ndim):
     assert _compare(m, 2, p) <= TOL
 
 
-@pytest.mark.parametrize("nx", [4, 8])
-def test_ddgic_published_column_on_device(nx):
-    """The published DDGIC P1 errors (doc/sine_diffusion_1d_study.md:51-52)
-    through the device residual, inverse mass and RK3-TVD stepper."""
+@pytest.mark.parametrize("nx,order", [(4, 1), (8, 1), (4, 2), (8, 2), (4, 3), (8, 3)])
+def test_ddgic_published_column_on_device(nx, order):
+    """The published DDGIC errors (doc/sine_diffusion_1d_study.md:51-52,
+    P1-P3) through the device residual, inverse mass and RK3-TVD stepper,
+    with the published reading of the interface correction."""
     from iceicle_amd import engine
-    published = {4: 0.0824449545, 8: 0.027170573}
+    published = {(4, 1): 0.0824449545, (8, 1): 0.027170573, (4, 2): 0.0145741899, (8, 2): 0.00193508108,
+                 (4, 3): 0.00120722602, (8, 3): 6.88325795e-05}
+    tol = {1: 4e-7, 2: 3e-7, 3: 1e-6}
     m = abi.mesh_desc(1, [nx], [0.0], [2 * math.pi], ["DIRICHLET", "DIRICHLET"])
     prm = abi.default_params(abi.FLUX_BURGERS)
     prm.mu = 1.0
     prm.sigma_ic = 0.5
     prm.ddgic_form = abi.DDGIC_PUBLISHED
     prm.dirichlet[0] = abi.Field.make("CONSTANT", [0.0])
-    space = engine.FESpace(m, 1)
+    space = engine.FESpace(m, order)
     disc = engine.ConservationLawDDG(space, prm)
     u = torch.empty(disc.size, dtype=torch.float64, device="cuda")
     engine.project(disc, abi.Field.make("SINE", [1, 1, 0]), u)
@@ -247,5 +250,6 @@ def test_ddgic_published_column_on_device(nx):
     st = engine.ExplicitStepper(disc, abi.stepper_desc(abi.STEP_RK3_TVD, dt=dt, tfinal=1.0))
     n, t = st.solve(u)
     err = engine.l2_error(disc, abi.Field.make("EXP_DECAY_SINE", [1.0, 1.0]), u)
-    print(f"1D heat DDGIC nx={nx} P1: L2={err:.9g} (published {published[nx]})", flush=True)
-    assert abs(err - published[nx]) / published[nx] < 4e-7
+    ref = published[(nx, order)]
+    print(f"1D heat DDGIC nx={nx} P{order}: L2={err:.9g} (published {ref})", flush=True)
+    assert abs(err - ref) / ref < tol[order]

@@ -51,11 +51,23 @@ def test_burgers_2d_periodic_coupled(order):
     assert _run(2, [4, 5], order, p, ["PERIODIC"] * 4, perturb=0.15, coupled=True) <= TOL
 
 
+@pytest.mark.parametrize("form", [abi.DDGIC_SOURCE, abi.DDGIC_PUBLISHED])
 @pytest.mark.parametrize("order", [1, 2, 3])
-def test_burgers_2d_mixed_bcs_ddgic(order):
+def test_burgers_2d_mixed_bcs_ddgic(order, form):
+    """Both readings of the interface correction on a perturbed mesh; the
+    published one averages each side's test gradient with the mirror
+    point's, whose geometry differs from the face point's here."""
     p = cases.burgers_params(2, sigma_ic=0.5, a=(0.3, -0.2, 0))
+    p.ddgic_form = form
     bct = ["DIRICHLET", "NEUMANN", "EXTRAPOLATION", "DIRICHLET"]
     assert _run(2, [4, 3], order, p, bct, perturb=0.2, bcflag=[0, 1, 0, 1]) <= TOL
+
+
+@pytest.mark.parametrize("order", [2, 3, 4])
+def test_burgers_1d_ddgic_published(order):
+    p = cases.burgers_params(1, sigma_ic=0.5)
+    p.ddgic_form = abi.DDGIC_PUBLISHED
+    assert _run(1, [6], order, p, ["DIRICHLET", "DIRICHLET"], perturb=0.2, bcflag=[0, 1]) <= TOL
 
 
 @pytest.mark.parametrize("order", [1, 2, 3, 4])
@@ -98,10 +110,13 @@ def test_ns_3d(order):
     assert _run(3, [2, 3, 2], order, p, bct, perturb=0.1) <= TOL
 
 
+@pytest.mark.parametrize("form", [abi.DDGIC_SOURCE, abi.DDGIC_PUBLISHED])
 @pytest.mark.parametrize("order", [2, 3])
-def test_burgers_3d_ddgic(order):
-    """The DDGIC lifting term (sigma_ic != 0) of the 3D element kernel."""
+def test_burgers_3d_ddgic(order, form):
+    """The DDGIC lifting term (sigma_ic != 0) of the 3D element kernel, both
+    readings."""
     p = cases.burgers_params(3, mu=0.05, a=(0.3, 0.0, -0.2), sigma_ic=0.5)
+    p.ddgic_form = form
     bct = ["DIRICHLET", "PERIODIC", "NEUMANN", "DIRICHLET", "PERIODIC", "EXTRAPOLATION"]
     assert _run(3, [2, 3, 3], order, p, bct, perturb=0.1, coupled=True, bcflag=[0, 0, 0, 1, 0, 0]) <= TOL
 
