@@ -70,6 +70,41 @@ struct StageComb {
     bool store_k = true;  // also store k_new (a later stage reads it)
 };
 
+// out = base + dt (c0 k0 + c1 k1 + cn kn) over the NL contiguous values of a
+// line at o, with NPV earlier increments: straight-line loads of each input
+// (so they merge into wide loads), all read before out, which may be base,
+// is stored; then kn itself if the stage keeps it
+template <int NPV, int NL, class KT>
+__device__ __forceinline__ void stage_combine_line(const StageComb<KT>& cb, KT* __restrict__ k, size_t o, double dt,
+                                                   const KT* kn) {
+    double sum[NL], bv[NL];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        double acc = 0.0;
+        if constexpr (NPV > 0) acc = cb.c0 * (double)cb.k0[o + i];
+        if constexpr (NPV > 1) acc += cb.c1 * (double)cb.k1[o + i];
+        sum[i] = acc;
+        bv[i] = (double)cb.base[o + i];
+    }
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        sum[i] += cb.cn * (double)kn[i];
+        cb.out[o + i] = (KT)(bv[i] + dt * sum[i]);
+    }
+    if (cb.store_k) {
+#pragma unroll
+        for (int i = 0; i < NL; ++i) k[o + i] = kn[i];
+    }
+}
+
+template <int NL, class KT>
+__device__ __forceinline__ void stage_combine_line(const StageComb<KT>& cb, KT* __restrict__ k, size_t o, double dt,
+                                                   const KT* kn) {
+    if (cb.nprev == 0) stage_combine_line<0, NL, KT>(cb, k, o, dt, kn);
+    else if (cb.nprev == 1) stage_combine_line<1, NL, KT>(cb, k, o, dt, kn);
+    else stage_combine_line<2, NL, KT>(cb, k, o, dt, kn);
+}
+
 // k = M^-1 r for every element and component.  Elements whose mass matrix is
 // singular (a zero detJ at a quadrature point) get 1e-8 r, the fallback of
 // tvd_rk3.hpp:190-204.
@@ -211,18 +246,35 @@ __global__ __launch_bounds__(TPB) void minv_kernel(DevSpace S, const double* __r
 // place: a thread rewrites the entries of its own line) replace the six
 // LDS round trips of the node-per-thread kernel; the residual is read and
 // the result written as contiguous lines of N nodes.
+// target threads per workgroup of the line kernels (whole elements)
+#ifndef ICE_MINV_LINES_THREADS
+#define ICE_MINV_LINES_THREADS 128
+#endif
+#ifndef ICE_MINV_LINES2_THREADS
+#define ICE_MINV_LINES2_THREADS 128
+#endif
+// minimum waves per SIMD the 3D line kernel's registers must allow where a
+// line holds at most this many values (its LDS allows three; longer lines
+// spill under the cap)
+#ifndef ICE_MINV_LINES_MINW
+#define ICE_MINV_LINES_MINW 3
+#endif
+#ifndef ICE_MINV_LINES_CAPVALS
+#define ICE_MINV_LINES_CAPVALS 20
+#endif
 template <int P, int M, class KT = double>
 struct MinvLines {
     static constexpr int N = P + 1, NN = N * N, NB = NN * N;
     static constexpr int PL = NN + (N == 4 ? 4 : 1);  // plane stride (N = 4: conflict-free transposes)
     static constexpr int VS = N * PL;
     static constexpr int EBUF = M * VS;
-    static constexpr int EPB = 128 / NN;
+    static constexpr int EPB = ICE_MINV_LINES_THREADS / NN;
     static constexpr int BLOCK = ((EPB * NN + 63) / 64) * 64;
+    static constexpr int MINW = (N * M <= ICE_MINV_LINES_CAPVALS) ? ICE_MINV_LINES_MINW : 1;
 };
 
 template <int P, int M, class KT = double>
-__global__ __launch_bounds__((MinvLines<P, M, KT>::BLOCK)) void minv_lines_kernel(DevSpace S,
+__global__ __launch_bounds__((MinvLines<P, M, KT>::BLOCK), (MinvLines<P, M, KT>::MINW)) void minv_lines_kernel(DevSpace S,
                                                                                   const double* __restrict__ r,
                                                                                   KT* __restrict__ k,
                                                                                   StageComb<KT> cb) {
@@ -373,38 +425,9 @@ __global__ __launch_bounds__((MinvLines<P, M, KT>::BLOCK)) void minv_lines_kerne
             for (int v = 0; v < M; ++v) k[o + i * M + v] = kn[i][v];
         return;
     }
-    // the line's base and earlier increments are read before anything is
-    // stored (out may be base: the last stage updates the state in place)
+    // (the last stage updates the state in place: out is base)
     const double dt = (cb.stp && cb.stp->done) ? 0.0 : (cb.dte ? cb.dte[e] : *cb.dtp);
-    double sum[N][M];
-#pragma unroll
-    for (int i = 0; i < N; ++i)
-#pragma unroll
-        for (int v = 0; v < M; ++v) sum[i][v] = cb.nprev > 0 ? cb.c0 * (double)cb.k0[o + i * M + v] : 0.0;
-    if (cb.nprev > 1) {
-#pragma unroll
-        for (int i = 0; i < N; ++i)
-#pragma unroll
-            for (int v = 0; v < M; ++v) sum[i][v] += cb.c1 * (double)cb.k1[o + i * M + v];
-    }
-    double bv[N][M];
-#pragma unroll
-    for (int i = 0; i < N; ++i)
-#pragma unroll
-        for (int v = 0; v < M; ++v) bv[i][v] = (double)cb.base[o + i * M + v];
-#pragma unroll
-    for (int i = 0; i < N; ++i)
-#pragma unroll
-        for (int v = 0; v < M; ++v) {
-            sum[i][v] += cb.cn * (double)kn[i][v];
-            cb.out[o + i * M + v] = (KT)(bv[i][v] + dt * sum[i][v]);
-        }
-    if (cb.store_k) {
-#pragma unroll
-        for (int i = 0; i < N; ++i)
-#pragma unroll
-            for (int v = 0; v < M; ++v) k[o + i * M + v] = kn[i][v];
-    }
+    stage_combine_line<N * M, KT>(cb, k, o, dt, &kn[0][0]);
 }
 
 // The same in 2D: a thread holds the line (t, .) of N nodes, applies B^-T
@@ -419,12 +442,15 @@ struct MinvLines2 {
     static constexpr int PL = N + (N % 2 == 0 ? 1 : 0);  // odd line stride
     static constexpr int VS = N * PL;
     static constexpr int EBUF = M * VS;
-    static constexpr int EPB = 128 / N;
+    static constexpr int EPB = ICE_MINV_LINES2_THREADS / N;
     static constexpr int BLOCK = ((EPB * N + 63) / 64) * 64;
+    // waves per SIMD the registers must allow with several variables (the
+    // three stage-combination paths otherwise cost P2-P3 a wave)
+    static constexpr int MINW = (M > 1 && P <= 2) ? 5 : (M > 1 && P == 3) ? 4 : 1;
 };
 
 template <int P, int M, class KT = double>
-__global__ __launch_bounds__((MinvLines2<P, M>::BLOCK)) void minv_lines2_kernel(DevSpace S,
+__global__ __launch_bounds__((MinvLines2<P, M>::BLOCK), (MinvLines2<P, M>::MINW)) void minv_lines2_kernel(DevSpace S,
                                                                                 const double* __restrict__ r,
                                                                                 KT* __restrict__ k,
                                                                                 StageComb<KT> cb) {
@@ -524,26 +550,9 @@ __global__ __launch_bounds__((MinvLines2<P, M>::BLOCK)) void minv_lines2_kernel(
             kn[i][v] = sg ? (KT)(1e-8 * r[o + i * M + v]) : (KT)acc;
         }
     if (cb.out) {
-        // the line's base and earlier increments are read before anything
-        // is stored (out may be base)
         const double dt = (cb.stp && cb.stp->done) ? 0.0 : (cb.dte ? cb.dte[e] : *cb.dtp);
-        double sum[N][M], bv[N][M];
-#pragma unroll
-        for (int i = 0; i < N; ++i)
-#pragma unroll
-            for (int v = 0; v < M; ++v) {
-                sum[i][v] = cb.nprev > 0 ? cb.c0 * (double)cb.k0[o + i * M + v] : 0.0;
-                if (cb.nprev > 1) sum[i][v] += cb.c1 * (double)cb.k1[o + i * M + v];
-                bv[i][v] = (double)cb.base[o + i * M + v];
-            }
-#pragma unroll
-        for (int i = 0; i < N; ++i)
-#pragma unroll
-            for (int v = 0; v < M; ++v) {
-                sum[i][v] += cb.cn * (double)kn[i][v];
-                cb.out[o + i * M + v] = (KT)(bv[i][v] + dt * sum[i][v]);
-            }
-        if (!cb.store_k) return;
+        stage_combine_line<N * M, KT>(cb, k, o, dt, &kn[0][0]);
+        return;
     }
 #pragma unroll
     for (int i = 0; i < N; ++i)
