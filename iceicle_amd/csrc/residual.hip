@@ -110,6 +110,17 @@ __device__ __forceinline__ P local_params(const P& p, const double* __restrict__
     return q;
 }
 
+// Minimum waves per SIMD of the 2D element kernel: the FP64 Burgers P1
+// instantiation sits two registers above an occupancy step (82 VGPRs, 5
+// waves) and fits 80 (6 waves) without scratch.  Caps that spill were
+// slower: 3D Euler face fluxes at 6 waves (83 -> 80 VGPRs, 20 B of scratch:
+// 0.375 -> 0.427 ms at 64^3 P3) and 2D Burgers P4 traces at 4 waves
+// (140 -> 128 VGPRs, 32 B: the 512-instance P4 group 0.63 -> 0.66 ms).
+template <int P, class PH>
+constexpr int elem2_min_waves() {
+    return (P == 1 && std::is_same<PH, Burgers<2, double>>::value) ? 6 : ICE_ELEM_MINW;
+}
+
 template <class PH, int D>
 struct TraceRec {
     // Euler needs only values: its diffusive flux is identically zero
@@ -1072,7 +1083,7 @@ __global__ __launch_bounds__(256) void element_kernel(DevSpace S, PhysParams ph0
 // point and basis function.  The lifting skips faces on which the basis
 // function's node does not lie (its trace there is zero).
 template <int P, class PH, int TPB, class UT = double>
-__global__ __launch_bounds__((ElemTile<2, P, TPB>::BLOCK), ICE_ELEM_MINW) void element_kernel_2d(DevSpace S, typename PH::Params ph0,
+__global__ __launch_bounds__((ElemTile<2, P, TPB>::BLOCK), (elem2_min_waves<P, PH>())) void element_kernel_2d(DevSpace S, typename PH::Params ph0,
                                                                         const double* __restrict__ emu,
                                                                         const double* __restrict__ srcload,
                                                                         const UT* __restrict__ U,
