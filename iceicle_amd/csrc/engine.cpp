@@ -1486,6 +1486,10 @@ int make_step_graph(ice_stepper* const* steppers, UT* const* u, int n, int nstep
         (void)hipGraphDestroy(g->graph);
         return fail(ICE_ERR_HIP, "graph instantiation failed");
     }
+    // upload the executable graph now rather than on its first launch (an
+    // optimisation only: a failure leaves the upload to the launch)
+    if (hipGraphUpload(g->exec, nullptr) == hipSuccess) (void)hipStreamSynchronize(nullptr);
+    else (void)hipGetLastError();
     *out = reinterpret_cast<ice_step_graph*>(g.release());
     return ICE_OK;
 }

@@ -110,3 +110,20 @@ def test_euler_2d_cfl_explicit():
     err = cases.rel_inf(u.cpu().numpy(), ref)
     print(f"NS vortex steps={n} t={t:.6e} t_ref={t_ref:.6e} err={err:.2e}")
     assert abs(t - t_ref) <= 1e-12 * t_ref and err <= 1e-11
+
+
+
+def test_cfl_stepper_graph_capture_is_refused():
+    """A CFL stepper recomputes its step from wave speeds that the residual
+    accumulates into a buffer cleared every step, which a captured graph
+    does not replay reliably here: capture refuses it with
+    ICE_ERR_UNSUPPORTED instead of recording it; fixed-step steppers
+    capture (the batch graph tests)."""
+    import torch
+    from iceicle_amd import engine
+    mesh, order, params, ic = workloads.burgers_2d(8, 2)
+    disc = engine.ConservationLawDDG(engine.FESpace(mesh, order), params)
+    u = torch.zeros(disc.size, dtype=torch.float64, device="cuda")
+    st = engine.ExplicitStepper(disc, abi.stepper_desc(abi.STEP_RK3_SSP, cfl=0.2, ntime=10))
+    with pytest.raises(engine.IceError, match="fixed step sizes"):
+        engine.StepGraph([st], [u], 10)
