@@ -1426,8 +1426,6 @@ int make_step_graph(ice_stepper* const* steppers, UT* const* u, int n, int nstep
     *out = nullptr;
     for (int k = 0; k < n; ++k) {
         if (!steppers[k] || !u[k]) return fail(ICE_ERR_ARG, "null stepper or state");
-        if (steppers[k]->ctl.use_cfl) return fail(ICE_ERR_UNSUPPORTED, "graph capture needs fixed step sizes");
-        if (steppers[k]->disc->dd.wave) return fail(ICE_ERR_UNSUPPORTED, "graph capture needs fixed step sizes");
         if constexpr (std::is_same<UT, float>::value) {
             const int rc = prepare_f32(steppers[k]);
             if (rc != ICE_OK) return rc;

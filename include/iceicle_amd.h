@@ -492,9 +492,13 @@ int ice_stepper_destroy(ice_stepper* stepper);
 int ice_stepper_set_element_dt(ice_stepper* stepper, const double* dt);
 
 /* One HIP graph holding nsteps steps of every stepper, each advancing its own
- * device state u[k] (fixed step sizes only): a single graph launch then runs
- * all of them.  The state pointers are baked into the graph.  No reference
- * counterpart. */
+ * device state u[k]: a single graph launch then runs all of them.  Fixed,
+ * per-element and CFL step sizes all stay on the device (a CFL stepper's
+ * wave-speed maxima are reset by its step kernels), and the termination
+ * condition is a device flag, so the steps need no host synchronisation;
+ * steps past the termination change nothing.  The state pointers are baked
+ * into the graph and the executable graph is uploaded at creation.  No
+ * reference counterpart. */
 typedef struct ice_step_graph ice_step_graph;
 int ice_step_graph_create(ice_stepper* const* steppers, double* const* u_dev, int n, int nsteps,
                           ice_step_graph** out);

@@ -113,17 +113,35 @@ def test_euler_2d_cfl_explicit():
 
 
 
-def test_cfl_stepper_graph_capture_is_refused():
-    """A CFL stepper recomputes its step from wave speeds that the residual
-    accumulates into a buffer cleared every step, which a captured graph
-    does not replay reliably here: capture refuses it with
-    ICE_ERR_UNSUPPORTED instead of recording it; fixed-step steppers
-    capture (the batch graph tests)."""
+
+def test_cfl_step_graph_matches_eager():
+    """100 RK3-SSP steps of the 2D Burgers 32x32 P2 workload at CFL 0.2 (the
+    step size recomputed on the device from the wave speeds every step; the
+    wave-speed maxima are reset by the step kernels, not by memsets)
+    replayed from one HIP graph equal the eager solve bit for bit, with the
+    same step count and final time; a second replay changes nothing, since
+    the device-side termination flag is set."""
     import torch
     from iceicle_amd import engine
-    mesh, order, params, ic = workloads.burgers_2d(8, 2)
-    disc = engine.ConservationLawDDG(engine.FESpace(mesh, order), params)
-    u = torch.zeros(disc.size, dtype=torch.float64, device="cuda")
-    st = engine.ExplicitStepper(disc, abi.stepper_desc(abi.STEP_RK3_SSP, cfl=0.2, ntime=10))
-    with pytest.raises(engine.IceError, match="fixed step sizes"):
-        engine.StepGraph([st], [u], 10)
+    mesh, order, params, ic = workloads.burgers_2d(32, 2)
+    space = engine.FESpace(mesh, order)
+    disc = engine.ConservationLawDDG(space, params)
+    u0 = torch.empty(disc.size, dtype=torch.float64, device="cuda")
+    engine.project(disc, ic, u0)
+    desc = abi.stepper_desc(abi.STEP_RK3_SSP, cfl=0.2, ntime=100)
+    ua = u0.clone()
+    sa = engine.ExplicitStepper(disc, desc)
+    n, t = sa.solve(ua)
+    ub = u0.clone()
+    sb = engine.ExplicitStepper(disc, desc)
+    g = engine.StepGraph([sb], [ub], 100)
+    g.launch()
+    torch.cuda.synchronize()
+    stb = sb.status()
+    assert n == 100 and stb["itime"] == 100 and stb["done"]
+    assert stb["time"] == t
+    assert torch.equal(ua, ub)
+    keep = ub.clone()
+    g.launch()
+    torch.cuda.synchronize()
+    assert torch.equal(keep, ub)
