@@ -259,13 +259,18 @@ __global__ __launch_bounds__(TPB) void minv_kernel(DevSpace S, const double* __r
 #ifndef ICE_MINV_LINES_MINW
 #define ICE_MINV_LINES_MINW 3
 #endif
+// plane padding of the N = 4 line kernel's LDS buffer (4: conflict-free
+// transposes)
+#ifndef ICE_MINV_LINES_PAD4
+#define ICE_MINV_LINES_PAD4 4
+#endif
 #ifndef ICE_MINV_LINES_CAPVALS
 #define ICE_MINV_LINES_CAPVALS 20
 #endif
 template <int P, int M, class KT = double>
 struct MinvLines {
     static constexpr int N = P + 1, NN = N * N, NB = NN * N;
-    static constexpr int PL = NN + (N == 4 ? 4 : 1);  // plane stride (N = 4: conflict-free transposes)
+    static constexpr int PL = NN + (N == 4 ? ICE_MINV_LINES_PAD4 : 1);  // plane stride (padding: conflict-free transposes)
     static constexpr int VS = N * PL;
     static constexpr int EBUF = M * VS;
     static constexpr int EPB = ICE_MINV_LINES_THREADS / NN;
